@@ -1,0 +1,254 @@
+"""ctypes declarations of the public C ABI (include/box2d/*.h, include/b2mi.h).
+
+`B2Api(lib_path, scenes_path)` binds one shared library that exports the reference's world /
+body / shape / step API (this package's libb2mi.so, or any other library with the same ABI)
+together with the scene library linked against it. Structures mirror the C layouts exactly
+(reference include/box2d/types.h, collision.h, id.h). Libraries are opened RTLD_LOCAL so that
+several implementations of the same symbol names can live in one process.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_bool, c_float, c_int, c_int32, c_uint16,
+                    c_uint32, c_uint64, c_void_p)
+
+import numpy as np
+
+
+class Vec2(Structure):
+    _fields_ = [("x", c_float), ("y", c_float)]
+
+
+class Rot(Structure):
+    _fields_ = [("c", c_float), ("s", c_float)]
+
+
+class Transform(Structure):
+    _fields_ = [("p", Vec2), ("q", Rot)]
+
+
+class AABB(Structure):
+    _fields_ = [("lowerBound", Vec2), ("upperBound", Vec2)]
+
+
+class WorldId(Structure):
+    _fields_ = [("index1", c_uint16), ("revision", c_uint16)]
+
+
+class BodyId(Structure):
+    _fields_ = [("index1", c_int32), ("world0", c_uint16), ("revision", c_uint16)]
+
+
+class ShapeId(Structure):
+    _fields_ = [("index1", c_int32), ("world0", c_uint16), ("revision", c_uint16)]
+
+
+class WorldDef(Structure):
+    _fields_ = [("gravity", Vec2), ("restitutionThreshold", c_float),
+                ("contactPushoutVelocity", c_float), ("hitEventThreshold", c_float),
+                ("contactHertz", c_float), ("contactDampingRatio", c_float),
+                ("jointHertz", c_float), ("jointDampingRatio", c_float),
+                ("enableSleep", c_bool), ("enableContinous", c_bool), ("workerCount", c_int32),
+                ("enqueueTask", c_void_p), ("finishTask", c_void_p),
+                ("userTaskContext", c_void_p), ("internalValue", c_int32)]
+
+
+class ManifoldPoint(Structure):
+    _fields_ = [("point", Vec2), ("anchorA", Vec2), ("anchorB", Vec2), ("separation", c_float),
+                ("normalImpulse", c_float), ("tangentImpulse", c_float),
+                ("maxNormalImpulse", c_float), ("normalVelocity", c_float), ("id", c_uint16),
+                ("persisted", c_bool)]
+
+
+class Manifold(Structure):
+    _fields_ = [("points", ManifoldPoint * 2), ("normal", Vec2), ("pointCount", c_int32)]
+
+
+class Counters(Structure):
+    _fields_ = [(n, c_int32) for n in (
+        "staticBodyCount bodyCount shapeCount contactCount jointCount islandCount stackUsed "
+        "staticTreeHeight treeHeight byteCount taskCount").split()] + [("colorCounts", c_int32 * 12)]
+
+
+class Profile(Structure):
+    _fields_ = [(n, c_float) for n in (
+        "step pairs collide solve buildIslands solveConstraints prepareTasks solverTasks "
+        "prepareConstraints integrateVelocities warmStart solveVelocities integratePositions "
+        "relaxVelocities applyRestitution storeImpulses finalizeBodies splitIslands sleepIslands "
+        "hitEvents broadphase continuous").split()]
+
+
+class ContactRecord(Structure):
+    _fields_ = [("contactId", c_int32), ("shapeIdA", c_int32), ("shapeIdB", c_int32),
+                ("colorIndex", c_int32), ("localIndex", c_int32), ("simFlags", c_uint32),
+                ("friction", c_float), ("restitution", c_float), ("manifold", Manifold)]
+
+
+class BodyRecord(Structure):
+    _fields_ = [("bodyId", c_int32), ("type", c_int32), ("setIndex", c_int32),
+                ("localIndex", c_int32), ("transform", Transform), ("center", Vec2),
+                ("linearVelocity", Vec2), ("angularVelocity", c_float), ("pad", c_float)]
+
+
+assert ctypes.sizeof(WorldDef) == 80
+assert ctypes.sizeof(Manifold) == 108
+assert ctypes.sizeof(BodyRecord) == 56
+
+# numpy views of the records (same byte layout)
+MANIFOLD_POINT_DTYPE = np.dtype([("point", "<f4", 2), ("anchorA", "<f4", 2), ("anchorB", "<f4", 2),
+                                 ("separation", "<f4"), ("normalImpulse", "<f4"),
+                                 ("tangentImpulse", "<f4"), ("maxNormalImpulse", "<f4"),
+                                 ("normalVelocity", "<f4"), ("id", "<u2"), ("persisted", "u1"),
+                                 ("_pad", "u1")])
+CONTACT_DTYPE = np.dtype([("contactId", "<i4"), ("shapeIdA", "<i4"), ("shapeIdB", "<i4"),
+                          ("colorIndex", "<i4"), ("localIndex", "<i4"), ("simFlags", "<u4"),
+                          ("friction", "<f4"), ("restitution", "<f4"),
+                          ("points", MANIFOLD_POINT_DTYPE, 2), ("normal", "<f4", 2),
+                          ("pointCount", "<i4")])
+BODY_DTYPE = np.dtype([("bodyId", "<i4"), ("type", "<i4"), ("setIndex", "<i4"),
+                       ("localIndex", "<i4"), ("p", "<f4", 2), ("q", "<f4", 2),
+                       ("center", "<f4", 2), ("v", "<f4", 2), ("w", "<f4"), ("pad", "<f4")])
+assert CONTACT_DTYPE.itemsize == ctypes.sizeof(ContactRecord)
+assert BODY_DTYPE.itemsize == ctypes.sizeof(BodyRecord)
+
+WORLD_FACTORY = ctypes.CFUNCTYPE(WorldId, POINTER(WorldDef), c_void_p)
+
+
+def _open(path: str) -> ctypes.CDLL:
+    return ctypes.CDLL(path, mode=os.RTLD_LOCAL)
+
+
+class B2Api:
+    """One implementation of the public C API plus its scene library."""
+
+    def __init__(self, lib_path: str, scenes_path: str | None = None):
+        self.lib_path = lib_path
+        self.lib = _open(lib_path)
+        self.scenes = _open(scenes_path) if scenes_path else None
+        L = self.lib
+        L.b2DefaultWorldDef.restype = WorldDef
+        L.b2CreateWorld.restype = WorldId
+        L.b2CreateWorld.argtypes = [POINTER(WorldDef)]
+        L.b2DestroyWorld.argtypes = [WorldId]
+        L.b2World_IsValid.argtypes = [WorldId]
+        L.b2World_IsValid.restype = c_bool
+        L.b2World_Step.argtypes = [WorldId, c_float, c_int]
+        L.b2World_GetCounters.argtypes = [WorldId]
+        L.b2World_GetCounters.restype = Counters
+        L.b2World_GetProfile.argtypes = [WorldId]
+        L.b2World_GetProfile.restype = Profile
+        L.b2World_EnableWarmStarting.argtypes = [WorldId, c_bool]
+        L.b2World_EnableContinuous.argtypes = [WorldId, c_bool]
+        L.b2Body_GetPosition.argtypes = [BodyId]
+        L.b2Body_GetPosition.restype = Vec2
+        L.b2Body_GetRotation.argtypes = [BodyId]
+        L.b2Body_GetRotation.restype = Rot
+        L.b2Body_GetLinearVelocity.argtypes = [BodyId]
+        L.b2Body_GetLinearVelocity.restype = Vec2
+        L.b2Body_GetAngularVelocity.argtypes = [BodyId]
+        L.b2Body_GetAngularVelocity.restype = c_float
+        L.b2Body_SetLinearVelocity.argtypes = [BodyId, Vec2]
+        L.b2Body_SetAngularVelocity.argtypes = [BodyId, c_float]
+        L.b2Body_GetMass.argtypes = [BodyId]
+        L.b2Body_GetMass.restype = c_float
+        L.b2Body_GetInertiaTensor.argtypes = [BodyId]
+        L.b2Body_GetInertiaTensor.restype = c_float
+        L.b2Body_GetContactCapacity.argtypes = [BodyId]
+        L.b2Body_GetContactCapacity.restype = c_int
+        L.b2mi_World_DumpContacts.argtypes = [WorldId, c_void_p, c_int]
+        L.b2mi_World_DumpContacts.restype = c_int
+        L.b2mi_World_DumpBodies.argtypes = [WorldId, c_void_p, c_int]
+        L.b2mi_World_DumpBodies.restype = c_int
+        if self.scenes is not None:
+            S = self.scenes
+            S.scene_set_world_factory.argtypes = [c_void_p, c_void_p]
+            S.scene_large_pyramid.argtypes = [POINTER(WorldDef), c_int, POINTER(WorldId), c_void_p, c_int]
+            S.scene_pyramid_grid.argtypes = [POINTER(WorldDef), c_int, c_int, c_int, c_uint64,
+                                             POINTER(WorldId), c_void_p, c_int]
+            S.scene_tumbler.argtypes = [POINTER(WorldDef), c_int, c_uint64, POINTER(WorldId), c_void_p, c_int]
+            S.scene_batch_pyramid.argtypes = [POINTER(WorldDef), c_uint64, c_int, POINTER(WorldId),
+                                              c_void_p, c_int, POINTER(c_float)]
+            S.scene_drop_boxes.argtypes = [POINTER(WorldDef), c_uint64, c_int, c_int, POINTER(WorldId),
+                                           c_void_p, c_int, POINTER(c_float)]
+            S.scene_shape_zoo.argtypes = [POINTER(WorldDef), c_int, c_uint64, POINTER(WorldId), c_void_p, c_int]
+            S.scene_hello.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
+            S.scene_step_n.argtypes = [WorldId, c_float, c_int, c_int]
+
+    # ---- worlds ----
+    def default_world_def(self, **overrides) -> WorldDef:
+        d = self.lib.b2DefaultWorldDef()
+        for k, v in overrides.items():
+            setattr(d, k, v)
+        return d
+
+    def step(self, world: WorldId, dt: float = 1.0 / 60.0, substeps: int = 4, n: int = 1):
+        if self.scenes is not None:
+            self.scenes.scene_step_n(world, c_float(dt), substeps, n)
+        else:
+            for _ in range(n):
+                self.lib.b2World_Step(world, c_float(dt), substeps)
+
+    def counters(self, world: WorldId) -> Counters:
+        return self.lib.b2World_GetCounters(world)
+
+    def destroy(self, world: WorldId):
+        self.lib.b2DestroyWorld(world)
+
+    # ---- scenes (return world id and the dynamic body ids) ----
+    def _bodies(self, n_max: int):
+        return (BodyId * max(n_max, 1))()
+
+    def scene(self, name: str, world_def: WorldDef | None = None, **kw):
+        d = world_def if world_def is not None else self.default_world_def(enableSleep=False)
+        w = WorldId()
+        S = self.scenes
+        if name == "large_pyramid":
+            base = kw.get("base", 100)
+            ids = self._bodies(base * (base + 1) // 2)
+            n = S.scene_large_pyramid(ctypes.byref(d), base, ctypes.byref(w), ids, len(ids))
+        elif name == "pyramid_grid":
+            rows, cols, base = kw.get("rows", 2), kw.get("cols", 2), kw.get("base", 10)
+            ids = self._bodies(rows * cols * base * (base + 1) // 2)
+            n = S.scene_pyramid_grid(ctypes.byref(d), rows, cols, base, kw.get("seed", 1),
+                                     ctypes.byref(w), ids, len(ids))
+        elif name == "tumbler":
+            count = kw.get("count", 500)
+            ids = self._bodies(count)
+            n = S.scene_tumbler(ctypes.byref(d), count, kw.get("seed", 1), ctypes.byref(w), ids, len(ids))
+        elif name == "batch_pyramid":
+            ids = self._bodies(820)
+            f = c_float()
+            n = S.scene_batch_pyramid(ctypes.byref(d), kw.get("seed", 1), kw.get("index", 0),
+                                      ctypes.byref(w), ids, len(ids), ctypes.byref(f))
+        elif name == "drop_boxes":
+            count = kw.get("count", 16)
+            ids = self._bodies(count)
+            params = (c_float * 2)()
+            n = S.scene_drop_boxes(ctypes.byref(d), kw.get("seed", 1), kw.get("index", 0), count,
+                                   ctypes.byref(w), ids, len(ids), params)
+        elif name == "shape_zoo":
+            copies = kw.get("copies", 4)
+            ids = self._bodies(copies * 5)
+            n = S.scene_shape_zoo(ctypes.byref(d), copies, kw.get("seed", 1), ctypes.byref(w), ids, len(ids))
+        elif name == "hello":
+            ids = self._bodies(1)
+            n = S.scene_hello(ctypes.byref(d), ctypes.byref(w), ids, len(ids))
+        else:
+            raise ValueError(f"unknown scene {name}")
+        return w, list(ids)[:n]
+
+    # ---- inspection ----
+    def dump_contacts(self, world: WorldId) -> np.ndarray:
+        n = self.lib.b2mi_World_DumpContacts(world, None, 0)
+        out = np.zeros(n, dtype=CONTACT_DTYPE)
+        if n:
+            self.lib.b2mi_World_DumpContacts(world, out.ctypes.data, n)
+        return out
+
+    def dump_bodies(self, world: WorldId) -> np.ndarray:
+        n = self.lib.b2mi_World_DumpBodies(world, None, 0)
+        out = np.zeros(n, dtype=BODY_DTYPE)
+        if n:
+            self.lib.b2mi_World_DumpBodies(world, out.ctypes.data, n)
+        return out

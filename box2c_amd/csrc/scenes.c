@@ -1,0 +1,460 @@
+// Benchmark and test scenes, written against the public C API only (include/box2d/box2d.h),
+// so the same file is linked against this library, against the CPU oracle and against the
+// reference build; each scene is then bit-for-bit the same input for all three.
+//
+// Scene definitions:
+//  - large pyramid: the reference's benchmark/large_pyramid.c:7-55 layout, base parameterized.
+//  - pyramid grid: rows x cols pyramids (base 10, box half-extent 0.5) spaced 12 m apart,
+//    after benchmark/many_pyramids.c:7-30; every row stands on its own static box of one
+//    ground body; each box x is jittered by U(-0.01, 0.01) from a PCG32 stream.
+//  - tumbler: a kinematic square container spinning at 0.25 rad/s about its center (after
+//    samples/sample_benchmark.cpp:407-467) filled with half boxes / half circles of size
+//    0.125 on a seeded jittered 0.4 m grid.
+//  - batch pyramid: one world of a batch; base drawn from {4..40}, friction U(0.4, 0.8).
+//  - drop boxes: one world of the differentiable batch; boxes with half-extents U(0.2, 0.6)
+//    dropped from heights U(1, 4) with N(0, 2) linear and N(0, 1) angular velocity.
+//  - shape zoo: every supported shape pair (circle, capsule, polygon, rounded polygon,
+//    segment) resting and colliding, for narrowphase coverage.
+#include "box2d/box2d.h"
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined( __GNUC__ )
+	#define SCENE_API __attribute__( ( visibility( "default" ) ) )
+#else
+	#define SCENE_API
+#endif
+
+// ---- PCG32 (O'Neill 2014, pcg32_random_r), seeded per (seed, stream) ----
+typedef struct Pcg32
+{
+	uint64_t state;
+	uint64_t inc;
+} Pcg32;
+
+static uint32_t pcg32_next( Pcg32* r )
+{
+	uint64_t old = r->state;
+	r->state = old * 6364136223846793005ULL + r->inc;
+	uint32_t xorshifted = (uint32_t)( ( ( old >> 18u ) ^ old ) >> 27u );
+	uint32_t rot = (uint32_t)( old >> 59u );
+	return ( xorshifted >> rot ) | ( xorshifted << ( ( -rot ) & 31u ) );
+}
+
+static Pcg32 pcg32_seed( uint64_t seed, uint64_t stream )
+{
+	Pcg32 r = { 0u, ( stream << 1u ) | 1u };
+	pcg32_next( &r );
+	r.state += seed;
+	pcg32_next( &r );
+	return r;
+}
+
+// uniform in [0, 1) with 24 random bits
+static float pcg32_unit( Pcg32* r )
+{
+	return (float)( pcg32_next( r ) >> 8 ) * ( 1.0f / 16777216.0f );
+}
+
+static float pcg32_uniform( Pcg32* r, float lo, float hi )
+{
+	return lo + ( hi - lo ) * pcg32_unit( r );
+}
+
+// Box-Muller normal
+static float pcg32_normal( Pcg32* r, float sigma )
+{
+	float u1 = pcg32_unit( r );
+	float u2 = pcg32_unit( r );
+	if ( u1 < 1.0e-7f )
+		u1 = 1.0e-7f;
+	return sigma * sqrtf( -2.0f * logf( u1 ) ) * cosf( 2.0f * b2_pi * u2 );
+}
+
+// ---- world factory (lets a batch API own the world) ----
+typedef b2WorldId SceneWorldFactory( const b2WorldDef* def, void* context );
+static SceneWorldFactory* s_factory = 0;
+static void* s_factoryContext = 0;
+
+SCENE_API void scene_set_world_factory( SceneWorldFactory* factory, void* context )
+{
+	s_factory = factory;
+	s_factoryContext = context;
+}
+
+static b2WorldId scene_create_world( const b2WorldDef* def )
+{
+	if ( s_factory )
+		return s_factory( def, s_factoryContext );
+	return b2CreateWorld( def );
+}
+
+static void record( b2BodyId* out, int capacity, int index, b2BodyId id )
+{
+	if ( out && index < capacity )
+		out[index] = id;
+}
+
+// ---- large pyramid (reference benchmark/large_pyramid.c) ----
+SCENE_API int scene_large_pyramid( const b2WorldDef* def, int baseCount, b2WorldId* outWorld, b2BodyId* outBodies,
+								   int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 100.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.enableSleep = false;
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+
+	float h = 0.5f;
+	b2Polygon box = b2MakeSquare( h );
+	float shift = 1.0f * h;
+	int n = 0;
+	for ( int i = 0; i < baseCount; ++i )
+	{
+		float y = ( 2.0f * i + 1.0f ) * shift;
+		for ( int j = i; j < baseCount; ++j )
+		{
+			float x = ( i + 1.0f ) * shift + 2.0f * ( j - i ) * shift - h * baseCount;
+			bodyDef.position = ( b2Vec2 ){ x, y };
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			record( outBodies, capacity, n++, bodyId );
+		}
+	}
+	return n;
+}
+
+// ---- pyramid grid ----
+SCENE_API int scene_pyramid_grid( const b2WorldDef* def, int rows, int cols, int baseCount, uint64_t seed,
+								  b2WorldId* outWorld, b2BodyId* outBodies, int capacity )
+{
+	const float extent = 0.5f;
+	const float pitch = 12.0f;
+	Pcg32 rng = pcg32_seed( seed, 2 );
+
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+
+	float halfWidth = 0.5f * pitch * cols + 2.0f;
+	float x0 = -0.5f * pitch * ( cols - 1 );
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		for ( int r = 0; r < rows; ++r )
+		{
+			b2Polygon box = b2MakeOffsetBox( halfWidth, 0.5f, ( b2Vec2 ){ 0.0f, r * pitch - 0.5f }, 0.0f );
+			b2CreatePolygonShape( groundId, &shapeDef, &box );
+		}
+	}
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.enableSleep = false;
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	b2Polygon box = b2MakeSquare( extent );
+	int n = 0;
+	for ( int r = 0; r < rows; ++r )
+	{
+		float baseY = r * pitch;
+		for ( int c = 0; c < cols; ++c )
+		{
+			float centerX = x0 + c * pitch;
+			for ( int i = 0; i < baseCount; ++i )
+			{
+				float y = ( 2.0f * i + 1.0f ) * extent + baseY;
+				for ( int j = i; j < baseCount; ++j )
+				{
+					float x = ( i + 1.0f ) * extent + 2.0f * ( j - i ) * extent - extent * baseCount + centerX;
+					x += pcg32_uniform( &rng, -0.01f, 0.01f );
+					bodyDef.position = ( b2Vec2 ){ x, y };
+					b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+					b2CreatePolygonShape( bodyId, &shapeDef, &box );
+					record( outBodies, capacity, n++, bodyId );
+				}
+			}
+		}
+	}
+	return n;
+}
+
+// ---- tumbler ----
+SCENE_API int scene_tumbler( const b2WorldDef* def, int shapeCount, uint64_t seed, b2WorldId* outWorld,
+							 b2BodyId* outBodies, int capacity )
+{
+	const float pitch = 0.4f;
+	const float size = 0.125f;
+	Pcg32 rng = pcg32_seed( seed, 3 );
+
+	int side = 1;
+	while ( side * side < shapeCount )
+		side += 1;
+	float halfInterior = 0.5f * pitch * side + 0.5f;
+	if ( halfInterior < 10.0f )
+		halfInterior = 10.0f;
+
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_kinematicBody;
+		bodyDef.angularVelocity = 0.25f;
+		bodyDef.enableSleep = false;
+		b2BodyId containerId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		float t = 0.25f, H = halfInterior + t;
+		b2Polygon wall;
+		wall = b2MakeOffsetBox( t, H, ( b2Vec2 ){ H, 0.0f }, 0.0f );
+		b2CreatePolygonShape( containerId, &shapeDef, &wall );
+		wall = b2MakeOffsetBox( t, H, ( b2Vec2 ){ -H, 0.0f }, 0.0f );
+		b2CreatePolygonShape( containerId, &shapeDef, &wall );
+		wall = b2MakeOffsetBox( H, t, ( b2Vec2 ){ 0.0f, H }, 0.0f );
+		b2CreatePolygonShape( containerId, &shapeDef, &wall );
+		wall = b2MakeOffsetBox( H, t, ( b2Vec2 ){ 0.0f, -H }, 0.0f );
+		b2CreatePolygonShape( containerId, &shapeDef, &wall );
+	}
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.enableSleep = false;
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+	shapeDef.friction = 0.6f;
+	b2Polygon box = b2MakeSquare( size );
+	b2Circle circle = { { 0.0f, 0.0f }, size };
+	float start = -0.5f * pitch * ( side - 1 );
+	int n = 0;
+	for ( int k = 0; k < shapeCount; ++k )
+	{
+		int ix = k % side, iy = k / side;
+		float x = start + pitch * ix + pcg32_uniform( &rng, -0.05f, 0.05f );
+		float y = start + pitch * iy + pcg32_uniform( &rng, -0.05f, 0.05f );
+		bodyDef.position = ( b2Vec2 ){ x, y };
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		if ( pcg32_next( &rng ) & 1u )
+			b2CreatePolygonShape( bodyId, &shapeDef, &box );
+		else
+			b2CreateCircleShape( bodyId, &shapeDef, &circle );
+		record( outBodies, capacity, n++, bodyId );
+	}
+	return n;
+}
+
+// ---- one world of the uneven pyramid batch ----
+SCENE_API int scene_batch_pyramid( const b2WorldDef* def, uint64_t seed, int worldIndex, b2WorldId* outWorld,
+								   b2BodyId* outBodies, int capacity, float* outFriction )
+{
+	Pcg32 rng = pcg32_seed( seed, 1000u + (uint64_t)worldIndex );
+	int baseCount = 4 + (int)( pcg32_next( &rng ) % 37u );
+	float friction = pcg32_uniform( &rng, 0.4f, 0.8f );
+	if ( outFriction )
+		*outFriction = friction;
+
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 0.5f * baseCount + 10.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.friction = friction;
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.enableSleep = false;
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.friction = friction;
+	float h = 0.5f;
+	b2Polygon box = b2MakeSquare( h );
+	int n = 0;
+	for ( int i = 0; i < baseCount; ++i )
+	{
+		float y = ( 2.0f * i + 1.0f ) * h;
+		for ( int j = i; j < baseCount; ++j )
+		{
+			float x = ( i + 1.0f ) * h + 2.0f * ( j - i ) * h - h * baseCount;
+			bodyDef.position = ( b2Vec2 ){ x, y };
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			record( outBodies, capacity, n++, bodyId );
+		}
+	}
+	return n;
+}
+
+// ---- one world of the differentiable drop batch ----
+// outParams (optional) receives friction, restitution.
+SCENE_API int scene_drop_boxes( const b2WorldDef* def, uint64_t seed, int worldIndex, int boxCount, b2WorldId* outWorld,
+								b2BodyId* outBodies, int capacity, float* outParams )
+{
+	Pcg32 rng = pcg32_seed( seed, 500000u + (uint64_t)worldIndex );
+	float friction = pcg32_uniform( &rng, 0.2f, 1.0f );
+	float restitution = pcg32_uniform( &rng, 0.0f, 0.3f );
+	if ( outParams )
+	{
+		outParams[0] = friction;
+		outParams[1] = restitution;
+	}
+
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 0.75f * boxCount + 10.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.friction = friction;
+		shapeDef.restitution = 0.0f;
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	int n = 0;
+	for ( int i = 0; i < boxCount; ++i )
+	{
+		float hx = pcg32_uniform( &rng, 0.2f, 0.6f );
+		float hy = pcg32_uniform( &rng, 0.2f, 0.6f );
+		float y = pcg32_uniform( &rng, 1.0f, 4.0f );
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.enableSleep = false;
+		bodyDef.position = ( b2Vec2 ){ ( i - 0.5f * ( boxCount - 1 ) ) * 1.5f, y };
+		bodyDef.linearVelocity = ( b2Vec2 ){ pcg32_normal( &rng, 2.0f ), pcg32_normal( &rng, 2.0f ) };
+		bodyDef.angularVelocity = pcg32_normal( &rng, 1.0f );
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.friction = friction;
+		shapeDef.restitution = restitution;
+		b2Polygon box = b2MakeBox( hx, hy );
+		b2CreatePolygonShape( bodyId, &shapeDef, &box );
+		record( outBodies, capacity, n++, bodyId );
+	}
+	return n;
+}
+
+// ---- shape zoo: every supported shape pair ----
+SCENE_API int scene_shape_zoo( const b2WorldDef* def, int copies, uint64_t seed, b2WorldId* outWorld, b2BodyId* outBodies,
+							   int capacity )
+{
+	Pcg32 rng = pcg32_seed( seed, 7 );
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+
+	// ground: a box, a slanted box and a segment
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2Polygon box = b2MakeOffsetBox( 40.0f, 0.5f, ( b2Vec2 ){ 0.0f, -0.5f }, 0.0f );
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+		b2Polygon ramp = b2MakeOffsetBox( 6.0f, 0.25f, ( b2Vec2 ){ -30.0f, 3.0f }, -0.3f );
+		b2CreatePolygonShape( groundId, &shapeDef, &ramp );
+		b2Segment seg = { { 20.0f, 2.0f }, { 38.0f, 4.0f } };
+		b2CreateSegmentShape( groundId, &shapeDef, &seg );
+	}
+
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	int n = 0;
+	for ( int k = 0; k < copies; ++k )
+	{
+		for ( int kind = 0; kind < 5; ++kind )
+		{
+			b2BodyDef bodyDef = b2DefaultBodyDef();
+			bodyDef.type = b2_dynamicBody;
+			bodyDef.enableSleep = false;
+			float x = -36.0f + 3.6f * ( ( k * 5 + kind ) % 20 ) + pcg32_uniform( &rng, -0.3f, 0.3f );
+			float y = 1.0f + 1.6f * ( ( k * 5 + kind ) / 20 ) + pcg32_uniform( &rng, 0.0f, 0.5f );
+			bodyDef.position = ( b2Vec2 ){ x, y };
+			bodyDef.rotation = b2MakeRot( pcg32_uniform( &rng, -1.0f, 1.0f ) );
+			bodyDef.angularVelocity = pcg32_uniform( &rng, -2.0f, 2.0f );
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			shapeDef.friction = pcg32_uniform( &rng, 0.2f, 0.9f );
+			shapeDef.restitution = ( kind == 1 ) ? 0.3f : 0.0f;
+			switch ( kind )
+			{
+				case 0:
+				{
+					b2Polygon box = b2MakeBox( pcg32_uniform( &rng, 0.2f, 0.7f ), pcg32_uniform( &rng, 0.2f, 0.7f ) );
+					b2CreatePolygonShape( bodyId, &shapeDef, &box );
+				}
+				break;
+				case 1:
+				{
+					b2Circle circle = { { 0.0f, 0.0f }, pcg32_uniform( &rng, 0.2f, 0.6f ) };
+					b2CreateCircleShape( bodyId, &shapeDef, &circle );
+				}
+				break;
+				case 2:
+				{
+					b2Capsule capsule = { { -0.4f, 0.0f }, { 0.4f, 0.0f }, pcg32_uniform( &rng, 0.15f, 0.35f ) };
+					b2CreateCapsuleShape( bodyId, &shapeDef, &capsule );
+				}
+				break;
+				case 3:
+				{
+					b2Polygon rbox = b2MakeRoundedBox( 0.3f, 0.2f, 0.1f );
+					b2CreatePolygonShape( bodyId, &shapeDef, &rbox );
+				}
+				break;
+				default:
+				{
+					// two shapes on one body: an offset box and a circle
+					b2Polygon obox = b2MakeOffsetBox( 0.3f, 0.3f, ( b2Vec2 ){ 0.3f, 0.0f }, 0.2f );
+					b2CreatePolygonShape( bodyId, &shapeDef, &obox );
+					b2Circle circle = { { -0.3f, 0.1f }, 0.25f };
+					b2CreateCircleShape( bodyId, &shapeDef, &circle );
+				}
+				break;
+			}
+			record( outBodies, capacity, n++, bodyId );
+		}
+	}
+	return n;
+}
+
+// ---- HelloWorld of the reference test suite (test/test_world.c:17-97) ----
+SCENE_API int scene_hello( const b2WorldDef* def, b2WorldId* outWorld, b2BodyId* outBodies, int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	b2BodyDef groundBodyDef = b2DefaultBodyDef();
+	groundBodyDef.position = ( b2Vec2 ){ 0.0f, -10.0f };
+	b2BodyId groundId = b2CreateBody( worldId, &groundBodyDef );
+	b2Polygon groundBox = b2MakeBox( 50.0f, 10.0f );
+	b2ShapeDef groundShapeDef = b2DefaultShapeDef();
+	b2CreatePolygonShape( groundId, &groundShapeDef, &groundBox );
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.position = ( b2Vec2 ){ 0.0f, 4.0f };
+	b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+	b2Polygon dynamicBox = b2MakeBox( 1.0f, 1.0f );
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+	shapeDef.friction = 0.3f;
+	b2CreatePolygonShape( bodyId, &shapeDef, &dynamicBox );
+	record( outBodies, capacity, 0, bodyId );
+	return 1;
+}
+
+// Steps a world n times (saves one ctypes crossing per step in tests and benches).
+SCENE_API void scene_step_n( b2WorldId worldId, float timeStep, int subStepCount, int n )
+{
+	for ( int i = 0; i < n; ++i )
+		b2World_Step( worldId, timeStep, subStepCount );
+}

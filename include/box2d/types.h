@@ -1,0 +1,169 @@
+// Definition structs, enums and reporting structs of the public API.
+//
+// Field order and types follow the reference's include/box2d/types.h (task callbacks 30-48,
+// b2WorldDef 56-109, b2BodyDef 128-192, b2Filter 197-213, b2ShapeType 227-235, b2ShapeDef
+// 241-286, b2Profile 397-421, b2Counters 424-438, b2ContactData 1010-1015) so that a
+// definition filled in by unchanged application code means the same thing here.
+#pragma once
+
+#include "base.h"
+#include "collision.h"
+#include "id.h"
+#include "math_functions.h"
+
+#include <stdbool.h>
+#include <stdint.h>
+
+// Task plugin seam of the reference. Accepted and stored for source compatibility; the
+// device pipeline does not call it (there is no host parallel-for on the hot path).
+typedef void b2TaskCallback( int32_t startIndex, int32_t endIndex, uint32_t workerIndex, void* taskContext );
+typedef void* b2EnqueueTaskCallback( b2TaskCallback* task, int32_t itemCount, int32_t minRange, void* taskContext,
+									 void* userContext );
+typedef void b2FinishTaskCallback( void* userTask, void* userContext );
+
+typedef struct b2WorldDef
+{
+	b2Vec2 gravity;
+	float restitutionThreshold;
+	float contactPushoutVelocity;
+	float hitEventThreshold;
+	float contactHertz;
+	float contactDampingRatio;
+	float jointHertz;
+	float jointDampingRatio;
+	bool enableSleep;
+	bool enableContinous;
+	int32_t workerCount;
+	b2EnqueueTaskCallback* enqueueTask;
+	b2FinishTaskCallback* finishTask;
+	void* userTaskContext;
+	int32_t internalValue;
+} b2WorldDef;
+
+B2_API b2WorldDef b2DefaultWorldDef( void );
+
+typedef enum b2BodyType
+{
+	b2_staticBody = 0,
+	b2_kinematicBody = 1,
+	b2_dynamicBody = 2,
+	b2_bodyTypeCount,
+} b2BodyType;
+
+typedef struct b2BodyDef
+{
+	b2BodyType type;
+	b2Vec2 position;
+	b2Rot rotation;
+	b2Vec2 linearVelocity;
+	float angularVelocity;
+	float linearDamping;
+	float angularDamping;
+	float gravityScale;
+	float sleepThreshold;
+	void* userData;
+	bool enableSleep;
+	bool isAwake;
+	bool fixedRotation;
+	bool isBullet;
+	bool isEnabled;
+	bool automaticMass;
+	int32_t internalValue;
+} b2BodyDef;
+
+B2_API b2BodyDef b2DefaultBodyDef( void );
+
+typedef struct b2Filter
+{
+	uint32_t categoryBits;
+	uint32_t maskBits;
+	int32_t groupIndex;
+} b2Filter;
+
+B2_API b2Filter b2DefaultFilter( void );
+
+typedef enum b2ShapeType
+{
+	b2_circleShape,
+	b2_capsuleShape,
+	b2_segmentShape,
+	b2_polygonShape,
+	b2_smoothSegmentShape,
+	b2_shapeTypeCount
+} b2ShapeType;
+
+typedef struct b2ShapeDef
+{
+	void* userData;
+	float friction;
+	float restitution;
+	float density;
+	b2Filter filter;
+	uint32_t customColor;
+	bool isSensor;
+	bool enableSensorEvents;
+	bool enableContactEvents;
+	bool enableHitEvents;
+	bool enablePreSolveEvents;
+	bool forceContactCreation;
+	int32_t internalValue;
+} b2ShapeDef;
+
+B2_API b2ShapeDef b2DefaultShapeDef( void );
+
+// Per-stage wall-clock milliseconds of the last step.
+typedef struct b2Profile
+{
+	float step;
+	float pairs;
+	float collide;
+	float solve;
+	float buildIslands;
+	float solveConstraints;
+	float prepareTasks;
+	float solverTasks;
+	float prepareConstraints;
+	float integrateVelocities;
+	float warmStart;
+	float solveVelocities;
+	float integratePositions;
+	float relaxVelocities;
+	float applyRestitution;
+	float storeImpulses;
+	float finalizeBodies;
+	float splitIslands;
+	float sleepIslands;
+	float hitEvents;
+	float broadphase;
+	float continuous;
+} b2Profile;
+
+// Object counts; colorCounts[11] is the overflow set of the constraint graph.
+typedef struct b2Counters
+{
+	int32_t staticBodyCount;
+	int32_t bodyCount;
+	int32_t shapeCount;
+	int32_t contactCount;
+	int32_t jointCount;
+	int32_t islandCount;
+	int32_t stackUsed;
+	int32_t staticTreeHeight;
+	int32_t treeHeight;
+	int32_t byteCount;
+	int32_t taskCount;
+	int32_t colorCounts[12];
+} b2Counters;
+
+// Touching contact of a body or shape, as returned by b2Body_GetContactData.
+typedef struct b2ContactData
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+	b2Manifold manifold;
+} b2ContactData;
+
+// User filter and pre-solve hooks. They would run inside the data-parallel narrowphase;
+// this library refuses them (see b2World_SetCustomFilterCallback).
+typedef bool b2CustomFilterFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, void* context );
+typedef bool b2PreSolveFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, b2Manifold* manifold, void* context );
