@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import (POINTER, Structure, c_bool, c_float, c_int, c_int32, c_uint16,
+from ctypes import (POINTER, Structure, c_bool, c_char_p, c_float, c_int, c_int32, c_uint16,
                     c_uint32, c_uint64, c_void_p)
 
 import numpy as np
@@ -112,7 +112,7 @@ BODY_DTYPE = np.dtype([("bodyId", "<i4"), ("type", "<i4"), ("setIndex", "<i4"),
 assert CONTACT_DTYPE.itemsize == ctypes.sizeof(ContactRecord)
 assert BODY_DTYPE.itemsize == ctypes.sizeof(BodyRecord)
 
-WORLD_FACTORY = ctypes.CFUNCTYPE(WorldId, POINTER(WorldDef), c_void_p)
+WORLD_FACTORY = ctypes.CFUNCTYPE(c_uint32, POINTER(WorldDef), c_void_p)
 
 
 def _open(path: str) -> ctypes.CDLL:
@@ -252,3 +252,58 @@ class B2Api:
         if n:
             self.lib.b2mi_World_DumpBodies(world, out.ctypes.data, n)
         return out
+
+    # ---- extensions of libb2mi.so (batching, launch shape, timing) ----
+    def bind_product(self):
+        L = self.lib
+        L.b2mi_CreateBatch.restype = c_int
+        L.b2mi_DestroyBatch.argtypes = [c_int]
+        L.b2mi_Batch_CreateWorld.argtypes = [c_int, POINTER(WorldDef)]
+        L.b2mi_Batch_CreateWorld.restype = WorldId
+        L.b2mi_Batch_GetWorldCount.argtypes = [c_int]
+        L.b2mi_Batch_GetWorldCount.restype = c_int
+        L.b2mi_Batch_Step.argtypes = [c_int, c_float, c_int]
+        L.b2mi_Batch_Configure.argtypes = [c_int, c_int, c_int]
+        L.b2mi_World_Configure.argtypes = [WorldId, c_int, c_int]
+        L.b2mi_Batch_GetWorkItemCount.argtypes = [c_int]
+        L.b2mi_Batch_GetWorkItemCount.restype = c_int
+        L.b2mi_Batch_GetTiming.argtypes = [c_int, POINTER(c_float), POINTER(c_float), POINTER(c_float)]
+        L.b2mi_World_GetTiming.argtypes = [WorldId, POINTER(c_float), POINTER(c_float), POINTER(c_float)]
+        L.b2mi_DeviceCount.restype = c_int
+        L.b2mi_GetLastError.restype = c_char_p
+        self._factories = []
+
+    def create_batch(self) -> int:
+        b = self.lib.b2mi_CreateBatch()
+        if b <= 0:
+            raise RuntimeError("no free batch")
+        return b
+
+    def use_batch(self, batch: int | None):
+        """Routes the scene builders' world creation into a batch (None = b2CreateWorld)."""
+        if batch is None:
+            self.scenes.scene_set_world_factory(None, None)
+            return
+        lib = self.lib
+
+        def factory(defp, _ctx):
+            w = lib.b2mi_Batch_CreateWorld(batch, defp)
+            return w.index1 | (w.revision << 16)
+
+        cb = WORLD_FACTORY(factory)
+        self._factories.append(cb)
+        self.scenes.scene_set_world_factory(ctypes.cast(cb, c_void_p), None)
+
+    def batch_step(self, batch: int, dt: float = 1.0 / 60.0, substeps: int = 4, n: int = 1):
+        for _ in range(n):
+            self.lib.b2mi_Batch_Step(batch, c_float(dt), substeps)
+
+    def batch_timing(self, batch: int):
+        a, b, c = c_float(), c_float(), c_float()
+        self.lib.b2mi_Batch_GetTiming(batch, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def world_timing(self, world: WorldId):
+        a, b, c = c_float(), c_float(), c_float()
+        self.lib.b2mi_World_GetTiming(world, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value

@@ -1,0 +1,226 @@
+// Device-resident data layout of the step pipeline, shared by the host driver (world.cpp)
+// and the HIP kernels (kernels.hip). Plain-old-data only.
+//
+// One DevWorld per world holds pointers to that world's device arrays; a step launches one
+// narrowphase kernel over (world, contactId) pairs and one solver kernel over work items.
+// A work item is a union of whole islands (connected components of non-static bodies through
+// touching contacts), so no two work items write the same body and a workgroup can run every
+// stage and substep of its work item with workgroup barriers only.
+#pragma once
+
+#include <stdint.h>
+
+namespace b2mi
+{
+
+enum ShapeType : int
+{
+	kCircle = 0,
+	kCapsule = 1,
+	kSegment = 2,
+	kPolygon = 3,
+};
+
+// body reference stored in a shape: awake sim index, or static sim index | kStaticRef
+constexpr int kStaticRef = 0x40000000;
+
+// DevBodySim::flags
+enum : uint32_t
+{
+	kBodyDynamic = 1u << 0,
+	kBodyKinematic = 1u << 1,
+	kBodyBullet = 1u << 2,
+	kBodyEnableSleep = 1u << 3,
+};
+
+// DevShape::flags
+enum : uint32_t
+{
+	kShapeHitEvents = 1u << 0,
+};
+
+// contact sim flags (same values as the reference's src/contact.h:96-115)
+enum : uint32_t
+{
+	kSimTouching = 0x00010000,
+	kSimDisjoint = 0x00020000,
+	kSimStarted = 0x00040000,
+	kSimStopped = 0x00080000,
+	kSimHitEvent = 0x00100000,
+};
+
+// DevWorld::flags
+enum : uint32_t
+{
+	kWorldWarmStart = 1u << 0,
+	kWorldContinuous = 1u << 1,
+	kWorldSleep = 1u << 2,
+};
+
+// same byte layout as b2ManifoldPoint / b2Manifold of the public API
+struct DevManifoldPoint
+{
+	float pointX, pointY;
+	float anchorAX, anchorAY;
+	float anchorBX, anchorBY;
+	float separation;
+	float normalImpulse;
+	float tangentImpulse;
+	float maxNormalImpulse;
+	float normalVelocity;
+	uint16_t id;
+	uint8_t persisted;
+	uint8_t pad;
+};
+
+struct DevManifold
+{
+	DevManifoldPoint points[2];
+	float normalX, normalY;
+	int pointCount;
+};
+
+struct DevContact
+{
+	DevManifold manifold;
+	int shapeIdA, shapeIdB;
+	uint32_t simFlags;
+	int bodySimIndexA, bodySimIndexB;
+	float invMassA, invIA, invMassB, invIB;
+	float friction, restitution;
+	int pad0, pad1;
+};
+
+struct DevShape
+{
+	int type;
+	int body;
+	uint32_t flags;
+	int count;
+	float radius, friction, restitution, pad;
+	float aabb[4];	  // lower x, lower y, upper x, upper y
+	float fatAABB[4]; // same order
+	float centroid[2];
+	float pad2[2];
+	float v[8][2];
+	float n[8][2];
+};
+
+struct DevBodySim
+{
+	float p[2], q[2]; // transform: origin and rotation (c, s)
+	float center[2], center0[2];
+	float rotation0[2], localCenter[2];
+	float force[2];
+	float torque, mass;
+	float invMass, invI, minExtent, maxExtent;
+	float linearDamping, angularDamping, gravityScale, sleepThreshold;
+	uint32_t flags;
+	int shapeStart, shapeCount, pad;
+};
+
+// Body state (the reference's 32-byte b2BodyState): two float4 per body
+//   s0 = (v.x, v.y, w, flags), s1 = (dp.x, dp.y, dq.c, dq.s)
+
+struct DevWorld
+{
+	float* state;		 // [2 * 4 * awakeCapacity]
+	DevBodySim* sims;	 // awake set
+	DevBodySim* statics; // static set
+	DevShape* shapes;	 // by shape id
+	int* bodyShapes;	 // per awake sim: shape ids, head-first body list order
+	DevContact* contacts; // by contact id
+	float gravityX, gravityY;
+	float dt, inv_dt, h, inv_h;
+	float contactSoft[3]; // biasRate, massScale, impulseScale
+	float staticSoft[3];
+	float restitutionThreshold, pushout;
+	float maxLinearSpeed, maxAngularSpeed;
+	float warmStartScale;
+	int subSteps;
+	uint32_t flags;
+	int pad;
+};
+
+// groups of a work item in execution order: 0 = overflow (serial), 1..11 = colours 0..10
+constexpr int kGroups = 12;
+
+struct WorkItem
+{
+	int world;
+	int bodyStart;	// into the launch's body list / private state
+	int bodyCount;	// owned + shadow (kinematic) bodies
+	int ownedCount; // the first ownedCount bodies are finalized and written back
+	int slotStart;	// into the launch's constraint slots
+	int groupStart[kGroups + 1]; // relative to slotStart
+	int pad[2];
+};
+
+// narrowphase state change (sorted by world, contactId on the host)
+struct ChangeRecord
+{
+	int world;
+	int contactId;
+	uint32_t flags;
+	int pad;
+};
+
+// shape AABB update of finalize (sorted by world, sim index on the host)
+enum : uint32_t
+{
+	kEnlarged = 1u << 0,
+	kFast = 1u << 1,
+	kBulletFast = 1u << 2,
+};
+
+struct EnlargeRecord
+{
+	int world;
+	int simIndex;
+	int shapeId;
+	uint32_t flags;
+	float fatAABB[4];
+	float sweepBox[4]; // fast shapes: union of previous and final AABB (static-tree query box)
+};
+
+// new contact: device record to scatter into DevWorld::contacts[contactId]
+struct ContactInit
+{
+	int world;
+	int contactId;
+	int shapeIdA, shapeIdB;
+	float friction, restitution;
+	int pad[2];
+};
+
+// number of float/int fields of one solver constraint (structure of arrays)
+enum ConstraintField : int
+{
+	cfInvMassA,
+	cfInvMassB,
+	cfInvIA,
+	cfInvIB,
+	cfNormalX,
+	cfNormalY,
+	cfFriction,
+	cfRestitution,
+	cfBiasRate,
+	cfMassScale,
+	cfImpulseScale,
+	// per point p (0, 1): base + 11 * p
+	cfAnchorAX,
+	cfAnchorAY,
+	cfAnchorBX,
+	cfAnchorBY,
+	cfBaseSeparation,
+	cfRelativeVelocity,
+	cfNormalImpulse,
+	cfTangentImpulse,
+	cfMaxNormalImpulse,
+	cfNormalMass,
+	cfTangentMass,
+	cfPointFields = 11,
+	cfFloatCount = cfAnchorAX + 2 * 11,
+};
+
+} // namespace b2mi

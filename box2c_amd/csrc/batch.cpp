@@ -1,0 +1,145 @@
+// Batched worlds: many independent worlds advanced together, one narrowphase launch and one
+// solver launch per step for the whole batch. Each world keeps its own ids, trees, contact
+// lists and colours exactly as a world created with b2CreateWorld would.
+#include "world_internal.hpp"
+
+#include <vector>
+
+namespace b2mi
+{
+World* b2mi_internal_createWorld( const b2WorldDef* def, int firstSlot, int endSlot );
+void b2mi_internal_destroyWorld( World* w );
+const char* b2mi_internal_lastError();
+
+constexpr int kMaxBatches = 256;
+static StepGroup* g_batches[kMaxBatches] = {};
+} // namespace b2mi
+
+using namespace b2mi;
+
+static StepGroup* getBatch( int batch )
+{
+	B2MI_CHECK( 1 <= batch && batch <= kMaxBatches && g_batches[batch - 1] != nullptr );
+	return g_batches[batch - 1];
+}
+
+extern "C" {
+
+B2_API int b2mi_CreateBatch( void )
+{
+	for ( int i = 0; i < kMaxBatches; ++i )
+		if ( g_batches[i] == nullptr )
+		{
+			g_batches[i] = new StepGroup();
+			return i + 1;
+		}
+	return 0;
+}
+
+B2_API void b2mi_DestroyBatch( int batch )
+{
+	StepGroup* g = getBatch( batch );
+	for ( World* w : g->worlds )
+	{
+		w->group = nullptr;
+		b2mi_internal_destroyWorld( w );
+	}
+	g->worlds.clear();
+	delete g;
+	g_batches[batch - 1] = nullptr;
+}
+
+B2_API b2WorldId b2mi_Batch_CreateWorld( int batch, const b2WorldDef* def )
+{
+	StepGroup* g = getBatch( batch );
+	World* w = b2mi_internal_createWorld( def, kMaxPlainWorlds, kMaxWorldSlots );
+	if ( w == nullptr )
+		return b2WorldId{ 0, 0 };
+	w->batchWorld = true;
+	w->group = g;
+	w->groupIndex = (int)g->worlds.size();
+	g->worlds.push_back( w );
+	g->structureDirty = true;
+	g->collideListDirty = true;
+	return b2WorldId{ (uint16_t)( w->slot + 1 ), w->revision };
+}
+
+B2_API int b2mi_Batch_GetWorldCount( int batch )
+{
+	return (int)getBatch( batch )->worlds.size();
+}
+
+// blockSize: threads per workgroup (0 = chosen from the largest colour group);
+// targetSlots: islands are packed into one work item up to this many constraints (0 = 256).
+B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots )
+{
+	StepGroup* g = getBatch( batch );
+	g->blockSize = blockSize;
+	g->targetSlots = targetSlots;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+}
+
+B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount )
+{
+	StepGroup* g = getBatch( batch );
+	if ( timeStep == 0.0f || g->worlds.empty() )
+		return;
+	g->step( timeStep, subStepCount );
+}
+
+B2_API void b2mi_Batch_GetTiming( int batch, float* msCollide, float* msSolve, float* msHost )
+{
+	StepGroup* g = getBatch( batch );
+	*msCollide = g->msCollide;
+	*msSolve = g->msSolve;
+	*msHost = g->msHost;
+}
+
+B2_API int b2mi_Batch_GetWorkItemCount( int batch )
+{
+	return getBatch( batch )->itemCount;
+}
+
+// the same solver settings for a single world (its own one-world group)
+B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots )
+{
+	World* w = getWorld( worldId );
+	B2MI_CHECK( w->batchWorld == false );
+	if ( w->group == nullptr )
+	{
+		w->group = new StepGroup();
+		w->group->worlds.push_back( w );
+		w->groupIndex = 0;
+	}
+	w->group->blockSize = blockSize;
+	w->group->targetSlots = targetSlots;
+	w->topologyDirty = true;
+}
+
+B2_API void b2mi_World_GetTiming( b2WorldId worldId, float* msCollide, float* msSolve, float* msHost )
+{
+	World* w = getWorld( worldId );
+	*msCollide = *msSolve = *msHost = 0.0f;
+	if ( w->group )
+	{
+		*msCollide = w->group->msCollide;
+		*msSolve = w->group->msSolve;
+		*msHost = w->group->msHost;
+	}
+}
+
+B2_API int b2mi_DeviceCount( void )
+{
+	int count = 0;
+	if ( hipGetDeviceCount( &count ) != hipSuccess )
+		return 0;
+	return count;
+}
+
+B2_API const char* b2mi_GetLastError( void )
+{
+	return b2mi_internal_lastError();
+}
+
+} // extern "C"

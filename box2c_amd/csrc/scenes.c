@@ -73,7 +73,8 @@ static float pcg32_normal( Pcg32* r, float sigma )
 }
 
 // ---- world factory (lets a batch API own the world) ----
-typedef b2WorldId SceneWorldFactory( const b2WorldDef* def, void* context );
+// returns the world id packed as index1 | revision << 16 (plain integer for ctypes callers)
+typedef uint32_t SceneWorldFactory( const b2WorldDef* def, void* context );
 static SceneWorldFactory* s_factory = 0;
 static void* s_factoryContext = 0;
 
@@ -86,7 +87,11 @@ SCENE_API void scene_set_world_factory( SceneWorldFactory* factory, void* contex
 static b2WorldId scene_create_world( const b2WorldDef* def )
 {
 	if ( s_factory )
-		return s_factory( def, s_factoryContext );
+	{
+		uint32_t packed = s_factory( def, s_factoryContext );
+		b2WorldId id = { (uint16_t)( packed & 0xffffu ), (uint16_t)( packed >> 16 ) };
+		return id;
+	}
 	return b2CreateWorld( def );
 }
 

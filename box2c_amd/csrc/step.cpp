@@ -1,0 +1,1313 @@
+// Shapes, contacts, broadphase pairs, graph colouring, work-item building and the step of
+// a group of worlds (one world for b2World_Step, many for a batch). Line references are to
+// the reference.
+#include "world_internal.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace b2mi
+{
+
+hipError_t launch_init_contacts( const DevWorld* worlds, const ContactInit* inits, int count, hipStream_t stream );
+hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, ChangeRecord* changes, int* changeCount,
+						   hipStream_t stream );
+hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
+						 const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity, float4* priv,
+						 EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
+
+// defined in world.cpp
+void b2mi_internal_beginEdit( World* w );
+void b2mi_internal_uploadBodies( World* w );
+void b2mi_internal_updateBodyMass( World* w, Body* body );
+void b2mi_internal_downloadMirror( World* w );
+void b2mi_internal_downloadContacts( World* w );
+World* b2mi_internal_worldOrNull( int slot );
+
+template <typename T>
+static void grow( T*& ptr, int& capacity, int needed )
+{
+	if ( needed <= capacity && ptr != nullptr )
+		return;
+	int cap = std::max( 64, capacity );
+	while ( cap < needed )
+		cap *= 2;
+	if ( ptr )
+	{
+		HIP_CHECK( hipStreamSynchronize( stream() ) );
+		HIP_CHECK( hipFree( ptr ) );
+	}
+	HIP_CHECK( hipMalloc( (void**)&ptr, sizeof( T ) * (size_t)cap ) );
+	capacity = cap;
+}
+
+// ---------------------------------------------------------------------------------------
+// shapes (src/shape.c:38-217)
+
+static b2Vec2 shapeCentroid( const Shape& s )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return b2Lerp( s.capsule.center1, s.capsule.center2, 0.5f );
+		case b2_circleShape:
+			return s.circle.center;
+		case b2_polygonShape:
+			return s.polygon.centroid;
+		case b2_segmentShape:
+			return b2Lerp( s.segment.point1, s.segment.point2, 0.5f );
+		default:
+			return b2Vec2{ 0.0f, 0.0f };
+	}
+}
+
+static void fillDevShape( DevShape& d, const Shape& s )
+{
+	memset( &d, 0, sizeof( d ) );
+	d.type = s.type;
+	d.flags = s.enableHitEvents ? kShapeHitEvents : 0u;
+	d.friction = s.friction;
+	d.restitution = s.restitution;
+	switch ( s.type )
+	{
+		case b2_circleShape:
+			d.v[0][0] = s.circle.center.x;
+			d.v[0][1] = s.circle.center.y;
+			d.radius = s.circle.radius;
+			d.count = 1;
+			break;
+		case b2_capsuleShape:
+			d.v[0][0] = s.capsule.center1.x;
+			d.v[0][1] = s.capsule.center1.y;
+			d.v[1][0] = s.capsule.center2.x;
+			d.v[1][1] = s.capsule.center2.y;
+			d.radius = s.capsule.radius;
+			d.count = 2;
+			break;
+		case b2_segmentShape:
+			d.v[0][0] = s.segment.point1.x;
+			d.v[0][1] = s.segment.point1.y;
+			d.v[1][0] = s.segment.point2.x;
+			d.v[1][1] = s.segment.point2.y;
+			d.count = 2;
+			break;
+		case b2_polygonShape:
+			d.count = s.polygon.count;
+			d.radius = s.polygon.radius;
+			d.centroid[0] = s.polygon.centroid.x;
+			d.centroid[1] = s.polygon.centroid.y;
+			for ( int i = 0; i < s.polygon.count; ++i )
+			{
+				d.v[i][0] = s.polygon.vertices[i].x;
+				d.v[i][1] = s.polygon.vertices[i].y;
+				d.n[i][0] = s.polygon.normals[i].x;
+				d.n[i][1] = s.polygon.normals[i].y;
+			}
+			break;
+		default:
+			break;
+	}
+	d.aabb[0] = s.aabb.lowerBound.x;
+	d.aabb[1] = s.aabb.lowerBound.y;
+	d.aabb[2] = s.aabb.upperBound.x;
+	d.aabb[3] = s.aabb.upperBound.y;
+	d.fatAABB[0] = s.fatAABB.lowerBound.x;
+	d.fatAABB[1] = s.fatAABB.lowerBound.y;
+	d.fatAABB[2] = s.fatAABB.upperBound.x;
+	d.fatAABB[3] = s.fatAABB.upperBound.y;
+}
+
+static b2AABB computeShapeAABB( const Shape& s, b2Transform xf )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return b2ComputeCapsuleAABB( &s.capsule, xf );
+		case b2_circleShape:
+			return b2ComputeCircleAABB( &s.circle, xf );
+		case b2_polygonShape:
+			return b2ComputePolygonAABB( &s.polygon, xf );
+		case b2_segmentShape:
+			return b2ComputeSegmentAABB( &s.segment, xf );
+		default:
+			return b2AABB{ xf.p, xf.p };
+	}
+}
+
+static b2ShapeId createShape( b2BodyId bodyId, const b2ShapeDef* def, const void* geometry, b2ShapeType type )
+{
+	B2MI_CHECK( def->internalValue == kCookie );
+	if ( def->isSensor )
+		fatal( "sensor shapes are not supported by the device pipeline", __FILE__, __LINE__ );
+	World* w = worldAt( bodyId.world0 );
+	if ( w->locked )
+		return b2ShapeId{ 0, 0, 0 };
+	int bodyIndex = bodyId.index1 - 1;
+	B2MI_CHECK( 0 <= bodyIndex && bodyIndex < (int)w->bodies.size() );
+	Body* body = &w->bodies[bodyIndex];
+	B2MI_CHECK( body->id == bodyIndex && body->revision == bodyId.revision );
+	b2mi_internal_beginEdit( w );
+
+	const DevBodySim& sim = body->setIndex == 0 ? w->staticSims[body->localIndex] : w->awakeSims[body->localIndex];
+	b2Transform xf = { { sim.p[0], sim.p[1] }, { sim.q[0], sim.q[1] } };
+
+	int shapeId = w->shapeIdPool.alloc();
+	if ( shapeId == (int)w->shapes.size() )
+	{
+		w->shapes.push_back( Shape{} );
+		w->devShapes.push_back( DevShape{} );
+	}
+	Shape& s = w->shapes[shapeId];
+	uint16_t revision = s.revision;
+	s = Shape{};
+	switch ( type )
+	{
+		case b2_capsuleShape:
+			s.capsule = *(const b2Capsule*)geometry;
+			break;
+		case b2_circleShape:
+			s.circle = *(const b2Circle*)geometry;
+			break;
+		case b2_polygonShape:
+			s.polygon = *(const b2Polygon*)geometry;
+			break;
+		case b2_segmentShape:
+			s.segment = *(const b2Segment*)geometry;
+			break;
+		default:
+			fatal( "unsupported shape type", __FILE__, __LINE__ );
+	}
+	s.id = shapeId;
+	s.bodyId = body->id;
+	s.type = type;
+	s.density = def->density;
+	s.friction = def->friction;
+	s.restitution = def->restitution;
+	s.filter = def->filter;
+	s.userData = def->userData;
+	s.enableHitEvents = def->enableHitEvents;
+	s.enableContactEvents = def->enableContactEvents;
+	s.revision = revision + 1;
+	(void)shapeCentroid;
+
+	// b2CreateShapeProxy / b2UpdateShapeAABBs (shape.c:38-59, 692-702)
+	b2AABB aabb = computeShapeAABB( s, xf );
+	aabb.lowerBound.x -= kSpeculative;
+	aabb.lowerBound.y -= kSpeculative;
+	aabb.upperBound.x += kSpeculative;
+	aabb.upperBound.y += kSpeculative;
+	s.aabb = aabb;
+	float margin = body->type == b2_staticBody ? kSpeculative : kAabbMargin;
+	s.fatAABB = b2AABB{ { aabb.lowerBound.x - margin, aabb.lowerBound.y - margin },
+						{ aabb.upperBound.x + margin, aabb.upperBound.y + margin } };
+	int proxyType = body->type;
+	int proxyId = w->trees[proxyType].createProxy( s.fatAABB, s.filter.categoryBits, shapeId );
+	s.proxyKey = ( proxyId << 2 ) | proxyType;
+	if ( proxyType != b2_staticBody || def->forceContactCreation )
+	{
+		if ( !w->moveSet.get( (uint32_t)s.proxyKey ) )
+		{
+			w->moveSet.set( (uint32_t)s.proxyKey );
+			w->moveArray.push_back( s.proxyKey );
+		}
+	}
+
+	// head insertion in the body's shape list (shape.c:137-148)
+	if ( body->headShapeId != -1 )
+		w->shapes[body->headShapeId].prevShapeId = shapeId;
+	s.prevShapeId = -1;
+	s.nextShapeId = body->headShapeId;
+	body->headShapeId = shapeId;
+	body->shapeCount += 1;
+	fillDevShape( w->devShapes[shapeId], s );
+	if ( body->automaticMass )
+		b2mi_internal_updateBodyMass( w, body );
+	return b2ShapeId{ shapeId + 1, bodyId.world0, s.revision };
+}
+
+// ---------------------------------------------------------------------------------------
+// contacts (src/contact.c:185-414, src/constraint_graph.c:64-214)
+
+// which shape type pairs have a manifold function and which order is primary
+// (src/contact.c:150-183); circle=0 capsule=1 segment=2 polygon=3
+static bool manifoldFcn( int typeA, int typeB, bool* primary )
+{
+	static const signed char table[4][4] = {
+		{ 1, -1, -1, -1 },
+		{ 1, 1, -1, -1 },
+		{ 1, 1, 0, 1 },
+		{ 1, 1, -1, 1 },
+	};
+	if ( typeA < 0 || typeA > 3 || typeB < 0 || typeB > 3 )
+		return false;
+	int v = table[typeA][typeB];
+	if ( v == 0 )
+		return false;
+	*primary = v > 0;
+	return true;
+}
+
+static uint64_t pairKey( int a, int b )
+{
+	return a < b ? ( uint64_t( a ) << 32 | uint64_t( b ) ) : ( uint64_t( b ) << 32 | uint64_t( a ) );
+}
+
+static void createContact( World* w, Shape* shapeA, Shape* shapeB )
+{
+	bool primary;
+	if ( !manifoldFcn( shapeA->type, shapeB->type, &primary ) )
+		return;
+	if ( !primary )
+	{
+		createContact( w, shapeB, shapeA );
+		return;
+	}
+	int contactId = w->contactIdPool.alloc();
+	if ( contactId == (int)w->contacts.size() )
+		w->contacts.push_back( Contact{} );
+	Contact& c = w->contacts[contactId];
+	c.contactId = contactId;
+	c.setIndex = 2;
+	c.colorIndex = -1;
+	c.localIndex = (int)w->nonTouching.size();
+	c.shapeIdA = shapeA->id;
+	c.shapeIdB = shapeB->id;
+	c.flags = 0;
+	Body* bodies[2] = { &w->bodies[shapeA->bodyId], &w->bodies[shapeB->bodyId] };
+	for ( int e = 0; e < 2; ++e )
+	{
+		Body* b = bodies[e];
+		c.edges[e].bodyId = b->id;
+		c.edges[e].prevKey = -1;
+		c.edges[e].nextKey = b->headContactKey;
+		int key = ( contactId << 1 ) | e;
+		int head = b->headContactKey;
+		if ( head != -1 )
+			w->contacts[head >> 1].edges[head & 1].prevKey = key;
+		b->headContactKey = key;
+		b->contactCount += 1;
+	}
+	w->pairSet.insert( pairKey( shapeA->id, shapeB->id ) );
+	w->nonTouching.push_back( contactId );
+
+	ContactInit init{};
+	init.world = w->groupIndex;
+	init.contactId = contactId;
+	init.shapeIdA = shapeA->id;
+	init.shapeIdB = shapeB->id;
+	init.friction = sqrtf( shapeA->friction * shapeB->friction );
+	init.restitution = shapeA->restitution > shapeB->restitution ? shapeA->restitution : shapeB->restitution;
+	w->pendingInits.push_back( init );
+	w->topologyDirty = true;
+}
+
+static void removeFromList( World* w, std::vector<int>& list, int localIndex )
+{
+	int last = (int)list.size() - 1;
+	if ( localIndex < last )
+	{
+		list[localIndex] = list[last];
+		w->contacts[list[localIndex]].localIndex = localIndex;
+	}
+	list.pop_back();
+}
+
+// src/constraint_graph.c:64-183 (colour choice only; the sim lives on the device by id)
+static void addContactToGraph( World* w, Contact& c )
+{
+	int colorIndex = kOverflow;
+	int bodyIdA = c.edges[0].bodyId, bodyIdB = c.edges[1].bodyId;
+	bool staticA = w->bodies[bodyIdA].setIndex == 0, staticB = w->bodies[bodyIdB].setIndex == 0;
+	if ( !staticA && !staticB )
+	{
+		for ( int i = 0; i < kOverflow; ++i )
+		{
+			BitSet& set = w->colorBodySets[i];
+			if ( set.get( bodyIdA ) || set.get( bodyIdB ) )
+				continue;
+			set.set( bodyIdA );
+			set.set( bodyIdB );
+			colorIndex = i;
+			break;
+		}
+	}
+	else if ( !staticA )
+	{
+		for ( int i = 1; i < kOverflow; ++i )
+		{
+			BitSet& set = w->colorBodySets[i];
+			if ( set.get( bodyIdA ) )
+				continue;
+			set.set( bodyIdA );
+			colorIndex = i;
+			break;
+		}
+	}
+	else if ( !staticB )
+	{
+		for ( int i = 1; i < kOverflow; ++i )
+		{
+			BitSet& set = w->colorBodySets[i];
+			if ( set.get( bodyIdB ) )
+				continue;
+			set.set( bodyIdB );
+			colorIndex = i;
+			break;
+		}
+	}
+	c.colorIndex = colorIndex;
+	c.localIndex = (int)w->colors[colorIndex].size();
+	w->colors[colorIndex].push_back( c.contactId );
+}
+
+// src/constraint_graph.c:185-214
+static void removeContactFromGraph( World* w, int bodyIdA, int bodyIdB, int colorIndex, int localIndex )
+{
+	if ( colorIndex != kOverflow )
+	{
+		w->colorBodySets[colorIndex].clear( bodyIdA );
+		w->colorBodySets[colorIndex].clear( bodyIdB );
+	}
+	removeFromList( w, w->colors[colorIndex], localIndex );
+}
+
+// src/contact.c:335-414
+void destroyContactImpl( World* w, Contact* c )
+{
+	w->pairSet.erase( pairKey( c->shapeIdA, c->shapeIdB ) );
+	int contactId = c->contactId;
+	for ( int e = 0; e < 2; ++e )
+	{
+		ContactEdge& edge = c->edges[e];
+		Body& b = w->bodies[edge.bodyId];
+		if ( edge.prevKey != -1 )
+			w->contacts[edge.prevKey >> 1].edges[edge.prevKey & 1].nextKey = edge.nextKey;
+		if ( edge.nextKey != -1 )
+			w->contacts[edge.nextKey >> 1].edges[edge.nextKey & 1].prevKey = edge.prevKey;
+		int key = ( contactId << 1 ) | e;
+		if ( b.headContactKey == key )
+			b.headContactKey = edge.nextKey;
+		b.contactCount -= 1;
+	}
+	if ( c->colorIndex != -1 )
+		removeContactFromGraph( w, c->edges[0].bodyId, c->edges[1].bodyId, c->colorIndex, c->localIndex );
+	else
+		removeFromList( w, w->nonTouching, c->localIndex );
+	c->contactId = -1;
+	c->setIndex = -1;
+	c->colorIndex = -1;
+	c->localIndex = -1;
+	w->contactIdPool.free( contactId );
+	w->topologyDirty = true;
+}
+
+// ---------------------------------------------------------------------------------------
+// broadphase pair update (src/broad_phase.c:162-436)
+
+static bool shouldShapesCollide( b2Filter a, b2Filter b )
+{
+	if ( a.groupIndex == b.groupIndex && a.groupIndex != 0 )
+		return a.groupIndex > 0;
+	return ( a.maskBits & b.categoryBits ) != 0 && ( a.categoryBits & b.maskBits ) != 0;
+}
+
+struct MovePair
+{
+	int a, b, next;
+};
+
+static void updatePairs( World* w )
+{
+	int moveCount = (int)w->moveArray.size();
+	if ( moveCount == 0 )
+		return;
+	std::vector<MovePair> pairs;
+	std::vector<int> heads( moveCount, -1 );
+	for ( int i = 0; i < moveCount; ++i )
+	{
+		int queryProxyKey = w->moveArray[i];
+		int proxyType = queryProxyKey & 3;
+		const TreeNode& node = w->trees[proxyType].nodes[queryProxyKey >> 2];
+		b2AABB fatAABB = node.aabb;
+		int queryShapeIndex = node.userData;
+		int* head = &heads[i];
+		auto query = [&]( int treeType ) {
+			w->trees[treeType].query( fatAABB, b2_defaultMaskBits, [&]( int proxyId, int shapeId ) {
+				int proxyKey = ( proxyId << 2 ) | treeType;
+				if ( proxyKey == queryProxyKey )
+					return true;
+				if ( treeType != b2_staticBody && w->moveSet.get( (uint32_t)proxyKey ) && proxyKey < queryProxyKey )
+					return true;
+				if ( w->pairSet.count( pairKey( shapeId, queryShapeIndex ) ) )
+					return true;
+				int shapeIdA, shapeIdB;
+				if ( proxyKey < queryProxyKey )
+				{
+					shapeIdA = shapeId;
+					shapeIdB = queryShapeIndex;
+				}
+				else
+				{
+					shapeIdA = queryShapeIndex;
+					shapeIdB = shapeId;
+				}
+				const Shape& sa = w->shapes[shapeIdA];
+				const Shape& sb = w->shapes[shapeIdB];
+				if ( sa.bodyId == sb.bodyId )
+					return true;
+				if ( !shouldShapesCollide( sa.filter, sb.filter ) )
+					return true;
+				if ( w->bodies[sa.bodyId].type != b2_dynamicBody && w->bodies[sb.bodyId].type != b2_dynamicBody )
+					return true;
+				pairs.push_back( MovePair{ shapeIdA, shapeIdB, *head } );
+				*head = (int)pairs.size() - 1;
+				return true;
+			} );
+		};
+		if ( proxyType == b2_dynamicBody )
+		{
+			query( b2_kinematicBody );
+			query( b2_staticBody );
+		}
+		query( b2_dynamicBody );
+	}
+	for ( int i = 0; i < moveCount; ++i )
+		for ( int p = heads[i]; p != -1; p = pairs[p].next )
+			createContact( w, &w->shapes[pairs[p].a], &w->shapes[pairs[p].b] );
+	for ( int key : w->moveArray )
+		w->moveSet.clear( (uint32_t)key );
+	w->moveArray.clear();
+}
+
+// ---------------------------------------------------------------------------------------
+// work items: unions of islands (connected components through touching contacts of the
+// non-static, non-kinematic bodies), packed to a target size
+
+struct Component
+{
+	std::vector<int> bodies; // awake sim indices
+	int slotCount = 0;
+};
+
+static int findRoot( std::vector<int>& parent, int x )
+{
+	while ( parent[x] != x )
+	{
+		parent[x] = parent[parent[x]];
+		x = parent[x];
+	}
+	return x;
+}
+
+static void buildWorkItems( StepGroup* g )
+{
+	g->hItems.clear();
+	g->hBodyList.clear();
+	g->hSlotContact.clear();
+	g->hSlotBodies.clear();
+	g->maxGroupSize = 0;
+	int target = g->targetSlots > 0 ? g->targetSlots : 256;
+
+	for ( World* w : g->worlds )
+	{
+		int na = (int)w->awakeSimBody.size();
+		if ( na == 0 )
+			continue;
+		std::vector<int> parent( na );
+		for ( int i = 0; i < na; ++i )
+			parent[i] = i;
+		auto simOfBody = [&]( int bodyId ) -> int {
+			const Body& b = w->bodies[bodyId];
+			return b.setIndex == 2 && b.type == b2_dynamicBody ? b.localIndex : -1;
+		};
+		for ( int c = 0; c < kColorCount; ++c )
+			for ( int id : w->colors[c] )
+			{
+				const Contact& ct = w->contacts[id];
+				int a = simOfBody( ct.edges[0].bodyId ), b = simOfBody( ct.edges[1].bodyId );
+				if ( a >= 0 && b >= 0 )
+				{
+					int ra = findRoot( parent, a ), rb = findRoot( parent, b );
+					if ( ra != rb )
+						parent[std::max( ra, rb )] = std::min( ra, rb );
+				}
+			}
+		// components in order of their smallest sim index; kinematic bodies alone
+		std::vector<int> compOf( na, -1 );
+		std::vector<Component> comps;
+		for ( int i = 0; i < na; ++i )
+		{
+			int r = findRoot( parent, i );
+			if ( compOf[r] < 0 )
+			{
+				compOf[r] = (int)comps.size();
+				comps.push_back( Component{} );
+			}
+			compOf[i] = compOf[r];
+			comps[compOf[i]].bodies.push_back( i );
+		}
+		// a contact belongs to the component of its dynamic body
+		auto contactComp = [&]( const Contact& ct ) {
+			int a = simOfBody( ct.edges[0].bodyId );
+			if ( a < 0 )
+				a = simOfBody( ct.edges[1].bodyId );
+			return compOf[a];
+		};
+		std::vector<std::vector<int>> compSlots[kColorCount];
+		for ( int c = 0; c < kColorCount; ++c )
+			compSlots[c].assign( comps.size(), std::vector<int>() );
+		for ( int c = 0; c < kColorCount; ++c )
+			for ( int id : w->colors[c] )
+			{
+				int k = contactComp( w->contacts[id] );
+				compSlots[c][k].push_back( id );
+				comps[k].slotCount += 1;
+			}
+
+		// greedy packing in component order
+		size_t ci = 0;
+		while ( ci < comps.size() )
+		{
+			size_t cj = ci;
+			int slots = 0, bodies = 0;
+			do
+			{
+				slots += comps[cj].slotCount;
+				bodies += (int)comps[cj].bodies.size();
+				cj += 1;
+			} while ( cj < comps.size() && slots + comps[cj].slotCount <= target && bodies + (int)comps[cj].bodies.size() <= 4 * target );
+
+			WorkItem wi{};
+			wi.world = w->groupIndex;
+			wi.bodyStart = (int)g->hBodyList.size();
+			std::vector<int> localOf( 0 );
+			// owned bodies
+			std::vector<int> items;
+			for ( size_t k = ci; k < cj; ++k )
+				for ( int b : comps[k].bodies )
+					g->hBodyList.push_back( b );
+			wi.ownedCount = (int)g->hBodyList.size() - wi.bodyStart;
+			// local index map (owned, then kinematic shadows on demand)
+			std::vector<std::pair<int, int>> shadowMap;
+			auto localIndex = [&]( int bodyId ) -> int {
+				const Body& b = w->bodies[bodyId];
+				if ( b.setIndex != 2 )
+					return -1;
+				if ( b.type == b2_dynamicBody )
+				{
+					// position within the owned range: search (components are small) via map
+					return -2 - b.localIndex; // resolved below
+				}
+				for ( auto& pr : shadowMap )
+					if ( pr.first == b.localIndex )
+						return pr.second;
+				int li = (int)g->hBodyList.size() - wi.bodyStart;
+				g->hBodyList.push_back( b.localIndex );
+				shadowMap.push_back( { b.localIndex, li } );
+				return li;
+			};
+			std::vector<int> ownedPos;
+			{
+				// sim index -> owned position for this work item
+				int n = wi.ownedCount;
+				ownedPos.clear();
+				for ( int k = 0; k < n; ++k )
+					ownedPos.push_back( g->hBodyList[wi.bodyStart + k] );
+			}
+			std::vector<int> posOfSim;
+			int maxSim = 0;
+			for ( int s : ownedPos )
+				maxSim = std::max( maxSim, s );
+			posOfSim.assign( maxSim + 1, -1 );
+			for ( int k = 0; k < (int)ownedPos.size(); ++k )
+				posOfSim[ownedPos[k]] = k;
+			auto resolve = [&]( int li ) { return li <= -2 ? posOfSim[-2 - li] : li; };
+
+			wi.slotStart = (int)g->hSlotContact.size();
+			// group 0 = overflow (colour 11) then colours 0..10
+			for ( int gidx = 0; gidx < kGroups; ++gidx )
+			{
+				int color = gidx == 0 ? kOverflow : gidx - 1;
+				wi.groupStart[gidx] = (int)g->hSlotContact.size() - wi.slotStart;
+				for ( size_t k = ci; k < cj; ++k )
+					for ( int id : compSlots[color][k] )
+					{
+						const Contact& ct = w->contacts[id];
+						int la = resolve( localIndex( ct.edges[0].bodyId ) );
+						int lb = resolve( localIndex( ct.edges[1].bodyId ) );
+						g->hSlotContact.push_back( id );
+						g->hSlotBodies.push_back( make_int2( la, lb ) );
+					}
+				int size = (int)g->hSlotContact.size() - wi.slotStart - wi.groupStart[gidx];
+				g->maxGroupSize = std::max( g->maxGroupSize, size );
+			}
+			wi.groupStart[kGroups] = (int)g->hSlotContact.size() - wi.slotStart;
+			wi.bodyCount = (int)g->hBodyList.size() - wi.bodyStart;
+			g->maxGroupSize = std::max( g->maxGroupSize, wi.ownedCount );
+			g->hItems.push_back( wi );
+			ci = cj;
+		}
+		w->topologyDirty = false;
+	}
+	g->itemCount = (int)g->hItems.size();
+	g->slotCount = (int)g->hSlotContact.size();
+	g->bodyListCount = (int)g->hBodyList.size();
+
+	// every index the solver kernel dereferences is checked here, on the host
+	for ( const WorkItem& wi : g->hItems )
+	{
+		World* w = g->worlds[wi.world];
+		int na = (int)w->awakeSimBody.size();
+		B2MI_CHECK( wi.bodyStart >= 0 && wi.bodyStart + wi.bodyCount <= g->bodyListCount );
+		B2MI_CHECK( 0 <= wi.ownedCount && wi.ownedCount <= wi.bodyCount );
+		for ( int i = 0; i < wi.bodyCount; ++i )
+			B2MI_CHECK( 0 <= g->hBodyList[wi.bodyStart + i] && g->hBodyList[wi.bodyStart + i] < na );
+		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= g->slotCount );
+		for ( int k = 0; k < kGroups; ++k )
+			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
+		for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
+		{
+			int slot = wi.slotStart + k;
+			int2 lb = g->hSlotBodies[slot];
+			B2MI_CHECK( -1 <= lb.x && lb.x < wi.bodyCount && -1 <= lb.y && lb.y < wi.bodyCount );
+			B2MI_CHECK( 0 <= g->hSlotContact[slot] && g->hSlotContact[slot] < w->capContacts );
+		}
+	}
+
+	hipStream_t s = stream();
+	grow( g->dItems, g->capItems, std::max( g->itemCount, 1 ) );
+	grow( g->dBodyList, g->capBodyList, std::max( g->bodyListCount, 1 ) );
+	int slotCap = g->capSlots;
+	grow( g->dSlotContact, slotCap, std::max( g->slotCount, 1 ) );
+	grow( g->dSlotBodies, g->capSlots, std::max( g->slotCount, 1 ) );
+	int consNeeded = std::max( g->slotCount, 1 );
+	if ( g->capCons < consNeeded || g->dCons == nullptr )
+	{
+		if ( g->dCons )
+			HIP_CHECK( hipFree( g->dCons ) );
+		g->capCons = std::max( 64, consNeeded + consNeeded / 4 );
+		HIP_CHECK( hipMalloc( (void**)&g->dCons, sizeof( float ) * (size_t)cfFloatCount * g->capCons ) );
+	}
+	grow( g->dPriv, g->capPriv, 2 * std::max( g->bodyListCount, 1 ) );
+	if ( g->itemCount )
+		HIP_CHECK( hipMemcpyAsync( g->dItems, g->hItems.data(), sizeof( WorkItem ) * g->itemCount, hipMemcpyHostToDevice, s ) );
+	if ( g->bodyListCount )
+		HIP_CHECK( hipMemcpyAsync( g->dBodyList, g->hBodyList.data(), sizeof( int ) * g->bodyListCount, hipMemcpyHostToDevice, s ) );
+	if ( g->slotCount )
+	{
+		HIP_CHECK( hipMemcpyAsync( g->dSlotContact, g->hSlotContact.data(), sizeof( int ) * g->slotCount, hipMemcpyHostToDevice, s ) );
+		HIP_CHECK( hipMemcpyAsync( g->dSlotBodies, g->hSlotBodies.data(), sizeof( int2 ) * g->slotCount, hipMemcpyHostToDevice, s ) );
+	}
+}
+
+static void buildCollideList( StepGroup* g )
+{
+	g->hCollide.clear();
+	for ( World* w : g->worlds )
+	{
+		for ( int c = 0; c < kColorCount; ++c )
+			for ( int id : w->colors[c] )
+				g->hCollide.push_back( make_int2( w->groupIndex, id ) );
+		for ( int id : w->nonTouching )
+			g->hCollide.push_back( make_int2( w->groupIndex, id ) );
+	}
+	g->collideCount = (int)g->hCollide.size();
+	grow( g->dCollideList, g->capCollide, std::max( g->collideCount, 1 ) );
+	if ( g->collideCount )
+		HIP_CHECK( hipMemcpyAsync( g->dCollideList, g->hCollide.data(), sizeof( int2 ) * g->collideCount, hipMemcpyHostToDevice,
+								   stream() ) );
+	grow( g->dChanges, g->capChanges, std::max( g->collideCount, 1 ) );
+}
+
+// src/solver.h:144-156
+static void makeSoft( float hertz, float zeta, float h, float* out )
+{
+	if ( hertz == 0.0f )
+	{
+		out[0] = 0.0f;
+		out[1] = 1.0f;
+		out[2] = 0.0f;
+		return;
+	}
+	float omega = 2.0f * b2_pi * hertz;
+	float a1 = 2.0f * zeta + h * omega;
+	float a2 = h * omega * a1;
+	float a3 = 1.0f / ( 1.0f + a2 );
+	out[0] = omega / a1;
+	out[1] = a2 * a3;
+	out[2] = a3;
+}
+
+static void fillDevWorld( DevWorld& d, World* w, float timeStep, int subStepCount )
+{
+	memset( &d, 0, sizeof( d ) );
+	d.state = w->dState;
+	d.sims = w->dSims;
+	d.statics = w->dStatics;
+	d.shapes = w->dShapes;
+	d.bodyShapes = w->dBodyShapes;
+	d.contacts = w->dContacts;
+	d.gravityX = w->gravity.x;
+	d.gravityY = w->gravity.y;
+	// src/world.c:717-746
+	int sub = std::max( 1, subStepCount );
+	d.subSteps = sub;
+	d.dt = timeStep;
+	d.inv_dt = timeStep > 0.0f ? 1.0f / timeStep : 0.0f;
+	d.h = timeStep > 0.0f ? timeStep / sub : 0.0f;
+	d.inv_h = timeStep > 0.0f ? sub * d.inv_dt : 0.0f;
+	float contactHertz = b2MinFloat( w->contactHertz, 0.25f * d.inv_h );
+	makeSoft( contactHertz, w->contactDampingRatio, d.h, d.contactSoft );
+	makeSoft( 2.0f * contactHertz, w->contactDampingRatio, d.h, d.staticSoft );
+	d.restitutionThreshold = w->restitutionThreshold;
+	d.pushout = w->contactPushoutVelocity;
+	d.maxLinearSpeed = 4.0f * d.inv_dt;				  // b2_maxTranslation * inv_dt
+	d.maxAngularSpeed = ( 0.25f * b2_pi ) * d.inv_dt; // b2_maxRotation * inv_dt
+	d.warmStartScale = w->enableWarmStarting ? 1.0f : 0.0f;
+	d.flags = ( w->enableWarmStarting ? kWorldWarmStart : 0u ) | ( w->enableContinuous ? kWorldContinuous : 0u ) |
+			  ( w->enableSleep ? kWorldSleep : 0u );
+}
+
+// ensure the device contact array covers every allocated contact id
+static void ensureContactCapacity( World* w )
+{
+	int needed = std::max( w->contactIdPool.nextIndex, 1 );
+	if ( needed <= w->capContacts && w->dContacts )
+		return;
+	int cap = std::max( 64, w->capContacts );
+	while ( cap < needed )
+		cap *= 2;
+	DevContact* p = nullptr;
+	HIP_CHECK( hipMalloc( (void**)&p, sizeof( DevContact ) * (size_t)cap ) );
+	if ( w->dContacts && w->capContacts > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( p, w->dContacts, sizeof( DevContact ) * (size_t)w->capContacts, hipMemcpyDeviceToDevice, stream() ) );
+		HIP_CHECK( hipStreamSynchronize( stream() ) );
+		HIP_CHECK( hipFree( w->dContacts ) );
+	}
+	w->dContacts = p;
+	w->capContacts = cap;
+	if ( w->group )
+		w->group->structureDirty = true;
+}
+
+static void bufferMove( World* w, int proxyKey )
+{
+	// src/broad_phase.h:76-83
+	if ( !w->moveSet.get( (uint32_t)proxyKey ) )
+	{
+		w->moveSet.set( (uint32_t)proxyKey );
+		w->moveArray.push_back( proxyKey );
+	}
+}
+
+// Enlarge pass after finalize (src/solver.c:1741-1797 and 1877-1942 for fast shapes).
+static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
+{
+	if ( count == 0 )
+		return;
+	std::vector<const EnlargeRecord*> order( count );
+	for ( int i = 0; i < count; ++i )
+		order[i] = recs + i;
+	// sim order; within a body the head-first shape list order
+	std::vector<int> rank( w->shapes.size(), 0 );
+	for ( int i = 0; i < count; ++i )
+	{
+		const Body& b = w->bodies[w->shapes[recs[i].shapeId].bodyId];
+		int r = 0;
+		for ( int id = b.headShapeId; id != -1; id = w->shapes[id].nextShapeId, ++r )
+			if ( id == recs[i].shapeId )
+				rank[id] = r;
+	}
+	std::sort( order.begin(), order.end(), [&]( const EnlargeRecord* a, const EnlargeRecord* b ) {
+		if ( a->simIndex != b->simIndex )
+			return a->simIndex < b->simIndex;
+		return rank[a->shapeId] < rank[b->shapeId];
+	} );
+	for ( const EnlargeRecord* r : order )
+	{
+		Shape& s = w->shapes[r->shapeId];
+		b2AABB fat = { { r->fatAABB[0], r->fatAABB[1] }, { r->fatAABB[2], r->fatAABB[3] } };
+		if ( r->flags & kEnlarged )
+			s.fatAABB = fat;
+		if ( r->flags & kFast )
+		{
+			if ( r->flags & kBulletFast )
+				fatal( "continuous collision of bullet bodies is not supported", __FILE__, __LINE__ );
+			// the sweep must not meet a static shape, or time of impact would be needed
+			b2AABB box = { { r->sweepBox[0], r->sweepBox[1] }, { r->sweepBox[2], r->sweepBox[3] } };
+			int hits = 0;
+			w->trees[b2_staticBody].query( box, b2_defaultMaskBits, [&]( int, int shapeId ) {
+				const Shape& other = w->shapes[shapeId];
+				if ( other.bodyId != s.bodyId && shouldShapesCollide( s.filter, other.filter ) )
+					hits += 1;
+				return true;
+			} );
+			if ( hits )
+				fatal( "continuous collision against static shapes (time of impact) is not supported", __FILE__, __LINE__ );
+			bufferMove( w, s.proxyKey );
+		}
+		else if ( r->flags & kEnlarged )
+		{
+			w->trees[s.proxyKey & 3].enlargeProxy( s.proxyKey >> 2, fat );
+			bufferMove( w, s.proxyKey );
+		}
+	}
+	for ( const EnlargeRecord* r : order )
+		if ( ( r->flags & kFast ) && ( r->flags & kEnlarged ) )
+		{
+			const Shape& s = w->shapes[r->shapeId];
+			w->trees[b2_dynamicBody].enlargeProxy( s.proxyKey >> 2, s.fatAABB );
+		}
+}
+
+void StepGroup::step( float timeStep, int subStepCount )
+{
+	auto t0 = std::chrono::steady_clock::now();
+	hipStream_t s = stream();
+	for ( World* w : worlds )
+	{
+		B2MI_CHECK( w->locked == false );
+		w->locked = true;
+		w->profile = b2Profile{};
+		if ( w->bodiesDirty )
+			b2mi_internal_uploadBodies( w );
+	}
+
+	// 1. pairs and new contacts (host, serial per world)
+	hInits.clear();
+	for ( World* w : worlds )
+	{
+		auto tp = std::chrono::steady_clock::now();
+		updatePairs( w );
+		ensureContactCapacity( w );
+		hInits.insert( hInits.end(), w->pendingInits.begin(), w->pendingInits.end() );
+		w->pendingInits.clear();
+		w->profile.pairs = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - tp ).count();
+	}
+	if ( structureDirty || (int)hWorlds.size() != (int)worlds.size() )
+	{
+		hWorlds.resize( worlds.size() );
+		structureDirty = true;
+	}
+	bool paramsChanged = structureDirty;
+	for ( size_t i = 0; i < worlds.size(); ++i )
+	{
+		DevWorld d;
+		fillDevWorld( d, worlds[i], timeStep, subStepCount );
+		if ( memcmp( &d, &hWorlds[i], sizeof( DevWorld ) ) != 0 )
+		{
+			hWorlds[i] = d;
+			paramsChanged = true;
+		}
+	}
+	if ( paramsChanged )
+	{
+		grow( dWorlds, capWorlds, (int)worlds.size() );
+		HIP_CHECK( hipMemcpyAsync( dWorlds, hWorlds.data(), sizeof( DevWorld ) * worlds.size(), hipMemcpyHostToDevice, s ) );
+		structureDirty = false;
+	}
+	if ( dCounters == nullptr )
+	{
+		HIP_CHECK( hipMalloc( (void**)&dCounters, sizeof( int ) * 4 ) );
+		HIP_CHECK( hipHostMalloc( (void**)&hCounters, sizeof( int ) * 4, hipHostMallocDefault ) );
+	}
+	if ( !hInits.empty() )
+	{
+		grow( dInits, capInits, (int)hInits.size() );
+		HIP_CHECK( hipMemcpyAsync( dInits, hInits.data(), sizeof( ContactInit ) * hInits.size(), hipMemcpyHostToDevice, s ) );
+		HIP_CHECK( launch_init_contacts( dWorlds, dInits, (int)hInits.size(), s ) );
+	}
+
+	// 2. narrowphase
+	bool anyTopology = false;
+	for ( World* w : worlds )
+		anyTopology = anyTopology || w->topologyDirty;
+	if ( anyTopology || collideListDirty )
+	{
+		buildCollideList( this );
+		collideListDirty = false;
+	}
+	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
+	hipEvent_t e0, e1, e2;
+	HIP_CHECK( hipEventCreate( &e0 ) );
+	HIP_CHECK( hipEventCreate( &e1 ) );
+	HIP_CHECK( hipEventCreate( &e2 ) );
+	HIP_CHECK( hipEventRecord( e0, s ) );
+	HIP_CHECK( launch_collide( dWorlds, dCollideList, collideCount, dChanges, dCounters, s ) );
+	HIP_CHECK( hipEventRecord( e1, s ) );
+	HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+
+	// the tree rebuild overlaps the narrowphase (src/world.c:367-379, 420)
+	for ( World* w : worlds )
+	{
+		w->trees[b2_dynamicBody].rebuild( false );
+		w->trees[b2_kinematicBody].rebuild( false );
+	}
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	int changeCount = hCounters[0];
+	if ( changeCount > 0 )
+	{
+		hChanges.resize( changeCount );
+		HIP_CHECK( hipMemcpyAsync( hChanges.data(), dChanges, sizeof( ChangeRecord ) * changeCount, hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		std::sort( hChanges.begin(), hChanges.end(), []( const ChangeRecord& a, const ChangeRecord& b ) {
+			return a.world != b.world ? a.world < b.world : a.contactId < b.contactId;
+		} );
+		// serial state changes in contact-id order (src/world.c:506-667)
+		for ( const ChangeRecord& r : hChanges )
+		{
+			World* w = worlds[r.world];
+			Contact& c = w->contacts[r.contactId];
+			int colorIndex = c.colorIndex, localIndex = c.localIndex;
+			if ( r.flags & kSimDisjoint )
+			{
+				c.flags &= ~1u;
+				destroyContactImpl( w, &c );
+			}
+			else if ( r.flags & kSimStarted )
+			{
+				c.flags |= 1u;
+				addContactToGraph( w, c );
+				removeFromList( w, w->nonTouching, localIndex );
+				w->topologyDirty = true;
+			}
+			else if ( r.flags & kSimStopped )
+			{
+				c.flags &= ~1u;
+				int bodyIdA = c.edges[0].bodyId, bodyIdB = c.edges[1].bodyId;
+				c.colorIndex = -1;
+				c.localIndex = (int)w->nonTouching.size();
+				w->nonTouching.push_back( c.contactId );
+				removeContactFromGraph( w, bodyIdA, bodyIdB, colorIndex, localIndex );
+				w->topologyDirty = true;
+			}
+		}
+	}
+
+	// 3. solver (one workgroup per work item)
+	anyTopology = false;
+	for ( World* w : worlds )
+		anyTopology = anyTopology || w->topologyDirty;
+	if ( anyTopology || itemCount == 0 )
+	{
+		buildWorkItems( this );
+		collideListDirty = true;
+	}
+	int totalShapes = 0;
+	for ( World* w : worlds )
+		totalShapes += (int)w->shapes.size();
+	grow( dEnlarge, capEnlarge, std::max( totalShapes, 1 ) );
+	int block = blockSize;
+	if ( block <= 0 )
+	{
+		block = 64;
+		while ( block < maxGroupSize && block < 1024 )
+			block *= 2;
+	}
+	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, dBodyList, dSlotContact, dSlotBodies, dCons, capCons, dPriv,
+							 dEnlarge, dCounters + 1, capEnlarge, s ) );
+	HIP_CHECK( hipEventRecord( e2, s ) );
+	HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	int enlargeCount = hCounters[1];
+	B2MI_CHECK( enlargeCount <= capEnlarge );
+	if ( enlargeCount > 0 )
+	{
+		hEnlarge.resize( enlargeCount );
+		HIP_CHECK( hipMemcpyAsync( hEnlarge.data(), dEnlarge, sizeof( EnlargeRecord ) * enlargeCount, hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		std::stable_sort( hEnlarge.begin(), hEnlarge.end(),
+						  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
+		int begin = 0;
+		while ( begin < enlargeCount )
+		{
+			int end = begin;
+			while ( end < enlargeCount && hEnlarge[end].world == hEnlarge[begin].world )
+				end += 1;
+			processEnlarged( worlds[hEnlarge[begin].world], hEnlarge.data() + begin, end - begin );
+			begin = end;
+		}
+	}
+	HIP_CHECK( hipEventElapsedTime( &msCollide, e0, e1 ) );
+	HIP_CHECK( hipEventElapsedTime( &msSolve, e1, e2 ) );
+	HIP_CHECK( hipEventDestroy( e0 ) );
+	HIP_CHECK( hipEventDestroy( e1 ) );
+	HIP_CHECK( hipEventDestroy( e2 ) );
+	float msTotal = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t0 ).count();
+	msHost = msTotal - msCollide - msSolve;
+	for ( World* w : worlds )
+	{
+		w->mirrorValid = false;
+		w->contactsDownloaded = false;
+		w->locked = false;
+		w->profile.step = msTotal;
+		w->profile.collide = msCollide;
+		w->profile.solve = msSolve;
+	}
+}
+
+
+} // namespace b2mi
+
+using namespace b2mi;
+
+extern "C" {
+
+B2_API b2ShapeId b2CreateCircleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Circle* circle )
+{
+	return createShape( bodyId, def, circle, b2_circleShape );
+}
+
+B2_API b2ShapeId b2CreateCapsuleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Capsule* capsule )
+{
+	float lengthSqr = b2DistanceSquared( capsule->center1, capsule->center2 );
+	if ( lengthSqr <= kLinearSlop * kLinearSlop )
+	{
+		b2Circle circle = { b2Lerp( capsule->center1, capsule->center2, 0.5f ), capsule->radius };
+		return createShape( bodyId, def, &circle, b2_circleShape );
+	}
+	return createShape( bodyId, def, capsule, b2_capsuleShape );
+}
+
+B2_API b2ShapeId b2CreatePolygonShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Polygon* polygon )
+{
+	return createShape( bodyId, def, polygon, b2_polygonShape );
+}
+
+B2_API b2ShapeId b2CreateSegmentShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Segment* segment )
+{
+	float lengthSqr = b2DistanceSquared( segment->point1, segment->point2 );
+	if ( lengthSqr <= kLinearSlop * kLinearSlop )
+		return b2ShapeId{ 0, 0, 0 };
+	return createShape( bodyId, def, segment, b2_segmentShape );
+}
+
+B2_API bool b2Shape_IsValid( b2ShapeId id )
+{
+	World* w = b2mi_internal_worldOrNull( id.world0 );
+	if ( w == nullptr || id.index1 < 1 || id.index1 > (int)w->shapes.size() )
+		return false;
+	const Shape& s = w->shapes[id.index1 - 1];
+	return s.id == id.index1 - 1 && s.revision == id.revision;
+}
+
+static Shape* getShape( World* w, b2ShapeId id )
+{
+	int index = id.index1 - 1;
+	B2MI_CHECK( 0 <= index && index < (int)w->shapes.size() );
+	Shape* s = &w->shapes[index];
+	B2MI_CHECK( s->id == index && s->revision == id.revision );
+	return s;
+}
+
+#define B2MI_SHAPE( shapeId )                 \
+	World* w = worldAt( ( shapeId ).world0 ); \
+	Shape* shape = getShape( w, ( shapeId ) )
+
+B2_API b2ShapeType b2Shape_GetType( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->type;
+}
+
+B2_API b2BodyId b2Shape_GetBody( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	const Body& b = w->bodies[shape->bodyId];
+	return b2BodyId{ b.id + 1, shapeId.world0, b.revision };
+}
+
+B2_API float b2Shape_GetDensity( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->density;
+}
+
+// Friction and restitution are mixed into a contact when it is created
+// (src/contact.c:323-324), so changing them affects contacts created afterwards.
+B2_API void b2Shape_SetFriction( b2ShapeId shapeId, float friction )
+{
+	B2MI_SHAPE( shapeId );
+	b2mi_internal_beginEdit( w );
+	shape->friction = friction;
+	w->devShapes[shape->id].friction = friction;
+}
+
+B2_API float b2Shape_GetFriction( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->friction;
+}
+
+B2_API void b2Shape_SetRestitution( b2ShapeId shapeId, float restitution )
+{
+	B2MI_SHAPE( shapeId );
+	b2mi_internal_beginEdit( w );
+	shape->restitution = restitution;
+	w->devShapes[shape->id].restitution = restitution;
+}
+
+B2_API float b2Shape_GetRestitution( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->restitution;
+}
+
+B2_API b2Filter b2Shape_GetFilter( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->filter;
+}
+
+B2_API b2Circle b2Shape_GetCircle( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->circle;
+}
+
+B2_API b2Segment b2Shape_GetSegment( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->segment;
+}
+
+B2_API b2Capsule b2Shape_GetCapsule( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->capsule;
+}
+
+B2_API b2Polygon b2Shape_GetPolygon( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->polygon;
+}
+
+B2_API b2AABB b2Shape_GetAABB( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	b2mi_internal_downloadMirror( w );
+	return shape->aabb;
+}
+
+B2_API int b2Shape_GetContactCapacity( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return w->bodies[shape->bodyId].contactCount;
+}
+
+B2_API int b2Shape_GetContactData( b2ShapeId shapeId, b2ContactData* contactData, int capacity )
+{
+	B2MI_SHAPE( shapeId );
+	b2mi_internal_downloadContacts( w );
+	const Body& body = w->bodies[shape->bodyId];
+	int index = 0;
+	for ( int key = body.headContactKey; key != -1 && index < capacity; )
+	{
+		const Contact& c = w->contacts[key >> 1];
+		if ( ( c.shapeIdA == shape->id || c.shapeIdB == shape->id ) && ( c.flags & 1u ) )
+		{
+			const Shape& a = w->shapes[c.shapeIdA];
+			const Shape& b = w->shapes[c.shapeIdB];
+			contactData[index].shapeIdA = b2ShapeId{ a.id + 1, shapeId.world0, a.revision };
+			contactData[index].shapeIdB = b2ShapeId{ b.id + 1, shapeId.world0, b.revision };
+			memcpy( &contactData[index].manifold, &w->devContacts[c.contactId].manifold, sizeof( b2Manifold ) );
+			index += 1;
+		}
+		key = c.edges[key & 1].nextKey;
+	}
+	return index;
+}
+
+// ---- step ----
+
+// src/world.c:676-776
+B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount )
+{
+	World* w = getWorld( worldId );
+	if ( w->locked )
+		return;
+	w->profile = b2Profile{};
+	if ( timeStep == 0.0f )
+		return;
+	if ( w->group == nullptr )
+	{
+		w->group = new StepGroup();
+		w->group->worlds.push_back( w );
+		w->groupIndex = 0;
+	}
+	B2MI_CHECK( w->batchWorld == false && "batch worlds are stepped with b2mi_Batch_Step" );
+	w->group->step( timeStep, subStepCount );
+}
+
+// ---- inspection ----
+
+B2_API int b2mi_World_DumpContacts( b2WorldId worldId, b2mi_ContactRecord* records, int capacity )
+{
+	World* w = getWorld( worldId );
+	b2mi_internal_downloadContacts( w );
+	int n = 0;
+	auto emit = [&]( int id, int color, int local ) {
+		if ( n < capacity )
+		{
+			b2mi_ContactRecord& r = records[n];
+			const DevContact& d = w->devContacts[id];
+			r.contactId = id;
+			r.shapeIdA = d.shapeIdA;
+			r.shapeIdB = d.shapeIdB;
+			r.colorIndex = color;
+			r.localIndex = local;
+			r.simFlags = d.simFlags;
+			r.friction = d.friction;
+			r.restitution = d.restitution;
+			memcpy( &r.manifold, &d.manifold, sizeof( b2Manifold ) );
+		}
+		n += 1;
+	};
+	for ( int c = 0; c < kColorCount; ++c )
+		for ( int i = 0; i < (int)w->colors[c].size(); ++i )
+			emit( w->colors[c][i], c, i );
+	for ( int i = 0; i < (int)w->nonTouching.size(); ++i )
+		emit( w->nonTouching[i], -1, i );
+	return n;
+}
+
+B2_API int b2mi_World_DumpBodies( b2WorldId worldId, b2mi_BodyRecord* records, int capacity )
+{
+	World* w = getWorld( worldId );
+	b2mi_internal_downloadMirror( w );
+	int n = 0;
+	for ( int id = 0; id < (int)w->bodies.size(); ++id )
+	{
+		const Body& b = w->bodies[id];
+		if ( b.setIndex == -1 )
+			continue;
+		if ( n < capacity )
+		{
+			b2mi_BodyRecord& r = records[n];
+			memset( &r, 0, sizeof( r ) );
+			const DevBodySim& sim = b.setIndex == 0 ? w->staticSims[b.localIndex] : w->awakeSims[b.localIndex];
+			r.bodyId = id;
+			r.type = b.type;
+			r.setIndex = b.setIndex;
+			r.localIndex = b.localIndex;
+			r.transform = b2Transform{ { sim.p[0], sim.p[1] }, { sim.q[0], sim.q[1] } };
+			r.center = b2Vec2{ sim.center[0], sim.center[1] };
+			if ( b.setIndex == 2 )
+			{
+				const float* st = &w->awakeState[8 * b.localIndex];
+				r.linearVelocity = b2Vec2{ st[0], st[1] };
+				r.angularVelocity = st[2];
+			}
+		}
+		n += 1;
+	}
+	return n;
+}
+
+} // extern "C"

@@ -1,0 +1,285 @@
+// Host-side world model of the MI355X step pipeline.
+//
+// The host keeps the serial, order-defining bookkeeping of the reference (ids, shape and
+// contact lists, broadphase trees, pair set, move buffer, greedy graph colouring) and the
+// device keeps every numeric array the step reads or writes (body states and sims, shape
+// AABBs, contact manifolds). Host mirrors of the numeric arrays are downloaded lazily for
+// the getters and uploaded again after a setter changed them.
+#pragma once
+
+#include "b2mi_layout.h"
+#include "tree.hpp"
+
+#include "box2d/box2d.h"
+#include "b2mi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+namespace b2mi
+{
+
+constexpr int kColorCount = 12;
+constexpr int kOverflow = 11;
+constexpr int kMaxPlainWorlds = 128; // src/core.h:129
+constexpr int kMaxWorldSlots = 65535;
+constexpr int kCookie = 1152023; // src/core.h:159
+constexpr float kLinearSlop = 0.005f;
+constexpr float kSpeculative = 4.0f * kLinearSlop;
+constexpr float kAabbMargin = 0.1f;
+constexpr float kHuge = 100000.0f;
+
+void fatal( const char* what, const char* file, int line );
+#define B2MI_CHECK( cond )                          \
+	do                                              \
+	{                                               \
+		if ( !( cond ) )                            \
+			::b2mi::fatal( #cond, __FILE__, __LINE__ ); \
+	} while ( 0 )
+#define HIP_CHECK( expr )                                                \
+	do                                                                   \
+	{                                                                    \
+		hipError_t e_ = ( expr );                                        \
+		if ( e_ != hipSuccess )                                          \
+			::b2mi::fatal( hipGetErrorString( e_ ), __FILE__, __LINE__ ); \
+	} while ( 0 )
+
+struct IdPool
+{
+	std::vector<int> freeArray;
+	int nextIndex = 0;
+	int alloc()
+	{
+		if ( !freeArray.empty() )
+		{
+			int id = freeArray.back();
+			freeArray.pop_back();
+			return id;
+		}
+		return nextIndex++;
+	}
+	// src/id_pool.c:36-47: freed ids always go to the free array
+	void free( int id )
+	{
+		freeArray.push_back( id );
+	}
+	int count() const
+	{
+		return nextIndex - (int)freeArray.size();
+	}
+};
+
+struct BitSet
+{
+	std::vector<uint64_t> bits;
+	void setCountAndClear( uint32_t bitCount )
+	{
+		bits.assign( ( bitCount + 63 ) / 64, 0 );
+	}
+	void set( uint32_t i )
+	{
+		uint32_t b = i / 64;
+		if ( b >= bits.size() )
+			bits.resize( b + 1, 0 );
+		bits[b] |= uint64_t( 1 ) << ( i % 64 );
+	}
+	void clear( uint32_t i )
+	{
+		uint32_t b = i / 64;
+		if ( b < bits.size() )
+			bits[b] &= ~( uint64_t( 1 ) << ( i % 64 ) );
+	}
+	bool get( uint32_t i ) const
+	{
+		uint32_t b = i / 64;
+		return b < bits.size() && ( ( bits[b] >> ( i % 64 ) ) & 1 );
+	}
+};
+
+struct Body
+{
+	void* userData = nullptr;
+	int setIndex = -1; // -1 free, 0 static, 2 awake
+	int localIndex = -1;
+	int headContactKey = -1;
+	int contactCount = 0;
+	int headShapeId = -1;
+	int shapeCount = 0;
+	int id = -1;
+	b2BodyType type = b2_staticBody;
+	uint16_t revision = 0;
+	bool fixedRotation = false;
+	bool automaticMass = true;
+};
+
+struct Shape
+{
+	int id = -1;
+	int bodyId = -1;
+	int prevShapeId = -1;
+	int nextShapeId = -1;
+	b2ShapeType type = b2_circleShape;
+	float density = 0, friction = 0, restitution = 0;
+	b2AABB aabb{}, fatAABB{};
+	int proxyKey = -1;
+	b2Filter filter{};
+	void* userData = nullptr;
+	union
+	{
+		b2Capsule capsule;
+		b2Circle circle;
+		b2Polygon polygon;
+		b2Segment segment;
+	};
+	uint16_t revision = 0;
+	bool enableHitEvents = false;
+	bool enableContactEvents = true;
+	Shape() : polygon{} {}
+};
+
+struct ContactEdge
+{
+	int bodyId, prevKey, nextKey;
+};
+
+struct Contact
+{
+	int setIndex = -1; // -1 free, 2 awake
+	int colorIndex = -1;
+	int localIndex = -1;
+	ContactEdge edges[2];
+	int shapeIdA = -1, shapeIdB = -1;
+	int contactId = -1;
+	uint32_t flags = 0; // bit0 touching
+};
+
+struct StepGroup;
+
+struct World
+{
+	int slot = -1; // index in the world table (worldId.index1 - 1)
+	uint16_t revision = 0;
+	bool inUse = false;
+	bool locked = false;
+	bool batchWorld = false;
+
+	std::vector<Body> bodies;
+	IdPool bodyIdPool;
+	std::vector<int> staticSimBody; // static set: sim index -> body id
+	std::vector<int> awakeSimBody;	// awake set: sim index -> body id
+
+	// host mirrors of the numeric arrays (valid when mirrorValid)
+	std::vector<DevBodySim> staticSims;
+	std::vector<DevBodySim> awakeSims;
+	std::vector<float> awakeState; // 8 floats per awake body
+	std::vector<DevShape> devShapes;
+	std::vector<DevContact> devContacts;
+	bool mirrorValid = true;
+	bool bodiesDirty = true; // host mirror of bodies/shapes must be uploaded
+	bool contactsDownloaded = false;
+
+	std::vector<Shape> shapes;
+	IdPool shapeIdPool;
+
+	std::vector<Contact> contacts;
+	IdPool contactIdPool;
+	std::vector<int> nonTouching; // awake non-touching contact ids (array order of the reference)
+	std::vector<int> colors[kColorCount];
+	BitSet colorBodySets[kColorCount];
+
+	DynamicTree trees[3];
+	BitSet moveSet;
+	std::vector<int> moveArray;
+	std::unordered_set<uint64_t> pairSet;
+
+	b2Vec2 gravity{ 0.0f, -10.0f };
+	float hitEventThreshold = 1, restitutionThreshold = 1, contactPushoutVelocity = 3;
+	float contactHertz = 30, contactDampingRatio = 10, jointHertz = 60, jointDampingRatio = 2;
+	bool enableSleep = true, enableWarmStarting = true, enableContinuous = true;
+
+	b2Profile profile{};
+	bool topologyDirty = true; // colour lists / bodies changed since the work items were built
+
+	// device arrays
+	float* dState = nullptr;
+	DevBodySim* dSims = nullptr;
+	DevBodySim* dStatics = nullptr;
+	DevShape* dShapes = nullptr;
+	int* dBodyShapes = nullptr;
+	DevContact* dContacts = nullptr;
+	int capAwake = 0, capStatic = 0, capShapes = 0, capContacts = 0;
+
+	StepGroup* group = nullptr;
+	int groupIndex = -1;
+
+	// pending new contacts (device init) of the current step
+	std::vector<ContactInit> pendingInits;
+
+	~World();
+};
+
+// a set of worlds advanced together by one narrowphase launch and one solver launch
+struct StepGroup
+{
+	std::vector<World*> worlds;
+	int blockSize = 0;		  // 0 = automatic
+	int targetSlots = 0;	  // work-item packing target (constraint slots), 0 = automatic
+	bool structureDirty = true;
+
+	// launch buffers (device)
+	DevWorld* dWorlds = nullptr;
+	int capWorlds = 0;
+	int2* dCollideList = nullptr;
+	int capCollide = 0;
+	int collideCount = 0;
+	bool collideListDirty = true;
+	ChangeRecord* dChanges = nullptr;
+	int capChanges = 0;
+	int* dCounters = nullptr; // [0] changes, [1] enlarge
+	int* hCounters = nullptr; // pinned
+	EnlargeRecord* dEnlarge = nullptr;
+	int capEnlarge = 0;
+	ContactInit* dInits = nullptr;
+	int capInits = 0;
+	WorkItem* dItems = nullptr;
+	int capItems = 0;
+	int itemCount = 0;
+	int* dBodyList = nullptr;
+	int capBodyList = 0;
+	int* dSlotContact = nullptr;
+	int2* dSlotBodies = nullptr;
+	int capSlots = 0;
+	float* dCons = nullptr;
+	int capCons = 0;
+	float4* dPriv = nullptr;
+	int capPriv = 0;
+	int slotCount = 0, bodyListCount = 0;
+	int maxGroupSize = 0;
+
+	// host staging
+	std::vector<DevWorld> hWorlds;
+	std::vector<WorkItem> hItems;
+	std::vector<int> hBodyList;
+	std::vector<int> hSlotContact;
+	std::vector<int2> hSlotBodies;
+	std::vector<int2> hCollide;
+	std::vector<ContactInit> hInits;
+	std::vector<ChangeRecord> hChanges;
+	std::vector<EnlargeRecord> hEnlarge;
+
+	// timing of the last step (ms)
+	float msCollide = 0, msSolve = 0, msHost = 0;
+
+	~StepGroup();
+	void step( float timeStep, int subStepCount );
+};
+
+hipStream_t stream();
+World* getWorld( b2WorldId id );
+World* worldAt( int slot );
+
+} // namespace b2mi

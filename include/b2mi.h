@@ -47,3 +47,34 @@ typedef struct b2mi_BodyRecord
 // Both return the number of records available; at most capacity are written.
 B2_API int b2mi_World_DumpContacts( b2WorldId worldId, b2mi_ContactRecord* records, int capacity );
 B2_API int b2mi_World_DumpBodies( b2WorldId worldId, b2mi_BodyRecord* records, int capacity );
+
+// ---- batching ----------------------------------------------------------------------------
+
+// A batch advances many independent worlds with one narrowphase launch and one solver
+// launch per step. Returns a batch handle (> 0), or 0 when none is free.
+B2_API int b2mi_CreateBatch( void );
+B2_API void b2mi_DestroyBatch( int batch );
+
+// Creates a world that belongs to the batch. Bodies and shapes are added with the regular
+// API (b2CreateBody, b2CreatePolygonShape, ...). Batch worlds do not count against the
+// 128-world limit of b2CreateWorld.
+B2_API b2WorldId b2mi_Batch_CreateWorld( int batch, const b2WorldDef* def );
+B2_API int b2mi_Batch_GetWorldCount( int batch );
+
+// Steps every world of the batch (the batch counterpart of b2World_Step).
+B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount );
+
+// Solver launch shape: threads per workgroup (0 = automatic) and the number of contact
+// constraints packed into one work item (0 = 256).
+B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots );
+B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots );
+B2_API int b2mi_Batch_GetWorkItemCount( int batch );
+
+// Device time of the last step's narrowphase and solver launches and the remaining host
+// time, in milliseconds.
+B2_API void b2mi_Batch_GetTiming( int batch, float* msCollide, float* msSolve, float* msHost );
+B2_API void b2mi_World_GetTiming( b2WorldId worldId, float* msCollide, float* msSolve, float* msHost );
+
+// ---- device ----------------------------------------------------------------------------
+B2_API int b2mi_DeviceCount( void );
+B2_API const char* b2mi_GetLastError( void );

@@ -13,7 +13,13 @@
 	#define B2_EXPORT_ATTR
 #endif
 
-#ifdef __cplusplus
+#if defined( __HIPCC__ )
+	// the inline math is also compiled for the GPU: host and device share one definition
+	#define B2_API extern "C" B2_EXPORT_ATTR
+	#define B2_INLINE __host__ __device__ inline
+	#define B2_LITERAL( T ) T
+	#define B2_ZERO_INIT {}
+#elif defined( __cplusplus )
 	#define B2_API extern "C" B2_EXPORT_ATTR
 	#define B2_INLINE inline
 	#define B2_LITERAL( T ) T

@@ -1,0 +1,98 @@
+"""Shared test utilities: the three implementations of the public C API and comparisons.
+
+  product()   box2c_amd/libb2mi.so (HIP path; needs a GPU to step)
+  oracle()    oracle/liboracle.so (CPU restatement; the checker)
+  oracle64()  oracle/liboracle_f64.so (same, solver in double)
+  reference() oracle/_ref/libbox2d_ref.so (the reference built from its own sources), or None
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from box2c_amd.capi import B2Api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_cache: dict[str, B2Api | None] = {}
+
+
+def _get(name, lib, scenes):
+    if name not in _cache:
+        _cache[name] = B2Api(lib, scenes) if os.path.exists(lib) and os.path.exists(scenes) else None
+    return _cache[name]
+
+
+def oracle() -> B2Api:
+    api = _get("oracle", os.path.join(ORACLE_DIR, "liboracle.so"), os.path.join(ORACLE_DIR, "liboracle_scenes.so"))
+    assert api is not None, "oracle not built (make -C oracle)"
+    return api
+
+
+def oracle64() -> B2Api:
+    api = _get("oracle64", os.path.join(ORACLE_DIR, "liboracle_f64.so"), os.path.join(ORACLE_DIR, "liboracle_scenes.so"))
+    assert api is not None, "oracle not built (make -C oracle)"
+    return api
+
+
+def reference() -> B2Api | None:
+    return _get("ref", os.path.join(ORACLE_DIR, "_ref", "libbox2d_ref.so"),
+                os.path.join(ORACLE_DIR, "_ref", "libref_scenes.so"))
+
+
+def product() -> B2Api:
+    import box2c_amd
+    return box2c_amd.load()
+
+
+def masked(contacts: np.ndarray) -> np.ndarray:
+    """Contact records with the struct padding byte cleared (it is uninitialized memory in
+    the reference's manifold points)."""
+    c = contacts.copy()
+    c["points"]["_pad"] = 0
+    return c
+
+
+def first_contact_mismatch(a: np.ndarray, b: np.ndarray):
+    """None when both contact dumps are bit-identical, else a description of the first
+    difference."""
+    a, b = masked(a), masked(b)
+    if len(a) != len(b):
+        return f"contact count {len(a)} != {len(b)}"
+    if np.array_equal(a.view(np.uint8), b.view(np.uint8)):
+        return None
+    for i in range(len(a)):
+        if a[i].tobytes() != b[i].tobytes():
+            return f"record {i}:\n  {a[i]}\n  {b[i]}"
+    return "differs"
+
+
+def first_body_mismatch(a: np.ndarray, b: np.ndarray):
+    if len(a) != len(b):
+        return f"body count {len(a)} != {len(b)}"
+    if np.array_equal(a.view(np.uint8), b.view(np.uint8)):
+        return None
+    for i in range(len(a)):
+        if a[i].tobytes() != b[i].tobytes():
+            return f"body {i}:\n  {a[i]}\n  {b[i]}"
+    return "differs"
+
+
+# scene name, scene kwargs, world-def overrides, steps
+PARITY_SCENES = [
+    ("hello", {}, {}, 120),
+    ("large_pyramid", {"base": 20}, {}, 60),
+    ("pyramid_grid", {"rows": 2, "cols": 3, "base": 10, "seed": 3}, {}, 40),
+    ("tumbler", {"count": 400, "seed": 5}, {}, 120),
+    ("shape_zoo", {"copies": 8, "seed": 2}, {"enableContinous": False}, 100),
+    ("drop_boxes", {"count": 16, "seed": 7, "index": 3}, {"enableContinous": False}, 120),
+    ("batch_pyramid", {"seed": 1, "index": 5}, {}, 60),
+]
+
+
+def make_world(api: B2Api, name: str, kw: dict, overrides: dict):
+    d = api.default_world_def(enableSleep=False)
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return api.scene(name, d, **kw)

@@ -1,0 +1,92 @@
+"""Parity of the HIP step pipeline with the CPU oracle.
+
+Bar: bit-identical. After every step, every awake contact record (contact id, shape pair,
+graph colour, index in its colour array, flags, friction, restitution, the manifold with
+feature ids, anchors, separations and accumulated impulses) and every body record (pose,
+center, velocity) must match the oracle byte for byte; struct padding is excluded.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,kw,overrides,steps", helpers.PARITY_SCENES,
+                         ids=[s[0] for s in helpers.PARITY_SCENES])
+def test_step_parity_with_oracle(name, kw, overrides, steps):
+    gpu, cpu = helpers.product(), helpers.oracle()
+    wg, _ = helpers.make_world(gpu, name, kw, overrides)
+    wc, _ = helpers.make_world(cpu, name, kw, overrides)
+    try:
+        for s in range(steps):
+            gpu.step(wg)
+            cpu.step(wc)
+            m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+            assert m is None, f"{name} step {s}: contacts differ: {m}"
+            m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+            assert m is None, f"{name} step {s}: bodies differ: {m}"
+        cg, cc = gpu.counters(wg), cpu.counters(wc)
+        assert list(cg.colorCounts) == list(cc.colorCounts)
+        assert cg.contactCount == cc.contactCount
+    finally:
+        gpu.destroy(wg)
+        cpu.destroy(wc)
+
+
+def test_batch_parity_with_oracle():
+    """16 uneven pyramid worlds in one batch launch == the same worlds stepped one by one."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    batch = gpu.create_batch()
+    gpu.use_batch(batch)
+    worlds_g, worlds_c = [], []
+    try:
+        for i in range(16):
+            wg, _ = gpu.scene("batch_pyramid", seed=11, index=i)
+            wc, _ = cpu.scene("batch_pyramid", seed=11, index=i)
+            worlds_g.append(wg)
+            worlds_c.append(wc)
+    finally:
+        gpu.use_batch(None)
+    try:
+        for s in range(40):
+            gpu.batch_step(batch)
+            for wc in worlds_c:
+                cpu.step(wc)
+            if s % 8 == 7:
+                for i, (wg, wc) in enumerate(zip(worlds_g, worlds_c)):
+                    m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+                    assert m is None, f"world {i} step {s}: {m}"
+                    m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+                    assert m is None, f"world {i} step {s}: {m}"
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
+
+
+def test_gpu_run_to_run_bit_reproducible():
+    gpu = helpers.product()
+    dumps = []
+    for _ in range(2):
+        w, _ = helpers.make_world(gpu, "tumbler", {"count": 300, "seed": 4}, {})
+        gpu.step(w, n=60)
+        dumps.append((helpers.masked(gpu.dump_contacts(w)), gpu.dump_bodies(w)))
+        gpu.destroy(w)
+    assert np.array_equal(dumps[0][0].view(np.uint8), dumps[1][0].view(np.uint8))
+    assert np.array_equal(dumps[0][1].view(np.uint8), dumps[1][1].view(np.uint8))
+
+
+def test_hello_world_rests_on_ground():
+    """The reference's HelloWorld test (test/test_world.c:17-97): a box dropped on the
+    ground for 90 steps ends within 0.01 of (0, 1) with angle below 0.01."""
+    gpu = helpers.product()
+    d = gpu.default_world_def()
+    w, ids = gpu.scene("hello", d)
+    gpu.step(w, n=90)
+    p = gpu.lib.b2Body_GetPosition(ids[0])
+    q = gpu.lib.b2Body_GetRotation(ids[0])
+    gpu.destroy(w)
+    assert abs(p.x) < 0.01 and abs(p.y - 1.0) < 0.01
+    assert abs(np.arctan2(q.s, q.c)) < 0.01
