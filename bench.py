@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP step pipeline in bodies x substeps / s on one MI355X.
+
+Workload (default): one world holding a 40 x 40 grid of base-10 box pyramids (88,000
+dynamic boxes, half-extent 0.5, friction 0.6), dt = 1/60 s, 4 substeps, synthetic input
+from box2c_amd/csrc/scenes.c. W warmup steps (the first creates every contact) are excluded;
+K steps are timed. Every step of the timed region runs the full pipeline: broadphase pair
+update, narrowphase, colouring updates, the solver (prepare, 4 x (integrate velocities, warm
+start, solve, integrate positions, relax), restitution, store impulses) and finalize.
+
+Printed JSON line: metric/value/unit per the benchmark contract, plus
+  roofline      the solver kernel (k_solve, the dominant launch): algorithmic bytes per
+                launch / its mean duration from HIP events on its stream;
+  cpu_baseline  the reference (oracle/_ref, built from its own sources) on the host cores
+                with a pthread task pool, stepping the same scene for a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bodies×substeps/s (1 GPU): plain vs batched vs fwd+adjoint at ≤1e-5 step error"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_bodies: int, n_contacts: int, substeps: int) -> dict:
+    """Bytes one step must move, counting each field once per pass (per-body and per-contact
+    figures derived from the reference's data layout: integrate velocities 60 B + integrate
+    positions 44 B per body per substep; warm start 128 B + solve 232 B + relax 232 B per
+    contact per substep; prepare 264 + restitution 232 + store 56 B per contact per step;
+    finalize 172 B per body per step; narrowphase ~300 B per contact per step)."""
+    solve = substeps * (104 * n_bodies + 592 * n_contacts) + (264 + 232 + 56) * n_contacts + 172 * n_bodies
+    collide = 300 * n_contacts
+    return {"solve": solve, "collide": collide, "step": solve + collide}
+
+
+def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
+    d = api.default_world_def(enableSleep=False)
+    w, ids = api.scene(scene, d, rows=rows, cols=cols, base=10, seed=7)
+    n_bodies = len(ids)
+    api.step(w, n=warmup)
+    counters = api.counters(w)
+    t_solve = t_collide = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        api.lib.b2World_Step(w, 1.0 / 60.0, 4)
+        c, s, _h = api.world_timing(w)
+        t_collide += c
+        t_solve += s
+    elapsed = time.perf_counter() - t0
+    counters = api.counters(w)
+    n_contacts = sum(counters.colorCounts)
+    api.destroy(w)
+    return {
+        "bodies": n_bodies, "contacts": n_contacts, "elapsed": elapsed,
+        "ms_solve": t_solve / steps, "ms_collide": t_collide / steps,
+        "colors": list(counters.colorCounts),
+    }
+
+
+def cpu_baseline(rows: int, cols: int, threads: int, sample_steps: int, warmup: int = 20):
+    """The reference stepping the same scene with `threads` workers, timed over a bounded
+    number of steps after the same warmup as the GPU leg (bench.py's CPU leg; the oracle directory holds the
+    reference build and is used here only as the baseline)."""
+    from box2c_amd.capi import B2Api
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
+    if not (os.path.exists(lib) and os.path.exists(scenes)):
+        return None
+    ref = B2Api(lib, scenes)
+    ref.lib.ref_pool_start(threads)
+    d = ref.default_world_def(enableSleep=False)
+    ref.lib.ref_pool_apply(__import__("ctypes").byref(d))
+    w, ids = ref.scene("pyramid_grid", d, rows=rows, cols=cols, base=10, seed=7)
+    ref.step(w, n=warmup)
+    t0 = time.perf_counter()
+    ref.step(w, n=sample_steps)
+    elapsed = time.perf_counter() - t0
+    ref.destroy(w)
+    ref.lib.ref_pool_stop()
+    return {
+        "value": len(ids) * 4 * sample_steps / elapsed, "unit": "bodies×substeps/s", "cores": threads,
+        "kind": "reference",
+        "sample": f"pyramid_grid {rows}x{cols} ({len(ids)} boxes), {sample_steps} steps after {warmup} warmup, "
+                  f"{threads} worker threads, {elapsed:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--rows", type=int, default=40)
+    ap.add_argument("--cols", type=int, default=40)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: worlds are independent and the multi-GPU "
+                                   "layout (worlds sharded across GPUs) is design only; run with --gpus 1"}))
+        sys.exit(2)
+
+    import box2c_amd
+    api = box2c_amd.load()
+    if api.lib.b2mi_DeviceCount() < 1:
+        raise SystemExit("no HIP device visible")
+
+    r = run_gpu(api, "pyramid_grid", args.steps, args.warmup, args.rows, args.cols)
+    substeps = 4
+    value = r["bodies"] * substeps * args.steps / r["elapsed"]
+    ab = algorithmic_bytes(r["bodies"], r["contacts"], substeps)
+    achieved = ab["solve"] / (r["ms_solve"] * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("k_solve_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bodies×substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pyramid_grid_{args.rows}x{args.cols}_base10_plain_f32",
+            "bodies": r["bodies"], "touching_contacts": r["contacts"], "substeps": substeps,
+            "dt": 1.0 / 60.0, "graph_colors": r["colors"],
+            "ms_solve_kernel": r["ms_solve"], "ms_collide_kernel": r["ms_collide"],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "k_solve", "algorithmic_bytes_per_launch": ab["solve"],
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.rows, args.cols, args.cpu_threads, args.cpu_steps, args.warmup)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
