@@ -92,6 +92,20 @@ def cpu_baseline(rows: int, cols: int, threads: int, sample_steps: int, warmup: 
     }
 
 
+def calibrate_copy(mib: int = 1024, reps: int = 5):
+    """A plain device copy of known size (torch elementwise kernel, `mib` MiB read and `mib`
+    MiB written per launch), run under the same HBM counters as the solver so that the
+    counters' bytes per dispatch can be scaled to true bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda:0", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +116,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain copy kernel of known size (for HBM counter scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: worlds are independent and the multi-GPU "

@@ -54,6 +54,53 @@ class WorldDef(Structure):
                 ("userTaskContext", c_void_p), ("internalValue", c_int32)]
 
 
+class BodyDef(Structure):
+    _fields_ = [("type", c_int32), ("position", Vec2), ("rotation", Rot), ("linearVelocity", Vec2),
+                ("angularVelocity", c_float), ("linearDamping", c_float), ("angularDamping", c_float),
+                ("gravityScale", c_float), ("sleepThreshold", c_float), ("userData", c_void_p),
+                ("enableSleep", c_bool), ("isAwake", c_bool), ("fixedRotation", c_bool),
+                ("isBullet", c_bool), ("isEnabled", c_bool), ("automaticMass", c_bool),
+                ("internalValue", c_int32)]
+
+
+class Filter(Structure):
+    _fields_ = [("categoryBits", c_uint32), ("maskBits", c_uint32), ("groupIndex", c_int32)]
+
+
+class ShapeDef(Structure):
+    _fields_ = [("userData", c_void_p), ("friction", c_float), ("restitution", c_float),
+                ("density", c_float), ("filter", Filter), ("customColor", c_uint32),
+                ("isSensor", c_bool), ("enableSensorEvents", c_bool), ("enableContactEvents", c_bool),
+                ("enableHitEvents", c_bool), ("enablePreSolveEvents", c_bool),
+                ("forceContactCreation", c_bool), ("internalValue", c_int32)]
+
+
+class Polygon(Structure):
+    _fields_ = [("vertices", Vec2 * 8), ("normals", Vec2 * 8), ("centroid", Vec2), ("radius", c_float),
+                ("count", c_int32)]
+
+
+class Circle(Structure):
+    _fields_ = [("center", Vec2), ("radius", c_float)]
+
+
+class Capsule(Structure):
+    _fields_ = [("center1", Vec2), ("center2", Vec2), ("radius", c_float)]
+
+
+class Segment(Structure):
+    _fields_ = [("point1", Vec2), ("point2", Vec2)]
+
+
+class MassData(Structure):
+    _fields_ = [("mass", c_float), ("center", Vec2), ("rotationalInertia", c_float)]
+
+
+class SegmentDistanceResult(Structure):
+    _fields_ = [("closest1", Vec2), ("closest2", Vec2), ("fraction1", c_float), ("fraction2", c_float),
+                ("distanceSquared", c_float)]
+
+
 class ManifoldPoint(Structure):
     _fields_ = [("point", Vec2), ("anchorA", Vec2), ("anchorB", Vec2), ("separation", c_float),
                 ("normalImpulse", c_float), ("tangentImpulse", c_float),
@@ -92,6 +139,9 @@ class BodyRecord(Structure):
 
 
 assert ctypes.sizeof(WorldDef) == 80
+assert ctypes.sizeof(BodyDef) == 72
+assert ctypes.sizeof(ShapeDef) == 48
+assert ctypes.sizeof(Polygon) == 144
 assert ctypes.sizeof(Manifold) == 108
 assert ctypes.sizeof(BodyRecord) == 56
 
@@ -111,6 +161,19 @@ BODY_DTYPE = np.dtype([("bodyId", "<i4"), ("type", "<i4"), ("setIndex", "<i4"),
                        ("center", "<f4", 2), ("v", "<f4", 2), ("w", "<f4"), ("pad", "<f4")])
 assert CONTACT_DTYPE.itemsize == ctypes.sizeof(ContactRecord)
 assert BODY_DTYPE.itemsize == ctypes.sizeof(BodyRecord)
+
+# the public manifold functions (include/box2d/collision.h): name, shape A type, shape B type
+COLLIDE_FUNCTIONS = [
+    ("b2CollideCircles", Circle, Circle),
+    ("b2CollideCapsuleAndCircle", Capsule, Circle),
+    ("b2CollideSegmentAndCircle", Segment, Circle),
+    ("b2CollidePolygonAndCircle", Polygon, Circle),
+    ("b2CollideCapsules", Capsule, Capsule),
+    ("b2CollideSegmentAndCapsule", Segment, Capsule),
+    ("b2CollidePolygonAndCapsule", Polygon, Capsule),
+    ("b2CollidePolygons", Polygon, Polygon),
+    ("b2CollideSegmentAndPolygon", Segment, Polygon),
+]
 
 WORLD_FACTORY = ctypes.CFUNCTYPE(c_uint32, POINTER(WorldDef), c_void_p)
 
@@ -156,6 +219,39 @@ class B2Api:
         L.b2Body_GetInertiaTensor.restype = c_float
         L.b2Body_GetContactCapacity.argtypes = [BodyId]
         L.b2Body_GetContactCapacity.restype = c_int
+        L.b2DefaultBodyDef.restype = BodyDef
+        L.b2DefaultShapeDef.restype = ShapeDef
+        L.b2CreateBody.argtypes = [WorldId, POINTER(BodyDef)]
+        L.b2CreateBody.restype = BodyId
+        L.b2DestroyBody.argtypes = [BodyId]
+        L.b2Body_IsValid.argtypes = [BodyId]
+        L.b2Body_IsValid.restype = c_bool
+        for name, shape in (("Polygon", Polygon), ("Circle", Circle), ("Capsule", Capsule), ("Segment", Segment)):
+            f = getattr(L, f"b2Create{name}Shape")
+            f.argtypes = [BodyId, POINTER(ShapeDef), POINTER(shape)]
+            f.restype = ShapeId
+        L.b2MakeBox.argtypes = [c_float, c_float]
+        L.b2MakeBox.restype = Polygon
+        L.b2MakeSquare.argtypes = [c_float]
+        L.b2MakeSquare.restype = Polygon
+        L.b2MakeRoundedBox.argtypes = [c_float, c_float, c_float]
+        L.b2MakeRoundedBox.restype = Polygon
+        L.b2MakeOffsetBox.argtypes = [c_float, c_float, Vec2, c_float]
+        L.b2MakeOffsetBox.restype = Polygon
+        for name, shape in (("Polygon", Polygon), ("Circle", Circle), ("Capsule", Capsule)):
+            f = getattr(L, f"b2Compute{name}Mass")
+            f.argtypes = [POINTER(shape), c_float]
+            f.restype = MassData
+        for name, shape in (("Polygon", Polygon), ("Circle", Circle), ("Capsule", Capsule), ("Segment", Segment)):
+            f = getattr(L, f"b2Compute{name}AABB")
+            f.argtypes = [POINTER(shape), Transform]
+            f.restype = AABB
+        L.b2SegmentDistance.argtypes = [Vec2, Vec2, Vec2, Vec2]
+        L.b2SegmentDistance.restype = SegmentDistanceResult
+        for name, sa, sb in COLLIDE_FUNCTIONS:
+            f = getattr(L, name)
+            f.argtypes = [POINTER(sa), Transform, POINTER(sb), Transform]
+            f.restype = Manifold
         L.b2mi_World_DumpContacts.argtypes = [WorldId, c_void_p, c_int]
         L.b2mi_World_DumpContacts.restype = c_int
         L.b2mi_World_DumpBodies.argtypes = [WorldId, c_void_p, c_int]
@@ -173,6 +269,10 @@ class B2Api:
                                            c_void_p, c_int, POINTER(c_float)]
             S.scene_shape_zoo.argtypes = [POINTER(WorldDef), c_int, c_uint64, POINTER(WorldId), c_void_p, c_int]
             S.scene_hello.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
+            S.scene_tilted_stacks.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
+            S.scene_single.argtypes = [POINTER(WorldDef), c_int, c_float, c_float, c_float, c_float, c_float,
+                                       c_float, c_float, c_float, c_float, c_float, c_int,
+                                       POINTER(WorldId), c_void_p, c_int]
             S.scene_step_n.argtypes = [WorldId, c_float, c_int, c_int]
 
     # ---- worlds ----
@@ -234,6 +334,17 @@ class B2Api:
         elif name == "hello":
             ids = self._bodies(1)
             n = S.scene_hello(ctypes.byref(d), ctypes.byref(w), ids, len(ids))
+        elif name == "tilted_stacks":
+            ids = self._bodies(100)
+            n = S.scene_tilted_stacks(ctypes.byref(d), ctypes.byref(w), ids, len(ids))
+        elif name == "single":
+            ids = self._bodies(1)
+            shape = {"circle": 0, "box": 3}[kw.get("shape", "box")]
+            n = S.scene_single(ctypes.byref(d), shape, kw.get("size", 0.5), kw.get("x", 0.0), kw.get("y", 0.5),
+                               kw.get("vx", 0.0), kw.get("vy", 0.0), kw.get("w", 0.0), kw.get("friction", 0.6),
+                               kw.get("restitution", 0.0), kw.get("linearDamping", 0.0),
+                               kw.get("angularDamping", 0.0), int(kw.get("ground", True)),
+                               ctypes.byref(w), ids, len(ids))
         else:
             raise ValueError(f"unknown scene {name}")
         return w, list(ids)[:n]

@@ -457,6 +457,95 @@ SCENE_API int scene_hello( const b2WorldDef* def, b2WorldId* outWorld, b2BodyId*
 	return 1;
 }
 
+// The reference's determinism test scene (test/test_determinism.c:90-150): 10 columns of
+// 10 rounded boxes (half-extent 0.45, radius 0.05, friction 0.3), each column tilted by a
+// 0.2 m offset per row, on a 1000 x 1 ground box. The test steps it 100 times with 3
+// substeps and sleep disabled.
+SCENE_API int scene_tilted_stacks( const b2WorldDef* def, b2WorldId* outWorld, b2BodyId* outBodies, int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bd );
+		b2Polygon box = b2MakeBox( 1000.0f, 1.0f );
+		b2ShapeDef sd = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &sd, &box );
+	}
+	b2Polygon box = b2MakeRoundedBox( 0.45f, 0.45f, 0.05f );
+	b2ShapeDef sd = b2DefaultShapeDef();
+	sd.density = 1.0f;
+	sd.friction = 0.3f;
+	const int columns = 10, rows = 10;
+	float offset = 0.2f, dx = 5.0f;
+	float xroot = -0.5f * dx * ( columns - 1.0f );
+	for ( int j = 0; j < columns; ++j )
+	{
+		float x = xroot + j * dx;
+		for ( int i = 0; i < rows; ++i )
+		{
+			b2BodyDef bd = b2DefaultBodyDef();
+			bd.type = b2_dynamicBody;
+			bd.position = ( b2Vec2 ){ x + offset * i, 0.5f + 1.0f * i };
+			b2BodyId bodyId = b2CreateBody( worldId, &bd );
+			b2CreatePolygonShape( bodyId, &sd, &box );
+			record( outBodies, capacity, j * rows + i, bodyId );
+		}
+	}
+	return columns * rows;
+}
+
+// One dynamic body over (or on) a static ground box whose top face is the line y = 0
+// (half-extents 50 x 1, centred at (0, -1)); used for closed-form checks of gravity,
+// damping, friction and restitution.
+//   shape   0 = circle of radius `size`, 3 = square box of half-extent `size`
+//   ground  0 = no ground body
+// The ground shape gets the same friction and restitution as the body, so the mixed
+// values (sqrt(fA fB), max(rA, rB)) equal the body's.
+SCENE_API int scene_single( const b2WorldDef* def, int shape, float size, float x, float y, float vx, float vy,
+							float w, float friction, float restitution, float linearDamping, float angularDamping,
+							int ground, b2WorldId* outWorld, b2BodyId* outBodies, int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	if ( ground )
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bd );
+		b2Polygon box = b2MakeBox( 50.0f, 1.0f );
+		b2ShapeDef sd = b2DefaultShapeDef();
+		sd.friction = friction;
+		sd.restitution = restitution;
+		b2CreatePolygonShape( groundId, &sd, &box );
+	}
+	b2BodyDef bd = b2DefaultBodyDef();
+	bd.type = b2_dynamicBody;
+	bd.position = ( b2Vec2 ){ x, y };
+	bd.linearVelocity = ( b2Vec2 ){ vx, vy };
+	bd.angularVelocity = w;
+	bd.linearDamping = linearDamping;
+	bd.angularDamping = angularDamping;
+	b2BodyId bodyId = b2CreateBody( worldId, &bd );
+	b2ShapeDef sd = b2DefaultShapeDef();
+	sd.density = 1.0f;
+	sd.friction = friction;
+	sd.restitution = restitution;
+	if ( shape == 0 )
+	{
+		b2Circle c = { { 0.0f, 0.0f }, size };
+		b2CreateCircleShape( bodyId, &sd, &c );
+	}
+	else
+	{
+		b2Polygon box = b2MakeBox( size, size );
+		b2CreatePolygonShape( bodyId, &sd, &box );
+	}
+	record( outBodies, capacity, 0, bodyId );
+	return 1;
+}
+
 // Steps a world n times (saves one ctypes crossing per step in tests and benches).
 SCENE_API void scene_step_n( b2WorldId worldId, float timeStep, int subStepCount, int n )
 {

@@ -31,7 +31,8 @@ def oracle() -> B2Api:
 
 
 def oracle64() -> B2Api:
-    api = _get("oracle64", os.path.join(ORACLE_DIR, "liboracle_f64.so"), os.path.join(ORACLE_DIR, "liboracle_scenes.so"))
+    api = _get("oracle64", os.path.join(ORACLE_DIR, "liboracle_f64.so"),
+               os.path.join(ORACLE_DIR, "liboracle_f64_scenes.so"))
     assert api is not None, "oracle not built (make -C oracle)"
     return api
 
@@ -79,16 +80,32 @@ def first_body_mismatch(a: np.ndarray, b: np.ndarray):
     return "differs"
 
 
-# scene name, scene kwargs, world-def overrides, steps
+# scene name, scene kwargs, world-def overrides, steps, substeps
 PARITY_SCENES = [
-    ("hello", {}, {}, 120),
-    ("large_pyramid", {"base": 20}, {}, 60),
-    ("pyramid_grid", {"rows": 2, "cols": 3, "base": 10, "seed": 3}, {}, 40),
-    ("tumbler", {"count": 400, "seed": 5}, {}, 120),
-    ("shape_zoo", {"copies": 8, "seed": 2}, {"enableContinous": False}, 100),
-    ("drop_boxes", {"count": 16, "seed": 7, "index": 3}, {"enableContinous": False}, 120),
-    ("batch_pyramid", {"seed": 1, "index": 5}, {}, 60),
+    ("hello", {}, {}, 120, 4),
+    ("large_pyramid", {"base": 20}, {}, 60, 4),
+    ("pyramid_grid", {"rows": 2, "cols": 3, "base": 10, "seed": 3}, {}, 40, 4),
+    ("tumbler", {"count": 400, "seed": 5}, {}, 120, 4),
+    ("shape_zoo", {"copies": 8, "seed": 2}, {"enableContinous": False}, 100, 4),
+    ("drop_boxes", {"count": 16, "seed": 7, "index": 3}, {"enableContinous": False}, 120, 4),
+    ("batch_pyramid", {"seed": 1, "index": 5}, {}, 60, 4),
+    ("tilted_stacks", {}, {}, 100, 3),
 ]
+
+
+def manifold_bytes(m) -> np.ndarray:
+    """Raw bytes of a b2Manifold with the two manifold points' padding bytes cleared."""
+    import ctypes
+    b = np.frombuffer(ctypes.string_at(ctypes.addressof(m), ctypes.sizeof(m)), dtype=np.uint8).copy()
+    return mask_manifold_bytes(b)
+
+
+def mask_manifold_bytes(b: np.ndarray) -> np.ndarray:
+    """Clears the padding byte of each manifold point (offset 47 of each 48-byte point)."""
+    b = b.copy()
+    b[..., 47] = 0
+    b[..., 48 + 47] = 0
+    return b
 
 
 def make_world(api: B2Api, name: str, kw: dict, overrides: dict):

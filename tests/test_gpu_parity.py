@@ -13,16 +13,16 @@ from tests import helpers
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name,kw,overrides,steps", helpers.PARITY_SCENES,
+@pytest.mark.parametrize("name,kw,overrides,steps,substeps", helpers.PARITY_SCENES,
                          ids=[s[0] for s in helpers.PARITY_SCENES])
-def test_step_parity_with_oracle(name, kw, overrides, steps):
+def test_step_parity_with_oracle(name, kw, overrides, steps, substeps):
     gpu, cpu = helpers.product(), helpers.oracle()
     wg, _ = helpers.make_world(gpu, name, kw, overrides)
     wc, _ = helpers.make_world(cpu, name, kw, overrides)
     try:
         for s in range(steps):
-            gpu.step(wg)
-            cpu.step(wc)
+            gpu.step(wg, substeps=substeps)
+            cpu.step(wc, substeps=substeps)
             m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
             assert m is None, f"{name} step {s}: contacts differ: {m}"
             m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
