@@ -139,13 +139,15 @@ def main():
     ab = algorithmic_bytes(r["bodies"], r["contacts"], substeps)
     achieved = ab["solve"] / (r["ms_solve"] * 1e-3) / 1e9
 
+    # HBM bytes per k_solve launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes of the same
+    # workload, scaled by the counters' reading of a copy kernel of known size
+    workload = f"pyramid_grid_{args.rows}x{args.cols}_base10_plain_f32"
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"hbm_pyramid_grid_{args.rows}x{args.cols}_base10.json")
     if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get("k_solve_bytes_per_launch")
-        except Exception:
-            traffic = None
+        t = json.load(open(tfile))
+        if t.get("workload") == workload:
+            traffic = t.get("k_solve_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -161,7 +163,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"pyramid_grid_{args.rows}x{args.cols}_base10_plain_f32",
+            "workload": workload,
             "bodies": r["bodies"], "touching_contacts": r["contacts"], "substeps": substeps,
             "dt": 1.0 / 60.0, "graph_colors": r["colors"],
             "ms_solve_kernel": r["ms_solve"], "ms_collide_kernel": r["ms_collide"],

@@ -933,10 +933,12 @@ void StepGroup::step( float timeStep, int subStepCount )
 		collideListDirty = false;
 	}
 	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
-	hipEvent_t e0, e1, e2;
+	// events bracket each kernel launch alone (the stream is idle while the host works)
+	hipEvent_t e0, e1, e2, e3;
 	HIP_CHECK( hipEventCreate( &e0 ) );
 	HIP_CHECK( hipEventCreate( &e1 ) );
 	HIP_CHECK( hipEventCreate( &e2 ) );
+	HIP_CHECK( hipEventCreate( &e3 ) );
 	HIP_CHECK( hipEventRecord( e0, s ) );
 	HIP_CHECK( launch_collide( dWorlds, dCollideList, collideCount, dChanges, dCounters, s ) );
 	HIP_CHECK( hipEventRecord( e1, s ) );
@@ -1009,6 +1011,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 		while ( block < maxGroupSize && block < 1024 )
 			block *= 2;
 	}
+	HIP_CHECK( hipEventRecord( e3, s ) );
 	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, dBodyList, dSlotContact, dSlotBodies, dCons, capCons, dPriv,
 							 dEnlarge, dCounters + 1, capEnlarge, s ) );
 	HIP_CHECK( hipEventRecord( e2, s ) );
@@ -1034,10 +1037,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 		}
 	}
 	HIP_CHECK( hipEventElapsedTime( &msCollide, e0, e1 ) );
-	HIP_CHECK( hipEventElapsedTime( &msSolve, e1, e2 ) );
+	HIP_CHECK( hipEventElapsedTime( &msSolve, e3, e2 ) );
 	HIP_CHECK( hipEventDestroy( e0 ) );
 	HIP_CHECK( hipEventDestroy( e1 ) );
 	HIP_CHECK( hipEventDestroy( e2 ) );
+	HIP_CHECK( hipEventDestroy( e3 ) );
 	float msTotal = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t0 ).count();
 	msHost = msTotal - msCollide - msSolve;
 	for ( World* w : worlds )
