@@ -376,6 +376,8 @@ class B2Api:
         L.b2mi_Batch_Step.argtypes = [c_int, c_float, c_int]
         L.b2mi_Batch_Configure.argtypes = [c_int, c_int, c_int]
         L.b2mi_World_Configure.argtypes = [WorldId, c_int, c_int]
+        L.b2mi_Batch_SetLdsBudget.argtypes = [c_int, c_int]
+        L.b2mi_World_SetLdsBudget.argtypes = [WorldId, c_int]
         L.b2mi_Batch_GetWorkItemCount.argtypes = [c_int]
         L.b2mi_Batch_GetWorkItemCount.restype = c_int
         L.b2mi_Batch_GetTiming.argtypes = [c_int, POINTER(c_float), POINTER(c_float), POINTER(c_float)]

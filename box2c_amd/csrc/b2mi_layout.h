@@ -153,7 +153,8 @@ struct WorkItem
 	int ownedCount; // the first ownedCount bodies are finalized and written back
 	int slotStart;	// into the launch's constraint slots
 	int groupStart[kGroups + 1]; // relative to slotStart
-	int pad[2];
+	int ldsBytes;				 // > 0: the item's solver state lives in this much LDS; 0: in HBM
+	int pad;
 };
 
 // narrowphase state change (sorted by world, contactId on the host)
@@ -222,5 +223,15 @@ enum ConstraintField : int
 	cfPointFields = 11,
 	cfFloatCount = cfAnchorAX + 2 * 11,
 };
+
+// Bytes of LDS a work item's solver state needs: per body two float4, per slot the two local
+// body indices and the constraint fields. Items above the budget run from HBM.
+constexpr int kMaxItemLdsBytes = 64 * 1024;
+// threads per solver workgroup: at most this many (the kernel's launch bound)
+constexpr int kSolveMaxBlock = 256;
+inline int itemLdsBytes( int bodyCount, int slotCount )
+{
+	return 32 * bodyCount + 8 * slotCount + 4 * cfFloatCount * slotCount;
+}
 
 } // namespace b2mi

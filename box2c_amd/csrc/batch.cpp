@@ -80,6 +80,16 @@ B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots )
 		w->topologyDirty = true;
 }
 
+// maxBytes: largest LDS footprint of a work item kept in LDS (-1 = 64 KiB default, 0 = every
+// work item keeps its solver state in HBM). Results are identical either way.
+B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes )
+{
+	StepGroup* g = getBatch( batch );
+	g->ldsBudget = maxBytes;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+}
+
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount )
 {
 	StepGroup* g = getBatch( batch );
@@ -114,6 +124,20 @@ B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSl
 	}
 	w->group->blockSize = blockSize;
 	w->group->targetSlots = targetSlots;
+	w->topologyDirty = true;
+}
+
+B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes )
+{
+	World* w = getWorld( worldId );
+	B2MI_CHECK( w->batchWorld == false );
+	if ( w->group == nullptr )
+	{
+		w->group = new StepGroup();
+		w->group->worlds.push_back( w );
+		w->groupIndex = 0;
+	}
+	w->group->ldsBudget = maxBytes;
 	w->topologyDirty = true;
 }
 

@@ -185,7 +185,8 @@ struct Body
 	float dpx, dpy, dqc, dqs;
 };
 
-__device__ inline Body load_body( const float4* P, int i )
+template <class S>
+__device__ inline Body load_body( const S& C, int i )
 {
 	Body b;
 	if ( i < 0 )
@@ -201,7 +202,7 @@ __device__ inline Body load_body( const float4* P, int i )
 		b.dqs = 0.0f;
 		return b;
 	}
-	float4 s0 = P[2 * i], s1 = P[2 * i + 1];
+	float4 s0 = C.b0( i ), s1 = C.b1( i );
 	b.vx = s0.x;
 	b.vy = s0.y;
 	b.w = s0.z;
@@ -213,42 +214,89 @@ __device__ inline Body load_body( const float4* P, int i )
 	return b;
 }
 
-__device__ inline void store_velocity( float4* P, int i, const Body& b )
+template <class S>
+__device__ inline void store_velocity( const S& C, int i, const Body& b )
 {
 	if ( i < 0 )
 		return;
-	P[2 * i] = make_float4( b.vx, b.vy, b.w, b.flags );
+	C.b0( i ) = make_float4( b.vx, b.vy, b.w, b.flags );
 }
 
-struct Cons
+// Solver state of one work item, addressed by local slot k (0 .. slots-1) and local body i:
+// the constraint fields (cf* enum), each slot's two local body indices, and the private body
+// state (two float4 per body: v.x, v.y, w, flags | dp.x, dp.y, dq.c, dq.s).
+//
+// GlobalStore keeps it in HBM (any work item size). LdsStore keeps it in the workgroup's LDS,
+// so the substep passes (about 3 x colours x substeps dependent phases per step) run at LDS
+// latency; the host picks it for every work item whose state fits the LDS budget.
+struct GlobalStore
 {
 	float* f;
 	int cap;
-	__device__ inline float& at( int field, int slot ) const
+	int base;
+	const int2* sb;
+	float4* P;
+	__device__ inline float& at( int field, int k ) const
 	{
-		return f[(size_t)field * cap + slot];
+		return f[(size_t)field * cap + base + k];
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return sb[k];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return P[2 * i + 1];
+	}
+};
+
+struct LdsStore
+{
+	float* f;
+	int n;
+	int2* sb;
+	float4* P;
+	__device__ inline float& at( int field, int k ) const
+	{
+		return f[field * n + k];
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return sb[k];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return P[2 * i + 1];
 	}
 };
 
 #define PF( name, p ) ( cf##name + cfPointFields * ( p ) )
 
 // contact_solver.c:597-808 (lanes) and 21-147 (overflow): one constraint slot
-__device__ void prepare_slot( const Cons& C, int slot, const DevContact* __restrict__ contact, int la, int lb, const float4* P,
-							  const DevWorld& W )
+template <class S>
+__device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict__ contact, int la, int lb, const DevWorld& W )
 {
 	const DevManifold& m = contact->manifold;
 	float mA = contact->invMassA, iA = contact->invIA, mB = contact->invMassB, iB = contact->invIB;
 	float vAx = 0.0f, vAy = 0.0f, wA = 0.0f, vBx = 0.0f, vBy = 0.0f, wB = 0.0f;
 	if ( la >= 0 )
 	{
-		float4 s = P[2 * la];
+		float4 s = C.b0( la );
 		vAx = s.x;
 		vAy = s.y;
 		wA = s.z;
 	}
 	if ( lb >= 0 )
 	{
-		float4 s = P[2 * lb];
+		float4 s = C.b0( lb );
 		vBx = s.x;
 		vBy = s.y;
 		wB = s.z;
@@ -315,9 +363,10 @@ __device__ inline void lane_apply( Body& bA, Body& bB, float mA, float iA, float
 }
 
 // contact_solver.c:810-863
-__device__ void lane_warm_start( const Cons& C, int slot, int la, int lb, float4* P )
+template <class S>
+__device__ void lane_warm_start( const S& C, int slot, int la, int lb )
 {
-	Body bA = load_body( P, la ), bB = load_body( P, lb );
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float tx = ny, ty = 0.0f - nx;
@@ -335,14 +384,15 @@ __device__ void lane_warm_start( const Cons& C, int slot, int la, int lb, float4
 		bB.vx = bB.vx + mB * Px;
 		bB.vy = bB.vy + mB * Py;
 	}
-	store_velocity( P, la, bA );
-	store_velocity( P, lb, bB );
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
 }
 
 // contact_solver.c:865-1075
-__device__ void lane_solve( const Cons& C, int slot, int la, int lb, float4* P, float inv_h, float minBiasVel, bool useBias )
+template <class S>
+__device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias )
 {
-	Body bA = load_body( P, la ), bB = load_body( P, lb );
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float biasRate = useBias ? C.at( cfBiasRate, slot ) : 0.0f;
@@ -391,14 +441,15 @@ __device__ void lane_solve( const Cons& C, int slot, int la, int lb, float4* P, 
 		C.at( PF( TangentImpulse, j ), slot ) = newImpulse;
 		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * tx, impulse * ty );
 	}
-	store_velocity( P, la, bA );
-	store_velocity( P, lb, bB );
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
 }
 
 // contact_solver.c:1077-1178
-__device__ void lane_restitution( const Cons& C, int slot, int la, int lb, float4* P, float threshold )
+template <class S>
+__device__ void lane_restitution( const S& C, int slot, int la, int lb, float threshold )
 {
-	Body bA = load_body( P, la ), bB = load_body( P, lb );
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float restitution = C.at( cfRestitution, slot );
@@ -419,14 +470,15 @@ __device__ void lane_restitution( const Cons& C, int slot, int la, int lb, float
 		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
 		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * nx, impulse * ny );
 	}
-	store_velocity( P, la, bA );
-	store_velocity( P, lb, bB );
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
 }
 
 // ---- overflow (serial) stages, contact_solver.c:149-471 ----
-__device__ void ovf_warm_start( const Cons& C, int slot, int la, int lb, float4* P, int pointCount )
+template <class S>
+__device__ void ovf_warm_start( const S& C, int slot, int la, int lb, int pointCount )
 {
-	Body bA = load_body( P, la ), bB = load_body( P, lb );
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float tx = ny, ty = -nx;
@@ -444,14 +496,15 @@ __device__ void ovf_warm_start( const Cons& C, int slot, int la, int lb, float4*
 		bB.vx = bB.vx + mB * Px;
 		bB.vy = bB.vy + mB * Py;
 	}
-	store_velocity( P, la, bA );
-	store_velocity( P, lb, bB );
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
 }
 
-__device__ void ovf_solve( const Cons& C, int slot, int la, int lb, float4* P, int pointCount, float inv_h, float pushout,
+template <class S>
+__device__ void ovf_solve( const S& C, int slot, int la, int lb, int pointCount, float inv_h, float pushout,
 						   bool useBias )
 {
-	Body bA = load_body( P, la ), bB = load_body( P, lb );
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float tx = ny, ty = -nx;
@@ -513,16 +566,17 @@ __device__ void ovf_solve( const Cons& C, int slot, int la, int lb, float4* P, i
 		bB.vy = bB.vy + mB * Py;
 		bB.w += iB * ( rBx * Py - rBy * Px );
 	}
-	store_velocity( P, la, bA );
-	store_velocity( P, lb, bB );
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
 }
 
-__device__ void ovf_restitution( const Cons& C, int slot, int la, int lb, float4* P, int pointCount, float threshold )
+template <class S>
+__device__ void ovf_restitution( const S& C, int slot, int la, int lb, int pointCount, float threshold )
 {
 	float restitution = C.at( cfRestitution, slot );
 	if ( restitution == 0.0f )
 		return;
-	Body bA = load_body( P, la ), bB = load_body( P, lb );
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	for ( int j = 0; j < pointCount; ++j )
@@ -549,16 +603,17 @@ __device__ void ovf_restitution( const Cons& C, int slot, int la, int lb, float4
 		bB.vy = bB.vy + mB * Py;
 		bB.w += iB * ( rBx * Py - rBy * Px );
 	}
-	store_velocity( P, la, bA );
-	store_velocity( P, lb, bB );
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
 }
 
 // solver.c:34-93 for one body of the private set
-__device__ void integrate_velocity( float4* P, int i, const DevBodySim& sim, const DevWorld& W )
+template <class S>
+__device__ void integrate_velocity( const S& C, int i, const DevBodySim& sim, const DevWorld& W )
 {
 	float h = W.h;
 	float maxLinearSpeed = W.maxLinearSpeed, maxAngularSpeed = W.maxAngularSpeed;
-	float4 s0 = P[2 * i];
+	float4 s0 = C.b0( i );
 	float vx = s0.x, vy = s0.y, w = s0.z;
 	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
 	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
@@ -581,14 +636,15 @@ __device__ void integrate_velocity( float4* P, int i, const DevBodySim& sim, con
 		float ratio = maxAngularSpeed / ( w < 0 ? -w : w );
 		w *= ratio;
 	}
-	P[2 * i] = make_float4( vx, vy, w, s0.w );
+	C.b0( i ) = make_float4( vx, vy, w, s0.w );
 }
 
 // solver.c:146-163
-__device__ void integrate_position( float4* P, int i, float h )
+template <class S>
+__device__ void integrate_position( const S& C, int i, float h )
 {
-	float4 s0 = P[2 * i];
-	float4 s1 = P[2 * i + 1];
+	float4 s0 = C.b0( i );
+	float4 s1 = C.b1( i );
 	float da = h * s0.z;
 	float c2 = s1.z - da * s1.w;
 	float s2 = s1.w + da * s1.z;
@@ -598,7 +654,7 @@ __device__ void integrate_position( float4* P, int i, float h )
 	s1.w = s2 * inv;
 	s1.x = s1.x + h * s0.x;
 	s1.y = s1.y + h * s0.y;
-	P[2 * i + 1] = s1;
+	C.b1( i ) = s1;
 }
 
 __device__ inline void shape_aabb( const DevShape& s, b2Transform xf, float* out )
@@ -651,11 +707,10 @@ __device__ inline bool aabb_contains( const float* a, const float* b )
 }
 
 // solver.c:165-351 (+ the no-hit branch of continuous, solver.c:1035-1066) for one body
-__device__ void finalize_body( const DevWorld& W, int simIndex, float4* P, int li, int worldIndex, EnlargeRecord* enl,
+__device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float4 s1, int worldIndex, EnlargeRecord* enl,
 							   int* enlCount, int enlCapacity )
 {
 	DevBodySim& sim = W.sims[simIndex];
-	float4 s0 = P[2 * li], s1 = P[2 * li + 1];
 	b2Vec2 v = v2( s0.x, s0.y );
 	float w = s0.z;
 	b2Vec2 center = b2Add( v2( sim.center[0], sim.center[1] ), v2( s1.x, s1.y ) );
@@ -764,47 +819,37 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4* P, int l
 			}
 		}
 	}
-	// reset deltas, keep velocity
-	P[2 * li + 1] = make_float4( 0.0f, 0.0f, 1.0f, 0.0f );
 }
 
-// One workgroup per work item; all stages of the step with workgroup barriers.
-__global__ void __launch_bounds__( 1024 ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
-												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
-												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
-												   float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity )
+// All stages of one step for one work item, by one workgroup, with workgroup barriers between
+// dependent phases. `C` holds the item's private state; `bl` its body list (sim indices) and
+// `sc` the contact id of each of its slots.
+template <class S>
+__device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, const int* __restrict__ bl,
+							const int* __restrict__ sc, EnlargeRecord* enl, int* enlCount, int enlCapacity )
 {
-	const WorkItem wi = items[blockIdx.x];
-	const DevWorld& W = worlds[wi.world];
 	const int tid = threadIdx.x, nt = blockDim.x;
-	float4* P = priv + 2 * (size_t)wi.bodyStart;
-	const int* bl = bodyList + wi.bodyStart;
 	const float4* gstate = reinterpret_cast<const float4*>( W.state );
-	Cons C;
-	C.f = cfBase;
-	C.cap = cfCapacity;
-	const int s0 = wi.slotStart;
 	const int total = wi.groupStart[kGroups];
 
 	// stage private body state
 	for ( int i = tid; i < wi.bodyCount; i += nt )
 	{
 		int b = bl[i];
-		P[2 * i] = gstate[2 * b];
-		P[2 * i + 1] = gstate[2 * b + 1];
+		C.b0( i ) = gstate[2 * b];
+		C.b1( i ) = gstate[2 * b + 1];
 	}
 	__syncthreads();
 
 	// prepare (all groups; the overflow group uses the same per-point formulas)
 	for ( int k = tid; k < total; k += nt )
 	{
-		int slot = s0 + k;
-		int2 lb = slotBodies[slot];
-		prepare_slot( C, slot, W.contacts + slotContact[slot], lb.x, lb.y, P, W );
+		int2 lb = C.bodies( k );
+		prepare_slot( C, k, W.contacts + sc[k], lb.x, lb.y, W );
 	}
 	__syncthreads();
 
-	const int ovfBegin = s0 + wi.groupStart[0], ovfEnd = s0 + wi.groupStart[1];
+	const int ovfBegin = wi.groupStart[0], ovfEnd = wi.groupStart[1];
 	const bool hasOverflow = ovfEnd > ovfBegin;
 	const float inv_h = W.inv_h, h = W.h;
 	const float pushout = W.pushout, minBiasVel = -W.pushout;
@@ -813,30 +858,30 @@ __global__ void __launch_bounds__( 1024 ) k_solve( const DevWorld* __restrict__ 
 	for ( int sub = 0; sub < W.subSteps; ++sub )
 	{
 		for ( int i = tid; i < wi.bodyCount; i += nt )
-			integrate_velocity( P, i, W.sims[bl[i]], W );
+			integrate_velocity( C, i, W.sims[bl[i]], W );
 		__syncthreads();
 
 		// warm start: overflow (always, src/solver.c:639), then colours if enabled
 		if ( hasOverflow )
 		{
 			if ( tid == 0 )
-				for ( int slot = ovfBegin; slot < ovfEnd; ++slot )
+				for ( int k = ovfBegin; k < ovfEnd; ++k )
 				{
-					int2 lb = slotBodies[slot];
-					ovf_warm_start( C, slot, lb.x, lb.y, P, W.contacts[slotContact[slot]].manifold.pointCount );
+					int2 lb = C.bodies( k );
+					ovf_warm_start( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount );
 				}
 			__syncthreads();
 		}
 		if ( warm )
 			for ( int g = 1; g < kGroups; ++g )
 			{
-				int b = s0 + wi.groupStart[g], e = s0 + wi.groupStart[g + 1];
+				int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 				if ( b == e )
 					continue;
-				for ( int slot = b + tid; slot < e; slot += nt )
+				for ( int k = b + tid; k < e; k += nt )
 				{
-					int2 lb = slotBodies[slot];
-					lane_warm_start( C, slot, lb.x, lb.y, P );
+					int2 lb = C.bodies( k );
+					lane_warm_start( C, k, lb.x, lb.y );
 				}
 				__syncthreads();
 			}
@@ -845,50 +890,50 @@ __global__ void __launch_bounds__( 1024 ) k_solve( const DevWorld* __restrict__ 
 		if ( hasOverflow )
 		{
 			if ( tid == 0 )
-				for ( int slot = ovfBegin; slot < ovfEnd; ++slot )
+				for ( int k = ovfBegin; k < ovfEnd; ++k )
 				{
-					int2 lb = slotBodies[slot];
-					ovf_solve( C, slot, lb.x, lb.y, P, W.contacts[slotContact[slot]].manifold.pointCount, inv_h, pushout, true );
+					int2 lb = C.bodies( k );
+					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true );
 				}
 			__syncthreads();
 		}
 		for ( int g = 1; g < kGroups; ++g )
 		{
-			int b = s0 + wi.groupStart[g], e = s0 + wi.groupStart[g + 1];
+			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 			if ( b == e )
 				continue;
-			for ( int slot = b + tid; slot < e; slot += nt )
+			for ( int k = b + tid; k < e; k += nt )
 			{
-				int2 lb = slotBodies[slot];
-				lane_solve( C, slot, lb.x, lb.y, P, inv_h, minBiasVel, true );
+				int2 lb = C.bodies( k );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true );
 			}
 			__syncthreads();
 		}
 
 		for ( int i = tid; i < wi.bodyCount; i += nt )
-			integrate_position( P, i, h );
+			integrate_position( C, i, h );
 		__syncthreads();
 
 		// relax
 		if ( hasOverflow )
 		{
 			if ( tid == 0 )
-				for ( int slot = ovfBegin; slot < ovfEnd; ++slot )
+				for ( int k = ovfBegin; k < ovfEnd; ++k )
 				{
-					int2 lb = slotBodies[slot];
-					ovf_solve( C, slot, lb.x, lb.y, P, W.contacts[slotContact[slot]].manifold.pointCount, inv_h, pushout, false );
+					int2 lb = C.bodies( k );
+					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false );
 				}
 			__syncthreads();
 		}
 		for ( int g = 1; g < kGroups; ++g )
 		{
-			int b = s0 + wi.groupStart[g], e = s0 + wi.groupStart[g + 1];
+			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 			if ( b == e )
 				continue;
-			for ( int slot = b + tid; slot < e; slot += nt )
+			for ( int k = b + tid; k < e; k += nt )
 			{
-				int2 lb = slotBodies[slot];
-				lane_solve( C, slot, lb.x, lb.y, P, inv_h, minBiasVel, false );
+				int2 lb = C.bodies( k );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false );
 			}
 			__syncthreads();
 		}
@@ -899,22 +944,22 @@ __global__ void __launch_bounds__( 1024 ) k_solve( const DevWorld* __restrict__ 
 	if ( hasOverflow )
 	{
 		if ( tid == 0 )
-			for ( int slot = ovfBegin; slot < ovfEnd; ++slot )
+			for ( int k = ovfBegin; k < ovfEnd; ++k )
 			{
-				int2 lb = slotBodies[slot];
-				ovf_restitution( C, slot, lb.x, lb.y, P, W.contacts[slotContact[slot]].manifold.pointCount, threshold );
+				int2 lb = C.bodies( k );
+				ovf_restitution( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, threshold );
 			}
 		__syncthreads();
 	}
 	for ( int g = 1; g < kGroups; ++g )
 	{
-		int b = s0 + wi.groupStart[g], e = s0 + wi.groupStart[g + 1];
+		int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 		if ( b == e )
 			continue;
-		for ( int slot = b + tid; slot < e; slot += nt )
+		for ( int k = b + tid; k < e; k += nt )
 		{
-			int2 lb = slotBodies[slot];
-			lane_restitution( C, slot, lb.x, lb.y, P, threshold );
+			int2 lb = C.bodies( k );
+			lane_restitution( C, k, lb.x, lb.y, threshold );
 		}
 		__syncthreads();
 	}
@@ -922,26 +967,63 @@ __global__ void __launch_bounds__( 1024 ) k_solve( const DevWorld* __restrict__ 
 	// store impulses (contact_solver.c:442-471 overflow: pointCount points; 1180-1293 lanes: both)
 	for ( int k = tid; k < total; k += nt )
 	{
-		int slot = s0 + k;
-		DevManifold& m = W.contacts[slotContact[slot]].manifold;
-		int n = slot < ovfEnd ? m.pointCount : 2;
+		DevManifold& m = W.contacts[sc[k]].manifold;
+		int n = k < ovfEnd ? m.pointCount : 2;
 		for ( int j = 0; j < n; ++j )
 		{
-			m.points[j].normalImpulse = C.at( PF( NormalImpulse, j ), slot );
-			m.points[j].tangentImpulse = C.at( PF( TangentImpulse, j ), slot );
-			m.points[j].maxNormalImpulse = C.at( PF( MaxNormalImpulse, j ), slot );
-			m.points[j].normalVelocity = C.at( PF( RelativeVelocity, j ), slot );
+			m.points[j].normalImpulse = C.at( PF( NormalImpulse, j ), k );
+			m.points[j].tangentImpulse = C.at( PF( TangentImpulse, j ), k );
+			m.points[j].maxNormalImpulse = C.at( PF( MaxNormalImpulse, j ), k );
+			m.points[j].normalVelocity = C.at( PF( RelativeVelocity, j ), k );
 		}
 	}
 
-	// finalize owned bodies and write their state back
+	// finalize owned bodies and write their state back (velocity kept, deltas reset)
 	float4* gs = reinterpret_cast<float4*>( W.state );
 	for ( int i = tid; i < wi.ownedCount; i += nt )
 	{
 		int b = bl[i];
-		finalize_body( W, b, P, i, wi.world, enl, enlCount, enlCapacity );
-		gs[2 * b] = P[2 * i];
-		gs[2 * b + 1] = P[2 * i + 1];
+		float4 s0 = C.b0( i ), s1 = C.b1( i );
+		finalize_body( W, b, s0, s1, wi.world, enl, enlCount, enlCapacity );
+		gs[2 * b] = s0;
+		gs[2 * b + 1] = make_float4( 0.0f, 0.0f, 1.0f, 0.0f );
+	}
+}
+
+// One workgroup per work item. Items with ldsBytes > 0 keep their state in the dynamic LDS
+// block (layout: body state, slot bodies, constraint fields); the others in HBM.
+__global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
+												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
+												   float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity )
+{
+	extern __shared__ float4 lds[];
+	const WorkItem wi = items[blockIdx.x];
+	const DevWorld& W = worlds[wi.world];
+	const int* bl = bodyList + wi.bodyStart;
+	const int* sc = slotContact + wi.slotStart;
+	const int total = wi.groupStart[kGroups];
+	if ( wi.ldsBytes > 0 )
+	{
+		LdsStore C;
+		C.P = lds;
+		C.sb = reinterpret_cast<int2*>( lds + 2 * wi.bodyCount );
+		C.f = reinterpret_cast<float*>( C.sb + total );
+		C.n = total;
+		for ( int k = threadIdx.x; k < total; k += blockDim.x )
+			C.sb[k] = slotBodies[wi.slotStart + k];
+		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
+		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
+	}
+	else
+	{
+		GlobalStore C;
+		C.f = cfBase;
+		C.cap = cfCapacity;
+		C.base = wi.slotStart;
+		C.sb = slotBodies + wi.slotStart;
+		C.P = priv + 2 * (size_t)wi.bodyStart;
+		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
 	}
 }
 
@@ -970,14 +1052,16 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 	return hipGetLastError();
 }
 
-hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
-						 const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity, float4* priv,
-						 EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
+hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, int ldsBytes,
+						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
+						 float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
-	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotContact, slotBodies,
-						cf, cfCapacity, priv, enl, enlCount, enlCapacity );
+	if ( ldsBytes < 0 || ldsBytes > kMaxItemLdsBytes || blockSize < 64 || blockSize > kSolveMaxBlock )
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
+						slotBodies, cf, cfCapacity, priv, enl, enlCount, enlCapacity );
 	return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Shapes, contacts, broadphase pairs, graph colouring, work-item building and the step of
 // a group of worlds (one world for b2World_Step, many for a batch). Line references are to
 // the reference.
+#include "launch.hpp"
 #include "world_internal.hpp"
 
 #include <algorithm>
@@ -13,12 +14,6 @@
 namespace b2mi
 {
 
-hipError_t launch_init_contacts( const DevWorld* worlds, const ContactInit* inits, int count, hipStream_t stream );
-hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, ChangeRecord* changes, int* changeCount,
-						   hipStream_t stream );
-hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
-						 const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity, float4* priv,
-						 EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
 
 // defined in world.cpp
 void b2mi_internal_beginEdit( World* w );
@@ -510,6 +505,7 @@ static void buildWorkItems( StepGroup* g )
 	g->hSlotContact.clear();
 	g->hSlotBodies.clear();
 	g->maxGroupSize = 0;
+	g->maxItemLds = 0;
 	int target = g->targetSlots > 0 ? g->targetSlots : 256;
 
 	for ( World* w : g->worlds )
@@ -647,6 +643,12 @@ static void buildWorkItems( StepGroup* g )
 			}
 			wi.groupStart[kGroups] = (int)g->hSlotContact.size() - wi.slotStart;
 			wi.bodyCount = (int)g->hBodyList.size() - wi.bodyStart;
+			{
+				int budget = g->ldsBudget < 0 ? kMaxItemLdsBytes : std::min( g->ldsBudget, kMaxItemLdsBytes );
+				int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
+				wi.ldsBytes = bytes <= budget ? bytes : 0;
+				g->maxItemLds = std::max( g->maxItemLds, wi.ldsBytes );
+			}
 			g->maxGroupSize = std::max( g->maxGroupSize, wi.ownedCount );
 			g->hItems.push_back( wi );
 			ci = cj;
@@ -667,6 +669,8 @@ static void buildWorkItems( StepGroup* g )
 		for ( int i = 0; i < wi.bodyCount; ++i )
 			B2MI_CHECK( 0 <= g->hBodyList[wi.bodyStart + i] && g->hBodyList[wi.bodyStart + i] < na );
 		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= g->slotCount );
+		B2MI_CHECK( wi.ldsBytes == 0 || ( wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) &&
+										  wi.ldsBytes <= g->maxItemLds && g->maxItemLds <= kMaxItemLdsBytes ) );
 		for ( int k = 0; k < kGroups; ++k )
 			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
 		for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
@@ -1008,12 +1012,13 @@ void StepGroup::step( float timeStep, int subStepCount )
 	if ( block <= 0 )
 	{
 		block = 64;
-		while ( block < maxGroupSize && block < 1024 )
+		while ( block < maxGroupSize && block < kSolveMaxBlock )
 			block *= 2;
 	}
+	B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
 	HIP_CHECK( hipEventRecord( e3, s ) );
-	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, dBodyList, dSlotContact, dSlotBodies, dCons, capCons, dPriv,
-							 dEnlarge, dCounters + 1, capEnlarge, s ) );
+	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons, capCons,
+							 dPriv, dEnlarge, dCounters + 1, capEnlarge, s ) );
 	HIP_CHECK( hipEventRecord( e2, s ) );
 	HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
 	HIP_CHECK( hipStreamSynchronize( s ) );

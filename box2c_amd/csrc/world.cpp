@@ -3,6 +3,7 @@
 // Serial, order-defining work (ids, pairs, contact creation, graph colouring, tree
 // maintenance) runs here exactly as the reference orders it; the data-parallel work
 // (narrowphase, solver, finalize) runs in kernels.hip. Line references are to the reference.
+#include "launch.hpp"
 #include "world_internal.hpp"
 
 #include <algorithm>
@@ -17,12 +18,6 @@
 namespace b2mi
 {
 
-hipError_t launch_init_contacts( const DevWorld* worlds, const ContactInit* inits, int count, hipStream_t stream );
-hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, ChangeRecord* changes, int* changeCount,
-						   hipStream_t stream );
-hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
-						 const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity, float4* priv,
-						 EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
 
 // ---------------------------------------------------------------------------------------
 // errors

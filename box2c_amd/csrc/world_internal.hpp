@@ -228,6 +228,7 @@ struct StepGroup
 	std::vector<World*> worlds;
 	int blockSize = 0;		  // 0 = automatic
 	int targetSlots = 0;	  // work-item packing target (constraint slots), 0 = automatic
+	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kMaxItemLdsBytes, 0 = HBM only)
 	bool structureDirty = true;
 
 	// launch buffers (device)
@@ -259,6 +260,7 @@ struct StepGroup
 	int capPriv = 0;
 	int slotCount = 0, bodyListCount = 0;
 	int maxGroupSize = 0;
+	int maxItemLds = 0; // dynamic LDS bytes of the solver launch (largest LDS-resident item)
 
 	// host staging
 	std::vector<DevWorld> hWorlds;

@@ -64,11 +64,17 @@ B2_API int b2mi_Batch_GetWorldCount( int batch );
 // Steps every world of the batch (the batch counterpart of b2World_Step).
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount );
 
-// Solver launch shape: threads per workgroup (0 = automatic) and the number of contact
-// constraints packed into one work item (0 = 256).
+// Solver launch shape: threads per workgroup (0 = automatic, else a multiple of 64 up to 256)
+// and the number of contact constraints packed into one work item (0 = 256).
 B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots );
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots );
 B2_API int b2mi_Batch_GetWorkItemCount( int batch );
+
+// Largest LDS footprint (bytes) of a work item whose solver state is kept in LDS; larger work
+// items keep it in HBM. -1 = the 64 KiB default, 0 = HBM for every work item. The results are
+// bit-identical either way; only the speed differs.
+B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes );
+B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
 
 // Device time of the last step's narrowphase and solver launches and the remaining host
 // time, in milliseconds.

@@ -90,3 +90,26 @@ def test_hello_world_rests_on_ground():
     gpu.destroy(w)
     assert abs(p.x) < 0.01 and abs(p.y - 1.0) < 0.01
     assert abs(np.arctan2(q.s, q.c)) < 0.01
+
+
+@pytest.mark.parametrize("name,kw,overrides,steps,substeps", helpers.PARITY_SCENES[:4] + helpers.PARITY_SCENES[6:],
+                         ids=[s[0] for s in helpers.PARITY_SCENES[:4] + helpers.PARITY_SCENES[6:]])
+@pytest.mark.parametrize("budget", [0, 24 * 1024], ids=["hbm", "lds24k"])
+def test_step_parity_solver_storage(name, kw, overrides, steps, substeps, budget):
+    """The solver keeps a work item's state in LDS when it fits the budget and in HBM
+    otherwise; both give the oracle's bits (budget 0 forces HBM for every item)."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    wg, _ = helpers.make_world(gpu, name, kw, overrides)
+    wc, _ = helpers.make_world(cpu, name, kw, overrides)
+    gpu.lib.b2mi_World_SetLdsBudget(wg, budget)
+    try:
+        for s in range(min(steps, 40)):
+            gpu.step(wg, substeps=substeps)
+            cpu.step(wc, substeps=substeps)
+        m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+        assert m is None, f"{name}: contacts differ: {m}"
+        m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+        assert m is None, f"{name}: bodies differ: {m}"
+    finally:
+        gpu.destroy(wg)
+        cpu.destroy(wc)
