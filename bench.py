@@ -47,12 +47,16 @@ def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
     api.step(w, n=warmup)
     counters = api.counters(w)
     t_solve = t_collide = 0.0
+    host = {"pairs": 0.0, "broadphase": 0.0, "buildIslands": 0.0, "finalizeBodies": 0.0, "step": 0.0}
     t0 = time.perf_counter()
     for _ in range(steps):
         api.lib.b2World_Step(w, 1.0 / 60.0, 4)
         c, s, _h = api.world_timing(w)
         t_collide += c
         t_solve += s
+        prof = api.lib.b2World_GetProfile(w)
+        for k in host:
+            host[k] += getattr(prof, k)
     elapsed = time.perf_counter() - t0
     counters = api.counters(w)
     n_contacts = sum(counters.colorCounts)
@@ -61,6 +65,7 @@ def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
         "bodies": n_bodies, "contacts": n_contacts, "elapsed": elapsed,
         "ms_solve": t_solve / steps, "ms_collide": t_collide / steps,
         "colors": list(counters.colorCounts),
+        "host_ms": {k: v / steps for k, v in host.items()},
     }
 
 
@@ -167,6 +172,7 @@ def main():
             "bodies": r["bodies"], "touching_contacts": r["contacts"], "substeps": substeps,
             "dt": 1.0 / 60.0, "graph_colors": r["colors"],
             "ms_solve_kernel": r["ms_solve"], "ms_collide_kernel": r["ms_collide"],
+            "ms_step_phases": r["host_ms"],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

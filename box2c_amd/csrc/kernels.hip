@@ -110,12 +110,18 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 		else
 			simFlags &= ~kSimHitEvent;
 
-		DevManifold old = c->manifold;
+		// warm-start transfer by feature id (src/contact.c:555-568); the two points are
+		// unrolled so the manifolds stay in registers
+		uint16_t oldId[2] = { c->manifold.points[0].id, c->manifold.points[1].id };
+		float oldNormal[2] = { c->manifold.points[0].normalImpulse, c->manifold.points[1].normalImpulse };
+		float oldTangent[2] = { c->manifold.points[0].tangentImpulse, c->manifold.points[1].tangentImpulse };
+		int oldCount = c->manifold.pointCount;
 		DevManifold nm;
 		memset( &nm, 0, sizeof( nm ) );
 		nm.normalX = m.normal.x;
 		nm.normalY = m.normal.y;
 		nm.pointCount = m.pointCount;
+#pragma unroll
 		for ( int k = 0; k < 2; ++k )
 		{
 			DevManifoldPoint& p = nm.points[k];
@@ -127,29 +133,30 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 			p.anchorBY = m.anchorB[k].y;
 			p.separation = m.separation[k];
 			p.id = m.id[k];
-		}
-		for ( int k = 0; k < m.pointCount; ++k )
-		{
-			DevManifoldPoint& p = nm.points[k];
-			b2Vec2 aA = b2Sub( v2( p.anchorAX, p.anchorAY ), offsetA );
-			b2Vec2 aB = b2Sub( v2( p.anchorBX, p.anchorBY ), offsetB );
-			p.anchorAX = aA.x;
-			p.anchorAY = aA.y;
-			p.anchorBX = aB.x;
-			p.anchorBY = aB.y;
-			p.normalImpulse = 0.0f;
-			p.tangentImpulse = 0.0f;
-			p.maxNormalImpulse = 0.0f;
-			p.normalVelocity = 0.0f;
-			p.persisted = 0;
-			for ( int j = 0; j < old.pointCount; ++j )
+			if ( k < m.pointCount )
 			{
-				if ( old.points[j].id == p.id )
+				b2Vec2 aA = b2Sub( v2( p.anchorAX, p.anchorAY ), offsetA );
+				b2Vec2 aB = b2Sub( v2( p.anchorBX, p.anchorBY ), offsetB );
+				p.anchorAX = aA.x;
+				p.anchorAY = aA.y;
+				p.anchorBX = aB.x;
+				p.anchorBY = aB.y;
+				p.normalImpulse = 0.0f;
+				p.tangentImpulse = 0.0f;
+				p.maxNormalImpulse = 0.0f;
+				p.normalVelocity = 0.0f;
+				p.persisted = 0;
+				bool found = false;
+#pragma unroll
+				for ( int j = 0; j < 2; ++j )
 				{
-					p.normalImpulse = old.points[j].normalImpulse;
-					p.tangentImpulse = old.points[j].tangentImpulse;
-					p.persisted = 1;
-					break;
+					if ( !found && j < oldCount && oldId[j] == p.id )
+					{
+						p.normalImpulse = oldNormal[j];
+						p.tangentImpulse = oldTangent[j];
+						p.persisted = 1;
+						found = true;
+					}
 				}
 			}
 		}
@@ -998,7 +1005,8 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 												   float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity )
 {
 	extern __shared__ float4 lds[];
-	const WorkItem wi = items[blockIdx.x];
+	// read in place (uniform scalar loads); a private copy would be indexed dynamically from scratch
+	const WorkItem& wi = items[blockIdx.x];
 	const DevWorld& W = worlds[wi.world];
 	const int* bl = bodyList + wi.bodyStart;
 	const int* sc = slotContact + wi.slotStart;

@@ -70,6 +70,23 @@ struct Poly
 	int count;
 };
 
+// Element i of a polygon array. Every polygon array here is indexed either by a constant (fully
+// unrolled loops over the 8 slots, guarded by i < count) or through pick(), so the arrays live
+// in registers on the GPU instead of per-lane scratch memory.
+B2MI_HD inline b2Vec2 pick( const b2Vec2 ( &a )[8], int i )
+{
+	// bitwise select: a compare-and-select chain would be folded back into one indexed load
+	uint32_t x = 0u, y = 0u;
+#pragma unroll
+	for ( int k = 0; k < 8; ++k )
+	{
+		uint32_t mask = 0u - (uint32_t)( i == k );
+		x |= __builtin_bit_cast( uint32_t, a[k].x ) & mask;
+		y |= __builtin_bit_cast( uint32_t, a[k].y ) & mask;
+	}
+	return v2( __builtin_bit_cast( float, x ), __builtin_bit_cast( float, y ) );
+}
+
 struct Manifold
 {
 	b2Vec2 normal;
@@ -109,6 +126,12 @@ B2MI_HD inline void mf_finish_point( Manifold& m, b2Transform xfA, b2Transform x
 // manifold.c:14-30
 B2MI_HD inline void make_capsule( Poly& s, b2Vec2 p1, b2Vec2 p2, float radius )
 {
+#pragma unroll
+	for ( int i = 2; i < 8; ++i )
+	{
+		s.v[i] = v2( 0.0f, 0.0f );
+		s.n[i] = v2( 0.0f, 0.0f );
+	}
 	s.v[0] = p1;
 	s.v[1] = p2;
 	b2Vec2 d = b2Sub( p2, p1 );
@@ -180,20 +203,24 @@ B2MI_HD inline void collide_polygon_circle( Manifold& m, const Poly& A, b2Transf
 	float radius = radiusA + rB;
 	int normalIndex = 0;
 	float separation = -FLT_MAX;
-	for ( int i = 0; i < A.count; ++i )
+#pragma unroll
+	for ( int i = 0; i < 8; ++i )
 	{
-		float s = b2Dot( A.n[i], b2Sub( c, A.v[i] ) );
-		if ( s > separation )
+		if ( i < A.count )
 		{
-			separation = s;
-			normalIndex = i;
+			float s = b2Dot( A.n[i], b2Sub( c, A.v[i] ) );
+			if ( s > separation )
+			{
+				separation = s;
+				normalIndex = i;
+			}
 		}
 	}
 	if ( separation > radius + K_SPECULATIVE )
 		return;
 	int i1 = normalIndex;
 	int i2 = i1 + 1 < A.count ? i1 + 1 : 0;
-	b2Vec2 v1 = A.v[i1], v2_ = A.v[i2];
+	b2Vec2 v1 = pick( A.v, i1 ), v2_ = pick( A.v, i2 );
 	float u1 = b2Dot( b2Sub( c, v1 ), b2Sub( v2_, v1 ) );
 	float u2 = b2Dot( b2Sub( c, v2_ ), b2Sub( v1, v2_ ) );
 	if ( ( u1 < 0.0f || u2 < 0.0f ) && separation > FLT_EPSILON )
@@ -209,7 +236,7 @@ B2MI_HD inline void collide_polygon_circle( Manifold& m, const Poly& A, b2Transf
 	}
 	else
 	{
-		b2Vec2 normal = A.n[normalIndex];
+		b2Vec2 normal = pick( A.n, normalIndex );
 		b2Vec2 ca = b2MulAdd( c, radiusA - b2Dot( b2Sub( c, v1 ), normal ), normal );
 		b2Vec2 cb = b2MulSub( c, rB, normal );
 		mf_finish_point( m, xfA, xfB, normal, b2Lerp( ca, cb, 0.5f ), separation - radius );
@@ -221,21 +248,29 @@ B2MI_HD inline float find_max_separation( int* edgeIndex, const Poly& p1, const 
 {
 	int bestIndex = 0;
 	float maxSeparation = -FLT_MAX;
-	for ( int i = 0; i < p1.count; ++i )
+#pragma unroll
+	for ( int i = 0; i < 8; ++i )
 	{
-		b2Vec2 n = p1.n[i];
-		b2Vec2 v1 = p1.v[i];
-		float si = FLT_MAX;
-		for ( int j = 0; j < p2.count; ++j )
+		if ( i < p1.count )
 		{
-			float sij = b2Dot( n, b2Sub( p2.v[j], v1 ) );
-			if ( sij < si )
-				si = sij;
-		}
-		if ( si > maxSeparation )
-		{
-			maxSeparation = si;
-			bestIndex = i;
+			b2Vec2 n = p1.n[i];
+			b2Vec2 v1 = p1.v[i];
+			float si = FLT_MAX;
+#pragma unroll
+			for ( int j = 0; j < 8; ++j )
+			{
+				if ( j < p2.count )
+				{
+					float sij = b2Dot( n, b2Sub( p2.v[j], v1 ) );
+					if ( sij < si )
+						si = sij;
+				}
+			}
+			if ( si > maxSeparation )
+			{
+				maxSeparation = si;
+				bestIndex = i;
+			}
 		}
 	}
 	*edgeIndex = bestIndex;
@@ -301,30 +336,37 @@ B2MI_HD inline void segment_distance( b2Vec2 p1, b2Vec2 q1, b2Vec2 p2, b2Vec2 q2
 // manifold.c:276-411
 B2MI_HD inline void clip_polygons( Manifold& m, const Poly& polyA, const Poly& polyB, int edgeA, int edgeB, bool flip )
 {
-	const Poly* poly1;
-	const Poly* poly2;
 	int i11, i12, i21, i22;
+	b2Vec2 normal, v11, v12, v21, v22;
+	float r1, r2;
 	if ( flip )
 	{
-		poly1 = &polyB;
-		poly2 = &polyA;
 		i11 = edgeB;
 		i12 = edgeB + 1 < polyB.count ? edgeB + 1 : 0;
 		i21 = edgeA;
 		i22 = edgeA + 1 < polyA.count ? edgeA + 1 : 0;
+		normal = pick( polyB.n, i11 );
+		v11 = pick( polyB.v, i11 );
+		v12 = pick( polyB.v, i12 );
+		v21 = pick( polyA.v, i21 );
+		v22 = pick( polyA.v, i22 );
+		r1 = polyB.radius;
+		r2 = polyA.radius;
 	}
 	else
 	{
-		poly1 = &polyA;
-		poly2 = &polyB;
 		i11 = edgeA;
 		i12 = edgeA + 1 < polyA.count ? edgeA + 1 : 0;
 		i21 = edgeB;
 		i22 = edgeB + 1 < polyB.count ? edgeB + 1 : 0;
+		normal = pick( polyA.n, i11 );
+		v11 = pick( polyA.v, i11 );
+		v12 = pick( polyA.v, i12 );
+		v21 = pick( polyB.v, i21 );
+		v22 = pick( polyB.v, i22 );
+		r1 = polyA.radius;
+		r2 = polyB.radius;
 	}
-	b2Vec2 normal = poly1->n[i11];
-	b2Vec2 v11 = poly1->v[i11], v12 = poly1->v[i12];
-	b2Vec2 v21 = poly2->v[i21], v22 = poly2->v[i22];
 	b2Vec2 tangent = b2CrossSV( 1.0f, normal );
 	float lower1 = 0.0f;
 	float upper1 = b2Dot( b2Sub( v12, v11 ), tangent );
@@ -342,7 +384,6 @@ B2MI_HD inline void clip_polygons( Manifold& m, const Poly& polyA, const Poly& p
 		vUpper = v21;
 	float separationLower = b2Dot( b2Sub( vLower, v11 ), normal );
 	float separationUpper = b2Dot( b2Sub( vUpper, v11 ), normal );
-	float r1 = poly1->radius, r2 = poly2->radius;
 	vLower = b2MulAdd( vLower, 0.5f * ( r1 - r2 - separationLower ), normal );
 	vUpper = b2MulAdd( vUpper, 0.5f * ( r1 - r2 - separationUpper ), normal );
 	float radius = r1 + r2;
@@ -403,18 +444,20 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 	localA.radius = polygonA.radius;
 	localA.v[0] = v2( 0.0f, 0.0f );
 	localA.n[0] = polygonA.n[0];
-	for ( int i = 1; i < localA.count; ++i )
+#pragma unroll
+	for ( int i = 1; i < 8; ++i )
 	{
-		localA.v[i] = b2Sub( polygonA.v[i], origin );
+		localA.v[i] = i < localA.count ? b2Sub( polygonA.v[i], origin ) : v2( 0.0f, 0.0f );
 		localA.n[i] = polygonA.n[i];
 	}
 	Poly localB;
 	localB.count = polygonB.count;
 	localB.radius = polygonB.radius;
-	for ( int i = 0; i < localB.count; ++i )
+#pragma unroll
+	for ( int i = 0; i < 8; ++i )
 	{
-		localB.v[i] = b2TransformPoint( xf, polygonB.v[i] );
-		localB.n[i] = b2RotateVector( xf.q, polygonB.n[i] );
+		localB.v[i] = i < localB.count ? b2TransformPoint( xf, polygonB.v[i] ) : v2( 0.0f, 0.0f );
+		localB.n[i] = i < localB.count ? b2RotateVector( xf.q, polygonB.n[i] ) : v2( 0.0f, 0.0f );
 	}
 
 	int edgeA = 0;
@@ -429,13 +472,14 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 	if ( separationA >= separationB )
 	{
 		flip = false;
-		b2Vec2 searchDirection = localA.n[edgeA];
+		b2Vec2 searchDirection = pick( localA.n, edgeA );
 		edgeB = 0;
 		float minDot = FLT_MAX;
-		for ( int i = 0; i < localB.count; ++i )
+#pragma unroll
+		for ( int i = 0; i < 8; ++i )
 		{
 			float dot = b2Dot( searchDirection, localB.n[i] );
-			if ( dot < minDot )
+			if ( i < localB.count && dot < minDot )
 			{
 				minDot = dot;
 				edgeB = i;
@@ -445,13 +489,14 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 	else
 	{
 		flip = true;
-		b2Vec2 searchDirection = localB.n[edgeB];
+		b2Vec2 searchDirection = pick( localB.n, edgeB );
 		edgeA = 0;
 		float minDot = FLT_MAX;
-		for ( int i = 0; i < localA.count; ++i )
+#pragma unroll
+		for ( int i = 0; i < 8; ++i )
 		{
 			float dot = b2Dot( searchDirection, localA.n[i] );
-			if ( dot < minDot )
+			if ( i < localA.count && dot < minDot )
 			{
 				minDot = dot;
 				edgeA = i;
@@ -465,8 +510,8 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 		int i12 = edgeA + 1 < localA.count ? edgeA + 1 : 0;
 		int i21 = edgeB;
 		int i22 = edgeB + 1 < localB.count ? edgeB + 1 : 0;
-		b2Vec2 v11 = localA.v[i11], v12 = localA.v[i12];
-		b2Vec2 v21 = localB.v[i21], v22 = localB.v[i22];
+		b2Vec2 v11 = pick( localA.v, i11 ), v12 = pick( localA.v, i12 );
+		b2Vec2 v21 = pick( localB.v, i21 ), v22 = pick( localB.v, i22 );
 		float f1, f2, distSq;
 		segment_distance( v11, v12, v21, v22, &f1, &f2, &distSq );
 		float rA = localA.radius, rB = localB.radius;
@@ -503,8 +548,11 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 	if ( m.pointCount > 0 )
 	{
 		m.normal = b2RotateVector( xfA.q, m.normal );
-		for ( int i = 0; i < m.pointCount; ++i )
+#pragma unroll
+		for ( int i = 0; i < 2; ++i )
 		{
+			if ( i >= m.pointCount )
+				break;
 			m.anchorA[i] = b2RotateVector( xfA.q, b2Add( m.anchorA[i], origin ) );
 			m.anchorB[i] = b2Add( m.anchorA[i], b2Sub( xfA.p, xfB.p ) );
 			m.point[i] = b2Add( xfA.p, m.anchorA[i] );
@@ -516,10 +564,11 @@ B2MI_HD inline void load_poly( Poly& p, const DevShape& s )
 {
 	p.count = s.count;
 	p.radius = s.radius;
-	for ( int i = 0; i < s.count; ++i )
+#pragma unroll
+	for ( int i = 0; i < 8; ++i )
 	{
-		p.v[i] = v2( s.v[i][0], s.v[i][1] );
-		p.n[i] = v2( s.n[i][0], s.n[i][1] );
+		p.v[i] = i < s.count ? v2( s.v[i][0], s.v[i][1] ) : v2( 0.0f, 0.0f );
+		p.n[i] = i < s.count ? v2( s.n[i][0], s.n[i][1] ) : v2( 0.0f, 0.0f );
 	}
 }
 

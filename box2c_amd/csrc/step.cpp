@@ -869,9 +869,22 @@ static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
 		}
 }
 
+static float msSince( std::chrono::steady_clock::time_point t )
+{
+	return std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t ).count();
+}
+
 void StepGroup::step( float timeStep, int subStepCount )
 {
 	auto t0 = std::chrono::steady_clock::now();
+	// host phases of this step (ms), reported through b2World_GetProfile:
+	//   pairs          pair update and contact creation
+	//   broadphase     dynamic-tree rebuild (overlaps the narrowphase kernel)
+	//   collide        narrowphase kernel (device events)
+	//   buildIslands   begin/end-touch processing and work-item rebuild
+	//   solve          solver kernel (device events)
+	//   finalizeBodies enlarged-proxy processing
+	float msBroad = 0.0f, msChanges = 0.0f, msEnlarge = 0.0f;
 	hipStream_t s = stream();
 	for ( World* w : worlds )
 	{
@@ -938,23 +951,25 @@ void StepGroup::step( float timeStep, int subStepCount )
 	}
 	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
 	// events bracket each kernel launch alone (the stream is idle while the host works)
-	hipEvent_t e0, e1, e2, e3;
-	HIP_CHECK( hipEventCreate( &e0 ) );
-	HIP_CHECK( hipEventCreate( &e1 ) );
-	HIP_CHECK( hipEventCreate( &e2 ) );
-	HIP_CHECK( hipEventCreate( &e3 ) );
+	if ( events[0] == nullptr )
+		for ( hipEvent_t& e : events )
+			HIP_CHECK( hipEventCreate( &e ) );
+	hipEvent_t e0 = events[0], e1 = events[1], e2 = events[2], e3 = events[3];
 	HIP_CHECK( hipEventRecord( e0, s ) );
 	HIP_CHECK( launch_collide( dWorlds, dCollideList, collideCount, dChanges, dCounters, s ) );
 	HIP_CHECK( hipEventRecord( e1, s ) );
 	HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ), hipMemcpyDeviceToHost, s ) );
 
 	// the tree rebuild overlaps the narrowphase (src/world.c:367-379, 420)
+	auto tb = std::chrono::steady_clock::now();
 	for ( World* w : worlds )
 	{
 		w->trees[b2_dynamicBody].rebuild( false );
 		w->trees[b2_kinematicBody].rebuild( false );
 	}
+	msBroad = msSince( tb );
 	HIP_CHECK( hipStreamSynchronize( s ) );
+	auto tc = std::chrono::steady_clock::now();
 	int changeCount = hCounters[0];
 	if ( changeCount > 0 )
 	{
@@ -1004,6 +1019,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 		buildWorkItems( this );
 		collideListDirty = true;
 	}
+	msChanges = msSince( tc );
 	int totalShapes = 0;
 	for ( World* w : worlds )
 		totalShapes += (int)w->shapes.size();
@@ -1024,6 +1040,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 	HIP_CHECK( hipStreamSynchronize( s ) );
 	int enlargeCount = hCounters[1];
 	B2MI_CHECK( enlargeCount <= capEnlarge );
+	auto te = std::chrono::steady_clock::now();
 	if ( enlargeCount > 0 )
 	{
 		hEnlarge.resize( enlargeCount );
@@ -1041,12 +1058,9 @@ void StepGroup::step( float timeStep, int subStepCount )
 			begin = end;
 		}
 	}
+	msEnlarge = msSince( te );
 	HIP_CHECK( hipEventElapsedTime( &msCollide, e0, e1 ) );
 	HIP_CHECK( hipEventElapsedTime( &msSolve, e3, e2 ) );
-	HIP_CHECK( hipEventDestroy( e0 ) );
-	HIP_CHECK( hipEventDestroy( e1 ) );
-	HIP_CHECK( hipEventDestroy( e2 ) );
-	HIP_CHECK( hipEventDestroy( e3 ) );
 	float msTotal = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t0 ).count();
 	msHost = msTotal - msCollide - msSolve;
 	for ( World* w : worlds )
@@ -1057,6 +1071,9 @@ void StepGroup::step( float timeStep, int subStepCount )
 		w->profile.step = msTotal;
 		w->profile.collide = msCollide;
 		w->profile.solve = msSolve;
+		w->profile.broadphase = msBroad;
+		w->profile.buildIslands = msChanges;
+		w->profile.finalizeBodies = msEnlarge;
 	}
 }
 

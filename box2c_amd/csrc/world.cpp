@@ -132,6 +132,9 @@ StepGroup::~StepGroup()
 	devFree( dSlotBodies );
 	devFree( dCons );
 	devFree( dPriv );
+	for ( hipEvent_t e : events )
+		if ( e )
+			(void)hipEventDestroy( e );
 }
 
 static World* createWorld( const b2WorldDef* def, int firstSlot, int endSlot )

@@ -275,6 +275,7 @@ struct StepGroup
 
 	// timing of the last step (ms)
 	float msCollide = 0, msSolve = 0, msHost = 0;
+	hipEvent_t events[4] = { nullptr, nullptr, nullptr, nullptr };
 
 	~StepGroup();
 	void step( float timeStep, int subStepCount );
