@@ -66,10 +66,21 @@ __global__ void k_init_contacts( const DevWorld* __restrict__ worlds, const Cont
 }
 
 // src/world.c:304-362 + src/contact.c:531-581 for one contact
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8). Remapping
+// block b to logical block xcd_block(b) gives each XCD one contiguous eighth of the list, so
+// the shapes and bodies a run of neighbouring contacts shares stay in that XCD's L2.
+__device__ inline int xcd_block( int b, int nblocks )
+{
+	const int kXcds = 8;
+	int q = nblocks / kXcds, r = nblocks % kXcds;
+	int x = b % kXcds, pos = b / kXcds;
+	return x * q + ( x < r ? x : r ) + pos;
+}
+
 __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __restrict__ list, int count,
 						   ChangeRecord* __restrict__ changes, int* __restrict__ changeCount )
 {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	int i = xcd_block( blockIdx.x, gridDim.x ) * blockDim.x + threadIdx.x;
 	if ( i >= count )
 		return;
 	int2 item = list[i];

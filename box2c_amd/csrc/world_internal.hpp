@@ -24,6 +24,8 @@ namespace b2mi
 {
 
 constexpr int kColorCount = 12;
+constexpr int kStaticSet = 0; // solver set indices (src/solver_set.h:12-19)
+constexpr int kAwakeSet = 2;
 constexpr int kOverflow = 11;
 constexpr int kMaxPlainWorlds = 128; // src/core.h:129
 constexpr int kMaxWorldSlots = 65535;
