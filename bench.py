@@ -69,6 +69,70 @@ def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
     }
 
 
+def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7):
+    """4096 (default) independent worlds, each one pyramid of base U{4..40} on its own ground box,
+    advanced together: two kernel launches per step for all worlds."""
+    d = api.default_world_def(enableSleep=False)
+    batch = api.create_batch()
+    api.use_batch(batch)
+    n_bodies = 0
+    try:
+        for i in range(n_worlds):
+            _, ids = api.scene("batch_pyramid", d, seed=seed, index=i)
+            n_bodies += len(ids)
+    finally:
+        api.use_batch(None)
+    api.batch_step(batch, n=warmup)
+    t_solve = t_collide = t_host = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        api.lib.b2mi_Batch_Step(batch, 1.0 / 60.0, 4)
+        c, s, h = api.batch_timing(batch)
+        t_collide += c
+        t_solve += s
+        t_host += h
+    elapsed = time.perf_counter() - t0
+    items = api.lib.b2mi_Batch_GetWorkItemCount(batch)
+    api.lib.b2mi_DestroyBatch(batch)
+    return {"bodies": n_bodies, "elapsed": elapsed, "ms_solve": t_solve / steps, "ms_collide": t_collide / steps,
+            "ms_host": t_host / steps, "work_items": items}
+
+
+def cpu_baseline_batch(threads: int, n_worlds: int, sample_steps: int, warmup: int, seed: int = 7):
+    """The reference on the same kind of worlds with world-level parallelism (each thread steps
+    its own worlds, the reference's best schedule for many small worlds). The reference holds at
+    most 128 worlds at once, so the sample is one wave: the first `n_worlds` (<= 128) worlds of
+    the batch."""
+    import ctypes
+    from box2c_amd.capi import B2Api, WorldId
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
+    if not (os.path.exists(lib) and os.path.exists(scenes)):
+        return None
+    ref = B2Api(lib, scenes)
+    f = ref.lib.ref_step_worlds_parallel
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+    f.restype = ctypes.c_double
+    d = ref.default_world_def(enableSleep=False)
+    made = []
+    for i in range(n_worlds):
+        w, ids = ref.scene("batch_pyramid", d, seed=seed, index=i)
+        made.append((len(ids), w))
+    made.sort(key=lambda t: -t[0])  # largest first: threads take worlds from a shared counter
+    worlds = (WorldId * n_worlds)(*[w for _, w in made])
+    n_bodies = sum(n for n, _ in made)
+    f(worlds, n_worlds, threads, warmup, 1.0 / 60.0, 4)
+    elapsed = f(worlds, n_worlds, threads, sample_steps, 1.0 / 60.0, 4)
+    for i in range(n_worlds):
+        ref.destroy(worlds[i])
+    return {
+        "value": n_bodies * 4 * sample_steps / elapsed, "unit": "bodies×substeps/s", "cores": threads,
+        "kind": "reference",
+        "sample": f"{n_worlds} batch-pyramid worlds ({n_bodies} boxes, the first wave of the batch), {sample_steps} steps "
+                  f"after {warmup} warmup, worlds taken largest first by {threads} threads, {elapsed:.1f} s",
+    }
+
+
 def cpu_baseline(rows: int, cols: int, threads: int, sample_steps: int, warmup: int = 20):
     """The reference stepping the same scene with `threads` workers, timed over a bounded
     number of steps after the same warmup as the GPU leg (bench.py's CPU leg; the oracle directory holds the
@@ -121,6 +185,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-worlds", type=int, default=4096, help="worlds of the batched leg (0 = skip it)")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-cpu-steps", type=int, default=1000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain copy kernel of known size (for HBM counter scaling)")
     args = ap.parse_args()
@@ -183,6 +250,22 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.rows, args.cols, args.cpu_threads, args.cpu_steps, args.warmup)
+    if args.batch_worlds > 0:
+        rb = run_gpu_batch(api, args.batch_worlds, args.batch_steps, args.warmup)
+        bval = rb["bodies"] * substeps * args.batch_steps / rb["elapsed"]
+        batched = {
+            "workload": f"batch_pyramids_{args.batch_worlds}_base4-40_plain_f32", "value": bval,
+            "unit": "bodies×substeps/s", "bodies": rb["bodies"], "worlds": args.batch_worlds,
+            "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps,
+            "ms_solve_kernel": rb["ms_solve"], "ms_collide_kernel": rb["ms_collide"], "ms_host": rb["ms_host"],
+            "work_items": rb["work_items"], "cpu_baseline": None,
+        }
+        if not args.no_cpu:
+            cb = cpu_baseline_batch(args.cpu_threads, min(128, args.batch_worlds), args.batch_cpu_steps, args.warmup)
+            batched["cpu_baseline"] = cb
+            if cb:
+                batched["vs_cpu"] = bval / cb["value"]
+        out["batched"] = batched
     print(json.dumps(out), flush=True)
 
 

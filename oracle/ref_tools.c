@@ -23,6 +23,7 @@
 #include "box2d/box2d.h"
 
 #include <pthread.h>
+#include <time.h>
 #include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
@@ -308,4 +309,57 @@ void ref_pool_apply( b2WorldDef* def )
 		def->finishTask = ref_pool_finish;
 		def->userTaskContext = NULL;
 	}
+}
+
+// ---- world-level parallelism: independent worlds stepped concurrently ----
+//
+// `threads` threads take worlds from a shared counter (pass them largest first for a good
+// balance) and step each taken world `steps` times with the reference's default
+// single-threaded task system. Worlds share no mutable state, so this is the reference's best
+// CPU schedule for many small worlds.
+typedef struct WorldsJob
+{
+	const b2WorldId* worlds;
+	int count, steps, subSteps;
+	float timeStep;
+	atomic_int next;
+} WorldsJob;
+
+static void* worlds_thread( void* p )
+{
+	WorldsJob* job = (WorldsJob*)p;
+	for ( ;; )
+	{
+		int i = atomic_fetch_add( &job->next, 1 );
+		if ( i >= job->count )
+			break;
+		for ( int s = 0; s < job->steps; ++s )
+			b2World_Step( job->worlds[i], job->timeStep, job->subSteps );
+	}
+	return NULL;
+}
+
+// Returns the wall-clock seconds of the whole run.
+double ref_step_worlds_parallel( const b2WorldId* worlds, int count, int threads, int steps, float timeStep, int subSteps )
+{
+	if ( threads < 1 )
+		threads = 1;
+	if ( threads > POOL_MAX_THREADS )
+		threads = POOL_MAX_THREADS;
+	pthread_t tid[POOL_MAX_THREADS];
+	WorldsJob job;
+	job.worlds = worlds;
+	job.count = count;
+	job.steps = steps;
+	job.subSteps = subSteps;
+	job.timeStep = timeStep;
+	atomic_init( &job.next, 0 );
+	struct timespec t0, t1;
+	clock_gettime( CLOCK_MONOTONIC, &t0 );
+	for ( int t = 0; t < threads; ++t )
+		pthread_create( tid + t, NULL, worlds_thread, &job );
+	for ( int t = 0; t < threads; ++t )
+		pthread_join( tid[t], NULL );
+	clock_gettime( CLOCK_MONOTONIC, &t1 );
+	return (double)( t1.tv_sec - t0.tv_sec ) + 1e-9 * (double)( t1.tv_nsec - t0.tv_nsec );
 }
