@@ -17,6 +17,7 @@ Printed JSON line: metric/value/unit per the benchmark contract, plus
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -69,7 +70,7 @@ def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
     }
 
 
-def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7):
+def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, autotune: bool = True):
     """4096 (default) independent worlds, each one pyramid of base U{4..40} on its own ground box,
     advanced together: two kernel launches per step for all worlds."""
     d = api.default_world_def(enableSleep=False)
@@ -83,6 +84,12 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7):
     finally:
         api.use_batch(None)
     api.batch_step(batch, n=warmup)
+    shape = None
+    if autotune:
+        best = api.lib.b2mi_Batch_Autotune(batch, 1.0 / 60.0, 4, 2)
+        b, t, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        api.lib.b2mi_Batch_GetLaunchShape(batch, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l))
+        shape = {"block": b.value, "target_slots": t.value, "lds_budget": l.value, "tuned_solve_ms": best}
     t_solve = t_collide = t_host = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -95,7 +102,7 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7):
     items = api.lib.b2mi_Batch_GetWorkItemCount(batch)
     api.lib.b2mi_DestroyBatch(batch)
     return {"bodies": n_bodies, "elapsed": elapsed, "ms_solve": t_solve / steps, "ms_collide": t_collide / steps,
-            "ms_host": t_host / steps, "work_items": items}
+            "ms_host": t_host / steps, "work_items": items, "launch_shape": shape}
 
 
 def cpu_baseline_batch(threads: int, n_worlds: int, sample_steps: int, warmup: int, seed: int = 7):
@@ -258,7 +265,7 @@ def main():
             "unit": "bodies×substeps/s", "bodies": rb["bodies"], "worlds": args.batch_worlds,
             "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps,
             "ms_solve_kernel": rb["ms_solve"], "ms_collide_kernel": rb["ms_collide"], "ms_host": rb["ms_host"],
-            "work_items": rb["work_items"], "cpu_baseline": None,
+            "work_items": rb["work_items"], "launch_shape": rb["launch_shape"], "cpu_baseline": None,
         }
         if not args.no_cpu:
             cb = cpu_baseline_batch(args.cpu_threads, min(128, args.batch_worlds), args.batch_cpu_steps, args.warmup)

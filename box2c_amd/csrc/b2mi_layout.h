@@ -225,8 +225,10 @@ enum ConstraintField : int
 };
 
 // Bytes of LDS a work item's solver state needs: per body two float4, per slot the two local
-// body indices and the constraint fields. Items above the budget run from HBM.
-constexpr int kMaxItemLdsBytes = 64 * 1024;
+// body indices and the constraint fields. Items above the budget run from HBM. A CDNA4 CU has
+// 160 KiB of LDS; the default budget leaves room for two or more items per CU.
+constexpr int kMaxItemLdsBytes = 160 * 1024;
+constexpr int kDefaultItemLdsBytes = 64 * 1024;
 // threads per solver workgroup: at most this many (the kernel's launch bound)
 constexpr int kSolveMaxBlock = 256;
 inline int itemLdsBytes( int bodyCount, int slotCount )

@@ -17,6 +17,68 @@ static StepGroup* g_batches[kMaxBatches] = {};
 
 using namespace b2mi;
 
+// The single-world counterpart of a batch: a world's own one-world step group.
+static StepGroup* worldGroup( b2WorldId worldId )
+{
+	World* w = getWorld( worldId );
+	B2MI_CHECK( w->batchWorld == false );
+	if ( w->group == nullptr )
+	{
+		w->group = new StepGroup();
+		w->group->worlds.push_back( w );
+		w->groupIndex = 0;
+	}
+	return w->group;
+}
+
+static void setShape( StepGroup* g, int blockSize, int targetSlots, int ldsBudget )
+{
+	g->blockSize = blockSize;
+	g->targetSlots = targetSlots;
+	g->ldsBudget = ldsBudget;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+}
+
+// Autotuner of the solver launch shape. Every shape gives bit-identical results, so the trials
+// are ordinary steps of the simulation: each candidate (threads per workgroup x work-item
+// packing target x LDS budget) runs one step to rebuild its work items and then `trialSteps`
+// measured steps; the candidate with the lowest solver-kernel time (device events) is kept.
+static float autotune( StepGroup* g, float timeStep, int subStepCount, int trialSteps )
+{
+	static const int blocks[] = { 0, 128, 256 };
+	static const int targets[] = { 128, 256, 512 };
+	static const int budgets[] = { 0, 64 * 1024, 96 * 1024, 160 * 1024 };
+	float best = 1e30f;
+	int bestBlock = 0, bestTarget = 0, bestBudget = -1;
+	trialSteps = trialSteps < 1 ? 1 : trialSteps;
+	for ( int budget : budgets )
+		for ( int target : targets )
+			for ( int block : blocks )
+			{
+				setShape( g, block, target, budget );
+				g->step( timeStep, subStepCount );
+				float t = 1e30f;
+				for ( int k = 0; k < trialSteps; ++k )
+				{
+					g->step( timeStep, subStepCount );
+					t = g->msSolve < t ? g->msSolve : t;
+				}
+				if ( t < best )
+				{
+					best = t;
+					bestBlock = block;
+					bestTarget = target;
+					bestBudget = budget;
+				}
+			}
+	setShape( g, bestBlock, bestTarget, bestBudget );
+	g->tunedBlock = bestBlock;
+	g->tunedTarget = bestTarget;
+	g->tunedBudget = bestBudget;
+	return best;
+}
+
 static StepGroup* getBatch( int batch )
 {
 	B2MI_CHECK( 1 <= batch && batch <= kMaxBatches && g_batches[batch - 1] != nullptr );
@@ -114,31 +176,38 @@ B2_API int b2mi_Batch_GetWorkItemCount( int batch )
 // the same solver settings for a single world (its own one-world group)
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots )
 {
-	World* w = getWorld( worldId );
-	B2MI_CHECK( w->batchWorld == false );
-	if ( w->group == nullptr )
-	{
-		w->group = new StepGroup();
-		w->group->worlds.push_back( w );
-		w->groupIndex = 0;
-	}
-	w->group->blockSize = blockSize;
-	w->group->targetSlots = targetSlots;
-	w->topologyDirty = true;
+	StepGroup* g = worldGroup( worldId );
+	setShape( g, blockSize, targetSlots, g->ldsBudget );
 }
 
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes )
 {
-	World* w = getWorld( worldId );
-	B2MI_CHECK( w->batchWorld == false );
-	if ( w->group == nullptr )
-	{
-		w->group = new StepGroup();
-		w->group->worlds.push_back( w );
-		w->groupIndex = 0;
-	}
-	w->group->ldsBudget = maxBytes;
-	w->topologyDirty = true;
+	StepGroup* g = worldGroup( worldId );
+	setShape( g, g->blockSize, g->targetSlots, maxBytes );
+}
+
+B2_API float b2mi_World_Autotune( b2WorldId worldId, float timeStep, int subStepCount, int trialSteps )
+{
+	StepGroup* g = worldGroup( worldId );
+	B2MI_CHECK( timeStep > 0.0f && subStepCount > 0 );
+	return autotune( g, timeStep, subStepCount, trialSteps );
+}
+
+B2_API float b2mi_Batch_Autotune( int batch, float timeStep, int subStepCount, int trialSteps )
+{
+	StepGroup* g = getBatch( batch );
+	B2MI_CHECK( timeStep > 0.0f && subStepCount > 0 );
+	if ( g->worlds.empty() )
+		return 0.0f;
+	return autotune( g, timeStep, subStepCount, trialSteps );
+}
+
+B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget )
+{
+	StepGroup* g = getBatch( batch );
+	*blockSize = g->blockSize;
+	*targetSlots = g->targetSlots;
+	*ldsBudget = g->ldsBudget;
 }
 
 B2_API void b2mi_World_GetTiming( b2WorldId worldId, float* msCollide, float* msSolve, float* msHost )

@@ -1079,6 +1079,15 @@ hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int item
 		return hipSuccess;
 	if ( ldsBytes < 0 || ldsBytes > kMaxItemLdsBytes || blockSize < 64 || blockSize > kSolveMaxBlock )
 		return hipErrorInvalidValue;
+	static int allowedLds = 0; // dynamic LDS above 64 KiB must be enabled per kernel
+	if ( ldsBytes > 64 * 1024 && allowedLds < kMaxItemLdsBytes )
+	{
+		hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( &k_solve ),
+											hipFuncAttributeMaxDynamicSharedMemorySize, kMaxItemLdsBytes );
+		if ( e != hipSuccess )
+			return e;
+		allowedLds = kMaxItemLdsBytes;
+	}
 	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
 						slotBodies, cf, cfCapacity, priv, enl, enlCount, enlCapacity );
 	return hipGetLastError();

@@ -644,7 +644,7 @@ static void buildWorkItems( StepGroup* g )
 			wi.groupStart[kGroups] = (int)g->hSlotContact.size() - wi.slotStart;
 			wi.bodyCount = (int)g->hBodyList.size() - wi.bodyStart;
 			{
-				int budget = g->ldsBudget < 0 ? kMaxItemLdsBytes : std::min( g->ldsBudget, kMaxItemLdsBytes );
+				int budget = g->ldsBudget < 0 ? kDefaultItemLdsBytes : std::min( g->ldsBudget, kMaxItemLdsBytes );
 				int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
 				wi.ldsBytes = bytes <= budget ? bytes : 0;
 				g->maxItemLds = std::max( g->maxItemLds, wi.ldsBytes );
@@ -655,6 +655,13 @@ static void buildWorkItems( StepGroup* g )
 		}
 		w->topologyDirty = false;
 	}
+	// Longest items first: the hardware dispatches workgroups in order, so the few large islands
+	// of an uneven batch start at once and the small ones fill the CUs behind them. Items are
+	// independent; their order does not change any result.
+	std::stable_sort( g->hItems.begin(), g->hItems.end(), []( const WorkItem& a, const WorkItem& b ) {
+		int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
+		return ca > cb;
+	} );
 	g->itemCount = (int)g->hItems.size();
 	g->slotCount = (int)g->hSlotContact.size();
 	g->bodyListCount = (int)g->hBodyList.size();

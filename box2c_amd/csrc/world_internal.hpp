@@ -230,7 +230,8 @@ struct StepGroup
 	std::vector<World*> worlds;
 	int blockSize = 0;		  // 0 = automatic
 	int targetSlots = 0;	  // work-item packing target (constraint slots), 0 = automatic
-	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kMaxItemLdsBytes, 0 = HBM only)
+	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kDefaultItemLdsBytes, 0 = HBM only)
+	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
 	bool structureDirty = true;
 
 	// launch buffers (device)

@@ -70,11 +70,20 @@ B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots );
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots );
 B2_API int b2mi_Batch_GetWorkItemCount( int batch );
 
-// Largest LDS footprint (bytes) of a work item whose solver state is kept in LDS; larger work
-// items keep it in HBM. -1 = the 64 KiB default, 0 = HBM for every work item. The results are
-// bit-identical either way; only the speed differs.
+// Largest LDS footprint (bytes, at most 160 KiB) of a work item whose solver state is kept in
+// LDS; larger work items keep it in HBM. -1 = the 64 KiB default, 0 = HBM for every work item.
+// The results are bit-identical either way; only the speed differs.
 B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes );
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
+
+// Autotuner: steps the batch (or world) under each candidate launch shape (threads per
+// workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
+// rebuild step, and keeps the shape with the fastest solver kernel. Every shape gives
+// bit-identical results, so these are ordinary simulation steps (3 x 3 x 4 candidates, i.e.
+// 36 x (1 + trialSteps) steps). Returns the best solver-kernel time in milliseconds.
+B2_API float b2mi_Batch_Autotune( int batch, float timeStep, int subStepCount, int trialSteps );
+B2_API float b2mi_World_Autotune( b2WorldId worldId, float timeStep, int subStepCount, int trialSteps );
+B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget );
 
 // Device time of the last step's narrowphase and solver launches and the remaining host
 // time, in milliseconds.

@@ -113,3 +113,61 @@ def test_step_parity_solver_storage(name, kw, overrides, steps, substeps, budget
     finally:
         gpu.destroy(wg)
         cpu.destroy(wc)
+
+
+def _batch_worlds(gpu, cpu, n, seed):
+    batch = gpu.create_batch()
+    gpu.use_batch(batch)
+    worlds_g, worlds_c = [], []
+    try:
+        for i in range(n):
+            worlds_g.append(gpu.scene("batch_pyramid", seed=seed, index=i)[0])
+            worlds_c.append(cpu.scene("batch_pyramid", seed=seed, index=i)[0])
+    finally:
+        gpu.use_batch(None)
+    return batch, worlds_g, worlds_c
+
+
+def _compare_worlds(gpu, cpu, worlds_g, worlds_c, label):
+    for i, (wg, wc) in enumerate(zip(worlds_g, worlds_c)):
+        m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+        assert m is None, f"{label} world {i}: {m}"
+        m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+        assert m is None, f"{label} world {i}: {m}"
+
+
+def test_batch_large_lds_budget_parity():
+    """Items up to 160 KiB of LDS (the kernel's dynamic-LDS attribute above 64 KiB)."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 12, 21)
+    try:
+        gpu.lib.b2mi_Batch_SetLdsBudget(batch, 160 * 1024)
+        for s in range(30):
+            gpu.batch_step(batch)
+            for wc in worlds_c:
+                cpu.step(wc)
+        _compare_worlds(gpu, cpu, worlds_g, worlds_c, "lds160k")
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
+
+
+def test_batch_autotune_keeps_parity():
+    """The autotuner's trial steps are ordinary steps: after tuning (36 candidate shapes, one
+    rebuild step and one measured step each) and more steps under the chosen shape, every world
+    still matches the oracle bit for bit."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 10, 5)
+    try:
+        gpu.batch_step(batch, n=3)
+        best = gpu.lib.b2mi_Batch_Autotune(batch, 1.0 / 60.0, 4, 1)
+        assert best > 0.0
+        gpu.batch_step(batch, n=5)
+        for wc in worlds_c:
+            cpu.step(wc, n=3 + 36 * 2 + 5)
+        _compare_worlds(gpu, cpu, worlds_g, worlds_c, "autotune")
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
