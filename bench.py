@@ -90,6 +90,7 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, au
         b, t, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         api.lib.b2mi_Batch_GetLaunchShape(batch, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l))
         shape = {"block": b.value, "target_slots": t.value, "lds_budget": l.value, "tuned_solve_ms": best}
+        api.batch_step(batch)  # rebuilds the work items under the chosen shape
     t_solve = t_collide = t_host = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):

@@ -153,8 +153,16 @@ struct WorkItem
 	int ownedCount; // the first ownedCount bodies are finalized and written back
 	int slotStart;	// into the launch's constraint slots
 	int groupStart[kGroups + 1]; // relative to slotStart
-	int ldsBytes;				 // > 0: the item's solver state lives in this much LDS; 0: in HBM
-	int pad;
+	int ldsBytes;				 // LDS bytes of the item's store (0 for kStoreHbm)
+	int store;					 // kStoreHbm, kStoreBodiesLds or kStoreLds
+};
+
+// Where a work item keeps its solver state during the step
+enum ItemStore : int
+{
+	kStoreHbm = 0,		 // bodies and constraints in HBM
+	kStoreBodiesLds = 1, // bodies in LDS, constraints in HBM
+	kStoreLds = 2,		 // bodies and constraints in LDS
 };
 
 // narrowphase state change (sorted by world, contactId on the host)
@@ -234,6 +242,10 @@ constexpr int kSolveMaxBlock = 256;
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {
 	return 32 * bodyCount + 8 * slotCount + 4 * cfFloatCount * slotCount;
+}
+inline int itemBodyLdsBytes( int bodyCount )
+{
+	return 32 * bodyCount;
 }
 
 } // namespace b2mi

@@ -272,6 +272,34 @@ struct GlobalStore
 	}
 };
 
+// Bodies in LDS, constraints in HBM: for items whose constraints exceed the LDS budget. The
+// bodies are the randomly accessed part (two per slot per phase), the constraint fields are
+// read and written with consecutive lanes on consecutive slots.
+struct BodiesLdsStore
+{
+	float* f;
+	int cap;
+	int base;
+	const int2* sb;
+	float4* P;
+	__device__ inline float& at( int field, int k ) const
+	{
+		return f[(size_t)field * cap + base + k];
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return sb[k];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return P[2 * i + 1];
+	}
+};
+
 struct LdsStore
 {
 	float* f;
@@ -1008,8 +1036,9 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	}
 }
 
-// One workgroup per work item. Items with ldsBytes > 0 keep their state in the dynamic LDS
-// block (layout: body state, slot bodies, constraint fields); the others in HBM.
+// One workgroup per work item. kStoreLds items keep their state in the dynamic LDS block
+// (layout: body state, slot bodies, constraint fields), kStoreBodiesLds items only the body
+// state, kStoreHbm items nothing.
 __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
@@ -1022,7 +1051,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	const int* bl = bodyList + wi.bodyStart;
 	const int* sc = slotContact + wi.slotStart;
 	const int total = wi.groupStart[kGroups];
-	if ( wi.ldsBytes > 0 )
+	if ( wi.store == kStoreLds )
 	{
 		LdsStore C;
 		C.P = lds;
@@ -1032,6 +1061,16 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
+		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
+	}
+	else if ( wi.store == kStoreBodiesLds )
+	{
+		BodiesLdsStore C;
+		C.f = cfBase;
+		C.cap = cfCapacity;
+		C.base = wi.slotStart;
+		C.sb = slotBodies + wi.slotStart;
+		C.P = lds;
 		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
 	}
 	else

@@ -646,7 +646,22 @@ static void buildWorkItems( StepGroup* g )
 			{
 				int budget = g->ldsBudget < 0 ? kDefaultItemLdsBytes : std::min( g->ldsBudget, kMaxItemLdsBytes );
 				int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
-				wi.ldsBytes = bytes <= budget ? bytes : 0;
+				int bodyBytes = itemBodyLdsBytes( wi.bodyCount );
+				if ( bytes <= budget )
+				{
+					wi.store = kStoreLds;
+					wi.ldsBytes = bytes;
+				}
+				else if ( budget > 0 && bodyBytes <= budget )
+				{
+					wi.store = kStoreBodiesLds;
+					wi.ldsBytes = bodyBytes;
+				}
+				else
+				{
+					wi.store = kStoreHbm;
+					wi.ldsBytes = 0;
+				}
 				g->maxItemLds = std::max( g->maxItemLds, wi.ldsBytes );
 			}
 			g->maxGroupSize = std::max( g->maxGroupSize, wi.ownedCount );
@@ -676,8 +691,10 @@ static void buildWorkItems( StepGroup* g )
 		for ( int i = 0; i < wi.bodyCount; ++i )
 			B2MI_CHECK( 0 <= g->hBodyList[wi.bodyStart + i] && g->hBodyList[wi.bodyStart + i] < na );
 		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= g->slotCount );
-		B2MI_CHECK( wi.ldsBytes == 0 || ( wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) &&
-										  wi.ldsBytes <= g->maxItemLds && g->maxItemLds <= kMaxItemLdsBytes ) );
+		B2MI_CHECK( g->maxItemLds <= kMaxItemLdsBytes && wi.ldsBytes <= g->maxItemLds );
+		B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
+					( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount ) ) ||
+					( wi.store == kStoreLds && wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) );
 		for ( int k = 0; k < kGroups; ++k )
 			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
 		for ( int k = 0; k < wi.groupStart[kGroups]; ++k )

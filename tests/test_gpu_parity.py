@@ -94,10 +94,11 @@ def test_hello_world_rests_on_ground():
 
 @pytest.mark.parametrize("name,kw,overrides,steps,substeps", helpers.PARITY_SCENES[:4] + helpers.PARITY_SCENES[6:],
                          ids=[s[0] for s in helpers.PARITY_SCENES[:4] + helpers.PARITY_SCENES[6:]])
-@pytest.mark.parametrize("budget", [0, 24 * 1024], ids=["hbm", "lds24k"])
+@pytest.mark.parametrize("budget", [0, 8 * 1024, 24 * 1024], ids=["hbm", "lds8k", "lds24k"])
 def test_step_parity_solver_storage(name, kw, overrides, steps, substeps, budget):
-    """The solver keeps a work item's state in LDS when it fits the budget and in HBM
-    otherwise; both give the oracle's bits (budget 0 forces HBM for every item)."""
+    """The solver keeps a work item's state in LDS when it fits the budget, only its bodies in LDS
+    when those fit, and everything in HBM otherwise; all give the oracle's bits (budget 0 forces
+    HBM for every item, 8 KiB makes the larger items keep only their bodies in LDS)."""
     gpu, cpu = helpers.product(), helpers.oracle()
     wg, _ = helpers.make_world(gpu, name, kw, overrides)
     wc, _ = helpers.make_world(cpu, name, kw, overrides)
