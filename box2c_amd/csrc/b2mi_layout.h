@@ -88,7 +88,11 @@ struct DevContact
 	int bodySimIndexA, bodySimIndexB;
 	float invMassA, invIA, invMassB, invIB;
 	float friction, restitution;
-	int pad0, pad1;
+	// For each new manifold point (2 bits each), the index of the previous manifold's point whose
+	// impulses it inherited as warm start, or 3 for none; the adjoint routes impulse gradients
+	// back through it.
+	uint32_t warmFrom;
+	int pad1;
 };
 
 struct DevShape
@@ -230,6 +234,13 @@ enum ConstraintField : int
 	cfTangentMass,
 	cfPointFields = 11,
 	cfFloatCount = cfAnchorAX + 2 * 11,
+};
+
+// Pose of a body at a step boundary (taped runs): center of mass, rotation, velocities
+struct StepPose
+{
+	float cx, cy, qc, qs;
+	float vx, vy, w, pad;
 };
 
 // Bytes of LDS a work item's solver state needs: per body two float4, per slot the two local

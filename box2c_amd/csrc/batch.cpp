@@ -79,6 +79,16 @@ static float autotune( StepGroup* g, float timeStep, int subStepCount, int trial
 	return best;
 }
 
+static StepGroup* getBatch( int batch );
+
+namespace b2mi
+{
+StepGroup* b2mi_internal_batch( int batch )
+{
+	return getBatch( batch );
+}
+} // namespace b2mi
+
 static StepGroup* getBatch( int batch )
 {
 	B2MI_CHECK( 1 <= batch && batch <= kMaxBatches && g_batches[batch - 1] != nullptr );

@@ -16,4 +16,26 @@ hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int item
 						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
 						 float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
 
+// replay and adjoint (adjoint.hip)
+int adjoint_scratch_floats( int bodyCount, int slotCount, int subSteps );
+hipError_t launch_snapshot( const DevWorld* worlds, const WorkItem* items, int itemCount, const int* slotContact, DevContact* snap,
+							hipStream_t stream );
+hipError_t launch_gather_poses( const DevWorld* worlds, const int* simBase, const int* simCount, int worldCount, int maxSims,
+								StepPose* out, hipStream_t stream );
+hipError_t launch_replay_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
+							   const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* poseOut,
+							   const int* simBase, float* cf, int cfCapacity, float4* priv, hipStream_t stream );
+hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
+								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
+								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
+								float* cf, int cfCapacity, float4* priv, float* scratch, const int* scratchOffset,
+								hipStream_t stream );
+hipError_t launch_link_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase,
+							  const int2* slotOfContact, int prevStep, int* prevSlot, hipStream_t stream );
+hipError_t launch_mark_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase, int step,
+							  int2* slotOfContact, hipStream_t stream );
+hipError_t launch_patch_params( const DevContact* src, DevContact* dst, const int* slotWorld, int slotCount, const float* params,
+								hipStream_t stream );
+hipError_t launch_fill_zero( float* p, size_t n, hipStream_t stream );
+
 } // namespace b2mi

@@ -18,6 +18,7 @@
 #include "box2d/box2d.h"
 
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #if defined( __GNUC__ )
@@ -304,12 +305,23 @@ SCENE_API int scene_batch_pyramid( const b2WorldDef* def, uint64_t seed, int wor
 
 // ---- one world of the differentiable drop batch ----
 // outParams (optional) receives friction, restitution.
-SCENE_API int scene_drop_boxes( const b2WorldDef* def, uint64_t seed, int worldIndex, int boxCount, b2WorldId* outWorld,
-								b2BodyId* outBodies, int capacity, float* outParams )
+// Sliding/tumbling boxes dropped on a ground box. Per world: friction U(0.2, 1) on every shape
+// (so every contact's mixed friction sqrt(f f) is f) and restitution U(0, 0.3) on the boxes (the
+// ground has 0, so every contact's max-mixed restitution is the boxes'); per box half-extents
+// U(0.2, 0.6), height U(1, 4), velocity N(0, 2) per axis, angular velocity N(0, 1).
+// scene_drop_boxes_ex replaces the sampled friction / restitution when they are >= 0 and the
+// sampled velocities when v0 / w0 are given (boxCount entries; v0 as x, y pairs).
+SCENE_API int scene_drop_boxes_ex( const b2WorldDef* def, uint64_t seed, int worldIndex, int boxCount, float frictionOverride,
+								   float restitutionOverride, const float* v0, const float* w0, b2WorldId* outWorld,
+								   b2BodyId* outBodies, int capacity, float* outParams )
 {
 	Pcg32 rng = pcg32_seed( seed, 500000u + (uint64_t)worldIndex );
 	float friction = pcg32_uniform( &rng, 0.2f, 1.0f );
 	float restitution = pcg32_uniform( &rng, 0.0f, 0.3f );
+	if ( frictionOverride >= 0.0f )
+		friction = frictionOverride;
+	if ( restitutionOverride >= 0.0f )
+		restitution = restitutionOverride;
 	if ( outParams )
 	{
 		outParams[0] = friction;
@@ -339,8 +351,9 @@ SCENE_API int scene_drop_boxes( const b2WorldDef* def, uint64_t seed, int worldI
 		bodyDef.type = b2_dynamicBody;
 		bodyDef.enableSleep = false;
 		bodyDef.position = ( b2Vec2 ){ ( i - 0.5f * ( boxCount - 1 ) ) * 1.5f, y };
-		bodyDef.linearVelocity = ( b2Vec2 ){ pcg32_normal( &rng, 2.0f ), pcg32_normal( &rng, 2.0f ) };
-		bodyDef.angularVelocity = pcg32_normal( &rng, 1.0f );
+		float vx = pcg32_normal( &rng, 2.0f ), vy = pcg32_normal( &rng, 2.0f ), w = pcg32_normal( &rng, 1.0f );
+		bodyDef.linearVelocity = v0 ? ( b2Vec2 ){ v0[2 * i], v0[2 * i + 1] } : ( b2Vec2 ){ vx, vy };
+		bodyDef.angularVelocity = w0 ? w0[i] : w;
 		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
 		b2ShapeDef shapeDef = b2DefaultShapeDef();
 		shapeDef.friction = friction;
@@ -350,6 +363,13 @@ SCENE_API int scene_drop_boxes( const b2WorldDef* def, uint64_t seed, int worldI
 		record( outBodies, capacity, n++, bodyId );
 	}
 	return n;
+}
+
+SCENE_API int scene_drop_boxes( const b2WorldDef* def, uint64_t seed, int worldIndex, int boxCount, b2WorldId* outWorld,
+								b2BodyId* outBodies, int capacity, float* outParams )
+{
+	return scene_drop_boxes_ex( def, seed, worldIndex, boxCount, -1.0f, -1.0f, NULL, NULL, outWorld, outBodies, capacity,
+								outParams );
 }
 
 // ---- shape zoo: every supported shape pair ----

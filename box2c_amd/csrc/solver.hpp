@@ -1,0 +1,524 @@
+// Solver arithmetic of one step, shared by the step kernel (kernels.hip) and the replay and
+// adjoint kernels (adjoint.hip): the stores that hold a work item's state, body loads,
+// prepare, the lane (graph colour) and overflow stages, and the two integrators. Each function
+// keeps the reference's operation order (src/contact_solver.c, src/solver.c); with
+// -ffp-contract=off and correctly rounded division and square root the results are the
+// reference's floats.
+#pragma once
+
+#include "narrowphase.hpp"
+
+namespace b2mi
+{
+
+// ------------------------------------------------------------------------------------
+// solver
+
+struct Body
+{
+	float vx, vy, w, flags;
+	float dpx, dpy, dqc, dqs;
+};
+
+template <class S>
+__device__ inline Body load_body( const S& C, int i )
+{
+	Body b;
+	if ( i < 0 )
+	{
+		// identity state of a static body (src/body.h:90)
+		b.vx = 0.0f;
+		b.vy = 0.0f;
+		b.w = 0.0f;
+		b.flags = 0.0f;
+		b.dpx = 0.0f;
+		b.dpy = 0.0f;
+		b.dqc = 1.0f;
+		b.dqs = 0.0f;
+		return b;
+	}
+	float4 s0 = C.b0( i ), s1 = C.b1( i );
+	b.vx = s0.x;
+	b.vy = s0.y;
+	b.w = s0.z;
+	b.flags = s0.w;
+	b.dpx = s1.x;
+	b.dpy = s1.y;
+	b.dqc = s1.z;
+	b.dqs = s1.w;
+	return b;
+}
+
+template <class S>
+__device__ inline void store_velocity( const S& C, int i, const Body& b )
+{
+	if ( i < 0 )
+		return;
+	C.b0( i ) = make_float4( b.vx, b.vy, b.w, b.flags );
+}
+
+// Solver state of one work item, addressed by local slot k (0 .. slots-1) and local body i:
+// the constraint fields (cf* enum), each slot's two local body indices, and the private body
+// state (two float4 per body: v.x, v.y, w, flags | dp.x, dp.y, dq.c, dq.s).
+//
+// GlobalStore keeps it in HBM (any work item size). LdsStore keeps it in the workgroup's LDS,
+// so the substep passes (about 3 x colours x substeps dependent phases per step) run at LDS
+// latency; the host picks it for every work item whose state fits the LDS budget.
+struct GlobalStore
+{
+	float* f;
+	int cap;
+	int base;
+	const int2* sb;
+	float4* P;
+	__device__ inline float& at( int field, int k ) const
+	{
+		return f[(size_t)field * cap + base + k];
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return sb[k];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return P[2 * i + 1];
+	}
+};
+
+// Bodies in LDS, constraints in HBM: for items whose constraints exceed the LDS budget. The
+// bodies are the randomly accessed part (two per slot per phase), the constraint fields are
+// read and written with consecutive lanes on consecutive slots.
+struct BodiesLdsStore
+{
+	float* f;
+	int cap;
+	int base;
+	const int2* sb;
+	float4* P;
+	__device__ inline float& at( int field, int k ) const
+	{
+		return f[(size_t)field * cap + base + k];
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return sb[k];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return P[2 * i + 1];
+	}
+};
+
+struct LdsStore
+{
+	float* f;
+	int n;
+	int2* sb;
+	float4* P;
+	__device__ inline float& at( int field, int k ) const
+	{
+		return f[field * n + k];
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return sb[k];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return P[2 * i + 1];
+	}
+};
+
+#define PF( name, p ) ( cf##name + cfPointFields * ( p ) )
+
+// contact_solver.c:597-808 (lanes) and 21-147 (overflow): one constraint slot
+template <class S>
+__device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict__ contact, int la, int lb, const DevWorld& W )
+{
+	const DevManifold& m = contact->manifold;
+	float mA = contact->invMassA, iA = contact->invIA, mB = contact->invMassB, iB = contact->invIB;
+	float vAx = 0.0f, vAy = 0.0f, wA = 0.0f, vBx = 0.0f, vBy = 0.0f, wB = 0.0f;
+	if ( la >= 0 )
+	{
+		float4 s = C.b0( la );
+		vAx = s.x;
+		vAy = s.y;
+		wA = s.z;
+	}
+	if ( lb >= 0 )
+	{
+		float4 s = C.b0( lb );
+		vBx = s.x;
+		vBy = s.y;
+		wB = s.z;
+	}
+	C.at( cfInvMassA, slot ) = mA;
+	C.at( cfInvMassB, slot ) = mB;
+	C.at( cfInvIA, slot ) = iA;
+	C.at( cfInvIB, slot ) = iB;
+	const float* soft = ( la < 0 || lb < 0 ) ? W.staticSoft : W.contactSoft;
+	C.at( cfBiasRate, slot ) = soft[0];
+	C.at( cfMassScale, slot ) = soft[1];
+	C.at( cfImpulseScale, slot ) = soft[2];
+	float nx = m.normalX, ny = m.normalY;
+	C.at( cfNormalX, slot ) = nx;
+	C.at( cfNormalY, slot ) = ny;
+	C.at( cfFriction, slot ) = contact->friction;
+	C.at( cfRestitution, slot ) = contact->restitution;
+	float tx = ny, ty = -nx;
+	float warm = W.warmStartScale;
+	for ( int j = 0; j < 2; ++j )
+	{
+		if ( j < m.pointCount )
+		{
+			const DevManifoldPoint& mp = m.points[j];
+			float rAx = mp.anchorAX, rAy = mp.anchorAY, rBx = mp.anchorBX, rBy = mp.anchorBY;
+			C.at( PF( AnchorAX, j ), slot ) = rAx;
+			C.at( PF( AnchorAY, j ), slot ) = rAy;
+			C.at( PF( AnchorBX, j ), slot ) = rBx;
+			C.at( PF( AnchorBY, j ), slot ) = rBy;
+			float dx = rBx - rAx, dy = rBy - rAy;
+			C.at( PF( BaseSeparation, j ), slot ) = mp.separation - ( dx * nx + dy * ny );
+			C.at( PF( NormalImpulse, j ), slot ) = warm * mp.normalImpulse;
+			C.at( PF( TangentImpulse, j ), slot ) = warm * mp.tangentImpulse;
+			C.at( PF( MaxNormalImpulse, j ), slot ) = 0.0f;
+			float rnA = rAx * ny - rAy * nx;
+			float rnB = rBx * ny - rBy * nx;
+			float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+			C.at( PF( NormalMass, j ), slot ) = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+			float rtA = rAx * ty - rAy * tx;
+			float rtB = rBx * ty - rBy * tx;
+			float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+			C.at( PF( TangentMass, j ), slot ) = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+			float vrAx = vAx + ( -wA ) * rAy, vrAy = vAy + wA * rAx;
+			float vrBx = vBx + ( -wB ) * rBy, vrBy = vBy + wB * rBx;
+			C.at( PF( RelativeVelocity, j ), slot ) = nx * ( vrBx - vrAx ) + ny * ( vrBy - vrAy );
+		}
+		else
+		{
+			for ( int f = 0; f < cfPointFields; ++f )
+				C.at( cfAnchorAX + cfPointFields * j + f, slot ) = 0.0f;
+		}
+	}
+}
+
+__device__ inline void lane_apply( Body& bA, Body& bB, float mA, float iA, float mB, float iB, float rAx, float rAy, float rBx,
+								   float rBy, float Px, float Py )
+{
+	bA.vx = bA.vx - mA * Px;
+	bA.vy = bA.vy - mA * Py;
+	bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+	bB.vx = bB.vx + mB * Px;
+	bB.vy = bB.vy + mB * Py;
+	bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+}
+
+// contact_solver.c:810-863
+template <class S>
+__device__ void lane_warm_start( const S& C, int slot, int la, int lb )
+{
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
+	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	float tx = ny, ty = 0.0f - nx;
+	for ( int j = 0; j < 2; ++j )
+	{
+		float ni = C.at( PF( NormalImpulse, j ), slot ), ti = C.at( PF( TangentImpulse, j ), slot );
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float Px = ni * nx + ti * tx;
+		float Py = ni * ny + ti * ty;
+		bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+		bA.vx = bA.vx - mA * Px;
+		bA.vy = bA.vy - mA * Py;
+		bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+	}
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
+}
+
+// contact_solver.c:865-1075
+template <class S>
+__device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias )
+{
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
+	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	float biasRate = useBias ? C.at( cfBiasRate, slot ) : 0.0f;
+	float massScale = useBias ? C.at( cfMassScale, slot ) : 1.0f;
+	float impulseScale = useBias ? C.at( cfImpulseScale, slot ) : 0.0f;
+	float dpx = bB.dpx - bA.dpx, dpy = bB.dpy - bA.dpy;
+	float nImp[2];
+	for ( int j = 0; j < 2; ++j )
+	{
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float rsAx = bA.dqc * rAx - bA.dqs * rAy, rsAy = bA.dqs * rAx + bA.dqc * rAy;
+		float rsBx = bB.dqc * rBx - bB.dqs * rBy, rsBy = bB.dqs * rBx + bB.dqc * rBy;
+		float dsx = dpx + ( rsBx - rsAx ), dsy = dpy + ( rsBy - rsAy );
+		float s = ( nx * dsx + ny * dsy ) + C.at( PF( BaseSeparation, j ), slot );
+		float specBias = s * inv_h;
+		float softBias = maxr( biasRate * s, minBiasVel );
+		float bias = s > 0.0f ? specBias : softBias;
+		float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
+		float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
+		float vn = dvx * nx + dvy * ny;
+		float ni = C.at( PF( NormalImpulse, j ), slot );
+		float negImpulse = C.at( PF( NormalMass, j ), slot ) * ( massScale * ( vn + bias ) ) + impulseScale * ni;
+		float newImpulse = maxr( ni - negImpulse, 0.0f );
+		float impulse = newImpulse - ni;
+		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
+		C.at( PF( MaxNormalImpulse, j ), slot ) = maxr( C.at( PF( MaxNormalImpulse, j ), slot ), newImpulse );
+		nImp[j] = newImpulse;
+		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * nx, impulse * ny );
+	}
+	float tx = ny, ty = 0.0f - nx;
+	float friction = C.at( cfFriction, slot );
+	for ( int j = 0; j < 2; ++j )
+	{
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
+		float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
+		float vt = dvx * tx + dvy * ty;
+		float negImpulse = C.at( PF( TangentMass, j ), slot ) * vt;
+		float maxFriction = friction * nImp[j];
+		float ti = C.at( PF( TangentImpulse, j ), slot );
+		float newImpulse = ti - negImpulse;
+		newImpulse = maxr( 0.0f - maxFriction, minr( newImpulse, maxFriction ) );
+		float impulse = newImpulse - ti;
+		C.at( PF( TangentImpulse, j ), slot ) = newImpulse;
+		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * tx, impulse * ty );
+	}
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
+}
+
+// contact_solver.c:1077-1178
+template <class S>
+__device__ void lane_restitution( const S& C, int slot, int la, int lb, float threshold )
+{
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
+	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	float restitution = C.at( cfRestitution, slot );
+	for ( int j = 0; j < 2; ++j )
+	{
+		float relVel = C.at( PF( RelativeVelocity, j ), slot );
+		bool skip = ( relVel + threshold ) > 0.0f || C.at( PF( MaxNormalImpulse, j ), slot ) == 0.0f;
+		float mass = skip ? 0.0f : C.at( PF( NormalMass, j ), slot );
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
+		float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
+		float vn = dvx * nx + dvy * ny;
+		float ni = C.at( PF( NormalImpulse, j ), slot );
+		float negImpulse = mass * ( vn + restitution * relVel );
+		float newImpulse = maxr( ni - negImpulse, 0.0f );
+		float impulse = newImpulse - ni;
+		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
+		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * nx, impulse * ny );
+	}
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
+}
+
+// ---- overflow (serial) stages, contact_solver.c:149-471 ----
+template <class S>
+__device__ void ovf_warm_start( const S& C, int slot, int la, int lb, int pointCount )
+{
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
+	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	float tx = ny, ty = -nx;
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		float ni = C.at( PF( NormalImpulse, j ), slot ), ti = C.at( PF( TangentImpulse, j ), slot );
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float Px = ni * nx + ti * tx;
+		float Py = ni * ny + ti * ty;
+		bA.w -= iA * ( rAx * Py - rAy * Px );
+		bA.vx = bA.vx + ( -mA ) * Px;
+		bA.vy = bA.vy + ( -mA ) * Py;
+		bB.w += iB * ( rBx * Py - rBy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+	}
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
+}
+
+template <class S>
+__device__ void ovf_solve( const S& C, int slot, int la, int lb, int pointCount, float inv_h, float pushout,
+						   bool useBias )
+{
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
+	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	float tx = ny, ty = -nx;
+	float dpx = bB.dpx - bA.dpx, dpy = bB.dpy - bA.dpy;
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float rBqx = bB.dqc * rBx - bB.dqs * rBy, rBqy = bB.dqs * rBx + bB.dqc * rBy;
+		float rAqx = bA.dqc * rAx - bA.dqs * rAy, rAqy = bA.dqs * rAx + bA.dqc * rAy;
+		float dsx = dpx + ( rBqx - rAqx ), dsy = dpy + ( rBqy - rAqy );
+		float s = ( dsx * nx + dsy * ny ) + C.at( PF( BaseSeparation, j ), slot );
+		float velocityBias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+		if ( s > 0.0f )
+			velocityBias = s * inv_h;
+		else if ( useBias )
+		{
+			velocityBias = maxr( C.at( cfBiasRate, slot ) * s, -pushout );
+			massScale = C.at( cfMassScale, slot );
+			impulseScale = C.at( cfImpulseScale, slot );
+		}
+		float vrAx = bA.vx + ( -bA.w ) * rAy, vrAy = bA.vy + bA.w * rAx;
+		float vrBx = bB.vx + ( -bB.w ) * rBy, vrBy = bB.vy + bB.w * rBx;
+		float vn = ( vrBx - vrAx ) * nx + ( vrBy - vrAy ) * ny;
+		float ni = C.at( PF( NormalImpulse, j ), slot );
+		float impulse = -C.at( PF( NormalMass, j ), slot ) * massScale * ( vn + velocityBias ) - impulseScale * ni;
+		float newImpulse = maxr( ni + impulse, 0.0f );
+		impulse = newImpulse - ni;
+		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
+		C.at( PF( MaxNormalImpulse, j ), slot ) = maxr( C.at( PF( MaxNormalImpulse, j ), slot ), impulse );
+		float Px = impulse * nx, Py = impulse * ny;
+		bA.vx = bA.vx - mA * Px;
+		bA.vy = bA.vy - mA * Py;
+		bA.w -= iA * ( rAx * Py - rAy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+		bB.w += iB * ( rBx * Py - rBy * Px );
+	}
+	float friction = C.at( cfFriction, slot );
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float vrBx = bB.vx + ( -bB.w ) * rBy, vrBy = bB.vy + bB.w * rBx;
+		float vrAx = bA.vx + ( -bA.w ) * rAy, vrAy = bA.vy + bA.w * rAx;
+		float vt = ( vrBx - vrAx ) * tx + ( vrBy - vrAy ) * ty;
+		float impulse = C.at( PF( TangentMass, j ), slot ) * ( -vt );
+		float maxFriction = friction * C.at( PF( NormalImpulse, j ), slot );
+		float ti = C.at( PF( TangentImpulse, j ), slot );
+		float sum = ti + impulse;
+		float newImpulse = sum < -maxFriction ? -maxFriction : ( sum > maxFriction ? maxFriction : sum );
+		impulse = newImpulse - ti;
+		C.at( PF( TangentImpulse, j ), slot ) = newImpulse;
+		float Px = impulse * tx, Py = impulse * ty;
+		bA.vx = bA.vx - mA * Px;
+		bA.vy = bA.vy - mA * Py;
+		bA.w -= iA * ( rAx * Py - rAy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+		bB.w += iB * ( rBx * Py - rBy * Px );
+	}
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
+}
+
+template <class S>
+__device__ void ovf_restitution( const S& C, int slot, int la, int lb, int pointCount, float threshold )
+{
+	float restitution = C.at( cfRestitution, slot );
+	if ( restitution == 0.0f )
+		return;
+	Body bA = load_body( C, la ), bB = load_body( C, lb );
+	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		float relVel = C.at( PF( RelativeVelocity, j ), slot );
+		if ( relVel > -threshold || C.at( PF( MaxNormalImpulse, j ), slot ) == 0.0f )
+			continue;
+		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
+		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float vrBx = bB.vx + ( -bB.w ) * rBy, vrBy = bB.vy + bB.w * rBx;
+		float vrAx = bA.vx + ( -bA.w ) * rAy, vrAy = bA.vy + bA.w * rAx;
+		float vn = ( vrBx - vrAx ) * nx + ( vrBy - vrAy ) * ny;
+		float ni = C.at( PF( NormalImpulse, j ), slot );
+		float impulse = -C.at( PF( NormalMass, j ), slot ) * ( vn + restitution * relVel );
+		float newImpulse = maxr( ni + impulse, 0.0f );
+		impulse = newImpulse - ni;
+		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
+		C.at( PF( MaxNormalImpulse, j ), slot ) = maxr( C.at( PF( MaxNormalImpulse, j ), slot ), impulse );
+		float Px = impulse * nx, Py = impulse * ny;
+		bA.vx = bA.vx - mA * Px;
+		bA.vy = bA.vy - mA * Py;
+		bA.w -= iA * ( rAx * Py - rAy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+		bB.w += iB * ( rBx * Py - rBy * Px );
+	}
+	store_velocity( C, la, bA );
+	store_velocity( C, lb, bB );
+}
+
+// solver.c:34-93 for one body of the private set
+template <class S>
+__device__ void integrate_velocity( const S& C, int i, const DevBodySim& sim, const DevWorld& W )
+{
+	float h = W.h;
+	float maxLinearSpeed = W.maxLinearSpeed, maxAngularSpeed = W.maxAngularSpeed;
+	float4 s0 = C.b0( i );
+	float vx = s0.x, vy = s0.y, w = s0.z;
+	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
+	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
+	float mg = sim.mass * sim.gravityScale;
+	float fx = sim.force[0] + mg * W.gravityX, fy = sim.force[1] + mg * W.gravityY;
+	float hm = h * sim.invMass;
+	float lvdx = hm * fx, lvdy = hm * fy;
+	float avd = h * sim.invI * sim.torque;
+	vx = lvdx + linearDamping * vx;
+	vy = lvdy + linearDamping * vy;
+	w = avd + angularDamping * w;
+	if ( vx * vx + vy * vy > maxLinearSpeed * maxLinearSpeed )
+	{
+		float ratio = maxLinearSpeed / sqrtf( vx * vx + vy * vy );
+		vx = ratio * vx;
+		vy = ratio * vy;
+	}
+	if ( w * w > maxAngularSpeed * maxAngularSpeed )
+	{
+		float ratio = maxAngularSpeed / ( w < 0 ? -w : w );
+		w *= ratio;
+	}
+	C.b0( i ) = make_float4( vx, vy, w, s0.w );
+}
+
+// solver.c:146-163
+template <class S>
+__device__ void integrate_position( const S& C, int i, float h )
+{
+	float4 s0 = C.b0( i );
+	float4 s1 = C.b1( i );
+	float da = h * s0.z;
+	float c2 = s1.z - da * s1.w;
+	float s2 = s1.w + da * s1.z;
+	float mag = sqrtf( s2 * s2 + c2 * c2 );
+	float inv = mag > 0.0f ? 1.0f / mag : 0.0f;
+	s1.z = c2 * inv;
+	s1.w = s2 * inv;
+	s1.x = s1.x + h * s0.x;
+	s1.y = s1.y + h * s0.y;
+	C.b1( i ) = s1;
+}
+
+
+} // namespace b2mi

@@ -506,6 +506,7 @@ static void buildWorkItems( StepGroup* g )
 	g->hSlotBodies.clear();
 	g->maxGroupSize = 0;
 	g->maxItemLds = 0;
+	g->itemsVersion += 1;
 	int target = g->targetSlots > 0 ? g->targetSlots : 256;
 
 	for ( World* w : g->worlds )
@@ -1075,6 +1076,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 			block *= 2;
 	}
 	B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
+	if ( tape )
+		tapeRecordStep( this, timeStep, subStepCount, block );
 	HIP_CHECK( hipEventRecord( e3, s ) );
 	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons, capCons,
 							 dPriv, dEnlarge, dCounters + 1, capEnlarge, s ) );

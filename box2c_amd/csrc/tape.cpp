@@ -1,0 +1,461 @@
+// Taped runs and the backward pass (the host side of adjoint.hip).
+//
+// While a tape is on, every step of the group records, right before its solver launch: the
+// work items and slot lists (a new copy only when they were rebuilt), a snapshot of every
+// slot's contact, and every `interval` steps a checkpoint of all body poses. The backward pass
+// rebuilds the step-start poses of one checkpoint segment at a time by replaying the solver,
+// then runs the adjoint of each step of the segment from the last to the first.
+#include "launch.hpp"
+#include "world_internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace b2mi
+{
+
+struct TapeVersion
+{
+	int itemsVersion = -1;
+	int itemCount = 0, slotCount = 0, bodyListCount = 0, block = 64;
+	WorkItem* dItems = nullptr;
+	int* dBodyList = nullptr;
+	int* dSlotContact = nullptr;
+	int2* dSlotBodies = nullptr;
+	int* dSlotWorld = nullptr;
+	int* dScratchOffset = nullptr;
+	size_t scratchFloats = 0;
+};
+
+struct TapeStep
+{
+	int version;
+	DevContact* dSnap;
+	int checkpoint; // index into checkpoints, or -1
+};
+
+struct Tape
+{
+	int interval = 10;
+	float timeStep = 0.0f;
+	int subSteps = 0;
+	int savedTarget = 0;
+	std::vector<int> simBase, simCount;
+	int totalSims = 0, maxSims = 0;
+	int* dSimBase = nullptr;
+	int* dSimCount = nullptr;
+	std::vector<TapeVersion> versions;
+	std::vector<TapeStep> steps;
+	std::vector<StepPose*> checkpoints;
+	std::vector<b2BodyId> bodies; // tape order: world by world, awake-set order
+};
+
+template <typename T>
+static T* devAlloc( size_t n )
+{
+	T* p = nullptr;
+	HIP_CHECK( hipMalloc( (void**)&p, sizeof( T ) * std::max<size_t>( n, 1 ) ) );
+	return p;
+}
+
+template <typename T>
+static void devRelease( T*& p )
+{
+	if ( p )
+		(void)hipFree( p );
+	p = nullptr;
+}
+
+void tapeFree( StepGroup* g )
+{
+	Tape* t = g->tape;
+	if ( t == nullptr )
+		return;
+	(void)hipStreamSynchronize( stream() );
+	for ( TapeVersion& v : t->versions )
+	{
+		devRelease( v.dItems );
+		devRelease( v.dBodyList );
+		devRelease( v.dSlotContact );
+		devRelease( v.dSlotBodies );
+		devRelease( v.dSlotWorld );
+		devRelease( v.dScratchOffset );
+	}
+	for ( TapeStep& s : t->steps )
+		devRelease( s.dSnap );
+	for ( StepPose*& p : t->checkpoints )
+		devRelease( p );
+	devRelease( t->dSimBase );
+	devRelease( t->dSimCount );
+	g->targetSlots = t->savedTarget;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+	delete t;
+	g->tape = nullptr;
+}
+
+static void tapeBegin( StepGroup* g, int interval )
+{
+	tapeFree( g );
+	Tape* t = new Tape();
+	t->interval = interval > 0 ? interval : 10;
+	t->savedTarget = g->targetSlots;
+	for ( World* w : g->worlds )
+	{
+		t->simBase.push_back( t->totalSims );
+		int n = (int)w->awakeSimBody.size();
+		t->simCount.push_back( n );
+		t->totalSims += n;
+		t->maxSims = std::max( t->maxSims, n );
+		for ( int i = 0; i < n; ++i )
+		{
+			const Body& b = w->bodies[w->awakeSimBody[i]];
+			if ( b.type != b2_dynamicBody )
+				fatal( "a taped run supports dynamic bodies only (no kinematic bodies)", __FILE__, __LINE__ );
+			t->bodies.push_back( b2BodyId{ b.id + 1, (uint16_t)w->slot, b.revision } );
+		}
+	}
+	hipStream_t s = stream();
+	t->dSimBase = devAlloc<int>( t->simBase.size() );
+	t->dSimCount = devAlloc<int>( t->simCount.size() );
+	if ( !t->simBase.empty() )
+	{
+		HIP_CHECK( hipMemcpyAsync( t->dSimBase, t->simBase.data(), sizeof( int ) * t->simBase.size(), hipMemcpyHostToDevice, s ) );
+		HIP_CHECK( hipMemcpyAsync( t->dSimCount, t->simCount.data(), sizeof( int ) * t->simCount.size(), hipMemcpyHostToDevice, s ) );
+	}
+	// one work item per world: the adjoint sums each world's parameter gradients in one workgroup
+	g->targetSlots = 1 << 24;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+	g->tape = t;
+}
+
+void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSize )
+{
+	Tape* t = g->tape;
+	hipStream_t s = stream();
+	if ( t->steps.empty() )
+	{
+		t->timeStep = timeStep;
+		t->subSteps = subStepCount;
+	}
+	else if ( t->timeStep != timeStep || t->subSteps != subStepCount )
+		fatal( "a taped run needs the same time step and substep count at every step", __FILE__, __LINE__ );
+	for ( size_t k = 0; k < g->worlds.size(); ++k )
+		if ( (int)g->worlds[k]->awakeSimBody.size() != t->simCount[k] )
+			fatal( "bodies were created or destroyed during a taped run", __FILE__, __LINE__ );
+
+	if ( t->versions.empty() || t->versions.back().itemsVersion != g->itemsVersion )
+	{
+		TapeVersion v;
+		v.itemsVersion = g->itemsVersion;
+		v.itemCount = g->itemCount;
+		v.slotCount = g->slotCount;
+		v.bodyListCount = g->bodyListCount;
+		v.block = blockSize;
+		std::vector<int> itemsOfWorld( g->worlds.size(), 0 );
+		std::vector<int> slotWorld( std::max( g->slotCount, 1 ), 0 );
+		std::vector<int> offsets( std::max( g->itemCount, 1 ), 0 );
+		size_t scratch = 0;
+		for ( int i = 0; i < g->itemCount; ++i )
+		{
+			const WorkItem& wi = g->hItems[i];
+			if ( wi.groupStart[1] != wi.groupStart[0] )
+				fatal( "a taped run does not support overflow constraints (more than 11 graph colours)", __FILE__, __LINE__ );
+			if ( wi.ownedCount != wi.bodyCount )
+				fatal( "a taped run does not support kinematic bodies", __FILE__, __LINE__ );
+			itemsOfWorld[wi.world] += 1;
+			for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
+				slotWorld[wi.slotStart + k] = wi.world;
+			B2MI_CHECK( scratch < ( (size_t)1 << 31 ) );
+			offsets[i] = (int)scratch;
+			scratch += (size_t)adjoint_scratch_floats( wi.bodyCount, wi.groupStart[kGroups], subStepCount );
+		}
+		for ( int n : itemsOfWorld )
+			B2MI_CHECK( n <= 1 );
+		v.scratchFloats = scratch;
+		v.dItems = devAlloc<WorkItem>( v.itemCount );
+		v.dBodyList = devAlloc<int>( v.bodyListCount );
+		v.dSlotContact = devAlloc<int>( v.slotCount );
+		v.dSlotBodies = devAlloc<int2>( v.slotCount );
+		v.dSlotWorld = devAlloc<int>( v.slotCount );
+		v.dScratchOffset = devAlloc<int>( v.itemCount );
+		if ( v.itemCount )
+		{
+			HIP_CHECK( hipMemcpyAsync( v.dItems, g->dItems, sizeof( WorkItem ) * v.itemCount, hipMemcpyDeviceToDevice, s ) );
+			HIP_CHECK( hipMemcpyAsync( v.dScratchOffset, offsets.data(), sizeof( int ) * v.itemCount, hipMemcpyHostToDevice, s ) );
+		}
+		if ( v.bodyListCount )
+			HIP_CHECK( hipMemcpyAsync( v.dBodyList, g->dBodyList, sizeof( int ) * v.bodyListCount, hipMemcpyDeviceToDevice, s ) );
+		if ( v.slotCount )
+		{
+			HIP_CHECK( hipMemcpyAsync( v.dSlotContact, g->dSlotContact, sizeof( int ) * v.slotCount, hipMemcpyDeviceToDevice, s ) );
+			HIP_CHECK( hipMemcpyAsync( v.dSlotBodies, g->dSlotBodies, sizeof( int2 ) * v.slotCount, hipMemcpyDeviceToDevice, s ) );
+			HIP_CHECK( hipMemcpyAsync( v.dSlotWorld, slotWorld.data(), sizeof( int ) * v.slotCount, hipMemcpyHostToDevice, s ) );
+		}
+		// the host vectors above are read by the copies: wait before they go out of scope
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		t->versions.push_back( v );
+	}
+
+	TapeStep st;
+	st.version = (int)t->versions.size() - 1;
+	st.checkpoint = -1;
+	if ( t->steps.size() % (size_t)t->interval == 0 )
+	{
+		StepPose* cp = devAlloc<StepPose>( t->totalSims );
+		HIP_CHECK( launch_gather_poses( g->dWorlds, t->dSimBase, t->dSimCount, (int)g->worlds.size(), t->maxSims, cp, s ) );
+		st.checkpoint = (int)t->checkpoints.size();
+		t->checkpoints.push_back( cp );
+	}
+	st.dSnap = devAlloc<DevContact>( g->slotCount );
+	HIP_CHECK( launch_snapshot( g->dWorlds, g->dItems, g->itemCount, g->dSlotContact, st.dSnap, s ) );
+	t->steps.push_back( st );
+}
+
+// d(loss)/d(start poses) and d(loss)/d(friction, restitution per world) from d(loss)/d(end poses)
+static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, float* gradParams )
+{
+	Tape* t = g->tape;
+	hipStream_t s = stream();
+	const int T = (int)t->steps.size();
+	const int n = t->totalSims;
+	const int W = (int)g->worlds.size();
+	if ( T == 0 )
+	{
+		for ( int i = 0; i < 7 * n; ++i )
+			gradStart[i] = gradEnd[i];
+		for ( int i = 0; i < 2 * W; ++i )
+			gradParams[i] = 0.0f;
+		return;
+	}
+	HIP_CHECK( hipStreamSynchronize( s ) );
+
+	int maxSlots = 1, maxBodies = 1;
+	size_t maxScratch = 1;
+	for ( const TapeVersion& v : t->versions )
+	{
+		maxSlots = std::max( maxSlots, v.slotCount );
+		maxBodies = std::max( maxBodies, v.bodyListCount );
+		maxScratch = std::max( maxScratch, v.scratchFloats );
+	}
+	std::vector<int> contactBase( W );
+	size_t contactTotal = 0;
+	for ( int k = 0; k < W; ++k )
+	{
+		contactBase[k] = (int)contactTotal;
+		contactTotal += (size_t)std::max( g->worlds[k]->capContacts, 1 );
+	}
+
+	const int K = t->interval;
+	StepPose* dAdj = devAlloc<StepPose>( n );
+	StepPose* dStates = devAlloc<StepPose>( (size_t)( K + 1 ) * n );
+	float* dParam = devAlloc<float>( 2 * W );
+	float* dCf = devAlloc<float>( (size_t)cfFloatCount * maxSlots );
+	float4* dPriv = devAlloc<float4>( 2 * (size_t)maxBodies );
+	float* dScratch = devAlloc<float>( maxScratch );
+	float* dLamA = devAlloc<float>( 4 * (size_t)maxSlots );
+	float* dLamB = devAlloc<float>( 4 * (size_t)maxSlots );
+	int* dPrevSlot = devAlloc<int>( maxSlots );
+	int* dContactBase = devAlloc<int>( W );
+	int2* dSlotOf = devAlloc<int2>( contactTotal );
+
+	std::vector<StepPose> h( n );
+	for ( int i = 0; i < n; ++i )
+	{
+		const float* e = gradEnd + 7 * i;
+		h[i] = StepPose{ e[0], e[1], e[2], e[3], e[4], e[5], e[6], 0.0f };
+	}
+	HIP_CHECK( hipMemcpyAsync( dAdj, h.data(), sizeof( StepPose ) * n, hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipMemcpyAsync( dContactBase, contactBase.data(), sizeof( int ) * W, hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( launch_fill_zero( dParam, 2 * (size_t)W, s ) );
+	HIP_CHECK( launch_fill_zero( reinterpret_cast<float*>( dSlotOf ), 2 * contactTotal, s ) );
+	HIP_CHECK( launch_fill_zero( dLamA, 4 * (size_t)maxSlots, s ) );
+
+	float* lamOut = dLamA;
+	float* lamPrev = dLamB;
+	for ( int c = ( ( T - 1 ) / K ) * K; c >= 0; c -= K )
+	{
+		int end = std::min( c + K, T );
+		const TapeStep& first = t->steps[c];
+		B2MI_CHECK( first.checkpoint >= 0 );
+		HIP_CHECK( hipMemcpyAsync( dStates, t->checkpoints[first.checkpoint], sizeof( StepPose ) * n, hipMemcpyDeviceToDevice, s ) );
+		for ( int step = c; step < end - 1; ++step )
+		{
+			const TapeStep& ts = t->steps[step];
+			const TapeVersion& v = t->versions[ts.version];
+			HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
+										   dStates + (size_t)( step - c ) * n, dStates + (size_t)( step - c + 1 ) * n, t->dSimBase, dCf,
+										   maxSlots, dPriv, s ) );
+		}
+		for ( int step = end - 1; step >= c; --step )
+		{
+			const TapeStep& ts = t->steps[step];
+			const TapeVersion& v = t->versions[ts.version];
+			// slots of the previous step, tagged step (= previous step index + 1), and the links to them
+			int prevTag = -1;
+			if ( step > 0 )
+			{
+				const TapeVersion& vp = t->versions[t->steps[step - 1].version];
+				HIP_CHECK( launch_mark_slots( vp.dSlotContact, vp.dSlotWorld, vp.slotCount, dContactBase, step, dSlotOf, s ) );
+				HIP_CHECK( launch_fill_zero( lamPrev, 4 * (size_t)vp.slotCount, s ) );
+				prevTag = step;
+			}
+			HIP_CHECK( launch_link_slots( v.dSlotContact, v.dSlotWorld, v.slotCount, dContactBase, dSlotOf, prevTag, dPrevSlot, s ) );
+			HIP_CHECK( launch_adjoint_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
+											dStates + (size_t)( step - c ) * n, dAdj, t->dSimBase, reinterpret_cast<const float4*>( lamOut ),
+											lamPrev, dPrevSlot, dParam, dCf, maxSlots, dPriv, dScratch, v.dScratchOffset, s ) );
+			std::swap( lamOut, lamPrev );
+		}
+	}
+	HIP_CHECK( hipMemcpyAsync( h.data(), dAdj, sizeof( StepPose ) * n, hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipMemcpyAsync( gradParams, dParam, sizeof( float ) * 2 * W, hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	for ( int i = 0; i < n; ++i )
+	{
+		float* o = gradStart + 7 * i;
+		o[0] = h[i].cx, o[1] = h[i].cy, o[2] = h[i].qc, o[3] = h[i].qs, o[4] = h[i].vx, o[5] = h[i].vy, o[6] = h[i].w;
+	}
+	devRelease( dAdj );
+	devRelease( dStates );
+	devRelease( dParam );
+	devRelease( dCf );
+	devRelease( dPriv );
+	devRelease( dScratch );
+	devRelease( dLamA );
+	devRelease( dLamB );
+	devRelease( dPrevSlot );
+	devRelease( dContactBase );
+	devRelease( dSlotOf );
+}
+
+// The taped solver steps again from the given start poses (n x 7), optionally with every
+// contact's friction and restitution replaced by per-world values: the function whose
+// derivatives the adjoint computes (same manifolds and contact sets).
+static void tapeReplay( StepGroup* g, const float* startPoses, const float* worldParams, float* endPoses )
+{
+	Tape* t = g->tape;
+	hipStream_t s = stream();
+	const int T = (int)t->steps.size();
+	const int n = t->totalSims;
+	const int W = (int)g->worlds.size();
+	int maxSlots = 1, maxBodies = 1;
+	for ( const TapeVersion& v : t->versions )
+	{
+		maxSlots = std::max( maxSlots, v.slotCount );
+		maxBodies = std::max( maxBodies, v.bodyListCount );
+	}
+	StepPose* dA = devAlloc<StepPose>( n );
+	StepPose* dB = devAlloc<StepPose>( n );
+	float* dCf = devAlloc<float>( (size_t)cfFloatCount * maxSlots );
+	float4* dPriv = devAlloc<float4>( 2 * (size_t)maxBodies );
+	DevContact* dPatched = devAlloc<DevContact>( maxSlots );
+	float* dParams = devAlloc<float>( 2 * W );
+	std::vector<StepPose> h( n );
+	for ( int i = 0; i < n; ++i )
+	{
+		const float* e = startPoses + 7 * i;
+		h[i] = StepPose{ e[0], e[1], e[2], e[3], e[4], e[5], e[6], 0.0f };
+	}
+	HIP_CHECK( hipMemcpyAsync( dA, h.data(), sizeof( StepPose ) * n, hipMemcpyHostToDevice, s ) );
+	if ( worldParams )
+		HIP_CHECK( hipMemcpyAsync( dParams, worldParams, sizeof( float ) * 2 * W, hipMemcpyHostToDevice, s ) );
+	for ( int step = 0; step < T; ++step )
+	{
+		const TapeStep& ts = t->steps[step];
+		const TapeVersion& v = t->versions[ts.version];
+		const DevContact* snap = ts.dSnap;
+		if ( worldParams )
+		{
+			HIP_CHECK( launch_patch_params( ts.dSnap, dPatched, v.dSlotWorld, v.slotCount, dParams, s ) );
+			snap = dPatched;
+		}
+		HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, snap, dA, dB, t->dSimBase,
+									   dCf, maxSlots, dPriv, s ) );
+		std::swap( dA, dB );
+	}
+	HIP_CHECK( hipMemcpyAsync( h.data(), dA, sizeof( StepPose ) * n, hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	for ( int i = 0; i < n; ++i )
+	{
+		float* o = endPoses + 7 * i;
+		o[0] = h[i].cx, o[1] = h[i].cy, o[2] = h[i].qc, o[3] = h[i].qs, o[4] = h[i].vx, o[5] = h[i].vy, o[6] = h[i].w;
+	}
+	devRelease( dA );
+	devRelease( dB );
+	devRelease( dCf );
+	devRelease( dPriv );
+	devRelease( dPatched );
+	devRelease( dParams );
+}
+
+StepGroup* b2mi_internal_batch( int batch );
+
+} // namespace b2mi
+
+using namespace b2mi;
+
+extern "C" {
+
+B2_API void b2mi_Batch_BeginTape( int batch, int checkpointInterval )
+{
+	tapeBegin( b2mi_internal_batch( batch ), checkpointInterval );
+}
+
+B2_API void b2mi_Batch_EndTape( int batch )
+{
+	tapeFree( b2mi_internal_batch( batch ) );
+}
+
+B2_API int b2mi_Batch_GetTapeStepCount( int batch )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	return g->tape ? (int)g->tape->steps.size() : 0;
+}
+
+B2_API int b2mi_Batch_GetTapeBodies( int batch, b2BodyId* bodies, int capacity )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	B2MI_CHECK( g->tape != nullptr );
+	int n = (int)g->tape->bodies.size();
+	for ( int i = 0; i < n && i < capacity; ++i )
+		bodies[i] = g->tape->bodies[i];
+	return n;
+}
+
+B2_API void b2mi_Batch_GetTapePoses( int batch, float* poses )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	Tape* t = g->tape;
+	B2MI_CHECK( t != nullptr );
+	hipStream_t s = stream();
+	StepPose* d = devAlloc<StepPose>( t->totalSims );
+	if ( g->dWorlds == nullptr )
+		fatal( "the batch has not been stepped", __FILE__, __LINE__ );
+	HIP_CHECK( launch_gather_poses( g->dWorlds, t->dSimBase, t->dSimCount, (int)g->worlds.size(), t->maxSims, d, s ) );
+	std::vector<StepPose> h( t->totalSims );
+	HIP_CHECK( hipMemcpyAsync( h.data(), d, sizeof( StepPose ) * h.size(), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	devRelease( d );
+	for ( size_t i = 0; i < h.size(); ++i )
+	{
+		float* o = poses + 7 * i;
+		o[0] = h[i].cx, o[1] = h[i].cy, o[2] = h[i].qc, o[3] = h[i].qs, o[4] = h[i].vx, o[5] = h[i].vy, o[6] = h[i].w;
+	}
+}
+
+B2_API void b2mi_Batch_ReplayTape( int batch, const float* startPoses, const float* worldParams, float* endPoses )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	B2MI_CHECK( g->tape != nullptr );
+	tapeReplay( g, startPoses, worldParams, endPoses );
+}
+
+B2_API void b2mi_Batch_Backward( int batch, const float* gradEnd, float* gradStart, float* gradParams )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	B2MI_CHECK( g->tape != nullptr );
+	tapeBackward( g, gradEnd, gradStart, gradParams );
+}
+
+} // extern "C"

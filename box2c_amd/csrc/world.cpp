@@ -118,6 +118,7 @@ World::~World()
 
 StepGroup::~StepGroup()
 {
+	tapeFree( this );
 	devFree( dWorlds );
 	devFree( dCollideList );
 	devFree( dChanges );

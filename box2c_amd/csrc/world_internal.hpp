@@ -224,6 +224,8 @@ struct World
 	~World();
 };
 
+struct Tape; // taped run of a step group (tape.cpp)
+
 // a set of worlds advanced together by one narrowphase launch and one solver launch
 struct StepGroup
 {
@@ -264,6 +266,8 @@ struct StepGroup
 	int slotCount = 0, bodyListCount = 0;
 	int maxGroupSize = 0;
 	int maxItemLds = 0; // dynamic LDS bytes of the solver launch (largest LDS-resident item)
+	int itemsVersion = 0; // incremented by every work-item rebuild
+	Tape* tape = nullptr; // recording for the adjoint, when on
 
 	// host staging
 	std::vector<DevWorld> hWorlds;
@@ -283,6 +287,10 @@ struct StepGroup
 	~StepGroup();
 	void step( float timeStep, int subStepCount );
 };
+
+// tape.cpp
+void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSize );
+void tapeFree( StepGroup* g );
 
 hipStream_t stream();
 World* getWorld( b2WorldId id );

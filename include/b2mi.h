@@ -90,6 +90,35 @@ B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlo
 B2_API void b2mi_Batch_GetTiming( int batch, float* msCollide, float* msSolve, float* msHost );
 B2_API void b2mi_World_GetTiming( b2WorldId worldId, float* msCollide, float* msSolve, float* msHost );
 
+// ---- gradients of a batch run ------------------------------------------------------------
+// A tape records every step of the batch from b2mi_Batch_BeginTape on: the solver's inputs of
+// each step (work items, slot lists, a snapshot of every slot's contact with the impulses it
+// inherited) and every checkpointInterval steps the poses of all bodies. b2mi_Batch_Backward
+// then returns gradients of a loss through the whole taped run: it replays each checkpoint
+// segment's solver steps and runs their hand-written adjoint in reverse. The narrowphase
+// manifold of each step is a constant of that step (contact points, normals and the contact
+// set do not move with the gradient). Requirements: dynamic bodies only, no more than 11
+// graph colours (no overflow constraints), the same time step and substep count at every
+// step, no bodies created or destroyed, no applied forces.
+//
+// Poses are 7 floats per body: center of mass (x, y), rotation (cos, sin), linear velocity
+// (x, y), angular velocity; bodies in the order of b2mi_Batch_GetTapeBodies (world by world).
+B2_API void b2mi_Batch_BeginTape( int batch, int checkpointInterval );
+B2_API void b2mi_Batch_EndTape( int batch );
+B2_API int b2mi_Batch_GetTapeStepCount( int batch );
+B2_API int b2mi_Batch_GetTapeBodies( int batch, b2BodyId* bodies, int capacity );
+B2_API void b2mi_Batch_GetTapePoses( int batch, float* poses );
+// Runs the taped solver steps again from startPoses (the same manifolds and contact sets), with
+// every contact's friction and restitution replaced by per-world values when worldParams
+// (friction, restitution per world) is given; endPoses receives the final poses. This is the
+// function b2mi_Batch_Backward differentiates.
+B2_API void b2mi_Batch_ReplayTape( int batch, const float* startPoses, const float* worldParams, float* endPoses );
+// gradEnd: d(loss)/d(final poses); gradStart: d(loss)/d(poses when the tape began);
+// gradParams: per world, d(loss)/d(friction) and d(loss)/d(restitution), where each contact's
+// mixed coefficient counts as the world's parameter (exact when every shape of a world has the
+// same friction, and restitution mixes to the bodies' value).
+B2_API void b2mi_Batch_Backward( int batch, const float* gradEnd, float* gradStart, float* gradParams );
+
 // ---- device ----------------------------------------------------------------------------
 B2_API int b2mi_DeviceCount( void );
 B2_API const char* b2mi_GetLastError( void );
