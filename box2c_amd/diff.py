@@ -20,11 +20,22 @@ from .capi import B2Api, BodyId, WorldId
 POSE = 7  # center x, y, rotation cos, sin, velocity x, y, angular velocity
 
 
-def bind(api: B2Api):
+def bind_scene(api: B2Api):
+    """The drop-box scene builder and the pose getters (any library with the public API)."""
     from .capi import Vec2
     L = api.lib
     L.b2Body_GetWorldCenterOfMass.argtypes = [BodyId]
     L.b2Body_GetWorldCenterOfMass.restype = Vec2
+    S = api.scenes
+    S.scene_drop_boxes_ex.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(WorldId), ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_void_p]
+    S.scene_drop_boxes_ex.restype = ctypes.c_int
+
+
+def bind(api: B2Api):
+    bind_scene(api)
+    L = api.lib
     L.b2mi_Batch_BeginTape.argtypes = [ctypes.c_int, ctypes.c_int]
     L.b2mi_Batch_EndTape.argtypes = [ctypes.c_int]
     L.b2mi_Batch_GetTapeStepCount.argtypes = [ctypes.c_int]
@@ -34,11 +45,6 @@ def bind(api: B2Api):
     L.b2mi_Batch_GetTapePoses.argtypes = [ctypes.c_int, ctypes.c_void_p]
     L.b2mi_Batch_ReplayTape.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.b2mi_Batch_Backward.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-    S = api.scenes
-    S.scene_drop_boxes_ex.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
-                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(WorldId), ctypes.c_void_p, ctypes.c_int,
-                                      ctypes.c_void_p]
-    S.scene_drop_boxes_ex.restype = ctypes.c_int
 
 
 def _ptr(a: np.ndarray):

@@ -34,7 +34,7 @@ def test_taped_run_matches_oracle():
     try:
         poses = r.run(f, e, v0, w0)
         cpu = helpers.oracle()
-        diff.bind(cpu)
+        diff.bind_scene(cpu)
         d = cpu.default_world_def(enableSleep=False)
         for i in range(r.worlds):
             w = WorldId()
