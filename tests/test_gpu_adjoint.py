@@ -35,7 +35,7 @@ def test_taped_run_matches_oracle():
         poses = r.run(f, e, v0, w0)
         cpu = helpers.oracle()
         diff.bind_scene(cpu)
-        d = cpu.default_world_def(enableSleep=False)
+        d = cpu.default_world_def(enableSleep=False, enableContinous=False)
         for i in range(r.worlds):
             w = WorldId()
             ids = (BodyId * r.boxes)()
@@ -110,7 +110,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
 
 
-@pytest.mark.parametrize("steps", [12, 40])
+@pytest.mark.parametrize("steps", [12, 40, 90])
 def test_adjoint_matches_finite_differences(steps):
     base, adj, fd = _fd_check(3, 4, steps, seed=5)
     noise = 1e-7 * abs(base) / 1e-3
