@@ -57,9 +57,10 @@ class DropBoxesRollout:
     run() builds the worlds afresh from the seed and the given parameters."""
 
     def __init__(self, api: B2Api, worlds: int, boxes: int = 16, steps: int = 120, substeps: int = 4,
-                 checkpoint: int = 10, seed: int = 11, dt: float = 1.0 / 60.0):
+                 checkpoint: int = 10, seed: int = 11, dt: float = 1.0 / 60.0, pre_steps: int = 0):
         bind(api)
         self.api, self.worlds, self.boxes, self.steps = api, worlds, boxes, steps
+        self.pre_steps = pre_steps  # untaped steps before the tape begins
         self.substeps, self.checkpoint, self.seed, self.dt = substeps, checkpoint, seed, dt
         self.batch = None
         self.start = None
@@ -116,6 +117,8 @@ class DropBoxesRollout:
                 assert n == self.boxes
         finally:
             api.use_batch(None)
+        for _ in range(self.pre_steps):
+            api.lib.b2mi_Batch_Step(self.batch, ctypes.c_float(self.dt), self.substeps)
         api.lib.b2mi_Batch_BeginTape(self.batch, self.checkpoint)
         n = api.lib.b2mi_Batch_GetTapeBodies(self.batch, None, 0)
         assert n == self.worlds * self.boxes, "tape order must be world by world, box by box"

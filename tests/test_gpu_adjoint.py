@@ -24,8 +24,8 @@ from tests import helpers
 pytestmark = pytest.mark.gpu
 
 
-def rollout(worlds=4, boxes=4, steps=24, checkpoint=5, seed=3):
-    r = diff.DropBoxesRollout(helpers.product(), worlds, boxes, steps, 4, checkpoint, seed=seed)
+def rollout(worlds=4, boxes=4, steps=24, checkpoint=5, seed=3, pre_steps=0):
+    r = diff.DropBoxesRollout(helpers.product(), worlds, boxes, steps, 4, checkpoint, seed=seed, pre_steps=pre_steps)
     return r, r.default_params()
 
 
@@ -66,8 +66,8 @@ def test_replay_reproduces_run():
         r.close()
 
 
-def _fd_check(worlds, boxes, steps, seed, h=1e-3):
-    r, (f, e, v0, w0) = rollout(worlds, boxes, steps, 5, seed)
+def _fd_check(worlds, boxes, steps, seed, h=1e-3, pre_steps=0):
+    r, (f, e, v0, w0) = rollout(worlds, boxes, steps, 5, seed, pre_steps)
     try:
         r.run(f, e, v0, w0)
         rng = np.random.default_rng(seed)
@@ -110,9 +110,9 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
 
 
-@pytest.mark.parametrize("steps", [12, 40, 90])
-def test_adjoint_matches_finite_differences(steps):
-    base, adj, fd = _fd_check(3, 4, steps, seed=5)
+@pytest.mark.parametrize("pre,steps", [(0, 12), (0, 40), (0, 90), (60, 1), (60, 3), (60, 10)])
+def test_adjoint_matches_finite_differences(pre, steps):
+    base, adj, fd = _fd_check(3, 4, steps, seed=5, pre_steps=pre)
     noise = 1e-7 * abs(base) / 1e-3
     ev = _rel(np.concatenate([np.ravel(adj[0]), np.ravel(adj[1])]), np.concatenate([np.ravel(fd[0]), np.ravel(fd[1])]))
     ep = _rel(np.concatenate([adj[2], adj[3]]), np.concatenate([fd[2], fd[3]]))
