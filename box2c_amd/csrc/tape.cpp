@@ -5,6 +5,7 @@
 // slot's contact, and every `interval` steps a checkpoint of all body poses. The backward pass
 // rebuilds the step-start poses of one checkpoint segment at a time by replaying the solver,
 // then runs the adjoint of each step of the segment from the last to the first.
+#include "b2mi.h"
 #include "launch.hpp"
 #include "world_internal.hpp"
 
@@ -449,6 +450,85 @@ B2_API void b2mi_Batch_ReplayTape( int batch, const float* startPoses, const flo
 	StepGroup* g = b2mi_internal_batch( batch );
 	B2MI_CHECK( g->tape != nullptr );
 	tapeReplay( g, startPoses, worldParams, endPoses );
+}
+
+B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, int capacity )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	Tape* t = g->tape;
+	B2MI_CHECK( t != nullptr && step >= 0 && step < (int)t->steps.size() );
+	const TapeStep& ts = t->steps[step];
+	const TapeVersion& v = t->versions[ts.version];
+	if ( slots == nullptr || capacity < v.slotCount )
+		return v.slotCount;
+	hipStream_t s = stream();
+	std::vector<WorkItem> items( v.itemCount );
+	std::vector<int> bodyList( v.bodyListCount ), slotContact( v.slotCount );
+	std::vector<int2> slotBodies( v.slotCount );
+	std::vector<DevContact> snap( v.slotCount );
+	if ( v.itemCount )
+		HIP_CHECK( hipMemcpyAsync( items.data(), v.dItems, sizeof( WorkItem ) * v.itemCount, hipMemcpyDeviceToHost, s ) );
+	if ( v.bodyListCount )
+		HIP_CHECK( hipMemcpyAsync( bodyList.data(), v.dBodyList, sizeof( int ) * v.bodyListCount, hipMemcpyDeviceToHost, s ) );
+	if ( v.slotCount )
+	{
+		HIP_CHECK( hipMemcpyAsync( slotContact.data(), v.dSlotContact, sizeof( int ) * v.slotCount, hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipMemcpyAsync( slotBodies.data(), v.dSlotBodies, sizeof( int2 ) * v.slotCount, hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipMemcpyAsync( snap.data(), ts.dSnap, sizeof( DevContact ) * v.slotCount, hipMemcpyDeviceToHost, s ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	for ( const WorkItem& wi : items )
+		for ( int gr = 1; gr < kGroups; ++gr )
+			for ( int k = wi.groupStart[gr]; k < wi.groupStart[gr + 1]; ++k )
+			{
+				int slot = wi.slotStart + k;
+				const DevContact& c = snap[slot];
+				b2mi_TapeSlot& o = slots[slot];
+				int2 lb = slotBodies[slot];
+				o.world = wi.world;
+				o.contactId = slotContact[slot];
+				o.color = gr - 1;
+				o.bodyA = lb.x < 0 ? -1 : t->simBase[wi.world] + bodyList[wi.bodyStart + lb.x];
+				o.bodyB = lb.y < 0 ? -1 : t->simBase[wi.world] + bodyList[wi.bodyStart + lb.y];
+				o.pointCount = c.manifold.pointCount;
+				o.warmFrom = c.warmFrom;
+				o.friction = c.friction;
+				o.restitution = c.restitution;
+				o.invMassA = c.invMassA;
+				o.invIA = c.invIA;
+				o.invMassB = c.invMassB;
+				o.invIB = c.invIB;
+				o.normal[0] = c.manifold.normalX;
+				o.normal[1] = c.manifold.normalY;
+				for ( int j = 0; j < 2; ++j )
+				{
+					const DevManifoldPoint& p = c.manifold.points[j];
+					o.anchorA[j][0] = p.anchorAX;
+					o.anchorA[j][1] = p.anchorAY;
+					o.anchorB[j][0] = p.anchorBX;
+					o.anchorB[j][1] = p.anchorBY;
+					o.separation[j] = p.separation;
+					o.normalImpulse[j] = p.normalImpulse;
+					o.tangentImpulse[j] = p.tangentImpulse;
+				}
+			}
+	return v.slotCount;
+}
+
+B2_API void b2mi_Batch_GetTapeWorldParams( int batch, float* params )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	B2MI_CHECK( g->tape != nullptr && g->hWorlds.size() == g->worlds.size() );
+	for ( size_t k = 0; k < g->hWorlds.size(); ++k )
+	{
+		const DevWorld& d = g->hWorlds[k];
+		float* o = params + B2MI_TAPE_WORLD_PARAMS * k;
+		o[0] = d.h, o[1] = d.inv_h, o[2] = d.gravityX, o[3] = d.gravityY;
+		o[4] = d.contactSoft[0], o[5] = d.contactSoft[1], o[6] = d.contactSoft[2];
+		o[7] = d.staticSoft[0], o[8] = d.staticSoft[1], o[9] = d.staticSoft[2];
+		o[10] = d.restitutionThreshold, o[11] = d.pushout, o[12] = d.maxLinearSpeed, o[13] = d.maxAngularSpeed;
+		o[14] = d.warmStartScale, o[15] = (float)d.subSteps;
+	}
 }
 
 B2_API void b2mi_Batch_Backward( int batch, const float* gradEnd, float* gradStart, float* gradParams )

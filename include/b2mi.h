@@ -103,11 +103,37 @@ B2_API void b2mi_World_GetTiming( b2WorldId worldId, float* msCollide, float* ms
 //
 // Poses are 7 floats per body: center of mass (x, y), rotation (cos, sin), linear velocity
 // (x, y), angular velocity; bodies in the order of b2mi_Batch_GetTapeBodies (world by world).
+// One constraint slot of a taped step, as the solver saw it (export for checks and tools).
+typedef struct b2mi_TapeSlot
+{
+	int world;		   // index of the world in the batch
+	int contactId;	   // the contact's id in its world (stable across steps)
+	int color;		   // graph colour (0..10); slots run colour by colour in ascending order
+	int bodyA, bodyB;  // index in the tape body order, -1 for a static body
+	int pointCount;
+	uint32_t warmFrom; // per point (2 bits): the previous step's point it inherited impulses from, 3 = none
+	float friction, restitution;
+	float invMassA, invIA, invMassB, invIB;
+	float normal[2];
+	float anchorA[2][2], anchorB[2][2]; // relative to the body centers of mass
+	float separation[2];
+	float normalImpulse[2], tangentImpulse[2]; // inherited, before the warm-start scale
+} b2mi_TapeSlot;
+
+// Per-world step parameters of a taped run: h, 1/h, gravity x, gravity y, contact softness
+// (bias rate, mass scale, impulse scale), static softness (same), restitution threshold,
+// pushout velocity, max linear speed, max angular speed, warm-start scale, substep count.
+#define B2MI_TAPE_WORLD_PARAMS 16
+
 B2_API void b2mi_Batch_BeginTape( int batch, int checkpointInterval );
 B2_API void b2mi_Batch_EndTape( int batch );
 B2_API int b2mi_Batch_GetTapeStepCount( int batch );
 B2_API int b2mi_Batch_GetTapeBodies( int batch, b2BodyId* bodies, int capacity );
 B2_API void b2mi_Batch_GetTapePoses( int batch, float* poses );
+// The slots of taped step `step` (returns their count; writes at most capacity).
+B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, int capacity );
+// B2MI_TAPE_WORLD_PARAMS floats per world.
+B2_API void b2mi_Batch_GetTapeWorldParams( int batch, float* params );
 // Runs the taped solver steps again from startPoses (the same manifolds and contact sets), with
 // every contact's friction and restitution replaced by per-world values when worldParams
 // (friction, restitution per world) is given; endPoses receives the final poses. This is the

@@ -141,3 +141,40 @@ def test_autograd_function():
         assert float(tv.grad.abs().sum()) > 0.0
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("pre,steps", [(0, 40), (60, 10), (0, 90)])
+def test_adjoint_matches_fp64_autograd(pre, steps):
+    """The float32 GPU adjoint against torch autograd through an FP64 restatement of the same
+    taped steps (tests/fp64_replay.py). Bars: final poses within 1e-3 of the float32 run; relative
+    gradient error below 1e-2 (velocities and start centers) and 2e-2 (friction, restitution)."""
+    import torch
+    from tests import fp64_replay
+
+    r, (f, e, v0, w0) = rollout(3, 4, steps, 5, 5, pre)
+    try:
+        final = r.run(f, e, v0, w0)
+        rng = np.random.default_rng(1)
+        weights = rng.normal(size=final.shape).astype(np.float32)
+        weights[:, :, 2:4] = 0.0
+        gf, ge, gv, gw, gs = r.backward(weights)
+        steps_, wp, mass, inertia, world_of = fp64_replay.export(r)
+        start = torch.tensor(r.start.reshape(-1, 7), dtype=torch.float64, requires_grad=True)
+        params = torch.tensor(np.stack([f, e], 1), dtype=torch.float64, requires_grad=True)
+        out = fp64_replay.replay(steps_, wp, mass, inertia, world_of, start, params)
+        err_pose = float(np.abs(out.detach().numpy() - final.reshape(-1, 7)).max())
+        loss = (torch.tensor(weights.reshape(-1, 7), dtype=torch.float64) * out).sum()
+        g_start, g_params = torch.autograd.grad(loss, [start, params])
+        g_start = g_start.numpy().reshape(final.shape)
+        g_params = g_params.numpy()
+        ev = _rel(np.concatenate([np.ravel(gs[:, :, 4:7]), np.ravel(gs[:, :, 0:2])]),
+                  np.concatenate([np.ravel(g_start[:, :, 4:7]), np.ravel(g_start[:, :, 0:2])]))
+        ep = _rel(np.concatenate([gf, ge]), np.concatenate([g_params[:, 0], g_params[:, 1]]))
+        msg = (f"pose err {err_pose:.2e}, velocity/center grad rel err {ev:.2e}, parameter grad rel err {ep:.2e}\n"
+               f"gpu friction {gf} fp64 {g_params[:, 0]}\ngpu restitution {ge} fp64 {g_params[:, 1]}")
+        print(msg)
+        assert err_pose < 1e-3, msg
+        assert ev < 1e-2, msg
+        assert ep < 2e-2, msg
+    finally:
+        r.close()
