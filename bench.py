@@ -106,6 +106,35 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, au
             "ms_host": t_host / steps, "work_items": items, "launch_shape": shape}
 
 
+def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed: int = 11):
+    """Forward (taped) rollout of `worlds` drop-box worlds and the adjoint of the loss
+    sum |p_final - target|^2 with respect to friction, restitution and initial velocities."""
+    import numpy as np
+    from box2c_amd import diff
+    r = diff.DropBoxesRollout(api, worlds, boxes, steps, 4, checkpoint, seed=seed)
+    f, e, v0, w0 = r.default_params()
+    try:
+        r.run(f, e, v0, w0)  # warm-up: kernels, allocations
+        final = r.run(f, e, v0, w0)
+        target = np.random.default_rng(seed).uniform(-5, 5, size=final[:, :, 0:2].shape).astype(np.float32)
+        g = np.zeros_like(final)
+        g[:, :, 0:2] = 2.0 * (final[:, :, 0:2] - target)
+        gf, ge, gv, gw, _ = r.backward(g)
+        fwd, bwd = r.forward_seconds, r.backward_seconds
+        finite = bool(np.isfinite(gf).all() and np.isfinite(ge).all() and np.isfinite(gv).all() and np.isfinite(gw).all())
+    finally:
+        r.close()
+    n = worlds * boxes
+    return {
+        "workload": f"drop_boxes_{worlds}x{boxes}_{steps}steps_adjoint_f32", "bodies": n, "steps": steps,
+        "checkpoint_interval": checkpoint, "forward_s": fwd, "backward_s": bwd, "adjoint_over_forward": bwd / fwd,
+        "value_forward": n * 4 * steps / fwd, "value_fwd_plus_adjoint": n * 4 * steps / (fwd + bwd),
+        "unit": "bodies×substeps/s", "gradients_finite": finite,
+        "note": "forward = the taped GPU step loop (narrowphase, solver, recording); backward = replay of each "
+                "checkpoint segment plus the adjoint kernels; manifolds held fixed per step; continuous collision off",
+    }
+
+
 def cpu_baseline_batch(threads: int, n_worlds: int, sample_steps: int, warmup: int, seed: int = 7):
     """The reference on the same kind of worlds with world-level parallelism (each thread steps
     its own worlds, the reference's best schedule for many small worlds). The reference holds at
@@ -196,6 +225,7 @@ def main():
     ap.add_argument("--batch-worlds", type=int, default=4096, help="worlds of the batched leg (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-cpu-steps", type=int, default=1000)
+    ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the adjoint leg (0 = skip it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain copy kernel of known size (for HBM counter scaling)")
     args = ap.parse_args()
@@ -274,6 +304,8 @@ def main():
             if cb:
                 batched["vs_cpu"] = bval / cb["value"]
         out["batched"] = batched
+    if args.adjoint_worlds > 0:
+        out["adjoint"] = run_adjoint(api, args.adjoint_worlds, 16, 120, 10)
     print(json.dumps(out), flush=True)
 
 

@@ -12,6 +12,7 @@ Tensors cross the boundary on the host (float32); the simulation itself runs on 
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 
@@ -64,6 +65,7 @@ class DropBoxesRollout:
         self.substeps, self.checkpoint, self.seed, self.dt = substeps, checkpoint, seed, dt
         self.batch = None
         self.start = None
+        self.forward_seconds = self.backward_seconds = 0.0
 
     def world_def(self):
         # continuous collision is off: falling boxes reach time-of-impact speeds, and the time of
@@ -131,9 +133,12 @@ class DropBoxesRollout:
             v = api.lib.b2Body_GetLinearVelocity(ids[k])
             self.start[k] = (c.x, c.y, q.c, q.s, v.x, v.y, api.lib.b2Body_GetAngularVelocity(ids[k]))
         self.start = self.start.reshape(self.worlds, self.boxes, POSE)
+        t0 = time.perf_counter()
         for _ in range(self.steps):
             api.lib.b2mi_Batch_Step(self.batch, ctypes.c_float(self.dt), self.substeps)
         out = np.zeros((n, POSE), np.float32)
+        api.lib.b2mi_Batch_GetTapePoses(self.batch, out.ctypes.data)  # synchronizes
+        self.forward_seconds = time.perf_counter() - t0
         api.lib.b2mi_Batch_GetTapePoses(self.batch, out.ctypes.data)
         return out.reshape(self.worlds, self.boxes, POSE)
 
@@ -143,7 +148,9 @@ class DropBoxesRollout:
         g = np.ascontiguousarray(grad_end, np.float32).reshape(-1, POSE)
         gs = np.zeros_like(g)
         gp = np.zeros((self.worlds, 2), np.float32)
+        t0 = time.perf_counter()
         self.api.lib.b2mi_Batch_Backward(self.batch, g.ctypes.data, gs.ctypes.data, gp.ctypes.data)
+        self.backward_seconds = time.perf_counter() - t0
         gs = gs.reshape(self.worlds, self.boxes, POSE)
         return gp[:, 0].copy(), gp[:, 1].copy(), gs[:, :, 4:6].copy(), gs[:, :, 6].copy(), gs
 
