@@ -132,7 +132,6 @@ B2_API b2WorldId b2mi_Batch_CreateWorld( int batch, const b2WorldDef* def )
 	w->groupIndex = (int)g->worlds.size();
 	g->worlds.push_back( w );
 	g->structureDirty = true;
-	g->collideListDirty = true;
 	return b2WorldId{ (uint16_t)( w->slot + 1 ), w->revision };
 }
 

@@ -85,6 +85,8 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 	if ( i >= count )
 		return;
 	int2 item = list[i];
+	if ( item.x < 0 )
+		return; // unused entry of a world's region
 	const DevWorld& W = worlds[item.x];
 	DevContact* c = W.contacts + item.y;
 	const DevShape& A = W.shapes[c->shapeIdA];

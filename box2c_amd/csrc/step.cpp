@@ -5,10 +5,16 @@
 #include "world_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 
 namespace b2mi
@@ -297,7 +303,7 @@ static void createContact( World* w, Shape* shapeA, Shape* shapeB )
 	init.friction = sqrtf( shapeA->friction * shapeB->friction );
 	init.restitution = shapeA->restitution > shapeB->restitution ? shapeA->restitution : shapeB->restitution;
 	w->pendingInits.push_back( init );
-	w->topologyDirty = true;
+	w->contactsDirty = true;
 }
 
 static void removeFromList( World* w, std::vector<int>& list, int localIndex )
@@ -388,7 +394,8 @@ void destroyContactImpl( World* w, Contact* c )
 			b.headContactKey = edge.nextKey;
 		b.contactCount -= 1;
 	}
-	if ( c->colorIndex != -1 )
+	bool wasTouching = c->colorIndex != -1;
+	if ( wasTouching )
 		removeContactFromGraph( w, c->edges[0].bodyId, c->edges[1].bodyId, c->colorIndex, c->localIndex );
 	else
 		removeFromList( w, w->nonTouching, c->localIndex );
@@ -397,7 +404,9 @@ void destroyContactImpl( World* w, Contact* c )
 	c->colorIndex = -1;
 	c->localIndex = -1;
 	w->contactIdPool.free( contactId );
-	w->topologyDirty = true;
+	w->contactsDirty = true;
+	if ( wasTouching )
+		w->topologyDirty = true;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -498,201 +507,190 @@ static int findRoot( std::vector<int>& parent, int x )
 	return x;
 }
 
-static void buildWorkItems( StepGroup* g )
+// The work items of one world: islands (connected components of dynamic bodies through
+// touching contacts; static and kinematic bodies do not join islands) packed in order up to
+// `target` constraint slots, each item's slots grouped as overflow first, then colours 0..10.
+// Writes w->solver with indices local to its lists; runs on any host thread (touches this
+// world only).
+static void buildWorldItems( World* w, int target, int budget )
 {
-	g->hItems.clear();
-	g->hBodyList.clear();
-	g->hSlotContact.clear();
-	g->hSlotBodies.clear();
-	g->maxGroupSize = 0;
-	g->maxItemLds = 0;
-	g->itemsVersion += 1;
-	int target = g->targetSlots > 0 ? g->targetSlots : 256;
-
-	for ( World* w : g->worlds )
+	SolverLists& L = w->solver;
+	L.items.clear();
+	L.bodyList.clear();
+	L.slotContact.clear();
+	L.slotBodies.clear();
+	L.maxGroupSize = 0;
+	L.maxItemLds = 0;
+	int na = (int)w->awakeSimBody.size();
+	if ( na == 0 )
 	{
-		int na = (int)w->awakeSimBody.size();
-		if ( na == 0 )
-			continue;
-		std::vector<int> parent( na );
-		for ( int i = 0; i < na; ++i )
-			parent[i] = i;
-		auto simOfBody = [&]( int bodyId ) -> int {
-			const Body& b = w->bodies[bodyId];
-			return b.setIndex == 2 && b.type == b2_dynamicBody ? b.localIndex : -1;
-		};
-		for ( int c = 0; c < kColorCount; ++c )
-			for ( int id : w->colors[c] )
-			{
-				const Contact& ct = w->contacts[id];
-				int a = simOfBody( ct.edges[0].bodyId ), b = simOfBody( ct.edges[1].bodyId );
-				if ( a >= 0 && b >= 0 )
-				{
-					int ra = findRoot( parent, a ), rb = findRoot( parent, b );
-					if ( ra != rb )
-						parent[std::max( ra, rb )] = std::min( ra, rb );
-				}
-			}
-		// components in order of their smallest sim index; kinematic bodies alone
-		std::vector<int> compOf( na, -1 );
-		std::vector<Component> comps;
-		for ( int i = 0; i < na; ++i )
-		{
-			int r = findRoot( parent, i );
-			if ( compOf[r] < 0 )
-			{
-				compOf[r] = (int)comps.size();
-				comps.push_back( Component{} );
-			}
-			compOf[i] = compOf[r];
-			comps[compOf[i]].bodies.push_back( i );
-		}
-		// a contact belongs to the component of its dynamic body
-		auto contactComp = [&]( const Contact& ct ) {
-			int a = simOfBody( ct.edges[0].bodyId );
-			if ( a < 0 )
-				a = simOfBody( ct.edges[1].bodyId );
-			return compOf[a];
-		};
-		std::vector<std::vector<int>> compSlots[kColorCount];
-		for ( int c = 0; c < kColorCount; ++c )
-			compSlots[c].assign( comps.size(), std::vector<int>() );
-		for ( int c = 0; c < kColorCount; ++c )
-			for ( int id : w->colors[c] )
-			{
-				int k = contactComp( w->contacts[id] );
-				compSlots[c][k].push_back( id );
-				comps[k].slotCount += 1;
-			}
-
-		// greedy packing in component order
-		size_t ci = 0;
-		while ( ci < comps.size() )
-		{
-			size_t cj = ci;
-			int slots = 0, bodies = 0;
-			do
-			{
-				slots += comps[cj].slotCount;
-				bodies += (int)comps[cj].bodies.size();
-				cj += 1;
-			} while ( cj < comps.size() && slots + comps[cj].slotCount <= target && bodies + (int)comps[cj].bodies.size() <= 4 * target );
-
-			WorkItem wi{};
-			wi.world = w->groupIndex;
-			wi.bodyStart = (int)g->hBodyList.size();
-			std::vector<int> localOf( 0 );
-			// owned bodies
-			std::vector<int> items;
-			for ( size_t k = ci; k < cj; ++k )
-				for ( int b : comps[k].bodies )
-					g->hBodyList.push_back( b );
-			wi.ownedCount = (int)g->hBodyList.size() - wi.bodyStart;
-			// local index map (owned, then kinematic shadows on demand)
-			std::vector<std::pair<int, int>> shadowMap;
-			auto localIndex = [&]( int bodyId ) -> int {
-				const Body& b = w->bodies[bodyId];
-				if ( b.setIndex != 2 )
-					return -1;
-				if ( b.type == b2_dynamicBody )
-				{
-					// position within the owned range: search (components are small) via map
-					return -2 - b.localIndex; // resolved below
-				}
-				for ( auto& pr : shadowMap )
-					if ( pr.first == b.localIndex )
-						return pr.second;
-				int li = (int)g->hBodyList.size() - wi.bodyStart;
-				g->hBodyList.push_back( b.localIndex );
-				shadowMap.push_back( { b.localIndex, li } );
-				return li;
-			};
-			std::vector<int> ownedPos;
-			{
-				// sim index -> owned position for this work item
-				int n = wi.ownedCount;
-				ownedPos.clear();
-				for ( int k = 0; k < n; ++k )
-					ownedPos.push_back( g->hBodyList[wi.bodyStart + k] );
-			}
-			std::vector<int> posOfSim;
-			int maxSim = 0;
-			for ( int s : ownedPos )
-				maxSim = std::max( maxSim, s );
-			posOfSim.assign( maxSim + 1, -1 );
-			for ( int k = 0; k < (int)ownedPos.size(); ++k )
-				posOfSim[ownedPos[k]] = k;
-			auto resolve = [&]( int li ) { return li <= -2 ? posOfSim[-2 - li] : li; };
-
-			wi.slotStart = (int)g->hSlotContact.size();
-			// group 0 = overflow (colour 11) then colours 0..10
-			for ( int gidx = 0; gidx < kGroups; ++gidx )
-			{
-				int color = gidx == 0 ? kOverflow : gidx - 1;
-				wi.groupStart[gidx] = (int)g->hSlotContact.size() - wi.slotStart;
-				for ( size_t k = ci; k < cj; ++k )
-					for ( int id : compSlots[color][k] )
-					{
-						const Contact& ct = w->contacts[id];
-						int la = resolve( localIndex( ct.edges[0].bodyId ) );
-						int lb = resolve( localIndex( ct.edges[1].bodyId ) );
-						g->hSlotContact.push_back( id );
-						g->hSlotBodies.push_back( make_int2( la, lb ) );
-					}
-				int size = (int)g->hSlotContact.size() - wi.slotStart - wi.groupStart[gidx];
-				g->maxGroupSize = std::max( g->maxGroupSize, size );
-			}
-			wi.groupStart[kGroups] = (int)g->hSlotContact.size() - wi.slotStart;
-			wi.bodyCount = (int)g->hBodyList.size() - wi.bodyStart;
-			{
-				int budget = g->ldsBudget < 0 ? kDefaultItemLdsBytes : std::min( g->ldsBudget, kMaxItemLdsBytes );
-				int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
-				int bodyBytes = itemBodyLdsBytes( wi.bodyCount );
-				if ( bytes <= budget )
-				{
-					wi.store = kStoreLds;
-					wi.ldsBytes = bytes;
-				}
-				else if ( budget > 0 && bodyBytes <= budget )
-				{
-					wi.store = kStoreBodiesLds;
-					wi.ldsBytes = bodyBytes;
-				}
-				else
-				{
-					wi.store = kStoreHbm;
-					wi.ldsBytes = 0;
-				}
-				g->maxItemLds = std::max( g->maxItemLds, wi.ldsBytes );
-			}
-			g->maxGroupSize = std::max( g->maxGroupSize, wi.ownedCount );
-			g->hItems.push_back( wi );
-			ci = cj;
-		}
 		w->topologyDirty = false;
+		return;
 	}
-	// Longest items first: the hardware dispatches workgroups in order, so the few large islands
-	// of an uneven batch start at once and the small ones fill the CUs behind them. Items are
-	// independent; their order does not change any result.
-	std::stable_sort( g->hItems.begin(), g->hItems.end(), []( const WorkItem& a, const WorkItem& b ) {
-		int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
-		return ca > cb;
-	} );
-	g->itemCount = (int)g->hItems.size();
-	g->slotCount = (int)g->hSlotContact.size();
-	g->bodyListCount = (int)g->hBodyList.size();
-
-	// every index the solver kernel dereferences is checked here, on the host
-	for ( const WorkItem& wi : g->hItems )
+	std::vector<int> parent( na );
+	for ( int i = 0; i < na; ++i )
+		parent[i] = i;
+	auto simOfBody = [&]( int bodyId ) -> int {
+		const Body& b = w->bodies[bodyId];
+		return b.setIndex == 2 && b.type == b2_dynamicBody ? b.localIndex : -1;
+	};
+	for ( int c = 0; c < kColorCount; ++c )
+		for ( int id : w->colors[c] )
+		{
+			const Contact& ct = w->contacts[id];
+			int a = simOfBody( ct.edges[0].bodyId ), b = simOfBody( ct.edges[1].bodyId );
+			if ( a >= 0 && b >= 0 )
+			{
+				int ra = findRoot( parent, a ), rb = findRoot( parent, b );
+				if ( ra != rb )
+					parent[std::max( ra, rb )] = std::min( ra, rb );
+			}
+		}
+	// components in order of their smallest sim index; kinematic bodies alone
+	std::vector<int> compOf( na, -1 );
+	std::vector<Component> comps;
+	for ( int i = 0; i < na; ++i )
 	{
-		World* w = g->worlds[wi.world];
-		int na = (int)w->awakeSimBody.size();
-		B2MI_CHECK( wi.bodyStart >= 0 && wi.bodyStart + wi.bodyCount <= g->bodyListCount );
+		int r = findRoot( parent, i );
+		if ( compOf[r] < 0 )
+		{
+			compOf[r] = (int)comps.size();
+			comps.push_back( Component{} );
+		}
+		compOf[i] = compOf[r];
+		comps[compOf[i]].bodies.push_back( i );
+	}
+	// a contact belongs to the component of its dynamic body
+	auto contactComp = [&]( const Contact& ct ) {
+		int a = simOfBody( ct.edges[0].bodyId );
+		if ( a < 0 )
+			a = simOfBody( ct.edges[1].bodyId );
+		return compOf[a];
+	};
+	std::vector<std::vector<int>> compSlots[kColorCount];
+	for ( int c = 0; c < kColorCount; ++c )
+		compSlots[c].assign( comps.size(), std::vector<int>() );
+	for ( int c = 0; c < kColorCount; ++c )
+		for ( int id : w->colors[c] )
+		{
+			int k = contactComp( w->contacts[id] );
+			compSlots[c][k].push_back( id );
+			comps[k].slotCount += 1;
+		}
+
+	// greedy packing in component order
+	size_t ci = 0;
+	while ( ci < comps.size() )
+	{
+		size_t cj = ci;
+		int slots = 0, bodies = 0;
+		do
+		{
+			slots += comps[cj].slotCount;
+			bodies += (int)comps[cj].bodies.size();
+			cj += 1;
+		} while ( cj < comps.size() && slots + comps[cj].slotCount <= target && bodies + (int)comps[cj].bodies.size() <= 4 * target );
+
+		WorkItem wi{};
+		wi.world = w->groupIndex;
+		wi.bodyStart = (int)L.bodyList.size();
+		std::vector<int> localOf( 0 );
+		// owned bodies
+		std::vector<int> items;
+		for ( size_t k = ci; k < cj; ++k )
+			for ( int b : comps[k].bodies )
+				L.bodyList.push_back( b );
+		wi.ownedCount = (int)L.bodyList.size() - wi.bodyStart;
+		// local index map (owned, then kinematic shadows on demand)
+		std::vector<std::pair<int, int>> shadowMap;
+		auto localIndex = [&]( int bodyId ) -> int {
+			const Body& b = w->bodies[bodyId];
+			if ( b.setIndex != 2 )
+				return -1;
+			if ( b.type == b2_dynamicBody )
+			{
+				// position within the owned range: search (components are small) via map
+				return -2 - b.localIndex; // resolved below
+			}
+			for ( auto& pr : shadowMap )
+				if ( pr.first == b.localIndex )
+					return pr.second;
+			int li = (int)L.bodyList.size() - wi.bodyStart;
+			L.bodyList.push_back( b.localIndex );
+			shadowMap.push_back( { b.localIndex, li } );
+			return li;
+		};
+		std::vector<int> ownedPos;
+		{
+			// sim index -> owned position for this work item
+			int n = wi.ownedCount;
+			ownedPos.clear();
+			for ( int k = 0; k < n; ++k )
+				ownedPos.push_back( L.bodyList[wi.bodyStart + k] );
+		}
+		std::vector<int> posOfSim;
+		int maxSim = 0;
+		for ( int s : ownedPos )
+			maxSim = std::max( maxSim, s );
+		posOfSim.assign( maxSim + 1, -1 );
+		for ( int k = 0; k < (int)ownedPos.size(); ++k )
+			posOfSim[ownedPos[k]] = k;
+		auto resolve = [&]( int li ) { return li <= -2 ? posOfSim[-2 - li] : li; };
+
+		wi.slotStart = (int)L.slotContact.size();
+		// group 0 = overflow (colour 11) then colours 0..10
+		for ( int gidx = 0; gidx < kGroups; ++gidx )
+		{
+			int color = gidx == 0 ? kOverflow : gidx - 1;
+			wi.groupStart[gidx] = (int)L.slotContact.size() - wi.slotStart;
+			for ( size_t k = ci; k < cj; ++k )
+				for ( int id : compSlots[color][k] )
+				{
+					const Contact& ct = w->contacts[id];
+					int la = resolve( localIndex( ct.edges[0].bodyId ) );
+					int lb = resolve( localIndex( ct.edges[1].bodyId ) );
+					L.slotContact.push_back( id );
+					L.slotBodies.push_back( make_int2( la, lb ) );
+				}
+			int size = (int)L.slotContact.size() - wi.slotStart - wi.groupStart[gidx];
+			L.maxGroupSize = std::max( L.maxGroupSize, size );
+		}
+		wi.groupStart[kGroups] = (int)L.slotContact.size() - wi.slotStart;
+		wi.bodyCount = (int)L.bodyList.size() - wi.bodyStart;
+		{
+			int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
+			int bodyBytes = itemBodyLdsBytes( wi.bodyCount );
+			if ( bytes <= budget )
+			{
+				wi.store = kStoreLds;
+				wi.ldsBytes = bytes;
+			}
+			else if ( budget > 0 && bodyBytes <= budget )
+			{
+				wi.store = kStoreBodiesLds;
+				wi.ldsBytes = bodyBytes;
+			}
+			else
+			{
+				wi.store = kStoreHbm;
+				wi.ldsBytes = 0;
+			}
+			L.maxItemLds = std::max( L.maxItemLds, wi.ldsBytes );
+		}
+		L.maxGroupSize = std::max( L.maxGroupSize, wi.ownedCount );
+		L.items.push_back( wi );
+		ci = cj;
+	}
+	// every index the solver kernel dereferences is checked here, on the host
+	int na2 = (int)w->awakeSimBody.size();
+	for ( const WorkItem& wi : L.items )
+	{
+		B2MI_CHECK( wi.bodyStart >= 0 && wi.bodyStart + wi.bodyCount <= (int)L.bodyList.size() );
 		B2MI_CHECK( 0 <= wi.ownedCount && wi.ownedCount <= wi.bodyCount );
 		for ( int i = 0; i < wi.bodyCount; ++i )
-			B2MI_CHECK( 0 <= g->hBodyList[wi.bodyStart + i] && g->hBodyList[wi.bodyStart + i] < na );
-		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= g->slotCount );
-		B2MI_CHECK( g->maxItemLds <= kMaxItemLdsBytes && wi.ldsBytes <= g->maxItemLds );
+			B2MI_CHECK( 0 <= L.bodyList[wi.bodyStart + i] && L.bodyList[wi.bodyStart + i] < na2 );
+		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= (int)L.slotContact.size() );
+		B2MI_CHECK( wi.ldsBytes <= kMaxItemLdsBytes );
 		B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
 					( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount ) ) ||
 					( wi.store == kStoreLds && wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) );
@@ -700,75 +698,166 @@ static void buildWorkItems( StepGroup* g )
 			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
 		for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
 		{
-			int slot = wi.slotStart + k;
-			int2 lb = g->hSlotBodies[slot];
+			int2 lb = L.slotBodies[wi.slotStart + k];
 			B2MI_CHECK( -1 <= lb.x && lb.x < wi.bodyCount && -1 <= lb.y && lb.y < wi.bodyCount );
-			B2MI_CHECK( 0 <= g->hSlotContact[slot] && g->hSlotContact[slot] < w->capContacts );
+			B2MI_CHECK( 0 <= L.slotContact[wi.slotStart + k] && L.slotContact[wi.slotStart + k] < w->capContacts );
 		}
 	}
-
-	hipStream_t s = stream();
-	grow( g->dItems, g->capItems, std::max( g->itemCount, 1 ) );
-	grow( g->dBodyList, g->capBodyList, std::max( g->bodyListCount, 1 ) );
-	int slotCap = g->capSlots;
-	grow( g->dSlotContact, slotCap, std::max( g->slotCount, 1 ) );
-	grow( g->dSlotBodies, g->capSlots, std::max( g->slotCount, 1 ) );
-	int consNeeded = std::max( g->slotCount, 1 );
-	if ( g->capCons < consNeeded || g->dCons == nullptr )
-	{
-		if ( g->dCons )
-			HIP_CHECK( hipFree( g->dCons ) );
-		g->capCons = std::max( 64, consNeeded + consNeeded / 4 );
-		HIP_CHECK( hipMalloc( (void**)&g->dCons, sizeof( float ) * (size_t)cfFloatCount * g->capCons ) );
-	}
-	grow( g->dPriv, g->capPriv, 2 * std::max( g->bodyListCount, 1 ) );
-	if ( g->itemCount )
-		HIP_CHECK( hipMemcpyAsync( g->dItems, g->hItems.data(), sizeof( WorkItem ) * g->itemCount, hipMemcpyHostToDevice, s ) );
-	if ( g->bodyListCount )
-		HIP_CHECK( hipMemcpyAsync( g->dBodyList, g->hBodyList.data(), sizeof( int ) * g->bodyListCount, hipMemcpyHostToDevice, s ) );
-	if ( g->slotCount )
-	{
-		HIP_CHECK( hipMemcpyAsync( g->dSlotContact, g->hSlotContact.data(), sizeof( int ) * g->slotCount, hipMemcpyHostToDevice, s ) );
-		HIP_CHECK( hipMemcpyAsync( g->dSlotBodies, g->hSlotBodies.data(), sizeof( int2 ) * g->slotCount, hipMemcpyHostToDevice, s ) );
-	}
+	w->topologyDirty = false;
+	w->solverPending = true;
 }
 
-static void buildCollideList( StepGroup* g )
+// Every contact of the world (touching ones in the colours, the others in nonTouching),
+// listed body by body in awake-set order so that the contacts of neighbouring bodies sit next
+// to each other: threads of one wave then share shape and body cache lines. The narrowphase's
+// state changes are sorted by contact id afterwards, so this order does not affect results.
+static void buildWorldCollide( World* w )
 {
-	// Every contact of the world (touching ones in the colours, the others in nonTouching),
-	// listed body by body in awake-set order so that the contacts of neighbouring bodies sit
-	// next to each other: threads of one wave then share shape and body cache lines. The
-	// narrowphase's state changes are sorted by contact id afterwards, so this order does not
-	// affect results.
-	g->hCollide.clear();
+	w->collideIds.clear();
+	for ( int si = 0; si < (int)w->awakeSimBody.size(); ++si )
+	{
+		const Body& b = w->bodies[w->awakeSimBody[si]];
+		int key = b.headContactKey;
+		while ( key != -1 )
+		{
+			int cid = key >> 1, edge = key & 1;
+			const Contact& c = w->contacts[cid];
+			const Body& other = w->bodies[c.edges[edge ^ 1].bodyId];
+			if ( other.setIndex != kAwakeSet || other.localIndex > si )
+				w->collideIds.push_back( cid );
+			key = c.edges[edge].nextKey;
+		}
+	}
+	size_t expected = w->nonTouching.size();
+	for ( int c = 0; c < kColorCount; ++c )
+		expected += w->colors[c].size();
+	B2MI_CHECK( w->collideIds.size() == expected );
+	w->contactsDirty = false;
+	w->collidePending = true;
+}
+
+static int withSlack( int n )
+{
+	return n + n / 4 + 8;
+}
+
+// Assigns every world a region (with slack) in the launch arrays and sizes the device arrays.
+static void layoutGroup( StepGroup* g )
+{
+	int bodies = 0, slots = 0, collide = 0;
 	for ( World* w : g->worlds )
 	{
-		size_t first = g->hCollide.size();
-		for ( int si = 0; si < (int)w->awakeSimBody.size(); ++si )
-		{
-			const Body& b = w->bodies[w->awakeSimBody[si]];
-			int key = b.headContactKey;
-			while ( key != -1 )
-			{
-				int cid = key >> 1, edge = key & 1;
-				const Contact& c = w->contacts[cid];
-				const Body& other = w->bodies[c.edges[edge ^ 1].bodyId];
-				if ( other.setIndex != kAwakeSet || other.localIndex > si )
-					g->hCollide.push_back( make_int2( w->groupIndex, cid ) );
-				key = c.edges[edge].nextKey;
-			}
-		}
-		size_t expected = w->nonTouching.size();
-		for ( int c = 0; c < kColorCount; ++c )
-			expected += w->colors[c].size();
-		B2MI_CHECK( g->hCollide.size() - first == expected );
+		w->bodyRegion = Region{ bodies, withSlack( (int)w->solver.bodyList.size() ) };
+		w->slotRegion = Region{ slots, withSlack( (int)w->solver.slotContact.size() ) };
+		w->collideRegion = Region{ collide, withSlack( (int)w->collideIds.size() ) };
+		bodies += w->bodyRegion.cap;
+		slots += w->slotRegion.cap;
+		collide += w->collideRegion.cap;
+		w->solverPending = true;
+		w->collidePending = true;
 	}
-	g->collideCount = (int)g->hCollide.size();
-	grow( g->dCollideList, g->capCollide, std::max( g->collideCount, 1 ) );
-	if ( g->collideCount )
-		HIP_CHECK( hipMemcpyAsync( g->dCollideList, g->hCollide.data(), sizeof( int2 ) * g->collideCount, hipMemcpyHostToDevice,
-								   stream() ) );
-	grow( g->dChanges, g->capChanges, std::max( g->collideCount, 1 ) );
+	g->bodyListCount = bodies;
+	g->slotCount = slots;
+	g->collideCount = collide;
+	g->hBodyList.assign( bodies, 0 );
+	g->hSlotContact.assign( slots, 0 );
+	g->hSlotBodies.assign( slots, make_int2( -1, -1 ) );
+	g->hCollide.assign( collide, make_int2( -1, -1 ) );
+	grow( g->dBodyList, g->capBodyList, std::max( bodies, 1 ) );
+	int slotCap = g->capSlots;
+	grow( g->dSlotContact, slotCap, std::max( slots, 1 ) );
+	grow( g->dSlotBodies, g->capSlots, std::max( slots, 1 ) );
+	grow( g->dCollideList, g->capCollide, std::max( collide, 1 ) );
+	grow( g->dChanges, g->capChanges, std::max( collide, 1 ) );
+	if ( g->capCons < std::max( slots, 1 ) || g->dCons == nullptr )
+	{
+		if ( g->dCons )
+		{
+			HIP_CHECK( hipStreamSynchronize( stream() ) );
+			HIP_CHECK( hipFree( g->dCons ) );
+		}
+		g->capCons = std::max( 64, slots + slots / 4 );
+		HIP_CHECK( hipMalloc( (void**)&g->dCons, sizeof( float ) * (size_t)cfFloatCount * g->capCons ) );
+	}
+	grow( g->dPriv, g->capPriv, 2 * std::max( bodies, 1 ) );
+	g->layoutValid = true;
+}
+
+template <typename T>
+static void uploadSpan( T* dst, const std::vector<T>& src, int first, int last, hipStream_t s )
+{
+	if ( last > first )
+		HIP_CHECK( hipMemcpyAsync( dst + first, src.data() + first, sizeof( T ) * (size_t)( last - first ), hipMemcpyHostToDevice, s ) );
+}
+
+// Writes the pending worlds' lists into their regions (absolute indices) and uploads the span
+// they cover; rebuilds the launch-ordered item array when any world's items changed.
+static void publishLists( StepGroup* g, bool itemsChanged )
+{
+	hipStream_t s = stream();
+	int bFirst = INT32_MAX, bLast = 0, sFirst = INT32_MAX, sLast = 0, cFirst = INT32_MAX, cLast = 0;
+	for ( World* w : g->worlds )
+	{
+		if ( w->solverPending )
+		{
+			const SolverLists& L = w->solver;
+			Region br = w->bodyRegion, sr = w->slotRegion;
+			std::copy( L.bodyList.begin(), L.bodyList.end(), g->hBodyList.begin() + br.base );
+			std::copy( L.slotContact.begin(), L.slotContact.end(), g->hSlotContact.begin() + sr.base );
+			std::copy( L.slotBodies.begin(), L.slotBodies.end(), g->hSlotBodies.begin() + sr.base );
+			bFirst = std::min( bFirst, br.base );
+			bLast = std::max( bLast, br.base + (int)L.bodyList.size() );
+			sFirst = std::min( sFirst, sr.base );
+			sLast = std::max( sLast, sr.base + (int)L.slotContact.size() );
+			w->solverPending = false;
+		}
+		if ( w->collidePending )
+		{
+			Region cr = w->collideRegion;
+			for ( int k = 0; k < cr.cap; ++k )
+				g->hCollide[cr.base + k] =
+					k < (int)w->collideIds.size() ? make_int2( w->groupIndex, w->collideIds[k] ) : make_int2( -1, -1 );
+			cFirst = std::min( cFirst, cr.base );
+			cLast = std::max( cLast, cr.base + cr.cap );
+			w->collidePending = false;
+		}
+	}
+	uploadSpan( g->dBodyList, g->hBodyList, bFirst, bLast, s );
+	uploadSpan( g->dSlotContact, g->hSlotContact, sFirst, sLast, s );
+	uploadSpan( g->dSlotBodies, g->hSlotBodies, sFirst, sLast, s );
+	uploadSpan( g->dCollideList, g->hCollide, cFirst, cLast, s );
+
+	if ( itemsChanged )
+	{
+		g->hItems.clear();
+		g->maxGroupSize = 0;
+		g->maxItemLds = 0;
+		for ( World* w : g->worlds )
+		{
+			for ( WorkItem wi : w->solver.items )
+			{
+				wi.bodyStart += w->bodyRegion.base;
+				wi.slotStart += w->slotRegion.base;
+				g->hItems.push_back( wi );
+			}
+			g->maxGroupSize = std::max( g->maxGroupSize, w->solver.maxGroupSize );
+			g->maxItemLds = std::max( g->maxItemLds, w->solver.maxItemLds );
+		}
+		// Longest items first: the hardware dispatches workgroups in order, so the few large
+		// islands of an uneven batch start at once and the small ones fill the CUs behind them.
+		// Items are independent; their order does not change any result.
+		std::stable_sort( g->hItems.begin(), g->hItems.end(), []( const WorkItem& a, const WorkItem& b ) {
+			int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
+			return ca > cb;
+		} );
+		g->itemCount = (int)g->hItems.size();
+		g->itemsVersion += 1;
+		grow( g->dItems, g->capItems, std::max( g->itemCount, 1 ) );
+		if ( g->itemCount )
+			HIP_CHECK( hipMemcpyAsync( g->dItems, g->hItems.data(), sizeof( WorkItem ) * g->itemCount, hipMemcpyHostToDevice, s ) );
+	}
+	// the host vectors are read by the copies above: they stay unchanged until the next step,
+	// which synchronizes the stream before it edits them
 }
 
 // src/solver.h:144-156
@@ -918,6 +1007,146 @@ static float msSince( std::chrono::steady_clock::time_point t )
 	return std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t ).count();
 }
 
+// Fork-join pool of host threads for per-world work; every task touches one world only. The
+// calling thread takes part; with one task, or one hardware thread, everything runs inline.
+class HostPool
+{
+public:
+	static HostPool& get()
+	{
+		static HostPool pool;
+		return pool;
+	}
+
+	void run( int n, const std::function<void( int )>& fn )
+	{
+		if ( n <= 1 || threads_.empty() )
+		{
+			for ( int i = 0; i < n; ++i )
+				fn( i );
+			return;
+		}
+		{
+			std::lock_guard<std::mutex> lk( m_ );
+			task_ = &fn;
+			count_ = n;
+			next_.store( 0 );
+			pending_ = (int)threads_.size();
+			gen_ += 1;
+		}
+		cv_.notify_all();
+		for ( int i; ( i = next_.fetch_add( 1 ) ) < n; )
+			fn( i );
+		std::unique_lock<std::mutex> lk( m_ );
+		done_.wait( lk, [&] { return pending_ == 0; } );
+		task_ = nullptr;
+	}
+
+private:
+	HostPool()
+	{
+		unsigned hw = std::thread::hardware_concurrency();
+		int n = (int)std::min( hw > 0 ? hw : 1u, 16u ) - 1;
+		for ( int i = 0; i < n; ++i )
+			threads_.emplace_back( [this] { worker(); } );
+	}
+	~HostPool()
+	{
+		{
+			std::lock_guard<std::mutex> lk( m_ );
+			quit_ = true;
+		}
+		cv_.notify_all();
+		for ( std::thread& t : threads_ )
+			t.join();
+	}
+	void worker()
+	{
+		uint64_t seen = 0;
+		for ( ;; )
+		{
+			std::unique_lock<std::mutex> lk( m_ );
+			cv_.wait( lk, [&] { return quit_ || gen_ != seen; } );
+			if ( quit_ )
+				return;
+			seen = gen_;
+			const std::function<void( int )>* fn = task_;
+			int n = count_;
+			lk.unlock();
+			for ( int i; ( i = next_.fetch_add( 1 ) ) < n; )
+				( *fn )( i );
+			lk.lock();
+			if ( --pending_ == 0 )
+				done_.notify_one();
+		}
+	}
+
+	std::vector<std::thread> threads_;
+	std::mutex m_;
+	std::condition_variable cv_, done_;
+	const std::function<void( int )>* task_ = nullptr;
+	int count_ = 0, pending_ = 0;
+	uint64_t gen_ = 0;
+	bool quit_ = false;
+	std::atomic<int> next_{ 0 };
+};
+
+static void forEachWorld( StepGroup* g, const std::function<void( World* )>& fn )
+{
+	HostPool::get().run( (int)g->worlds.size(), [&]( int i ) { fn( g->worlds[i] ); } );
+}
+
+// serial state changes of one world in contact-id order (src/world.c:506-667)
+static void applyChanges( World* w, const ChangeRecord* recs, int count )
+{
+	for ( int i = 0; i < count; ++i )
+	{
+		const ChangeRecord& r = recs[i];
+		Contact& c = w->contacts[r.contactId];
+		int colorIndex = c.colorIndex, localIndex = c.localIndex;
+		if ( r.flags & kSimDisjoint )
+		{
+			c.flags &= ~1u;
+			destroyContactImpl( w, &c );
+		}
+		else if ( r.flags & kSimStarted )
+		{
+			c.flags |= 1u;
+			addContactToGraph( w, c );
+			removeFromList( w, w->nonTouching, localIndex );
+			w->topologyDirty = true;
+		}
+		else if ( r.flags & kSimStopped )
+		{
+			c.flags &= ~1u;
+			int bodyIdA = c.edges[0].bodyId, bodyIdB = c.edges[1].bodyId;
+			c.colorIndex = -1;
+			c.localIndex = (int)w->nonTouching.size();
+			w->nonTouching.push_back( c.contactId );
+			removeContactFromGraph( w, bodyIdA, bodyIdB, colorIndex, localIndex );
+			w->topologyDirty = true;
+		}
+	}
+}
+
+// records sorted by world: [first, last) of each world
+template <typename R>
+static void splitByWorld( const std::vector<R>& recs, int worldCount, std::vector<int>& first, std::vector<int>& last )
+{
+	first.assign( worldCount, 0 );
+	last.assign( worldCount, 0 );
+	int n = (int)recs.size();
+	for ( int b = 0; b < n; )
+	{
+		int e = b;
+		while ( e < n && recs[e].world == recs[b].world )
+			e += 1;
+		first[recs[b].world] = b;
+		last[recs[b].world] = e;
+		b = e;
+	}
+}
+
 void StepGroup::step( float timeStep, int subStepCount )
 {
 	auto t0 = std::chrono::steady_clock::now();
@@ -928,7 +1157,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 	//   buildIslands   begin/end-touch processing and work-item rebuild
 	//   solve          solver kernel (device events)
 	//   finalizeBodies enlarged-proxy processing
-	float msBroad = 0.0f, msChanges = 0.0f, msEnlarge = 0.0f;
+	float msPairs = 0.0f, msBroad = 0.0f, msChanges = 0.0f, msEnlarge = 0.0f;
 	hipStream_t s = stream();
 	for ( World* w : worlds )
 	{
@@ -939,21 +1168,22 @@ void StepGroup::step( float timeStep, int subStepCount )
 			b2mi_internal_uploadBodies( w );
 	}
 
-	// 1. pairs and new contacts (host, serial per world)
+	// 1. pairs and new contacts (host, per world in parallel)
+	auto tp = std::chrono::steady_clock::now();
+	forEachWorld( this, []( World* w ) { updatePairs( w ); } );
 	hInits.clear();
 	for ( World* w : worlds )
 	{
-		auto tp = std::chrono::steady_clock::now();
-		updatePairs( w );
 		ensureContactCapacity( w );
 		hInits.insert( hInits.end(), w->pendingInits.begin(), w->pendingInits.end() );
 		w->pendingInits.clear();
-		w->profile.pairs = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - tp ).count();
 	}
+	msPairs = msSince( tp );
 	if ( structureDirty || (int)hWorlds.size() != (int)worlds.size() )
 	{
 		hWorlds.resize( worlds.size() );
 		structureDirty = true;
+		layoutValid = false;
 	}
 	bool paramsChanged = structureDirty;
 	for ( size_t i = 0; i < worlds.size(); ++i )
@@ -984,15 +1214,17 @@ void StepGroup::step( float timeStep, int subStepCount )
 		HIP_CHECK( launch_init_contacts( dWorlds, dInits, (int)hInits.size(), s ) );
 	}
 
-	// 2. narrowphase
-	bool anyTopology = false;
+	// 2. narrowphase over the worlds' contact lists
+	forEachWorld( this, []( World* w ) {
+		if ( w->contactsDirty )
+			buildWorldCollide( w );
+	} );
+	bool relayout = !layoutValid;
 	for ( World* w : worlds )
-		anyTopology = anyTopology || w->topologyDirty;
-	if ( anyTopology || collideListDirty )
-	{
-		buildCollideList( this );
-		collideListDirty = false;
-	}
+		relayout = relayout || (int)w->collideIds.size() > w->collideRegion.cap;
+	if ( relayout )
+		layoutGroup( this );
+	publishLists( this, relayout );
 	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
 	// events bracket each kernel launch alone (the stream is idle while the host works)
 	if ( events[0] == nullptr )
@@ -1006,11 +1238,10 @@ void StepGroup::step( float timeStep, int subStepCount )
 
 	// the tree rebuild overlaps the narrowphase (src/world.c:367-379, 420)
 	auto tb = std::chrono::steady_clock::now();
-	for ( World* w : worlds )
-	{
+	forEachWorld( this, []( World* w ) {
 		w->trees[b2_dynamicBody].rebuild( false );
 		w->trees[b2_kinematicBody].rebuild( false );
-	}
+	} );
 	msBroad = msSince( tb );
 	HIP_CHECK( hipStreamSynchronize( s ) );
 	auto tc = std::chrono::steady_clock::now();
@@ -1023,45 +1254,40 @@ void StepGroup::step( float timeStep, int subStepCount )
 		std::sort( hChanges.begin(), hChanges.end(), []( const ChangeRecord& a, const ChangeRecord& b ) {
 			return a.world != b.world ? a.world < b.world : a.contactId < b.contactId;
 		} );
-		// serial state changes in contact-id order (src/world.c:506-667)
-		for ( const ChangeRecord& r : hChanges )
-		{
-			World* w = worlds[r.world];
-			Contact& c = w->contacts[r.contactId];
-			int colorIndex = c.colorIndex, localIndex = c.localIndex;
-			if ( r.flags & kSimDisjoint )
-			{
-				c.flags &= ~1u;
-				destroyContactImpl( w, &c );
-			}
-			else if ( r.flags & kSimStarted )
-			{
-				c.flags |= 1u;
-				addContactToGraph( w, c );
-				removeFromList( w, w->nonTouching, localIndex );
-				w->topologyDirty = true;
-			}
-			else if ( r.flags & kSimStopped )
-			{
-				c.flags &= ~1u;
-				int bodyIdA = c.edges[0].bodyId, bodyIdB = c.edges[1].bodyId;
-				c.colorIndex = -1;
-				c.localIndex = (int)w->nonTouching.size();
-				w->nonTouching.push_back( c.contactId );
-				removeContactFromGraph( w, bodyIdA, bodyIdB, colorIndex, localIndex );
-				w->topologyDirty = true;
-			}
-		}
+		std::vector<int> first, last;
+		splitByWorld( hChanges, (int)worlds.size(), first, last );
+		HostPool::get().run( (int)worlds.size(), [&]( int i ) {
+			if ( last[i] > first[i] )
+				applyChanges( worlds[i], hChanges.data() + first[i], last[i] - first[i] );
+		} );
 	}
 
-	// 3. solver (one workgroup per work item)
-	anyTopology = false;
-	for ( World* w : worlds )
-		anyTopology = anyTopology || w->topologyDirty;
-	if ( anyTopology || itemCount == 0 )
+	// 3. solver (one workgroup per work item); rebuild the items of worlds whose graph changed
+	int target = targetSlots > 0 ? targetSlots : 256;
+	int budget = ldsBudget < 0 ? kDefaultItemLdsBytes : std::min( ldsBudget, kMaxItemLdsBytes );
+	if ( target != builtTarget || budget != builtBudget )
 	{
-		buildWorkItems( this );
-		collideListDirty = true;
+		for ( World* w : worlds )
+			w->topologyDirty = true;
+		builtTarget = target;
+		builtBudget = budget;
+	}
+	bool anyItems = false;
+	for ( World* w : worlds )
+		anyItems = anyItems || w->topologyDirty;
+	if ( anyItems )
+	{
+		forEachWorld( this, [&]( World* w ) {
+			if ( w->topologyDirty )
+				buildWorldItems( w, target, budget );
+		} );
+		relayout = false;
+		for ( World* w : worlds )
+			relayout = relayout || (int)w->solver.bodyList.size() > w->bodyRegion.cap ||
+					   (int)w->solver.slotContact.size() > w->slotRegion.cap;
+		if ( relayout )
+			layoutGroup( this );
+		publishLists( this, true );
 	}
 	msChanges = msSince( tc );
 	int totalShapes = 0;
@@ -1094,15 +1320,12 @@ void StepGroup::step( float timeStep, int subStepCount )
 		HIP_CHECK( hipStreamSynchronize( s ) );
 		std::stable_sort( hEnlarge.begin(), hEnlarge.end(),
 						  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
-		int begin = 0;
-		while ( begin < enlargeCount )
-		{
-			int end = begin;
-			while ( end < enlargeCount && hEnlarge[end].world == hEnlarge[begin].world )
-				end += 1;
-			processEnlarged( worlds[hEnlarge[begin].world], hEnlarge.data() + begin, end - begin );
-			begin = end;
-		}
+		std::vector<int> first, last;
+		splitByWorld( hEnlarge, (int)worlds.size(), first, last );
+		HostPool::get().run( (int)worlds.size(), [&]( int i ) {
+			if ( last[i] > first[i] )
+				processEnlarged( worlds[i], hEnlarge.data() + first[i], last[i] - first[i] );
+		} );
 	}
 	msEnlarge = msSince( te );
 	HIP_CHECK( hipEventElapsedTime( &msCollide, e0, e1 ) );
@@ -1115,6 +1338,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 		w->contactsDownloaded = false;
 		w->locked = false;
 		w->profile.step = msTotal;
+		w->profile.pairs = msPairs;
 		w->profile.collide = msCollide;
 		w->profile.solve = msSolve;
 		w->profile.broadphase = msBroad;

@@ -771,6 +771,7 @@ B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def )
 	b.fixedRotation = def->fixedRotation;
 	b.automaticMass = def->automaticMass;
 	w->topologyDirty = true;
+	w->contactsDirty = true;
 	return b2BodyId{ bodyId + 1, (uint16_t)w->slot, b.revision };
 }
 
@@ -1212,6 +1213,7 @@ B2_API void b2DestroyBody( b2BodyId bodyId )
 	body->localIndex = -1;
 	body->id = -1;
 	w->topologyDirty = true;
+	w->contactsDirty = true;
 }
 
 } // extern "C"

@@ -161,6 +161,22 @@ struct Contact
 
 struct StepGroup;
 
+// work items of one world, with body and slot indices local to these lists
+struct SolverLists
+{
+	std::vector<WorkItem> items;
+	std::vector<int> bodyList;	  // awake sim indices
+	std::vector<int> slotContact; // contact ids
+	std::vector<int2> slotBodies; // local body indices (-1 static)
+	int maxGroupSize = 0, maxItemLds = 0;
+};
+
+// a world's part of one of the step group's launch arrays
+struct Region
+{
+	int base = 0, cap = 0;
+};
+
 struct World
 {
 	int slot = -1; // index in the world table (worldId.index1 - 1)
@@ -205,6 +221,14 @@ struct World
 
 	b2Profile profile{};
 	bool topologyDirty = true; // colour lists / bodies changed since the work items were built
+	bool contactsDirty = true; // contacts created or destroyed since the narrowphase list was built
+
+	// this world's solver lists (indices local to the lists) and narrowphase list, and the
+	// regions they occupy in the step group's launch arrays
+	SolverLists solver;
+	std::vector<int> collideIds;
+	Region bodyRegion, slotRegion, collideRegion;
+	bool solverPending = true, collidePending = true; // region contents not yet uploaded
 
 	// device arrays
 	float* dState = nullptr;
@@ -234,6 +258,7 @@ struct StepGroup
 	int targetSlots = 0;	  // work-item packing target (constraint slots), 0 = automatic
 	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kDefaultItemLdsBytes, 0 = HBM only)
 	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
+	int builtTarget = -1, builtBudget = -1;				   // packing target and LDS budget of the current items
 	bool structureDirty = true;
 
 	// launch buffers (device)
@@ -242,7 +267,7 @@ struct StepGroup
 	int2* dCollideList = nullptr;
 	int capCollide = 0;
 	int collideCount = 0;
-	bool collideListDirty = true;
+	bool layoutValid = false; // regions of the worlds in the launch arrays are assigned
 	ChangeRecord* dChanges = nullptr;
 	int capChanges = 0;
 	int* dCounters = nullptr; // [0] changes, [1] enlarge
