@@ -19,6 +19,7 @@ struct TapeVersion
 {
 	int itemsVersion = -1;
 	int itemCount = 0, slotCount = 0, bodyListCount = 0, block = 64;
+	int usedSlots = 0; // slots that belong to work items
 	WorkItem* dItems = nullptr;
 	int* dBodyList = nullptr;
 	int* dSlotContact = nullptr;
@@ -166,6 +167,7 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 			if ( wi.ownedCount != wi.bodyCount )
 				fatal( "a taped run does not support kinematic bodies", __FILE__, __LINE__ );
 			itemsOfWorld[wi.world] += 1;
+			v.usedSlots += wi.groupStart[kGroups] - wi.groupStart[1];
 			for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
 				slotWorld[wi.slotStart + k] = wi.world;
 			B2MI_CHECK( scratch < ( (size_t)1 << 31 ) );
@@ -459,8 +461,9 @@ B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, i
 	B2MI_CHECK( t != nullptr && step >= 0 && step < (int)t->steps.size() );
 	const TapeStep& ts = t->steps[step];
 	const TapeVersion& v = t->versions[ts.version];
-	if ( slots == nullptr || capacity < v.slotCount )
-		return v.slotCount;
+	int used = v.usedSlots; // slots of work items (the launch arrays also hold unused entries)
+	if ( slots == nullptr || capacity < used )
+		return used;
 	hipStream_t s = stream();
 	std::vector<WorkItem> items( v.itemCount );
 	std::vector<int> bodyList( v.bodyListCount ), slotContact( v.slotCount );
@@ -477,13 +480,14 @@ B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, i
 		HIP_CHECK( hipMemcpyAsync( snap.data(), ts.dSnap, sizeof( DevContact ) * v.slotCount, hipMemcpyDeviceToHost, s ) );
 	}
 	HIP_CHECK( hipStreamSynchronize( s ) );
+	int out = 0;
 	for ( const WorkItem& wi : items )
 		for ( int gr = 1; gr < kGroups; ++gr )
 			for ( int k = wi.groupStart[gr]; k < wi.groupStart[gr + 1]; ++k )
 			{
 				int slot = wi.slotStart + k;
 				const DevContact& c = snap[slot];
-				b2mi_TapeSlot& o = slots[slot];
+				b2mi_TapeSlot& o = slots[out++];
 				int2 lb = slotBodies[slot];
 				o.world = wi.world;
 				o.contactId = slotContact[slot];
@@ -512,7 +516,7 @@ B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, i
 					o.tangentImpulse[j] = p.tangentImpulse;
 				}
 			}
-	return v.slotCount;
+	return out;
 }
 
 B2_API void b2mi_Batch_GetTapeWorldParams( int batch, float* params )

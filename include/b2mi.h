@@ -130,7 +130,8 @@ B2_API void b2mi_Batch_EndTape( int batch );
 B2_API int b2mi_Batch_GetTapeStepCount( int batch );
 B2_API int b2mi_Batch_GetTapeBodies( int batch, b2BodyId* bodies, int capacity );
 B2_API void b2mi_Batch_GetTapePoses( int batch, float* poses );
-// The slots of taped step `step` (returns their count; writes at most capacity).
+// The slots of taped step `step`, work item by work item (returns their count; writes them
+// only when capacity holds them all).
 B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, int capacity );
 // B2MI_TAPE_WORLD_PARAMS floats per world.
 B2_API void b2mi_Batch_GetTapeWorldParams( int batch, float* params );
