@@ -180,7 +180,10 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 	const int tid = threadIdx.x, nt = blockDim.x;
 	const int total = wi.groupStart[kGroups];
 	for ( int i = tid; i < wi.bodyCount; i += nt )
+	{
 		stage_pose( C, i, poseIn[simBase + bl[i]] );
+		C.mass( i ) = make_float2( W.sims[bl[i]].invMass, W.sims[bl[i]].invI );
+	}
 	__syncthreads();
 	for ( int k = tid; k < total; k += nt )
 	{
@@ -223,7 +226,7 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 			{
 				int2 lb = C.bodies( k );
 				tape_appl( C, tape, 3 * sub + 1, k, lb.x, lb.y );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
 			}
 			__syncthreads();
 		}
@@ -242,7 +245,7 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 			{
 				int2 lb = C.bodies( k );
 				tape_appl( C, tape, 3 * sub + 2, k, lb.x, lb.y );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
 			}
 			__syncthreads();
 		}
@@ -332,7 +335,8 @@ template <class S>
 __device__ void lane_warm_start_adj( const S& C, const AdjScratch& a, int k, int la, int lb )
 {
 	BodyAdj A = body_adj( a, la ), B = body_adj( a, lb );
-	float mA = C.at( cfInvMassA, k ), mB = C.at( cfInvMassB, k ), iA = C.at( cfInvIA, k ), iB = C.at( cfInvIB, k );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, k ), ny = C.at( cfNormalY, k );
 	float tx = ny, ty = 0.0f - nx;
 	float* sa = a.sAdj + (size_t)8 * k;
@@ -351,18 +355,20 @@ __device__ void lane_warm_start_adj( const S& C, const AdjScratch& a, int k, int
 // reverse of lane_solve from its taped inputs; returns d(loss)/d(friction) of this application
 template <class S>
 __device__ float lane_solve_adj( const S& C, const AdjScratch& a, int pass, int k, int la, int lb, float inv_h, float minBiasVel,
-								 bool useBias )
+								 bool useBias, const DevWorld& W )
 {
 	const float* t = a.applAt( pass, k );
 	Body bA, bB;
 	bA.vx = t[0], bA.vy = t[1], bA.w = t[2], bA.dpx = t[3], bA.dpy = t[4], bA.dqc = t[5], bA.dqs = t[6];
 	bB.vx = t[7], bB.vy = t[8], bB.w = t[9], bB.dpx = t[10], bB.dpy = t[11], bB.dqc = t[12], bB.dqs = t[13];
 	float ni0[2] = { t[14], t[15] }, ti0[2] = { t[16], t[17] };
-	float mA = C.at( cfInvMassA, k ), mB = C.at( cfInvMassB, k ), iA = C.at( cfInvIA, k ), iB = C.at( cfInvIB, k );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, k ), ny = C.at( cfNormalY, k );
-	float biasRate = useBias ? C.at( cfBiasRate, k ) : 0.0f;
-	float massScale = useBias ? C.at( cfMassScale, k ) : 1.0f;
-	float impulseScale = useBias ? C.at( cfImpulseScale, k ) : 0.0f;
+	const float* soft = contact_softness( W, la, lb );
+	float biasRate = useBias ? soft[0] : 0.0f;
+	float massScale = useBias ? soft[1] : 1.0f;
+	float impulseScale = useBias ? soft[2] : 0.0f;
 	float friction = C.at( cfFriction, k );
 	float tx = ny, ty = 0.0f - nx;
 	float rAx[2], rAy[2], rBx[2], rBy[2];
@@ -480,7 +486,8 @@ __device__ float lane_restitution_adj( const S& C, const AdjScratch& a, int pass
 	bA.vx = t[0], bA.vy = t[1], bA.w = t[2], bA.dpx = t[3], bA.dpy = t[4], bA.dqc = t[5], bA.dqs = t[6];
 	bB.vx = t[7], bB.vy = t[8], bB.w = t[9], bB.dpx = t[10], bB.dpy = t[11], bB.dqc = t[12], bB.dqs = t[13];
 	float ni0[2] = { t[14], t[15] };
-	float mA = C.at( cfInvMassA, k ), mB = C.at( cfInvMassB, k ), iA = C.at( cfInvIA, k ), iB = C.at( cfInvIB, k );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, k ), ny = C.at( cfNormalY, k );
 	float restitution = C.at( cfRestitution, k );
 	float rAx[2], rAy[2], rBx[2], rBy[2], mass[2], relVel[2], u[2];
@@ -583,7 +590,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_replay_step( const DevWorl
 																   const int* __restrict__ bodyList, const int2* __restrict__ slotBodies,
 																   const DevContact* __restrict__ snap, const StepPose* __restrict__ poseIn,
 																   StepPose* __restrict__ poseOut, const int* __restrict__ simBase,
-																   float* cfBase, int cfCapacity, float4* priv )
+																   float* cfBase, int cfCapacity, float4* priv, float2* mass )
 {
 	const WorkItem& wi = items[blockIdx.x];
 	const DevWorld& W = worlds[wi.world];
@@ -594,6 +601,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_replay_step( const DevWorl
 	C.base = wi.slotStart;
 	C.sb = slotBodies + wi.slotStart;
 	C.P = priv + 2 * (size_t)wi.bodyStart;
+	C.M = mass + wi.bodyStart;
 	NoTape tape;
 	replay_item( C, tape, W, wi, bl, snap + wi.slotStart, poseIn, simBase[wi.world] );
 	int base = simBase[wi.world];
@@ -614,7 +622,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_adjoint_step( const DevWor
 																	StepPose* __restrict__ adj, const int* __restrict__ simBase,
 																	const float4* __restrict__ lamOut, float* __restrict__ lamPrev,
 																	const int* __restrict__ prevSlot, float* __restrict__ paramGrad,
-																	float* cfBase, int cfCapacity, float4* priv, float* scratch,
+																	float* cfBase, int cfCapacity, float4* priv, float2* mass, float* scratch,
 																	const int* __restrict__ scratchOffset )
 {
 	const WorkItem& wi = items[blockIdx.x];
@@ -630,6 +638,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_adjoint_step( const DevWor
 	C.base = wi.slotStart;
 	C.sb = slotBodies + wi.slotStart;
 	C.P = priv + 2 * (size_t)wi.bodyStart;
+	C.M = mass + wi.bodyStart;
 	Tape tape;
 	tape.a = carve( scratch + scratchOffset[blockIdx.x], wi.bodyCount, total, S );
 	const AdjScratch& a = tape.a;
@@ -710,7 +719,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_adjoint_step( const DevWor
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				a.sAdj[(size_t)8 * k + 6] += lane_solve_adj( C, a, 3 * sub + 2, k, lb.x, lb.y, inv_h, minBiasVel, false );
+				a.sAdj[(size_t)8 * k + 6] += lane_solve_adj( C, a, 3 * sub + 2, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
 			}
 			__syncthreads();
 		}
@@ -725,7 +734,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_adjoint_step( const DevWor
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				a.sAdj[(size_t)8 * k + 6] += lane_solve_adj( C, a, 3 * sub + 1, k, lb.x, lb.y, inv_h, minBiasVel, true );
+				a.sAdj[(size_t)8 * k + 6] += lane_solve_adj( C, a, 3 * sub + 1, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
 			}
 			__syncthreads();
 		}
@@ -924,21 +933,21 @@ hipError_t launch_gather_poses( const DevWorld* worlds, const int* simBase, cons
 
 hipError_t launch_replay_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 							   const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* poseOut,
-							   const int* simBase, float* cf, int cfCapacity, float4* priv, hipStream_t stream )
+							   const int* simBase, float* cf, int cfCapacity, float4* priv, float2* mass, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
 	if ( blockSize < 64 || blockSize > kSolveMaxBlock )
 		return hipErrorInvalidValue;
 	hipLaunchKernelGGL( k_replay_step, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap, poseIn,
-						poseOut, simBase, cf, cfCapacity, priv );
+						poseOut, simBase, cf, cfCapacity, priv, mass );
 	return hipGetLastError();
 }
 
 hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
 								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
-								float* cf, int cfCapacity, float4* priv, float* scratch, const int* scratchOffset,
+								float* cf, int cfCapacity, float4* priv, float2* mass, float* scratch, const int* scratchOffset,
 								hipStream_t stream )
 {
 	if ( itemCount <= 0 )
@@ -946,7 +955,7 @@ hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, i
 	if ( blockSize < 64 || blockSize > kSolveMaxBlock )
 		return hipErrorInvalidValue;
 	hipLaunchKernelGGL( k_adjoint_step, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap, poseIn,
-						adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, scratch, scratchOffset );
+						adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, mass, scratch, scratchOffset );
 	return hipGetLastError();
 }
 

@@ -209,32 +209,35 @@ struct ContactInit
 // number of float/int fields of one solver constraint (structure of arrays)
 enum ConstraintField : int
 {
-	cfInvMassA,
-	cfInvMassB,
-	cfInvIA,
-	cfInvIB,
-	cfNormalX,
+	// read-only during the step, read by every pass (HBM in the bodies-in-LDS store)
+	cfNormalX = 0,
 	cfNormalY,
 	cfFriction,
-	cfRestitution,
-	cfBiasRate,
-	cfMassScale,
-	cfImpulseScale,
-	// per point p (0, 1): base + 11 * p
-	cfAnchorAX,
+	cfAnchorAX, // per point: 7 fields, point 1 follows point 0
 	cfAnchorAY,
 	cfAnchorBX,
 	cfAnchorBY,
 	cfBaseSeparation,
-	cfRelativeVelocity,
-	cfNormalImpulse,
-	cfTangentImpulse,
-	cfMaxNormalImpulse,
 	cfNormalMass,
 	cfTangentMass,
-	cfPointFields = 11,
-	cfFloatCount = cfAnchorAX + 2 * 11,
+	cfHotEnd = cfAnchorAX + 2 * 7,
+	// written by the passes (LDS in both LDS stores)
+	cfNormalImpulse = cfHotEnd, // per point: 3 fields
+	cfTangentImpulse,
+	cfMaxNormalImpulse,
+	cfMutableEnd = cfNormalImpulse + 2 * 3,
+	// read only by restitution and store (HBM in the bodies-in-LDS store)
+	cfRestitution = cfMutableEnd,
+	cfRelativeVelocity, // per point: 1 field
+	cfFloatCount = cfRelativeVelocity + 2,
+	cfMutableCount = cfMutableEnd - cfHotEnd,
 };
+
+// field of manifold point p (0 or 1) for a per-point field given at point 0
+constexpr int pointField( int field, int p )
+{
+	return field + p * ( field < cfHotEnd ? 7 : field < cfMutableEnd ? 3 : 1 );
+}
 
 // Pose of a body at a step boundary (taped runs): center of mass, rotation, velocities
 struct StepPose
@@ -243,8 +246,9 @@ struct StepPose
 	float vx, vy, w, pad;
 };
 
-// Bytes of LDS a work item's solver state needs: per body two float4, per slot the two local
-// body indices and the constraint fields. Items above the budget run from HBM. A CDNA4 CU has
+// Bytes of LDS a work item's solver state needs: per body two float4 of state and a float2 of
+// inverse mass and inertia, per slot the two local body indices and the constraint fields.
+// Items above the budget keep the read-only constraint fields in HBM, or everything. A CDNA4 CU has
 // 160 KiB of LDS; the default budget leaves room for two or more items per CU.
 constexpr int kMaxItemLdsBytes = 160 * 1024;
 constexpr int kDefaultItemLdsBytes = 64 * 1024;
@@ -252,11 +256,12 @@ constexpr int kDefaultItemLdsBytes = 64 * 1024;
 constexpr int kSolveMaxBlock = 256;
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {
-	return 32 * bodyCount + 8 * slotCount + 4 * cfFloatCount * slotCount;
+	return 40 * bodyCount + 8 * slotCount + 4 * cfFloatCount * slotCount;
 }
-inline int itemBodyLdsBytes( int bodyCount )
+// the bodies-in-LDS store: body state and masses, slot bodies, the impulses
+inline int itemBodyLdsBytes( int bodyCount, int slotCount )
 {
-	return 32 * bodyCount;
+	return 40 * bodyCount + 8 * slotCount + 4 * cfMutableCount * slotCount;
 }
 
 } // namespace b2mi

@@ -375,12 +375,13 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	const float4* gstate = reinterpret_cast<const float4*>( W.state );
 	const int total = wi.groupStart[kGroups];
 
-	// stage private body state
+	// stage private body state and masses
 	for ( int i = tid; i < wi.bodyCount; i += nt )
 	{
 		int b = bl[i];
 		C.b0( i ) = gstate[2 * b];
 		C.b1( i ) = gstate[2 * b + 1];
+		C.mass( i ) = make_float2( W.sims[b].invMass, W.sims[b].invI );
 	}
 	__syncthreads();
 
@@ -436,7 +437,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				for ( int k = ovfBegin; k < ovfEnd; ++k )
 				{
 					int2 lb = C.bodies( k );
-					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true );
+					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, W );
 				}
 			__syncthreads();
 		}
@@ -448,7 +449,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
 			}
 			__syncthreads();
 		}
@@ -464,7 +465,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				for ( int k = ovfBegin; k < ovfEnd; ++k )
 				{
 					int2 lb = C.bodies( k );
-					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false );
+					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, W );
 				}
 			__syncthreads();
 		}
@@ -476,7 +477,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
 			}
 			__syncthreads();
 		}
@@ -539,7 +540,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
-												   float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity )
+												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity )
 {
 	extern __shared__ float4 lds[];
 	// read in place (uniform scalar loads); a private copy would be indexed dynamically from scratch
@@ -550,9 +551,11 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	const int total = wi.groupStart[kGroups];
 	if ( wi.store == kStoreLds )
 	{
+		// LDS: body state, masses, slot bodies, constraint fields
 		LdsStore C;
 		C.P = lds;
-		C.sb = reinterpret_cast<int2*>( lds + 2 * wi.bodyCount );
+		C.M = reinterpret_cast<float2*>( lds + 2 * wi.bodyCount );
+		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount );
 		C.f = reinterpret_cast<float*>( C.sb + total );
 		C.n = total;
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
@@ -562,12 +565,18 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	}
 	else if ( wi.store == kStoreBodiesLds )
 	{
+		// LDS: body state, masses, slot bodies, impulses; HBM: the other constraint fields
 		BodiesLdsStore C;
 		C.f = cfBase;
 		C.cap = cfCapacity;
 		C.base = wi.slotStart;
-		C.sb = slotBodies + wi.slotStart;
 		C.P = lds;
+		C.M = reinterpret_cast<float2*>( lds + 2 * wi.bodyCount );
+		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount );
+		C.mut = reinterpret_cast<float*>( C.sb + total );
+		C.n = total;
+		for ( int k = threadIdx.x; k < total; k += blockDim.x )
+			C.sb[k] = slotBodies[wi.slotStart + k];
 		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
 	}
 	else
@@ -578,6 +587,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.base = wi.slotStart;
 		C.sb = slotBodies + wi.slotStart;
 		C.P = priv + 2 * (size_t)wi.bodyStart;
+		C.M = mass + wi.bodyStart;
 		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
 	}
 }
@@ -609,7 +619,7 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 
 hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, int ldsBytes,
 						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
-						 float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
+						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
@@ -625,7 +635,7 @@ hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int item
 		allowedLds = kMaxItemLdsBytes;
 	}
 	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
-						slotBodies, cf, cfCapacity, priv, enl, enlCount, enlCapacity );
+						slotBodies, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity );
 	return hipGetLastError();
 }
 

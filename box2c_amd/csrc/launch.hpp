@@ -14,7 +14,7 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 						   hipStream_t stream );
 hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, int ldsBytes,
 						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
-						 float4* priv, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
+						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
 
 // replay and adjoint (adjoint.hip)
 int adjoint_scratch_floats( int bodyCount, int slotCount, int subSteps );
@@ -24,11 +24,11 @@ hipError_t launch_gather_poses( const DevWorld* worlds, const int* simBase, cons
 								StepPose* out, hipStream_t stream );
 hipError_t launch_replay_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 							   const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* poseOut,
-							   const int* simBase, float* cf, int cfCapacity, float4* priv, hipStream_t stream );
+							   const int* simBase, float* cf, int cfCapacity, float4* priv, float2* mass, hipStream_t stream );
 hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
 								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
-								float* cf, int cfCapacity, float4* priv, float* scratch, const int* scratchOffset,
+								float* cf, int cfCapacity, float4* priv, float2* mass, float* scratch, const int* scratchOffset,
 								hipStream_t stream );
 hipError_t launch_link_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase,
 							  const int2* slotOfContact, int prevStep, int* prevSlot, hipStream_t stream );

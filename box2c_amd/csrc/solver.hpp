@@ -58,12 +58,15 @@ __device__ inline void store_velocity( const S& C, int i, const Body& b )
 }
 
 // Solver state of one work item, addressed by local slot k (0 .. slots-1) and local body i:
-// the constraint fields (cf* enum), each slot's two local body indices, and the private body
-// state (two float4 per body: v.x, v.y, w, flags | dp.x, dp.y, dq.c, dq.s).
+// the constraint fields (cf* enum), each slot's two local body indices, the private body state
+// (two float4 per body: v.x, v.y, w, flags | dp.x, dp.y, dq.c, dq.s) and each body's inverse
+// mass and inertia (the same floats the narrowphase copies into a contact).
 //
 // GlobalStore keeps it in HBM (any work item size). LdsStore keeps it in the workgroup's LDS,
 // so the substep passes (about 3 x colours x substeps dependent phases per step) run at LDS
-// latency; the host picks it for every work item whose state fits the LDS budget.
+// latency. BodiesLdsStore, for items too large for that, keeps the bodies, the slot bodies and
+// the impulses in LDS and the read-only constraint fields in HBM: per phase a slot then only
+// reads HBM, coalesced across lanes. The host picks the store per work item from the LDS budget.
 struct GlobalStore
 {
 	float* f;
@@ -71,6 +74,7 @@ struct GlobalStore
 	int base;
 	const int2* sb;
 	float4* P;
+	float2* M;
 	__device__ inline float& at( int field, int k ) const
 	{
 		return f[(size_t)field * cap + base + k];
@@ -86,21 +90,27 @@ struct GlobalStore
 	__device__ inline float4& b1( int i ) const
 	{
 		return P[2 * i + 1];
+	}
+	__device__ inline float2& mass( int i ) const
+	{
+		return M[i];
 	}
 };
 
-// Bodies in LDS, constraints in HBM: for items whose constraints exceed the LDS budget. The
-// bodies are the randomly accessed part (two per slot per phase), the constraint fields are
-// read and written with consecutive lanes on consecutive slots.
 struct BodiesLdsStore
 {
-	float* f;
+	float* f; // HBM: all fields (the mutable ones unused)
 	int cap;
 	int base;
-	const int2* sb;
+	float* mut; // LDS: the impulse fields
+	int n;
+	int2* sb;
 	float4* P;
+	float2* M;
 	__device__ inline float& at( int field, int k ) const
 	{
+		if ( field >= cfHotEnd && field < cfMutableEnd )
+			return mut[( field - cfHotEnd ) * n + k];
 		return f[(size_t)field * cap + base + k];
 	}
 	__device__ inline int2 bodies( int k ) const
@@ -114,6 +124,10 @@ struct BodiesLdsStore
 	__device__ inline float4& b1( int i ) const
 	{
 		return P[2 * i + 1];
+	}
+	__device__ inline float2& mass( int i ) const
+	{
+		return M[i];
 	}
 };
 
@@ -123,6 +137,7 @@ struct LdsStore
 	int n;
 	int2* sb;
 	float4* P;
+	float2* M;
 	__device__ inline float& at( int field, int k ) const
 	{
 		return f[field * n + k];
@@ -139,16 +154,36 @@ struct LdsStore
 	{
 		return P[2 * i + 1];
 	}
+	__device__ inline float2& mass( int i ) const
+	{
+		return M[i];
+	}
 };
 
-#define PF( name, p ) ( cf##name + cfPointFields * ( p ) )
+// inverse mass and inertia of local body i (the static body, -1, has none)
+template <class S>
+__device__ inline float2 body_mass( const S& C, int i )
+{
+	return i < 0 ? make_float2( 0.0f, 0.0f ) : C.mass( i );
+}
+
+// soft constraint coefficients of a contact: static ones when either body is static
+// (src/contact_solver.c:650-651)
+__device__ inline const float* contact_softness( const DevWorld& W, int la, int lb )
+{
+	return ( la < 0 || lb < 0 ) ? W.staticSoft : W.contactSoft;
+}
+
+#define PF( name, p ) pointField( cf##name, ( p ) )
 
 // contact_solver.c:597-808 (lanes) and 21-147 (overflow): one constraint slot
 template <class S>
 __device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict__ contact, int la, int lb, const DevWorld& W )
 {
 	const DevManifold& m = contact->manifold;
-	float mA = contact->invMassA, iA = contact->invIA, mB = contact->invMassB, iB = contact->invIB;
+	// the bodies' inverse masses (equal to the ones the narrowphase copied into the contact)
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, iA = MA.y, mB = MB.x, iB = MB.y;
 	float vAx = 0.0f, vAy = 0.0f, wA = 0.0f, vBx = 0.0f, vBy = 0.0f, wB = 0.0f;
 	if ( la >= 0 )
 	{
@@ -164,14 +199,6 @@ __device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict
 		vBy = s.y;
 		wB = s.z;
 	}
-	C.at( cfInvMassA, slot ) = mA;
-	C.at( cfInvMassB, slot ) = mB;
-	C.at( cfInvIA, slot ) = iA;
-	C.at( cfInvIB, slot ) = iB;
-	const float* soft = ( la < 0 || lb < 0 ) ? W.staticSoft : W.contactSoft;
-	C.at( cfBiasRate, slot ) = soft[0];
-	C.at( cfMassScale, slot ) = soft[1];
-	C.at( cfImpulseScale, slot ) = soft[2];
 	float nx = m.normalX, ny = m.normalY;
 	C.at( cfNormalX, slot ) = nx;
 	C.at( cfNormalY, slot ) = ny;
@@ -208,8 +235,17 @@ __device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict
 		}
 		else
 		{
-			for ( int f = 0; f < cfPointFields; ++f )
-				C.at( cfAnchorAX + cfPointFields * j + f, slot ) = 0.0f;
+			C.at( PF( AnchorAX, j ), slot ) = 0.0f;
+			C.at( PF( AnchorAY, j ), slot ) = 0.0f;
+			C.at( PF( AnchorBX, j ), slot ) = 0.0f;
+			C.at( PF( AnchorBY, j ), slot ) = 0.0f;
+			C.at( PF( BaseSeparation, j ), slot ) = 0.0f;
+			C.at( PF( NormalMass, j ), slot ) = 0.0f;
+			C.at( PF( TangentMass, j ), slot ) = 0.0f;
+			C.at( PF( NormalImpulse, j ), slot ) = 0.0f;
+			C.at( PF( TangentImpulse, j ), slot ) = 0.0f;
+			C.at( PF( MaxNormalImpulse, j ), slot ) = 0.0f;
+			C.at( PF( RelativeVelocity, j ), slot ) = 0.0f;
 		}
 	}
 }
@@ -230,7 +266,8 @@ template <class S>
 __device__ void lane_warm_start( const S& C, int slot, int la, int lb )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
-	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float tx = ny, ty = 0.0f - nx;
 	for ( int j = 0; j < 2; ++j )
@@ -253,14 +290,16 @@ __device__ void lane_warm_start( const S& C, int slot, int la, int lb )
 
 // contact_solver.c:865-1075
 template <class S>
-__device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias )
+__device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias, const DevWorld& W )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
-	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
-	float biasRate = useBias ? C.at( cfBiasRate, slot ) : 0.0f;
-	float massScale = useBias ? C.at( cfMassScale, slot ) : 1.0f;
-	float impulseScale = useBias ? C.at( cfImpulseScale, slot ) : 0.0f;
+	const float* soft = contact_softness( W, la, lb );
+	float biasRate = useBias ? soft[0] : 0.0f;
+	float massScale = useBias ? soft[1] : 1.0f;
+	float impulseScale = useBias ? soft[2] : 0.0f;
 	float dpx = bB.dpx - bA.dpx, dpy = bB.dpy - bA.dpy;
 	float nImp[2];
 	for ( int j = 0; j < 2; ++j )
@@ -313,7 +352,8 @@ template <class S>
 __device__ void lane_restitution( const S& C, int slot, int la, int lb, float threshold )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
-	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float restitution = C.at( cfRestitution, slot );
 	for ( int j = 0; j < 2; ++j )
@@ -342,7 +382,8 @@ template <class S>
 __device__ void ovf_warm_start( const S& C, int slot, int la, int lb, int pointCount )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
-	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float tx = ny, ty = -nx;
 	for ( int j = 0; j < pointCount; ++j )
@@ -365,10 +406,11 @@ __device__ void ovf_warm_start( const S& C, int slot, int la, int lb, int pointC
 
 template <class S>
 __device__ void ovf_solve( const S& C, int slot, int la, int lb, int pointCount, float inv_h, float pushout,
-						   bool useBias )
+						   bool useBias, const DevWorld& W )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
-	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	float tx = ny, ty = -nx;
 	float dpx = bB.dpx - bA.dpx, dpy = bB.dpy - bA.dpy;
@@ -385,9 +427,10 @@ __device__ void ovf_solve( const S& C, int slot, int la, int lb, int pointCount,
 			velocityBias = s * inv_h;
 		else if ( useBias )
 		{
-			velocityBias = maxr( C.at( cfBiasRate, slot ) * s, -pushout );
-			massScale = C.at( cfMassScale, slot );
-			impulseScale = C.at( cfImpulseScale, slot );
+			const float* soft = contact_softness( W, la, lb );
+			velocityBias = maxr( soft[0] * s, -pushout );
+			massScale = soft[1];
+			impulseScale = soft[2];
 		}
 		float vrAx = bA.vx + ( -bA.w ) * rAy, vrAy = bA.vy + bA.w * rAx;
 		float vrBx = bB.vx + ( -bB.w ) * rBy, vrBy = bB.vy + bB.w * rBx;
@@ -440,7 +483,8 @@ __device__ void ovf_restitution( const S& C, int slot, int la, int lb, int point
 	if ( restitution == 0.0f )
 		return;
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
-	float mA = C.at( cfInvMassA, slot ), mB = C.at( cfInvMassB, slot ), iA = C.at( cfInvIA, slot ), iB = C.at( cfInvIB, slot );
+	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
+	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
 	for ( int j = 0; j < pointCount; ++j )
 	{

@@ -659,7 +659,7 @@ static void buildWorldItems( World* w, int target, int budget )
 		wi.bodyCount = (int)L.bodyList.size() - wi.bodyStart;
 		{
 			int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
-			int bodyBytes = itemBodyLdsBytes( wi.bodyCount );
+			int bodyBytes = itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
 			if ( bytes <= budget )
 			{
 				wi.store = kStoreLds;
@@ -692,7 +692,7 @@ static void buildWorldItems( World* w, int target, int budget )
 		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= (int)L.slotContact.size() );
 		B2MI_CHECK( wi.ldsBytes <= kMaxItemLdsBytes );
 		B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
-					( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount ) ) ||
+					( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) ||
 					( wi.store == kStoreLds && wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) );
 		for ( int k = 0; k < kGroups; ++k )
 			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
@@ -780,6 +780,7 @@ static void layoutGroup( StepGroup* g )
 		HIP_CHECK( hipMalloc( (void**)&g->dCons, sizeof( float ) * (size_t)cfFloatCount * g->capCons ) );
 	}
 	grow( g->dPriv, g->capPriv, 2 * std::max( bodies, 1 ) );
+	grow( g->dMass, g->capMass, std::max( bodies, 1 ) );
 	g->layoutValid = true;
 }
 
@@ -1306,7 +1307,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 		tapeRecordStep( this, timeStep, subStepCount, block );
 	HIP_CHECK( hipEventRecord( e3, s ) );
 	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons, capCons,
-							 dPriv, dEnlarge, dCounters + 1, capEnlarge, s ) );
+							 dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, s ) );
 	HIP_CHECK( hipEventRecord( e2, s ) );
 	HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
 	HIP_CHECK( hipStreamSynchronize( s ) );

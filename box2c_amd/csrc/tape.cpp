@@ -256,6 +256,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 	float* dParam = devAlloc<float>( 2 * W );
 	float* dCf = devAlloc<float>( (size_t)cfFloatCount * maxSlots );
 	float4* dPriv = devAlloc<float4>( 2 * (size_t)maxBodies );
+	float2* dMass = devAlloc<float2>( maxBodies );
 	float* dScratch = devAlloc<float>( maxScratch );
 	float* dLamA = devAlloc<float>( 4 * (size_t)maxSlots );
 	float* dLamB = devAlloc<float>( 4 * (size_t)maxSlots );
@@ -289,7 +290,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 			const TapeVersion& v = t->versions[ts.version];
 			HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
 										   dStates + (size_t)( step - c ) * n, dStates + (size_t)( step - c + 1 ) * n, t->dSimBase, dCf,
-										   maxSlots, dPriv, s ) );
+										   maxSlots, dPriv, dMass, s ) );
 		}
 		for ( int step = end - 1; step >= c; --step )
 		{
@@ -307,7 +308,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 			HIP_CHECK( launch_link_slots( v.dSlotContact, v.dSlotWorld, v.slotCount, dContactBase, dSlotOf, prevTag, dPrevSlot, s ) );
 			HIP_CHECK( launch_adjoint_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
 											dStates + (size_t)( step - c ) * n, dAdj, t->dSimBase, reinterpret_cast<const float4*>( lamOut ),
-											lamPrev, dPrevSlot, dParam, dCf, maxSlots, dPriv, dScratch, v.dScratchOffset, s ) );
+											lamPrev, dPrevSlot, dParam, dCf, maxSlots, dPriv, dMass, dScratch, v.dScratchOffset, s ) );
 			std::swap( lamOut, lamPrev );
 		}
 	}
@@ -324,6 +325,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 	devRelease( dParam );
 	devRelease( dCf );
 	devRelease( dPriv );
+	devRelease( dMass );
 	devRelease( dScratch );
 	devRelease( dLamA );
 	devRelease( dLamB );
@@ -352,6 +354,7 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 	StepPose* dB = devAlloc<StepPose>( n );
 	float* dCf = devAlloc<float>( (size_t)cfFloatCount * maxSlots );
 	float4* dPriv = devAlloc<float4>( 2 * (size_t)maxBodies );
+	float2* dMass = devAlloc<float2>( maxBodies );
 	DevContact* dPatched = devAlloc<DevContact>( maxSlots );
 	float* dParams = devAlloc<float>( 2 * W );
 	std::vector<StepPose> h( n );
@@ -374,7 +377,7 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 			snap = dPatched;
 		}
 		HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, snap, dA, dB, t->dSimBase,
-									   dCf, maxSlots, dPriv, s ) );
+									   dCf, maxSlots, dPriv, dMass, s ) );
 		std::swap( dA, dB );
 	}
 	HIP_CHECK( hipMemcpyAsync( h.data(), dA, sizeof( StepPose ) * n, hipMemcpyDeviceToHost, s ) );
@@ -388,6 +391,7 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 	devRelease( dB );
 	devRelease( dCf );
 	devRelease( dPriv );
+	devRelease( dMass );
 	devRelease( dPatched );
 	devRelease( dParams );
 }

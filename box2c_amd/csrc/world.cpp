@@ -133,6 +133,7 @@ StepGroup::~StepGroup()
 	devFree( dSlotBodies );
 	devFree( dCons );
 	devFree( dPriv );
+	devFree( dMass );
 	for ( hipEvent_t e : events )
 		if ( e )
 			(void)hipEventDestroy( e );

@@ -288,6 +288,8 @@ struct StepGroup
 	int capCons = 0;
 	float4* dPriv = nullptr;
 	int capPriv = 0;
+	float2* dMass = nullptr; // inverse mass and inertia per body-list entry (HBM store)
+	int capMass = 0;
 	int slotCount = 0, bodyListCount = 0;
 	int maxGroupSize = 0;
 	int maxItemLds = 0; // dynamic LDS bytes of the solver launch (largest LDS-resident item)
