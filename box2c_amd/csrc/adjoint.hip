@@ -586,7 +586,7 @@ __device__ void integrate_position_adj( const AdjScratch& a, int i, const float*
 // ---- kernels ----
 
 // One workgroup per work item: the step's solver from poseIn to poseOut (no tape).
-__global__ void __launch_bounds__( kSolveMaxBlock ) k_replay_step( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+__global__ void __launch_bounds__( kAdjointMaxBlock ) k_replay_step( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 																   const int* __restrict__ bodyList, const int2* __restrict__ slotBodies,
 																   const DevContact* __restrict__ snap, const StepPose* __restrict__ poseIn,
 																   StepPose* __restrict__ poseOut, const int* __restrict__ simBase,
@@ -616,7 +616,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_replay_step( const DevWorl
 // stored impulses through the warm-start links (prevSlot: per slot of this step, the previous
 // step's slot of the same contact, or -1). paramGrad accumulates per world d/d(friction) and
 // d/d(restitution).
-__global__ void __launch_bounds__( kSolveMaxBlock ) k_adjoint_step( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+__global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 																	const int* __restrict__ bodyList, const int2* __restrict__ slotBodies,
 																	const DevContact* __restrict__ snap, const StepPose* __restrict__ poseIn,
 																	StepPose* __restrict__ adj, const int* __restrict__ simBase,
@@ -937,7 +937,7 @@ hipError_t launch_replay_step( const DevWorld* worlds, const WorkItem* items, in
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
-	if ( blockSize < 64 || blockSize > kSolveMaxBlock )
+	if ( blockSize < 64 || blockSize > kAdjointMaxBlock )
 		return hipErrorInvalidValue;
 	hipLaunchKernelGGL( k_replay_step, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap, poseIn,
 						poseOut, simBase, cf, cfCapacity, priv, mass );
@@ -952,7 +952,7 @@ hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, i
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
-	if ( blockSize < 64 || blockSize > kSolveMaxBlock )
+	if ( blockSize < 64 || blockSize > kAdjointMaxBlock )
 		return hipErrorInvalidValue;
 	hipLaunchKernelGGL( k_adjoint_step, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap, poseIn,
 						adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, mass, scratch, scratchOffset );

@@ -252,8 +252,10 @@ struct StepPose
 // 160 KiB of LDS; the default budget leaves room for two or more items per CU.
 constexpr int kMaxItemLdsBytes = 160 * 1024;
 constexpr int kDefaultItemLdsBytes = 64 * 1024;
-// threads per solver workgroup: at most this many (the kernel's launch bound)
-constexpr int kSolveMaxBlock = 256;
+// threads per solver workgroup: at most this many (the step kernel's launch bound; the replay
+// and adjoint kernels, which hold more state per thread, take at most kAdjointMaxBlock)
+constexpr int kSolveMaxBlock = 1024;
+constexpr int kAdjointMaxBlock = 256;
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {
 	return 40 * bodyCount + 8 * slotCount + 4 * cfFloatCount * slotCount;

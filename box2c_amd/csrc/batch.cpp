@@ -46,7 +46,7 @@ static void setShape( StepGroup* g, int blockSize, int targetSlots, int ldsBudge
 // measured steps; the candidate with the lowest solver-kernel time (device events) is kept.
 static float autotune( StepGroup* g, float timeStep, int subStepCount, int trialSteps )
 {
-	static const int blocks[] = { 0, 128, 256 };
+	static const int blocks[] = { 0, 128, 256, 512, 1024 };
 	static const int targets[] = { 128, 256, 512 };
 	static const int budgets[] = { 0, 64 * 1024, 96 * 1024, 160 * 1024 };
 	float best = 1e30f;

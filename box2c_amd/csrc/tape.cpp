@@ -154,7 +154,7 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 		v.itemCount = g->itemCount;
 		v.slotCount = g->slotCount;
 		v.bodyListCount = g->bodyListCount;
-		v.block = blockSize;
+		v.block = std::min( blockSize, kAdjointMaxBlock );
 		std::vector<int> itemsOfWorld( g->worlds.size(), 0 );
 		std::vector<int> slotWorld( std::max( g->slotCount, 1 ), 0 );
 		std::vector<int> offsets( std::max( g->itemCount, 1 ), 0 );

@@ -64,7 +64,7 @@ B2_API int b2mi_Batch_GetWorldCount( int batch );
 // Steps every world of the batch (the batch counterpart of b2World_Step).
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount );
 
-// Solver launch shape: threads per workgroup (0 = automatic, else a multiple of 64 up to 256)
+// Solver launch shape: threads per workgroup (0 = automatic, else a multiple of 64 up to 1024)
 // and the number of contact constraints packed into one work item (0 = 256).
 B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots );
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots );
@@ -79,8 +79,8 @@ B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
 // Autotuner: steps the batch (or world) under each candidate launch shape (threads per
 // workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
 // rebuild step, and keeps the shape with the fastest solver kernel. Every shape gives
-// bit-identical results, so these are ordinary simulation steps (3 x 3 x 4 candidates, i.e.
-// 36 x (1 + trialSteps) steps). Returns the best solver-kernel time in milliseconds.
+// bit-identical results, so these are ordinary simulation steps (5 x 3 x 4 candidates, i.e.
+// 60 x (1 + trialSteps) steps). Returns the best solver-kernel time in milliseconds.
 B2_API float b2mi_Batch_Autotune( int batch, float timeStep, int subStepCount, int trialSteps );
 B2_API float b2mi_World_Autotune( b2WorldId worldId, float timeStep, int subStepCount, int trialSteps );
 B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget );

@@ -155,7 +155,7 @@ def test_batch_large_lds_budget_parity():
 
 
 def test_batch_autotune_keeps_parity():
-    """The autotuner's trial steps are ordinary steps: after tuning (36 candidate shapes, one
+    """The autotuner's trial steps are ordinary steps: after tuning (60 candidate shapes, one
     rebuild step and one measured step each) and more steps under the chosen shape, every world
     still matches the oracle bit for bit."""
     gpu, cpu = helpers.product(), helpers.oracle()
@@ -166,9 +166,27 @@ def test_batch_autotune_keeps_parity():
         assert best > 0.0
         gpu.batch_step(batch, n=5)
         for wc in worlds_c:
-            cpu.step(wc, n=3 + 36 * 2 + 5)
+            cpu.step(wc, n=3 + 60 * 2 + 5)
         _compare_worlds(gpu, cpu, worlds_g, worlds_c, "autotune")
     finally:
         gpu.lib.b2mi_DestroyBatch(batch)
         for wc in worlds_c:
             cpu.destroy(wc)
+
+
+@pytest.mark.parametrize("block", [64, 512, 1024])
+def test_step_parity_block_sizes(block):
+    """Any solver workgroup size gives the oracle's bits (strided loops over slots and bodies)."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    wg, _ = helpers.make_world(gpu, "large_pyramid", {"base": 20}, {})
+    wc, _ = helpers.make_world(cpu, "large_pyramid", {"base": 20}, {})
+    gpu.lib.b2mi_World_Configure(wg, block, 0)
+    try:
+        for s in range(30):
+            gpu.step(wg)
+            cpu.step(wc)
+        assert helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc)) is None
+        assert helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc)) is None
+    finally:
+        gpu.destroy(wg)
+        cpu.destroy(wc)
