@@ -159,6 +159,7 @@ struct WorkItem
 	int groupStart[kGroups + 1]; // relative to slotStart
 	int ldsBytes;				 // LDS bytes of the item's store (0 for kStoreHbm)
 	int store;					 // kStoreHbm, kStoreBodiesLds or kStoreLds
+	int block;					 // threads of its workgroup when the launch shape is per item
 };
 
 // Where a work item keeps its solver state during the step
@@ -251,7 +252,7 @@ struct StepPose
 // Items above the budget keep the read-only constraint fields in HBM, or everything. A CDNA4 CU has
 // 160 KiB of LDS; the default budget leaves room for two or more items per CU.
 constexpr int kMaxItemLdsBytes = 160 * 1024;
-constexpr int kDefaultItemLdsBytes = 64 * 1024;
+constexpr int kDefaultItemLdsBytes = 160 * 1024;
 // threads per solver workgroup: at most this many (the step kernel's launch bound; the replay
 // and adjoint kernels, which hold more state per thread, take at most kAdjointMaxBlock)
 constexpr int kSolveMaxBlock = 1024;

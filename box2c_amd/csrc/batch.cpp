@@ -140,7 +140,7 @@ B2_API int b2mi_Batch_GetWorldCount( int batch )
 	return (int)getBatch( batch )->worlds.size();
 }
 
-// blockSize: threads per workgroup (0 = chosen from the largest colour group);
+// blockSize: threads per workgroup (0 = per work item, from its largest colour group);
 // targetSlots: islands are packed into one work item up to this many constraints (0 = 256).
 B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots )
 {
@@ -151,7 +151,7 @@ B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots )
 		w->topologyDirty = true;
 }
 
-// maxBytes: largest LDS footprint of a work item kept in LDS (-1 = 64 KiB default, 0 = every
+// maxBytes: largest LDS footprint of a work item kept in LDS (-1 = 160 KiB default, 0 = every
 // work item keeps its solver state in HBM). Results are identical either way.
 B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes )
 {

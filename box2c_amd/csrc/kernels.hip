@@ -540,11 +540,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
-												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity )
+												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
+												   int itemOffset )
 {
 	extern __shared__ float4 lds[];
 	// read in place (uniform scalar loads); a private copy would be indexed dynamically from scratch
-	const WorkItem& wi = items[blockIdx.x];
+	const WorkItem& wi = items[itemOffset + blockIdx.x];
 	const DevWorld& W = worlds[wi.world];
 	const int* bl = bodyList + wi.bodyStart;
 	const int* sc = slotContact + wi.slotStart;
@@ -617,7 +618,7 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 	return hipGetLastError();
 }
 
-hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, int ldsBytes,
+hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize, int ldsBytes,
 						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
 						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
 {
@@ -635,7 +636,7 @@ hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int item
 		allowedLds = kMaxItemLdsBytes;
 	}
 	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
-						slotBodies, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity );
+						slotBodies, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset );
 	return hipGetLastError();
 }
 

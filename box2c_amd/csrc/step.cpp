@@ -638,6 +638,7 @@ static void buildWorldItems( World* w, int target, int budget )
 		auto resolve = [&]( int li ) { return li <= -2 ? posOfSim[-2 - li] : li; };
 
 		wi.slotStart = (int)L.slotContact.size();
+		int itemMaxGroup = 0;
 		// group 0 = overflow (colour 11) then colours 0..10
 		for ( int gidx = 0; gidx < kGroups; ++gidx )
 		{
@@ -654,7 +655,12 @@ static void buildWorldItems( World* w, int target, int budget )
 				}
 			int size = (int)L.slotContact.size() - wi.slotStart - wi.groupStart[gidx];
 			L.maxGroupSize = std::max( L.maxGroupSize, size );
+			itemMaxGroup = std::max( itemMaxGroup, size );
 		}
+		// per-item workgroup: one thread per slot of its largest colour group
+		wi.block = 64;
+		while ( wi.block < itemMaxGroup && wi.block < kSolveMaxBlock )
+			wi.block *= 2;
 		wi.groupStart[kGroups] = (int)L.slotContact.size() - wi.slotStart;
 		wi.bodyCount = (int)L.bodyList.size() - wi.bodyStart;
 		{
@@ -844,13 +850,26 @@ static void publishLists( StepGroup* g, bool itemsChanged )
 			g->maxGroupSize = std::max( g->maxGroupSize, w->solver.maxGroupSize );
 			g->maxItemLds = std::max( g->maxItemLds, w->solver.maxItemLds );
 		}
-		// Longest items first: the hardware dispatches workgroups in order, so the few large
-		// islands of an uneven batch start at once and the small ones fill the CUs behind them.
-		// Items are independent; their order does not change any result.
+		// Grouped by workgroup size (one launch per size, largest size first) and longest first
+		// within a size: the hardware dispatches workgroups in order, so the few large islands of an
+		// uneven batch start at once and the small ones fill the CUs behind them. Items are
+		// independent; their order does not change any result.
 		std::stable_sort( g->hItems.begin(), g->hItems.end(), []( const WorkItem& a, const WorkItem& b ) {
+			if ( a.block != b.block )
+				return a.block > b.block;
 			int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
 			return ca > cb;
 		} );
+		g->classes.clear();
+		for ( int i = 0; i < (int)g->hItems.size(); ++i )
+		{
+			const WorkItem& wi = g->hItems[i];
+			if ( g->classes.empty() || g->classes.back().block != wi.block )
+				g->classes.push_back( LaunchClass{ i, 0, wi.block, 0 } );
+			LaunchClass& c = g->classes.back();
+			c.count += 1;
+			c.lds = std::max( c.lds, wi.ldsBytes );
+		}
 		g->itemCount = (int)g->hItems.size();
 		g->itemsVersion += 1;
 		grow( g->dItems, g->capItems, std::max( g->itemCount, 1 ) );
@@ -1295,19 +1314,49 @@ void StepGroup::step( float timeStep, int subStepCount )
 	for ( World* w : worlds )
 		totalShapes += (int)w->shapes.size();
 	grow( dEnlarge, capEnlarge, std::max( totalShapes, 1 ) );
-	int block = blockSize;
-	if ( block <= 0 )
-	{
-		block = 64;
-		while ( block < maxGroupSize && block < kSolveMaxBlock )
-			block *= 2;
-	}
-	B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
+	// one launch for every item at a fixed workgroup size, or one per item size class (the items'
+	// own sizes, blockSize 0), those on separate streams so that they run side by side
+	int tapeBlock = kAdjointMaxBlock;
+	if ( blockSize > 0 )
+		B2MI_CHECK( blockSize >= 64 && blockSize <= kSolveMaxBlock && blockSize % 64 == 0 );
+	else
+		for ( const LaunchClass& c : classes )
+			B2MI_CHECK( c.block >= 64 && c.block <= kSolveMaxBlock && c.block % 64 == 0 );
+	if ( blockSize > 0 )
+		tapeBlock = std::min( blockSize, kAdjointMaxBlock );
 	if ( tape )
-		tapeRecordStep( this, timeStep, subStepCount, block );
+		tapeRecordStep( this, timeStep, subStepCount, tapeBlock );
 	HIP_CHECK( hipEventRecord( e3, s ) );
-	HIP_CHECK( launch_solve( dWorlds, dItems, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons, capCons,
-							 dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, s ) );
+	if ( blockSize > 0 || classes.size() <= 1 )
+	{
+		int block = blockSize > 0 ? blockSize : ( classes.empty() ? 64 : classes[0].block );
+		HIP_CHECK( launch_solve( dWorlds, dItems, 0, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons,
+								 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, s ) );
+	}
+	else
+	{
+		if ( forkEvent == nullptr )
+		{
+			HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
+			for ( hipEvent_t& e : joinEvents )
+				HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+		}
+		HIP_CHECK( hipEventRecord( forkEvent, s ) );
+		int n = (int)classes.size();
+		for ( int c = 0; c < n; ++c )
+		{
+			const LaunchClass& lc = classes[c];
+			hipStream_t cs = c == 0 ? s : auxStream( ( c - 1 ) % kAuxStreams );
+			if ( c > 0 )
+				HIP_CHECK( hipStreamWaitEvent( cs, forkEvent, 0 ) );
+			HIP_CHECK( launch_solve( dWorlds, dItems, lc.first, lc.count, lc.block, lc.lds, dBodyList, dSlotContact, dSlotBodies, dCons,
+									 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, cs ) );
+			if ( c > 0 )
+				HIP_CHECK( hipEventRecord( joinEvents[( c - 1 ) % kAuxStreams], cs ) );
+		}
+		for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
+			HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
+	}
 	HIP_CHECK( hipEventRecord( e2, s ) );
 	HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
 	HIP_CHECK( hipStreamSynchronize( s ) );

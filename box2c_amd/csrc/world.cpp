@@ -57,6 +57,18 @@ hipStream_t stream()
 	return g_stream;
 }
 
+hipStream_t auxStream( int i )
+{
+	static hipStream_t streams[kAuxStreams] = {};
+	static std::once_flag once;
+	std::call_once( once, [] {
+		for ( hipStream_t& st : streams )
+			HIP_CHECK( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) );
+	} );
+	B2MI_CHECK( 0 <= i && i < kAuxStreams );
+	return streams[i];
+}
+
 template <typename T>
 static void devGrow( T*& ptr, int& capacity, int needed, bool preserve, int preserveCount = 0 )
 {
@@ -135,6 +147,11 @@ StepGroup::~StepGroup()
 	devFree( dPriv );
 	devFree( dMass );
 	for ( hipEvent_t e : events )
+		if ( e )
+			(void)hipEventDestroy( e );
+	if ( forkEvent )
+		(void)hipEventDestroy( forkEvent );
+	for ( hipEvent_t e : joinEvents )
 		if ( e )
 			(void)hipEventDestroy( e );
 }

@@ -250,6 +250,14 @@ struct World
 
 struct Tape; // taped run of a step group (tape.cpp)
 
+// items of one solver workgroup size: a contiguous range of the launch-ordered items
+struct LaunchClass
+{
+	int first, count, block, lds;
+};
+
+constexpr int kAuxStreams = 3;
+
 // a set of worlds advanced together by one narrowphase launch and one solver launch
 struct StepGroup
 {
@@ -310,6 +318,9 @@ struct StepGroup
 	// timing of the last step (ms)
 	float msCollide = 0, msSolve = 0, msHost = 0;
 	hipEvent_t events[4] = { nullptr, nullptr, nullptr, nullptr };
+	std::vector<LaunchClass> classes;
+	hipEvent_t forkEvent = nullptr;
+	hipEvent_t joinEvents[kAuxStreams] = {};
 
 	~StepGroup();
 	void step( float timeStep, int subStepCount );
@@ -320,6 +331,7 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 void tapeFree( StepGroup* g );
 
 hipStream_t stream();
+hipStream_t auxStream( int i ); // extra streams (default priority) for concurrent launches
 World* getWorld( b2WorldId id );
 World* worldAt( int slot );
 

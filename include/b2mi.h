@@ -64,14 +64,16 @@ B2_API int b2mi_Batch_GetWorldCount( int batch );
 // Steps every world of the batch (the batch counterpart of b2World_Step).
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount );
 
-// Solver launch shape: threads per workgroup (0 = automatic, else a multiple of 64 up to 1024)
-// and the number of contact constraints packed into one work item (0 = 256).
+// Solver launch shape: threads per workgroup (a multiple of 64 up to 1024, or 0 = each work item
+// its own: the power of two at or above its largest colour group, items of one size launched
+// together and the sizes side by side on separate streams) and the number of contact
+// constraints packed into one work item (0 = 256).
 B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots );
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots );
 B2_API int b2mi_Batch_GetWorkItemCount( int batch );
 
 // Largest LDS footprint (bytes, at most 160 KiB) of a work item whose solver state is kept in
-// LDS; larger work items keep it in HBM. -1 = the 64 KiB default, 0 = HBM for every work item.
+// LDS; larger work items keep it in HBM. -1 = the 160 KiB default, 0 = HBM for every work item.
 // The results are bit-identical either way; only the speed differs.
 B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes );
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );

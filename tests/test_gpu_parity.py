@@ -174,6 +174,25 @@ def test_batch_autotune_keeps_parity():
             cpu.destroy(wc)
 
 
+@pytest.mark.parametrize("target", [128, 1024])
+def test_batch_per_item_block_sizes_parity(target):
+    """Workgroup size per work item (blockSize 0): items of each size go to their own launch,
+    the launches run side by side on separate streams, and every world keeps the oracle's bits."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 12, 3)
+    try:
+        gpu.lib.b2mi_Batch_Configure(batch, 0, target)
+        for s in range(30):
+            gpu.batch_step(batch)
+            for wc in worlds_c:
+                cpu.step(wc)
+        _compare_worlds(gpu, cpu, worlds_g, worlds_c, f"per-item target {target}")
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
+
+
 @pytest.mark.parametrize("block", [64, 512, 1024])
 def test_step_parity_block_sizes(block):
     """Any solver workgroup size gives the oracle's bits (strided loops over slots and bodies)."""
