@@ -859,6 +859,12 @@ __global__ void k_link_slots( const int* __restrict__ slotContact, const int* __
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( k >= slotCount )
 		return;
+	// slots between the worlds' regions (world -1) hold no contact
+	if ( slotWorld[k] < 0 )
+	{
+		prevSlot[k] = -1;
+		return;
+	}
 	int2 e = slotOfContact[contactBase[slotWorld[k]] + slotContact[k]];
 	prevSlot[k] = e.x == prevStep ? e.y : -1;
 }
@@ -868,7 +874,7 @@ __global__ void k_mark_slots( const int* __restrict__ slotContact, const int* __
 							  const int* __restrict__ contactBase, int step, int2* __restrict__ slotOfContact )
 {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( k >= slotCount )
+	if ( k >= slotCount || slotWorld[k] < 0 )
 		return;
 	slotOfContact[contactBase[slotWorld[k]] + slotContact[k]] = make_int2( step, k );
 }
@@ -879,7 +885,7 @@ __global__ void k_patch_params( const DevContact* __restrict__ src, DevContact* 
 								int slotCount, const float* __restrict__ params )
 {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( k >= slotCount )
+	if ( k >= slotCount || slotWorld[k] < 0 )
 		return;
 	DevContact c = src[k];
 	c.friction = params[2 * slotWorld[k]];

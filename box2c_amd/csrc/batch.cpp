@@ -129,6 +129,7 @@ B2_API b2WorldId b2mi_Batch_CreateWorld( int batch, const b2WorldDef* def )
 		return b2WorldId{ 0, 0 };
 	w->batchWorld = true;
 	w->group = g;
+	w->mirrorEpoch = w->contactsEpoch = g->epoch;
 	w->groupIndex = (int)g->worlds.size();
 	g->worlds.push_back( w );
 	g->structureDirty = true;
@@ -157,6 +158,17 @@ B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes )
 {
 	StepGroup* g = getBatch( batch );
 	g->ldsBudget = maxBytes;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+}
+
+// fullMaxBytes: largest LDS footprint of a work item that keeps all its solver state in LDS (-1 =
+// the LDS budget); larger items within the budget keep bodies and impulses in LDS and read their
+// constant constraint fields from HBM. Smaller footprints let more work items share a CU.
+B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes )
+{
+	StepGroup* g = getBatch( batch );
+	g->ldsFullBudget = fullMaxBytes;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
 }
@@ -209,6 +221,43 @@ B2_API float b2mi_Batch_Autotune( int batch, float timeStep, int subStepCount, i
 	if ( g->worlds.empty() )
 		return 0.0f;
 	return autotune( g, timeStep, subStepCount, trialSteps );
+}
+
+// The solver launches of the last step: per launch, (first item, item count, threads per
+// workgroup, dynamic LDS bytes). Returns the launch count; writes at most `capacity` records.
+B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity )
+{
+	StepGroup* g = getBatch( batch );
+	int n = 0;
+	auto put = [&]( int first, int count, int block, int lds ) {
+		if ( records != nullptr && n < capacity )
+		{
+			records[4 * n + 0] = first;
+			records[4 * n + 1] = count;
+			records[4 * n + 2] = block;
+			records[4 * n + 3] = lds;
+		}
+		n += 1;
+	};
+	if ( g->blockSize > 0 || g->classes.size() <= 1 )
+	{
+		if ( g->itemCount > 0 )
+			put( 0, g->itemCount, g->blockSize > 0 ? g->blockSize : g->classes[0].block, g->maxItemLds );
+	}
+	else
+		for ( const LaunchClass& c : g->classes )
+			put( c.first, c.count, c.block, c.lds );
+	return n;
+}
+
+// Host clock of the last step (ms since it began) at ten points: pair update started, collide
+// lists started, narrowphase launched, narrowphase wait started, wait returned, solver setup
+// started, solver launched, solver wait started, wait returned, enlarge processing done.
+B2_API void b2mi_Batch_GetHostTimeline( int batch, float* marks )
+{
+	StepGroup* g = getBatch( batch );
+	for ( int i = 0; i < 10; ++i )
+		marks[i] = g->hostPhases[i];
 }
 
 B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget )

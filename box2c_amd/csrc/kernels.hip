@@ -541,9 +541,11 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
 												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
-												   int itemOffset )
+												   int itemOffset, const int* abortFlag )
 {
 	extern __shared__ float4 lds[];
+	if ( abortFlag != nullptr && *abortFlag != 0 )
+		return;
 	// read in place (uniform scalar loads); a private copy would be indexed dynamically from scratch
 	const WorkItem& wi = items[itemOffset + blockIdx.x];
 	const DevWorld& W = worlds[wi.world];
@@ -620,7 +622,8 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 
 hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize, int ldsBytes,
 						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
-						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
+						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, const int* abortFlag,
+						 hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
@@ -636,7 +639,7 @@ hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int item
 		allowedLds = kMaxItemLdsBytes;
 	}
 	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
-						slotBodies, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset );
+						slotBodies, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag );
 	return hipGetLastError();
 }
 

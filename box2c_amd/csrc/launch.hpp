@@ -14,7 +14,8 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 						   hipStream_t stream );
 hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize, int ldsBytes,
 						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
-						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
+						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, const int* abortFlag,
+						 hipStream_t stream );
 
 // replay and adjoint (adjoint.hip)
 int adjoint_scratch_floats( int bodyCount, int slotCount, int subSteps );

@@ -145,7 +145,7 @@ static b2ShapeId createShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 	if ( def->isSensor )
 		fatal( "sensor shapes are not supported by the device pipeline", __FILE__, __LINE__ );
 	World* w = worldAt( bodyId.world0 );
-	if ( w->locked )
+	if ( worldLocked( w ) )
 		return b2ShapeId{ 0, 0, 0 };
 	int bodyIndex = bodyId.index1 - 1;
 	B2MI_CHECK( 0 <= bodyIndex && bodyIndex < (int)w->bodies.size() );
@@ -512,7 +512,7 @@ static int findRoot( std::vector<int>& parent, int x )
 // `target` constraint slots, each item's slots grouped as overflow first, then colours 0..10.
 // Writes w->solver with indices local to its lists; runs on any host thread (touches this
 // world only).
-static void buildWorldItems( World* w, int target, int budget )
+static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 {
 	SolverLists& L = w->solver;
 	L.items.clear();
@@ -666,7 +666,7 @@ static void buildWorldItems( World* w, int target, int budget )
 		{
 			int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
 			int bodyBytes = itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
-			if ( bytes <= budget )
+			if ( bytes <= budget && bytes <= fullBudget )
 			{
 				wi.store = kStoreLds;
 				wi.ldsBytes = bytes;
@@ -797,13 +797,14 @@ static void uploadSpan( T* dst, const std::vector<T>& src, int first, int last, 
 		HIP_CHECK( hipMemcpyAsync( dst + first, src.data() + first, sizeof( T ) * (size_t)( last - first ), hipMemcpyHostToDevice, s ) );
 }
 
-// Writes the pending worlds' lists into their regions (absolute indices) and uploads the span
-// they cover; rebuilds the launch-ordered item array when any world's items changed.
-static void publishLists( StepGroup* g, bool itemsChanged )
+// Writes the pending worlds' lists (those of `pending`, or of every world) into their regions
+// (absolute indices) and uploads the span they cover; rebuilds the launch-ordered item array
+// when any world's items changed.
+static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool itemsChanged )
 {
 	hipStream_t s = stream();
 	int bFirst = INT32_MAX, bLast = 0, sFirst = INT32_MAX, sLast = 0, cFirst = INT32_MAX, cLast = 0;
-	for ( World* w : g->worlds )
+	for ( World* w : pending ? *pending : g->worlds )
 	{
 		if ( w->solverPending )
 		{
@@ -1167,6 +1168,70 @@ static void splitByWorld( const std::vector<R>& recs, int worldCount, std::vecto
 	}
 }
 
+// Worlds picked out by a parallel pass over a group (one slot per world, claimed through an
+// atomic counter), then put back in group order.
+struct WorldPick
+{
+	std::vector<World*> list;
+	std::atomic<int> count{ 0 };
+	void reset( size_t n )
+	{
+		list.resize( n );
+		count.store( 0 );
+	}
+	void add( World* w )
+	{
+		list[count.fetch_add( 1 )] = w;
+	}
+	void finish()
+	{
+		list.resize( count.load() );
+		std::sort( list.begin(), list.end(), []( const World* a, const World* b ) { return a->groupIndex < b->groupIndex; } );
+	}
+};
+
+// Solver launches of one step: one launch for every item at a fixed workgroup size, or one per
+// item size class (the items' own sizes, blockSize 0), those on separate streams so that they
+// run side by side. With `abortFlag` set every workgroup returns at once when the flag is
+// nonzero (a speculative launch behind the narrowphase, void when contacts changed state).
+void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
+{
+	if ( blockSize > 0 )
+		B2MI_CHECK( blockSize >= 64 && blockSize <= kSolveMaxBlock && blockSize % 64 == 0 );
+	else
+		for ( const LaunchClass& c : classes )
+			B2MI_CHECK( c.block >= 64 && c.block <= kSolveMaxBlock && c.block % 64 == 0 );
+	if ( blockSize > 0 || classes.size() <= 1 )
+	{
+		int block = blockSize > 0 ? blockSize : ( classes.empty() ? 64 : classes[0].block );
+		HIP_CHECK( launch_solve( dWorlds, dItems, 0, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons,
+								 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, s ) );
+		return;
+	}
+	if ( forkEvent == nullptr )
+	{
+		HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
+		for ( hipEvent_t& e : joinEvents )
+			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+	}
+	HIP_CHECK( hipEventRecord( forkEvent, s ) );
+	int n = (int)classes.size();
+	for ( int c = 0; c < n; ++c )
+	{
+		const LaunchClass& lc = classes[c];
+		hipStream_t cs = c == 0 ? s : auxStream( ( c - 1 ) % kAuxStreams );
+		if ( c > 0 && c <= kAuxStreams )
+			HIP_CHECK( hipStreamWaitEvent( cs, forkEvent, 0 ) );
+		HIP_CHECK( launch_solve( dWorlds, dItems, lc.first, lc.count, lc.block, lc.lds, dBodyList, dSlotContact, dSlotBodies, dCons,
+								 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
+	}
+	for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
+	{
+		HIP_CHECK( hipEventRecord( joinEvents[c - 1], auxStream( c - 1 ) ) );
+		HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
+	}
+}
+
 void StepGroup::step( float timeStep, int subStepCount )
 {
 	auto t0 = std::chrono::steady_clock::now();
@@ -1179,47 +1244,86 @@ void StepGroup::step( float timeStep, int subStepCount )
 	//   finalizeBodies enlarged-proxy processing
 	float msPairs = 0.0f, msBroad = 0.0f, msChanges = 0.0f, msEnlarge = 0.0f;
 	hipStream_t s = stream();
-	for ( World* w : worlds )
+	B2MI_CHECK( stepping == false );
+	stepping = true;
+	profile = b2Profile{};
+	const int nw = (int)worlds.size();
+	if ( structureDirty || (int)hWorlds.size() != nw )
 	{
-		B2MI_CHECK( w->locked == false );
-		w->locked = true;
-		w->profile = b2Profile{};
+		hWorlds.resize( nw );
+		structureDirty = true;
+	}
+	const int target = targetSlots > 0 ? targetSlots : 256;
+	const int budget = ldsBudget < 0 ? kDefaultItemLdsBytes : std::min( ldsBudget, kMaxItemLdsBytes );
+	const int fullBudget = ldsFullBudget < 0 ? budget : std::min( ldsFullBudget, budget );
+	const bool reshape = target != builtTarget || budget != builtBudget || fullBudget != builtFullBudget;
+	builtTarget = target;
+	builtBudget = budget;
+	builtFullBudget = fullBudget;
+
+	// 1. per world in parallel: pair update and new contacts (src/broad_phase.c:333-436), the
+	//    narrowphase list when contacts were created or destroyed, and the world's device
+	//    parameters. Worlds that need uploads or device allocations finish serially after it.
+	hostPhases[0] = msSince( t0 );
+	auto tp = std::chrono::steady_clock::now();
+	WorldPick serial, collideDirty, topology;
+	serial.reset( nw );
+	collideDirty.reset( nw );
+	topology.reset( nw );
+	std::atomic<bool> paramsChanged{ false }, regionFull{ false };
+	std::atomic<int> shapeTotal{ 0 };
+	auto refreshParams = [&]( World* w ) {
+		DevWorld d;
+		fillDevWorld( d, w, timeStep, subStepCount );
+		DevWorld& h = hWorlds[w->groupIndex];
+		if ( memcmp( &d, &h, sizeof( DevWorld ) ) != 0 )
+		{
+			h = d;
+			paramsChanged.store( true );
+		}
+	};
+	forEachWorld( this, [&]( World* w ) {
+		updatePairs( w );
+		if ( w->contactsDirty )
+		{
+			buildWorldCollide( w );
+			collideDirty.add( w );
+		}
+		if ( (int)w->collideIds.size() > w->collideRegion.cap )
+			regionFull.store( true );
+		if ( reshape )
+			w->topologyDirty = true;
+		if ( w->topologyDirty )
+			topology.add( w );
+		shapeTotal.fetch_add( (int)w->shapes.size() );
+		if ( w->bodiesDirty || !w->pendingInits.empty() || w->contactIdPool.nextIndex > w->capContacts || w->dContacts == nullptr )
+			serial.add( w );
+		else
+			refreshParams( w );
+	} );
+	serial.finish();
+	collideDirty.finish();
+	topology.finish();
+	hInits.clear();
+	for ( World* w : serial.list )
+	{
 		if ( w->bodiesDirty )
 			b2mi_internal_uploadBodies( w );
-	}
-
-	// 1. pairs and new contacts (host, per world in parallel)
-	auto tp = std::chrono::steady_clock::now();
-	forEachWorld( this, []( World* w ) { updatePairs( w ); } );
-	hInits.clear();
-	for ( World* w : worlds )
-	{
 		ensureContactCapacity( w );
 		hInits.insert( hInits.end(), w->pendingInits.begin(), w->pendingInits.end() );
 		w->pendingInits.clear();
+		refreshParams( w );
 	}
 	msPairs = msSince( tp );
-	if ( structureDirty || (int)hWorlds.size() != (int)worlds.size() )
+	if ( structureDirty )
 	{
-		hWorlds.resize( worlds.size() );
-		structureDirty = true;
 		layoutValid = false;
+		paramsChanged.store( true );
 	}
-	bool paramsChanged = structureDirty;
-	for ( size_t i = 0; i < worlds.size(); ++i )
+	if ( paramsChanged.load() )
 	{
-		DevWorld d;
-		fillDevWorld( d, worlds[i], timeStep, subStepCount );
-		if ( memcmp( &d, &hWorlds[i], sizeof( DevWorld ) ) != 0 )
-		{
-			hWorlds[i] = d;
-			paramsChanged = true;
-		}
-	}
-	if ( paramsChanged )
-	{
-		grow( dWorlds, capWorlds, (int)worlds.size() );
-		HIP_CHECK( hipMemcpyAsync( dWorlds, hWorlds.data(), sizeof( DevWorld ) * worlds.size(), hipMemcpyHostToDevice, s ) );
+		grow( dWorlds, capWorlds, std::max( nw, 1 ) );
+		HIP_CHECK( hipMemcpyAsync( dWorlds, hWorlds.data(), sizeof( DevWorld ) * nw, hipMemcpyHostToDevice, s ) );
 		structureDirty = false;
 	}
 	if ( dCounters == nullptr )
@@ -1235,26 +1339,35 @@ void StepGroup::step( float timeStep, int subStepCount )
 	}
 
 	// 2. narrowphase over the worlds' contact lists
-	forEachWorld( this, []( World* w ) {
-		if ( w->contactsDirty )
-			buildWorldCollide( w );
-	} );
-	bool relayout = !layoutValid;
-	for ( World* w : worlds )
-		relayout = relayout || (int)w->collideIds.size() > w->collideRegion.cap;
+	hostPhases[1] = msSince( t0 );
+	bool relayout = !layoutValid || regionFull.load();
 	if ( relayout )
 		layoutGroup( this );
-	publishLists( this, relayout );
+	publishLists( this, relayout ? nullptr : &collideDirty.list, relayout );
 	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
+	grow( dEnlarge, capEnlarge, std::max( shapeTotal.load(), 1 ) );
 	// events bracket each kernel launch alone (the stream is idle while the host works)
 	if ( events[0] == nullptr )
 		for ( hipEvent_t& e : events )
 			HIP_CHECK( hipEventCreate( &e ) );
 	hipEvent_t e0 = events[0], e1 = events[1], e2 = events[2], e3 = events[3];
+	hostPhases[2] = msSince( t0 );
 	HIP_CHECK( hipEventRecord( e0, s ) );
 	HIP_CHECK( launch_collide( dWorlds, dCollideList, collideCount, dChanges, dCounters, s ) );
 	HIP_CHECK( hipEventRecord( e1, s ) );
-	HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	// When no work item needs rebuilding, the solver goes right behind the narrowphase: a step
+	// in which no contact begins or ends touching (the usual one) then needs no host round trip.
+	// Otherwise the launch returns at once and the solver runs after the state changes below.
+	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0;
+	if ( speculate )
+	{
+		HIP_CHECK( hipEventRecord( e3, s ) );
+		launchSolve( s, dCounters );
+		HIP_CHECK( hipEventRecord( e2, s ) );
+		HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ) * 2, hipMemcpyDeviceToHost, s ) );
+	}
+	else
+		HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ), hipMemcpyDeviceToHost, s ) );
 
 	// the tree rebuild overlaps the narrowphase (src/world.c:367-379, 420)
 	auto tb = std::chrono::steady_clock::now();
@@ -1263,9 +1376,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 		w->trees[b2_kinematicBody].rebuild( false );
 	} );
 	msBroad = msSince( tb );
+	hostPhases[3] = msSince( t0 );
 	HIP_CHECK( hipStreamSynchronize( s ) );
 	auto tc = std::chrono::steady_clock::now();
-	int changeCount = hCounters[0];
+	hostPhases[4] = msSince( t0 );
+	const int changeCount = hCounters[0];
 	if ( changeCount > 0 )
 	{
 		hChanges.resize( changeCount );
@@ -1275,91 +1390,53 @@ void StepGroup::step( float timeStep, int subStepCount )
 			return a.world != b.world ? a.world < b.world : a.contactId < b.contactId;
 		} );
 		std::vector<int> first, last;
-		splitByWorld( hChanges, (int)worlds.size(), first, last );
-		HostPool::get().run( (int)worlds.size(), [&]( int i ) {
+		splitByWorld( hChanges, nw, first, last );
+		HostPool::get().run( nw, [&]( int i ) {
 			if ( last[i] > first[i] )
 				applyChanges( worlds[i], hChanges.data() + first[i], last[i] - first[i] );
 		} );
 	}
 
 	// 3. solver (one workgroup per work item); rebuild the items of worlds whose graph changed
-	int target = targetSlots > 0 ? targetSlots : 256;
-	int budget = ldsBudget < 0 ? kDefaultItemLdsBytes : std::min( ldsBudget, kMaxItemLdsBytes );
-	if ( target != builtTarget || budget != builtBudget )
+	hostPhases[5] = msSince( t0 );
+	if ( !speculate || changeCount > 0 )
 	{
-		for ( World* w : worlds )
-			w->topologyDirty = true;
-		builtTarget = target;
-		builtBudget = budget;
-	}
-	bool anyItems = false;
-	for ( World* w : worlds )
-		anyItems = anyItems || w->topologyDirty;
-	if ( anyItems )
-	{
-		forEachWorld( this, [&]( World* w ) {
-			if ( w->topologyDirty )
-				buildWorldItems( w, target, budget );
-		} );
-		relayout = false;
-		for ( World* w : worlds )
-			relayout = relayout || (int)w->solver.bodyList.size() > w->bodyRegion.cap ||
+		if ( changeCount > 0 )
+		{
+			// the state changes may have marked more worlds
+			topology.reset( nw );
+			forEachWorld( this, [&]( World* w ) {
+				if ( w->topologyDirty )
+					topology.add( w );
+			} );
+			topology.finish();
+		}
+		if ( !topology.list.empty() )
+		{
+			HostPool::get().run( (int)topology.list.size(),
+								 [&]( int i ) { buildWorldItems( topology.list[i], target, budget, fullBudget ); } );
+			bool full = false;
+			for ( World* w : topology.list )
+				full = full || (int)w->solver.bodyList.size() > w->bodyRegion.cap ||
 					   (int)w->solver.slotContact.size() > w->slotRegion.cap;
-		if ( relayout )
-			layoutGroup( this );
-		publishLists( this, true );
-	}
-	msChanges = msSince( tc );
-	int totalShapes = 0;
-	for ( World* w : worlds )
-		totalShapes += (int)w->shapes.size();
-	grow( dEnlarge, capEnlarge, std::max( totalShapes, 1 ) );
-	// one launch for every item at a fixed workgroup size, or one per item size class (the items'
-	// own sizes, blockSize 0), those on separate streams so that they run side by side
-	int tapeBlock = kAdjointMaxBlock;
-	if ( blockSize > 0 )
-		B2MI_CHECK( blockSize >= 64 && blockSize <= kSolveMaxBlock && blockSize % 64 == 0 );
-	else
-		for ( const LaunchClass& c : classes )
-			B2MI_CHECK( c.block >= 64 && c.block <= kSolveMaxBlock && c.block % 64 == 0 );
-	if ( blockSize > 0 )
-		tapeBlock = std::min( blockSize, kAdjointMaxBlock );
-	if ( tape )
-		tapeRecordStep( this, timeStep, subStepCount, tapeBlock );
-	HIP_CHECK( hipEventRecord( e3, s ) );
-	if ( blockSize > 0 || classes.size() <= 1 )
-	{
-		int block = blockSize > 0 ? blockSize : ( classes.empty() ? 64 : classes[0].block );
-		HIP_CHECK( launch_solve( dWorlds, dItems, 0, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons,
-								 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, s ) );
+			if ( full )
+				layoutGroup( this );
+			publishLists( this, full ? nullptr : &topology.list, true );
+		}
+		if ( tape )
+			tapeRecordStep( this, timeStep, subStepCount, blockSize > 0 ? std::min( blockSize, kAdjointMaxBlock ) : kAdjointMaxBlock );
+		hostPhases[6] = msSince( t0 );
+		HIP_CHECK( hipEventRecord( e3, s ) );
+		launchSolve( s, nullptr );
+		HIP_CHECK( hipEventRecord( e2, s ) );
+		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		hostPhases[7] = msSince( t0 );
+		HIP_CHECK( hipStreamSynchronize( s ) );
 	}
 	else
-	{
-		if ( forkEvent == nullptr )
-		{
-			HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
-			for ( hipEvent_t& e : joinEvents )
-				HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
-		}
-		HIP_CHECK( hipEventRecord( forkEvent, s ) );
-		int n = (int)classes.size();
-		for ( int c = 0; c < n; ++c )
-		{
-			const LaunchClass& lc = classes[c];
-			hipStream_t cs = c == 0 ? s : auxStream( ( c - 1 ) % kAuxStreams );
-			if ( c > 0 )
-				HIP_CHECK( hipStreamWaitEvent( cs, forkEvent, 0 ) );
-			HIP_CHECK( launch_solve( dWorlds, dItems, lc.first, lc.count, lc.block, lc.lds, dBodyList, dSlotContact, dSlotBodies, dCons,
-									 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, cs ) );
-			if ( c > 0 )
-				HIP_CHECK( hipEventRecord( joinEvents[( c - 1 ) % kAuxStreams], cs ) );
-		}
-		for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
-			HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
-	}
-	HIP_CHECK( hipEventRecord( e2, s ) );
-	HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
-	HIP_CHECK( hipStreamSynchronize( s ) );
+		hostPhases[6] = hostPhases[7] = hostPhases[5];
+	msChanges = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - tc ).count();
+	hostPhases[8] = msSince( t0 );
 	int enlargeCount = hCounters[1];
 	B2MI_CHECK( enlargeCount <= capEnlarge );
 	auto te = std::chrono::steady_clock::now();
@@ -1371,37 +1448,33 @@ void StepGroup::step( float timeStep, int subStepCount )
 		std::stable_sort( hEnlarge.begin(), hEnlarge.end(),
 						  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
 		std::vector<int> first, last;
-		splitByWorld( hEnlarge, (int)worlds.size(), first, last );
-		HostPool::get().run( (int)worlds.size(), [&]( int i ) {
+		splitByWorld( hEnlarge, nw, first, last );
+		HostPool::get().run( nw, [&]( int i ) {
 			if ( last[i] > first[i] )
 				processEnlarged( worlds[i], hEnlarge.data() + first[i], last[i] - first[i] );
 		} );
 	}
 	msEnlarge = msSince( te );
+	hostPhases[9] = msSince( t0 );
 	HIP_CHECK( hipEventElapsedTime( &msCollide, e0, e1 ) );
 	HIP_CHECK( hipEventElapsedTime( &msSolve, e3, e2 ) );
 	float msTotal = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t0 ).count();
 	msHost = msTotal - msCollide - msSolve;
-	for ( World* w : worlds )
-	{
-		w->mirrorValid = false;
-		w->contactsDownloaded = false;
-		w->locked = false;
-		w->profile.step = msTotal;
-		w->profile.pairs = msPairs;
-		w->profile.collide = msCollide;
-		w->profile.solve = msSolve;
-		w->profile.broadphase = msBroad;
-		w->profile.buildIslands = msChanges;
-		w->profile.finalizeBodies = msEnlarge;
-	}
+	profile.step = msTotal;
+	profile.pairs = msPairs;
+	profile.collide = msCollide;
+	profile.solve = msSolve;
+	profile.broadphase = msBroad;
+	profile.buildIslands = msChanges;
+	profile.finalizeBodies = msEnlarge;
+	// device state moved on: host mirrors and downloaded contacts of every world are stale
+	epoch += 1;
+	stepping = false;
 }
-
 
 } // namespace b2mi
 
 using namespace b2mi;
-
 extern "C" {
 
 B2_API b2ShapeId b2CreateCircleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Circle* circle )
@@ -1576,9 +1649,11 @@ B2_API int b2Shape_GetContactData( b2ShapeId shapeId, b2ContactData* contactData
 B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount )
 {
 	World* w = getWorld( worldId );
-	if ( w->locked )
+	if ( worldLocked( w ) )
 		return;
 	w->profile = b2Profile{};
+	if ( w->group )
+		w->group->profile = b2Profile{};
 	if ( timeStep == 0.0f )
 		return;
 	if ( w->group == nullptr )

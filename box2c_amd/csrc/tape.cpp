@@ -156,7 +156,9 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 		v.bodyListCount = g->bodyListCount;
 		v.block = std::min( blockSize, kAdjointMaxBlock );
 		std::vector<int> itemsOfWorld( g->worlds.size(), 0 );
-		std::vector<int> slotWorld( std::max( g->slotCount, 1 ), 0 );
+		// world of every slot of the launch arrays; -1 for the unused slots between the worlds'
+		// regions, whose device contents are undefined
+		std::vector<int> slotWorld( std::max( g->slotCount, 1 ), -1 );
 		std::vector<int> offsets( std::max( g->itemCount, 1 ), 0 );
 		size_t scratch = 0;
 		for ( int i = 0; i < g->itemCount; ++i )

@@ -200,7 +200,7 @@ static void destroyWorld( World* w )
 
 static void downloadMirror( World* w )
 {
-	if ( w->mirrorValid )
+	if ( w->mirrorValid && w->mirrorEpoch == stepEpoch( w ) )
 		return;
 	hipStream_t s = stream();
 	int na = (int)w->awakeSimBody.size();
@@ -223,11 +223,12 @@ static void downloadMirror( World* w )
 		sh.fatAABB = b2AABB{ { d.fatAABB[0], d.fatAABB[1] }, { d.fatAABB[2], d.fatAABB[3] } };
 	}
 	w->mirrorValid = true;
+	w->mirrorEpoch = stepEpoch( w );
 }
 
 static void downloadContacts( World* w )
 {
-	if ( w->contactsDownloaded )
+	if ( w->contactsDownloaded && w->contactsEpoch == stepEpoch( w ) )
 		return;
 	int n = w->contactIdPool.nextIndex;
 	w->devContacts.resize( n );
@@ -237,6 +238,7 @@ static void downloadContacts( World* w )
 		HIP_CHECK( hipStreamSynchronize( stream() ) );
 	}
 	w->contactsDownloaded = true;
+	w->contactsEpoch = stepEpoch( w );
 }
 
 // writes the host mirror (bodies, shapes, per-body shape lists) to the device
@@ -310,7 +312,7 @@ static void uploadBodies( World* w )
 // the host mirror is about to be modified by an API call
 static void beginEdit( World* w )
 {
-	B2MI_CHECK( w->locked == false );
+	B2MI_CHECK( worldLocked( w ) == false );
 	downloadMirror( w );
 	w->bodiesDirty = true;
 }
@@ -518,7 +520,8 @@ B2_API void b2World_EnableWarmStarting( b2WorldId worldId, bool flag )
 
 B2_API b2Profile b2World_GetProfile( b2WorldId worldId )
 {
-	return getWorld( worldId )->profile;
+	World* w = getWorld( worldId );
+	return w->group ? w->group->profile : w->profile;
 }
 
 B2_API b2Counters b2World_GetCounters( b2WorldId worldId )
@@ -734,7 +737,7 @@ B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def )
 {
 	B2MI_CHECK( def->internalValue == kCookie );
 	World* w = getWorld( worldId );
-	if ( w->locked )
+	if ( worldLocked( w ) )
 		return b2BodyId{ 0, 0, 0 };
 	if ( def->isEnabled == false )
 		fatal( "disabled bodies are not supported", __FILE__, __LINE__ );

@@ -182,7 +182,6 @@ struct World
 	int slot = -1; // index in the world table (worldId.index1 - 1)
 	uint16_t revision = 0;
 	bool inUse = false;
-	bool locked = false;
 	bool batchWorld = false;
 
 	std::vector<Body> bodies;
@@ -196,7 +195,8 @@ struct World
 	std::vector<float> awakeState; // 8 floats per awake body
 	std::vector<DevShape> devShapes;
 	std::vector<DevContact> devContacts;
-	bool mirrorValid = true;
+	bool mirrorValid = true; // and mirrorEpoch is the group's step epoch
+	uint64_t mirrorEpoch = 0, contactsEpoch = 0;
 	bool bodiesDirty = true; // host mirror of bodies/shapes must be uploaded
 	bool contactsDownloaded = false;
 
@@ -266,7 +266,8 @@ struct StepGroup
 	int targetSlots = 0;	  // work-item packing target (constraint slots), 0 = automatic
 	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kDefaultItemLdsBytes, 0 = HBM only)
 	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
-	int builtTarget = -1, builtBudget = -1;				   // packing target and LDS budget of the current items
+	int ldsFullBudget = -1;	  // max LDS bytes of an item keeping all its state in LDS (-1 = ldsBudget)
+	int builtTarget = -1, builtBudget = -1, builtFullBudget = -1;				   // packing target and LDS budget of the current items
 	bool structureDirty = true;
 
 	// launch buffers (device)
@@ -319,12 +320,32 @@ struct StepGroup
 	float msCollide = 0, msSolve = 0, msHost = 0;
 	hipEvent_t events[4] = { nullptr, nullptr, nullptr, nullptr };
 	std::vector<LaunchClass> classes;
+	// host clock (ms since the step began) at: pair update, collide-list build, collide launch,
+	// narrowphase wait, wait returned, change processing done, solver launch, solver wait, wait
+	// returned, enlarge processing done
+	float hostPhases[10] = {};
+	bool stepping = false; // inside step(): the API refuses edits of its worlds
+	uint64_t epoch = 0;	   // steps taken; host mirrors of an older epoch are stale
+	b2Profile profile{};   // of the last step, shared by the group's worlds
+	void launchSolve( hipStream_t s, const int* abortFlag );
 	hipEvent_t forkEvent = nullptr;
 	hipEvent_t joinEvents[kAuxStreams] = {};
 
 	~StepGroup();
 	void step( float timeStep, int subStepCount );
 };
+
+// a world whose group is stepping accepts no edits
+inline bool worldLocked( const World* w )
+{
+	return w->group != nullptr && w->group->stepping;
+}
+
+// step epoch of a world's group (0 before it has one)
+inline uint64_t stepEpoch( const World* w )
+{
+	return w->group != nullptr ? w->group->epoch : 0;
+}
 
 // tape.cpp
 void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSize );

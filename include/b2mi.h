@@ -78,6 +78,12 @@ B2_API int b2mi_Batch_GetWorkItemCount( int batch );
 B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes );
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
 
+// Largest LDS footprint (bytes) of a work item that keeps all its solver state in LDS (-1 = the
+// LDS budget). Items above it but within the budget keep their bodies and impulses in LDS and
+// read the constant constraint fields from HBM, a smaller footprint that lets more items share
+// a CU. Results are bit-identical either way.
+B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes );
+
 // Autotuner: steps the batch (or world) under each candidate launch shape (threads per
 // workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
 // rebuild step, and keeps the shape with the fastest solver kernel. Every shape gives
@@ -86,6 +92,16 @@ B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
 B2_API float b2mi_Batch_Autotune( int batch, float timeStep, int subStepCount, int trialSteps );
 B2_API float b2mi_World_Autotune( b2WorldId worldId, float timeStep, int subStepCount, int trialSteps );
 B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget );
+
+// Host clock of the last step, in ms since it began, at ten points: pair update started,
+// collide lists started, narrowphase launched, narrowphase wait started, wait returned, solver
+// setup started, solver launched, solver wait started, wait returned, enlarge processing done.
+B2_API void b2mi_Batch_GetHostTimeline( int batch, float* marks );
+
+// The solver launches of the last step, 4 ints each: first work item, work-item count, threads
+// per workgroup, dynamic LDS bytes per workgroup. Returns the number of launches and writes at
+// most `capacity` records.
+B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity );
 
 // Device time of the last step's narrowphase and solver launches and the remaining host
 // time, in milliseconds.

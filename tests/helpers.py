@@ -42,7 +42,19 @@ def reference() -> B2Api | None:
                 os.path.join(ORACLE_DIR, "_ref", "libref_scenes.so"))
 
 
+_emu_api = None
+
+
 def product() -> B2Api:
+    """The library under test: the HIP build, or with B2MI_TEST_LIBRARY=emu its CPU emulation
+    (tests/emu/build_emu.py), which runs the same host driver and kernels on the host."""
+    global _emu_api
+    if os.environ.get("B2MI_TEST_LIBRARY") == "emu":
+        if _emu_api is None:
+            emu = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu", "_build")
+            _emu_api = B2Api(os.path.join(emu, "libb2mi.so"), os.path.join(emu, "libb2mi_scenes.so"))
+            _emu_api.bind_product()
+        return _emu_api
     import box2c_amd
     return box2c_amd.load()
 

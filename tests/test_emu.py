@@ -1,0 +1,63 @@
+"""The step pipeline, kernels included, on the CPU emulation of the HIP runtime
+(tests/emu/): the same host driver and kernel sources compiled as host C++ with
+AddressSanitizer, so that an out-of-bounds access of any kernel or host path is reported on a
+machine without a GPU. A subset of the GPU tests runs against it in a child process (the
+sanitizer runtime must be loaded before the interpreter's libraries) and must match the oracle
+bit for bit, as on the GPU."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+EMU = os.path.join(HERE, "emu")
+LIB = os.path.join(EMU, "_build", "libb2mi.so")
+
+# small cases that cover the taped run and its adjoint, the batch, the storage variants, the
+# per-item launch classes and the steps whose solver launch goes right behind the narrowphase
+CASES = [
+    "tests/test_gpu_adjoint.py::test_taped_run_matches_oracle",
+    "tests/test_gpu_adjoint.py::test_replay_reproduces_run",
+    "tests/test_gpu_adjoint.py::test_autograd_function",
+    "tests/test_gpu_parity.py::test_step_parity_with_oracle[hello]",
+    "tests/test_gpu_parity.py::test_step_parity_with_oracle[shape_zoo]",
+    "tests/test_gpu_parity.py::test_step_parity_with_oracle[batch_pyramid]",
+    "tests/test_gpu_parity.py::test_small_batch_parity",
+    "tests/test_gpu_parity.py::test_gpu_run_to_run_bit_reproducible",
+    "tests/test_gpu_parity.py::test_step_parity_solver_storage[lds8k-tilted_stacks]",
+]
+
+
+def _sources_newer_than(path):
+    if not os.path.exists(path):
+        return True
+    t = os.path.getmtime(path)
+    dirs = [os.path.join(ROOT, "box2c_amd", "csrc"), EMU, os.path.join(EMU, "hip")]
+    for d in dirs:
+        for f in os.listdir(d):
+            if f.endswith((".cpp", ".hip", ".hpp", ".h", ".c")) and os.path.getmtime(os.path.join(d, f)) > t:
+                return True
+    return False
+
+
+def _asan_runtime():
+    r = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True)
+    path = r.stdout.strip()
+    return path if os.path.isabs(path) and os.path.exists(path) else None
+
+
+def test_pipeline_on_cpu_emulation():
+    asan = _asan_runtime()
+    if asan is None:
+        pytest.skip("no AddressSanitizer runtime")
+    if _sources_newer_than(LIB):
+        subprocess.run([sys.executable, os.path.join(EMU, "build_emu.py")], check=True, capture_output=True)
+    env = dict(os.environ)
+    env["B2MI_TEST_LIBRARY"] = "emu"
+    env["ASAN_OPTIONS"] = "detect_leaks=0:verify_asan_link_order=0"
+    env["LD_PRELOAD"] = asan + (":" + env["LD_PRELOAD"] if env.get("LD_PRELOAD") else "")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", *CASES], cwd=ROOT, env=env,
+                       capture_output=True, text=True, timeout=1200)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]

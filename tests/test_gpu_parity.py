@@ -174,6 +174,22 @@ def test_batch_autotune_keeps_parity():
             cpu.destroy(wc)
 
 
+def test_small_batch_parity():
+    """Three uneven worlds in one batch (regions with slack between them in the launch arrays)."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 3, 17)
+    try:
+        for s in range(20):
+            gpu.batch_step(batch)
+            for wc in worlds_c:
+                cpu.step(wc)
+        _compare_worlds(gpu, cpu, worlds_g, worlds_c, "small batch")
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
+
+
 @pytest.mark.parametrize("target", [128, 1024])
 def test_batch_per_item_block_sizes_parity(target):
     """Workgroup size per work item (blockSize 0): items of each size go to their own launch,
