@@ -106,6 +106,50 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, au
             "ms_host": t_host / steps, "work_items": items, "launch_shape": shape}
 
 
+def run_tumbler(api, shapes: int, steps: int, warmup: int, seed: int = 5):
+    """A kinematic square container spinning at 0.25 rad/s, filled with `shapes` half boxes and
+    half circles of size 0.125 on a jittered 0.4 m grid (the container grows with the count)."""
+    d = api.default_world_def(enableSleep=False)
+    w, ids = api.scene("tumbler", d, count=shapes, seed=seed)
+    api.step(w, n=warmup)
+    t_solve = t_collide = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        api.lib.b2World_Step(w, 1.0 / 60.0, 4)
+        c, s, _h = api.world_timing(w)
+        t_collide += c
+        t_solve += s
+    elapsed = time.perf_counter() - t0
+    counters = api.counters(w)
+    api.destroy(w)
+    return {"bodies": len(ids), "contacts": sum(counters.colorCounts), "colors": list(counters.colorCounts),
+            "elapsed": elapsed, "ms_solve": t_solve / steps, "ms_collide": t_collide / steps}
+
+
+def cpu_baseline_tumbler(shapes: int, threads: int, sample_steps: int, warmup: int, seed: int = 5):
+    """The reference on the same tumbler with `threads` workers, over a bounded number of steps."""
+    from box2c_amd.capi import B2Api
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
+    if not (os.path.exists(lib) and os.path.exists(scenes)):
+        return None
+    ref = B2Api(lib, scenes)
+    ref.lib.ref_pool_start(threads)
+    d = ref.default_world_def(enableSleep=False)
+    ref.lib.ref_pool_apply(__import__("ctypes").byref(d))
+    w, ids = ref.scene("tumbler", d, count=shapes, seed=seed)
+    ref.step(w, n=warmup)
+    t0 = time.perf_counter()
+    ref.step(w, n=sample_steps)
+    elapsed = time.perf_counter() - t0
+    ref.destroy(w)
+    ref.lib.ref_pool_stop()
+    return {"value": len(ids) * 4 * sample_steps / elapsed, "unit": "bodies×substeps/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"tumbler of {len(ids)} shapes, {sample_steps} steps after {warmup} warmup, {threads} worker threads, "
+                      f"{elapsed:.1f} s"}
+
+
 def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed: int = 11):
     """Forward (taped) rollout of `worlds` drop-box worlds and the adjoint of the loss
     sum |p_final - target|^2 with respect to friction, restitution and initial velocities."""
@@ -223,9 +267,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-worlds", type=int, default=4096, help="worlds of the batched leg (0 = skip it)")
-    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-cpu-steps", type=int, default=1000)
     ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the adjoint leg (0 = skip it)")
+    ap.add_argument("--tumbler-shapes", type=int, default=40000, help="shapes of the tumbler leg (0 = skip it)")
+    ap.add_argument("--tumbler-steps", type=int, default=600)
+    ap.add_argument("--tumbler-cpu-steps", type=int, default=60)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain copy kernel of known size (for HBM counter scaling)")
     args = ap.parse_args()
@@ -306,6 +353,22 @@ def main():
         out["batched"] = batched
     if args.adjoint_worlds > 0:
         out["adjoint"] = run_adjoint(api, args.adjoint_worlds, 16, 120, 10)
+    if args.tumbler_shapes > 0:
+        rt = run_tumbler(api, args.tumbler_shapes, args.tumbler_steps, args.warmup)
+        tval = rt["bodies"] * substeps * args.tumbler_steps / rt["elapsed"]
+        tumbler = {
+            "workload": f"tumbler_{args.tumbler_shapes}_kinematic_container_plain_f32", "value": tval,
+            "unit": "bodies×substeps/s", "bodies": rt["bodies"], "touching_contacts": rt["contacts"],
+            "graph_colors": rt["colors"], "steps": args.tumbler_steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * rt["elapsed"] / args.tumbler_steps, "ms_solve_kernel": rt["ms_solve"],
+            "ms_collide_kernel": rt["ms_collide"], "cpu_baseline": None,
+        }
+        if not args.no_cpu:
+            ct = cpu_baseline_tumbler(args.tumbler_shapes, args.cpu_threads, args.tumbler_cpu_steps, args.warmup)
+            tumbler["cpu_baseline"] = ct
+            if ct:
+                tumbler["vs_cpu"] = tval / ct["value"]
+        out["tumbler"] = tumbler
     print(json.dumps(out), flush=True)
 
 

@@ -596,8 +596,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_replay_step( const DevWo
 	const DevWorld& W = worlds[wi.world];
 	const int* bl = bodyList + wi.bodyStart;
 	GlobalStore C;
-	C.f = cfBase;
-	C.cap = cfCapacity;
+	C.rec = reinterpret_cast<float4*>( cfBase );
 	C.base = wi.slotStart;
 	C.sb = slotBodies + wi.slotStart;
 	C.P = priv + 2 * (size_t)wi.bodyStart;
@@ -633,8 +632,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 	const int base = simBase[wi.world];
 	const int S = W.subSteps;
 	GlobalStore C;
-	C.f = cfBase;
-	C.cap = cfCapacity;
+	C.rec = reinterpret_cast<float4*>( cfBase );
 	C.base = wi.slotStart;
 	C.sb = slotBodies + wi.slotStart;
 	C.P = priv + 2 * (size_t)wi.bodyStart;

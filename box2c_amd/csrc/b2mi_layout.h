@@ -207,37 +207,44 @@ struct ContactInit
 	int pad[2];
 };
 
-// number of float/int fields of one solver constraint (structure of arrays)
+// Solver constraint of one slot: a record of seven float4 (112 bytes), so that a lane reads a
+// quad of related fields with one 128-bit load and every field sits at a fixed offset:
+//   q0  body A, body B (local indices as int bits), normal x, normal y
+//   q1  point 0 anchors: A x, A y, B x, B y            q2  the same for point 1
+//   q3  point 0 base separation, normal mass, tangent mass; friction
+//   q4  point 1 base separation, normal mass, tangent mass; restitution
+//   q5  point 0 normal impulse, tangent impulse, max normal impulse, relative velocity
+//   q6  the same for point 1
+// q0-q4 are read-only during the substep passes; q5 and q6 are written by them.
 enum ConstraintField : int
 {
-	// read-only during the step, read by every pass (HBM in the bodies-in-LDS store)
-	cfNormalX = 0,
+	cfBodyA = 0,
+	cfBodyB,
+	cfNormalX,
 	cfNormalY,
-	cfFriction,
-	cfAnchorAX, // per point: 7 fields, point 1 follows point 0
+	cfAnchorAX = 4, // per point (point 1 at +4)
 	cfAnchorAY,
 	cfAnchorBX,
 	cfAnchorBY,
-	cfBaseSeparation,
+	cfBaseSeparation = 12, // per point
 	cfNormalMass,
 	cfTangentMass,
-	cfHotEnd = cfAnchorAX + 2 * 7,
-	// written by the passes (LDS in both LDS stores)
-	cfNormalImpulse = cfHotEnd, // per point: 3 fields
+	cfFriction = 15,
+	cfRestitution = 19,
+	cfNormalImpulse = 20, // per point
 	cfTangentImpulse,
 	cfMaxNormalImpulse,
-	cfMutableEnd = cfNormalImpulse + 2 * 3,
-	// read only by restitution and store (HBM in the bodies-in-LDS store)
-	cfRestitution = cfMutableEnd,
-	cfRelativeVelocity, // per point: 1 field
-	cfFloatCount = cfRelativeVelocity + 2,
-	cfMutableCount = cfMutableEnd - cfHotEnd,
+	cfRelativeVelocity,
+	cfFloatCount = 28,
+	cfQuads = 7,
+	cfMutableQuad = 5, // first written quad (q5, q6)
+	cfMutableCount = 8,
 };
 
 // field of manifold point p (0 or 1) for a per-point field given at point 0
 constexpr int pointField( int field, int p )
 {
-	return field + p * ( field < cfHotEnd ? 7 : field < cfMutableEnd ? 3 : 1 );
+	return field + 4 * p;
 }
 
 // Pose of a body at a step boundary (taped runs): center of mass, rotation, velocities
@@ -248,7 +255,7 @@ struct StepPose
 };
 
 // Bytes of LDS a work item's solver state needs: per body two float4 of state and a float2 of
-// inverse mass and inertia, per slot the two local body indices and the constraint fields.
+// inverse mass and inertia, per slot its constraint record (which holds its two local bodies).
 // Items above the budget keep the read-only constraint fields in HBM, or everything. A CDNA4 CU has
 // 160 KiB of LDS; the default budget leaves room for two or more items per CU.
 constexpr int kMaxItemLdsBytes = 160 * 1024;
@@ -259,9 +266,9 @@ constexpr int kSolveMaxBlock = 1024;
 constexpr int kAdjointMaxBlock = 256;
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {
-	return 40 * bodyCount + 8 * slotCount + 4 * cfFloatCount * slotCount;
+	return 40 * bodyCount + 4 * cfFloatCount * slotCount;
 }
-// the bodies-in-LDS store: body state and masses, slot bodies, the impulses
+// the bodies-in-LDS store: body state and masses, the written quads and the slot bodies
 inline int itemBodyLdsBytes( int bodyCount, int slotCount )
 {
 	return 40 * bodyCount + 8 * slotCount + 4 * cfMutableCount * slotCount;

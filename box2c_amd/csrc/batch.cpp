@@ -38,6 +38,7 @@ static void setShape( StepGroup* g, int blockSize, int targetSlots, int ldsBudge
 	g->ldsBudget = ldsBudget;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
+	g->allActive = true;
 }
 
 // Autotuner of the solver launch shape. Every shape gives bit-identical results, so the trials
@@ -133,6 +134,7 @@ B2_API b2WorldId b2mi_Batch_CreateWorld( int batch, const b2WorldDef* def )
 	w->groupIndex = (int)g->worlds.size();
 	g->worlds.push_back( w );
 	g->structureDirty = true;
+	g->allActive = true;
 	return b2WorldId{ (uint16_t)( w->slot + 1 ), w->revision };
 }
 
@@ -150,6 +152,7 @@ B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots )
 	g->targetSlots = targetSlots;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
+	g->allActive = true;
 }
 
 // maxBytes: largest LDS footprint of a work item kept in LDS (-1 = 160 KiB default, 0 = every
@@ -160,6 +163,7 @@ B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes )
 	g->ldsBudget = maxBytes;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
+	g->allActive = true;
 }
 
 // fullMaxBytes: largest LDS footprint of a work item that keeps all its solver state in LDS (-1 =
@@ -171,6 +175,7 @@ B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes )
 	g->ldsFullBudget = fullMaxBytes;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
+	g->allActive = true;
 }
 
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount )

@@ -515,10 +515,11 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		int n = k < ovfEnd ? m.pointCount : 2;
 		for ( int j = 0; j < n; ++j )
 		{
-			m.points[j].normalImpulse = C.at( PF( NormalImpulse, j ), k );
-			m.points[j].tangentImpulse = C.at( PF( TangentImpulse, j ), k );
-			m.points[j].maxNormalImpulse = C.at( PF( MaxNormalImpulse, j ), k );
-			m.points[j].normalVelocity = C.at( PF( RelativeVelocity, j ), k );
+			float4 imp = C.quad( k, cfMutableQuad + j );
+			m.points[j].normalImpulse = imp.x;
+			m.points[j].tangentImpulse = imp.y;
+			m.points[j].maxNormalImpulse = imp.z;
+			m.points[j].normalVelocity = imp.w;
 		}
 	}
 
@@ -554,30 +555,29 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	const int total = wi.groupStart[kGroups];
 	if ( wi.store == kStoreLds )
 	{
-		// LDS: body state, masses, slot bodies, constraint fields
+		// LDS (float4 arrays first, 16-byte aligned): body state, constraint records, masses
 		LdsStore C;
 		C.P = lds;
-		C.M = reinterpret_cast<float2*>( lds + 2 * wi.bodyCount );
-		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount );
-		C.f = reinterpret_cast<float*>( C.sb + total );
-		C.n = total;
+		C.rec = lds + 2 * wi.bodyCount;
+		C.M = reinterpret_cast<float2*>( C.rec + cfQuads * total );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
-			C.sb[k] = slotBodies[wi.slotStart + k];
+		{
+			int2 b = slotBodies[wi.slotStart + k];
+			C.rec[k * cfQuads] = make_float4( __builtin_bit_cast( float, b.x ), __builtin_bit_cast( float, b.y ), 0.0f, 0.0f );
+		}
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
 		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
 	}
 	else if ( wi.store == kStoreBodiesLds )
 	{
-		// LDS: body state, masses, slot bodies, impulses; HBM: the other constraint fields
+		// LDS: body state, written quads, masses, slot bodies; HBM: the read-only quads
 		BodiesLdsStore C;
-		C.f = cfBase;
-		C.cap = cfCapacity;
+		C.rec = reinterpret_cast<float4*>( cfBase );
 		C.base = wi.slotStart;
 		C.P = lds;
-		C.M = reinterpret_cast<float2*>( lds + 2 * wi.bodyCount );
+		C.mut = lds + 2 * wi.bodyCount;
+		C.M = reinterpret_cast<float2*>( C.mut + 2 * total );
 		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount );
-		C.mut = reinterpret_cast<float*>( C.sb + total );
-		C.n = total;
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
 		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
@@ -585,8 +585,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	else
 	{
 		GlobalStore C;
-		C.f = cfBase;
-		C.cap = cfCapacity;
+		C.rec = reinterpret_cast<float4*>( cfBase );
 		C.base = wi.slotStart;
 		C.sb = slotBodies + wi.slotStart;
 		C.P = priv + 2 * (size_t)wi.bodyStart;

@@ -58,26 +58,34 @@ __device__ inline void store_velocity( const S& C, int i, const Body& b )
 }
 
 // Solver state of one work item, addressed by local slot k (0 .. slots-1) and local body i:
-// the constraint fields (cf* enum), each slot's two local body indices, the private body state
-// (two float4 per body: v.x, v.y, w, flags | dp.x, dp.y, dq.c, dq.s) and each body's inverse
-// mass and inertia (the same floats the narrowphase copies into a contact).
+// each slot's constraint record (seven float4, see ConstraintField), each slot's two local body
+// indices, the private body state (two float4 per body: v.x, v.y, w, flags | dp.x, dp.y, dq.c,
+// dq.s) and each body's inverse mass and inertia (the same floats the narrowphase copies into a
+// contact). Every store offers the record as fields (at) and as quads (quad, setQuad).
 //
 // GlobalStore keeps it in HBM (any work item size). LdsStore keeps it in the workgroup's LDS,
 // so the substep passes (about 3 x colours x substeps dependent phases per step) run at LDS
 // latency. BodiesLdsStore, for items too large for that, keeps the bodies, the slot bodies and
-// the impulses in LDS and the read-only constraint fields in HBM: per phase a slot then only
-// reads HBM, coalesced across lanes. The host picks the store per work item from the LDS budget.
+// the written quads in LDS and reads the read-only quads from HBM, coalesced across lanes. The
+// host picks the store per work item from the LDS budget.
 struct GlobalStore
 {
-	float* f;
-	int cap;
+	float4* rec; // record of slot k at rec[(base + k) * cfQuads]
 	int base;
 	const int2* sb;
 	float4* P;
 	float2* M;
 	__device__ inline float& at( int field, int k ) const
 	{
-		return f[(size_t)field * cap + base + k];
+		return reinterpret_cast<float*>( rec + (size_t)( base + k ) * cfQuads )[field];
+	}
+	__device__ inline float4 quad( int k, int q ) const
+	{
+		return rec[(size_t)( base + k ) * cfQuads + q];
+	}
+	__device__ inline void setQuad( int k, int q, float4 v ) const
+	{
+		rec[(size_t)( base + k ) * cfQuads + q] = v;
 	}
 	__device__ inline int2 bodies( int k ) const
 	{
@@ -99,19 +107,28 @@ struct GlobalStore
 
 struct BodiesLdsStore
 {
-	float* f; // HBM: all fields (the mutable ones unused)
-	int cap;
+	float4* rec; // HBM: the read-only quads of slot k at rec[(base + k) * cfQuads]
 	int base;
-	float* mut; // LDS: the impulse fields
-	int n;
+	float4* mut; // LDS: the written quads of slot k at mut[2k], mut[2k + 1]
 	int2* sb;
 	float4* P;
 	float2* M;
 	__device__ inline float& at( int field, int k ) const
 	{
-		if ( field >= cfHotEnd && field < cfMutableEnd )
-			return mut[( field - cfHotEnd ) * n + k];
-		return f[(size_t)field * cap + base + k];
+		if ( field >= 4 * cfMutableQuad )
+			return reinterpret_cast<float*>( mut + 2 * k )[field - 4 * cfMutableQuad];
+		return reinterpret_cast<float*>( rec + (size_t)( base + k ) * cfQuads )[field];
+	}
+	__device__ inline float4 quad( int k, int q ) const
+	{
+		return q >= cfMutableQuad ? mut[2 * k + q - cfMutableQuad] : rec[(size_t)( base + k ) * cfQuads + q];
+	}
+	__device__ inline void setQuad( int k, int q, float4 v ) const
+	{
+		if ( q >= cfMutableQuad )
+			mut[2 * k + q - cfMutableQuad] = v;
+		else
+			rec[(size_t)( base + k ) * cfQuads + q] = v;
 	}
 	__device__ inline int2 bodies( int k ) const
 	{
@@ -131,20 +148,28 @@ struct BodiesLdsStore
 	}
 };
 
+// the slot bodies live in q0.x and q0.y of the record (int bits)
 struct LdsStore
 {
-	float* f;
-	int n;
-	int2* sb;
+	float4* rec; // record of slot k at rec[k * cfQuads]
 	float4* P;
 	float2* M;
 	__device__ inline float& at( int field, int k ) const
 	{
-		return f[field * n + k];
+		return reinterpret_cast<float*>( rec + k * cfQuads )[field];
+	}
+	__device__ inline float4 quad( int k, int q ) const
+	{
+		return rec[k * cfQuads + q];
+	}
+	__device__ inline void setQuad( int k, int q, float4 v ) const
+	{
+		rec[k * cfQuads + q] = v;
 	}
 	__device__ inline int2 bodies( int k ) const
 	{
-		return sb[k];
+		float4 q0 = rec[k * cfQuads];
+		return make_int2( __builtin_bit_cast( int, q0.x ), __builtin_bit_cast( int, q0.y ) );
 	}
 	__device__ inline float4& b0( int i ) const
 	{
@@ -200,54 +225,49 @@ __device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict
 		wB = s.z;
 	}
 	float nx = m.normalX, ny = m.normalY;
-	C.at( cfNormalX, slot ) = nx;
-	C.at( cfNormalY, slot ) = ny;
-	C.at( cfFriction, slot ) = contact->friction;
-	C.at( cfRestitution, slot ) = contact->restitution;
+	C.setQuad( slot, 0, make_float4( __builtin_bit_cast( float, la ), __builtin_bit_cast( float, lb ), nx, ny ) );
 	float tx = ny, ty = -nx;
 	float warm = W.warmStartScale;
+	float4 anc[2], pm[2], imp[2];
+#pragma unroll
 	for ( int j = 0; j < 2; ++j )
 	{
 		if ( j < m.pointCount )
 		{
 			const DevManifoldPoint& mp = m.points[j];
 			float rAx = mp.anchorAX, rAy = mp.anchorAY, rBx = mp.anchorBX, rBy = mp.anchorBY;
-			C.at( PF( AnchorAX, j ), slot ) = rAx;
-			C.at( PF( AnchorAY, j ), slot ) = rAy;
-			C.at( PF( AnchorBX, j ), slot ) = rBx;
-			C.at( PF( AnchorBY, j ), slot ) = rBy;
+			anc[j] = make_float4( rAx, rAy, rBx, rBy );
 			float dx = rBx - rAx, dy = rBy - rAy;
-			C.at( PF( BaseSeparation, j ), slot ) = mp.separation - ( dx * nx + dy * ny );
-			C.at( PF( NormalImpulse, j ), slot ) = warm * mp.normalImpulse;
-			C.at( PF( TangentImpulse, j ), slot ) = warm * mp.tangentImpulse;
-			C.at( PF( MaxNormalImpulse, j ), slot ) = 0.0f;
+			float baseSeparation = mp.separation - ( dx * nx + dy * ny );
 			float rnA = rAx * ny - rAy * nx;
 			float rnB = rBx * ny - rBy * nx;
 			float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
-			C.at( PF( NormalMass, j ), slot ) = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+			float normalMass = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
 			float rtA = rAx * ty - rAy * tx;
 			float rtB = rBx * ty - rBy * tx;
 			float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
-			C.at( PF( TangentMass, j ), slot ) = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+			float tangentMass = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+			pm[j] = make_float4( baseSeparation, normalMass, tangentMass, 0.0f );
 			float vrAx = vAx + ( -wA ) * rAy, vrAy = vAy + wA * rAx;
 			float vrBx = vBx + ( -wB ) * rBy, vrBy = vBy + wB * rBx;
-			C.at( PF( RelativeVelocity, j ), slot ) = nx * ( vrBx - vrAx ) + ny * ( vrBy - vrAy );
+			float relativeVelocity = nx * ( vrBx - vrAx ) + ny * ( vrBy - vrAy );
+			imp[j] = make_float4( warm * mp.normalImpulse, warm * mp.tangentImpulse, 0.0f, relativeVelocity );
 		}
 		else
 		{
-			C.at( PF( AnchorAX, j ), slot ) = 0.0f;
-			C.at( PF( AnchorAY, j ), slot ) = 0.0f;
-			C.at( PF( AnchorBX, j ), slot ) = 0.0f;
-			C.at( PF( AnchorBY, j ), slot ) = 0.0f;
-			C.at( PF( BaseSeparation, j ), slot ) = 0.0f;
-			C.at( PF( NormalMass, j ), slot ) = 0.0f;
-			C.at( PF( TangentMass, j ), slot ) = 0.0f;
-			C.at( PF( NormalImpulse, j ), slot ) = 0.0f;
-			C.at( PF( TangentImpulse, j ), slot ) = 0.0f;
-			C.at( PF( MaxNormalImpulse, j ), slot ) = 0.0f;
-			C.at( PF( RelativeVelocity, j ), slot ) = 0.0f;
+			anc[j] = make_float4( 0.0f, 0.0f, 0.0f, 0.0f );
+			pm[j] = make_float4( 0.0f, 0.0f, 0.0f, 0.0f );
+			imp[j] = make_float4( 0.0f, 0.0f, 0.0f, 0.0f );
 		}
 	}
+	pm[0].w = contact->friction;
+	pm[1].w = contact->restitution;
+	C.setQuad( slot, 1, anc[0] );
+	C.setQuad( slot, 2, anc[1] );
+	C.setQuad( slot, 3, pm[0] );
+	C.setQuad( slot, 4, pm[1] );
+	C.setQuad( slot, 5, imp[0] );
+	C.setQuad( slot, 6, imp[1] );
 }
 
 __device__ inline void lane_apply( Body& bA, Body& bB, float mA, float iA, float mB, float iB, float rAx, float rAy, float rBx,
@@ -268,13 +288,17 @@ __device__ void lane_warm_start( const S& C, int slot, int la, int lb )
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
-	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	const float4 q0 = C.quad( slot, 0 );
+	const float4 anc[2] = { C.quad( slot, 1 ), C.quad( slot, 2 ) };
+	const float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
+	float nx = q0.z, ny = q0.w;
 	float tx = ny, ty = 0.0f - nx;
+#pragma unroll
 	for ( int j = 0; j < 2; ++j )
 	{
-		float ni = C.at( PF( NormalImpulse, j ), slot ), ti = C.at( PF( TangentImpulse, j ), slot );
-		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
-		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float ni = imp[j].x, ti = imp[j].y;
+		float rAx = anc[j].x, rAy = anc[j].y;
+		float rBx = anc[j].z, rBy = anc[j].w;
 		float Px = ni * nx + ti * tx;
 		float Py = ni * ny + ti * ty;
 		bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
@@ -295,54 +319,62 @@ __device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, f
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
-	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
+	const float4 q0 = C.quad( slot, 0 );
+	const float4 anc[2] = { C.quad( slot, 1 ), C.quad( slot, 2 ) };
+	const float4 pm[2] = { C.quad( slot, 3 ), C.quad( slot, 4 ) };
+	float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
+	float nx = q0.z, ny = q0.w;
 	const float* soft = contact_softness( W, la, lb );
 	float biasRate = useBias ? soft[0] : 0.0f;
 	float massScale = useBias ? soft[1] : 1.0f;
 	float impulseScale = useBias ? soft[2] : 0.0f;
 	float dpx = bB.dpx - bA.dpx, dpy = bB.dpy - bA.dpy;
 	float nImp[2];
+#pragma unroll
 	for ( int j = 0; j < 2; ++j )
 	{
-		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
-		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float rAx = anc[j].x, rAy = anc[j].y;
+		float rBx = anc[j].z, rBy = anc[j].w;
 		float rsAx = bA.dqc * rAx - bA.dqs * rAy, rsAy = bA.dqs * rAx + bA.dqc * rAy;
 		float rsBx = bB.dqc * rBx - bB.dqs * rBy, rsBy = bB.dqs * rBx + bB.dqc * rBy;
 		float dsx = dpx + ( rsBx - rsAx ), dsy = dpy + ( rsBy - rsAy );
-		float s = ( nx * dsx + ny * dsy ) + C.at( PF( BaseSeparation, j ), slot );
+		float s = ( nx * dsx + ny * dsy ) + pm[j].x;
 		float specBias = s * inv_h;
 		float softBias = maxr( biasRate * s, minBiasVel );
 		float bias = s > 0.0f ? specBias : softBias;
 		float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
 		float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
 		float vn = dvx * nx + dvy * ny;
-		float ni = C.at( PF( NormalImpulse, j ), slot );
-		float negImpulse = C.at( PF( NormalMass, j ), slot ) * ( massScale * ( vn + bias ) ) + impulseScale * ni;
+		float ni = imp[j].x;
+		float negImpulse = pm[j].y * ( massScale * ( vn + bias ) ) + impulseScale * ni;
 		float newImpulse = maxr( ni - negImpulse, 0.0f );
 		float impulse = newImpulse - ni;
-		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
-		C.at( PF( MaxNormalImpulse, j ), slot ) = maxr( C.at( PF( MaxNormalImpulse, j ), slot ), newImpulse );
+		imp[j].x = newImpulse;
+		imp[j].z = maxr( imp[j].z, newImpulse );
 		nImp[j] = newImpulse;
 		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * nx, impulse * ny );
 	}
 	float tx = ny, ty = 0.0f - nx;
-	float friction = C.at( cfFriction, slot );
+	float friction = pm[0].w;
+#pragma unroll
 	for ( int j = 0; j < 2; ++j )
 	{
-		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
-		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float rAx = anc[j].x, rAy = anc[j].y;
+		float rBx = anc[j].z, rBy = anc[j].w;
 		float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
 		float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
 		float vt = dvx * tx + dvy * ty;
-		float negImpulse = C.at( PF( TangentMass, j ), slot ) * vt;
+		float negImpulse = pm[j].z * vt;
 		float maxFriction = friction * nImp[j];
-		float ti = C.at( PF( TangentImpulse, j ), slot );
+		float ti = imp[j].y;
 		float newImpulse = ti - negImpulse;
 		newImpulse = maxr( 0.0f - maxFriction, minr( newImpulse, maxFriction ) );
 		float impulse = newImpulse - ti;
-		C.at( PF( TangentImpulse, j ), slot ) = newImpulse;
+		imp[j].y = newImpulse;
 		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * tx, impulse * ty );
 	}
+	C.setQuad( slot, 5, imp[0] );
+	C.setQuad( slot, 6, imp[1] );
 	store_velocity( C, la, bA );
 	store_velocity( C, lb, bB );
 }
@@ -354,25 +386,32 @@ __device__ void lane_restitution( const S& C, int slot, int la, int lb, float th
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
-	float nx = C.at( cfNormalX, slot ), ny = C.at( cfNormalY, slot );
-	float restitution = C.at( cfRestitution, slot );
+	const float4 q0 = C.quad( slot, 0 );
+	const float4 anc[2] = { C.quad( slot, 1 ), C.quad( slot, 2 ) };
+	const float4 pm[2] = { C.quad( slot, 3 ), C.quad( slot, 4 ) };
+	float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
+	float nx = q0.z, ny = q0.w;
+	float restitution = pm[1].w;
+#pragma unroll
 	for ( int j = 0; j < 2; ++j )
 	{
-		float relVel = C.at( PF( RelativeVelocity, j ), slot );
-		bool skip = ( relVel + threshold ) > 0.0f || C.at( PF( MaxNormalImpulse, j ), slot ) == 0.0f;
-		float mass = skip ? 0.0f : C.at( PF( NormalMass, j ), slot );
-		float rAx = C.at( PF( AnchorAX, j ), slot ), rAy = C.at( PF( AnchorAY, j ), slot );
-		float rBx = C.at( PF( AnchorBX, j ), slot ), rBy = C.at( PF( AnchorBY, j ), slot );
+		float relVel = imp[j].w;
+		bool skip = ( relVel + threshold ) > 0.0f || imp[j].z == 0.0f;
+		float mass = skip ? 0.0f : pm[j].y;
+		float rAx = anc[j].x, rAy = anc[j].y;
+		float rBx = anc[j].z, rBy = anc[j].w;
 		float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
 		float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
 		float vn = dvx * nx + dvy * ny;
-		float ni = C.at( PF( NormalImpulse, j ), slot );
+		float ni = imp[j].x;
 		float negImpulse = mass * ( vn + restitution * relVel );
 		float newImpulse = maxr( ni - negImpulse, 0.0f );
 		float impulse = newImpulse - ni;
-		C.at( PF( NormalImpulse, j ), slot ) = newImpulse;
+		imp[j].x = newImpulse;
 		lane_apply( bA, bB, mA, iA, mB, iB, rAx, rAy, rBx, rBy, impulse * nx, impulse * ny );
 	}
+	C.setQuad( slot, 5, imp[0] );
+	C.setQuad( slot, 6, imp[1] );
 	store_velocity( C, la, bA );
 	store_velocity( C, lb, bB );
 }

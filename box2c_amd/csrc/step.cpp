@@ -1260,6 +1260,19 @@ void StepGroup::step( float timeStep, int subStepCount )
 	builtTarget = target;
 	builtBudget = budget;
 	builtFullBudget = fullBudget;
+	// the worlds this step visits on the host: every world, or those listed since the last step
+	std::vector<World*> visit;
+	{
+		std::lock_guard<std::mutex> lock( activeMutex );
+		visit.swap( active );
+	}
+	for ( World* w : visit )
+		w->activeQueued.store( false, std::memory_order_relaxed );
+	const bool visitAll = allActive || structureDirty || reshape || timeStep != lastTimeStep || subStepCount != lastSubSteps;
+	allActive = false;
+	lastTimeStep = timeStep;
+	lastSubSteps = subStepCount;
+	const std::vector<World*>& pass = visitAll ? worlds : visit;
 
 	// 1. per world in parallel: pair update and new contacts (src/broad_phase.c:333-436), the
 	//    narrowphase list when contacts were created or destroyed, and the world's device
@@ -1271,7 +1284,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 	collideDirty.reset( nw );
 	topology.reset( nw );
 	std::atomic<bool> paramsChanged{ false }, regionFull{ false };
-	std::atomic<int> shapeTotal{ 0 };
+	std::atomic<int> shapeDelta{ 0 };
 	auto refreshParams = [&]( World* w ) {
 		DevWorld d;
 		fillDevWorld( d, w, timeStep, subStepCount );
@@ -1282,7 +1295,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 			paramsChanged.store( true );
 		}
 	};
-	forEachWorld( this, [&]( World* w ) {
+	HostPool::get().run( (int)pass.size(), [&]( int i ) {
+		World* w = pass[i];
 		updatePairs( w );
 		if ( w->contactsDirty )
 		{
@@ -1295,7 +1309,12 @@ void StepGroup::step( float timeStep, int subStepCount )
 			w->topologyDirty = true;
 		if ( w->topologyDirty )
 			topology.add( w );
-		shapeTotal.fetch_add( (int)w->shapes.size() );
+		int ns = (int)w->shapes.size();
+		if ( ns != w->countedShapes )
+		{
+			shapeDelta.fetch_add( ns - w->countedShapes );
+			w->countedShapes = ns;
+		}
 		if ( w->bodiesDirty || !w->pendingInits.empty() || w->contactIdPool.nextIndex > w->capContacts || w->dContacts == nullptr )
 			serial.add( w );
 		else
@@ -1345,7 +1364,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 		layoutGroup( this );
 	publishLists( this, relayout ? nullptr : &collideDirty.list, relayout );
 	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
-	grow( dEnlarge, capEnlarge, std::max( shapeTotal.load(), 1 ) );
+	shapeTotal += shapeDelta.load();
+	grow( dEnlarge, capEnlarge, std::max( shapeTotal, 1 ) );
 	// events bracket each kernel launch alone (the stream is idle while the host works)
 	if ( events[0] == nullptr )
 		for ( hipEvent_t& e : events )
@@ -1381,6 +1401,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 	auto tc = std::chrono::steady_clock::now();
 	hostPhases[4] = msSince( t0 );
 	const int changeCount = hCounters[0];
+	std::vector<World*> changedTopology;
 	if ( changeCount > 0 )
 	{
 		hChanges.resize( changeCount );
@@ -1395,21 +1416,27 @@ void StepGroup::step( float timeStep, int subStepCount )
 			if ( last[i] > first[i] )
 				applyChanges( worlds[i], hChanges.data() + first[i], last[i] - first[i] );
 		} );
+		// the changed worlds: their items may need rebuilding now, their lists next step
+		for ( int i = 0; i < nw; ++i )
+			if ( last[i] > first[i] )
+			{
+				markActive( worlds[i] );
+				if ( worlds[i]->topologyDirty )
+					changedTopology.push_back( worlds[i] );
+			}
 	}
 
 	// 3. solver (one workgroup per work item); rebuild the items of worlds whose graph changed
 	hostPhases[5] = msSince( t0 );
 	if ( !speculate || changeCount > 0 )
 	{
-		if ( changeCount > 0 )
+		if ( !changedTopology.empty() )
 		{
-			// the state changes may have marked more worlds
-			topology.reset( nw );
-			forEachWorld( this, [&]( World* w ) {
-				if ( w->topologyDirty )
-					topology.add( w );
-			} );
-			topology.finish();
+			// the state changes marked more worlds
+			std::vector<World*>& list = topology.list;
+			list.insert( list.end(), changedTopology.begin(), changedTopology.end() );
+			std::sort( list.begin(), list.end(), []( const World* a, const World* b ) { return a->groupIndex < b->groupIndex; } );
+			list.erase( std::unique( list.begin(), list.end() ), list.end() );
 		}
 		if ( !topology.list.empty() )
 		{
@@ -1453,6 +1480,9 @@ void StepGroup::step( float timeStep, int subStepCount )
 			if ( last[i] > first[i] )
 				processEnlarged( worlds[i], hEnlarge.data() + first[i], last[i] - first[i] );
 		} );
+		for ( int i = 0; i < nw; ++i )
+			if ( last[i] > first[i] )
+				markActive( worlds[i] );
 	}
 	msEnlarge = msSince( te );
 	hostPhases[9] = msSince( t0 );

@@ -92,6 +92,7 @@ void tapeFree( StepGroup* g )
 	g->targetSlots = t->savedTarget;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
+	g->allActive = true;
 	delete t;
 	g->tape = nullptr;
 }
@@ -129,6 +130,7 @@ static void tapeBegin( StepGroup* g, int interval )
 	g->targetSlots = 1 << 24;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
+	g->allActive = true;
 	g->tape = t;
 }
 

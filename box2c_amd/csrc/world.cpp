@@ -104,10 +104,14 @@ static void devFree( T*& ptr )
 static World* g_worlds[kMaxWorldSlots] = {};
 static uint16_t g_revisions[kMaxWorldSlots] = {};
 
+// every API call reaches its world through here: the world is then visited by the host pass of
+// its group's next step, whatever the call changed
 World* worldAt( int slot )
 {
 	B2MI_CHECK( 0 <= slot && slot < kMaxWorldSlots && g_worlds[slot] != nullptr );
-	return g_worlds[slot];
+	World* w = g_worlds[slot];
+	markActive( w );
+	return w;
 }
 
 World* getWorld( b2WorldId id )

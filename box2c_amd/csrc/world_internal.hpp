@@ -15,7 +15,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <unordered_set>
 #include <vector>
@@ -241,6 +243,8 @@ struct World
 
 	StepGroup* group = nullptr;
 	int groupIndex = -1;
+	std::atomic<bool> activeQueued{ false }; // listed in group->active for the next step
+	int countedShapes = 0;					  // shapes counted in group->shapeTotal
 
 	// pending new contacts (device init) of the current step
 	std::vector<ContactInit> pendingInits;
@@ -325,6 +329,15 @@ struct StepGroup
 	// returned, enlarge processing done
 	float hostPhases[10] = {};
 	bool stepping = false; // inside step(): the API refuses edits of its worlds
+	// Worlds the next step must visit on the host: those reached through the API since the last
+	// step (any call that names the world or one of its bodies or shapes) and those whose
+	// contacts or proxies the last step changed. Every world when allActive is set.
+	std::vector<World*> active;
+	std::mutex activeMutex;
+	bool allActive = true;
+	float lastTimeStep = -1.0f;
+	int lastSubSteps = -1;
+	int shapeTotal = 0;
 	uint64_t epoch = 0;	   // steps taken; host mirrors of an older epoch are stale
 	b2Profile profile{};   // of the last step, shared by the group's worlds
 	void launchSolve( hipStream_t s, const int* abortFlag );
@@ -334,6 +347,19 @@ struct StepGroup
 	~StepGroup();
 	void step( float timeStep, int subStepCount );
 };
+
+// lists the world for the host pass of its group's next step
+inline void markActive( World* w )
+{
+	if ( w->group == nullptr || w->activeQueued.load( std::memory_order_relaxed ) )
+		return;
+	std::lock_guard<std::mutex> lock( w->group->activeMutex );
+	if ( !w->activeQueued.load( std::memory_order_relaxed ) )
+	{
+		w->activeQueued.store( true, std::memory_order_relaxed );
+		w->group->active.push_back( w );
+	}
+}
 
 // a world whose group is stepping accepts no edits
 inline bool worldLocked( const World* w )

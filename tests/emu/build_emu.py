@@ -22,8 +22,8 @@ sys.path.insert(0, ROOT)
 from box2c_amd.build import SOURCES  # noqa: E402
 
 
-def build(asan: bool = True, opt: str = "-O1") -> str:
-    os.makedirs(OUT, exist_ok=True)
+def build(asan: bool = True, opt: str = "-O1", out: str = OUT) -> str:
+    os.makedirs(out, exist_ok=True)
     flags = ["-std=c++17", "-fPIC", opt, "-g", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
              "-DB2MI_BUILDING_LIBRARY", "-Wno-unknown-pragmas", "-Wno-ignored-attributes"]
     if asan:
@@ -32,19 +32,19 @@ def build(asan: bool = True, opt: str = "-O1") -> str:
     objs = []
     for src in SOURCES + ["emu_runtime.cpp"]:
         path = os.path.join(HERE, src) if src == "emu_runtime.cpp" else os.path.join(CSRC, src)
-        obj = os.path.join(OUT, src.replace(".", "_") + ".o")
+        obj = os.path.join(out, src.replace(".", "_") + ".o")
         cmd = ["g++", "-x", "c++", *flags, *inc, "-c", path, "-o", obj]
         print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    lib = os.path.join(OUT, "libb2mi.so")
+    lib = os.path.join(out, "libb2mi.so")
     link = ["g++", "-shared", "-o", lib, *objs, "-lpthread"]
     if asan:
         link.insert(1, "-fsanitize=address")
     subprocess.check_call(link)
-    scenes = os.path.join(OUT, "libb2mi_scenes.so")
+    scenes = os.path.join(out, "libb2mi_scenes.so")
     cmd = ["gcc", "-O1", "-g", "-std=gnu17", "-fPIC", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), "-shared",
-           "-o", scenes, os.path.join(CSRC, "scenes.c"), "-L", OUT, "-lb2mi", "-Wl,-rpath,$ORIGIN", "-lm"]
+           "-o", scenes, os.path.join(CSRC, "scenes.c"), "-L", out, "-lb2mi", "-Wl,-rpath,$ORIGIN", "-lm"]
     if asan:
         cmd.insert(1, "-fsanitize=address")
     subprocess.check_call(cmd)
@@ -52,4 +52,5 @@ def build(asan: bool = True, opt: str = "-O1") -> str:
 
 
 if __name__ == "__main__":
-    build(asan="--no-asan" not in sys.argv)
+    out = sys.argv[sys.argv.index("--out") + 1] if "--out" in sys.argv else OUT
+    build(asan="--no-asan" not in sys.argv, out=out)

@@ -51,7 +51,7 @@ def product() -> B2Api:
     global _emu_api
     if os.environ.get("B2MI_TEST_LIBRARY") == "emu":
         if _emu_api is None:
-            emu = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu", "_build")
+            emu = os.environ.get("B2MI_EMU_DIR") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu", "_build")
             _emu_api = B2Api(os.path.join(emu, "libb2mi.so"), os.path.join(emu, "libb2mi_scenes.so"))
             _emu_api.bind_product()
         return _emu_api
