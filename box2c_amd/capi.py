@@ -270,6 +270,7 @@ class B2Api:
             S.scene_shape_zoo.argtypes = [POINTER(WorldDef), c_int, c_uint64, POINTER(WorldId), c_void_p, c_int]
             S.scene_hello.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
             S.scene_tilted_stacks.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
+            S.scene_overflow.argtypes = [POINTER(WorldDef), c_int, c_float, POINTER(WorldId), c_void_p, c_int]
             S.scene_single.argtypes = [POINTER(WorldDef), c_int, c_float, c_float, c_float, c_float, c_float,
                                        c_float, c_float, c_float, c_float, c_float, c_int,
                                        POINTER(WorldId), c_void_p, c_int]
@@ -334,6 +335,10 @@ class B2Api:
         elif name == "hello":
             ids = self._bodies(1)
             n = S.scene_hello(ctypes.byref(d), ctypes.byref(w), ids, len(ids))
+        elif name == "overflow":
+            count = kw.get("count", 30)
+            ids = self._bodies(1 + count)
+            n = S.scene_overflow(ctypes.byref(d), count, kw.get("rakeVx", 0.0), ctypes.byref(w), ids, len(ids))
         elif name == "tilted_stacks":
             ids = self._bodies(100)
             n = S.scene_tilted_stacks(ctypes.byref(d), ctypes.byref(w), ids, len(ids))

@@ -160,6 +160,11 @@ struct WorkItem
 	int ldsBytes;				 // LDS bytes of the item's store (0 for kStoreHbm)
 	int store;					 // kStoreHbm, kStoreBodiesLds or kStoreLds
 	int block;					 // threads of its workgroup when the launch shape is per item
+	// The overflow group in levels: slots of one level share no body whose state they change,
+	// so a level runs in parallel and the levels in turn give the serial order's results.
+	// Level boundaries (relative to slotStart) at ovfLevels[ovfLevelBegin .. + ovfLevelCount].
+	int ovfLevelBegin;
+	int ovfLevelCount;
 };
 
 // Where a work item keeps its solver state during the step

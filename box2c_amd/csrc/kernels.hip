@@ -369,7 +369,8 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 // `sc` the contact id of each of its slots.
 template <class S>
 __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, const int* __restrict__ bl,
-							const int* __restrict__ sc, EnlargeRecord* enl, int* enlCount, int enlCapacity )
+							const int* __restrict__ sc, const int* __restrict__ levels, EnlargeRecord* enl, int* enlCount,
+							int enlCapacity )
 {
 	const int tid = threadIdx.x, nt = blockDim.x;
 	const float4* gstate = reinterpret_cast<const float4*>( W.state );
@@ -394,7 +395,8 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	__syncthreads();
 
 	const int ovfBegin = wi.groupStart[0], ovfEnd = wi.groupStart[1];
-	const bool hasOverflow = ovfEnd > ovfBegin;
+	// the overflow group runs level by level (slots of one level share no changed body)
+	const int levelCount = ovfEnd > ovfBegin ? wi.ovfLevelCount : 0;
 	const float inv_h = W.inv_h, h = W.h;
 	const float pushout = W.pushout, minBiasVel = -W.pushout;
 	const bool warm = ( W.flags & kWorldWarmStart ) != 0;
@@ -406,14 +408,13 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		__syncthreads();
 
 		// warm start: overflow (always, src/solver.c:639), then colours if enabled
-		if ( hasOverflow )
+		for ( int L = 0; L < levelCount; ++L )
 		{
-			if ( tid == 0 )
-				for ( int k = ovfBegin; k < ovfEnd; ++k )
-				{
-					int2 lb = C.bodies( k );
-					ovf_warm_start( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount );
-				}
+			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
+			{
+				int2 lb = C.bodies( k );
+				ovf_warm_start( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount );
+			}
 			__syncthreads();
 		}
 		if ( warm )
@@ -431,14 +432,13 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			}
 
 		// solve with bias
-		if ( hasOverflow )
+		for ( int L = 0; L < levelCount; ++L )
 		{
-			if ( tid == 0 )
-				for ( int k = ovfBegin; k < ovfEnd; ++k )
-				{
-					int2 lb = C.bodies( k );
-					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, W );
-				}
+			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
+			{
+				int2 lb = C.bodies( k );
+				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, W );
+			}
 			__syncthreads();
 		}
 		for ( int g = 1; g < kGroups; ++g )
@@ -459,14 +459,13 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		__syncthreads();
 
 		// relax
-		if ( hasOverflow )
+		for ( int L = 0; L < levelCount; ++L )
 		{
-			if ( tid == 0 )
-				for ( int k = ovfBegin; k < ovfEnd; ++k )
-				{
-					int2 lb = C.bodies( k );
-					ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, W );
-				}
+			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
+			{
+				int2 lb = C.bodies( k );
+				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, W );
+			}
 			__syncthreads();
 		}
 		for ( int g = 1; g < kGroups; ++g )
@@ -485,14 +484,13 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 
 	// restitution
 	const float threshold = W.restitutionThreshold;
-	if ( hasOverflow )
+	for ( int L = 0; L < levelCount; ++L )
 	{
-		if ( tid == 0 )
-			for ( int k = ovfBegin; k < ovfEnd; ++k )
-			{
-				int2 lb = C.bodies( k );
-				ovf_restitution( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, threshold );
-			}
+		for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
+		{
+			int2 lb = C.bodies( k );
+			ovf_restitution( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, threshold );
+		}
 		__syncthreads();
 	}
 	for ( int g = 1; g < kGroups; ++g )
@@ -540,7 +538,8 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 // state, kStoreHbm items nothing.
 __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
-												   const int2* __restrict__ slotBodies, float* cfBase, int cfCapacity,
+												   const int2* __restrict__ slotBodies, const int* __restrict__ ovfLevels, float* cfBase,
+												   int cfCapacity,
 												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
 												   int itemOffset, const int* abortFlag )
 {
@@ -566,7 +565,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 			C.rec[k * cfQuads] = make_float4( __builtin_bit_cast( float, b.x ), __builtin_bit_cast( float, b.y ), 0.0f, 0.0f );
 		}
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
-		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
+		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else if ( wi.store == kStoreBodiesLds )
 	{
@@ -580,7 +579,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
-		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
+		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else
 	{
@@ -590,7 +589,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = slotBodies + wi.slotStart;
 		C.P = priv + 2 * (size_t)wi.bodyStart;
 		C.M = mass + wi.bodyStart;
-		solve_item( C, W, wi, bl, sc, enl, enlCount, enlCapacity );
+		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
 }
 
@@ -620,7 +619,8 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 }
 
 hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize, int ldsBytes,
-						 const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, int cfCapacity,
+						 const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels, float* cf,
+						 int cfCapacity,
 						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, const int* abortFlag,
 						 hipStream_t stream )
 {
@@ -638,7 +638,7 @@ hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int item
 		allowedLds = kMaxItemLdsBytes;
 	}
 	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
-						slotBodies, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag );
+						slotBodies, ovfLevels, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag );
 	return hipGetLastError();
 }
 

@@ -516,6 +516,60 @@ SCENE_API int scene_tilted_stacks( const b2WorldDef* def, b2WorldId* outWorld, b
 	return columns * rows;
 }
 
+// Overflow constraints: a dynamic plank on the ground carrying `count` small boxes, and a
+// kinematic rake resting on the boxes. The plank and the rake each touch every box, and the
+// graph colouring holds at most one of a body's contacts per colour, so the contacts beyond the
+// twelfth go to the overflow set: those of the plank share a dynamic body, those of the rake a
+// kinematic one. `rakeVx` is the rake's linear velocity x (-0.0 makes that component negative
+// zero).
+SCENE_API int scene_overflow( const b2WorldDef* def, int count, float rakeVx, b2WorldId* outWorld, b2BodyId* outBodies,
+							  int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bd );
+		b2Polygon box = b2MakeBox( 100.0f, 1.0f );
+		b2ShapeDef sd = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &sd, &box );
+	}
+	float pitch = 0.7f, half = 0.3f;
+	float length = 0.5f * pitch * count + 1.0f;
+	b2ShapeDef sd = b2DefaultShapeDef();
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.type = b2_dynamicBody;
+		bd.position = ( b2Vec2 ){ 0.0f, 0.25f };
+		b2BodyId plankId = b2CreateBody( worldId, &bd );
+		b2Polygon plank = b2MakeBox( length, 0.25f );
+		b2CreatePolygonShape( plankId, &sd, &plank );
+		record( outBodies, capacity, 0, plankId );
+	}
+	b2Polygon box = b2MakeBox( half, half );
+	float x0 = -0.5f * pitch * ( count - 1 );
+	for ( int i = 0; i < count; ++i )
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.type = b2_dynamicBody;
+		bd.position = ( b2Vec2 ){ x0 + pitch * i, 0.5f + half };
+		b2BodyId bodyId = b2CreateBody( worldId, &bd );
+		b2CreatePolygonShape( bodyId, &sd, &box );
+		record( outBodies, capacity, 1 + i, bodyId );
+	}
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.type = b2_kinematicBody;
+		bd.position = ( b2Vec2 ){ 0.0f, 0.5f + 2.0f * half + 0.2f };
+		bd.linearVelocity = ( b2Vec2 ){ rakeVx, 0.0f };
+		b2BodyId rakeId = b2CreateBody( worldId, &bd );
+		b2Polygon rake = b2MakeBox( length, 0.2f );
+		b2CreatePolygonShape( rakeId, &sd, &rake );
+	}
+	return 1 + count;
+}
+
 // One dynamic body over (or on) a static ground box whose top face is the line y = 0
 // (half-extents 50 x 1, centred at (0, -1)); used for closed-form checks of gravity,
 // damping, friction and restitution.

@@ -424,14 +424,15 @@ struct MovePair
 	int a, b, next;
 };
 
-static void updatePairs( World* w )
+class HostPool;
+static void hostPoolRun( int n, const std::function<void( int )>& fn );
+
+// src/broad_phase.c:278-331 (b2FindPairsTask) for moves [begin, end): the new pairs of each
+// moved proxy, as a list through `next` that starts at heads[i] (most recent first, as the
+// reference prepends to its per-proxy list); reads the trees and sets only
+static void findPairs( const World* w, int begin, int end, std::vector<MovePair>& pairs, int* heads )
 {
-	int moveCount = (int)w->moveArray.size();
-	if ( moveCount == 0 )
-		return;
-	std::vector<MovePair> pairs;
-	std::vector<int> heads( moveCount, -1 );
-	for ( int i = 0; i < moveCount; ++i )
+	for ( int i = begin; i < end; ++i )
 	{
 		int queryProxyKey = w->moveArray[i];
 		int proxyType = queryProxyKey & 3;
@@ -479,9 +480,31 @@ static void updatePairs( World* w )
 		}
 		query( b2_dynamicBody );
 	}
+}
+
+// src/broad_phase.c:333-436: the queries of the moved proxies (in parallel chunks when
+// `parallel`), then contact creation in move order on this thread
+static void updatePairs( World* w, bool parallel )
+{
+	int moveCount = (int)w->moveArray.size();
+	if ( moveCount == 0 )
+		return;
+	constexpr int kChunk = 128;
+	int chunks = ( moveCount + kChunk - 1 ) / kChunk;
+	std::vector<std::vector<MovePair>> pairs( chunks );
+	std::vector<int> heads( moveCount, -1 );
+	auto run = [&]( int c ) { findPairs( w, c * kChunk, std::min( moveCount, ( c + 1 ) * kChunk ), pairs[c], heads.data() ); };
+	if ( parallel && chunks > 1 )
+		hostPoolRun( chunks, run );
+	else
+		for ( int c = 0; c < chunks; ++c )
+			run( c );
 	for ( int i = 0; i < moveCount; ++i )
-		for ( int p = heads[i]; p != -1; p = pairs[p].next )
-			createContact( w, &w->shapes[pairs[p].a], &w->shapes[pairs[p].b] );
+	{
+		const std::vector<MovePair>& list = pairs[i / kChunk];
+		for ( int p = heads[i]; p != -1; p = list[p].next )
+			createContact( w, &w->shapes[list[p].a], &w->shapes[list[p].b] );
+	}
 	for ( int key : w->moveArray )
 		w->moveSet.clear( (uint32_t)key );
 	w->moveArray.clear();
@@ -518,6 +541,7 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 	L.items.clear();
 	L.bodyList.clear();
 	L.slotContact.clear();
+	L.ovfLevels.clear();
 	L.slotBodies.clear();
 	L.maxGroupSize = 0;
 	L.maxItemLds = 0;
@@ -657,6 +681,68 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 			L.maxGroupSize = std::max( L.maxGroupSize, size );
 			itemMaxGroup = std::max( itemMaxGroup, size );
 		}
+		// Overflow levels (group 0, src/contact_solver.c:212-345 runs it serially): a slot's level
+		// is one past the last level that changed one of its bodies, and the slots are ordered by
+		// level, keeping their order within a level. Static bodies are never changed; nor are
+		// kinematic ones, whose zero inverse mass and inertia leave their velocity bits as they
+		// are, unless a component is -0 (then they count as changed).
+		wi.ovfLevelBegin = (int)L.ovfLevels.size();
+		wi.ovfLevelCount = 0;
+		{
+			const int first = wi.slotStart + wi.groupStart[0], end = wi.slotStart + wi.groupStart[1];
+			if ( end > first )
+			{
+				const int nb = (int)L.bodyList.size() - wi.bodyStart;
+				auto changed = [&]( int li ) {
+					if ( li < 0 )
+						return false;
+					if ( li < wi.ownedCount )
+						return true;
+					const float* st = &w->awakeState[8 * (size_t)L.bodyList[wi.bodyStart + li]];
+					for ( int c = 0; c < 3; ++c )
+						if ( st[c] == 0.0f && std::signbit( st[c] ) )
+							return true;
+					return false;
+				};
+				std::vector<int> last( nb, -1 ), level( end - first );
+				int maxLevel = 0;
+				for ( int k = first; k < end; ++k )
+				{
+					int2 lb = L.slotBodies[k];
+					bool ca = changed( lb.x ), cb = changed( lb.y );
+					int lv = 0;
+					if ( ca )
+						lv = std::max( lv, last[lb.x] + 1 );
+					if ( cb )
+						lv = std::max( lv, last[lb.y] + 1 );
+					if ( ca )
+						last[lb.x] = lv;
+					if ( cb )
+						last[lb.y] = lv;
+					level[k - first] = lv;
+					maxLevel = std::max( maxLevel, lv );
+				}
+				std::vector<int> order( end - first );
+				for ( int k = 0; k < end - first; ++k )
+					order[k] = k;
+				std::stable_sort( order.begin(), order.end(), [&]( int a, int b ) { return level[a] < level[b]; } );
+				std::vector<int> contacts( end - first );
+				std::vector<int2> bodies( end - first );
+				for ( int k = 0; k < end - first; ++k )
+				{
+					contacts[k] = L.slotContact[first + order[k]];
+					bodies[k] = L.slotBodies[first + order[k]];
+				}
+				std::copy( contacts.begin(), contacts.end(), L.slotContact.begin() + first );
+				std::copy( bodies.begin(), bodies.end(), L.slotBodies.begin() + first );
+				for ( int k = 0; k < end - first; ++k )
+					if ( k == 0 || level[order[k]] != level[order[k - 1]] )
+						L.ovfLevels.push_back( wi.groupStart[0] + k );
+				L.ovfLevels.push_back( wi.groupStart[1] );
+				wi.ovfLevelCount = maxLevel + 1;
+				B2MI_CHECK( (int)L.ovfLevels.size() - wi.ovfLevelBegin == wi.ovfLevelCount + 1 );
+			}
+		}
 		// per-item workgroup: one thread per slot of its largest colour group
 		wi.block = 64;
 		while ( wi.block < itemMaxGroup && wi.block < kSolveMaxBlock )
@@ -713,27 +799,18 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 	w->solverPending = true;
 }
 
-// Every contact of the world (touching ones in the colours, the others in nonTouching),
-// listed body by body in awake-set order so that the contacts of neighbouring bodies sit next
-// to each other: threads of one wave then share shape and body cache lines. The narrowphase's
-// state changes are sorted by contact id afterwards, so this order does not affect results.
+// Every contact of the world (touching ones in the colours, the others in nonTouching), in
+// contact-id order: one sequential pass over the contact array. Ids are handed out as the pair
+// update finds new pairs, moved proxy by moved proxy, so neighbouring ids tend to share bodies
+// and shapes. The narrowphase's state changes are sorted by contact id afterwards, so this order
+// does not affect results.
 static void buildWorldCollide( World* w )
 {
 	w->collideIds.clear();
-	for ( int si = 0; si < (int)w->awakeSimBody.size(); ++si )
-	{
-		const Body& b = w->bodies[w->awakeSimBody[si]];
-		int key = b.headContactKey;
-		while ( key != -1 )
-		{
-			int cid = key >> 1, edge = key & 1;
-			const Contact& c = w->contacts[cid];
-			const Body& other = w->bodies[c.edges[edge ^ 1].bodyId];
-			if ( other.setIndex != kAwakeSet || other.localIndex > si )
-				w->collideIds.push_back( cid );
-			key = c.edges[edge].nextKey;
-		}
-	}
+	const int n = (int)w->contacts.size();
+	for ( int cid = 0; cid < n; ++cid )
+		if ( w->contacts[cid].contactId >= 0 )
+			w->collideIds.push_back( cid );
 	size_t expected = w->nonTouching.size();
 	for ( int c = 0; c < kColorCount; ++c )
 		expected += w->colors[c].size();
@@ -838,14 +915,18 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 	if ( itemsChanged )
 	{
 		g->hItems.clear();
+		g->hOvfLevels.clear();
 		g->maxGroupSize = 0;
 		g->maxItemLds = 0;
 		for ( World* w : g->worlds )
 		{
+			const int levelBase = (int)g->hOvfLevels.size();
+			g->hOvfLevels.insert( g->hOvfLevels.end(), w->solver.ovfLevels.begin(), w->solver.ovfLevels.end() );
 			for ( WorkItem wi : w->solver.items )
 			{
 				wi.bodyStart += w->bodyRegion.base;
 				wi.slotStart += w->slotRegion.base;
+				wi.ovfLevelBegin += levelBase;
 				g->hItems.push_back( wi );
 			}
 			g->maxGroupSize = std::max( g->maxGroupSize, w->solver.maxGroupSize );
@@ -876,6 +957,9 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		grow( g->dItems, g->capItems, std::max( g->itemCount, 1 ) );
 		if ( g->itemCount )
 			HIP_CHECK( hipMemcpyAsync( g->dItems, g->hItems.data(), sizeof( WorkItem ) * g->itemCount, hipMemcpyHostToDevice, s ) );
+		grow( g->dOvfLevels, g->capOvfLevels, std::max( (int)g->hOvfLevels.size(), 1 ) );
+		if ( !g->hOvfLevels.empty() )
+			HIP_CHECK( hipMemcpyAsync( g->dOvfLevels, g->hOvfLevels.data(), sizeof( int ) * g->hOvfLevels.size(), hipMemcpyHostToDevice, s ) );
 	}
 	// the host vectors are read by the copies above: they stay unchanged until the next step,
 	// which synchronizes the stream before it edits them
@@ -1112,6 +1196,11 @@ private:
 	std::atomic<int> next_{ 0 };
 };
 
+static void hostPoolRun( int n, const std::function<void( int )>& fn )
+{
+	HostPool::get().run( n, fn );
+}
+
 static void forEachWorld( StepGroup* g, const std::function<void( World* )>& fn )
 {
 	HostPool::get().run( (int)g->worlds.size(), [&]( int i ) { fn( g->worlds[i] ); } );
@@ -1204,8 +1293,8 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 	if ( blockSize > 0 || classes.size() <= 1 )
 	{
 		int block = blockSize > 0 ? blockSize : ( classes.empty() ? 64 : classes[0].block );
-		HIP_CHECK( launch_solve( dWorlds, dItems, 0, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dCons,
-								 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, s ) );
+		HIP_CHECK( launch_solve( dWorlds, dItems, 0, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
+								 dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, s ) );
 		return;
 	}
 	if ( forkEvent == nullptr )
@@ -1222,8 +1311,8 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		hipStream_t cs = c == 0 ? s : auxStream( ( c - 1 ) % kAuxStreams );
 		if ( c > 0 && c <= kAuxStreams )
 			HIP_CHECK( hipStreamWaitEvent( cs, forkEvent, 0 ) );
-		HIP_CHECK( launch_solve( dWorlds, dItems, lc.first, lc.count, lc.block, lc.lds, dBodyList, dSlotContact, dSlotBodies, dCons,
-								 capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
+		HIP_CHECK( launch_solve( dWorlds, dItems, lc.first, lc.count, lc.block, lc.lds, dBodyList, dSlotContact, dSlotBodies,
+								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
 	}
 	for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
 	{
@@ -1295,9 +1384,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 			paramsChanged.store( true );
 		}
 	};
+	// one world alone takes the pool for its own pair queries
+	const bool nestedPool = pass.size() == 1;
 	HostPool::get().run( (int)pass.size(), [&]( int i ) {
 		World* w = pass[i];
-		updatePairs( w );
+		updatePairs( w, nestedPool );
 		if ( w->contactsDirty )
 		{
 			buildWorldCollide( w );
@@ -1376,9 +1467,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 	HIP_CHECK( launch_collide( dWorlds, dCollideList, collideCount, dChanges, dCounters, s ) );
 	HIP_CHECK( hipEventRecord( e1, s ) );
 	// When no work item needs rebuilding, the solver goes right behind the narrowphase: a step
-	// in which no contact begins or ends touching (the usual one) then needs no host round trip.
-	// Otherwise the launch returns at once and the solver runs after the state changes below.
-	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0;
+	// in which no contact begins or ends touching (the usual one in a batch of settled worlds)
+	// then needs no host round trip. Otherwise the launch returns at once and the solver runs
+	// after the state changes below. Groups whose contacts change at most steps (one large
+	// world) stop speculating after a miss and try again every 16th step.
+	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && ( specMisses == 0 || epoch % 16 == 0 );
 	if ( speculate )
 	{
 		HIP_CHECK( hipEventRecord( e3, s ) );
@@ -1401,6 +1494,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	auto tc = std::chrono::steady_clock::now();
 	hostPhases[4] = msSince( t0 );
 	const int changeCount = hCounters[0];
+	if ( speculate )
+		specMisses = changeCount > 0 ? specMisses + 1 : 0;
 	std::vector<World*> changedTopology;
 	if ( changeCount > 0 )
 	{

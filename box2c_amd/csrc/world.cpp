@@ -144,6 +144,7 @@ StepGroup::~StepGroup()
 	devFree( dEnlarge );
 	devFree( dInits );
 	devFree( dItems );
+	devFree( dOvfLevels );
 	devFree( dBodyList );
 	devFree( dSlotContact );
 	devFree( dSlotBodies );

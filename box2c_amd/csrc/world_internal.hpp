@@ -170,6 +170,7 @@ struct SolverLists
 	std::vector<int> bodyList;	  // awake sim indices
 	std::vector<int> slotContact; // contact ids
 	std::vector<int2> slotBodies; // local body indices (-1 static)
+	std::vector<int> ovfLevels;	  // overflow level boundaries of the items (relative to each item's slots)
 	int maxGroupSize = 0, maxItemLds = 0;
 };
 
@@ -291,6 +292,9 @@ struct StepGroup
 	int capInits = 0;
 	WorkItem* dItems = nullptr;
 	int capItems = 0;
+	std::vector<int> hOvfLevels; // the worlds' overflow level boundaries, concatenated
+	int* dOvfLevels = nullptr;
+	int capOvfLevels = 0;
 	int itemCount = 0;
 	int* dBodyList = nullptr;
 	int capBodyList = 0;
@@ -335,6 +339,7 @@ struct StepGroup
 	std::vector<World*> active;
 	std::mutex activeMutex;
 	bool allActive = true;
+	int specMisses = 0; // consecutive speculative solver launches voided by contact changes
 	float lastTimeStep = -1.0f;
 	int lastSubSteps = -1;
 	int shapeTotal = 0;

@@ -102,6 +102,8 @@ PARITY_SCENES = [
     ("drop_boxes", {"count": 16, "seed": 7, "index": 3}, {"enableContinous": False}, 120, 4),
     ("batch_pyramid", {"seed": 1, "index": 5}, {}, 60, 4),
     ("tilted_stacks", {}, {}, 100, 3),
+    ("overflow", {"count": 30}, {}, 60, 4),
+    ("overflow", {"count": 30, "rakeVx": -0.0}, {}, 60, 4),
 ]
 
 
