@@ -335,10 +335,12 @@ class B2Api:
         elif name == "hello":
             ids = self._bodies(1)
             n = S.scene_hello(ctypes.byref(d), ctypes.byref(w), ids, len(ids))
-        elif name == "overflow":
+        elif name in ("overflow", "overflow_negzero"):
+            # overflow_negzero: the kinematic rake's velocity x is -0.0
             count = kw.get("count", 30)
             ids = self._bodies(1 + count)
-            n = S.scene_overflow(ctypes.byref(d), count, kw.get("rakeVx", 0.0), ctypes.byref(w), ids, len(ids))
+            vx = -0.0 if name == "overflow_negzero" else 0.0
+            n = S.scene_overflow(ctypes.byref(d), count, vx, ctypes.byref(w), ids, len(ids))
         elif name == "tilted_stacks":
             ids = self._bodies(100)
             n = S.scene_tilted_stacks(ctypes.byref(d), ctypes.byref(w), ids, len(ids))

@@ -39,6 +39,8 @@ SCENES = [
     ("drop_boxes", {"count": 16, "seed": 7, "index": 3}, {"enableContinous": False}, 120, 4, True),
     ("batch_pyramid", {"seed": 1, "index": 5}, {}, 60, 4, False),
     ("tilted_stacks", {}, {}, 100, 3, True),
+    ("overflow", {"count": 30}, {}, 60, 4, True),
+    ("overflow_negzero", {"count": 30}, {}, 60, 4, True),
 ]
 
 PAIRS_PER_FUNCTION = 96

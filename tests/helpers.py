@@ -103,7 +103,7 @@ PARITY_SCENES = [
     ("batch_pyramid", {"seed": 1, "index": 5}, {}, 60, 4),
     ("tilted_stacks", {}, {}, 100, 3),
     ("overflow", {"count": 30}, {}, 60, 4),
-    ("overflow", {"count": 30, "rakeVx": -0.0}, {}, 60, 4),
+    ("overflow_negzero", {"count": 30}, {}, 60, 4),
 ]
 
 

@@ -28,8 +28,8 @@ CASES = [
     "tests/test_gpu_parity.py::test_gpu_run_to_run_bit_reproducible",
     "tests/test_gpu_parity.py::test_step_parity_solver_storage[lds8k-tilted_stacks]",
     "tests/test_gpu_parity.py::test_step_parity_solver_storage[hbm-tumbler]",
-    "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow0]",
-    "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow1]",
+    "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow]",
+    "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow_negzero]",
 ]
 
 
