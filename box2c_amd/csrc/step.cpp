@@ -936,7 +936,14 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		// within a size: the hardware dispatches workgroups in order, so the few large islands of an
 		// uneven batch start at once and the small ones fill the CUs behind them. Items are
 		// independent; their order does not change any result.
-		std::stable_sort( g->hItems.begin(), g->hItems.end(), []( const WorkItem& a, const WorkItem& b ) {
+		// Items of one world chunk are consecutive (chunks in order).
+		const int nw = (int)g->worlds.size(), chunks = chunkCountFor( nw );
+		g->chunkCount = chunks;
+		auto chunkOf = [&]( const WorkItem& wi ) { return chunkOfWorld( wi.world, nw, chunks ); };
+		std::stable_sort( g->hItems.begin(), g->hItems.end(), [&]( const WorkItem& a, const WorkItem& b ) {
+			int ka = chunkOf( a ), kb = chunkOf( b );
+			if ( ka != kb )
+				return ka < kb;
 			if ( a.block != b.block )
 				return a.block > b.block;
 			int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
@@ -946,8 +953,9 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		for ( int i = 0; i < (int)g->hItems.size(); ++i )
 		{
 			const WorkItem& wi = g->hItems[i];
-			if ( g->classes.empty() || g->classes.back().block != wi.block )
-				g->classes.push_back( LaunchClass{ i, 0, wi.block, 0 } );
+			int chunk = chunkOf( wi );
+			if ( g->classes.empty() || g->classes.back().block != wi.block || g->classes.back().chunk != chunk )
+				g->classes.push_back( LaunchClass{ i, 0, wi.block, 0, chunk } );
 			LaunchClass& c = g->classes.back();
 			c.count += 1;
 			c.lds = std::max( c.lds, wi.ldsBytes );
@@ -1321,6 +1329,21 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 	}
 }
 
+// the solver launches of one world chunk (its size classes in turn) on stream s
+void StepGroup::launchSolveChunk( int chunk, hipStream_t s, const int* abortFlag )
+{
+	for ( const LaunchClass& c : classes )
+	{
+		if ( c.chunk != chunk )
+			continue;
+		int block = blockSize > 0 ? blockSize : c.block;
+		int lds = blockSize > 0 ? maxItemLds : c.lds;
+		B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
+		HIP_CHECK( launch_solve( dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
+								 dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, s ) );
+	}
+}
+
 void StepGroup::step( float timeStep, int subStepCount )
 {
 	auto t0 = std::chrono::steady_clock::now();
@@ -1438,8 +1461,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	}
 	if ( dCounters == nullptr )
 	{
-		HIP_CHECK( hipMalloc( (void**)&dCounters, sizeof( int ) * 4 ) );
-		HIP_CHECK( hipHostMalloc( (void**)&hCounters, sizeof( int ) * 4, hipHostMallocDefault ) );
+		HIP_CHECK( hipMalloc( (void**)&dCounters, sizeof( int ) * kCounterCount ) );
+		HIP_CHECK( hipHostMalloc( (void**)&hCounters, sizeof( int ) * kCounterCount, hipHostMallocDefault ) );
 	}
 	if ( !hInits.empty() )
 	{
@@ -1454,7 +1477,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 	if ( relayout )
 		layoutGroup( this );
 	publishLists( this, relayout ? nullptr : &collideDirty.list, relayout );
-	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * 4, s ) );
+	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * kCounterCount, s ) );
 	shapeTotal += shapeDelta.load();
 	grow( dEnlarge, capEnlarge, std::max( shapeTotal, 1 ) );
 	// events bracket each kernel launch alone (the stream is idle while the host works)
@@ -1463,24 +1486,68 @@ void StepGroup::step( float timeStep, int subStepCount )
 			HIP_CHECK( hipEventCreate( &e ) );
 	hipEvent_t e0 = events[0], e1 = events[1], e2 = events[2], e3 = events[3];
 	hostPhases[2] = msSince( t0 );
-	HIP_CHECK( hipEventRecord( e0, s ) );
-	HIP_CHECK( launch_collide( dWorlds, dCollideList, collideCount, dChanges, dCounters, s ) );
-	HIP_CHECK( hipEventRecord( e1, s ) );
 	// When no work item needs rebuilding, the solver goes right behind the narrowphase: a step
 	// in which no contact begins or ends touching (the usual one in a batch of settled worlds)
 	// then needs no host round trip. Otherwise the launch returns at once and the solver runs
 	// after the state changes below. Groups whose contacts change at most steps (one large
 	// world) stop speculating after a miss and try again every 16th step.
 	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && ( specMisses == 0 || epoch % 16 == 0 );
-	if ( speculate )
+	// World chunks: consecutive worlds, so each chunk's narrowphase list is one range and its
+	// state changes go to that range of the change buffer with a counter of their own. A
+	// chunk's speculative solver launch follows its narrowphase on the chunk's stream and is
+	// voided by that chunk's changes only.
+	const int chunks = chunkCount;
+	B2MI_CHECK( chunks >= 1 && chunks <= kMaxChunks );
+	for ( int c = 0; c <= chunks; ++c )
 	{
-		HIP_CHECK( hipEventRecord( e3, s ) );
-		launchSolve( s, dCounters );
-		HIP_CHECK( hipEventRecord( e2, s ) );
-		HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ) * 2, hipMemcpyDeviceToHost, s ) );
+		int wi = (int)( (int64_t)c * nw / chunks );
+		chunkCollide[c] = wi < nw ? worlds[wi]->collideRegion.base : collideCount;
 	}
-	else
-		HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	for ( int c = 0; c < chunks; ++c )
+		B2MI_CHECK( 0 <= chunkCollide[c] && chunkCollide[c] <= chunkCollide[c + 1] && chunkCollide[c + 1] <= collideCount );
+	auto chunkStream = [&]( int c ) { return c == 0 ? s : auxStream( c - 1 ); };
+	if ( forkEvent == nullptr )
+	{
+		HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
+		for ( hipEvent_t& e : joinEvents )
+			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+	}
+	HIP_CHECK( hipEventRecord( e0, s ) );
+	if ( chunks > 1 )
+	{
+		HIP_CHECK( hipEventRecord( forkEvent, s ) );
+		for ( int c = 1; c < chunks; ++c )
+			HIP_CHECK( hipStreamWaitEvent( chunkStream( c ), forkEvent, 0 ) );
+	}
+	for ( int c = 0; c < chunks; ++c )
+	{
+		hipStream_t cs = chunkStream( c );
+		int first = chunkCollide[c], count = chunkCollide[c + 1] - chunkCollide[c];
+		HIP_CHECK( launch_collide( dWorlds, dCollideList + first, count, dChanges + first, dCounters + kChunkCounter + c, cs ) );
+		if ( c == 0 )
+		{
+			// device times: e0-e1 the first chunk's narrowphase; e3-e2 the rest of the step (with
+			// several chunks it holds their narrowphases too, beside the solver)
+			HIP_CHECK( hipEventRecord( e1, s ) );
+			if ( speculate )
+				HIP_CHECK( hipEventRecord( e3, s ) );
+		}
+		if ( speculate )
+		{
+			if ( chunks == 1 )
+				launchSolve( s, dCounters + kChunkCounter );
+			else
+				launchSolveChunk( c, cs, dCounters + kChunkCounter + c );
+		}
+	}
+	for ( int c = 1; c < chunks; ++c )
+	{
+		HIP_CHECK( hipEventRecord( joinEvents[c - 1], chunkStream( c ) ) );
+		HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
+	}
+	if ( speculate )
+		HIP_CHECK( hipEventRecord( e2, s ) );
+	HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ) * kCounterCount, hipMemcpyDeviceToHost, s ) );
 
 	// the tree rebuild overlaps the narrowphase (src/world.c:367-379, 420)
 	auto tb = std::chrono::steady_clock::now();
@@ -1493,14 +1560,30 @@ void StepGroup::step( float timeStep, int subStepCount )
 	HIP_CHECK( hipStreamSynchronize( s ) );
 	auto tc = std::chrono::steady_clock::now();
 	hostPhases[4] = msSince( t0 );
-	const int changeCount = hCounters[0];
+	int changeCount = 0;
+	bool chunkChanged[kMaxChunks] = {};
+	for ( int c = 0; c < chunks; ++c )
+	{
+		int n = hCounters[kChunkCounter + c];
+		B2MI_CHECK( 0 <= n && n <= chunkCollide[c + 1] - chunkCollide[c] );
+		chunkChanged[c] = n > 0;
+		changeCount += n;
+	}
 	if ( speculate )
 		specMisses = changeCount > 0 ? specMisses + 1 : 0;
 	std::vector<World*> changedTopology;
 	if ( changeCount > 0 )
 	{
 		hChanges.resize( changeCount );
-		HIP_CHECK( hipMemcpyAsync( hChanges.data(), dChanges, sizeof( ChangeRecord ) * changeCount, hipMemcpyDeviceToHost, s ) );
+		int at = 0;
+		for ( int c = 0; c < chunks; ++c )
+		{
+			int n = hCounters[kChunkCounter + c];
+			if ( n > 0 )
+				HIP_CHECK( hipMemcpyAsync( hChanges.data() + at, dChanges + chunkCollide[c], sizeof( ChangeRecord ) * n,
+										   hipMemcpyDeviceToHost, s ) );
+			at += n;
+		}
 		HIP_CHECK( hipStreamSynchronize( s ) );
 		std::sort( hChanges.begin(), hChanges.end(), []( const ChangeRecord& a, const ChangeRecord& b ) {
 			return a.world != b.world ? a.world < b.world : a.contactId < b.contactId;
@@ -1521,7 +1604,9 @@ void StepGroup::step( float timeStep, int subStepCount )
 			}
 	}
 
-	// 3. solver (one workgroup per work item); rebuild the items of worlds whose graph changed
+	// 3. solver (one workgroup per work item); rebuild the items of worlds whose graph changed.
+	// After a speculative launch only the chunks with state changes run again: the others have
+	// already advanced their worlds.
 	hostPhases[5] = msSince( t0 );
 	if ( !speculate || changeCount > 0 )
 	{
@@ -1545,11 +1630,32 @@ void StepGroup::step( float timeStep, int subStepCount )
 				layoutGroup( this );
 			publishLists( this, full ? nullptr : &topology.list, true );
 		}
+		B2MI_CHECK( chunkCount == chunks );
 		if ( tape )
 			tapeRecordStep( this, timeStep, subStepCount, blockSize > 0 ? std::min( blockSize, kAdjointMaxBlock ) : kAdjointMaxBlock );
 		hostPhases[6] = msSince( t0 );
 		HIP_CHECK( hipEventRecord( e3, s ) );
-		launchSolve( s, nullptr );
+		if ( chunks == 1 )
+			launchSolve( s, nullptr );
+		else
+		{
+			int used = 0;
+			HIP_CHECK( hipEventRecord( forkEvent, s ) );
+			for ( int c = 0; c < chunks; ++c )
+				if ( !speculate || chunkChanged[c] )
+				{
+					if ( c > 0 )
+						HIP_CHECK( hipStreamWaitEvent( chunkStream( c ), forkEvent, 0 ) );
+					launchSolveChunk( c, chunkStream( c ), nullptr );
+					if ( c > 0 )
+					{
+						HIP_CHECK( hipEventRecord( joinEvents[c - 1], chunkStream( c ) ) );
+						HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
+					}
+					used += 1;
+				}
+			B2MI_CHECK( used > 0 );
+		}
 		HIP_CHECK( hipEventRecord( e2, s ) );
 		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
 		hostPhases[7] = msSince( t0 );

@@ -255,13 +255,30 @@ struct World
 
 struct Tape; // taped run of a step group (tape.cpp)
 
-// items of one solver workgroup size: a contiguous range of the launch-ordered items
+// items of one world chunk and one solver workgroup size: a contiguous range of the
+// launch-ordered items
 struct LaunchClass
 {
-	int first, count, block, lds;
+	int first, count, block, lds, chunk;
 };
 
 constexpr int kAuxStreams = 3;
+// A group of many worlds is stepped in this many chunks of consecutive worlds, each chunk's
+// narrowphase and solver on a stream of its own, so that the solver of one chunk runs beside
+// the narrowphase of the next.
+constexpr int kMaxChunks = kAuxStreams + 1;
+constexpr int kChunkMinWorlds = 64;
+inline int chunkCountFor( int worldCount )
+{
+	return worldCount >= kChunkMinWorlds ? kMaxChunks : 1;
+}
+inline int chunkOfWorld( int worldIndex, int worldCount, int chunks )
+{
+	return (int)( (int64_t)worldIndex * chunks / worldCount );
+}
+// device counters of a step: [1] enlarged shapes, [kChunkCounter + c] state changes of chunk c
+constexpr int kChunkCounter = 4;
+constexpr int kCounterCount = kChunkCounter + kMaxChunks;
 
 // a set of worlds advanced together by one narrowphase launch and one solver launch
 struct StepGroup
@@ -346,6 +363,9 @@ struct StepGroup
 	uint64_t epoch = 0;	   // steps taken; host mirrors of an older epoch are stale
 	b2Profile profile{};   // of the last step, shared by the group's worlds
 	void launchSolve( hipStream_t s, const int* abortFlag );
+	void launchSolveChunk( int chunk, hipStream_t s, const int* abortFlag );
+	int chunkCount = 1;							 // of the launch-ordered items
+	int chunkCollide[kMaxChunks + 1] = {};		 // collide-list range of each chunk
 	hipEvent_t forkEvent = nullptr;
 	hipEvent_t joinEvents[kAuxStreams] = {};
 

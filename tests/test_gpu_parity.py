@@ -190,6 +190,35 @@ def test_small_batch_parity():
             cpu.destroy(wc)
 
 
+def test_chunked_batch_parity():
+    """72 worlds: the batch is stepped in world chunks, each chunk's narrowphase and solver on a
+    stream of its own; a chunk whose contacts change state runs its solver again after the host
+    work while the others keep their speculative results. Falling boxes change state at some
+    steps and not at others."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    d = gpu.default_world_def(enableSleep=False, enableContinous=False)
+    dc = cpu.default_world_def(enableSleep=False, enableContinous=False)
+    batch = gpu.create_batch()
+    gpu.use_batch(batch)
+    worlds_g, worlds_c = [], []
+    try:
+        for i in range(72):
+            worlds_g.append(gpu.scene("drop_boxes", d, count=4, seed=9, index=i)[0])
+            worlds_c.append(cpu.scene("drop_boxes", dc, count=4, seed=9, index=i)[0])
+    finally:
+        gpu.use_batch(None)
+    try:
+        for s in range(45):
+            gpu.batch_step(batch)
+            for wc in worlds_c:
+                cpu.step(wc)
+        _compare_worlds(gpu, cpu, worlds_g, worlds_c, "chunked batch")
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
+
+
 @pytest.mark.parametrize("target", [128, 1024])
 def test_batch_per_item_block_sizes_parity(target):
     """Workgroup size per work item (blockSize 0): items of each size go to their own launch,
