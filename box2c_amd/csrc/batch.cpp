@@ -178,6 +178,17 @@ B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes )
 	g->allActive = true;
 }
 
+// chunks: world chunks whose narrowphase and solver launches are pipelined (0 = one chunk).
+// Results are identical for every count.
+B2_API void b2mi_Batch_SetWorldChunks( int batch, int chunks )
+{
+	StepGroup* g = getBatch( batch );
+	B2MI_CHECK( chunks >= 0 );
+	g->chunksRequested = chunks;
+	g->structureDirty = true;
+	g->allActive = true;
+}
+
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount )
 {
 	StepGroup* g = getBatch( batch );

@@ -937,7 +937,7 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		// uneven batch start at once and the small ones fill the CUs behind them. Items are
 		// independent; their order does not change any result.
 		// Items of one world chunk are consecutive (chunks in order).
-		const int nw = (int)g->worlds.size(), chunks = chunkCountFor( nw );
+		const int nw = (int)g->worlds.size(), chunks = chunkCountFor( nw, g->chunksRequested );
 		g->chunkCount = chunks;
 		auto chunkOf = [&]( const WorkItem& wi ) { return chunkOfWorld( wi.world, nw, chunks ); };
 		std::stable_sort( g->hItems.begin(), g->hItems.end(), [&]( const WorkItem& a, const WorkItem& b ) {
@@ -1310,6 +1310,8 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
 		for ( hipEvent_t& e : joinEvents )
 			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+		for ( hipEvent_t& e : collideDone )
+			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
 	}
 	HIP_CHECK( hipEventRecord( forkEvent, s ) );
 	int n = (int)classes.size();
@@ -1329,9 +1331,13 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 	}
 }
 
-// the solver launches of one world chunk (its size classes in turn) on stream s
-void StepGroup::launchSolveChunk( int chunk, hipStream_t s, const int* abortFlag )
+// The solver launches of one world chunk, its size classes side by side: class j on extra
+// stream j (mod the stream count), each stream first waiting for `ready` (recorded on the
+// group's stream) and then joined back into it through joinEvents.
+void StepGroup::launchSolveChunk( int chunk, hipEvent_t ready, const int* abortFlag )
 {
+	int j = 0;
+	bool used[kAuxStreams] = {};
 	for ( const LaunchClass& c : classes )
 	{
 		if ( c.chunk != chunk )
@@ -1339,9 +1345,24 @@ void StepGroup::launchSolveChunk( int chunk, hipStream_t s, const int* abortFlag
 		int block = blockSize > 0 ? blockSize : c.block;
 		int lds = blockSize > 0 ? maxItemLds : c.lds;
 		B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
+		int k = j++ % kAuxStreams;
+		hipStream_t as = auxStream( k );
+		if ( !used[k] )
+			HIP_CHECK( hipStreamWaitEvent( as, ready, 0 ) );
+		used[k] = true;
 		HIP_CHECK( launch_solve( dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
-								 dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, s ) );
+								 dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, as ) );
 	}
+	for ( int k = 0; k < kAuxStreams; ++k )
+		if ( used[k] )
+			HIP_CHECK( hipEventRecord( joinEvents[k], auxStream( k ) ) );
+}
+
+// makes the group's stream wait for every extra stream's last recorded join event
+void StepGroup::joinAux( hipStream_t s )
+{
+	for ( int k = 0; k < kAuxStreams; ++k )
+		HIP_CHECK( hipStreamWaitEvent( s, joinEvents[k], 0 ) );
 }
 
 void StepGroup::step( float timeStep, int subStepCount )
@@ -1505,25 +1526,19 @@ void StepGroup::step( float timeStep, int subStepCount )
 	}
 	for ( int c = 0; c < chunks; ++c )
 		B2MI_CHECK( 0 <= chunkCollide[c] && chunkCollide[c] <= chunkCollide[c + 1] && chunkCollide[c + 1] <= collideCount );
-	auto chunkStream = [&]( int c ) { return c == 0 ? s : auxStream( c - 1 ); };
 	if ( forkEvent == nullptr )
 	{
 		HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
 		for ( hipEvent_t& e : joinEvents )
 			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+		for ( hipEvent_t& e : collideDone )
+			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
 	}
 	HIP_CHECK( hipEventRecord( e0, s ) );
-	if ( chunks > 1 )
-	{
-		HIP_CHECK( hipEventRecord( forkEvent, s ) );
-		for ( int c = 1; c < chunks; ++c )
-			HIP_CHECK( hipStreamWaitEvent( chunkStream( c ), forkEvent, 0 ) );
-	}
 	for ( int c = 0; c < chunks; ++c )
 	{
-		hipStream_t cs = chunkStream( c );
 		int first = chunkCollide[c], count = chunkCollide[c + 1] - chunkCollide[c];
-		HIP_CHECK( launch_collide( dWorlds, dCollideList + first, count, dChanges + first, dCounters + kChunkCounter + c, cs ) );
+		HIP_CHECK( launch_collide( dWorlds, dCollideList + first, count, dChanges + first, dCounters + kChunkCounter + c, s ) );
 		if ( c == 0 )
 		{
 			// device times: e0-e1 the first chunk's narrowphase; e3-e2 the rest of the step (with
@@ -1532,19 +1547,16 @@ void StepGroup::step( float timeStep, int subStepCount )
 			if ( speculate )
 				HIP_CHECK( hipEventRecord( e3, s ) );
 		}
-		if ( speculate )
+		if ( speculate && chunks > 1 )
 		{
-			if ( chunks == 1 )
-				launchSolve( s, dCounters + kChunkCounter );
-			else
-				launchSolveChunk( c, cs, dCounters + kChunkCounter + c );
+			HIP_CHECK( hipEventRecord( collideDone[c], s ) );
+			launchSolveChunk( c, collideDone[c], dCounters + kChunkCounter + c );
 		}
 	}
-	for ( int c = 1; c < chunks; ++c )
-	{
-		HIP_CHECK( hipEventRecord( joinEvents[c - 1], chunkStream( c ) ) );
-		HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
-	}
+	if ( speculate && chunks == 1 )
+		launchSolve( s, dCounters + kChunkCounter );
+	if ( speculate && chunks > 1 )
+		joinAux( s );
 	if ( speculate )
 		HIP_CHECK( hipEventRecord( e2, s ) );
 	HIP_CHECK( hipMemcpyAsync( hCounters, dCounters, sizeof( int ) * kCounterCount, hipMemcpyDeviceToHost, s ) );
@@ -1644,17 +1656,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 			for ( int c = 0; c < chunks; ++c )
 				if ( !speculate || chunkChanged[c] )
 				{
-					if ( c > 0 )
-						HIP_CHECK( hipStreamWaitEvent( chunkStream( c ), forkEvent, 0 ) );
-					launchSolveChunk( c, chunkStream( c ), nullptr );
-					if ( c > 0 )
-					{
-						HIP_CHECK( hipEventRecord( joinEvents[c - 1], chunkStream( c ) ) );
-						HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
-					}
+					launchSolveChunk( c, forkEvent, nullptr );
 					used += 1;
 				}
 			B2MI_CHECK( used > 0 );
+			joinAux( s );
 		}
 		HIP_CHECK( hipEventRecord( e2, s ) );
 		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );

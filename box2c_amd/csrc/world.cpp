@@ -156,6 +156,9 @@ StepGroup::~StepGroup()
 			(void)hipEventDestroy( e );
 	if ( forkEvent )
 		(void)hipEventDestroy( forkEvent );
+	for ( hipEvent_t e : collideDone )
+		if ( e )
+			(void)hipEventDestroy( e );
 	for ( hipEvent_t e : joinEvents )
 		if ( e )
 			(void)hipEventDestroy( e );

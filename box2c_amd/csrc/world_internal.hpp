@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <mutex>
@@ -263,14 +264,16 @@ struct LaunchClass
 };
 
 constexpr int kAuxStreams = 3;
-// A group of many worlds is stepped in this many chunks of consecutive worlds, each chunk's
-// narrowphase and solver on a stream of its own, so that the solver of one chunk runs beside
-// the narrowphase of the next.
-constexpr int kMaxChunks = kAuxStreams + 1;
-constexpr int kChunkMinWorlds = 64;
-inline int chunkCountFor( int worldCount )
+// A group of worlds can be stepped in up to this many chunks of consecutive worlds: the chunks'
+// narrowphase launches run in turn on the group's stream and each chunk's solver as soon as its
+// narrowphase is done, beside the narrowphase of the next chunk. Off by default: on 4096
+// pyramid worlds the solver launches already fill the GPU and the overlap measured no gain.
+constexpr int kMaxChunks = kAuxStreams;
+inline int chunkCountFor( int worldCount, int requested )
 {
-	return worldCount >= kChunkMinWorlds ? kMaxChunks : 1;
+	if ( requested > 0 )
+		return std::min( std::min( requested, kMaxChunks ), std::max( worldCount, 1 ) );
+	return 1;
 }
 inline int chunkOfWorld( int worldIndex, int worldCount, int chunks )
 {
@@ -363,11 +366,14 @@ struct StepGroup
 	uint64_t epoch = 0;	   // steps taken; host mirrors of an older epoch are stale
 	b2Profile profile{};   // of the last step, shared by the group's worlds
 	void launchSolve( hipStream_t s, const int* abortFlag );
-	void launchSolveChunk( int chunk, hipStream_t s, const int* abortFlag );
+	void launchSolveChunk( int chunk, hipEvent_t ready, const int* abortFlag );
+	void joinAux( hipStream_t s );
 	int chunkCount = 1;							 // of the launch-ordered items
+	int chunksRequested = 0;					 // 0 = automatic
 	int chunkCollide[kMaxChunks + 1] = {};		 // collide-list range of each chunk
 	hipEvent_t forkEvent = nullptr;
 	hipEvent_t joinEvents[kAuxStreams] = {};
+	hipEvent_t collideDone[kMaxChunks] = {};
 
 	~StepGroup();
 	void step( float timeStep, int subStepCount );

@@ -84,6 +84,11 @@ B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
 // a CU. Results are bit-identical either way.
 B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes );
 
+// World chunks (0 = one chunk; at most 3): the batch's narrowphase runs chunk after chunk and
+// each chunk's solver starts as soon as its chunk's narrowphase is done, beside the next one.
+// Results are bit-identical for every count.
+B2_API void b2mi_Batch_SetWorldChunks( int batch, int chunks );
+
 // Autotuner: steps the batch (or world) under each candidate launch shape (threads per
 // workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
 // rebuild step, and keeps the shape with the fastest solver kernel. Every shape gives

@@ -25,7 +25,7 @@ CASES = [
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[shape_zoo]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[batch_pyramid]",
     "tests/test_gpu_parity.py::test_small_batch_parity",
-    "tests/test_gpu_parity.py::test_chunked_batch_parity",
+    "tests/test_gpu_parity.py::test_chunked_batch_parity[3]",
     "tests/test_gpu_parity.py::test_gpu_run_to_run_bit_reproducible",
     "tests/test_gpu_parity.py::test_step_parity_solver_storage[lds8k-tilted_stacks]",
     "tests/test_gpu_parity.py::test_step_parity_solver_storage[hbm-tumbler]",

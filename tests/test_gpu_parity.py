@@ -190,11 +190,12 @@ def test_small_batch_parity():
             cpu.destroy(wc)
 
 
-def test_chunked_batch_parity():
-    """72 worlds: the batch is stepped in world chunks, each chunk's narrowphase and solver on a
-    stream of its own; a chunk whose contacts change state runs its solver again after the host
-    work while the others keep their speculative results. Falling boxes change state at some
-    steps and not at others."""
+@pytest.mark.parametrize("chunks", [1, 3])
+def test_chunked_batch_parity(chunks):
+    """72 worlds, in one chunk or in three whose solver launches follow each chunk's narrowphase:
+    a chunk whose contacts change state runs its solver again after the host work while the
+    others keep their speculative results. Falling boxes change state at some steps and not at
+    others."""
     gpu, cpu = helpers.product(), helpers.oracle()
     d = gpu.default_world_def(enableSleep=False, enableContinous=False)
     dc = cpu.default_world_def(enableSleep=False, enableContinous=False)
@@ -207,6 +208,7 @@ def test_chunked_batch_parity():
             worlds_c.append(cpu.scene("drop_boxes", dc, count=4, seed=9, index=i)[0])
     finally:
         gpu.use_batch(None)
+    gpu.lib.b2mi_Batch_SetWorldChunks(batch, chunks)
     try:
         for s in range(45):
             gpu.batch_step(batch)
