@@ -273,6 +273,9 @@ def main():
     ap.add_argument("--tumbler-shapes", type=int, default=40000, help="shapes of the tumbler leg (0 = skip it)")
     ap.add_argument("--tumbler-steps", type=int, default=600)
     ap.add_argument("--tumbler-cpu-steps", type=int, default=60)
+    ap.add_argument("--legs", default="plain,batched,adjoint,tumbler",
+                    help="comma-separated legs to run (a profiling run may keep one; the JSON line then lacks the others)")
+    ap.add_argument("--batch-no-autotune", action="store_true", help="batched leg at the default launch shape")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain copy kernel of known size (for HBM counter scaling)")
     args = ap.parse_args()
@@ -290,8 +293,23 @@ def main():
     if api.lib.b2mi_DeviceCount() < 1:
         raise SystemExit("no HIP device visible")
 
-    r = run_gpu(api, "pyramid_grid", args.steps, args.warmup, args.rows, args.cols)
+    legs = set(args.legs.split(","))
     substeps = 4
+    if "plain" not in legs:
+        out = {"metric": METRIC, "legs": sorted(legs)}
+        if "batched" in legs and args.batch_worlds > 0:
+            rb = run_gpu_batch(api, args.batch_worlds, args.batch_steps, args.warmup, autotune=not args.batch_no_autotune)
+            out["batched"] = {"value": rb["bodies"] * substeps * args.batch_steps / rb["elapsed"],
+                              "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps, "launch_shape": rb["launch_shape"]}
+        if "adjoint" in legs and args.adjoint_worlds > 0:
+            out["adjoint"] = run_adjoint(api, args.adjoint_worlds, 16, 120, 10)
+        if "tumbler" in legs and args.tumbler_shapes > 0:
+            rt = run_tumbler(api, args.tumbler_shapes, args.tumbler_steps, args.warmup)
+            out["tumbler"] = {"value": rt["bodies"] * substeps * args.tumbler_steps / rt["elapsed"],
+                              "ms_per_step": 1e3 * rt["elapsed"] / args.tumbler_steps}
+        print(json.dumps(out), flush=True)
+        return
+    r = run_gpu(api, "pyramid_grid", args.steps, args.warmup, args.rows, args.cols)
     value = r["bodies"] * substeps * args.steps / r["elapsed"]
     ab = algorithmic_bytes(r["bodies"], r["contacts"], substeps)
     achieved = ab["solve"] / (r["ms_solve"] * 1e-3) / 1e9
