@@ -271,6 +271,7 @@ class B2Api:
             S.scene_hello.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
             S.scene_tilted_stacks.argtypes = [POINTER(WorldDef), POINTER(WorldId), c_void_p, c_int]
             S.scene_overflow.argtypes = [POINTER(WorldDef), c_int, c_float, POINTER(WorldId), c_void_p, c_int]
+            S.scene_polygon_zoo.argtypes = [POINTER(WorldDef), c_int, c_uint64, POINTER(WorldId), c_void_p, c_int]
             S.scene_single.argtypes = [POINTER(WorldDef), c_int, c_float, c_float, c_float, c_float, c_float,
                                        c_float, c_float, c_float, c_float, c_float, c_int,
                                        POINTER(WorldId), c_void_p, c_int]
@@ -335,6 +336,10 @@ class B2Api:
         elif name == "hello":
             ids = self._bodies(1)
             n = S.scene_hello(ctypes.byref(d), ctypes.byref(w), ids, len(ids))
+        elif name == "polygon_zoo":
+            copies = kw.get("copies", 4)
+            ids = self._bodies(copies * 8)
+            n = S.scene_polygon_zoo(ctypes.byref(d), copies, kw.get("seed", 1), ctypes.byref(w), ids, len(ids))
         elif name in ("overflow", "overflow_negzero"):
             # overflow_negzero: the kinematic rake's velocity x is -0.0
             count = kw.get("count", 30)

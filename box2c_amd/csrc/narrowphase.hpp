@@ -62,23 +62,28 @@ B2MI_HD inline b2Vec2 d_length_normalize( float* length, b2Vec2 v )
 // ------------------------------------------------------------------------------------
 // narrowphase
 
-struct Poly
+// a polygon of at most N vertices (N = 8, b2_maxPolygonVertices; N = 4 for the box and
+// capsule pairs that dominate the scenes, with half the loop trips and selects)
+template <int N>
+struct PolyN
 {
-	b2Vec2 v[8];
-	b2Vec2 n[8];
+	b2Vec2 v[N];
+	b2Vec2 n[N];
 	float radius;
 	int count;
 };
+using Poly = PolyN<8>;
 
 // Element i of a polygon array. Every polygon array here is indexed either by a constant (fully
 // unrolled loops over the 8 slots, guarded by i < count) or through pick(), so the arrays live
 // in registers on the GPU instead of per-lane scratch memory.
-B2MI_HD inline b2Vec2 pick( const b2Vec2 ( &a )[8], int i )
+template <int N>
+B2MI_HD inline b2Vec2 pick( const b2Vec2 ( &a )[N], int i )
 {
 	// bitwise select: a compare-and-select chain would be folded back into one indexed load
 	uint32_t x = 0u, y = 0u;
 #pragma unroll
-	for ( int k = 0; k < 8; ++k )
+	for ( int k = 0; k < N; ++k )
 	{
 		uint32_t mask = 0u - (uint32_t)( i == k );
 		x |= __builtin_bit_cast( uint32_t, a[k].x ) & mask;
@@ -124,10 +129,11 @@ B2MI_HD inline void mf_finish_point( Manifold& m, b2Transform xfA, b2Transform x
 }
 
 // manifold.c:14-30
-B2MI_HD inline void make_capsule( Poly& s, b2Vec2 p1, b2Vec2 p2, float radius )
+template <int N>
+B2MI_HD inline void make_capsule( PolyN<N>& s, b2Vec2 p1, b2Vec2 p2, float radius )
 {
 #pragma unroll
-	for ( int i = 2; i < 8; ++i )
+	for ( int i = 2; i < N; ++i )
 	{
 		s.v[i] = v2( 0.0f, 0.0f );
 		s.n[i] = v2( 0.0f, 0.0f );
@@ -195,7 +201,8 @@ B2MI_HD inline void collide_capsule_circle( Manifold& m, b2Vec2 p1, b2Vec2 p2, f
 }
 
 // manifold.c:137-253
-B2MI_HD inline void collide_polygon_circle( Manifold& m, const Poly& A, b2Transform xfA, b2Vec2 cB, float rB, b2Transform xfB )
+template <int N>
+B2MI_HD inline void collide_polygon_circle( Manifold& m, const PolyN<N>& A, b2Transform xfA, b2Vec2 cB, float rB, b2Transform xfB )
 {
 	b2Transform xf = b2InvMulTransforms( xfA, xfB );
 	b2Vec2 c = b2TransformPoint( xf, cB );
@@ -204,7 +211,7 @@ B2MI_HD inline void collide_polygon_circle( Manifold& m, const Poly& A, b2Transf
 	int normalIndex = 0;
 	float separation = -FLT_MAX;
 #pragma unroll
-	for ( int i = 0; i < 8; ++i )
+	for ( int i = 0; i < N; ++i )
 	{
 		if ( i < A.count )
 		{
@@ -244,12 +251,13 @@ B2MI_HD inline void collide_polygon_circle( Manifold& m, const Poly& A, b2Transf
 }
 
 // manifold.c:414-450
-B2MI_HD inline float find_max_separation( int* edgeIndex, const Poly& p1, const Poly& p2 )
+template <int N>
+B2MI_HD inline float find_max_separation( int* edgeIndex, const PolyN<N>& p1, const PolyN<N>& p2 )
 {
 	int bestIndex = 0;
 	float maxSeparation = -FLT_MAX;
 #pragma unroll
-	for ( int i = 0; i < 8; ++i )
+	for ( int i = 0; i < N; ++i )
 	{
 		if ( i < p1.count )
 		{
@@ -257,7 +265,7 @@ B2MI_HD inline float find_max_separation( int* edgeIndex, const Poly& p1, const 
 			b2Vec2 v1 = p1.v[i];
 			float si = FLT_MAX;
 #pragma unroll
-			for ( int j = 0; j < 8; ++j )
+			for ( int j = 0; j < N; ++j )
 			{
 				if ( j < p2.count )
 				{
@@ -334,7 +342,8 @@ B2MI_HD inline void segment_distance( b2Vec2 p1, b2Vec2 q1, b2Vec2 p2, b2Vec2 q2
 }
 
 // manifold.c:276-411
-B2MI_HD inline void clip_polygons( Manifold& m, const Poly& polyA, const Poly& polyB, int edgeA, int edgeB, bool flip )
+template <int N>
+B2MI_HD inline void clip_polygons( Manifold& m, const PolyN<N>& polyA, const PolyN<N>& polyB, int edgeA, int edgeB, bool flip )
 {
 	int i11, i12, i21, i22;
 	b2Vec2 normal, v11, v12, v21, v22;
@@ -431,7 +440,9 @@ B2MI_HD inline bool vertex_point( Manifold& m, b2Vec2 vA, b2Vec2 vB, float distS
 }
 
 // manifold.c:470-700
-B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Transform xfA, const Poly& polygonB, b2Transform xfB )
+template <int N>
+B2MI_HD inline void collide_polygons( Manifold& m, const PolyN<N>& polygonA, b2Transform xfA, const PolyN<N>& polygonB,
+									  b2Transform xfB )
 {
 	b2Vec2 origin = polygonA.v[0];
 	b2Transform sfA;
@@ -439,22 +450,22 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 	sfA.q = xfA.q;
 	b2Transform xf = b2InvMulTransforms( sfA, xfB );
 
-	Poly localA;
+	PolyN<N> localA;
 	localA.count = polygonA.count;
 	localA.radius = polygonA.radius;
 	localA.v[0] = v2( 0.0f, 0.0f );
 	localA.n[0] = polygonA.n[0];
 #pragma unroll
-	for ( int i = 1; i < 8; ++i )
+	for ( int i = 1; i < N; ++i )
 	{
 		localA.v[i] = i < localA.count ? b2Sub( polygonA.v[i], origin ) : v2( 0.0f, 0.0f );
 		localA.n[i] = polygonA.n[i];
 	}
-	Poly localB;
+	PolyN<N> localB;
 	localB.count = polygonB.count;
 	localB.radius = polygonB.radius;
 #pragma unroll
-	for ( int i = 0; i < 8; ++i )
+	for ( int i = 0; i < N; ++i )
 	{
 		localB.v[i] = i < localB.count ? b2TransformPoint( xf, polygonB.v[i] ) : v2( 0.0f, 0.0f );
 		localB.n[i] = i < localB.count ? b2RotateVector( xf.q, polygonB.n[i] ) : v2( 0.0f, 0.0f );
@@ -476,7 +487,7 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 		edgeB = 0;
 		float minDot = FLT_MAX;
 #pragma unroll
-		for ( int i = 0; i < 8; ++i )
+		for ( int i = 0; i < N; ++i )
 		{
 			float dot = b2Dot( searchDirection, localB.n[i] );
 			if ( i < localB.count && dot < minDot )
@@ -493,7 +504,7 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 		edgeA = 0;
 		float minDot = FLT_MAX;
 #pragma unroll
-		for ( int i = 0; i < 8; ++i )
+		for ( int i = 0; i < N; ++i )
 		{
 			float dot = b2Dot( searchDirection, localA.n[i] );
 			if ( i < localA.count && dot < minDot )
@@ -560,16 +571,34 @@ B2MI_HD inline void collide_polygons( Manifold& m, const Poly& polygonA, b2Trans
 	}
 }
 
-B2MI_HD inline void load_poly( Poly& p, const DevShape& s )
+template <int N>
+B2MI_HD inline void load_poly( PolyN<N>& p, const DevShape& s )
 {
 	p.count = s.count;
 	p.radius = s.radius;
 #pragma unroll
-	for ( int i = 0; i < 8; ++i )
+	for ( int i = 0; i < N; ++i )
 	{
 		p.v[i] = i < s.count ? v2( s.v[i][0], s.v[i][1] ) : v2( 0.0f, 0.0f );
 		p.n[i] = i < s.count ? v2( s.n[i][0], s.n[i][1] ) : v2( 0.0f, 0.0f );
 	}
+}
+
+// polygon, capsule or segment pairs at N vertex slots
+template <int N>
+B2MI_HD inline void collide_pair( Manifold& m, const DevShape& A, b2Transform xfA, const DevShape& B, b2Transform xfB, b2Vec2 a0,
+								  b2Vec2 a1, b2Vec2 b0, b2Vec2 b1 )
+{
+	PolyN<N> pa, pb;
+	if ( A.type == kPolygon )
+		load_poly( pa, A );
+	else
+		make_capsule( pa, a0, a1, A.type == kSegment ? 0.0f : A.radius );
+	if ( B.type == kPolygon )
+		load_poly( pb, B );
+	else
+		make_capsule( pb, b0, b1, B.type == kSegment ? 0.0f : B.radius );
+	collide_polygons( m, pa, xfA, pb, xfB );
 }
 
 // dispatch of src/contact.c:150-183 (shape A is always the primary type of the pair)
@@ -587,6 +616,12 @@ B2MI_HD inline void collide_shapes( Manifold& m, const DevShape& A, b2Transform 
 			collide_capsule_circle( m, a0, a1, A.radius, xfA, b0, B.radius, xfB );
 		else if ( ta == kSegment )
 			collide_capsule_circle( m, a0, a1, 0.0f, xfA, b0, B.radius, xfB );
+		else if ( A.count <= 4 )
+		{
+			PolyN<4> pa;
+			load_poly( pa, A );
+			collide_polygon_circle( m, pa, xfA, b0, B.radius, xfB );
+		}
 		else
 		{
 			Poly pa;
@@ -595,16 +630,12 @@ B2MI_HD inline void collide_shapes( Manifold& m, const DevShape& A, b2Transform 
 		}
 		return;
 	}
-	Poly pa, pb;
-	if ( ta == kPolygon )
-		load_poly( pa, A );
+	// the same code at 4 or 8 vertex slots: the arithmetic on the used slots is the same
+	int na = ta == kPolygon ? A.count : 2, nb = tb == kPolygon ? B.count : 2;
+	if ( na <= 4 && nb <= 4 )
+		collide_pair<4>( m, A, xfA, B, xfB, a0, a1, b0, b1 );
 	else
-		make_capsule( pa, a0, a1, ta == kSegment ? 0.0f : A.radius );
-	if ( tb == kPolygon )
-		load_poly( pb, B );
-	else
-		make_capsule( pb, b0, b1, tb == kSegment ? 0.0f : B.radius );
-	collide_polygons( m, pa, xfA, pb, xfB );
+		collide_pair<8>( m, A, xfA, B, xfB, a0, a1, b0, b1 );
 }
 
 } // namespace b2mi

@@ -18,6 +18,7 @@
 #include "box2d/box2d.h"
 
 #include <math.h>
+#include <string.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -445,6 +446,82 @@ SCENE_API int scene_shape_zoo( const b2WorldDef* def, int copies, uint64_t seed,
 					b2CreateCircleShape( bodyId, &shapeDef, &circle );
 				}
 				break;
+			}
+			record( outBodies, capacity, n++, bodyId );
+		}
+	}
+	return n;
+}
+
+// A regular polygon of `count` vertices (3..8) around the origin, built field by field (the
+// vertices on a circle of `radius`, outward unit normals, the centroid at the origin).
+static b2Polygon regular_polygon( int count, float radius, float roundness )
+{
+	b2Polygon p;
+	memset( &p, 0, sizeof( p ) );
+	p.count = count;
+	for ( int i = 0; i < count; ++i )
+	{
+		float a = 6.2831853f * (float)i / (float)count;
+		p.vertices[i] = ( b2Vec2 ){ radius * cosf( a ), radius * sinf( a ) };
+	}
+	for ( int i = 0; i < count; ++i )
+	{
+		b2Vec2 e = { p.vertices[( i + 1 ) % count].x - p.vertices[i].x, p.vertices[( i + 1 ) % count].y - p.vertices[i].y };
+		float len = sqrtf( e.x * e.x + e.y * e.y );
+		p.normals[i] = ( b2Vec2 ){ e.y / len, -e.x / len };
+	}
+	p.radius = roundness;
+	return p;
+}
+
+// Polygons of 3 to 8 vertices (rounded ones too), boxes and circles piled on a ground box:
+// narrowphase pairs beyond four vertex slots.
+SCENE_API int scene_polygon_zoo( const b2WorldDef* def, int copies, uint64_t seed, b2WorldId* outWorld, b2BodyId* outBodies,
+								 int capacity )
+{
+	Pcg32 rng = pcg32_seed( seed, 11 );
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2Polygon box = b2MakeBox( 30.0f, 1.0f );
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	int n = 0;
+	for ( int k = 0; k < copies; ++k )
+	{
+		for ( int kind = 0; kind < 8; ++kind )
+		{
+			b2BodyDef bodyDef = b2DefaultBodyDef();
+			bodyDef.type = b2_dynamicBody;
+			float x = -12.0f + 1.5f * ( ( k * 8 + kind ) % 16 ) + pcg32_uniform( &rng, -0.2f, 0.2f );
+			float y = 0.8f + 1.4f * ( ( k * 8 + kind ) / 16 ) + pcg32_uniform( &rng, 0.0f, 0.4f );
+			bodyDef.position = ( b2Vec2 ){ x, y };
+			bodyDef.rotation = b2MakeRot( pcg32_uniform( &rng, -1.0f, 1.0f ) );
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			shapeDef.friction = pcg32_uniform( &rng, 0.3f, 0.8f );
+			float r = pcg32_uniform( &rng, 0.35f, 0.6f );
+			if ( kind < 6 )
+			{
+				// 3, 5, 6, 7, 8 vertices; the sixth kind a rounded hexagon
+				static const int counts[6] = { 3, 5, 6, 7, 8, 6 };
+				b2Polygon poly = regular_polygon( counts[kind], r, kind == 5 ? 0.08f : 0.0f );
+				b2CreatePolygonShape( bodyId, &shapeDef, &poly );
+			}
+			else if ( kind == 6 )
+			{
+				b2Polygon box = b2MakeBox( 0.6f * r, 0.4f * r );
+				b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			}
+			else
+			{
+				b2Circle circle = { { 0.0f, 0.0f }, 0.6f * r };
+				b2CreateCircleShape( bodyId, &shapeDef, &circle );
 			}
 			record( outBodies, capacity, n++, bodyId );
 		}

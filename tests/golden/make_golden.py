@@ -41,6 +41,7 @@ SCENES = [
     ("tilted_stacks", {}, {}, 100, 3, True),
     ("overflow", {"count": 30}, {}, 60, 4, True),
     ("overflow_negzero", {"count": 30}, {}, 60, 4, True),
+    ("polygon_zoo", {"copies": 6, "seed": 4}, {"enableContinous": False}, 90, 4, True),
 ]
 
 PAIRS_PER_FUNCTION = 96

@@ -104,6 +104,7 @@ PARITY_SCENES = [
     ("tilted_stacks", {}, {}, 100, 3),
     ("overflow", {"count": 30}, {}, 60, 4),
     ("overflow_negzero", {"count": 30}, {}, 60, 4),
+    ("polygon_zoo", {"copies": 6, "seed": 4}, {"enableContinous": False}, 90, 4),
 ]
 
 

@@ -23,6 +23,7 @@ CASES = [
     "tests/test_gpu_adjoint.py::test_autograd_function",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[hello]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[shape_zoo]",
+    "tests/test_gpu_parity.py::test_step_parity_with_oracle[polygon_zoo]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[batch_pyramid]",
     "tests/test_gpu_parity.py::test_small_batch_parity",
     "tests/test_gpu_parity.py::test_chunked_batch_parity[3]",
