@@ -255,14 +255,8 @@ B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity )
 		}
 		n += 1;
 	};
-	if ( g->blockSize > 0 || g->classes.size() <= 1 )
-	{
-		if ( g->itemCount > 0 )
-			put( 0, g->itemCount, g->blockSize > 0 ? g->blockSize : g->classes[0].block, g->maxItemLds );
-	}
-	else
-		for ( const LaunchClass& c : g->classes )
-			put( c.first, c.count, c.block, c.lds );
+	for ( const LaunchClass& c : g->classes )
+		put( c.first, c.count, g->blockSize > 0 ? g->blockSize : c.block, g->blockSize > 0 ? g->maxItemLds : c.lds );
 	return n;
 }
 

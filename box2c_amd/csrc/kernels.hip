@@ -364,6 +364,21 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 	}
 }
 
+// This thread's first slot of the colour group after g (cyclically over the colours 0..10,
+// groups 1..kGroups-1), or -1: the colour sweeps visit the same slots in every pass, so a store
+// can load a slot's read-only quads one phase ahead.
+__device__ inline int next_colour_slot( const WorkItem& wi, int g, int tid )
+{
+	for ( int step = 1; step < kGroups; ++step )
+	{
+		int n = 1 + ( g - 1 + step ) % ( kGroups - 1 );
+		int b = wi.groupStart[n], e = wi.groupStart[n + 1];
+		if ( b < e )
+			return b + tid < e ? b + tid : -1;
+	}
+	return -1;
+}
+
 // All stages of one step for one work item, by one workgroup, with workgroup barriers between
 // dependent phases. `C` holds the item's private state; `bl` its body list (sim indices) and
 // `sc` the contact id of each of its slots.
@@ -394,6 +409,19 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	}
 	__syncthreads();
 
+	// read-only quads of this thread's slot in the next colour group, loaded one phase ahead by
+	// stores that read them from HBM (the sweeps visit the same slots in every pass)
+	float4 pre[cfMutableQuad];
+	int preSlot = -1;
+	auto ahead = [&]( int g ) {
+		if constexpr ( S::kPrefetch )
+		{
+			int nk = next_colour_slot( wi, g, tid );
+			if ( nk >= 0 )
+				C.loadQuads( nk, pre );
+			preSlot = nk;
+		}
+	};
 	const int ovfBegin = wi.groupStart[0], ovfEnd = wi.groupStart[1];
 	// the overflow group runs level by level (slots of one level share no changed body)
 	const int levelCount = ovfEnd > ovfBegin ? wi.ovfLevelCount : 0;
@@ -426,8 +454,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				for ( int k = b + tid; k < e; k += nt )
 				{
 					int2 lb = C.bodies( k );
-					lane_warm_start( C, k, lb.x, lb.y );
+					if ( S::kPrefetch && k == preSlot )
+						lane_warm_start_q( C, k, lb.x, lb.y, pre );
+					else
+						lane_warm_start( C, k, lb.x, lb.y );
 				}
+				ahead( g );
 				__syncthreads();
 			}
 
@@ -449,8 +481,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
+				if ( S::kPrefetch && k == preSlot )
+					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, true, W );
+				else
+					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
 			}
+			ahead( g );
 			__syncthreads();
 		}
 
@@ -476,8 +512,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
+				if ( S::kPrefetch && k == preSlot )
+					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, false, W );
+				else
+					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
 			}
+			ahead( g );
 			__syncthreads();
 		}
 	}
@@ -501,8 +541,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		for ( int k = b + tid; k < e; k += nt )
 		{
 			int2 lb = C.bodies( k );
-			lane_restitution( C, k, lb.x, lb.y, threshold );
+			if ( S::kPrefetch && k == preSlot )
+				lane_restitution_q( C, k, lb.x, lb.y, pre, threshold );
+			else
+				lane_restitution( C, k, lb.x, lb.y, threshold );
 		}
+		ahead( g );
 		__syncthreads();
 	}
 
@@ -511,8 +555,11 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	{
 		DevManifold& m = W.contacts[sc[k]].manifold;
 		int n = k < ovfEnd ? m.pointCount : 2;
-		for ( int j = 0; j < n; ++j )
+#pragma unroll
+		for ( int j = 0; j < 2; ++j )
 		{
+			if ( j >= n )
+				break;
 			float4 imp = C.quad( k, cfMutableQuad + j );
 			m.points[j].normalImpulse = imp.x;
 			m.points[j].tangentImpulse = imp.y;
@@ -533,17 +580,20 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	}
 }
 
-// One workgroup per work item. kStoreLds items keep their state in the dynamic LDS block
-// (layout: body state, slot bodies, constraint fields), kStoreBodiesLds items only the body
-// state, kStoreHbm items nothing.
+// the dynamic LDS block of the solver kernels (its size is given at launch)
+extern __shared__ float4 lds[];
+
+// One workgroup per work item; one kernel per store (every item of a launch has that store).
+// kStoreLds items keep their state in the dynamic LDS block (layout: body state, constraint
+// records, masses), kStoreBodiesLds items the body state and the written quads, kStoreHbm items
+// nothing.
+template <int Store>
 __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, const int* __restrict__ ovfLevels, float* cfBase,
-												   int cfCapacity,
-												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
-												   int itemOffset, const int* abortFlag )
+												   int cfCapacity, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount,
+												   int enlCapacity, int itemOffset, const int* abortFlag )
 {
-	extern __shared__ float4 lds[];
 	if ( abortFlag != nullptr && *abortFlag != 0 )
 		return;
 	// read in place (uniform scalar loads); a private copy would be indexed dynamically from scratch
@@ -552,7 +602,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	const int* bl = bodyList + wi.bodyStart;
 	const int* sc = slotContact + wi.slotStart;
 	const int total = wi.groupStart[kGroups];
-	if ( wi.store == kStoreLds )
+	if constexpr ( Store == kStoreLds )
 	{
 		// LDS (float4 arrays first, 16-byte aligned): body state, constraint records, masses
 		LdsStore C;
@@ -567,9 +617,10 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
 		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
-	else if ( wi.store == kStoreBodiesLds )
+	else if constexpr ( Store == kStoreBodiesLds )
 	{
-		// LDS: body state, written quads, masses, slot bodies; HBM: the read-only quads
+		// LDS: body state, written quads, masses, slot bodies; HBM: the read-only quads, loaded one
+		// colour phase ahead
 		BodiesLdsStore C;
 		C.rec = reinterpret_cast<float4*>( cfBase );
 		C.base = wi.slotStart;
@@ -618,27 +669,38 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 	return hipGetLastError();
 }
 
-hipError_t launch_solve( const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize, int ldsBytes,
-						 const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels, float* cf,
-						 int cfCapacity,
-						 float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, const int* abortFlag,
-						 hipStream_t stream )
+hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize,
+						 int ldsBytes, const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels,
+						 float* cf, int cfCapacity, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
+						 const int* abortFlag, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
 	if ( ldsBytes < 0 || ldsBytes > kMaxItemLdsBytes || blockSize < 64 || blockSize > kSolveMaxBlock )
 		return hipErrorInvalidValue;
-	static int allowedLds = 0; // dynamic LDS above 64 KiB must be enabled per kernel
-	if ( ldsBytes > 64 * 1024 && allowedLds < kMaxItemLdsBytes )
+	const void* kernels[3] = { reinterpret_cast<const void*>( &k_solve<kStoreHbm> ),
+							   reinterpret_cast<const void*>( &k_solve<kStoreBodiesLds> ),
+							   reinterpret_cast<const void*>( &k_solve<kStoreLds> ) };
+	if ( store < 0 || store > 2 )
+		return hipErrorInvalidValue;
+	static int allowedLds[3] = { 0, 0, 0 }; // dynamic LDS above 64 KiB must be enabled per kernel
+	if ( ldsBytes > 64 * 1024 && allowedLds[store] < kMaxItemLdsBytes )
 	{
-		hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( &k_solve ),
-											hipFuncAttributeMaxDynamicSharedMemorySize, kMaxItemLdsBytes );
+		hipError_t e = hipFuncSetAttribute( kernels[store], hipFuncAttributeMaxDynamicSharedMemorySize, kMaxItemLdsBytes );
 		if ( e != hipSuccess )
 			return e;
-		allowedLds = kMaxItemLdsBytes;
+		allowedLds[store] = kMaxItemLdsBytes;
 	}
-	hipLaunchKernelGGL( k_solve, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact,
-						slotBodies, ovfLevels, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag );
+#define B2MI_LAUNCH_SOLVE( S )                                                                                                  \
+	hipLaunchKernelGGL( k_solve<S>, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact, \
+						slotBodies, ovfLevels, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag )
+	if ( store == kStoreLds )
+		B2MI_LAUNCH_SOLVE( kStoreLds );
+	else if ( store == kStoreBodiesLds )
+		B2MI_LAUNCH_SOLVE( kStoreBodiesLds );
+	else
+		B2MI_LAUNCH_SOLVE( kStoreHbm );
+#undef B2MI_LAUNCH_SOLVE
 	return hipGetLastError();
 }
 

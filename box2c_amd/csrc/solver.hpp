@@ -87,6 +87,13 @@ struct GlobalStore
 	{
 		rec[(size_t)( base + k ) * cfQuads + q] = v;
 	}
+	static constexpr bool kPrefetch = false;
+	__device__ inline void loadQuads( int k, float4 ( &r )[cfMutableQuad] ) const
+	{
+#pragma unroll
+		for ( int q = 0; q < cfMutableQuad; ++q )
+			r[q] = quad( k, q );
+	}
 	__device__ inline int2 bodies( int k ) const
 	{
 		return sb[k];
@@ -109,6 +116,8 @@ struct BodiesLdsStore
 {
 	float4* rec; // HBM: the read-only quads of slot k at rec[(base + k) * cfQuads]
 	int base;
+	// the colour sweeps load a slot's read-only quads one phase ahead (loadQuads)
+	static constexpr bool kPrefetch = true;
 	float4* mut; // LDS: the written quads of slot k at mut[2k], mut[2k + 1]
 	int2* sb;
 	float4* P;
@@ -121,7 +130,16 @@ struct BodiesLdsStore
 	}
 	__device__ inline float4 quad( int k, int q ) const
 	{
-		return q >= cfMutableQuad ? mut[2 * k + q - cfMutableQuad] : rec[(size_t)( base + k ) * cfQuads + q];
+		if ( q >= cfMutableQuad )
+			return mut[2 * k + q - cfMutableQuad];
+		return rec[(size_t)( base + k ) * cfQuads + q];
+	}
+	__device__ inline void loadQuads( int k, float4 ( &r )[cfMutableQuad] ) const
+	{
+		const float4* p = rec + (size_t)( base + k ) * cfQuads;
+#pragma unroll
+		for ( int q = 0; q < cfMutableQuad; ++q )
+			r[q] = p[q];
 	}
 	__device__ inline void setQuad( int k, int q, float4 v ) const
 	{
@@ -165,6 +183,13 @@ struct LdsStore
 	__device__ inline void setQuad( int k, int q, float4 v ) const
 	{
 		rec[k * cfQuads + q] = v;
+	}
+	static constexpr bool kPrefetch = false;
+	__device__ inline void loadQuads( int k, float4 ( &r )[cfMutableQuad] ) const
+	{
+#pragma unroll
+		for ( int q = 0; q < cfMutableQuad; ++q )
+			r[q] = quad( k, q );
 	}
 	__device__ inline int2 bodies( int k ) const
 	{
@@ -283,13 +308,13 @@ __device__ inline void lane_apply( Body& bA, Body& bB, float mA, float iA, float
 
 // contact_solver.c:810-863
 template <class S>
-__device__ void lane_warm_start( const S& C, int slot, int la, int lb )
+__device__ void lane_warm_start_q( const S& C, int slot, int la, int lb, const float4 ( &r )[cfMutableQuad] )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
-	const float4 q0 = C.quad( slot, 0 );
-	const float4 anc[2] = { C.quad( slot, 1 ), C.quad( slot, 2 ) };
+	const float4 q0 = r[0];
+	const float4 anc[2] = { r[1], r[2] };
 	const float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
 	float nx = q0.z, ny = q0.w;
 	float tx = ny, ty = 0.0f - nx;
@@ -314,14 +339,15 @@ __device__ void lane_warm_start( const S& C, int slot, int la, int lb )
 
 // contact_solver.c:865-1075
 template <class S>
-__device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias, const DevWorld& W )
+__device__ void lane_solve_q( const S& C, int slot, int la, int lb, const float4 ( &r )[cfMutableQuad], float inv_h, float minBiasVel,
+							  bool useBias, const DevWorld& W )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
-	const float4 q0 = C.quad( slot, 0 );
-	const float4 anc[2] = { C.quad( slot, 1 ), C.quad( slot, 2 ) };
-	const float4 pm[2] = { C.quad( slot, 3 ), C.quad( slot, 4 ) };
+	const float4 q0 = r[0];
+	const float4 anc[2] = { r[1], r[2] };
+	const float4 pm[2] = { r[3], r[4] };
 	float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
 	float nx = q0.z, ny = q0.w;
 	const float* soft = contact_softness( W, la, lb );
@@ -381,14 +407,14 @@ __device__ void lane_solve( const S& C, int slot, int la, int lb, float inv_h, f
 
 // contact_solver.c:1077-1178
 template <class S>
-__device__ void lane_restitution( const S& C, int slot, int la, int lb, float threshold )
+__device__ void lane_restitution_q( const S& C, int slot, int la, int lb, const float4 ( &r )[cfMutableQuad], float threshold )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
-	const float4 q0 = C.quad( slot, 0 );
-	const float4 anc[2] = { C.quad( slot, 1 ), C.quad( slot, 2 ) };
-	const float4 pm[2] = { C.quad( slot, 3 ), C.quad( slot, 4 ) };
+	const float4 q0 = r[0];
+	const float4 anc[2] = { r[1], r[2] };
+	const float4 pm[2] = { r[3], r[4] };
 	float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
 	float nx = q0.z, ny = q0.w;
 	float restitution = pm[1].w;
@@ -414,6 +440,32 @@ __device__ void lane_restitution( const S& C, int slot, int la, int lb, float th
 	C.setQuad( slot, 6, imp[1] );
 	store_velocity( C, la, bA );
 	store_velocity( C, lb, bB );
+}
+
+// the lane stages with the slot's read-only quads loaded from its store
+template <class S>
+__device__ inline void lane_warm_start( const S& C, int slot, int la, int lb )
+{
+	float4 r[cfMutableQuad];
+	C.loadQuads( slot, r );
+	lane_warm_start_q( C, slot, la, lb, r );
+}
+
+template <class S>
+__device__ inline void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias,
+								   const DevWorld& W )
+{
+	float4 r[cfMutableQuad];
+	C.loadQuads( slot, r );
+	lane_solve_q( C, slot, la, lb, r, inv_h, minBiasVel, useBias, W );
+}
+
+template <class S>
+__device__ inline void lane_restitution( const S& C, int slot, int la, int lb, float threshold )
+{
+	float4 r[cfMutableQuad];
+	C.loadQuads( slot, r );
+	lane_restitution_q( C, slot, la, lb, r, threshold );
 }
 
 // ---- overflow (serial) stages, contact_solver.c:149-471 ----

@@ -946,6 +946,8 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 				return ka < kb;
 			if ( a.block != b.block )
 				return a.block > b.block;
+			if ( a.store != b.store )
+				return a.store < b.store;
 			int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
 			return ca > cb;
 		} );
@@ -954,8 +956,9 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		{
 			const WorkItem& wi = g->hItems[i];
 			int chunk = chunkOf( wi );
-			if ( g->classes.empty() || g->classes.back().block != wi.block || g->classes.back().chunk != chunk )
-				g->classes.push_back( LaunchClass{ i, 0, wi.block, 0, chunk } );
+			if ( g->classes.empty() || g->classes.back().block != wi.block || g->classes.back().chunk != chunk ||
+				 g->classes.back().store != wi.store )
+				g->classes.push_back( LaunchClass{ i, 0, wi.block, 0, chunk, wi.store } );
 			LaunchClass& c = g->classes.back();
 			c.count += 1;
 			c.lds = std::max( c.lds, wi.ldsBytes );
@@ -1293,16 +1296,21 @@ struct WorldPick
 // nonzero (a speculative launch behind the narrowphase, void when contacts changed state).
 void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 {
-	if ( blockSize > 0 )
-		B2MI_CHECK( blockSize >= 64 && blockSize <= kSolveMaxBlock && blockSize % 64 == 0 );
-	else
-		for ( const LaunchClass& c : classes )
-			B2MI_CHECK( c.block >= 64 && c.block <= kSolveMaxBlock && c.block % 64 == 0 );
-	if ( blockSize > 0 || classes.size() <= 1 )
+	for ( const LaunchClass& c : classes )
 	{
-		int block = blockSize > 0 ? blockSize : ( classes.empty() ? 64 : classes[0].block );
-		HIP_CHECK( launch_solve( dWorlds, dItems, 0, itemCount, block, maxItemLds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
-								 dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, s ) );
+		int block = blockSize > 0 ? blockSize : c.block;
+		B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
+	}
+	auto launch = [&]( const LaunchClass& c, hipStream_t cs ) {
+		int block = blockSize > 0 ? blockSize : c.block;
+		int lds = blockSize > 0 ? maxItemLds : c.lds;
+		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
+								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
+	};
+	if ( classes.size() <= 1 )
+	{
+		if ( !classes.empty() )
+			launch( classes[0], s );
 		return;
 	}
 	if ( forkEvent == nullptr )
@@ -1317,12 +1325,10 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 	int n = (int)classes.size();
 	for ( int c = 0; c < n; ++c )
 	{
-		const LaunchClass& lc = classes[c];
 		hipStream_t cs = c == 0 ? s : auxStream( ( c - 1 ) % kAuxStreams );
 		if ( c > 0 && c <= kAuxStreams )
 			HIP_CHECK( hipStreamWaitEvent( cs, forkEvent, 0 ) );
-		HIP_CHECK( launch_solve( dWorlds, dItems, lc.first, lc.count, lc.block, lc.lds, dBodyList, dSlotContact, dSlotBodies,
-								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
+		launch( classes[c], cs );
 	}
 	for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
 	{
@@ -1350,8 +1356,8 @@ void StepGroup::launchSolveChunk( int chunk, hipEvent_t ready, const int* abortF
 		if ( !used[k] )
 			HIP_CHECK( hipStreamWaitEvent( as, ready, 0 ) );
 		used[k] = true;
-		HIP_CHECK( launch_solve( dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
-								 dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, as ) );
+		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
+								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, as ) );
 	}
 	for ( int k = 0; k < kAuxStreams; ++k )
 		if ( used[k] )

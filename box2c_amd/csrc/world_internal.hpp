@@ -260,7 +260,7 @@ struct Tape; // taped run of a step group (tape.cpp)
 // launch-ordered items
 struct LaunchClass
 {
-	int first, count, block, lds, chunk;
+	int first, count, block, lds, chunk, store;
 };
 
 constexpr int kAuxStreams = 3;

@@ -23,13 +23,6 @@ thread_local EmuThreadState* emu_current = nullptr;
 thread_local uint3 emu_blockIdx{ 0, 0, 0 };
 thread_local dim3 emu_blockDim, emu_gridDim;
 
-// dynamic shared memory of the kernels (the one `extern __shared__` array of the library)
-namespace b2mi
-{
-constexpr size_t kEmuSharedBytes = 160 * 1024;
-thread_local float4 lds[kEmuSharedBytes / sizeof( float4 )];
-} // namespace b2mi
-
 namespace
 {
 

@@ -65,6 +65,16 @@ inline int4 make_int4( int x, int y, int z, int w )
 	return int4{ x, y, z, w };
 }
 
+// The dynamic shared memory of the library's kernels (their one `extern __shared__` array,
+// b2mi::lds), defined here as an inline variable so every translation unit accesses it
+// directly: an `extern thread_local` array defined in another unit is reached through a TLS
+// wrapper function that the defining unit need not emit.
+namespace b2mi
+{
+constexpr size_t kEmuSharedBytes = 160 * 1024;
+inline thread_local float4 lds[kEmuSharedBytes / sizeof( float4 )];
+} // namespace b2mi
+
 // ---- execution state of the running kernel thread ----
 struct EmuThreadState
 {
