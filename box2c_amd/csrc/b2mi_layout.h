@@ -8,6 +8,7 @@
 // stage and substep of its work item with workgroup barriers only.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace b2mi
@@ -95,33 +96,52 @@ struct DevContact
 	int pad1;
 };
 
+// Two 128-byte lines: the first holds all a shape of up to four vertices gives the narrowphase
+// (vertices with their edge normals, type, body, count, radius, fat AABB), so a box pair
+// touches one line per shape; vertices 4..7 and the centroid are in the second.
 struct DevShape
 {
+	float vn[4][4]; // vertices 0..3: x, y, edge normal x, y
 	int type;
 	int body;
-	uint32_t flags;
 	int count;
-	float radius, friction, restitution, pad;
-	float aabb[4];	  // lower x, lower y, upper x, upper y
-	float fatAABB[4]; // same order
+	float radius;
+	float fatAABB[4]; // lower x, lower y, upper x, upper y
+	float aabb[4];	  // same order
+	uint32_t flags;
+	float friction, restitution, pad;
+	float vnHigh[4][4]; // vertices 4..7
 	float centroid[2];
-	float pad2[2];
-	float v[8][2];
-	float n[8][2];
+	float pad2[14];
+	// vertex i: [0], [1] position, [2], [3] edge normal
+	__host__ __device__ inline float* vert( int i )
+	{
+		return i < 4 ? vn[i] : vnHigh[i - 4];
+	}
+	__host__ __device__ inline const float* vert( int i ) const
+	{
+		return i < 4 ? vn[i] : vnHigh[i - 4];
+	}
 };
+static_assert( sizeof( DevShape ) == 256, "two cache lines" );
 
+// One 128-byte line; the first 32 bytes are what the narrowphase reads.
 struct DevBodySim
 {
 	float p[2], q[2]; // transform: origin and rotation (c, s)
+	float localCenter[2];
+	float invMass, invI;
 	float center[2], center0[2];
-	float rotation0[2], localCenter[2];
+	float rotation0[2];
 	float force[2];
 	float torque, mass;
-	float invMass, invI, minExtent, maxExtent;
+	float minExtent, maxExtent;
 	float linearDamping, angularDamping, gravityScale, sleepThreshold;
 	uint32_t flags;
 	int shapeStart, shapeCount, pad;
+	float pad2[4];
 };
+static_assert( sizeof( DevBodySim ) == 128, "one cache line" );
 
 // Body state (the reference's 32-byte b2BodyState): two float4 per body
 //   s0 = (v.x, v.y, w, flags), s1 = (dp.x, dp.y, dq.c, dq.s)

@@ -206,15 +206,15 @@ __device__ inline void shape_aabb( const DevShape& s, b2Transform xf, float* out
 	b2Vec2 lower, upper;
 	if ( s.type == kCircle )
 	{
-		b2Vec2 p = b2TransformPoint( xf, v2( s.v[0][0], s.v[0][1] ) );
+		b2Vec2 p = b2TransformPoint( xf, v2( s.vn[0][0], s.vn[0][1] ) );
 		float r = s.radius;
 		lower = v2( p.x - r, p.y - r );
 		upper = v2( p.x + r, p.y + r );
 	}
 	else if ( s.type == kCapsule || s.type == kSegment )
 	{
-		b2Vec2 a = b2TransformPoint( xf, v2( s.v[0][0], s.v[0][1] ) );
-		b2Vec2 b = b2TransformPoint( xf, v2( s.v[1][0], s.v[1][1] ) );
+		b2Vec2 a = b2TransformPoint( xf, v2( s.vn[0][0], s.vn[0][1] ) );
+		b2Vec2 b = b2TransformPoint( xf, v2( s.vn[1][0], s.vn[1][1] ) );
 		lower = b2Min( a, b );
 		upper = b2Max( a, b );
 		if ( s.type == kCapsule )
@@ -226,11 +226,11 @@ __device__ inline void shape_aabb( const DevShape& s, b2Transform xf, float* out
 	}
 	else
 	{
-		lower = b2TransformPoint( xf, v2( s.v[0][0], s.v[0][1] ) );
+		lower = b2TransformPoint( xf, v2( s.vn[0][0], s.vn[0][1] ) );
 		upper = lower;
 		for ( int i = 1; i < s.count; ++i )
 		{
-			b2Vec2 v = b2TransformPoint( xf, v2( s.v[i][0], s.v[i][1] ) );
+			b2Vec2 v = b2TransformPoint( xf, v2( s.vert( i )[0], s.vert( i )[1] ) );
 			lower = b2Min( lower, v );
 			upper = b2Max( upper, v );
 		}

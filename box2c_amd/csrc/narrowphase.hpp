@@ -579,8 +579,8 @@ B2MI_HD inline void load_poly( PolyN<N>& p, const DevShape& s )
 #pragma unroll
 	for ( int i = 0; i < N; ++i )
 	{
-		p.v[i] = i < s.count ? v2( s.v[i][0], s.v[i][1] ) : v2( 0.0f, 0.0f );
-		p.n[i] = i < s.count ? v2( s.n[i][0], s.n[i][1] ) : v2( 0.0f, 0.0f );
+		p.v[i] = i < s.count ? v2( s.vert( i )[0], s.vert( i )[1] ) : v2( 0.0f, 0.0f );
+		p.n[i] = i < s.count ? v2( s.vert( i )[2], s.vert( i )[3] ) : v2( 0.0f, 0.0f );
 	}
 }
 
@@ -606,8 +606,8 @@ B2MI_HD inline void collide_shapes( Manifold& m, const DevShape& A, b2Transform 
 {
 	mf_clear( m );
 	int ta = A.type, tb = B.type;
-	b2Vec2 a0 = v2( A.v[0][0], A.v[0][1] ), a1 = v2( A.v[1][0], A.v[1][1] );
-	b2Vec2 b0 = v2( B.v[0][0], B.v[0][1] ), b1 = v2( B.v[1][0], B.v[1][1] );
+	b2Vec2 a0 = v2( A.vn[0][0], A.vn[0][1] ), a1 = v2( A.vn[1][0], A.vn[1][1] );
+	b2Vec2 b0 = v2( B.vn[0][0], B.vn[0][1] ), b1 = v2( B.vn[1][0], B.vn[1][1] );
 	if ( tb == kCircle )
 	{
 		if ( ta == kCircle )

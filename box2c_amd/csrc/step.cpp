@@ -76,24 +76,24 @@ static void fillDevShape( DevShape& d, const Shape& s )
 	switch ( s.type )
 	{
 		case b2_circleShape:
-			d.v[0][0] = s.circle.center.x;
-			d.v[0][1] = s.circle.center.y;
+			d.vn[0][0] = s.circle.center.x;
+			d.vn[0][1] = s.circle.center.y;
 			d.radius = s.circle.radius;
 			d.count = 1;
 			break;
 		case b2_capsuleShape:
-			d.v[0][0] = s.capsule.center1.x;
-			d.v[0][1] = s.capsule.center1.y;
-			d.v[1][0] = s.capsule.center2.x;
-			d.v[1][1] = s.capsule.center2.y;
+			d.vn[0][0] = s.capsule.center1.x;
+			d.vn[0][1] = s.capsule.center1.y;
+			d.vn[1][0] = s.capsule.center2.x;
+			d.vn[1][1] = s.capsule.center2.y;
 			d.radius = s.capsule.radius;
 			d.count = 2;
 			break;
 		case b2_segmentShape:
-			d.v[0][0] = s.segment.point1.x;
-			d.v[0][1] = s.segment.point1.y;
-			d.v[1][0] = s.segment.point2.x;
-			d.v[1][1] = s.segment.point2.y;
+			d.vn[0][0] = s.segment.point1.x;
+			d.vn[0][1] = s.segment.point1.y;
+			d.vn[1][0] = s.segment.point2.x;
+			d.vn[1][1] = s.segment.point2.y;
 			d.count = 2;
 			break;
 		case b2_polygonShape:
@@ -103,10 +103,10 @@ static void fillDevShape( DevShape& d, const Shape& s )
 			d.centroid[1] = s.polygon.centroid.y;
 			for ( int i = 0; i < s.polygon.count; ++i )
 			{
-				d.v[i][0] = s.polygon.vertices[i].x;
-				d.v[i][1] = s.polygon.vertices[i].y;
-				d.n[i][0] = s.polygon.normals[i].x;
-				d.n[i][1] = s.polygon.normals[i].y;
+				d.vert( i )[0] = s.polygon.vertices[i].x;
+				d.vert( i )[1] = s.polygon.vertices[i].y;
+				d.vert( i )[2] = s.polygon.normals[i].x;
+				d.vert( i )[3] = s.polygon.normals[i].y;
 			}
 			break;
 		default:
