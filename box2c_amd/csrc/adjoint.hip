@@ -195,6 +195,7 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 	const float inv_h = W.inv_h, h = W.h;
 	const float minBiasVel = -W.pushout;
 	const bool warm = ( W.flags & kWorldWarmStart ) != 0;
+	const Softness soft = world_softness( W );
 	for ( int sub = 0; sub < W.subSteps; ++sub )
 	{
 		for ( int i = tid; i < wi.bodyCount; i += nt )
@@ -226,7 +227,7 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 			{
 				int2 lb = C.bodies( k );
 				tape_appl( C, tape, 3 * sub + 1, k, lb.x, lb.y );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, soft );
 			}
 			__syncthreads();
 		}
@@ -245,7 +246,7 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 			{
 				int2 lb = C.bodies( k );
 				tape_appl( C, tape, 3 * sub + 2, k, lb.x, lb.y );
-				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
+				lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, soft );
 			}
 			__syncthreads();
 		}
@@ -365,10 +366,10 @@ __device__ float lane_solve_adj( const S& C, const AdjScratch& a, int pass, int 
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, k ), ny = C.at( cfNormalY, k );
-	const float* soft = contact_softness( W, la, lb );
-	float biasRate = useBias ? soft[0] : 0.0f;
-	float massScale = useBias ? soft[1] : 1.0f;
-	float impulseScale = useBias ? soft[2] : 0.0f;
+	const Softness sw = world_softness( W );
+	float biasRate = useBias ? contact_soft( sw, la, lb, 0 ) : 0.0f;
+	float massScale = useBias ? contact_soft( sw, la, lb, 1 ) : 1.0f;
+	float impulseScale = useBias ? contact_soft( sw, la, lb, 2 ) : 0.0f;
 	float friction = C.at( cfFriction, k );
 	float tx = ny, ty = 0.0f - nx;
 	float rAx[2], rAy[2], rBx[2], rBy[2];

@@ -185,7 +185,17 @@ struct WorkItem
 	// Level boundaries (relative to slotStart) at ovfLevels[ovfLevelBegin .. + ovfLevelCount].
 	int ovfLevelBegin;
 	int ovfLevelCount;
+	// For each colour group g (1 .. kGroups-1), the next non-empty colour group after it in the
+	// sweep order (cyclically, g itself last), 0 if none: 4 bits at bit 4 (g % 8) of
+	// nextGroup[g / 8]. A sweep loads a lane's slot of that group one phase ahead.
+	int nextGroup[2];
 };
+
+// the next non-empty colour group after g (see WorkItem::nextGroup)
+__host__ __device__ inline int next_group( const WorkItem& wi, int g )
+{
+	return ( wi.nextGroup[g >> 3] >> ( 4 * ( g & 7 ) ) ) & 15;
+}
 
 // Where a work item keeps its solver state during the step
 enum ItemStore : int

@@ -364,19 +364,16 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 	}
 }
 
-// This thread's first slot of the colour group after g (cyclically over the colours 0..10,
+// This thread's first slot of the next non-empty colour group after g (cyclically over the
 // groups 1..kGroups-1), or -1: the colour sweeps visit the same slots in every pass, so a store
 // can load a slot's read-only quads one phase ahead.
 __device__ inline int next_colour_slot( const WorkItem& wi, int g, int tid )
 {
-	for ( int step = 1; step < kGroups; ++step )
-	{
-		int n = 1 + ( g - 1 + step ) % ( kGroups - 1 );
-		int b = wi.groupStart[n], e = wi.groupStart[n + 1];
-		if ( b < e )
-			return b + tid < e ? b + tid : -1;
-	}
-	return -1;
+	int n = next_group( wi, g );
+	if ( n == 0 )
+		return -1;
+	int b = wi.groupStart[n], e = wi.groupStart[n + 1];
+	return b + tid < e ? b + tid : -1;
 }
 
 // All stages of one step for one work item, by one workgroup, with workgroup barriers between
@@ -409,14 +406,15 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	}
 	__syncthreads();
 
-	// read-only quads of this thread's slot in the next colour group, loaded one phase ahead by
-	// stores that read them from HBM (the sweeps visit the same slots in every pass)
+	// read-only quads of this thread's slot in the next colour group (q0 holds its bodies),
+	// loaded one phase ahead, before the barrier: the sweeps visit the same slots in every pass,
+	// so after the barrier a lane starts on its bodies at once
 	float4 pre[cfMutableQuad];
 	int preSlot = -1;
-	auto ahead = [&]( int g ) {
+	// (the slot, nk, is looked up at the start of a phase and its quads loaded at the end)
+	auto ahead = [&]( int nk ) {
 		if constexpr ( S::kPrefetch )
 		{
-			int nk = next_colour_slot( wi, g, tid );
 			if ( nk >= 0 )
 				C.loadQuads( nk, pre );
 			preSlot = nk;
@@ -428,10 +426,22 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	const float inv_h = W.inv_h, h = W.h;
 	const float pushout = W.pushout, minBiasVel = -W.pushout;
 	const bool warm = ( W.flags & kWorldWarmStart ) != 0;
+	const Softness soft = world_softness( W );
+
+	// the velocity terms of this thread's first two bodies stay in registers over the substeps
+	VelocityTerms vt[2] = {};
+#pragma unroll
+	for ( int r = 0; r < 2; ++r )
+		if ( tid + r * nt < wi.bodyCount )
+			vt[r] = velocity_terms( W.sims[bl[tid + r * nt]], W );
 
 	for ( int sub = 0; sub < W.subSteps; ++sub )
 	{
-		for ( int i = tid; i < wi.bodyCount; i += nt )
+#pragma unroll
+		for ( int r = 0; r < 2; ++r )
+			if ( tid + r * nt < wi.bodyCount )
+				integrate_velocity_t( C, tid + r * nt, vt[r], W );
+		for ( int i = tid + 2 * nt; i < wi.bodyCount; i += nt )
 			integrate_velocity( C, i, W.sims[bl[i]], W );
 		__syncthreads();
 
@@ -451,15 +461,21 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 				if ( b == e )
 					continue;
+				const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
 				for ( int k = b + tid; k < e; k += nt )
 				{
-					int2 lb = C.bodies( k );
 					if ( S::kPrefetch && k == preSlot )
+					{
+						int2 lb = quad_bodies( pre[0] );
 						lane_warm_start_q( C, k, lb.x, lb.y, pre );
+					}
 					else
+					{
+						int2 lb = C.bodies( k );
 						lane_warm_start( C, k, lb.x, lb.y );
+					}
 				}
-				ahead( g );
+				ahead( nk );
 				__syncthreads();
 			}
 
@@ -469,7 +485,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, W );
+				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, soft );
 			}
 			__syncthreads();
 		}
@@ -478,15 +494,21 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 			if ( b == e )
 				continue;
+			const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
 			for ( int k = b + tid; k < e; k += nt )
 			{
-				int2 lb = C.bodies( k );
 				if ( S::kPrefetch && k == preSlot )
-					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, true, W );
+				{
+					int2 lb = quad_bodies( pre[0] );
+					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, true, soft );
+				}
 				else
-					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
+				{
+					int2 lb = C.bodies( k );
+					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, soft );
+				}
 			}
-			ahead( g );
+			ahead( nk );
 			__syncthreads();
 		}
 
@@ -500,7 +522,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, W );
+				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, soft );
 			}
 			__syncthreads();
 		}
@@ -509,15 +531,21 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 			if ( b == e )
 				continue;
+			const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
 			for ( int k = b + tid; k < e; k += nt )
 			{
-				int2 lb = C.bodies( k );
 				if ( S::kPrefetch && k == preSlot )
-					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, false, W );
+				{
+					int2 lb = quad_bodies( pre[0] );
+					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, false, soft );
+				}
 				else
-					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
+				{
+					int2 lb = C.bodies( k );
+					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, soft );
+				}
 			}
-			ahead( g );
+			ahead( nk );
 			__syncthreads();
 		}
 	}
@@ -538,15 +566,21 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 		if ( b == e )
 			continue;
+		const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
 		for ( int k = b + tid; k < e; k += nt )
 		{
-			int2 lb = C.bodies( k );
 			if ( S::kPrefetch && k == preSlot )
+			{
+				int2 lb = quad_bodies( pre[0] );
 				lane_restitution_q( C, k, lb.x, lb.y, pre, threshold );
+			}
 			else
+			{
+				int2 lb = C.bodies( k );
 				lane_restitution( C, k, lb.x, lb.y, threshold );
+			}
 		}
-		ahead( g );
+		ahead( nk );
 		__syncthreads();
 	}
 

@@ -67,7 +67,8 @@ __device__ inline void store_velocity( const S& C, int i, const Body& b )
 // so the substep passes (about 3 x colours x substeps dependent phases per step) run at LDS
 // latency. BodiesLdsStore, for items too large for that, keeps the bodies, the slot bodies and
 // the written quads in LDS and reads the read-only quads from HBM, coalesced across lanes. The
-// host picks the store per work item from the LDS budget.
+// host picks the store per work item from the LDS budget. The stores that read the quads from HBM
+// have the colour sweeps load a slot's read-only quads (loadQuads) one phase ahead (kPrefetch).
 struct GlobalStore
 {
 	float4* rec; // record of slot k at rec[(base + k) * cfQuads]
@@ -87,7 +88,7 @@ struct GlobalStore
 	{
 		rec[(size_t)( base + k ) * cfQuads + q] = v;
 	}
-	static constexpr bool kPrefetch = false;
+	static constexpr bool kPrefetch = true;
 	__device__ inline void loadQuads( int k, float4 ( &r )[cfMutableQuad] ) const
 	{
 #pragma unroll
@@ -210,6 +211,12 @@ struct LdsStore
 	}
 };
 
+// the slot bodies a record's q0 carries (int bits in x and y)
+__device__ inline int2 quad_bodies( float4 q0 )
+{
+	return make_int2( __builtin_bit_cast( int, q0.x ), __builtin_bit_cast( int, q0.y ) );
+}
+
 // inverse mass and inertia of local body i (the static body, -1, has none)
 template <class S>
 __device__ inline float2 body_mass( const S& C, int i )
@@ -217,11 +224,32 @@ __device__ inline float2 body_mass( const S& C, int i )
 	return i < 0 ? make_float2( 0.0f, 0.0f ) : C.mass( i );
 }
 
-// soft constraint coefficients of a contact: static ones when either body is static
-// (src/contact_solver.c:650-651)
-__device__ inline const float* contact_softness( const DevWorld& W, int la, int lb )
+// The world's two sets of soft constraint coefficients (biasRate, massScale, impulseScale),
+// read once per work item (uniform loads) so that the per-lane choice below is a register
+// select rather than a load from a lane-dependent address in the middle of every solve.
+struct Softness
 {
-	return ( la < 0 || lb < 0 ) ? W.staticSoft : W.contactSoft;
+	float contact[3];
+	float stat[3];
+};
+
+__device__ inline Softness world_softness( const DevWorld& W )
+{
+	Softness s;
+#pragma unroll
+	for ( int i = 0; i < 3; ++i )
+	{
+		s.contact[i] = W.contactSoft[i];
+		s.stat[i] = W.staticSoft[i];
+	}
+	return s;
+}
+
+// soft constraint coefficient i of a contact: the static set when either body is static
+// (src/contact_solver.c:650-651)
+__device__ inline float contact_soft( const Softness& S, int la, int lb, int i )
+{
+	return ( la < 0 || lb < 0 ) ? S.stat[i] : S.contact[i];
 }
 
 #define PF( name, p ) pointField( cf##name, ( p ) )
@@ -340,7 +368,7 @@ __device__ void lane_warm_start_q( const S& C, int slot, int la, int lb, const f
 // contact_solver.c:865-1075
 template <class S>
 __device__ void lane_solve_q( const S& C, int slot, int la, int lb, const float4 ( &r )[cfMutableQuad], float inv_h, float minBiasVel,
-							  bool useBias, const DevWorld& W )
+							  bool useBias, const Softness& soft )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
@@ -350,10 +378,9 @@ __device__ void lane_solve_q( const S& C, int slot, int la, int lb, const float4
 	const float4 pm[2] = { r[3], r[4] };
 	float4 imp[2] = { C.quad( slot, 5 ), C.quad( slot, 6 ) };
 	float nx = q0.z, ny = q0.w;
-	const float* soft = contact_softness( W, la, lb );
-	float biasRate = useBias ? soft[0] : 0.0f;
-	float massScale = useBias ? soft[1] : 1.0f;
-	float impulseScale = useBias ? soft[2] : 0.0f;
+	float biasRate = useBias ? contact_soft( soft, la, lb, 0 ) : 0.0f;
+	float massScale = useBias ? contact_soft( soft, la, lb, 1 ) : 1.0f;
+	float impulseScale = useBias ? contact_soft( soft, la, lb, 2 ) : 0.0f;
 	float dpx = bB.dpx - bA.dpx, dpy = bB.dpy - bA.dpy;
 	float nImp[2];
 #pragma unroll
@@ -453,11 +480,11 @@ __device__ inline void lane_warm_start( const S& C, int slot, int la, int lb )
 
 template <class S>
 __device__ inline void lane_solve( const S& C, int slot, int la, int lb, float inv_h, float minBiasVel, bool useBias,
-								   const DevWorld& W )
+								   const Softness& soft )
 {
 	float4 r[cfMutableQuad];
 	C.loadQuads( slot, r );
-	lane_solve_q( C, slot, la, lb, r, inv_h, minBiasVel, useBias, W );
+	lane_solve_q( C, slot, la, lb, r, inv_h, minBiasVel, useBias, soft );
 }
 
 template <class S>
@@ -497,7 +524,7 @@ __device__ void ovf_warm_start( const S& C, int slot, int la, int lb, int pointC
 
 template <class S>
 __device__ void ovf_solve( const S& C, int slot, int la, int lb, int pointCount, float inv_h, float pushout,
-						   bool useBias, const DevWorld& W )
+						   bool useBias, const Softness& soft )
 {
 	Body bA = load_body( C, la ), bB = load_body( C, lb );
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
@@ -518,10 +545,9 @@ __device__ void ovf_solve( const S& C, int slot, int la, int lb, int pointCount,
 			velocityBias = s * inv_h;
 		else if ( useBias )
 		{
-			const float* soft = contact_softness( W, la, lb );
-			velocityBias = maxr( soft[0] * s, -pushout );
-			massScale = soft[1];
-			impulseScale = soft[2];
+			velocityBias = maxr( contact_soft( soft, la, lb, 0 ) * s, -pushout );
+			massScale = contact_soft( soft, la, lb, 1 );
+			impulseScale = contact_soft( soft, la, lb, 2 );
 		}
 		float vrAx = bA.vx + ( -bA.w ) * rAy, vrAy = bA.vy + bA.w * rAx;
 		float vrBx = bB.vx + ( -bB.w ) * rBy, vrBy = bB.vy + bB.w * rBx;
@@ -605,24 +631,40 @@ __device__ void ovf_restitution( const S& C, int slot, int la, int lb, int point
 	store_velocity( C, lb, bB );
 }
 
-// solver.c:34-93 for one body of the private set
-template <class S>
-__device__ void integrate_velocity( const S& C, int i, const DevBodySim& sim, const DevWorld& W )
+// The per-body terms of src/solver.c:34-93 that stay the same over a step: the damping factors
+// and the velocity deltas from gravity, force and torque (computed as the reference does, so
+// computing them once per step gives the same floats as once per substep).
+struct VelocityTerms
+{
+	float linearDamping, angularDamping;
+	float dvx, dvy, dw;
+};
+
+__device__ inline VelocityTerms velocity_terms( const DevBodySim& sim, const DevWorld& W )
 {
 	float h = W.h;
-	float maxLinearSpeed = W.maxLinearSpeed, maxAngularSpeed = W.maxAngularSpeed;
-	float4 s0 = C.b0( i );
-	float vx = s0.x, vy = s0.y, w = s0.z;
-	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
-	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
+	VelocityTerms t;
+	t.linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
+	t.angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
 	float mg = sim.mass * sim.gravityScale;
 	float fx = sim.force[0] + mg * W.gravityX, fy = sim.force[1] + mg * W.gravityY;
 	float hm = h * sim.invMass;
-	float lvdx = hm * fx, lvdy = hm * fy;
-	float avd = h * sim.invI * sim.torque;
-	vx = lvdx + linearDamping * vx;
-	vy = lvdy + linearDamping * vy;
-	w = avd + angularDamping * w;
+	t.dvx = hm * fx;
+	t.dvy = hm * fy;
+	t.dw = h * sim.invI * sim.torque;
+	return t;
+}
+
+// solver.c:34-93 for one body of the private set, given its step terms
+template <class S>
+__device__ void integrate_velocity_t( const S& C, int i, const VelocityTerms& t, const DevWorld& W )
+{
+	float maxLinearSpeed = W.maxLinearSpeed, maxAngularSpeed = W.maxAngularSpeed;
+	float4 s0 = C.b0( i );
+	float vx = s0.x, vy = s0.y, w = s0.z;
+	vx = t.dvx + t.linearDamping * vx;
+	vy = t.dvy + t.linearDamping * vy;
+	w = t.dw + t.angularDamping * w;
 	if ( vx * vx + vy * vy > maxLinearSpeed * maxLinearSpeed )
 	{
 		float ratio = maxLinearSpeed / sqrtf( vx * vx + vy * vy );
@@ -635,6 +677,12 @@ __device__ void integrate_velocity( const S& C, int i, const DevBodySim& sim, co
 		w *= ratio;
 	}
 	C.b0( i ) = make_float4( vx, vy, w, s0.w );
+}
+
+template <class S>
+__device__ void integrate_velocity( const S& C, int i, const DevBodySim& sim, const DevWorld& W )
+{
+	integrate_velocity_t( C, i, velocity_terms( sim, W ), W );
 }
 
 // solver.c:146-163

@@ -749,6 +749,17 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 			wi.block *= 2;
 		wi.groupStart[kGroups] = (int)L.slotContact.size() - wi.slotStart;
 		wi.bodyCount = (int)L.bodyList.size() - wi.bodyStart;
+		wi.nextGroup[0] = wi.nextGroup[1] = 0;
+		for ( int g = 1; g < kGroups; ++g )
+			for ( int step = 1; step < kGroups; ++step )
+			{
+				int n = 1 + ( g - 1 + step ) % ( kGroups - 1 );
+				if ( wi.groupStart[n] < wi.groupStart[n + 1] )
+				{
+					wi.nextGroup[g >> 3] |= n << ( 4 * ( g & 7 ) );
+					break;
+				}
+			}
 		{
 			int bytes = itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
 			int bodyBytes = itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] );
@@ -788,6 +799,8 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 					( wi.store == kStoreLds && wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) );
 		for ( int k = 0; k < kGroups; ++k )
 			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
+		for ( int g = 1; g < kGroups; ++g )
+			B2MI_CHECK( 0 <= next_group( wi, g ) && next_group( wi, g ) < kGroups );
 		for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
 		{
 			int2 lb = L.slotBodies[wi.slotStart + k];
