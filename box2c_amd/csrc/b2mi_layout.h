@@ -58,17 +58,16 @@ enum : uint32_t
 	kWorldSleep = 1u << 2,
 };
 
-// same byte layout as b2ManifoldPoint / b2Manifold of the public API
+// A manifold point as the kernels keep it: the reference's b2ManifoldPoint without the world
+// point and the normal velocity, which are output only and live in the contact's second line.
 struct DevManifoldPoint
 {
-	float pointX, pointY;
 	float anchorAX, anchorAY;
 	float anchorBX, anchorBY;
 	float separation;
 	float normalImpulse;
 	float tangentImpulse;
 	float maxNormalImpulse;
-	float normalVelocity;
 	uint16_t id;
 	uint8_t persisted;
 	uint8_t pad;
@@ -81,20 +80,35 @@ struct DevManifold
 	int pointCount;
 };
 
+// One 128-byte line: everything the narrowphase reads and writes and the solver prepares from
+// (manifold, shapes, flags, material, masses), so each of those kernels touches one line per
+// contact. The output-only fields are in DevContactOut.
 struct DevContact
 {
 	DevManifold manifold;
 	int shapeIdA, shapeIdB;
 	uint32_t simFlags;
-	int bodySimIndexA, bodySimIndexB;
-	float invMassA, invIA, invMassB, invIB;
 	float friction, restitution;
+	float invMassA, invIA, invMassB, invIB;
 	// For each new manifold point (2 bits each), the index of the previous manifold's point whose
 	// impulses it inherited as warm start, or 3 for none; the adjoint routes impulse gradients
 	// back through it.
 	uint32_t warmFrom;
-	int pad1;
+	int pad;
 };
+static_assert( sizeof( DevManifoldPoint ) == 36, "manifold point layout" );
+static_assert( sizeof( DevContact ) == 128, "one line per contact" );
+
+// The output-only part of a contact, in an array of its own (32 bytes per contact, so the
+// narrowphase's writes of neighbouring contacts fill whole lines): the world points and normal
+// velocities of the manifold points and the body sim indices.
+struct DevContactOut
+{
+	float pointX[2], pointY[2];
+	float normalVelocity[2];
+	int bodySimIndexA, bodySimIndexB;
+};
+static_assert( sizeof( DevContactOut ) == 32, "contact output layout" );
 
 // Two 128-byte lines: the first holds all a shape of up to four vertices gives the narrowphase
 // (vertices with their edge normals, type, body, count, radius, fat AABB), so a box pair
@@ -154,6 +168,7 @@ struct DevWorld
 	DevShape* shapes;	 // by shape id
 	int* bodyShapes;	 // per awake sim: shape ids, head-first body list order
 	DevContact* contacts; // by contact id
+	DevContactOut* contactOut; // by contact id
 	float gravityX, gravityY;
 	float dt, inv_dt, h, inv_h;
 	float contactSoft[3]; // biasRate, massScale, impulseScale

@@ -59,11 +59,14 @@ __global__ void k_init_contacts( const DevWorld* __restrict__ worlds, const Cont
 	memset( &z, 0, sizeof( z ) );
 	z.shapeIdA = ci.shapeIdA;
 	z.shapeIdB = ci.shapeIdB;
-	z.bodySimIndexA = -1;
-	z.bodySimIndexB = -1;
 	z.friction = ci.friction;
 	z.restitution = ci.restitution;
 	*c = z;
+	DevContactOut zo;
+	memset( &zo, 0, sizeof( zo ) );
+	zo.bodySimIndexA = -1;
+	zo.bodySimIndexB = -1;
+	worlds[ci.world].contactOut[ci.contactId] = zo;
 }
 
 // src/world.c:304-362 + src/contact.c:531-581 for one contact
@@ -106,10 +109,8 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 		bool staticA = ( A.body & kStaticRef ) != 0, staticB = ( B.body & kStaticRef ) != 0;
 		const DevBodySim& sa = staticA ? W.statics[A.body & ~kStaticRef] : W.sims[A.body];
 		const DevBodySim& sb = staticB ? W.statics[B.body & ~kStaticRef] : W.sims[B.body];
-		c->bodySimIndexA = staticA ? -1 : A.body;
 		c->invMassA = sa.invMass;
 		c->invIA = sa.invI;
-		c->bodySimIndexB = staticB ? -1 : B.body;
 		c->invMassB = sb.invMass;
 		c->invIB = sb.invI;
 		b2Transform xfA = sim_transform( sa ), xfB = sim_transform( sb );
@@ -136,12 +137,16 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 		nm.normalX = m.normal.x;
 		nm.normalY = m.normal.y;
 		nm.pointCount = m.pointCount;
+		DevContactOut out;
+		out.bodySimIndexA = staticA ? -1 : A.body;
+		out.bodySimIndexB = staticB ? -1 : B.body;
 #pragma unroll
 		for ( int k = 0; k < 2; ++k )
 		{
 			DevManifoldPoint& p = nm.points[k];
-			p.pointX = m.point[k].x;
-			p.pointY = m.point[k].y;
+			out.pointX[k] = m.point[k].x;
+			out.pointY[k] = m.point[k].y;
+			out.normalVelocity[k] = 0.0f;
 			p.anchorAX = m.anchorA[k].x;
 			p.anchorAY = m.anchorA[k].y;
 			p.anchorBX = m.anchorB[k].x;
@@ -159,7 +164,6 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 				p.normalImpulse = 0.0f;
 				p.tangentImpulse = 0.0f;
 				p.maxNormalImpulse = 0.0f;
-				p.normalVelocity = 0.0f;
 				p.persisted = 0;
 				bool found = false;
 #pragma unroll
@@ -178,6 +182,7 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 		}
 		c->manifold = nm;
 		c->warmFrom = warmFrom;
+		W.contactOut[item.y] = out;
 		if ( touching )
 			simFlags |= kSimTouching;
 		else
@@ -588,6 +593,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	for ( int k = tid; k < total; k += nt )
 	{
 		DevManifold& m = W.contacts[sc[k]].manifold;
+		DevContactOut& out = W.contactOut[sc[k]];
 		int n = k < ovfEnd ? m.pointCount : 2;
 #pragma unroll
 		for ( int j = 0; j < 2; ++j )
@@ -598,7 +604,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			m.points[j].normalImpulse = imp.x;
 			m.points[j].tangentImpulse = imp.y;
 			m.points[j].maxNormalImpulse = imp.z;
-			m.points[j].normalVelocity = imp.w;
+			out.normalVelocity[j] = imp.w;
 		}
 	}
 

@@ -1017,6 +1017,7 @@ static void fillDevWorld( DevWorld& d, World* w, float timeStep, int subStepCoun
 	d.shapes = w->dShapes;
 	d.bodyShapes = w->dBodyShapes;
 	d.contacts = w->dContacts;
+	d.contactOut = w->dContactOut;
 	d.gravityX = w->gravity.x;
 	d.gravityY = w->gravity.y;
 	// src/world.c:717-746
@@ -1048,14 +1049,20 @@ static void ensureContactCapacity( World* w )
 	while ( cap < needed )
 		cap *= 2;
 	DevContact* p = nullptr;
+	DevContactOut* q = nullptr;
 	HIP_CHECK( hipMalloc( (void**)&p, sizeof( DevContact ) * (size_t)cap ) );
+	HIP_CHECK( hipMalloc( (void**)&q, sizeof( DevContactOut ) * (size_t)cap ) );
 	if ( w->dContacts && w->capContacts > 0 )
 	{
 		HIP_CHECK( hipMemcpyAsync( p, w->dContacts, sizeof( DevContact ) * (size_t)w->capContacts, hipMemcpyDeviceToDevice, stream() ) );
+		HIP_CHECK( hipMemcpyAsync( q, w->dContactOut, sizeof( DevContactOut ) * (size_t)w->capContacts, hipMemcpyDeviceToDevice,
+								   stream() ) );
 		HIP_CHECK( hipStreamSynchronize( stream() ) );
 		HIP_CHECK( hipFree( w->dContacts ) );
+		HIP_CHECK( hipFree( w->dContactOut ) );
 	}
 	w->dContacts = p;
+	w->dContactOut = q;
 	w->capContacts = cap;
 	if ( w->group )
 		w->group->structureDirty = true;
@@ -1891,7 +1898,7 @@ B2_API int b2Shape_GetContactData( b2ShapeId shapeId, b2ContactData* contactData
 			const Shape& b = w->shapes[c.shapeIdB];
 			contactData[index].shapeIdA = b2ShapeId{ a.id + 1, shapeId.world0, a.revision };
 			contactData[index].shapeIdB = b2ShapeId{ b.id + 1, shapeId.world0, b.revision };
-			memcpy( &contactData[index].manifold, &w->devContacts[c.contactId].manifold, sizeof( b2Manifold ) );
+			contactData[index].manifold = publicManifold( w->devContacts[c.contactId], w->devContactOut[c.contactId] );
 			index += 1;
 		}
 		key = c.edges[key & 1].nextKey;
@@ -1942,7 +1949,7 @@ B2_API int b2mi_World_DumpContacts( b2WorldId worldId, b2mi_ContactRecord* recor
 			r.simFlags = d.simFlags;
 			r.friction = d.friction;
 			r.restitution = d.restitution;
-			memcpy( &r.manifold, &d.manifold, sizeof( b2Manifold ) );
+			r.manifold = publicManifold( d, w->devContactOut[id] );
 		}
 		n += 1;
 	};

@@ -130,6 +130,7 @@ World::~World()
 	devFree( dShapes );
 	devFree( dBodyShapes );
 	devFree( dContacts );
+	devFree( dContactOut );
 }
 
 StepGroup::~StepGroup()
@@ -240,9 +241,12 @@ static void downloadContacts( World* w )
 		return;
 	int n = w->contactIdPool.nextIndex;
 	w->devContacts.resize( n );
+	w->devContactOut.resize( n );
 	if ( n && w->dContacts )
 	{
 		HIP_CHECK( hipMemcpyAsync( w->devContacts.data(), w->dContacts, sizeof( DevContact ) * n, hipMemcpyDeviceToHost, stream() ) );
+		HIP_CHECK(
+			hipMemcpyAsync( w->devContactOut.data(), w->dContactOut, sizeof( DevContactOut ) * n, hipMemcpyDeviceToHost, stream() ) );
 		HIP_CHECK( hipStreamSynchronize( stream() ) );
 	}
 	w->contactsDownloaded = true;
@@ -1110,7 +1114,7 @@ B2_API int b2Body_GetContactData( b2BodyId bodyId, b2ContactData* contactData, i
 			const Shape& b = w->shapes[c.shapeIdB];
 			contactData[index].shapeIdA = b2ShapeId{ a.id + 1, bodyId.world0, a.revision };
 			contactData[index].shapeIdB = b2ShapeId{ b.id + 1, bodyId.world0, b.revision };
-			memcpy( &contactData[index].manifold, &w->devContacts[c.contactId].manifold, sizeof( b2Manifold ) );
+			contactData[index].manifold = publicManifold( w->devContacts[c.contactId], w->devContactOut[c.contactId] );
 			index += 1;
 		}
 		key = c.edges[key & 1].nextKey;

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <unordered_set>
@@ -199,6 +200,7 @@ struct World
 	std::vector<float> awakeState; // 8 floats per awake body
 	std::vector<DevShape> devShapes;
 	std::vector<DevContact> devContacts;
+	std::vector<DevContactOut> devContactOut;
 	bool mirrorValid = true; // and mirrorEpoch is the group's step epoch
 	uint64_t mirrorEpoch = 0, contactsEpoch = 0;
 	bool bodiesDirty = true; // host mirror of bodies/shapes must be uploaded
@@ -241,6 +243,7 @@ struct World
 	DevShape* dShapes = nullptr;
 	int* dBodyShapes = nullptr;
 	DevContact* dContacts = nullptr;
+	DevContactOut* dContactOut = nullptr;
 	int capAwake = 0, capStatic = 0, capShapes = 0, capContacts = 0;
 
 	StepGroup* group = nullptr;
@@ -412,5 +415,31 @@ hipStream_t stream();
 hipStream_t auxStream( int i ); // extra streams (default priority) for concurrent launches
 World* getWorld( b2WorldId id );
 World* worldAt( int slot );
+
+// the public b2Manifold of a device contact (its manifold plus the output-only world points and
+// normal velocities)
+inline b2Manifold publicManifold( const DevContact& d, const DevContactOut& e )
+{
+	b2Manifold m;
+	memset( &m, 0, sizeof( m ) );
+	for ( int j = 0; j < 2; ++j )
+	{
+		const DevManifoldPoint& p = d.manifold.points[j];
+		b2ManifoldPoint& o = m.points[j];
+		o.point = b2Vec2{ e.pointX[j], e.pointY[j] };
+		o.anchorA = b2Vec2{ p.anchorAX, p.anchorAY };
+		o.anchorB = b2Vec2{ p.anchorBX, p.anchorBY };
+		o.separation = p.separation;
+		o.normalImpulse = p.normalImpulse;
+		o.tangentImpulse = p.tangentImpulse;
+		o.maxNormalImpulse = p.maxNormalImpulse;
+		o.normalVelocity = e.normalVelocity[j];
+		o.id = p.id;
+		o.persisted = p.persisted != 0;
+	}
+	m.normal = b2Vec2{ d.manifold.normalX, d.manifold.normalY };
+	m.pointCount = d.manifold.pointCount;
+	return m;
+}
 
 } // namespace b2mi
