@@ -188,7 +188,7 @@ __device__ void replay_item( const S& C, const T& tape, const DevWorld& W, const
 	for ( int k = tid; k < total; k += nt )
 	{
 		int2 lb = C.bodies( k );
-		prepare_slot( C, k, snap + k, lb.x, lb.y, W );
+		prepare_slot( C, k, load_prep( snap + k ), lb.x, lb.y, W );
 	}
 	__syncthreads();
 

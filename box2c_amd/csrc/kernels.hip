@@ -403,11 +403,21 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	}
 	__syncthreads();
 
-	// prepare (all groups; the overflow group uses the same per-point formulas)
-	for ( int k = tid; k < total; k += nt )
+	// prepare (all groups; the overflow group uses the same per-point formulas), with the contact
+	// of a thread's next slot loaded while it prepares the current one
 	{
-		int2 lb = C.bodies( k );
-		prepare_slot( C, k, W.contacts + sc[k], lb.x, lb.y, W );
+		PrepInput cur;
+		if ( tid < total )
+			cur = load_prep( W.contacts + sc[tid] );
+		for ( int k = tid; k < total; k += nt )
+		{
+			PrepInput next;
+			if ( k + nt < total )
+				next = load_prep( W.contacts + sc[k + nt] );
+			int2 lb = C.bodies( k );
+			prepare_slot( C, k, cur, lb.x, lb.y, W );
+			cur = next;
+		}
 	}
 	__syncthreads();
 

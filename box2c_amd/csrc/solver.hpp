@@ -254,11 +254,29 @@ __device__ inline float contact_soft( const Softness& S, int la, int lb, int i )
 
 #define PF( name, p ) pointField( cf##name, ( p ) )
 
+// What prepare reads of a contact: its manifold (both point slots; the unused one is ignored)
+// and material. Loaded apart from the arithmetic so that a lane can fetch its next slot's
+// contact while it prepares the current one.
+struct PrepInput
+{
+	DevManifold m;
+	float friction, restitution;
+};
+
+__device__ inline PrepInput load_prep( const DevContact* __restrict__ contact )
+{
+	PrepInput in;
+	in.m = contact->manifold;
+	in.friction = contact->friction;
+	in.restitution = contact->restitution;
+	return in;
+}
+
 // contact_solver.c:597-808 (lanes) and 21-147 (overflow): one constraint slot
 template <class S>
-__device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict__ contact, int la, int lb, const DevWorld& W )
+__device__ void prepare_slot( const S& C, int slot, const PrepInput& in, int la, int lb, const DevWorld& W )
 {
-	const DevManifold& m = contact->manifold;
+	const DevManifold& m = in.m;
 	// the bodies' inverse masses (equal to the ones the narrowphase copied into the contact)
 	float2 MA = body_mass( C, la ), MB = body_mass( C, lb );
 	float mA = MA.x, iA = MA.y, mB = MB.x, iB = MB.y;
@@ -313,8 +331,8 @@ __device__ void prepare_slot( const S& C, int slot, const DevContact* __restrict
 			imp[j] = make_float4( 0.0f, 0.0f, 0.0f, 0.0f );
 		}
 	}
-	pm[0].w = contact->friction;
-	pm[1].w = contact->restitution;
+	pm[0].w = in.friction;
+	pm[1].w = in.restitution;
 	C.setQuad( slot, 1, anc[0] );
 	C.setQuad( slot, 2, anc[1] );
 	C.setQuad( slot, 3, pm[0] );
