@@ -307,9 +307,10 @@ struct StepPose
 // Bytes of LDS a work item's solver state needs: per body two float4 of state and a float2 of
 // inverse mass and inertia, per slot its constraint record (which holds its two local bodies).
 // Items above the budget keep the read-only constraint fields in HBM, or everything. A CDNA4 CU has
-// 160 KiB of LDS; the default budget leaves room for two or more items per CU.
-constexpr int kMaxItemLdsBytes = 160 * 1024;
-constexpr int kDefaultItemLdsBytes = 160 * 1024;
+// 160 KiB of LDS, of which the solver kernels' own words (the restitution vote) take the last
+// 256 bytes; the default budget leaves room for two or more items per CU.
+constexpr int kMaxItemLdsBytes = 160 * 1024 - 256;
+constexpr int kDefaultItemLdsBytes = kMaxItemLdsBytes;
 // threads per solver workgroup: at most this many (the step kernel's launch bound; the replay
 // and adjoint kernels, which hold more state per thread, take at most kAdjointMaxBlock)
 constexpr int kSolveMaxBlock = 1024;

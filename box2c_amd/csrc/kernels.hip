@@ -392,6 +392,11 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	const int tid = threadIdx.x, nt = blockDim.x;
 	const float4* gstate = reinterpret_cast<const float4*>( W.state );
 	const int total = wi.groupStart[kGroups];
+	// the workgroup's vote on skipping the colour restitution sweeps (set here, visible after the
+	// staging barrier; cleared by any thread that finds a slot or body that needs the sweeps)
+	__shared__ int restitutionVote;
+	if ( tid == 0 )
+		restitutionVote = 1;
 
 	// stage private body state and masses
 	for ( int i = tid; i < wi.bodyCount; i += nt )
@@ -576,7 +581,18 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		}
 		__syncthreads();
 	}
-	for ( int g = 1; g < kGroups; ++g )
+	// The colour sweeps would return their input bit for bit when every colour slot's stage is
+	// the identity and every body is plain (restitution_is_identity): then they are skipped.
+	bool identity = W.subSteps > 0;
+	for ( int k = ovfEnd + tid; k < total && identity; k += nt )
+		identity = slot_restitution_is_identity( C, k, threshold );
+	for ( int i = tid; i < wi.bodyCount && identity; i += nt )
+		identity = body_is_plain( C, i );
+	if ( !identity )
+		restitutionVote = 0; // every writer writes the same value
+	__syncthreads();
+	identity = restitutionVote != 0;
+	for ( int g = 1; g < kGroups && !identity; ++g )
 	{
 		int b = wi.groupStart[g], e = wi.groupStart[g + 1];
 		if ( b == e )

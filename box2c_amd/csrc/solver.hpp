@@ -383,6 +383,39 @@ __device__ void lane_warm_start_q( const S& C, int slot, int la, int lb, const f
 	store_velocity( C, lb, bB );
 }
 
+// Whether the colour restitution stage (lane_restitution_q) would leave a slot and its bodies bit
+// for bit as they are, whatever its bodies' velocities, given that they are plain
+// (body_is_plain); from the slot's read-only quads r and its written quads imp (slot_restitution_
+// is_identity reads them from a store). Neither point may
+// have restitution mass (the test of src/contact_solver.c:1096-1101: relative velocity above
+// -threshold, or no normal impulse) and the inputs must be finite and bounded: then the new
+// normal impulse equals the old, the applied impulse is a signed zero, and v - m (+-0) returns v
+// for every velocity component that is not -0. The bound (1e15) keeps every product and sum of
+// the stage finite.
+__device__ inline bool restitution_is_identity( const float4 ( &r )[cfMutableQuad], const float4 ( &imp )[2], float threshold )
+{
+	const float kBound = 1e15f;
+	bool ok = fabsf( r[0].z ) <= kBound && fabsf( r[0].w ) <= kBound && fabsf( r[4].w ) <= kBound;
+#pragma unroll
+	for ( int j = 0; j < 2; ++j )
+	{
+		const float4 a = r[1 + j];
+		bool noMass = ( imp[j].w + threshold ) > 0.0f || imp[j].z == 0.0f;
+		ok = ok && noMass && fabsf( imp[j].x ) <= kBound && fabsf( imp[j].w ) <= kBound;
+		ok = ok && fabsf( a.x ) <= kBound && fabsf( a.y ) <= kBound && fabsf( a.z ) <= kBound && fabsf( a.w ) <= kBound;
+	}
+	return ok;
+}
+
+template <class S>
+__device__ inline bool slot_restitution_is_identity( const S& C, int k, float threshold )
+{
+	float4 r[cfMutableQuad];
+	C.loadQuads( k, r );
+	const float4 imp[2] = { C.quad( k, 5 ), C.quad( k, 6 ) };
+	return restitution_is_identity( r, imp, threshold );
+}
+
 // contact_solver.c:865-1075
 template <class S>
 __device__ void lane_solve_q( const S& C, int slot, int la, int lb, const float4 ( &r )[cfMutableQuad], float inv_h, float minBiasVel,
@@ -485,6 +518,18 @@ __device__ void lane_restitution_q( const S& C, int slot, int la, int lb, const 
 	C.setQuad( slot, 6, imp[1] );
 	store_velocity( C, la, bA );
 	store_velocity( C, lb, bB );
+}
+
+// A body whose velocity no signed-zero impulse changes: finite, bounded components, none of
+// them -0, and bounded inverse mass and inertia.
+template <class S>
+__device__ inline bool body_is_plain( const S& C, int i )
+{
+	const float kBound = 1e15f;
+	float4 s0 = C.b0( i );
+	float2 m = C.mass( i );
+	auto plain = [&]( float x ) { return fabsf( x ) <= kBound && __builtin_bit_cast( uint32_t, x ) != 0x80000000u; };
+	return plain( s0.x ) && plain( s0.y ) && plain( s0.z ) && fabsf( m.x ) <= kBound && fabsf( m.y ) <= kBound;
 }
 
 // the lane stages with the slot's read-only quads loaded from its store

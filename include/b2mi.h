@@ -72,8 +72,9 @@ B2_API void b2mi_Batch_Configure( int batch, int blockSize, int targetSlots );
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots );
 B2_API int b2mi_Batch_GetWorkItemCount( int batch );
 
-// Largest LDS footprint (bytes, at most 160 KiB) of a work item whose solver state is kept in
-// LDS; larger work items keep it in HBM. -1 = the 160 KiB default, 0 = HBM for every work item.
+// Largest LDS footprint (bytes, at most 160 KiB less the 256 bytes the solver kernels keep for
+// themselves; larger values are clamped) of a work item whose solver state is kept in LDS;
+// larger work items keep it in HBM. -1 = that maximum, 0 = HBM for every work item.
 // The results are bit-identical either way; only the speed differs.
 B2_API void b2mi_Batch_SetLdsBudget( int batch, int maxBytes );
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes );
