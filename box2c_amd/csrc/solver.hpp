@@ -115,7 +115,10 @@ struct GlobalStore
 
 struct BodiesLdsStore
 {
-	float4* rec; // HBM: the read-only quads of slot k at rec[(base + k) * cfQuads]
+	// HBM: the read-only quads of slot k at rec[base * cfQuads + k * cfMutableQuad], an 80-byte
+	// stride inside the item's own record region (which starts at quad base * cfQuads, as an HBM
+	// store item's does, so items of different stores never share records)
+	float4* rec;
 	int base;
 	// the colour sweeps load a slot's read-only quads one phase ahead (loadQuads)
 	static constexpr bool kPrefetch = true;
@@ -127,17 +130,17 @@ struct BodiesLdsStore
 	{
 		if ( field >= 4 * cfMutableQuad )
 			return reinterpret_cast<float*>( mut + 2 * k )[field - 4 * cfMutableQuad];
-		return reinterpret_cast<float*>( rec + (size_t)( base + k ) * cfQuads )[field];
+		return reinterpret_cast<float*>( rec + (size_t)base * cfQuads + (size_t)k * cfMutableQuad )[field];
 	}
 	__device__ inline float4 quad( int k, int q ) const
 	{
 		if ( q >= cfMutableQuad )
 			return mut[2 * k + q - cfMutableQuad];
-		return rec[(size_t)( base + k ) * cfQuads + q];
+		return rec[(size_t)base * cfQuads + (size_t)k * cfMutableQuad + q];
 	}
 	__device__ inline void loadQuads( int k, float4 ( &r )[cfMutableQuad] ) const
 	{
-		const float4* p = rec + (size_t)( base + k ) * cfQuads;
+		const float4* p = rec + (size_t)base * cfQuads + (size_t)k * cfMutableQuad;
 #pragma unroll
 		for ( int q = 0; q < cfMutableQuad; ++q )
 			r[q] = p[q];
@@ -147,7 +150,7 @@ struct BodiesLdsStore
 		if ( q >= cfMutableQuad )
 			mut[2 * k + q - cfMutableQuad] = v;
 		else
-			rec[(size_t)( base + k ) * cfQuads + q] = v;
+			rec[(size_t)base * cfQuads + (size_t)k * cfMutableQuad + q] = v;
 	}
 	__device__ inline int2 bodies( int k ) const
 	{
