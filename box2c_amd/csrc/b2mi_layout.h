@@ -157,6 +157,17 @@ struct DevBodySim
 };
 static_assert( sizeof( DevBodySim ) == 128, "one cache line" );
 
+// The part of a body sim the narrowphase reads (the first 32 bytes of DevBodySim), kept again in
+// a compact array of its own so that the narrowphase's gathers touch 32 bytes per body rather
+// than a 128-byte line; written where the sims are (upload, finalize).
+struct DevPose
+{
+	float p[2], q[2];
+	float localCenter[2];
+	float invMass, invI;
+};
+static_assert( sizeof( DevPose ) == 32, "pose layout" );
+
 // Body state (the reference's 32-byte b2BodyState): two float4 per body
 //   s0 = (v.x, v.y, w, flags), s1 = (dp.x, dp.y, dq.c, dq.s)
 
@@ -165,6 +176,8 @@ struct DevWorld
 	float* state;		 // [2 * 4 * awakeCapacity]
 	DevBodySim* sims;	 // awake set
 	DevBodySim* statics; // static set
+	DevPose* poses;		 // awake set, the narrowphase's view of sims
+	DevPose* staticPoses; // static set, the narrowphase's view of statics
 	DevShape* shapes;	 // by shape id
 	int* bodyShapes;	 // per awake sim: shape ids, head-first body list order
 	DevContact* contacts; // by contact id

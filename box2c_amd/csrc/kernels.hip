@@ -27,7 +27,7 @@ namespace b2mi
 {
 
 
-__device__ inline b2Transform sim_transform( const DevBodySim& s )
+__device__ inline b2Transform sim_transform( const DevPose& s )
 {
 	b2Transform xf;
 	xf.p = v2( s.p[0], s.p[1] );
@@ -107,8 +107,8 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 	{
 		bool wasTouching = ( simFlags & kSimTouching ) != 0;
 		bool staticA = ( A.body & kStaticRef ) != 0, staticB = ( B.body & kStaticRef ) != 0;
-		const DevBodySim& sa = staticA ? W.statics[A.body & ~kStaticRef] : W.sims[A.body];
-		const DevBodySim& sb = staticB ? W.statics[B.body & ~kStaticRef] : W.sims[B.body];
+		const DevPose sa = staticA ? W.staticPoses[A.body & ~kStaticRef] : W.poses[A.body];
+		const DevPose sb = staticB ? W.staticPoses[B.body & ~kStaticRef] : W.poses[B.body];
 		c->invMassA = sa.invMass;
 		c->invIA = sa.invI;
 		c->invMassB = sb.invMass;
@@ -282,6 +282,16 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 	sim.q[1] = q.s;
 	sim.p[0] = p.x;
 	sim.p[1] = p.y;
+	DevPose pose;
+	pose.p[0] = p.x;
+	pose.p[1] = p.y;
+	pose.q[0] = q.c;
+	pose.q[1] = q.s;
+	pose.localCenter[0] = localCenter.x;
+	pose.localCenter[1] = localCenter.y;
+	pose.invMass = sim.invMass;
+	pose.invI = sim.invI;
+	W.poses[simIndex] = pose;
 	sim.force[0] = 0.0f;
 	sim.force[1] = 0.0f;
 	sim.torque = 0.0f;

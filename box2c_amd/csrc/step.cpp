@@ -1014,6 +1014,8 @@ static void fillDevWorld( DevWorld& d, World* w, float timeStep, int subStepCoun
 	d.state = w->dState;
 	d.sims = w->dSims;
 	d.statics = w->dStatics;
+	d.poses = w->dPoses;
+	d.staticPoses = w->dStaticPoses;
 	d.shapes = w->dShapes;
 	d.bodyShapes = w->dBodyShapes;
 	d.contacts = w->dContacts;

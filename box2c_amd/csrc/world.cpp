@@ -126,6 +126,8 @@ World::~World()
 {
 	devFree( dState );
 	devFree( dSims );
+	devFree( dPoses );
+	devFree( dStaticPoses );
 	devFree( dStatics );
 	devFree( dShapes );
 	devFree( dBodyShapes );
@@ -264,15 +266,19 @@ static void uploadBodies( World* w )
 	{
 		devFree( w->dState );
 		devFree( w->dSims );
+		devFree( w->dPoses );
 		w->capAwake = grow( na );
 		HIP_CHECK( hipMalloc( (void**)&w->dState, sizeof( float ) * 8 * (size_t)w->capAwake ) );
 		HIP_CHECK( hipMalloc( (void**)&w->dSims, sizeof( DevBodySim ) * (size_t)w->capAwake ) );
+		HIP_CHECK( hipMalloc( (void**)&w->dPoses, sizeof( DevPose ) * (size_t)w->capAwake ) );
 	}
 	if ( w->capStatic < nst || w->dStatics == nullptr )
 	{
 		devFree( w->dStatics );
+		devFree( w->dStaticPoses );
 		w->capStatic = grow( nst );
 		HIP_CHECK( hipMalloc( (void**)&w->dStatics, sizeof( DevBodySim ) * (size_t)w->capStatic ) );
+		HIP_CHECK( hipMalloc( (void**)&w->dStaticPoses, sizeof( DevPose ) * (size_t)w->capStatic ) );
 	}
 	if ( w->capShapes < ns || w->dShapes == nullptr )
 	{
@@ -304,13 +310,22 @@ static void uploadBodies( World* w )
 		const Body& b = w->bodies[sh.bodyId];
 		d.body = b.setIndex == 0 ? ( b.localIndex | kStaticRef ) : b.localIndex;
 	}
+	std::vector<DevPose> poses( na ), staticPoses( nst );
+	for ( int i = 0; i < na; ++i )
+		poses[i] = poseOf( w->awakeSims[i] );
+	for ( int i = 0; i < nst; ++i )
+		staticPoses[i] = poseOf( w->staticSims[i] );
 	if ( na )
 	{
 		HIP_CHECK( hipMemcpyAsync( w->dState, w->awakeState.data(), sizeof( float ) * 8 * na, hipMemcpyHostToDevice, s ) );
 		HIP_CHECK( hipMemcpyAsync( w->dSims, w->awakeSims.data(), sizeof( DevBodySim ) * na, hipMemcpyHostToDevice, s ) );
+		HIP_CHECK( hipMemcpyAsync( w->dPoses, poses.data(), sizeof( DevPose ) * na, hipMemcpyHostToDevice, s ) );
 	}
 	if ( nst )
+	{
 		HIP_CHECK( hipMemcpyAsync( w->dStatics, w->staticSims.data(), sizeof( DevBodySim ) * nst, hipMemcpyHostToDevice, s ) );
+		HIP_CHECK( hipMemcpyAsync( w->dStaticPoses, staticPoses.data(), sizeof( DevPose ) * nst, hipMemcpyHostToDevice, s ) );
+	}
 	if ( ns )
 		HIP_CHECK( hipMemcpyAsync( w->dShapes, w->devShapes.data(), sizeof( DevShape ) * ns, hipMemcpyHostToDevice, s ) );
 	if ( !bodyShapes.empty() )

@@ -240,6 +240,8 @@ struct World
 	float* dState = nullptr;
 	DevBodySim* dSims = nullptr;
 	DevBodySim* dStatics = nullptr;
+	DevPose* dPoses = nullptr;		 // capAwake
+	DevPose* dStaticPoses = nullptr; // capStatic
 	DevShape* dShapes = nullptr;
 	int* dBodyShapes = nullptr;
 	DevContact* dContacts = nullptr;
@@ -440,6 +442,14 @@ inline b2Manifold publicManifold( const DevContact& d, const DevContactOut& e )
 	m.normal = b2Vec2{ d.manifold.normalX, d.manifold.normalY };
 	m.pointCount = d.manifold.pointCount;
 	return m;
+}
+
+// the narrowphase's view of a body sim
+inline DevPose poseOf( const DevBodySim& s )
+{
+	DevPose p;
+	memcpy( &p, &s, sizeof( DevPose ) );
+	return p;
 }
 
 } // namespace b2mi
