@@ -625,11 +625,16 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		__syncthreads();
 	}
 
-	// store impulses (contact_solver.c:442-471 overflow: pointCount points; 1180-1293 lanes: both)
+	// store impulses (contact_solver.c:442-471 overflow: pointCount points; 1180-1293 lanes: both);
+	// a lane loads its next slot's contact id while it stores the current one
+	int contactNext = tid < total ? sc[tid] : 0;
 	for ( int k = tid; k < total; k += nt )
 	{
-		DevManifold& m = W.contacts[sc[k]].manifold;
-		DevContactOut& out = W.contactOut[sc[k]];
+		const int contact = contactNext;
+		if ( k + nt < total )
+			contactNext = sc[k + nt];
+		DevManifold& m = W.contacts[contact].manifold;
+		DevContactOut& out = W.contactOut[contact];
 		int n = k < ovfEnd ? m.pointCount : 2;
 #pragma unroll
 		for ( int j = 0; j < 2; ++j )
@@ -646,9 +651,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 
 	// finalize owned bodies and write their state back (velocity kept, deltas reset)
 	float4* gs = reinterpret_cast<float4*>( W.state );
+	int bodyNext = tid < wi.ownedCount ? bl[tid] : 0;
 	for ( int i = tid; i < wi.ownedCount; i += nt )
 	{
-		int b = bl[i];
+		const int b = bodyNext;
+		if ( i + nt < wi.ownedCount )
+			bodyNext = bl[i + nt];
 		float4 s0 = C.b0( i ), s1 = C.b1( i );
 		finalize_body( W, b, s0, s1, wi.world, enl, enlCount, enlCapacity );
 		gs[2 * b] = s0;
