@@ -154,6 +154,30 @@ def test_batch_large_lds_budget_parity():
             cpu.destroy(wc)
 
 
+def test_batch_mixed_stores_parity():
+    """A 96 KiB budget puts the largest worlds of this batch in the HBM store and the next ones in
+    the bodies-in-LDS store, launched side by side every step: each item's constraint records
+    must stay inside its own region of the shared record array."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 10, 5)
+    try:
+        gpu.lib.b2mi_Batch_SetLdsBudget(batch, 96 * 1024)
+        gpu.batch_step(batch)
+        n = gpu.lib.b2mi_Batch_GetSolverLaunches(batch, None, 0)
+        launches = np.zeros((n, 4), np.int32)
+        gpu.lib.b2mi_Batch_GetSolverLaunches(batch, launches.ctypes.data, n)
+        assert 0 in launches[:, 3] and (launches[:, 3] > 0).any(), launches
+        for s in range(30):
+            gpu.batch_step(batch)
+        for wc in worlds_c:
+            cpu.step(wc, n=31)
+        _compare_worlds(gpu, cpu, worlds_g, worlds_c, "mixed stores")
+    finally:
+        gpu.lib.b2mi_DestroyBatch(batch)
+        for wc in worlds_c:
+            cpu.destroy(wc)
+
+
 def test_batch_autotune_keeps_parity():
     """The autotuner's trial steps are ordinary steps: after tuning (60 candidate shapes, one
     rebuild step and one measured step each) and more steps under the chosen shape, every world
