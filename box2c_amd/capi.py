@@ -400,6 +400,8 @@ class B2Api:
         L.b2mi_Batch_GetSolverLaunches.restype = c_int
         L.b2mi_Batch_GetHostTimeline.argtypes = [c_int, ctypes.c_void_p]
         L.b2mi_Batch_SetWorldChunks.argtypes = [c_int, c_int]
+        L.b2mi_Batch_SetSplitSlots.argtypes = [c_int, c_int]
+        L.b2mi_World_SetSplitSlots.argtypes = [WorldId, c_int]
         L.b2mi_Batch_GetWorkItemCount.argtypes = [c_int]
         L.b2mi_Batch_GetWorkItemCount.restype = c_int
         L.b2mi_Batch_GetTiming.argtypes = [c_int, POINTER(c_float), POINTER(c_float), POINTER(c_float)]

@@ -225,6 +225,29 @@ __host__ __device__ inline int next_group( const WorkItem& wi, int g )
 	return ( wi.nextGroup[g >> 3] >> ( 4 * ( g & 7 ) ) ) & 15;
 }
 
+// The phases of a very large HBM-store work item when it runs as one grid launch per phase
+// (k_split_phase) rather than one workgroup: solve_item's stages, in its order.
+enum SplitPhase : int
+{
+	kSplitStage,
+	kSplitPrepare,
+	kSplitIntegrateVelocities,
+	kSplitOverflowWarmStart,
+	kSplitWarmStart,
+	kSplitOverflowSolve,
+	kSplitSolve,
+	kSplitIntegratePositions,
+	kSplitOverflowRelax,
+	kSplitRelax,
+	kSplitOverflowRestitution,
+	kSplitRestitution,
+	kSplitStoreImpulses,
+	kSplitFinalize,
+	kSplitPhaseCount
+};
+// default slot count from which an HBM-store work item runs phase by phase over the grid
+constexpr int kDefaultSplitSlots = 16384;
+
 // Where a work item keeps its solver state during the step
 enum ItemStore : int
 {

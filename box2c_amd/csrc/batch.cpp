@@ -217,6 +217,26 @@ B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSl
 	setShape( g, blockSize, targetSlots, g->ldsBudget );
 }
 
+// HBM-store work items of at least `slots` constraint slots run as one grid launch per solver
+// phase instead of in one workgroup (-1 = the default, 0 = never)
+static void setSplitSlots( StepGroup* g, int slots )
+{
+	g->splitSlots = slots;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+	g->allActive = true;
+}
+
+B2_API void b2mi_Batch_SetSplitSlots( int batch, int slots )
+{
+	setSplitSlots( getBatch( batch ), slots );
+}
+
+B2_API void b2mi_World_SetSplitSlots( b2WorldId worldId, int slots )
+{
+	setSplitSlots( worldGroup( worldId ), slots );
+}
+
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes )
 {
 	StepGroup* g = worldGroup( worldId );

@@ -728,6 +728,131 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 	}
 }
 
+
+// One phase of one very large work item (HBM store), spread over a grid: thread i takes body or
+// slot `first + i` of the phase's range. The host launches the phases in solve_item's order, one
+// kernel each, so every workgroup barrier of solve_item becomes a kernel boundary; each phase
+// applies the same per-body and per-slot functions to the same data, so the results are the
+// same bits (a colour's slots share no body, so their order does not matter; overflow levels
+// as in solve_item).
+__global__ void __launch_bounds__( 256 ) k_split_phase( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+														int itemIndex, int phase, int first, int count,
+														const int* __restrict__ bodyList, const int* __restrict__ slotContact,
+														const int2* __restrict__ slotBodies, float* cfBase, float4* priv, float2* mass,
+														EnlargeRecord* enl, int* enlCount, int enlCapacity )
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= count )
+		return;
+	const WorkItem& wi = items[itemIndex];
+	const DevWorld& W = worlds[wi.world];
+	GlobalStore C;
+	C.rec = reinterpret_cast<float4*>( cfBase );
+	C.base = wi.slotStart;
+	C.sb = slotBodies + wi.slotStart;
+	C.P = priv + 2 * (size_t)wi.bodyStart;
+	C.M = mass + wi.bodyStart;
+	const int* bl = bodyList + wi.bodyStart;
+	const int* sc = slotContact + wi.slotStart;
+	const int k = first + i;
+	const Softness soft = world_softness( W );
+	const float minBiasVel = -W.pushout;
+	switch ( phase )
+	{
+		case kSplitStage:
+		{
+			const float4* gstate = reinterpret_cast<const float4*>( W.state );
+			int b = bl[k];
+			C.b0( k ) = gstate[2 * b];
+			C.b1( k ) = gstate[2 * b + 1];
+			C.mass( k ) = make_float2( W.sims[b].invMass, W.sims[b].invI );
+			break;
+		}
+		case kSplitPrepare:
+		{
+			int2 lb = C.bodies( k );
+			prepare_slot( C, k, load_prep( W.contacts + sc[k] ), lb.x, lb.y, W );
+			break;
+		}
+		case kSplitIntegrateVelocities:
+			integrate_velocity( C, k, W.sims[bl[k]], W );
+			break;
+		case kSplitIntegratePositions:
+			integrate_position( C, k, W.h );
+			break;
+		case kSplitOverflowWarmStart:
+		{
+			int2 lb = C.bodies( k );
+			ovf_warm_start( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount );
+			break;
+		}
+		case kSplitWarmStart:
+		{
+			int2 lb = C.bodies( k );
+			lane_warm_start( C, k, lb.x, lb.y );
+			break;
+		}
+		case kSplitOverflowSolve:
+		case kSplitOverflowRelax:
+		{
+			int2 lb = C.bodies( k );
+			ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, W.inv_h, W.pushout, phase == kSplitOverflowSolve,
+					   soft );
+			break;
+		}
+		case kSplitSolve:
+		case kSplitRelax:
+		{
+			int2 lb = C.bodies( k );
+			lane_solve( C, k, lb.x, lb.y, W.inv_h, minBiasVel, phase == kSplitSolve, soft );
+			break;
+		}
+		case kSplitOverflowRestitution:
+		{
+			int2 lb = C.bodies( k );
+			ovf_restitution( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, W.restitutionThreshold );
+			break;
+		}
+		case kSplitRestitution:
+		{
+			int2 lb = C.bodies( k );
+			lane_restitution( C, k, lb.x, lb.y, W.restitutionThreshold );
+			break;
+		}
+		case kSplitStoreImpulses:
+		{
+			const int contact = sc[k];
+			DevManifold& m = W.contacts[contact].manifold;
+			DevContactOut& out = W.contactOut[contact];
+			int n = k < wi.groupStart[1] ? m.pointCount : 2;
+#pragma unroll
+			for ( int j = 0; j < 2; ++j )
+			{
+				if ( j >= n )
+					break;
+				float4 imp = C.quad( k, cfMutableQuad + j );
+				m.points[j].normalImpulse = imp.x;
+				m.points[j].tangentImpulse = imp.y;
+				m.points[j].maxNormalImpulse = imp.z;
+				out.normalVelocity[j] = imp.w;
+			}
+			break;
+		}
+		case kSplitFinalize:
+		{
+			float4* gs = reinterpret_cast<float4*>( W.state );
+			int b = bl[k];
+			float4 s0 = C.b0( k ), s1 = C.b1( k );
+			finalize_body( W, b, s0, s1, wi.world, enl, enlCount, enlCapacity );
+			gs[2 * b] = s0;
+			gs[2 * b + 1] = make_float4( 0.0f, 0.0f, 1.0f, 0.0f );
+			break;
+		}
+		default:
+			break;
+	}
+}
+
 } // namespace b2mi
 
 // ---- host launchers ----
@@ -785,6 +910,21 @@ hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* item
 	else
 		B2MI_LAUNCH_SOLVE( kStoreHbm );
 #undef B2MI_LAUNCH_SOLVE
+	return hipGetLastError();
+}
+
+
+hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, int first, int count,
+							   const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, float4* priv,
+							   float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
+{
+	if ( count <= 0 )
+		return hipSuccess;
+	if ( phase < 0 || phase >= kSplitPhaseCount || first < 0 )
+		return hipErrorInvalidValue;
+	int block = 256, grid = ( count + block - 1 ) / block;
+	hipLaunchKernelGGL( k_split_phase, dim3( grid ), dim3( block ), 0, stream, worlds, items, itemIndex, phase, first, count, bodyList,
+						slotContact, slotBodies, cf, priv, mass, enl, enlCount, enlCapacity );
 	return hipGetLastError();
 }
 

@@ -17,6 +17,11 @@ hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* item
 						 float* cf, int cfCapacity, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
 						 const int* abortFlag, hipStream_t stream );
 
+// one phase of a very large work item over a grid (k_split_phase; phases in SplitPhase)
+hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, int first, int count,
+							   const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, float4* priv,
+							   float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
+
 // replay and adjoint (adjoint.hip)
 int adjoint_scratch_floats( int bodyCount, int slotCount, int subSteps );
 hipError_t launch_snapshot( const DevWorld* worlds, const WorkItem* items, int itemCount, const int* slotContact, DevContact* snap,

@@ -965,9 +965,19 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 			return ca > cb;
 		} );
 		g->classes.clear();
+		g->splitItems.clear();
+		// very large HBM-store items run phase by phase over the whole grid (launchSplit) rather
+		// than in one workgroup; not in taped runs or chunked groups
+		const int splitSlots = g->splitSlots < 0 ? kDefaultSplitSlots : g->splitSlots;
+		const bool maySplit = splitSlots > 0 && g->tape == nullptr && chunks == 1;
 		for ( int i = 0; i < (int)g->hItems.size(); ++i )
 		{
 			const WorkItem& wi = g->hItems[i];
+			if ( maySplit && wi.store == kStoreHbm && wi.groupStart[kGroups] >= splitSlots )
+			{
+				g->splitItems.push_back( i );
+				continue;
+			}
 			int chunk = chunkOf( wi );
 			if ( g->classes.empty() || g->classes.back().block != wi.block || g->classes.back().chunk != chunk ||
 				 g->classes.back().store != wi.store )
@@ -1329,6 +1339,9 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
 								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
 	};
+	B2MI_CHECK( splitItems.empty() || abortFlag == nullptr );
+	for ( int item : splitItems )
+		launchSplit( item, s );
 	if ( classes.size() <= 1 )
 	{
 		if ( !classes.empty() )
@@ -1357,6 +1370,47 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		HIP_CHECK( hipEventRecord( joinEvents[c - 1], auxStream( c - 1 ) ) );
 		HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
 	}
+}
+
+// The phases of one very large work item (solve_item's order), each one grid launch on stream s.
+void StepGroup::launchSplit( int item, hipStream_t s )
+{
+	const WorkItem& wi = hItems[item];
+	const DevWorld& hw = hWorlds[wi.world];
+	auto run = [&]( int phase, int first, int count ) {
+		HIP_CHECK( launch_split_phase( dWorlds, dItems, item, phase, first, count, dBodyList, dSlotContact, dSlotBodies, dCons, dPriv,
+									   dMass, dEnlarge, dCounters + 1, capEnlarge, s ) );
+	};
+	const int total = wi.groupStart[kGroups];
+	const int levelCount = wi.groupStart[1] > wi.groupStart[0] ? wi.ovfLevelCount : 0;
+	const int* levels = hOvfLevels.data() + wi.ovfLevelBegin;
+	B2MI_CHECK( levelCount == 0 || wi.ovfLevelBegin + levelCount < (int)hOvfLevels.size() );
+	auto overflow = [&]( int phase ) {
+		for ( int L = 0; L < levelCount; ++L )
+			run( phase, levels[L], levels[L + 1] - levels[L] );
+	};
+	auto colours = [&]( int phase ) {
+		for ( int g = 1; g < kGroups; ++g )
+			run( phase, wi.groupStart[g], wi.groupStart[g + 1] - wi.groupStart[g] );
+	};
+	run( kSplitStage, 0, wi.bodyCount );
+	run( kSplitPrepare, 0, total );
+	for ( int sub = 0; sub < hw.subSteps; ++sub )
+	{
+		run( kSplitIntegrateVelocities, 0, wi.bodyCount );
+		overflow( kSplitOverflowWarmStart );
+		if ( hw.flags & kWorldWarmStart )
+			colours( kSplitWarmStart );
+		overflow( kSplitOverflowSolve );
+		colours( kSplitSolve );
+		run( kSplitIntegratePositions, 0, wi.bodyCount );
+		overflow( kSplitOverflowRelax );
+		colours( kSplitRelax );
+	}
+	overflow( kSplitOverflowRestitution );
+	colours( kSplitRestitution );
+	run( kSplitStoreImpulses, 0, total );
+	run( kSplitFinalize, 0, wi.ownedCount );
 }
 
 // The solver launches of one world chunk, its size classes side by side: class j on extra
@@ -1540,7 +1594,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	// then needs no host round trip. Otherwise the launch returns at once and the solver runs
 	// after the state changes below. Groups whose contacts change at most steps (one large
 	// world) stop speculating after a miss and try again every 16th step.
-	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && ( specMisses == 0 || epoch % 16 == 0 );
+	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && splitItems.empty() &&
+						   ( specMisses == 0 || epoch % 16 == 0 );
 	// World chunks: consecutive worlds, so each chunk's narrowphase list is one range and its
 	// state changes go to that range of the change buffer with a counter of their own. A
 	// chunk's speculative solver launch follows its narrowphase on the chunk's stream and is

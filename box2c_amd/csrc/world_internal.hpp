@@ -297,6 +297,7 @@ struct StepGroup
 	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kDefaultItemLdsBytes, 0 = HBM only)
 	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
 	int ldsFullBudget = -1;	  // max LDS bytes of an item keeping all its state in LDS (-1 = ldsBudget)
+	int splitSlots = -1;	  // HBM-store items of at least this many slots run phase by phase (-1 = kDefaultSplitSlots, 0 = never)
 	int builtTarget = -1, builtBudget = -1, builtFullBudget = -1;				   // packing target and LDS budget of the current items
 	bool structureDirty = true;
 
@@ -353,6 +354,8 @@ struct StepGroup
 	float msCollide = 0, msSolve = 0, msHost = 0;
 	hipEvent_t events[4] = { nullptr, nullptr, nullptr, nullptr };
 	std::vector<LaunchClass> classes;
+	std::vector<int> splitItems; // items (indices into hItems) that run as one grid launch per phase
+	void launchSplit( int item, hipStream_t s );
 	// host clock (ms since the step began) at: pair update, collide-list build, collide launch,
 	// narrowphase wait, wait returned, change processing done, solver launch, solver wait, wait
 	// returned, enlarge processing done

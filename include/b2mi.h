@@ -90,6 +90,14 @@ B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes );
 // Results are bit-identical for every count.
 B2_API void b2mi_Batch_SetWorldChunks( int batch, int chunks );
 
+// A work item whose solver state is kept in HBM and that has at least `slots` constraint slots
+// (one very large island, such as a 40,000-shape pile) runs as one grid-wide launch per solver
+// phase (stage, prepare, each substep's integrations and colour and overflow-level passes,
+// restitution, store, finalize) instead of in a single workgroup; results are the same bits.
+// -1 = the default (16384 slots), 0 = never. Not in taped runs or chunked batches.
+B2_API void b2mi_Batch_SetSplitSlots( int batch, int slots );
+B2_API void b2mi_World_SetSplitSlots( b2WorldId worldId, int slots );
+
 // Autotuner: steps the batch (or world) under each candidate launch shape (threads per
 // workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
 // rebuild step, and keeps the shape with the fastest solver kernel. Every shape gives

@@ -116,6 +116,31 @@ def test_step_parity_solver_storage(name, kw, overrides, steps, substeps, budget
         cpu.destroy(wc)
 
 
+@pytest.mark.parametrize("name", ["hello", "large_pyramid", "tumbler", "shape_zoo", "drop_boxes", "overflow",
+                                  "overflow_negzero"])
+def test_step_parity_split_items(name):
+    """Every work item kept in HBM and run phase by phase over the grid (split threshold of one
+    slot): the per-phase launches give the oracle's bits, overflow levels and restitution included."""
+    scene = {s[0]: s for s in helpers.PARITY_SCENES}[name]
+    _, kw, overrides, steps, substeps = scene
+    gpu, cpu = helpers.product(), helpers.oracle()
+    wg, _ = helpers.make_world(gpu, name, kw, overrides)
+    wc, _ = helpers.make_world(cpu, name, kw, overrides)
+    gpu.lib.b2mi_World_SetLdsBudget(wg, 0)
+    gpu.lib.b2mi_World_SetSplitSlots(wg, 1)
+    try:
+        for s in range(min(steps, 40)):
+            gpu.step(wg, substeps=substeps)
+            cpu.step(wc, substeps=substeps)
+        m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+        assert m is None, f"{name}: contacts differ: {m}"
+        m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+        assert m is None, f"{name}: bodies differ: {m}"
+    finally:
+        gpu.destroy(wg)
+        cpu.destroy(wc)
+
+
 def _batch_worlds(gpu, cpu, n, seed):
     batch = gpu.create_batch()
     gpu.use_batch(batch)
