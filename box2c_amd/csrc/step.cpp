@@ -551,24 +551,47 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 		w->topologyDirty = false;
 		return;
 	}
-	std::vector<int> parent( na );
-	for ( int i = 0; i < na; ++i )
-		parent[i] = i;
-	auto simOfBody = [&]( int bodyId ) -> int {
+	// One pass over the touching contacts (colour lists, overflow last) gathers what the rest
+	// needs of each: its id and its two bodies as awake-set references (>= 0: dynamic body, its
+	// sim index; <= -2: kinematic body, -2 - its sim index; -1: static).
+	auto ref = [&]( int bodyId ) -> int {
 		const Body& b = w->bodies[bodyId];
-		return b.setIndex == 2 && b.type == b2_dynamicBody ? b.localIndex : -1;
+		if ( b.setIndex != 2 )
+			return -1;
+		return b.type == b2_dynamicBody ? b.localIndex : -2 - b.localIndex;
 	};
+	struct Src
+	{
+		int id, a, b;
+	};
+	std::vector<Src> src;
+	int colorStart[kColorCount + 1];
+	{
+		size_t n = 0;
+		for ( int c = 0; c < kColorCount; ++c )
+			n += w->colors[c].size();
+		src.reserve( n );
+	}
 	for ( int c = 0; c < kColorCount; ++c )
+	{
+		colorStart[c] = (int)src.size();
 		for ( int id : w->colors[c] )
 		{
 			const Contact& ct = w->contacts[id];
-			int a = simOfBody( ct.edges[0].bodyId ), b = simOfBody( ct.edges[1].bodyId );
-			if ( a >= 0 && b >= 0 )
-			{
-				int ra = findRoot( parent, a ), rb = findRoot( parent, b );
-				if ( ra != rb )
-					parent[std::max( ra, rb )] = std::min( ra, rb );
-			}
+			src.push_back( Src{ id, ref( ct.edges[0].bodyId ), ref( ct.edges[1].bodyId ) } );
+		}
+	}
+	colorStart[kColorCount] = (int)src.size();
+
+	std::vector<int> parent( na );
+	for ( int i = 0; i < na; ++i )
+		parent[i] = i;
+	for ( const Src& e : src )
+		if ( e.a >= 0 && e.b >= 0 )
+		{
+			int ra = findRoot( parent, e.a ), rb = findRoot( parent, e.b );
+			if ( ra != rb )
+				parent[std::max( ra, rb )] = std::min( ra, rb );
 		}
 	// components in order of their smallest sim index; kinematic bodies alone
 	std::vector<int> compOf( na, -1 );
@@ -584,24 +607,38 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 		compOf[i] = compOf[r];
 		comps[compOf[i]].bodies.push_back( i );
 	}
-	// a contact belongs to the component of its dynamic body
-	auto contactComp = [&]( const Contact& ct ) {
-		int a = simOfBody( ct.edges[0].bodyId );
-		if ( a < 0 )
-			a = simOfBody( ct.edges[1].bodyId );
-		return compOf[a];
-	};
-	std::vector<std::vector<int>> compSlots[kColorCount];
-	for ( int c = 0; c < kColorCount; ++c )
-		compSlots[c].assign( comps.size(), std::vector<int>() );
-	for ( int c = 0; c < kColorCount; ++c )
-		for ( int id : w->colors[c] )
+	// A contact belongs to the component of its dynamic body. Per colour, the contacts are
+	// ordered by component (a stable counting sort, so each component keeps the colour list's
+	// order): component k's contacts of colour c are bySlot[compStart[c][k] .. compStart[c][k+1]).
+	const int nc = (int)comps.size();
+	std::vector<int> compStart( (size_t)kColorCount * ( nc + 1 ), 0 );
+	std::vector<int> bySlot( src.size() );
+	{
+		std::vector<int> compOfSrc( src.size() );
+		for ( size_t q = 0; q < src.size(); ++q )
 		{
-			int k = contactComp( w->contacts[id] );
-			compSlots[c][k].push_back( id );
-			comps[k].slotCount += 1;
+			int a = src[q].a >= 0 ? src[q].a : src[q].b;
+			B2MI_CHECK( a >= 0 );
+			compOfSrc[q] = compOf[a];
 		}
+		for ( int c = 0; c < kColorCount; ++c )
+		{
+			int* cs = compStart.data() + (size_t)c * ( nc + 1 );
+			for ( int q = colorStart[c]; q < colorStart[c + 1]; ++q )
+				cs[compOfSrc[q] + 1] += 1;
+			for ( int k = 0; k < nc; ++k )
+			{
+				comps[k].slotCount += cs[k + 1];
+				cs[k + 1] += cs[k];
+			}
+			std::vector<int> fill( cs, cs + nc );
+			for ( int q = colorStart[c]; q < colorStart[c + 1]; ++q )
+				bySlot[colorStart[c] + fill[compOfSrc[q]]++] = q;
+		}
+	}
 
+	// item-local position of each owned sim, set per item and cleared after it
+	std::vector<int> posOfSim( na, -1 );
 	// greedy packing in component order
 	size_t ci = 0;
 	while ( ci < comps.size() )
@@ -618,48 +655,31 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 		WorkItem wi{};
 		wi.world = w->groupIndex;
 		wi.bodyStart = (int)L.bodyList.size();
-		std::vector<int> localOf( 0 );
 		// owned bodies
-		std::vector<int> items;
 		for ( size_t k = ci; k < cj; ++k )
 			for ( int b : comps[k].bodies )
-				L.bodyList.push_back( b );
-		wi.ownedCount = (int)L.bodyList.size() - wi.bodyStart;
-		// local index map (owned, then kinematic shadows on demand)
-		std::vector<std::pair<int, int>> shadowMap;
-		auto localIndex = [&]( int bodyId ) -> int {
-			const Body& b = w->bodies[bodyId];
-			if ( b.setIndex != 2 )
-				return -1;
-			if ( b.type == b2_dynamicBody )
 			{
-				// position within the owned range: search (components are small) via map
-				return -2 - b.localIndex; // resolved below
+				posOfSim[b] = (int)L.bodyList.size() - wi.bodyStart;
+				L.bodyList.push_back( b );
 			}
+		wi.ownedCount = (int)L.bodyList.size() - wi.bodyStart;
+		// local index of a body reference: owned bodies by position, kinematic bodies as read-only
+		// shadows appended on first use, static bodies -1
+		std::vector<std::pair<int, int>> shadowMap;
+		auto localIndex = [&]( int r ) -> int {
+			if ( r >= 0 )
+				return posOfSim[r];
+			if ( r == -1 )
+				return -1;
+			int sim = -2 - r;
 			for ( auto& pr : shadowMap )
-				if ( pr.first == b.localIndex )
+				if ( pr.first == sim )
 					return pr.second;
 			int li = (int)L.bodyList.size() - wi.bodyStart;
-			L.bodyList.push_back( b.localIndex );
-			shadowMap.push_back( { b.localIndex, li } );
+			L.bodyList.push_back( sim );
+			shadowMap.push_back( { sim, li } );
 			return li;
 		};
-		std::vector<int> ownedPos;
-		{
-			// sim index -> owned position for this work item
-			int n = wi.ownedCount;
-			ownedPos.clear();
-			for ( int k = 0; k < n; ++k )
-				ownedPos.push_back( L.bodyList[wi.bodyStart + k] );
-		}
-		std::vector<int> posOfSim;
-		int maxSim = 0;
-		for ( int s : ownedPos )
-			maxSim = std::max( maxSim, s );
-		posOfSim.assign( maxSim + 1, -1 );
-		for ( int k = 0; k < (int)ownedPos.size(); ++k )
-			posOfSim[ownedPos[k]] = k;
-		auto resolve = [&]( int li ) { return li <= -2 ? posOfSim[-2 - li] : li; };
 
 		wi.slotStart = (int)L.slotContact.size();
 		int itemMaxGroup = 0;
@@ -668,19 +688,23 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 		{
 			int color = gidx == 0 ? kOverflow : gidx - 1;
 			wi.groupStart[gidx] = (int)L.slotContact.size() - wi.slotStart;
+			const int* cs = compStart.data() + (size_t)color * ( nc + 1 );
 			for ( size_t k = ci; k < cj; ++k )
-				for ( int id : compSlots[color][k] )
+				for ( int p = cs[k]; p < cs[k + 1]; ++p )
 				{
-					const Contact& ct = w->contacts[id];
-					int la = resolve( localIndex( ct.edges[0].bodyId ) );
-					int lb = resolve( localIndex( ct.edges[1].bodyId ) );
-					L.slotContact.push_back( id );
+					const Src& e = src[bySlot[colorStart[color] + p]];
+					int la = localIndex( e.a );
+					int lb = localIndex( e.b );
+					L.slotContact.push_back( e.id );
 					L.slotBodies.push_back( make_int2( la, lb ) );
 				}
 			int size = (int)L.slotContact.size() - wi.slotStart - wi.groupStart[gidx];
 			L.maxGroupSize = std::max( L.maxGroupSize, size );
 			itemMaxGroup = std::max( itemMaxGroup, size );
 		}
+		for ( size_t k = ci; k < cj; ++k )
+			for ( int b : comps[k].bodies )
+				posOfSim[b] = -1;
 		// Overflow levels (group 0, src/contact_solver.c:212-345 runs it serially): a slot's level
 		// is one past the last level that changed one of its bodies, and the slots are ordered by
 		// level, keeping their order within a level. Static bodies are never changed; nor are
