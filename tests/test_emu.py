@@ -32,6 +32,8 @@ CASES = [
     "tests/test_gpu_parity.py::test_step_parity_solver_storage[hbm-tumbler]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow_negzero]",
+    "tests/test_gpu_parity.py::test_step_parity_split_items[tumbler]",
+    "tests/test_gpu_parity.py::test_step_parity_split_items[overflow]",
 ]
 
 
