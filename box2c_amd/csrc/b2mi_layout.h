@@ -351,14 +351,15 @@ constexpr int kDefaultItemLdsBytes = kMaxItemLdsBytes;
 // and adjoint kernels, which hold more state per thread, take at most kAdjointMaxBlock)
 constexpr int kSolveMaxBlock = 1024;
 constexpr int kAdjointMaxBlock = 256;
+// (the LDS stores hold one more body, the static body's identity state, at index bodyCount)
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {
-	return 40 * bodyCount + 4 * cfFloatCount * slotCount;
+	return 40 * ( bodyCount + 1 ) + 4 * cfFloatCount * slotCount;
 }
 // the bodies-in-LDS store: body state and masses, the written quads and the slot bodies
 inline int itemBodyLdsBytes( int bodyCount, int slotCount )
 {
-	return 40 * bodyCount + 8 * slotCount + 4 * cfMutableCount * slotCount;
+	return 40 * ( bodyCount + 1 ) + 8 * slotCount + 4 * cfMutableCount * slotCount;
 }
 
 } // namespace b2mi

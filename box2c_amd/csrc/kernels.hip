@@ -408,7 +408,14 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	if ( tid == 0 )
 		restitutionVote = 1;
 
-	// stage private body state and masses
+	// stage private body state and masses (and the static body's identity state)
+	if constexpr ( S::kIdentityBody )
+		if ( tid == 0 )
+		{
+			C.b0( C.identity ) = make_float4( 0.0f, 0.0f, 0.0f, 0.0f );
+			C.b1( C.identity ) = make_float4( 0.0f, 0.0f, 1.0f, 0.0f );
+			C.mass( C.identity ) = make_float2( 0.0f, 0.0f );
+		}
 	for ( int i = tid; i < wi.bodyCount; i += nt )
 	{
 		int b = bl[i];
@@ -691,7 +698,8 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		// LDS (float4 arrays first, 16-byte aligned): body state, constraint records, masses
 		LdsStore C;
 		C.P = lds;
-		C.rec = lds + 2 * wi.bodyCount;
+		C.identity = wi.bodyCount;
+		C.rec = lds + 2 * ( wi.bodyCount + 1 );
 		C.M = reinterpret_cast<float2*>( C.rec + cfQuads * total );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 		{
@@ -709,9 +717,10 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.rec = reinterpret_cast<float4*>( cfBase );
 		C.base = wi.slotStart;
 		C.P = lds;
-		C.mut = lds + 2 * wi.bodyCount;
+		C.identity = wi.bodyCount;
+		C.mut = lds + 2 * ( wi.bodyCount + 1 );
 		C.M = reinterpret_cast<float2*>( C.mut + 2 * total );
-		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount );
+		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount + 1 );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
 		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );

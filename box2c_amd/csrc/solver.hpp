@@ -24,6 +24,20 @@ template <class S>
 __device__ inline Body load_body( const S& C, int i )
 {
 	Body b;
+	if constexpr ( S::kIdentityBody )
+	{
+		// the static body's identity state is stored at index C.identity: no branch
+		float4 s0 = C.b0( i < 0 ? C.identity : i ), s1 = C.b1( i < 0 ? C.identity : i );
+		b.vx = s0.x;
+		b.vy = s0.y;
+		b.w = s0.z;
+		b.flags = s0.w;
+		b.dpx = s1.x;
+		b.dpy = s1.y;
+		b.dqc = s1.z;
+		b.dqs = s1.w;
+		return b;
+	}
 	if ( i < 0 )
 	{
 		// identity state of a static body (src/body.h:90)
@@ -76,6 +90,7 @@ struct GlobalStore
 	const int2* sb;
 	float4* P;
 	float2* M;
+	static constexpr bool kIdentityBody = false;
 	__device__ inline float& at( int field, int k ) const
 	{
 		return reinterpret_cast<float*>( rec + (size_t)( base + k ) * cfQuads )[field];
@@ -126,6 +141,9 @@ struct BodiesLdsStore
 	int2* sb;
 	float4* P;
 	float2* M;
+	// the static body's identity state (v = 0, dp = 0, dq = (1, 0), no mass) at this index
+	static constexpr bool kIdentityBody = true;
+	int identity;
 	__device__ inline float& at( int field, int k ) const
 	{
 		if ( field >= 4 * cfMutableQuad )
@@ -176,6 +194,9 @@ struct LdsStore
 	float4* rec; // record of slot k at rec[k * cfQuads]
 	float4* P;
 	float2* M;
+	// the static body's identity state (v = 0, dp = 0, dq = (1, 0), no mass) at this index
+	static constexpr bool kIdentityBody = true;
+	int identity;
 	__device__ inline float& at( int field, int k ) const
 	{
 		return reinterpret_cast<float*>( rec + k * cfQuads )[field];
@@ -224,6 +245,8 @@ __device__ inline int2 quad_bodies( float4 q0 )
 template <class S>
 __device__ inline float2 body_mass( const S& C, int i )
 {
+	if constexpr ( S::kIdentityBody )
+		return C.mass( i < 0 ? C.identity : i );
 	return i < 0 ? make_float2( 0.0f, 0.0f ) : C.mass( i );
 }
 
