@@ -535,7 +535,10 @@ static int findRoot( std::vector<int>& parent, int x )
 // `target` constraint slots, each item's slots grouped as overflow first, then colours 0..10.
 // Writes w->solver with indices local to its lists; runs on any host thread (touches this
 // world only).
-static void buildWorldItems( World* w, int target, int budget, int fullBudget )
+// `parallel`: the stages run over index ranges on the host pool (the caller is not a pool task:
+// the pool does not nest); each range writes only its own outputs, so the lists are the same
+// either way.
+static void buildWorldItems( World* w, int target, int budget, int fullBudget, bool parallel )
 {
 	SolverLists& L = w->solver;
 	L.items.clear();
@@ -551,37 +554,56 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 		w->topologyDirty = false;
 		return;
 	}
-	// One pass over the touching contacts (colour lists, overflow last) gathers what the rest
-	// needs of each: its id and its two bodies as awake-set references (>= 0: dynamic body, its
-	// sim index; <= -2: kinematic body, -2 - its sim index; -1: static).
-	auto ref = [&]( int bodyId ) -> int {
-		const Body& b = w->bodies[bodyId];
-		if ( b.setIndex != 2 )
-			return -1;
-		return b.type == b2_dynamicBody ? b.localIndex : -2 - b.localIndex;
+	const int parts = parallel ? 64 : 1;
+	auto forRanges = [&]( int n, const std::function<void( int, int )>& fn ) {
+		auto part = [&]( int p ) { fn( (int)( (int64_t)n * p / parts ), (int)( (int64_t)n * ( p + 1 ) / parts ) ); };
+		if ( parallel )
+			hostPoolRun( parts, part );
+		else
+			part( 0 );
 	};
+	// What the rest needs of each touching contact, in colour-list order (overflow last): its id
+	// and its two bodies as awake-set references (>= 0: dynamic body, its sim index; <= -2:
+	// kinematic body, -2 - its sim index; -1: static). Gathered by passes over the body and
+	// contact arrays (a contact's colour and position in that colour's list are stored in it),
+	// not by following the colour lists into the contact array.
+	const int bodyCount = (int)w->bodies.size();
+	std::vector<int> refOf( bodyCount );
+	forRanges( bodyCount, [&]( int lo, int hi ) {
+		for ( int i = lo; i < hi; ++i )
+		{
+			const Body& b = w->bodies[i];
+			refOf[i] = b.setIndex != 2 ? -1 : b.type == b2_dynamicBody ? b.localIndex : -2 - b.localIndex;
+		}
+	} );
 	struct Src
 	{
 		int id, a, b;
 	};
-	std::vector<Src> src;
 	int colorStart[kColorCount + 1];
-	{
-		size_t n = 0;
-		for ( int c = 0; c < kColorCount; ++c )
-			n += w->colors[c].size();
-		src.reserve( n );
-	}
+	colorStart[0] = 0;
 	for ( int c = 0; c < kColorCount; ++c )
+		colorStart[c + 1] = colorStart[c] + (int)w->colors[c].size();
+	std::vector<Src> src( colorStart[kColorCount] );
 	{
-		colorStart[c] = (int)src.size();
-		for ( int id : w->colors[c] )
-		{
-			const Contact& ct = w->contacts[id];
-			src.push_back( Src{ id, ref( ct.edges[0].bodyId ), ref( ct.edges[1].bodyId ) } );
-		}
+		std::atomic<int> placed{ 0 };
+		forRanges( (int)w->contacts.size(), [&]( int lo, int hi ) {
+			int n = 0;
+			for ( int cid = lo; cid < hi; ++cid )
+			{
+				const Contact& ct = w->contacts[cid];
+				if ( ct.contactId < 0 || ct.colorIndex < 0 )
+					continue;
+				const int c = ct.colorIndex, li = ct.localIndex;
+				B2MI_CHECK( c < kColorCount && 0 <= li && li < (int)w->colors[c].size() && w->colors[c][li] == cid );
+				src[colorStart[c] + li] = Src{ cid, refOf[ct.edges[0].bodyId], refOf[ct.edges[1].bodyId] };
+				n += 1;
+			}
+			placed.fetch_add( n );
+		} );
+		// each list position names exactly one contact, so all of them were written
+		B2MI_CHECK( placed.load() == colorStart[kColorCount] );
 	}
-	colorStart[kColorCount] = (int)src.size();
 
 	std::vector<int> parent( na );
 	for ( int i = 0; i < na; ++i )
@@ -615,54 +637,79 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 	std::vector<int> bySlot( src.size() );
 	{
 		std::vector<int> compOfSrc( src.size() );
-		for ( size_t q = 0; q < src.size(); ++q )
-		{
-			int a = src[q].a >= 0 ? src[q].a : src[q].b;
-			B2MI_CHECK( a >= 0 );
-			compOfSrc[q] = compOf[a];
-		}
-		for ( int c = 0; c < kColorCount; ++c )
-		{
+		forRanges( (int)src.size(), [&]( int lo, int hi ) {
+			for ( int q = lo; q < hi; ++q )
+			{
+				int a = src[q].a >= 0 ? src[q].a : src[q].b;
+				B2MI_CHECK( a >= 0 );
+				compOfSrc[q] = compOf[a];
+			}
+		} );
+		auto sortColor = [&]( int c ) {
 			int* cs = compStart.data() + (size_t)c * ( nc + 1 );
 			for ( int q = colorStart[c]; q < colorStart[c + 1]; ++q )
 				cs[compOfSrc[q] + 1] += 1;
 			for ( int k = 0; k < nc; ++k )
-			{
-				comps[k].slotCount += cs[k + 1];
 				cs[k + 1] += cs[k];
-			}
 			std::vector<int> fill( cs, cs + nc );
 			for ( int q = colorStart[c]; q < colorStart[c + 1]; ++q )
 				bySlot[colorStart[c] + fill[compOfSrc[q]]++] = q;
+		};
+		if ( parallel )
+			hostPoolRun( kColorCount, sortColor );
+		else
+			for ( int c = 0; c < kColorCount; ++c )
+				sortColor( c );
+		for ( int c = 0; c < kColorCount; ++c )
+		{
+			const int* cs = compStart.data() + (size_t)c * ( nc + 1 );
+			for ( int k = 0; k < nc; ++k )
+				comps[k].slotCount += cs[k + 1] - cs[k];
 		}
 	}
 
-	// item-local position of each owned sim, set per item and cleared after it
-	std::vector<int> posOfSim( na, -1 );
-	// greedy packing in component order
-	size_t ci = 0;
-	while ( ci < comps.size() )
+	// greedy packing in component order: item i holds components [span[i].x, span[i].y)
+	std::vector<int2> spans;
+	for ( int ci = 0; ci < nc; )
 	{
-		size_t cj = ci;
+		int cj = ci;
 		int slots = 0, bodies = 0;
 		do
 		{
 			slots += comps[cj].slotCount;
 			bodies += (int)comps[cj].bodies.size();
 			cj += 1;
-		} while ( cj < comps.size() && slots + comps[cj].slotCount <= target && bodies + (int)comps[cj].bodies.size() <= 4 * target );
-
-		WorkItem wi{};
+		} while ( cj < nc && slots + comps[cj].slotCount <= target && bodies + (int)comps[cj].bodies.size() <= 4 * target );
+		spans.push_back( make_int2( ci, cj ) );
+		ci = cj;
+	}
+	// Each item is built into lists of its own (body, slot and level indices relative to the
+	// item), then the lists are laid end to end. posOfSim is the item-local position of each
+	// owned sim: items own disjoint bodies, and each clears its entries when done.
+	struct Built
+	{
+		WorkItem wi;
+		std::vector<int> bodies, slots, levels;
+		std::vector<int2> slotBodies;
+		int maxGroup = 0;
+	};
+	const int ni = (int)spans.size();
+	std::vector<Built> built( ni );
+	std::vector<int> posOfSim( na, -1 );
+	auto buildItem = [&]( int item ) {
+		Built& out = built[item];
+		const int ci = spans[item].x, cj = spans[item].y;
+		WorkItem& wi = out.wi;
+		wi = WorkItem{};
 		wi.world = w->groupIndex;
-		wi.bodyStart = (int)L.bodyList.size();
 		// owned bodies
-		for ( size_t k = ci; k < cj; ++k )
+		for ( int k = ci; k < cj; ++k )
 			for ( int b : comps[k].bodies )
 			{
-				posOfSim[b] = (int)L.bodyList.size() - wi.bodyStart;
-				L.bodyList.push_back( b );
+				posOfSim[b] = (int)out.bodies.size();
+				out.bodies.push_back( b );
 			}
-		wi.ownedCount = (int)L.bodyList.size() - wi.bodyStart;
+		wi.ownedCount = (int)out.bodies.size();
 		// local index of a body reference: owned bodies by position, kinematic bodies as read-only
 		// shadows appended on first use, static bodies -1
 		std::vector<std::pair<int, int>> shadowMap;
@@ -675,34 +722,31 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 			for ( auto& pr : shadowMap )
 				if ( pr.first == sim )
 					return pr.second;
-			int li = (int)L.bodyList.size() - wi.bodyStart;
-			L.bodyList.push_back( sim );
+			int li = (int)out.bodies.size();
+			out.bodies.push_back( sim );
 			shadowMap.push_back( { sim, li } );
 			return li;
 		};
-
-		wi.slotStart = (int)L.slotContact.size();
 		int itemMaxGroup = 0;
 		// group 0 = overflow (colour 11) then colours 0..10
 		for ( int gidx = 0; gidx < kGroups; ++gidx )
 		{
 			int color = gidx == 0 ? kOverflow : gidx - 1;
-			wi.groupStart[gidx] = (int)L.slotContact.size() - wi.slotStart;
+			wi.groupStart[gidx] = (int)out.slots.size();
 			const int* cs = compStart.data() + (size_t)color * ( nc + 1 );
-			for ( size_t k = ci; k < cj; ++k )
+			for ( int k = ci; k < cj; ++k )
 				for ( int p = cs[k]; p < cs[k + 1]; ++p )
 				{
 					const Src& e = src[bySlot[colorStart[color] + p]];
 					int la = localIndex( e.a );
 					int lb = localIndex( e.b );
-					L.slotContact.push_back( e.id );
-					L.slotBodies.push_back( make_int2( la, lb ) );
+					out.slots.push_back( e.id );
+					out.slotBodies.push_back( make_int2( la, lb ) );
 				}
-			int size = (int)L.slotContact.size() - wi.slotStart - wi.groupStart[gidx];
-			L.maxGroupSize = std::max( L.maxGroupSize, size );
+			int size = (int)out.slots.size() - wi.groupStart[gidx];
 			itemMaxGroup = std::max( itemMaxGroup, size );
 		}
-		for ( size_t k = ci; k < cj; ++k )
+		for ( int k = ci; k < cj; ++k )
 			for ( int b : comps[k].bodies )
 				posOfSim[b] = -1;
 		// Overflow levels (group 0, src/contact_solver.c:212-345 runs it serially): a slot's level
@@ -710,19 +754,18 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 		// level, keeping their order within a level. Static bodies are never changed; nor are
 		// kinematic ones, whose zero inverse mass and inertia leave their velocity bits as they
 		// are, unless a component is -0 (then they count as changed).
-		wi.ovfLevelBegin = (int)L.ovfLevels.size();
 		wi.ovfLevelCount = 0;
 		{
-			const int first = wi.slotStart + wi.groupStart[0], end = wi.slotStart + wi.groupStart[1];
+			const int first = wi.groupStart[0], end = wi.groupStart[1];
 			if ( end > first )
 			{
-				const int nb = (int)L.bodyList.size() - wi.bodyStart;
+				const int nb = (int)out.bodies.size();
 				auto changed = [&]( int li ) {
 					if ( li < 0 )
 						return false;
 					if ( li < wi.ownedCount )
 						return true;
-					const float* st = &w->awakeState[8 * (size_t)L.bodyList[wi.bodyStart + li]];
+					const float* st = &w->awakeState[8 * (size_t)out.bodies[li]];
 					for ( int c = 0; c < 3; ++c )
 						if ( st[c] == 0.0f && std::signbit( st[c] ) )
 							return true;
@@ -732,7 +775,7 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 				int maxLevel = 0;
 				for ( int k = first; k < end; ++k )
 				{
-					int2 lb = L.slotBodies[k];
+					int2 lb = out.slotBodies[k];
 					bool ca = changed( lb.x ), cb = changed( lb.y );
 					int lv = 0;
 					if ( ca )
@@ -754,25 +797,25 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 				std::vector<int2> bodies( end - first );
 				for ( int k = 0; k < end - first; ++k )
 				{
-					contacts[k] = L.slotContact[first + order[k]];
-					bodies[k] = L.slotBodies[first + order[k]];
+					contacts[k] = out.slots[first + order[k]];
+					bodies[k] = out.slotBodies[first + order[k]];
 				}
-				std::copy( contacts.begin(), contacts.end(), L.slotContact.begin() + first );
-				std::copy( bodies.begin(), bodies.end(), L.slotBodies.begin() + first );
+				std::copy( contacts.begin(), contacts.end(), out.slots.begin() + first );
+				std::copy( bodies.begin(), bodies.end(), out.slotBodies.begin() + first );
 				for ( int k = 0; k < end - first; ++k )
 					if ( k == 0 || level[order[k]] != level[order[k - 1]] )
-						L.ovfLevels.push_back( wi.groupStart[0] + k );
-				L.ovfLevels.push_back( wi.groupStart[1] );
+						out.levels.push_back( wi.groupStart[0] + k );
+				out.levels.push_back( wi.groupStart[1] );
 				wi.ovfLevelCount = maxLevel + 1;
-				B2MI_CHECK( (int)L.ovfLevels.size() - wi.ovfLevelBegin == wi.ovfLevelCount + 1 );
+				B2MI_CHECK( (int)out.levels.size() == wi.ovfLevelCount + 1 );
 			}
 		}
 		// per-item workgroup: one thread per slot of its largest colour group
 		wi.block = 64;
 		while ( wi.block < itemMaxGroup && wi.block < kSolveMaxBlock )
 			wi.block *= 2;
-		wi.groupStart[kGroups] = (int)L.slotContact.size() - wi.slotStart;
-		wi.bodyCount = (int)L.bodyList.size() - wi.bodyStart;
+		wi.groupStart[kGroups] = (int)out.slots.size();
+		wi.bodyCount = (int)out.bodies.size();
 		wi.nextGroup[0] = wi.nextGroup[1] = 0;
 		for ( int g = 1; g < kGroups; ++g )
 			for ( int step = 1; step < kGroups; ++step )
@@ -802,36 +845,72 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget )
 				wi.store = kStoreHbm;
 				wi.ldsBytes = 0;
 			}
-			L.maxItemLds = std::max( L.maxItemLds, wi.ldsBytes );
 		}
-		L.maxGroupSize = std::max( L.maxGroupSize, wi.ownedCount );
-		L.items.push_back( wi );
-		ci = cj;
+		out.maxGroup = std::max( itemMaxGroup, wi.ownedCount );
+	};
+	forRanges( ni, [&]( int lo, int hi ) {
+		for ( int i = lo; i < hi; ++i )
+			buildItem( i );
+	} );
+	// lay the items' lists end to end
+	{
+		int nb = 0, ns = 0, nl = 0;
+		for ( Built& b : built )
+		{
+			b.wi.bodyStart = nb;
+			b.wi.slotStart = ns;
+			b.wi.ovfLevelBegin = nl;
+			nb += (int)b.bodies.size();
+			ns += (int)b.slots.size();
+			nl += (int)b.levels.size();
+			L.maxGroupSize = std::max( L.maxGroupSize, b.maxGroup );
+			L.maxItemLds = std::max( L.maxItemLds, b.wi.ldsBytes );
+		}
+		L.bodyList.resize( nb );
+		L.slotContact.resize( ns );
+		L.slotBodies.resize( ns );
+		L.ovfLevels.resize( nl );
+		forRanges( ni, [&]( int lo, int hi ) {
+			for ( int i = lo; i < hi; ++i )
+			{
+				const Built& b = built[i];
+				std::copy( b.bodies.begin(), b.bodies.end(), L.bodyList.begin() + b.wi.bodyStart );
+				std::copy( b.slots.begin(), b.slots.end(), L.slotContact.begin() + b.wi.slotStart );
+				std::copy( b.slotBodies.begin(), b.slotBodies.end(), L.slotBodies.begin() + b.wi.slotStart );
+				std::copy( b.levels.begin(), b.levels.end(), L.ovfLevels.begin() + b.wi.ovfLevelBegin );
+			}
+		} );
+		L.items.resize( ni );
+		for ( int i = 0; i < ni; ++i )
+			L.items[i] = built[i].wi;
 	}
 	// every index the solver kernel dereferences is checked here, on the host
 	int na2 = (int)w->awakeSimBody.size();
-	for ( const WorkItem& wi : L.items )
-	{
-		B2MI_CHECK( wi.bodyStart >= 0 && wi.bodyStart + wi.bodyCount <= (int)L.bodyList.size() );
-		B2MI_CHECK( 0 <= wi.ownedCount && wi.ownedCount <= wi.bodyCount );
-		for ( int i = 0; i < wi.bodyCount; ++i )
-			B2MI_CHECK( 0 <= L.bodyList[wi.bodyStart + i] && L.bodyList[wi.bodyStart + i] < na2 );
-		B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= (int)L.slotContact.size() );
-		B2MI_CHECK( wi.ldsBytes <= kMaxItemLdsBytes );
-		B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
-					( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) ||
-					( wi.store == kStoreLds && wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) );
-		for ( int k = 0; k < kGroups; ++k )
-			B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
-		for ( int g = 1; g < kGroups; ++g )
-			B2MI_CHECK( 0 <= next_group( wi, g ) && next_group( wi, g ) < kGroups );
-		for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
+	forRanges( ni, [&]( int lo, int hi ) {
+		for ( int it = lo; it < hi; ++it )
 		{
-			int2 lb = L.slotBodies[wi.slotStart + k];
-			B2MI_CHECK( -1 <= lb.x && lb.x < wi.bodyCount && -1 <= lb.y && lb.y < wi.bodyCount );
-			B2MI_CHECK( 0 <= L.slotContact[wi.slotStart + k] && L.slotContact[wi.slotStart + k] < w->capContacts );
+			const WorkItem& wi = L.items[it];
+			B2MI_CHECK( wi.bodyStart >= 0 && wi.bodyStart + wi.bodyCount <= (int)L.bodyList.size() );
+			B2MI_CHECK( 0 <= wi.ownedCount && wi.ownedCount <= wi.bodyCount );
+			for ( int i = 0; i < wi.bodyCount; ++i )
+				B2MI_CHECK( 0 <= L.bodyList[wi.bodyStart + i] && L.bodyList[wi.bodyStart + i] < na2 );
+			B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= (int)L.slotContact.size() );
+			B2MI_CHECK( wi.ldsBytes <= kMaxItemLdsBytes );
+			B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
+						( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) ||
+						( wi.store == kStoreLds && wi.ldsBytes == itemLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) );
+			for ( int k = 0; k < kGroups; ++k )
+				B2MI_CHECK( wi.groupStart[k] <= wi.groupStart[k + 1] );
+			for ( int g = 1; g < kGroups; ++g )
+				B2MI_CHECK( 0 <= next_group( wi, g ) && next_group( wi, g ) < kGroups );
+			for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
+			{
+				int2 lb = L.slotBodies[wi.slotStart + k];
+				B2MI_CHECK( -1 <= lb.x && lb.x < wi.bodyCount && -1 <= lb.y && lb.y < wi.bodyCount );
+				B2MI_CHECK( 0 <= L.slotContact[wi.slotStart + k] && L.slotContact[wi.slotStart + k] < w->capContacts );
+			}
 		}
-	}
+	} );
 	w->topologyDirty = false;
 	w->solverPending = true;
 }
@@ -1740,7 +1819,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 		if ( !topology.list.empty() )
 		{
 			HostPool::get().run( (int)topology.list.size(),
-								 [&]( int i ) { buildWorldItems( topology.list[i], target, budget, fullBudget ); } );
+								 [&]( int i ) { buildWorldItems( topology.list[i], target, budget, fullBudget, topology.list.size() == 1 ); } );
 			bool full = false;
 			for ( World* w : topology.list )
 				full = full || (int)w->solver.bodyList.size() > w->bodyRegion.cap ||
