@@ -514,12 +514,6 @@ static void updatePairs( World* w, bool parallel )
 // work items: unions of islands (connected components through touching contacts of the
 // non-static, non-kinematic bodies), packed to a target size
 
-struct Component
-{
-	std::vector<int> bodies; // awake sim indices
-	int slotCount = 0;
-};
-
 static int findRoot( std::vector<int>& parent, int x )
 {
 	while ( parent[x] != x )
@@ -615,24 +609,30 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 			if ( ra != rb )
 				parent[std::max( ra, rb )] = std::min( ra, rb );
 		}
-	// components in order of their smallest sim index; kinematic bodies alone
-	std::vector<int> compOf( na, -1 );
-	std::vector<Component> comps;
+	// Components in order of their smallest sim index (a root is the smallest index of its
+	// set, so it is numbered before the rest of it); kinematic bodies alone. Component k's
+	// bodies, in ascending sim index, are compBodies[compBodyStart[k] .. compBodyStart[k+1]).
+	std::vector<int> compOf( na );
+	int nc = 0;
 	for ( int i = 0; i < na; ++i )
 	{
 		int r = findRoot( parent, i );
-		if ( compOf[r] < 0 )
-		{
-			compOf[r] = (int)comps.size();
-			comps.push_back( Component{} );
-		}
-		compOf[i] = compOf[r];
-		comps[compOf[i]].bodies.push_back( i );
+		compOf[i] = r == i ? nc++ : compOf[r];
 	}
+	std::vector<int> compBodyStart( nc + 1, 0 ), compBodies( na ), compSlots( nc, 0 );
+	for ( int i = 0; i < na; ++i )
+		compBodyStart[compOf[i] + 1] += 1;
+	for ( int k = 0; k < nc; ++k )
+		compBodyStart[k + 1] += compBodyStart[k];
+	{
+		std::vector<int> fill( compBodyStart.begin(), compBodyStart.end() - 1 );
+		for ( int i = 0; i < na; ++i )
+			compBodies[fill[compOf[i]]++] = i;
+	}
+	auto compSize = [&]( int k ) { return compBodyStart[k + 1] - compBodyStart[k]; };
 	// A contact belongs to the component of its dynamic body. Per colour, the contacts are
 	// ordered by component (a stable counting sort, so each component keeps the colour list's
 	// order): component k's contacts of colour c are bySlot[compStart[c][k] .. compStart[c][k+1]).
-	const int nc = (int)comps.size();
 	std::vector<int> compStart( (size_t)kColorCount * ( nc + 1 ), 0 );
 	std::vector<int> bySlot( src.size() );
 	{
@@ -664,7 +664,7 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		{
 			const int* cs = compStart.data() + (size_t)c * ( nc + 1 );
 			for ( int k = 0; k < nc; ++k )
-				comps[k].slotCount += cs[k + 1] - cs[k];
+				compSlots[k] += cs[k + 1] - cs[k];
 		}
 	}
 
@@ -676,10 +676,10 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		int slots = 0, bodies = 0;
 		do
 		{
-			slots += comps[cj].slotCount;
-			bodies += (int)comps[cj].bodies.size();
+			slots += compSlots[cj];
+			bodies += compSize( cj );
 			cj += 1;
-		} while ( cj < nc && slots + comps[cj].slotCount <= target && bodies + (int)comps[cj].bodies.size() <= 4 * target );
+		} while ( cj < nc && slots + compSlots[cj] <= target && bodies + compSize( cj ) <= 4 * target );
 		spans.push_back( make_int2( ci, cj ) );
 		ci = cj;
 	}
@@ -704,10 +704,10 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		wi.world = w->groupIndex;
 		// owned bodies
 		for ( int k = ci; k < cj; ++k )
-			for ( int b : comps[k].bodies )
+			for ( int q = compBodyStart[k]; q < compBodyStart[k + 1]; ++q )
 			{
-				posOfSim[b] = (int)out.bodies.size();
-				out.bodies.push_back( b );
+				posOfSim[compBodies[q]] = (int)out.bodies.size();
+				out.bodies.push_back( compBodies[q] );
 			}
 		wi.ownedCount = (int)out.bodies.size();
 		// local index of a body reference: owned bodies by position, kinematic bodies as read-only
@@ -747,8 +747,8 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 			itemMaxGroup = std::max( itemMaxGroup, size );
 		}
 		for ( int k = ci; k < cj; ++k )
-			for ( int b : comps[k].bodies )
-				posOfSim[b] = -1;
+			for ( int q = compBodyStart[k]; q < compBodyStart[k + 1]; ++q )
+				posOfSim[compBodies[q]] = -1;
 		// Overflow levels (group 0, src/contact_solver.c:212-345 runs it serially): a slot's level
 		// is one past the last level that changed one of its bodies, and the slots are ordered by
 		// level, keeping their order within a level. Static bodies are never changed; nor are
@@ -1198,24 +1198,44 @@ static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
 {
 	if ( count == 0 )
 		return;
-	std::vector<const EnlargeRecord*> order( count );
-	for ( int i = 0; i < count; ++i )
-		order[i] = recs + i;
-	// sim order; within a body the head-first shape list order
-	std::vector<int> rank( w->shapes.size(), 0 );
+	// sim order (a counting sort on the sim index); within a body the head-first shape list order
+	int maxSim = 0;
 	for ( int i = 0; i < count; ++i )
 	{
-		const Body& b = w->bodies[w->shapes[recs[i].shapeId].bodyId];
-		int r = 0;
-		for ( int id = b.headShapeId; id != -1; id = w->shapes[id].nextShapeId, ++r )
-			if ( id == recs[i].shapeId )
-				rank[id] = r;
+		B2MI_CHECK( recs[i].simIndex >= 0 && recs[i].simIndex < (int)w->awakeSimBody.size() );
+		maxSim = std::max( maxSim, recs[i].simIndex );
 	}
-	std::sort( order.begin(), order.end(), [&]( const EnlargeRecord* a, const EnlargeRecord* b ) {
-		if ( a->simIndex != b->simIndex )
-			return a->simIndex < b->simIndex;
-		return rank[a->shapeId] < rank[b->shapeId];
-	} );
+	std::vector<int> start( maxSim + 2, 0 );
+	for ( int i = 0; i < count; ++i )
+		start[recs[i].simIndex + 1] += 1;
+	for ( int k = 0; k <= maxSim; ++k )
+		start[k + 1] += start[k];
+	std::vector<const EnlargeRecord*> order( count );
+	{
+		std::vector<int> fill( start.begin(), start.end() - 1 );
+		for ( int i = 0; i < count; ++i )
+			order[fill[recs[i].simIndex]++] = recs + i;
+	}
+	for ( int k = 0; k <= maxSim; ++k )
+		if ( start[k + 1] - start[k] > 1 )
+		{
+			// several shapes of one body: order them by their position in its shape list
+			std::vector<std::pair<int, const EnlargeRecord*>> byRank;
+			for ( int q = start[k]; q < start[k + 1]; ++q )
+			{
+				const Body& b = w->bodies[w->shapes[order[q]->shapeId].bodyId];
+				int r = 0;
+				for ( int id = b.headShapeId; id != -1 && id != order[q]->shapeId; id = w->shapes[id].nextShapeId )
+					r += 1;
+				byRank.push_back( { r, order[q] } );
+			}
+			std::sort( byRank.begin(), byRank.end(),
+					   []( const std::pair<int, const EnlargeRecord*>& a, const std::pair<int, const EnlargeRecord*>& b ) {
+						   return a.first < b.first;
+					   } );
+			for ( int q = start[k]; q < start[k + 1]; ++q )
+				order[q] = byRank[q - start[k]].second;
+		}
 	for ( const EnlargeRecord* r : order )
 	{
 		Shape& s = w->shapes[r->shapeId];
