@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
 """Throughput of the HIP step pipeline in bodies x substeps / s on one MI355X.
 
-Workload (default): one world holding a 40 x 40 grid of base-10 box pyramids (88,000
-dynamic boxes, half-extent 0.5, friction 0.6), dt = 1/60 s, 4 substeps, synthetic input
+Workload of the headline (`value`): one world holding a 40 x 40 grid of base-10 box pyramids
+(88,000 dynamic boxes, half-extent 0.5, friction 0.6), dt = 1/60 s, 4 substeps, synthetic input
 from box2c_amd/csrc/scenes.c. W warmup steps (the first creates every contact) are excluded;
-K steps are timed. Every step of the timed region runs the full pipeline: broadphase pair
-update, narrowphase, colouring updates, the solver (prepare, 4 x (integrate velocities, warm
-start, solve, integrate positions, relax), restitution, store impulses) and finalize.
+steps W+1 .. W+K are timed. Every timed step runs the full pipeline: broadphase pair update,
+narrowphase, colouring updates, the solver (prepare, 4 x (integrate velocities, warm start,
+solve, integrate positions, relax), restitution, store impulses) and finalize.
 
-Printed JSON line: metric/value/unit per the benchmark contract, plus
-  roofline      the solver kernel (k_solve, the dominant launch): algorithmic bytes per
-                launch / its mean duration from HIP events on its stream;
-  cpu_baseline  the reference (oracle/_ref, built from its own sources) on the host cores
-                with a pthread task pool, stepping the same scene for a bounded sample.
+Further legs, each an object in the same JSON line: the reference's own CPU benchmark scene
+(large pyramid of base 100), the batch of 4,096 uneven pyramid worlds, the differentiable
+1,024-world drop-box rollout with its adjoint, and the 40,000-shape tumbler.
+
+Each leg carries
+  roofline      its solver kernel (k_solve, the dominant launch): algorithmic bytes per launch
+                (counted fields, each once per pass) / the kernel's mean duration from HIP events
+                on the stream it runs on, against the 8 TB/s HBM peak; beside it the HBM bytes per
+                launch measured by rocprofv3 counters (profiles/hbm_*.json) and the rate they give;
+  cpu_baseline  the reference (oracle/_ref, built from its own sources) on this host's cores with a
+                pthread task pool, over the same step window as the GPU leg (or a stated bounded
+                part of it), at the thread count of the host's CPU share, plus a 1..N thread curve.
 """
 from __future__ import annotations
 
@@ -28,6 +35,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "bodies×substeps/s (1 GPU): plain vs batched vs fwd+adjoint at ≤1e-5 step error"
 HBM_PEAK_GBS = 8000.0
+SUBSTEPS = 4
+DT = 1.0 / 60.0
 
 
 def algorithmic_bytes(n_bodies: int, n_contacts: int, substeps: int) -> dict:
@@ -41,17 +50,135 @@ def algorithmic_bytes(n_bodies: int, n_contacts: int, substeps: int) -> dict:
     return {"solve": solve, "collide": collide, "step": solve + collide}
 
 
-def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
+def roofline(workload: str, n_bodies: int, n_contacts: int, ms_solve: float) -> dict:
+    """The solver kernel's algorithmic-byte rate against the HBM peak, with the measured HBM
+    bytes per launch of the same workload when a counter summary is committed for it."""
+    ab = algorithmic_bytes(n_bodies, n_contacts, SUBSTEPS)
+    achieved = ab["solve"] / (ms_solve * 1e-3) / 1e9 if ms_solve > 0 else 0.0
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+           "traffic": None, "kernel": "k_solve", "algorithmic_bytes_per_launch": ab["solve"],
+           "note": "achieved = algorithmic bytes (each field once per pass) / mean k_solve duration; the kernel keeps "
+                   "work items in LDS across passes, so this can exceed what crosses HBM: measured_* is the HBM traffic"}
+    tfile = os.path.join(ROOT, "profiles", f"hbm_{workload}.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("workload") == workload and t.get("k_solve_bytes_per_launch"):
+            traffic = float(t["k_solve_bytes_per_launch"])
+            out["traffic"] = traffic
+            rate = traffic / (ms_solve * 1e-3) / 1e9 if ms_solve > 0 else 0.0
+            out["measured_hbm_GBs"] = rate
+            out["measured_hbm_frac"] = rate / HBM_PEAK_GBS
+    return out
+
+
+# ---- host CPU share ------------------------------------------------------------------------
+
+def host_cores() -> dict:
+    """The CPU share this process may use: physical cores among the CPUs of its affinity mask,
+    capped by a cgroup CPU quota and by the OMP_NUM_THREADS share when those are set."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            cores.add((open(f"{base}/physical_package_id").read().strip(), open(f"{base}/core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    physical = len(cores)
+    quota = None
+    try:
+        q, p = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) / int(p)))
+    except (OSError, ValueError):
+        pass
+    share = None
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = int(os.environ["OMP_NUM_THREADS"])
+    used = physical
+    for cap in (quota, share):
+        if cap:
+            used = min(used, cap)
+    return {"affinity_cpus": len(cpus), "physical_cores": physical, "cgroup_quota": quota, "omp_share": share,
+            "threads": max(1, used)}
+
+
+def progress(msg: str):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def thread_curve(n: int) -> list:
+    out, t = [], 1
+    while t < n:
+        out.append(t)
+        t *= 2
+    return out + [n]
+
+
+def _ref():
+    from box2c_amd.capi import B2Api
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
+    if not (os.path.exists(lib) and os.path.exists(scenes)):
+        return None
+    return B2Api(lib, scenes)
+
+
+def cpu_world_rate(ref, scene: str, kw: dict, threads: int, warmup: int, steps: int):
+    """The reference stepping one world with `threads` workers: W warmup steps, then `steps`
+    timed (benchmark/main.c:196-209: wall clock over the steps). Returns (bodies x substeps / s,
+    seconds)."""
+    ref.lib.ref_pool_start(threads)
+    try:
+        d = ref.default_world_def(enableSleep=False)
+        ref.lib.ref_pool_apply(ctypes.byref(d))
+        w, ids = ref.scene(scene, d, **kw)
+        ref.step(w, n=warmup)
+        t0 = time.perf_counter()
+        ref.step(w, n=steps)
+        elapsed = time.perf_counter() - t0
+        ref.destroy(w)
+    finally:
+        ref.lib.ref_pool_stop()
+    return len(ids) * SUBSTEPS * steps / elapsed, elapsed
+
+
+def cpu_world_baseline(scene: str, kw: dict, label: str, host: dict, warmup: int, steps: int, curve_steps: int):
+    ref = _ref()
+    if ref is None:
+        return None
+    n = host["threads"]
+    value, elapsed = cpu_world_rate(ref, scene, kw, n, warmup, steps)
+    curve = {}
+    for t in thread_curve(n):
+        if t == n and curve_steps == steps:
+            curve[str(t)] = value
+            continue
+        curve[str(t)] = cpu_world_rate(ref, scene, kw, t, warmup, curve_steps)[0]
+    return {"value": value, "unit": "bodies×substeps/s", "cores": n, "kind": "reference",
+            "sample": f"{label}: steps {warmup + 1}..{warmup + steps} (the GPU leg's window), {n} worker threads "
+                      f"(the host's CPU share: {host}), {elapsed:.1f} s",
+            "thread_curve": curve,
+            "thread_curve_sample": f"steps {warmup + 1}..{warmup + curve_steps} at each thread count"}
+
+
+# ---- GPU legs ------------------------------------------------------------------------------
+
+def run_world(api, scene: str, kw: dict, steps: int, warmup: int):
     d = api.default_world_def(enableSleep=False)
-    w, ids = api.scene(scene, d, rows=rows, cols=cols, base=10, seed=7)
-    n_bodies = len(ids)
+    w, ids = api.scene(scene, d, **kw)
     api.step(w, n=warmup)
-    counters = api.counters(w)
     t_solve = t_collide = 0.0
     host = {"pairs": 0.0, "broadphase": 0.0, "buildIslands": 0.0, "finalizeBodies": 0.0, "step": 0.0}
+    per_step = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        api.lib.b2World_Step(w, 1.0 / 60.0, 4)
+        ts = time.perf_counter()
+        api.lib.b2World_Step(w, DT, SUBSTEPS)
+        per_step.append(time.perf_counter() - ts)
         c, s, _h = api.world_timing(w)
         t_collide += c
         t_solve += s
@@ -60,19 +187,45 @@ def run_gpu(api, scene: str, steps: int, warmup: int, rows: int, cols: int):
             host[k] += getattr(prof, k)
     elapsed = time.perf_counter() - t0
     counters = api.counters(w)
-    n_contacts = sum(counters.colorCounts)
+    items = api.work_items(world=w)
     api.destroy(w)
     return {
-        "bodies": n_bodies, "contacts": n_contacts, "elapsed": elapsed,
-        "ms_solve": t_solve / steps, "ms_collide": t_collide / steps,
-        "colors": list(counters.colorCounts),
+        "bodies": len(ids), "contacts": sum(counters.colorCounts), "elapsed": elapsed, "per_step": per_step,
+        "ms_solve": t_solve / steps, "ms_collide": t_collide / steps, "colors": list(counters.colorCounts),
         "host_ms": {k: v / steps for k, v in host.items()},
+        "work_items": {"count": int(len(items)), "largest_slots": int(items[:, 0].max()) if len(items) else 0,
+                       "modes": sorted(set(items[:, 3].tolist())) if len(items) else []},
     }
 
 
-def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, autotune: bool = True):
-    """4096 (default) independent worlds, each one pyramid of base U{4..40} on its own ground box,
-    advanced together: two kernel launches per step for all worlds."""
+def world_leg(api, name: str, scene: str, kw: dict, steps: int, warmup: int, host: dict, cpu: bool, cpu_steps: int,
+              curve_steps: int):
+    progress(f"{name}: GPU, {warmup} warmup + {steps} timed steps")
+    r = run_world(api, scene, kw, steps, warmup)
+    value = r["bodies"] * SUBSTEPS * steps / r["elapsed"]
+    leg = {
+        "workload": name, "value": value, "unit": "bodies×substeps/s", "bodies": r["bodies"],
+        "touching_contacts": r["contacts"], "graph_colors": r["colors"], "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * r["elapsed"] / steps, "ms_solve_kernel": r["ms_solve"], "ms_collide_kernel": r["ms_collide"],
+        "ms_step_phases": r["host_ms"], "work_items": r["work_items"],
+        "roofline": roofline(name, r["bodies"], r["contacts"], r["ms_solve"]), "cpu_baseline": None,
+    }
+    if cpu:
+        progress(f"{name}: CPU reference, {cpu_steps} steps at {host['threads']} threads and the thread curve")
+        cb = cpu_world_baseline(scene, kw, name, host, warmup, cpu_steps, curve_steps)
+        leg["cpu_baseline"] = cb
+        if cb:
+            # GPU and CPU over the same steps (the CPU sample's window when it is shorter)
+            k = min(cpu_steps, steps)
+            window = r["bodies"] * SUBSTEPS * k / sum(r["per_step"][:k])
+            leg["vs_cpu"] = window / cb["value"]
+            if k < steps:
+                leg["value_cpu_window"] = window
+            leg["vs_cpu_note"] = f"GPU and CPU over the same steps {warmup + 1}..{warmup + k}"
+    return leg
+
+
+def build_batch(api, n_worlds: int, seed: int):
     d = api.default_world_def(enableSleep=False)
     batch = api.create_batch()
     api.use_batch(batch)
@@ -83,71 +236,121 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, au
             n_bodies += len(ids)
     finally:
         api.use_batch(None)
-    api.batch_step(batch, n=warmup)
-    shape = None
-    if autotune:
-        best = api.lib.b2mi_Batch_Autotune(batch, 1.0 / 60.0, 4, 2)
+    return batch, n_bodies
+
+
+def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, autotune: bool = True, shape=None):
+    """4096 (default) independent worlds, each one pyramid of base U{4..40} on its own ground box,
+    advanced together. The autotuner runs on a batch of its own (its 180 trial steps are ordinary
+    steps), so the timed batch starts from step 0 like the CPU's: W warmup steps, K timed."""
+    tuned = None
+    if shape is None and autotune:
+        tb, _ = build_batch(api, n_worlds, seed)
+        api.batch_step(tb, n=warmup)
+        best = api.lib.b2mi_Batch_Autotune(tb, DT, SUBSTEPS, 2)
         b, t, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        api.lib.b2mi_Batch_GetLaunchShape(batch, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l))
-        shape = {"block": b.value, "target_slots": t.value, "lds_budget": l.value, "tuned_solve_ms": best}
-        api.batch_step(batch)  # rebuilds the work items under the chosen shape
+        api.lib.b2mi_Batch_GetLaunchShape(tb, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l))
+        shape = (b.value, t.value, l.value)
+        tuned = {"block": b.value, "target_slots": t.value, "lds_budget": l.value, "tuned_solve_ms": best}
+        api.lib.b2mi_DestroyBatch(tb)
+    batch, n_bodies = build_batch(api, n_worlds, seed)
+    if shape is not None:
+        api.lib.b2mi_Batch_Configure(batch, shape[0], shape[1])
+        api.lib.b2mi_Batch_SetLdsBudget(batch, shape[2])
+    api.batch_step(batch, n=warmup)
     t_solve = t_collide = t_host = 0.0
+    per_step = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        api.lib.b2mi_Batch_Step(batch, 1.0 / 60.0, 4)
+        ts = time.perf_counter()
+        api.lib.b2mi_Batch_Step(batch, DT, SUBSTEPS)
+        per_step.append(time.perf_counter() - ts)
         c, s, h = api.batch_timing(batch)
         t_collide += c
         t_solve += s
         t_host += h
     elapsed = time.perf_counter() - t0
-    items = api.lib.b2mi_Batch_GetWorkItemCount(batch)
+    items = api.work_items(batch=batch)
+    contacts = 0
     api.lib.b2mi_DestroyBatch(batch)
-    return {"bodies": n_bodies, "elapsed": elapsed, "ms_solve": t_solve / steps, "ms_collide": t_collide / steps,
-            "ms_host": t_host / steps, "work_items": items, "launch_shape": shape}
+    return {"bodies": n_bodies, "elapsed": elapsed, "per_step": per_step, "ms_solve": t_solve / steps,
+            "ms_collide": t_collide / steps, "ms_host": t_host / steps, "work_items": int(len(items)),
+            "slots": int(items[:, 0].sum()) if len(items) else contacts,
+            "launch_shape": tuned or ({"block": shape[0], "target_slots": shape[1], "lds_budget": shape[2]} if shape else None)}
 
 
-def run_tumbler(api, shapes: int, steps: int, warmup: int, seed: int = 5):
-    """A kinematic square container spinning at 0.25 rad/s, filled with `shapes` half boxes and
-    half circles of size 0.125 on a jittered 0.4 m grid (the container grows with the count)."""
-    d = api.default_world_def(enableSleep=False)
-    w, ids = api.scene("tumbler", d, count=shapes, seed=seed)
-    api.step(w, n=warmup)
-    t_solve = t_collide = 0.0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        api.lib.b2World_Step(w, 1.0 / 60.0, 4)
-        c, s, _h = api.world_timing(w)
-        t_collide += c
-        t_solve += s
-    elapsed = time.perf_counter() - t0
-    counters = api.counters(w)
-    api.destroy(w)
-    return {"bodies": len(ids), "contacts": sum(counters.colorCounts), "colors": list(counters.colorCounts),
-            "elapsed": elapsed, "ms_solve": t_solve / steps, "ms_collide": t_collide / steps}
-
-
-def cpu_baseline_tumbler(shapes: int, threads: int, sample_steps: int, warmup: int, seed: int = 5):
-    """The reference on the same tumbler with `threads` workers, over a bounded number of steps."""
-    from box2c_amd.capi import B2Api
-    ref_dir = os.path.join(ROOT, "oracle", "_ref")
-    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
-    if not (os.path.exists(lib) and os.path.exists(scenes)):
+def cpu_baseline_batch(host: dict, n_worlds: int, warmup: int, steps: int, curve_steps: int, seed: int = 7):
+    """The reference on every world of the batch with world-level parallelism (each thread steps
+    whole worlds taken from a shared counter, largest first: the reference's best schedule for
+    many small worlds). It holds at most 128 worlds at once (src/core.h:129), so the batch runs in
+    waves of 128: each wave's worlds are built, stepped through the warmup, and timed over the
+    same steps. The thread curve is taken on the first wave."""
+    from box2c_amd.capi import WorldId
+    ref = _ref()
+    if ref is None:
         return None
-    ref = B2Api(lib, scenes)
-    ref.lib.ref_pool_start(threads)
+    f = ref.lib.ref_step_worlds_parallel
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+    f.restype = ctypes.c_double
     d = ref.default_world_def(enableSleep=False)
-    ref.lib.ref_pool_apply(__import__("ctypes").byref(d))
-    w, ids = ref.scene("tumbler", d, count=shapes, seed=seed)
-    ref.step(w, n=warmup)
-    t0 = time.perf_counter()
-    ref.step(w, n=sample_steps)
-    elapsed = time.perf_counter() - t0
-    ref.destroy(w)
-    ref.lib.ref_pool_stop()
-    return {"value": len(ids) * 4 * sample_steps / elapsed, "unit": "bodies×substeps/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"tumbler of {len(ids)} shapes, {sample_steps} steps after {warmup} warmup, {threads} worker threads, "
-                      f"{elapsed:.1f} s"}
+    n = host["threads"]
+
+    def wave(first, count, threads, timed):
+        made = []
+        for i in range(first, first + count):
+            w, ids = ref.scene("batch_pyramid", d, seed=seed, index=i)
+            made.append((len(ids), w))
+        made.sort(key=lambda t: -t[0])
+        arr = (WorldId * count)(*[w for _, w in made])
+        f(arr, count, threads, warmup, DT, SUBSTEPS)
+        elapsed = f(arr, count, threads, timed, DT, SUBSTEPS)
+        for i in range(count):
+            ref.destroy(arr[i])
+        return sum(b for b, _ in made), elapsed
+
+    bodies = elapsed = 0.0
+    for first in range(0, n_worlds, 128):
+        b, e = wave(first, min(128, n_worlds - first), n, steps)
+        bodies += b
+        elapsed += e
+    value = bodies * SUBSTEPS * steps / elapsed
+    curve = {}
+    for t in thread_curve(n):
+        b, e = wave(0, min(128, n_worlds), t, curve_steps)
+        curve[str(t)] = b * SUBSTEPS * curve_steps / e
+    return {"value": value, "unit": "bodies×substeps/s", "cores": n, "kind": "reference",
+            "sample": f"all {n_worlds} batch-pyramid worlds ({int(bodies)} boxes) in waves of 128, steps {warmup + 1}.."
+                      f"{warmup + steps} of each, worlds taken largest first by {n} threads (the host's CPU share: {host}), "
+                      f"{elapsed:.1f} s of timed stepping",
+            "thread_curve": curve,
+            "thread_curve_sample": f"the first wave (128 worlds), steps {warmup + 1}..{warmup + curve_steps}"}
+
+
+def batched_leg(api, args, host: dict, cpu: bool):
+    shape = tuple(int(x) for x in args.batch_shape.split(",")) if args.batch_shape else None
+    progress(f"batched: GPU, {args.batch_worlds} worlds")
+    rb = run_gpu_batch(api, args.batch_worlds, args.batch_steps, args.batch_warmup, autotune=not args.batch_no_autotune,
+                       shape=shape)
+    bval = rb["bodies"] * SUBSTEPS * args.batch_steps / rb["elapsed"]
+    name = f"batch_pyramids_{args.batch_worlds}_base4-40"
+    leg = {
+        "workload": name, "value": bval, "unit": "bodies×substeps/s", "bodies": rb["bodies"],
+        "worlds": args.batch_worlds, "steps": args.batch_steps, "warmup": args.batch_warmup,
+        "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps, "ms_solve_kernel": rb["ms_solve"],
+        "ms_collide_kernel": rb["ms_collide"], "ms_host": rb["ms_host"], "work_items": rb["work_items"],
+        "launch_shape": rb["launch_shape"], "cpu_baseline": None,
+        "roofline": roofline(name, rb["bodies"], rb["slots"], rb["ms_solve"]),
+    }
+    k = min(args.batch_steps, args.batch_cpu_steps)
+    leg["value_cpu_window"] = rb["bodies"] * SUBSTEPS * k / sum(rb["per_step"][:k])
+    if cpu:
+        progress(f"batched: CPU reference, {args.batch_worlds} worlds in waves of 128")
+        cb = cpu_baseline_batch(host, args.batch_worlds, args.batch_warmup, k, min(k, 20))
+        leg["cpu_baseline"] = cb
+        if cb:
+            leg["vs_cpu"] = leg["value_cpu_window"] / cb["value"]
+            leg["vs_cpu_note"] = f"GPU and CPU over the same steps {args.batch_warmup + 1}..{args.batch_warmup + k}"
+    return leg
 
 
 def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed: int = 11):
@@ -155,7 +358,7 @@ def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed:
     sum |p_final - target|^2 with respect to friction, restitution and initial velocities."""
     import numpy as np
     from box2c_amd import diff
-    r = diff.DropBoxesRollout(api, worlds, boxes, steps, 4, checkpoint, seed=seed)
+    r = diff.DropBoxesRollout(api, worlds, boxes, steps, SUBSTEPS, checkpoint, seed=seed)
     f, e, v0, w0 = r.default_params()
     try:
         r.run(f, e, v0, w0)  # warm-up: kernels, allocations
@@ -170,75 +373,12 @@ def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed:
         r.close()
     n = worlds * boxes
     return {
-        "workload": f"drop_boxes_{worlds}x{boxes}_{steps}steps_adjoint_f32", "bodies": n, "steps": steps,
+        "workload": f"drop_boxes_{worlds}x{boxes}_{steps}steps_adjoint", "bodies": n, "steps": steps,
         "checkpoint_interval": checkpoint, "forward_s": fwd, "backward_s": bwd, "adjoint_over_forward": bwd / fwd,
-        "value_forward": n * 4 * steps / fwd, "value_fwd_plus_adjoint": n * 4 * steps / (fwd + bwd),
+        "value_forward": n * SUBSTEPS * steps / fwd, "value_fwd_plus_adjoint": n * SUBSTEPS * steps / (fwd + bwd),
         "unit": "bodies×substeps/s", "gradients_finite": finite,
         "note": "forward = the taped GPU step loop (narrowphase, solver, recording); backward = replay of each "
                 "checkpoint segment plus the adjoint kernels; manifolds held fixed per step; continuous collision off",
-    }
-
-
-def cpu_baseline_batch(threads: int, n_worlds: int, sample_steps: int, warmup: int, seed: int = 7):
-    """The reference on the same kind of worlds with world-level parallelism (each thread steps
-    its own worlds, the reference's best schedule for many small worlds). The reference holds at
-    most 128 worlds at once, so the sample is one wave: the first `n_worlds` (<= 128) worlds of
-    the batch."""
-    import ctypes
-    from box2c_amd.capi import B2Api, WorldId
-    ref_dir = os.path.join(ROOT, "oracle", "_ref")
-    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
-    if not (os.path.exists(lib) and os.path.exists(scenes)):
-        return None
-    ref = B2Api(lib, scenes)
-    f = ref.lib.ref_step_worlds_parallel
-    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int]
-    f.restype = ctypes.c_double
-    d = ref.default_world_def(enableSleep=False)
-    made = []
-    for i in range(n_worlds):
-        w, ids = ref.scene("batch_pyramid", d, seed=seed, index=i)
-        made.append((len(ids), w))
-    made.sort(key=lambda t: -t[0])  # largest first: threads take worlds from a shared counter
-    worlds = (WorldId * n_worlds)(*[w for _, w in made])
-    n_bodies = sum(n for n, _ in made)
-    f(worlds, n_worlds, threads, warmup, 1.0 / 60.0, 4)
-    elapsed = f(worlds, n_worlds, threads, sample_steps, 1.0 / 60.0, 4)
-    for i in range(n_worlds):
-        ref.destroy(worlds[i])
-    return {
-        "value": n_bodies * 4 * sample_steps / elapsed, "unit": "bodies×substeps/s", "cores": threads,
-        "kind": "reference",
-        "sample": f"{n_worlds} batch-pyramid worlds ({n_bodies} boxes, the first wave of the batch), {sample_steps} steps "
-                  f"after {warmup} warmup, worlds taken largest first by {threads} threads, {elapsed:.1f} s",
-    }
-
-
-def cpu_baseline(rows: int, cols: int, threads: int, sample_steps: int, warmup: int = 20):
-    """The reference stepping the same scene with `threads` workers, timed over a bounded
-    number of steps after the same warmup as the GPU leg (bench.py's CPU leg; the oracle directory holds the
-    reference build and is used here only as the baseline)."""
-    from box2c_amd.capi import B2Api
-    ref_dir = os.path.join(ROOT, "oracle", "_ref")
-    lib, scenes = os.path.join(ref_dir, "libbox2d_ref.so"), os.path.join(ref_dir, "libref_scenes.so")
-    if not (os.path.exists(lib) and os.path.exists(scenes)):
-        return None
-    ref = B2Api(lib, scenes)
-    ref.lib.ref_pool_start(threads)
-    d = ref.default_world_def(enableSleep=False)
-    ref.lib.ref_pool_apply(__import__("ctypes").byref(d))
-    w, ids = ref.scene("pyramid_grid", d, rows=rows, cols=cols, base=10, seed=7)
-    ref.step(w, n=warmup)
-    t0 = time.perf_counter()
-    ref.step(w, n=sample_steps)
-    elapsed = time.perf_counter() - t0
-    ref.destroy(w)
-    ref.lib.ref_pool_stop()
-    return {
-        "value": len(ids) * 4 * sample_steps / elapsed, "unit": "bodies×substeps/s", "cores": threads,
-        "kind": "reference",
-        "sample": f"pyramid_grid {rows}x{cols} ({len(ids)} boxes), {sample_steps} steps after {warmup} warmup, "
-                  f"{threads} worker threads, {elapsed:.1f} s",
     }
 
 
@@ -259,23 +399,26 @@ def calibrate_copy(mib: int = 1024, reps: int = 5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=300, help="timed steps of the headline (pyramid grid) leg")
+    ap.add_argument("--warmup", type=int, default=20, help="warmup steps of the headline leg")
     ap.add_argument("--rows", type=int, default=40)
     ap.add_argument("--cols", type=int, default=40)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU worker threads (0 = the host's CPU share)")
+    ap.add_argument("--cpu-curve-steps", type=int, default=40, help="steps of each point of the CPU thread curve")
+    ap.add_argument("--pyramid-steps", type=int, default=500, help="timed steps of the large-pyramid leg")
     ap.add_argument("--batch-worlds", type=int, default=4096, help="worlds of the batched leg (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=300)
-    ap.add_argument("--batch-cpu-steps", type=int, default=1000)
+    ap.add_argument("--batch-warmup", type=int, default=5)
+    ap.add_argument("--batch-cpu-steps", type=int, default=30, help="steps of the batched leg's CPU sample")
+    ap.add_argument("--batch-no-autotune", action="store_true", help="batched leg at the default launch shape")
+    ap.add_argument("--batch-shape", default="", help="block,target,ldsBudget: skip the autotuner, use this shape")
     ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the adjoint leg (0 = skip it)")
     ap.add_argument("--tumbler-shapes", type=int, default=40000, help="shapes of the tumbler leg (0 = skip it)")
     ap.add_argument("--tumbler-steps", type=int, default=600)
-    ap.add_argument("--tumbler-cpu-steps", type=int, default=60)
-    ap.add_argument("--legs", default="plain,batched,adjoint,tumbler",
+    ap.add_argument("--tumbler-cpu-steps", type=int, default=600, help="steps of the tumbler leg's CPU sample")
+    ap.add_argument("--legs", default="plain,large_pyramid,batched,adjoint,tumbler",
                     help="comma-separated legs to run (a profiling run may keep one; the JSON line then lacks the others)")
-    ap.add_argument("--batch-no-autotune", action="store_true", help="batched leg at the default launch shape")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain copy kernel of known size (for HBM counter scaling)")
     args = ap.parse_args()
@@ -294,99 +437,42 @@ def main():
         raise SystemExit("no HIP device visible")
 
     legs = set(args.legs.split(","))
-    substeps = 4
-    if "plain" not in legs:
-        out = {"metric": METRIC, "legs": sorted(legs)}
-        if "batched" in legs and args.batch_worlds > 0:
-            rb = run_gpu_batch(api, args.batch_worlds, args.batch_steps, args.warmup, autotune=not args.batch_no_autotune)
-            out["batched"] = {"value": rb["bodies"] * substeps * args.batch_steps / rb["elapsed"],
-                              "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps, "launch_shape": rb["launch_shape"]}
-        if "adjoint" in legs and args.adjoint_worlds > 0:
-            out["adjoint"] = run_adjoint(api, args.adjoint_worlds, 16, 120, 10)
-        if "tumbler" in legs and args.tumbler_shapes > 0:
-            rt = run_tumbler(api, args.tumbler_shapes, args.tumbler_steps, args.warmup)
-            out["tumbler"] = {"value": rt["bodies"] * substeps * args.tumbler_steps / rt["elapsed"],
-                              "ms_per_step": 1e3 * rt["elapsed"] / args.tumbler_steps}
-        print(json.dumps(out), flush=True)
-        return
-    r = run_gpu(api, "pyramid_grid", args.steps, args.warmup, args.rows, args.cols)
-    value = r["bodies"] * substeps * args.steps / r["elapsed"]
-    ab = algorithmic_bytes(r["bodies"], r["contacts"], substeps)
-    achieved = ab["solve"] / (r["ms_solve"] * 1e-3) / 1e9
-
-    # HBM bytes per k_solve launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes of the same
-    # workload, scaled by the counters' reading of a copy kernel of known size
-    workload = f"pyramid_grid_{args.rows}x{args.cols}_base10_plain_f32"
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"hbm_pyramid_grid_{args.rows}x{args.cols}_base10.json")
-    if os.path.exists(tfile):
-        t = json.load(open(tfile))
-        if t.get("workload") == workload:
-            traffic = t.get("k_solve_bytes_per_launch")
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "bodies×substeps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": workload,
-            "bodies": r["bodies"], "touching_contacts": r["contacts"], "substeps": substeps,
-            "dt": 1.0 / 60.0, "graph_colors": r["colors"],
-            "ms_solve_kernel": r["ms_solve"], "ms_collide_kernel": r["ms_collide"],
-            "ms_step_phases": r["host_ms"],
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_solve", "algorithmic_bytes_per_launch": ab["solve"],
-        },
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.rows, args.cols, args.cpu_threads, args.cpu_steps, args.warmup)
-    if args.batch_worlds > 0:
-        rb = run_gpu_batch(api, args.batch_worlds, args.batch_steps, args.warmup)
-        bval = rb["bodies"] * substeps * args.batch_steps / rb["elapsed"]
-        batched = {
-            "workload": f"batch_pyramids_{args.batch_worlds}_base4-40_plain_f32", "value": bval,
-            "unit": "bodies×substeps/s", "bodies": rb["bodies"], "worlds": args.batch_worlds,
-            "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps,
-            "ms_solve_kernel": rb["ms_solve"], "ms_collide_kernel": rb["ms_collide"], "ms_host": rb["ms_host"],
-            "work_items": rb["work_items"], "launch_shape": rb["launch_shape"], "cpu_baseline": None,
-        }
-        if not args.no_cpu:
-            cb = cpu_baseline_batch(args.cpu_threads, min(128, args.batch_worlds), args.batch_cpu_steps, args.warmup)
-            batched["cpu_baseline"] = cb
-            if cb:
-                batched["vs_cpu"] = bval / cb["value"]
-        out["batched"] = batched
-    if args.adjoint_worlds > 0:
+    host = host_cores()
+    if args.cpu_threads > 0:
+        host["threads"] = args.cpu_threads
+    cpu = not args.no_cpu
+    out = {"metric": METRIC}
+    grid = f"pyramid_grid_{args.rows}x{args.cols}_base10"
+    if "plain" in legs:
+        leg = world_leg(api, grid, "pyramid_grid", {"rows": args.rows, "cols": args.cols, "base": 10, "seed": 7},
+                        args.steps, args.warmup, host, cpu, args.steps, min(args.steps, args.cpu_curve_steps))
+        out.update({
+            "value": leg["value"], "unit": "bodies×substeps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": leg["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": grid, "bodies": leg["bodies"], "touching_contacts": leg["touching_contacts"],
+                       "substeps": SUBSTEPS, "dt": DT, "graph_colors": leg["graph_colors"],
+                       "ms_solve_kernel": leg["ms_solve_kernel"], "ms_collide_kernel": leg["ms_collide_kernel"],
+                       "ms_step_phases": leg["ms_step_phases"], "work_items": leg["work_items"]},
+            "roofline": leg["roofline"], "cpu_baseline": leg["cpu_baseline"],
+        })
+        if leg.get("vs_cpu"):
+            out["vs_cpu"] = leg["vs_cpu"]
+    if "large_pyramid" in legs:
+        # the reference's own benchmark scene (benchmark/large_pyramid.c): one warm-up step, then timed steps
+        out["large_pyramid"] = world_leg(api, "large_pyramid_base100", "large_pyramid", {"base": 100}, args.pyramid_steps, 1,
+                                         host, cpu, args.pyramid_steps, min(args.pyramid_steps, 100))
+    if "batched" in legs and args.batch_worlds > 0:
+        out["batched"] = batched_leg(api, args, host, cpu)
+    if "adjoint" in legs and args.adjoint_worlds > 0:
+        progress("adjoint: taped forward and backward")
         out["adjoint"] = run_adjoint(api, args.adjoint_worlds, 16, 120, 10)
-    if args.tumbler_shapes > 0:
-        rt = run_tumbler(api, args.tumbler_shapes, args.tumbler_steps, args.warmup)
-        tval = rt["bodies"] * substeps * args.tumbler_steps / rt["elapsed"]
-        tumbler = {
-            "workload": f"tumbler_{args.tumbler_shapes}_kinematic_container_plain_f32", "value": tval,
-            "unit": "bodies×substeps/s", "bodies": rt["bodies"], "touching_contacts": rt["contacts"],
-            "graph_colors": rt["colors"], "steps": args.tumbler_steps, "warmup": args.warmup,
-            "ms_per_step": 1e3 * rt["elapsed"] / args.tumbler_steps, "ms_solve_kernel": rt["ms_solve"],
-            "ms_collide_kernel": rt["ms_collide"], "cpu_baseline": None,
-        }
-        if not args.no_cpu:
-            ct = cpu_baseline_tumbler(args.tumbler_shapes, args.cpu_threads, args.tumbler_cpu_steps, args.warmup)
-            tumbler["cpu_baseline"] = ct
-            if ct:
-                tumbler["vs_cpu"] = tval / ct["value"]
-        out["tumbler"] = tumbler
+    if "tumbler" in legs and args.tumbler_shapes > 0:
+        out["tumbler"] = world_leg(api, f"tumbler_{args.tumbler_shapes}", "tumbler", {"count": args.tumbler_shapes, "seed": 5},
+                                   args.tumbler_steps, args.warmup, host, cpu, min(args.tumbler_steps, args.tumbler_cpu_steps),
+                                   min(args.tumbler_steps, args.tumbler_cpu_steps, 30))
+    out["host"] = host
+    progress("done")
     print(json.dumps(out), flush=True)
 
 

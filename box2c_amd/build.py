@@ -39,18 +39,24 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-def build(verbose: bool = True) -> str:
-    objs = []
+def build(verbose: bool = True, jobs: int | None = None) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".", "_") + ".o")
         cmd = [HIPCC, *HIP_FLAGS, "-I", CSRC, "-I", os.path.join(ROOT, "include")]
         if src.endswith(".cpp"):
             cmd += ["-x", "hip"]
         cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # the objects compile independently (one hipcc per source, at most 8 at a time)
+    with ThreadPoolExecutor(max_workers=jobs or min(8, os.cpu_count() or 1)) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-Wl,-Bsymbolic", "-o", LIB, *objs])
     _run(["gcc", "-O2", "-std=gnu17", "-fPIC", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
           "-shared", "-o", SCENES, os.path.join(CSRC, "scenes.c"), "-L", HERE, "-lb2mi",

@@ -213,6 +213,8 @@ class B2Api:
         L.b2Body_GetAngularVelocity.restype = c_float
         L.b2Body_SetLinearVelocity.argtypes = [BodyId, Vec2]
         L.b2Body_SetAngularVelocity.argtypes = [BodyId, c_float]
+        L.b2Body_GetType.argtypes = [BodyId]
+        L.b2Body_GetType.restype = c_int
         L.b2Body_GetMass.argtypes = [BodyId]
         L.b2Body_GetMass.restype = c_float
         L.b2Body_GetInertiaTensor.argtypes = [BodyId]
@@ -403,6 +405,10 @@ class B2Api:
         L.b2mi_Batch_SetSplitSlots.argtypes = [c_int, c_int]
         L.b2mi_World_SetSplitSlots.argtypes = [WorldId, c_int]
         L.b2mi_Batch_GetWorkItemCount.argtypes = [c_int]
+        L.b2mi_Batch_GetWorkItems.argtypes = [c_int, ctypes.c_void_p, c_int]
+        L.b2mi_Batch_GetWorkItems.restype = c_int
+        L.b2mi_World_GetWorkItems.argtypes = [WorldId, ctypes.c_void_p, c_int]
+        L.b2mi_World_GetWorkItems.restype = c_int
         L.b2mi_Batch_GetWorkItemCount.restype = c_int
         L.b2mi_Batch_GetTiming.argtypes = [c_int, POINTER(c_float), POINTER(c_float), POINTER(c_float)]
         L.b2mi_World_GetTiming.argtypes = [WorldId, POINTER(c_float), POINTER(c_float), POINTER(c_float)]
@@ -439,6 +445,16 @@ class B2Api:
         a, b, c = c_float(), c_float(), c_float()
         self.lib.b2mi_Batch_GetTiming(batch, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
+
+    def work_items(self, world: WorldId | None = None, batch: int | None = None) -> np.ndarray:
+        """The last step's work items, one row each: slots, bodies, store, mode (0 one
+        workgroup, 1 one grid launch per phase)."""
+        f, h = (self.lib.b2mi_World_GetWorkItems, world) if world is not None else (self.lib.b2mi_Batch_GetWorkItems, batch)
+        n = f(h, None, 0)
+        out = np.zeros((n, 4), np.int32)
+        if n:
+            f(h, out.ctypes.data, n)
+        return out
 
     def world_timing(self, world: WorldId):
         a, b, c = c_float(), c_float(), c_float()

@@ -153,7 +153,8 @@ struct DevBodySim
 	float linearDamping, angularDamping, gravityScale, sleepThreshold;
 	uint32_t flags;
 	int shapeStart, shapeCount, pad;
-	float pad2[4];
+	float sleepTime; // seconds the body has been below its sleep threshold (src/solver.c:245, 277)
+	float pad2[3];
 };
 static_assert( sizeof( DevBodySim ) == 128, "one cache line" );
 

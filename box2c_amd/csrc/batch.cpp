@@ -280,6 +280,40 @@ B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity )
 	return n;
 }
 
+// The work items of the last solver launch, in launch order: per item (constraint slots, bodies
+// including kinematic shadows, store, how it ran: 0 one workgroup, 1 one grid launch per phase).
+static int workItems( StepGroup* g, int* records, int capacity )
+{
+	std::vector<int> mode( g->hItems.size(), 0 );
+	for ( int i : g->splitItems )
+		mode[i] = 1;
+	int n = 0;
+	for ( size_t i = 0; i < g->hItems.size(); ++i )
+	{
+		const WorkItem& wi = g->hItems[i];
+		if ( records != nullptr && n < capacity )
+		{
+			records[4 * n + 0] = wi.groupStart[kGroups];
+			records[4 * n + 1] = wi.bodyCount;
+			records[4 * n + 2] = wi.store;
+			records[4 * n + 3] = mode[i];
+		}
+		n += 1;
+	}
+	return n;
+}
+
+B2_API int b2mi_Batch_GetWorkItems( int batch, int* records, int capacity )
+{
+	return workItems( getBatch( batch ), records, capacity );
+}
+
+B2_API int b2mi_World_GetWorkItems( b2WorldId worldId, int* records, int capacity )
+{
+	World* w = getWorld( worldId );
+	return w->group != nullptr ? workItems( w->group, records, capacity ) : 0;
+}
+
 // Host clock of the last step (ms since it began) at ten points: pair update started, collide
 // lists started, narrowphase launched, narrowphase wait started, wait returned, solver setup
 // started, solver launched, solver wait started, wait returned, enlarge processing done.

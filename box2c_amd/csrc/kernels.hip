@@ -254,7 +254,8 @@ __device__ inline bool aabb_contains( const float* a, const float* b )
 	return a[0] <= b[0] && a[1] <= b[1] && b[2] <= a[2] && b[3] <= a[3];
 }
 
-// solver.c:165-351 (+ the no-hit branch of continuous, solver.c:1035-1066) for one body
+// solver.c:165-351 (+ the no-hit branch of continuous, solver.c:1035-1066) for one body.
+// enlCount[0] counts the enlarge records, enlCount[1] the bodies at the sleep time.
 __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float4 s1, int worldIndex, EnlargeRecord* enl,
 							   int* enlCount, int enlCapacity )
 {
@@ -300,10 +301,20 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 	bool enableSleep = ( W.flags & kWorldSleep ) != 0 && ( sim.flags & kBodyEnableSleep ) != 0;
 	if ( enableSleep == false || sleepVelocity > sim.sleepThreshold )
 	{
+		sim.sleepTime = 0.0f;
 		const float safetyFactor = 0.5f;
 		if ( ( sim.flags & kBodyDynamic ) && ( W.flags & kWorldContinuous ) && maxVelocity * W.dt > safetyFactor * sim.minExtent )
 			isFast = true;
 	}
+	else
+	{
+		// falling asleep (src/solver.c:271-278)
+		sim.sleepTime += W.dt;
+	}
+	// a body at the sleep time lets its island sleep when the rest of it is there too
+	// (src/solver.c:281-298); the host checks the islands of such bodies
+	if ( sim.sleepTime >= K_TIME_TO_SLEEP )
+		atomicAdd( enlCount + 1, 1 );
 	// center0/rotation0: immediately for normal bodies, after the (no-hit) sweep check for fast ones
 	sim.center0[0] = center.x;
 	sim.center0[1] = center.y;
@@ -682,8 +693,8 @@ template <int Store>
 __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, const int* __restrict__ ovfLevels, float* cfBase,
-												   int cfCapacity, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount,
-												   int enlCapacity, int itemOffset, const int* abortFlag )
+												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
+												   int itemOffset, const int* abortFlag )
 {
 	if ( abortFlag != nullptr && *abortFlag != 0 )
 		return;
@@ -889,7 +900,7 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 
 hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize,
 						 int ldsBytes, const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels,
-						 float* cf, int cfCapacity, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
+						 float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
 						 const int* abortFlag, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
@@ -911,7 +922,7 @@ hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* item
 	}
 #define B2MI_LAUNCH_SOLVE( S )                                                                                                  \
 	hipLaunchKernelGGL( k_solve<S>, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact, \
-						slotBodies, ovfLevels, cf, cfCapacity, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag )
+						slotBodies, ovfLevels, cf, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag )
 	if ( store == kStoreLds )
 		B2MI_LAUNCH_SOLVE( kStoreLds );
 	else if ( store == kStoreBodiesLds )

@@ -14,7 +14,7 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 						   hipStream_t stream );
 hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize,
 						 int ldsBytes, const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels,
-						 float* cf, int cfCapacity, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
+						 float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
 						 const int* abortFlag, hipStream_t stream );
 
 // one phase of a very large work item over a grid (k_split_phase; phases in SplitPhase)

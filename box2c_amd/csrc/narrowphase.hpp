@@ -24,6 +24,7 @@ namespace b2mi
 #define K_LINEAR_SLOP 0.005f
 #define K_SPECULATIVE ( 4.0f * K_LINEAR_SLOP )
 #define K_AABB_MARGIN 0.1f
+#define K_TIME_TO_SLEEP 0.5f // src/core.h:151
 
 B2MI_HD inline float maxr( float a, float b )
 {

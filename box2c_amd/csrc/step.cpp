@@ -983,6 +983,61 @@ static void layoutGroup( StepGroup* g )
 	g->layoutValid = true;
 }
 
+// Every range a solver launch dereferences, checked on the host against the device arrays
+// before the items are uploaded: each item's body entries (body list, private state, masses)
+// and slots (slot lists, constraint records of either record stride) lie inside its world's
+// regions and the arrays' capacities, no two items share a body entry or a slot, overflow level
+// boundaries lie inside the item's overflow group, and an LDS-store item's footprint fits the
+// launch. A work item's own indices (local bodies of each slot, sim indices) are checked where
+// it is built.
+static void validateItems( const StepGroup* g )
+{
+	const int nw = (int)g->worlds.size();
+	std::vector<int2> bodySpans, slotSpans;
+	bodySpans.reserve( g->hItems.size() );
+	slotSpans.reserve( g->hItems.size() );
+	for ( const WorkItem& wi : g->hItems )
+	{
+		B2MI_CHECK( 0 <= wi.world && wi.world < nw );
+		const World* w = g->worlds[wi.world];
+		const int total = wi.groupStart[kGroups];
+		B2MI_CHECK( wi.groupStart[0] == 0 && total >= 0 && wi.bodyCount >= 0 );
+		B2MI_CHECK( w->bodyRegion.base <= wi.bodyStart && wi.bodyStart + wi.bodyCount <= w->bodyRegion.base + w->bodyRegion.cap );
+		B2MI_CHECK( wi.bodyStart + wi.bodyCount <= g->capBodyList && 2 * ( wi.bodyStart + wi.bodyCount ) <= g->capPriv &&
+					wi.bodyStart + wi.bodyCount <= g->capMass );
+		B2MI_CHECK( w->slotRegion.base <= wi.slotStart && wi.slotStart + total <= w->slotRegion.base + w->slotRegion.cap );
+		// HBM records at (slotStart + k) * 7 quads, bodies-in-LDS records at slotStart * 7 + k * 5:
+		// both end before quad (slotStart + total) * 7
+		B2MI_CHECK( wi.slotStart + total <= g->capSlots && wi.slotStart + total <= g->capCons );
+		if ( wi.groupStart[1] > wi.groupStart[0] )
+		{
+			B2MI_CHECK( wi.ovfLevelCount >= 1 && wi.ovfLevelBegin >= 0 &&
+						wi.ovfLevelBegin + wi.ovfLevelCount + 1 <= (int)g->hOvfLevels.size() &&
+						wi.ovfLevelBegin + wi.ovfLevelCount + 1 <= g->capOvfLevels );
+			const int* lv = g->hOvfLevels.data() + wi.ovfLevelBegin;
+			B2MI_CHECK( lv[0] == wi.groupStart[0] && lv[wi.ovfLevelCount] == wi.groupStart[1] );
+			for ( int L = 0; L < wi.ovfLevelCount; ++L )
+				B2MI_CHECK( lv[L] < lv[L + 1] );
+		}
+		B2MI_CHECK( wi.ldsBytes >= 0 && wi.ldsBytes <= kMaxItemLdsBytes );
+		bodySpans.push_back( make_int2( wi.bodyStart, wi.bodyStart + wi.bodyCount ) );
+		slotSpans.push_back( make_int2( wi.slotStart, wi.slotStart + total ) );
+	}
+	auto disjoint = []( std::vector<int2>& spans ) {
+		std::sort( spans.begin(), spans.end(), []( int2 a, int2 b ) { return a.x < b.x; } );
+		for ( size_t i = 1; i < spans.size(); ++i )
+			B2MI_CHECK( spans[i - 1].y <= spans[i].x || spans[i - 1].y == spans[i - 1].x || spans[i].y == spans[i].x );
+	};
+	disjoint( bodySpans );
+	disjoint( slotSpans );
+	for ( const LaunchClass& c : g->classes )
+	{
+		B2MI_CHECK( c.first >= 0 && c.count > 0 && c.first + c.count <= (int)g->hItems.size() );
+		for ( int i = c.first; i < c.first + c.count; ++i )
+			B2MI_CHECK( g->hItems[i].store == c.store && g->hItems[i].ldsBytes <= c.lds && c.lds <= g->maxItemLds );
+	}
+}
+
 template <typename T>
 static void uploadSpan( T* dst, const std::vector<T>& src, int first, int last, hipStream_t s )
 {
@@ -1095,6 +1150,7 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		if ( g->itemCount )
 			HIP_CHECK( hipMemcpyAsync( g->dItems, g->hItems.data(), sizeof( WorkItem ) * g->itemCount, hipMemcpyHostToDevice, s ) );
 		grow( g->dOvfLevels, g->capOvfLevels, std::max( (int)g->hOvfLevels.size(), 1 ) );
+		validateItems( g );
 		if ( !g->hOvfLevels.empty() )
 			HIP_CHECK( hipMemcpyAsync( g->dOvfLevels, g->hOvfLevels.data(), sizeof( int ) * g->hOvfLevels.size(), hipMemcpyHostToDevice, s ) );
 	}
@@ -1273,6 +1329,48 @@ static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
 		}
 }
 
+// Island sleep (src/solver.c:1956-2001, src/solver_set.c:162-455) is not part of this
+// pipeline, so a world with sleep enabled must stop at the first step after which the reference
+// could put an island to sleep, rather than go on with results the reference would not give.
+// The reference's islands persist and are split lazily, but each one is a union of whole
+// components of the touching-contact graph over awake (dynamic and kinematic) bodies; it sleeps
+// only when every body in it has reached the sleep time (src/solver.c:281-298). So if it can
+// sleep, some component has all its bodies at the sleep time, which is what is checked here
+// (called only after a step in which some body reached it).
+static void checkIslandSleep( World* w )
+{
+	const int na = (int)w->awakeSimBody.size();
+	if ( !w->enableSleep || na == 0 )
+		return;
+	std::vector<DevBodySim> sims( na );
+	HIP_CHECK( hipMemcpyAsync( sims.data(), w->dSims, sizeof( DevBodySim ) * na, hipMemcpyDeviceToHost, stream() ) );
+	HIP_CHECK( hipStreamSynchronize( stream() ) );
+	std::vector<int> parent( na );
+	for ( int i = 0; i < na; ++i )
+		parent[i] = i;
+	for ( int c = 0; c < kColorCount; ++c )
+		for ( int id : w->colors[c] )
+		{
+			const Contact& ct = w->contacts[id];
+			const Body& a = w->bodies[ct.edges[0].bodyId];
+			const Body& b = w->bodies[ct.edges[1].bodyId];
+			if ( a.setIndex != kAwakeSet || b.setIndex != kAwakeSet )
+				continue;
+			int ra = findRoot( parent, a.localIndex ), rb = findRoot( parent, b.localIndex );
+			if ( ra != rb )
+				parent[std::max( ra, rb )] = std::min( ra, rb );
+		}
+	std::vector<char> awake( na, 0 );
+	for ( int i = 0; i < na; ++i )
+		if ( sims[i].sleepTime < 0.5f ) // b2_timeToSleep, src/core.h:151
+			awake[findRoot( parent, i )] = 1;
+	for ( int i = 0; i < na; ++i )
+		if ( findRoot( parent, i ) == i && !awake[i] )
+			fatal( "an island has reached the sleep time: island sleep is not supported by the device pipeline "
+				   "(create the world with b2WorldDef.enableSleep = false, or bodies with b2BodyDef.enableSleep = false)",
+				   __FILE__, __LINE__ );
+}
+
 static float msSince( std::chrono::steady_clock::time_point t )
 {
 	return std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t ).count();
@@ -1280,6 +1378,10 @@ static float msSince( std::chrono::steady_clock::time_point t )
 
 // Fork-join pool of host threads for per-world work; every task touches one world only. The
 // calling thread takes part; with one task, or one hardware thread, everything runs inline.
+// One caller at a time owns the pool: a call made while another thread's call holds it (worlds
+// stepped from several threads, which the API allows for different worlds), or made from inside
+// a task, runs its tasks inline on the calling thread, so concurrent callers never share the
+// pool's task state.
 class HostPool
 {
 public:
@@ -1291,12 +1393,24 @@ public:
 
 	void run( int n, const std::function<void( int )>& fn )
 	{
-		if ( n <= 1 || threads_.empty() )
+		std::unique_lock<std::mutex> owner( owner_, std::defer_lock );
+		if ( n <= 1 || threads_.empty() || insidePool() || !owner.try_lock() )
 		{
 			for ( int i = 0; i < n; ++i )
 				fn( i );
 			return;
 		}
+		struct Inside
+		{
+			Inside()
+			{
+				insidePool() = true;
+			}
+			~Inside()
+			{
+				insidePool() = false;
+			}
+		} inside;
 		{
 			std::lock_guard<std::mutex> lk( m_ );
 			task_ = &fn;
@@ -1331,8 +1445,15 @@ private:
 		for ( std::thread& t : threads_ )
 			t.join();
 	}
+	// set on the thread that owns the pool for the duration of its call, and on the workers
+	static bool& insidePool()
+	{
+		static thread_local bool inside = false;
+		return inside;
+	}
 	void worker()
 	{
+		insidePool() = true;
 		uint64_t seen = 0;
 		for ( ;; )
 		{
@@ -1353,6 +1474,7 @@ private:
 	}
 
 	std::vector<std::thread> threads_;
+	std::mutex owner_; // held by the caller whose tasks the pool runs
 	std::mutex m_;
 	std::condition_variable cv_, done_;
 	const std::function<void( int )>* task_ = nullptr;
@@ -1460,7 +1582,7 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		int block = blockSize > 0 ? blockSize : c.block;
 		int lds = blockSize > 0 ? maxItemLds : c.lds;
 		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
-								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
+								 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
 	};
 	B2MI_CHECK( splitItems.empty() || abortFlag == nullptr );
 	for ( int item : splitItems )
@@ -1556,7 +1678,7 @@ void StepGroup::launchSolveChunk( int chunk, hipEvent_t ready, const int* abortF
 			HIP_CHECK( hipStreamWaitEvent( as, ready, 0 ) );
 		used[k] = true;
 		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
-								 dOvfLevels, dCons, capCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, as ) );
+								 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, as ) );
 	}
 	for ( int k = 0; k < kAuxStreams; ++k )
 		if ( used[k] )
@@ -1869,7 +1991,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 			joinAux( s );
 		}
 		HIP_CHECK( hipEventRecord( e2, s ) );
-		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
 		hostPhases[7] = msSince( t0 );
 		HIP_CHECK( hipStreamSynchronize( s ) );
 	}
@@ -1898,6 +2020,9 @@ void StepGroup::step( float timeStep, int subStepCount )
 				markActive( worlds[i] );
 	}
 	msEnlarge = msSince( te );
+	if ( hCounters[2] > 0 )
+		for ( World* w : worlds )
+			checkIslandSleep( w );
 	hostPhases[9] = msSince( t0 );
 	HIP_CHECK( hipEventElapsedTime( &msCollide, e0, e1 ) );
 	HIP_CHECK( hipEventElapsedTime( &msSolve, e3, e2 ) );

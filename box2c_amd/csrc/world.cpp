@@ -902,6 +902,15 @@ B2_API float b2Body_GetAngularVelocity( b2BodyId bodyId )
 	return w->awakeState[8 * body->localIndex + 2];
 }
 
+// The overflow levels of the work items count a kinematic body as changed by the overflow
+// stages when one of its velocity components is -0 (v - 0 * P can turn it into +0); a new
+// velocity of an awake kinematic body therefore rebuilds the items.
+static void kinematicVelocitySet( World* w, const Body* body )
+{
+	if ( body->type == b2_kinematicBody )
+		w->topologyDirty = true;
+}
+
 B2_API void b2Body_SetLinearVelocity( b2BodyId bodyId, b2Vec2 v )
 {
 	B2MI_BODY( bodyId );
@@ -911,6 +920,7 @@ B2_API void b2Body_SetLinearVelocity( b2BodyId bodyId, b2Vec2 v )
 	float* st = &w->awakeState[8 * body->localIndex];
 	st[0] = v.x;
 	st[1] = v.y;
+	kinematicVelocitySet( w, body );
 }
 
 B2_API void b2Body_SetAngularVelocity( b2BodyId bodyId, float av )
@@ -920,6 +930,7 @@ B2_API void b2Body_SetAngularVelocity( b2BodyId bodyId, float av )
 		return;
 	beginEdit( w );
 	w->awakeState[8 * body->localIndex + 2] = av;
+	kinematicVelocitySet( w, body );
 }
 
 // src/body.c:799-904

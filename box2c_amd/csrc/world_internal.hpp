@@ -284,7 +284,8 @@ inline int chunkOfWorld( int worldIndex, int worldCount, int chunks )
 {
 	return (int)( (int64_t)worldIndex * chunks / worldCount );
 }
-// device counters of a step: [1] enlarged shapes, [kChunkCounter + c] state changes of chunk c
+// device counters of a step: [1] enlarged shapes, [2] bodies at the sleep time, [kChunkCounter + c]
+// state changes of chunk c
 constexpr int kChunkCounter = 4;
 constexpr int kCounterCount = kChunkCounter + kMaxChunks;
 
@@ -310,7 +311,7 @@ struct StepGroup
 	bool layoutValid = false; // regions of the worlds in the launch arrays are assigned
 	ChangeRecord* dChanges = nullptr;
 	int capChanges = 0;
-	int* dCounters = nullptr; // [0] changes, [1] enlarge
+	int* dCounters = nullptr; // kCounterCount counters of a step (above)
 	int* hCounters = nullptr; // pinned
 	EnlargeRecord* dEnlarge = nullptr;
 	int capEnlarge = 0;

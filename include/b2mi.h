@@ -117,6 +117,13 @@ B2_API void b2mi_Batch_GetHostTimeline( int batch, float* marks );
 // most `capacity` records.
 B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity );
 
+// The work items of the last step's solver, in launch order, 4 ints each: constraint slots,
+// bodies (kinematic shadows included), store (0 HBM, 1 bodies in LDS, 2 LDS) and how the item
+// ran (0 one workgroup, 1 one grid launch per solver phase). Returns the item count and writes
+// at most `capacity` records.
+B2_API int b2mi_Batch_GetWorkItems( int batch, int* records, int capacity );
+B2_API int b2mi_World_GetWorkItems( b2WorldId worldId, int* records, int capacity );
+
 // Device time of the last step's narrowphase and solver launches and the remaining host
 // time, in milliseconds.
 B2_API void b2mi_Batch_GetTiming( int batch, float* msCollide, float* msSolve, float* msHost );

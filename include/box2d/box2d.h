@@ -1,8 +1,10 @@
 // Public C API of the MI355X step pipeline: the world / body / shape / step subset of the
 // reference's include/box2d/box2d.h. Each declaration keeps the reference's name, signature
 // and argument meaning; the reference line that declares it is cited so a maintainer can
-// check the binding. Everything on the step path (pairs, narrowphase, solver, finalize)
-// executes as HIP kernels on the GPU; see b2mi.h for the device/batch/adjoint extensions.
+// check the binding. In a step, the narrowphase, the solver stages and finalize run as HIP
+// kernels on the GPU; the order-defining bookkeeping (broadphase pair update and dynamic-tree
+// maintenance, contact creation, begin/end-touch processing and graph colouring, work-item
+// building) runs on the host. See b2mi.h for the device/batch/adjoint extensions.
 #pragma once
 
 #include "base.h"

@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <mutex>
 #include <cstdlib>
 #include <sys/mman.h>
 #include <ucontext.h>
@@ -51,7 +52,9 @@ struct Launch
 };
 
 thread_local Launch* g_launch = nullptr;
+// fiber stacks for reuse, shared by launches from every host thread
 std::vector<void*> g_stackPool;
+std::mutex g_stackMutex;
 
 void fiberEntry()
 {
@@ -69,6 +72,7 @@ void fiberEntry()
 
 void* takeStack()
 {
+	std::lock_guard<std::mutex> lock( g_stackMutex );
 	if ( !g_stackPool.empty() )
 	{
 		void* p = g_stackPool.back();
@@ -198,8 +202,11 @@ void emu_launch( dim3 grid, dim3 block, size_t sharedBytes, const std::function<
 #if EMU_ASAN
 	ASAN_UNPOISON_MEMORY_REGION( shared, b2mi::kEmuSharedBytes );
 #endif
-	for ( Fiber& f : L.fibers )
-		g_stackPool.push_back( f.stack );
+	{
+		std::lock_guard<std::mutex> lock( g_stackMutex );
+		for ( Fiber& f : L.fibers )
+			g_stackPool.push_back( f.stack );
+	}
 	g_launch = outer;
 }
 
@@ -217,6 +224,7 @@ hipError_t hipGetDeviceCount( int* count )
 	*count = 1;
 	return hipSuccess;
 }
+
 hipError_t hipMalloc( void** p, size_t bytes )
 {
 	*p = malloc( bytes ? bytes : 1 );

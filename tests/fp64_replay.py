@@ -60,6 +60,44 @@ def export(rollout):
     return steps, wp.astype(np.float64), mass, inertia, world_of
 
 
+def export_worlds(rollout, worlds):
+    """export() restricted to the given worlds (renumbered 0..len-1 in the given order, bodies
+    likewise): worlds are independent, so the restriction replays exactly those worlds. Slots are
+    read through a numpy view of each step's array, so a large batch exports quickly."""
+    L = rollout.api.lib
+    L.b2mi_Batch_GetTapeSlots.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    L.b2mi_Batch_GetTapeSlots.restype = ctypes.c_int
+    L.b2mi_Batch_GetTapeWorldParams.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    b, boxes = rollout.batch, rollout.boxes
+    worlds = [int(i) for i in worlds]
+    remap = {w: k for k, w in enumerate(worlds)}
+    steps = []
+    for t in range(L.b2mi_Batch_GetTapeStepCount(b)):
+        n = L.b2mi_Batch_GetTapeSlots(b, t, None, 0)
+        arr = (TapeSlot * max(n, 1))()
+        L.b2mi_Batch_GetTapeSlots(b, t, arr, n)
+        rec = np.ctypeslib.as_array(arr)[:n].view(np.recarray)
+        keep = np.isin(rec.world, worlds)
+        sub = rec[keep].copy()
+        wk = np.array([remap[int(x)] for x in sub.world], np.int32)
+        for field in ("bodyA", "bodyB"):
+            v = getattr(sub, field)
+            local = np.where(v >= 0, v - sub.world * boxes + wk * boxes, -1)
+            setattr(sub, field, local)
+        sub.world = wk
+        steps.append(list(sub))
+    wp = np.zeros((rollout.worlds, WORLD_PARAMS), np.float32)
+    L.b2mi_Batch_GetTapeWorldParams(b, wp.ctypes.data)
+    n = L.b2mi_Batch_GetTapeBodies(b, None, 0)
+    ids = (BodyId * n)()
+    L.b2mi_Batch_GetTapeBodies(b, ids, n)
+    rows = [w * boxes + k for w in worlds for k in range(boxes)]
+    mass = np.array([L.b2Body_GetMass(ids[i]) for i in rows], np.float64)
+    inertia = np.array([L.b2Body_GetInertiaTensor(ids[i]) for i in rows], np.float64)
+    world_of = np.repeat(np.arange(len(worlds)), boxes)
+    return steps, wp[worlds].astype(np.float64), mass, inertia, world_of
+
+
 def _cross(r, p):
     return r[..., 0] * p[..., 1] - r[..., 1] * p[..., 0]
 

@@ -128,3 +128,24 @@ def make_world(api: B2Api, name: str, kw: dict, overrides: dict):
     for k, v in overrides.items():
         setattr(d, k, v)
     return api.scene(name, d, **kw)
+
+
+def stability(bodies: np.ndarray, contacts: np.ndarray, start: np.ndarray | None) -> dict:
+    """Aggregate measures of a stack from body and contact dumps: the highest dynamic body
+    center, the largest horizontal drift of a dynamic body from `start` (body records of the
+    same world at its start; skipped when None), the deepest penetration over the manifold points
+    of touching contacts, and the kinetic energy of unit-density half-extent-0.5 boxes
+    (m = 1, I = 1/6)."""
+    dyn = bodies["type"] == 2
+    out = {"top": float(bodies["center"][dyn, 1].max()) if dyn.any() else 0.0}
+    if start is not None:
+        sd = start["type"] == 2
+        out["drift"] = float(np.abs(bodies["center"][dyn, 0] - start["center"][sd, 0]).max())
+    touching = contacts["colorIndex"] >= 0
+    pc = contacts["pointCount"][touching]
+    sep = contacts["points"]["separation"][touching]
+    live = np.arange(2)[None, :] < pc[:, None]
+    out["penetration"] = float(max(0.0, -sep[live].min())) if live.any() else 0.0
+    v, w = bodies["v"][dyn].astype(np.float64), bodies["w"][dyn].astype(np.float64)
+    out["kinetic"] = float(0.5 * (v * v).sum() + 0.5 * (w * w).sum() / 6.0)
+    return out
