@@ -892,6 +892,27 @@ __global__ void k_patch_params( const DevContact* __restrict__ src, DevContact* 
 	dst[k] = c;
 }
 
+// poses between the API's rows of 7 floats (center x, y, rotation c, s, velocity x, y, angular
+// velocity) and StepPose records
+__global__ void k_poses_from_rows( const float* __restrict__ rows, StepPose* __restrict__ poses, int n )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= n )
+		return;
+	const float* e = rows + 7 * (size_t)i;
+	poses[i] = StepPose{ e[0], e[1], e[2], e[3], e[4], e[5], e[6], 0.0f };
+}
+
+__global__ void k_rows_from_poses( const StepPose* __restrict__ poses, float* __restrict__ rows, int n )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= n )
+		return;
+	StepPose p = poses[i];
+	float* o = rows + 7 * (size_t)i;
+	o[0] = p.cx, o[1] = p.cy, o[2] = p.qc, o[3] = p.qs, o[4] = p.vx, o[5] = p.vy, o[6] = p.w;
+}
+
 __global__ void k_fill_zero( float* __restrict__ p, size_t n )
 {
 	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -991,6 +1012,22 @@ hipError_t launch_patch_params( const DevContact* src, DevContact* dst, const in
 		return hipSuccess;
 	hipLaunchKernelGGL( k_patch_params, dim3( blocksFor( slotCount, 256 ) ), dim3( 256 ), 0, stream, src, dst, slotWorld, slotCount,
 						params );
+	return hipGetLastError();
+}
+
+hipError_t launch_poses_from_rows( const float* rows, StepPose* poses, int n, hipStream_t stream )
+{
+	if ( n <= 0 )
+		return hipSuccess;
+	hipLaunchKernelGGL( k_poses_from_rows, dim3( blocksFor( n, 256 ) ), dim3( 256 ), 0, stream, rows, poses, n );
+	return hipGetLastError();
+}
+
+hipError_t launch_rows_from_poses( const StepPose* poses, float* rows, int n, hipStream_t stream )
+{
+	if ( n <= 0 )
+		return hipSuccess;
+	hipLaunchKernelGGL( k_rows_from_poses, dim3( blocksFor( n, 256 ) ), dim3( 256 ), 0, stream, poses, rows, n );
 	return hipGetLastError();
 }
 

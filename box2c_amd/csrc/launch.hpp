@@ -43,5 +43,8 @@ hipError_t launch_mark_slots( const int* slotContact, const int* slotWorld, int 
 hipError_t launch_patch_params( const DevContact* src, DevContact* dst, const int* slotWorld, int slotCount, const float* params,
 								hipStream_t stream );
 hipError_t launch_fill_zero( float* p, size_t n, hipStream_t stream );
+// poses between rows of 7 floats (the API's layout) and StepPose records
+hipError_t launch_poses_from_rows( const float* rows, StepPose* poses, int n, hipStream_t stream );
+hipError_t launch_rows_from_poses( const StepPose* poses, float* rows, int n, hipStream_t stream );
 
 } // namespace b2mi

@@ -10,6 +10,7 @@
 #include "world_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace b2mi
@@ -400,6 +401,129 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 	devRelease( dParams );
 }
 
+// ---- the replay without allocation --------------------------------------------------------
+// b2mi_Batch_ReplayTapeNoAlloc keeps everything tapeReplay allocates per call (pose buffers,
+// constraint records, private body state, masses, patched contacts, parameters) in one fixed
+// array of device memory, `scratch`. Each GPU holds its own copy of it until the process ends;
+// its device address is looked up once per device and only that address is used on the host.
+// Every call on a GPU reuses that GPU's array, so calls that overlap on one GPU (from several
+// host threads, or on streams that are not ordered) race on it.
+constexpr size_t kReplayScratchBytes = size_t( B2MI_REPLAY_SCRATCH_BYTES );
+static __device__ __align__( 256 ) unsigned char scratch[kReplayScratchBytes];
+
+static unsigned char* replayScratch()
+{
+	static std::mutex lock;
+	static std::vector<void*> perDevice;
+	int device = 0;
+	HIP_CHECK( hipGetDevice( &device ) );
+	std::lock_guard<std::mutex> guard( lock );
+	if ( (int)perDevice.size() <= device )
+		perDevice.resize( device + 1, nullptr );
+	if ( perDevice[device] == nullptr )
+	{
+		void* p = nullptr;
+		hipError_t e = hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) );
+		if ( e != hipSuccess || p == nullptr )
+		{
+			fatal( "the replay scratch array has no device address", __FILE__, __LINE__ );
+			return nullptr;
+		}
+		perDevice[device] = p;
+	}
+	return static_cast<unsigned char*>( perDevice[device] );
+}
+
+// the scratch layout of a replay: byte offsets of its buffers, each 256-byte aligned
+struct ReplayLayout
+{
+	size_t rows, poseA, poseB, cf, priv, mass, patched, params, total;
+};
+
+static ReplayLayout replayLayout( const Tape* t, int worldCount, bool patched )
+{
+	int maxSlots = 1, maxBodies = 1;
+	for ( const TapeVersion& v : t->versions )
+	{
+		maxSlots = std::max( maxSlots, v.slotCount );
+		maxBodies = std::max( maxBodies, v.bodyListCount );
+	}
+	const size_t n = (size_t)std::max( t->totalSims, 1 );
+	ReplayLayout L{};
+	size_t at = 0;
+	auto take = [&]( size_t bytes ) {
+		size_t p = at;
+		at += ( bytes + 255 ) / 256 * 256;
+		return p;
+	};
+	L.rows = take( sizeof( float ) * 7 * n );
+	L.poseA = take( sizeof( StepPose ) * n );
+	L.poseB = take( sizeof( StepPose ) * n );
+	L.cf = take( sizeof( float ) * cfFloatCount * (size_t)maxSlots );
+	L.priv = take( sizeof( float4 ) * 2 * (size_t)maxBodies );
+	L.mass = take( sizeof( float2 ) * (size_t)maxBodies );
+	L.patched = patched ? take( sizeof( DevContact ) * (size_t)maxSlots ) : 0;
+	L.params = take( sizeof( float ) * 2 * (size_t)std::max( worldCount, 1 ) );
+	L.total = at;
+	return L;
+}
+
+// tapeReplay's steps on buffers carved from the scratch array; no allocation of any kind
+static bool tapeReplayNoAlloc( StepGroup* g, const float* startPoses, const float* worldParams, float* endPoses )
+{
+	Tape* t = g->tape;
+	const int W = (int)g->worlds.size();
+	const ReplayLayout L = replayLayout( t, W, worldParams != nullptr );
+	if ( L.total > kReplayScratchBytes )
+	{
+		fatal( "b2mi_Batch_ReplayTapeNoAlloc: the taped run needs more scratch than B2MI_REPLAY_SCRATCH_BYTES; use "
+			   "b2mi_Batch_ReplayTape",
+			   __FILE__, __LINE__ );
+		return false;
+	}
+	unsigned char* base = replayScratch();
+	if ( base == nullptr )
+		return false;
+	hipStream_t s = stream();
+	const int T = (int)t->steps.size();
+	const int n = t->totalSims;
+	int maxSlots = 1;
+	for ( const TapeVersion& v : t->versions )
+		maxSlots = std::max( maxSlots, v.slotCount );
+	float* rows = reinterpret_cast<float*>( base + L.rows );
+	StepPose* dA = reinterpret_cast<StepPose*>( base + L.poseA );
+	StepPose* dB = reinterpret_cast<StepPose*>( base + L.poseB );
+	float* dCf = reinterpret_cast<float*>( base + L.cf );
+	float4* dPriv = reinterpret_cast<float4*>( base + L.priv );
+	float2* dMass = reinterpret_cast<float2*>( base + L.mass );
+	DevContact* dPatched = worldParams ? reinterpret_cast<DevContact*>( base + L.patched ) : nullptr;
+	float* dParams = reinterpret_cast<float*>( base + L.params );
+	if ( n > 0 )
+		HIP_CHECK( hipMemcpyAsync( rows, startPoses, sizeof( float ) * 7 * (size_t)n, hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( launch_poses_from_rows( rows, dA, n, s ) );
+	if ( worldParams )
+		HIP_CHECK( hipMemcpyAsync( dParams, worldParams, sizeof( float ) * 2 * W, hipMemcpyHostToDevice, s ) );
+	for ( int step = 0; step < T; ++step )
+	{
+		const TapeStep& ts = t->steps[step];
+		const TapeVersion& v = t->versions[ts.version];
+		const DevContact* snap = ts.dSnap;
+		if ( worldParams )
+		{
+			HIP_CHECK( launch_patch_params( ts.dSnap, dPatched, v.dSlotWorld, v.slotCount, dParams, s ) );
+			snap = dPatched;
+		}
+		HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, snap, dA, dB, t->dSimBase,
+									   dCf, maxSlots, dPriv, dMass, s ) );
+		std::swap( dA, dB );
+	}
+	HIP_CHECK( launch_rows_from_poses( dA, rows, n, s ) );
+	if ( n > 0 )
+		HIP_CHECK( hipMemcpyAsync( endPoses, rows, sizeof( float ) * 7 * (size_t)n, hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	return true;
+}
+
 StepGroup* b2mi_internal_batch( int batch );
 
 } // namespace b2mi
@@ -460,6 +584,13 @@ B2_API void b2mi_Batch_ReplayTape( int batch, const float* startPoses, const flo
 	StepGroup* g = b2mi_internal_batch( batch );
 	B2MI_CHECK( g->tape != nullptr );
 	tapeReplay( g, startPoses, worldParams, endPoses );
+}
+
+B2_API int b2mi_Batch_ReplayTapeNoAlloc( int batch, const float* startPoses, const float* worldParams, float* endPoses )
+{
+	StepGroup* g = b2mi_internal_batch( batch );
+	B2MI_CHECK( g->tape != nullptr );
+	return tapeReplayNoAlloc( g, startPoses, worldParams, endPoses ) ? 1 : 0;
 }
 
 B2_API int b2mi_Batch_GetTapeSlots( int batch, int step, b2mi_TapeSlot* slots, int capacity )

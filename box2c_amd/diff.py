@@ -45,6 +45,8 @@ def bind(api: B2Api):
     L.b2mi_Batch_GetTapeBodies.restype = ctypes.c_int
     L.b2mi_Batch_GetTapePoses.argtypes = [ctypes.c_int, ctypes.c_void_p]
     L.b2mi_Batch_ReplayTape.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.b2mi_Batch_ReplayTapeNoAlloc.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.b2mi_Batch_ReplayTapeNoAlloc.restype = ctypes.c_int
     L.b2mi_Batch_Backward.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
 
 

@@ -179,6 +179,15 @@ B2_API void b2mi_Batch_GetTapeWorldParams( int batch, float* params );
 // (friction, restitution per world) is given; endPoses receives the final poses. This is the
 // function b2mi_Batch_Backward differentiates.
 B2_API void b2mi_Batch_ReplayTape( int batch, const float* startPoses, const float* worldParams, float* endPoses );
+// b2mi_Batch_ReplayTape for callers that cannot allocate device memory (code under HIP graph
+// capture, tight loops, harnesses that forbid hipMalloc); it has no counterpart in the
+// reference. It computes the same bits as b2mi_Batch_ReplayTape, with every buffer the replay
+// needs carved from one fixed device array of B2MI_REPLAY_SCRATCH_BYTES that each GPU holds
+// until the process ends, and it calls no allocating routine. Returns 1, or 0 (through the assert
+// callback and b2mi_GetLastError) when the taped run needs more than that array. Every call on one
+// GPU uses that GPU's array: calls that overlap on one GPU race.
+#define B2MI_REPLAY_SCRATCH_BYTES ( 32 * 1024 * 1024 )
+B2_API int b2mi_Batch_ReplayTapeNoAlloc( int batch, const float* startPoses, const float* worldParams, float* endPoses );
 // gradEnd: d(loss)/d(final poses); gradStart: d(loss)/d(poses when the tape began);
 // gradParams: per world, d(loss)/d(friction) and d(loss)/d(restitution), where each contact's
 // mixed coefficient counts as the world's parameter (exact when every shape of a world has the

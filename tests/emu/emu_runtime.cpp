@@ -224,7 +224,16 @@ hipError_t hipGetDeviceCount( int* count )
 	*count = 1;
 	return hipSuccess;
 }
-
+hipError_t hipGetDevice( int* device )
+{
+	*device = 0;
+	return hipSuccess;
+}
+hipError_t hipGetSymbolAddress( void** p, const void* symbol )
+{
+	*p = const_cast<void*>( symbol );
+	return hipSuccess;
+}
 hipError_t hipMalloc( void** p, size_t bytes )
 {
 	*p = malloc( bytes ? bytes : 1 );

@@ -17,6 +17,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__( ... )
+#define __align__( n ) alignas( n )
 // workgroups run one at a time on the launching thread, so its thread-local storage is the
 // workgroup's shared memory
 #define __shared__ thread_local
@@ -143,6 +144,10 @@ typedef ihipEvent_t* hipEvent_t;
 #define hipHostMallocDefault 0u
 
 hipError_t hipGetDeviceCount( int* count );
+hipError_t hipGetDevice( int* device );
+// a __device__ variable is an ordinary host variable here, so its device address is its address
+hipError_t hipGetSymbolAddress( void** p, const void* symbol );
+#define HIP_SYMBOL( x ) ( &( x ) )
 hipError_t hipMalloc( void** p, size_t bytes );
 hipError_t hipFree( void* p );
 hipError_t hipHostMalloc( void** p, size_t bytes, unsigned flags );
