@@ -21,7 +21,7 @@ SCENES = os.path.join(HERE, "libb2mi_scenes.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("B2MI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "adjoint.hip", "world.cpp", "step.cpp", "batch.cpp", "tape.cpp", "tree.cpp", "geometry.cpp",
+SOURCES = ["kernels.hip", "adjoint.hip", "broadphase.hip", "world.cpp", "step.cpp", "batch.cpp", "tape.cpp", "tree.cpp", "geometry.cpp",
            "collide_api.cpp"]
 
 HIP_FLAGS = [

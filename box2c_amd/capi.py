@@ -404,6 +404,13 @@ class B2Api:
         L.b2mi_Batch_SetWorldChunks.argtypes = [c_int, c_int]
         L.b2mi_Batch_SetSplitSlots.argtypes = [c_int, c_int]
         L.b2mi_World_SetSplitSlots.argtypes = [WorldId, c_int]
+        L.b2mi_Batch_SetGridBlocks.argtypes = [c_int, c_int]
+        L.b2mi_World_SetGridBlocks.argtypes = [WorldId, c_int]
+        L.b2mi_World_SetDevicePairs.argtypes = [WorldId, c_int]
+        L.b2mi_Batch_SetLaunchGraphs.argtypes = [c_int, c_int]
+        L.b2mi_World_SetLaunchGraphs.argtypes = [WorldId, c_int]
+        L.b2mi_Batch_GetLaunchGraphStats.argtypes = [c_int, POINTER(c_int), POINTER(c_int)]
+        L.b2mi_World_GetLaunchGraphStats.argtypes = [WorldId, POINTER(c_int), POINTER(c_int)]
         L.b2mi_Batch_GetWorkItemCount.argtypes = [c_int]
         L.b2mi_Batch_GetWorkItems.argtypes = [c_int, ctypes.c_void_p, c_int]
         L.b2mi_Batch_GetWorkItems.restype = c_int
@@ -455,6 +462,15 @@ class B2Api:
         if n:
             f(h, out.ctypes.data, n)
         return out
+
+    def graph_stats(self, world: WorldId | None = None, batch: int | None = None):
+        """(captures, replays) of the solver's launch graphs so far."""
+        a, b = c_int(), c_int()
+        if world is not None:
+            self.lib.b2mi_World_GetLaunchGraphStats(world, ctypes.byref(a), ctypes.byref(b))
+        else:
+            self.lib.b2mi_Batch_GetLaunchGraphStats(batch, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def world_timing(self, world: WorldId):
         a, b, c = c_float(), c_float(), c_float()

@@ -246,8 +246,14 @@ enum SplitPhase : int
 	kSplitFinalize,
 	kSplitPhaseCount
 };
-// default slot count from which an HBM-store work item runs phase by phase over the grid
+// default slot count from which an HBM-store work item runs over the grid (a cooperative launch,
+// or one launch per phase where cooperative launches are not available): in a batch, and in a
+// group of one world (measured on the 5,050-box pyramid's 14,950-slot island: 1.11 ms of solver
+// time in one workgroup, 0.41 ms over the grid)
 constexpr int kDefaultSplitSlots = 16384;
+constexpr int kDefaultSplitSlotsWorld = 4096;
+// threads per workgroup of the cooperative grid solver
+constexpr int kGridSolveBlock = 256;
 
 // Where a work item keeps its solver state during the step
 enum ItemStore : int
@@ -333,6 +339,54 @@ constexpr int pointField( int field, int p )
 {
 	return field + 4 * p;
 }
+
+// ---- broadphase pair queries on the device -------------------------------------------------
+
+// A node of a dynamic tree as the pair queries read it (uploaded from the host's trees, which keep
+// the reference's node layout and structure): its fat AABB, children (-1, -1 for a leaf), the
+// shape id of a leaf and its category bits.
+struct DevTreeNode
+{
+	float lowerX, lowerY, upperX, upperY;
+	int child1, child2;
+	int userData;
+	uint32_t categoryBits;
+};
+static_assert( sizeof( DevTreeNode ) == 32, "tree node layout" );
+
+// What the pair filters read of a shape: its body, the body's type and the shape's filter.
+struct DevShapeFilter
+{
+	int bodyId;
+	int bodyType; // b2BodyType
+	uint32_t categoryBits, maskBits;
+	int groupIndex;
+	int pad;
+};
+
+// the device side of one world's broadphase, passed to k_find_pairs by value
+struct DevBroadphase
+{
+	const DevTreeNode* nodes[3]; // per tree (b2BodyType order)
+	int roots[3];
+	const DevShapeFilter* shapes; // by shape id
+	const uint32_t* moved;		  // bitmap over proxy keys of the moved proxies
+	const unsigned long long* pairs; // open-addressing set of the shape pair keys with a contact
+	uint32_t pairMask;				 // table size - 1 (a power of two)
+};
+
+// a new pair found by the query of move `query`, the `rank`-th pair that query found
+struct PairRecord
+{
+	int query, rank, shapeIdA, shapeIdB;
+};
+
+// pair-set slots: never a key (shape ids are below 2^31)
+constexpr unsigned long long kPairEmpty = ~0ull;
+constexpr unsigned long long kPairTombstone = ~0ull - 1;
+// threads per workgroup of k_find_pairs and depth of each thread's traversal stack (in LDS)
+constexpr int kPairBlock = 128;
+constexpr int kPairStack = 64;
 
 // Pose of a body at a step boundary (taped runs): center of mass, rotation, velocities
 struct StepPose

@@ -237,6 +237,63 @@ B2_API void b2mi_World_SetSplitSlots( b2WorldId worldId, int slots )
 	setSplitSlots( worldGroup( worldId ), slots );
 }
 
+// workgroups of a very large item's cooperative launch (0 = automatic, -1 = one launch per phase)
+static void setGridBlocks( StepGroup* g, int blocks )
+{
+	g->gridBlocks = blocks < 0 ? -1 : blocks;
+	for ( World* w : g->worlds )
+		w->topologyDirty = true;
+	g->allActive = true;
+}
+
+B2_API void b2mi_Batch_SetGridBlocks( int batch, int blocks )
+{
+	setGridBlocks( getBatch( batch ), blocks );
+}
+
+B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks )
+{
+	setGridBlocks( worldGroup( worldId ), blocks );
+}
+
+B2_API void b2mi_World_GetDevicePairStats( b2WorldId worldId, float* stats )
+{
+	World* w = getWorld( worldId );
+	for ( int i = 0; i < 8; ++i )
+		stats[i] = w->devicePairs.phaseMs[i];
+}
+
+B2_API void b2mi_World_SetDevicePairs( b2WorldId worldId, int minMoves )
+{
+	getWorld( worldId )->devicePairsMin = minMoves < 0 ? -1 : minMoves;
+}
+
+B2_API void b2mi_Batch_SetLaunchGraphs( int batch, int enabled )
+{
+	getBatch( batch )->launchGraphs = enabled != 0;
+}
+
+B2_API void b2mi_World_SetLaunchGraphs( b2WorldId worldId, int enabled )
+{
+	worldGroup( worldId )->launchGraphs = enabled != 0;
+}
+
+static void graphStats( const StepGroup* g, int* captures, int* replays )
+{
+	*captures = g ? g->graphCaptures : 0;
+	*replays = g ? g->graphReplays : 0;
+}
+
+B2_API void b2mi_Batch_GetLaunchGraphStats( int batch, int* captures, int* replays )
+{
+	graphStats( getBatch( batch ), captures, replays );
+}
+
+B2_API void b2mi_World_GetLaunchGraphStats( b2WorldId worldId, int* captures, int* replays )
+{
+	graphStats( getWorld( worldId )->group, captures, replays );
+}
+
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes )
 {
 	StepGroup* g = worldGroup( worldId );
@@ -281,12 +338,13 @@ B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity )
 }
 
 // The work items of the last solver launch, in launch order: per item (constraint slots, bodies
-// including kinematic shadows, store, how it ran: 0 one workgroup, 1 one grid launch per phase).
+// including kinematic shadows, store, how it ran: 0 one workgroup, 1 one grid launch per phase,
+// 2 one cooperative grid launch).
 static int workItems( StepGroup* g, int* records, int capacity )
 {
 	std::vector<int> mode( g->hItems.size(), 0 );
 	for ( int i : g->splitItems )
-		mode[i] = 1;
+		mode[i] = g->splitPerPhase ? 1 : 2;
 	int n = 0;
 	for ( size_t i = 0; i < g->hItems.size(); ++i )
 	{

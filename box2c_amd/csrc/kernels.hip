@@ -402,22 +402,97 @@ __device__ inline int next_colour_slot( const WorkItem& wi, int g, int tid )
 	return b + tid < e ? b + tid : -1;
 }
 
-// All stages of one step for one work item, by one workgroup, with workgroup barriers between
-// dependent phases. `C` holds the item's private state; `bl` its body list (sim indices) and
-// `sc` the contact id of each of its slots.
-template <class S>
-__device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, const int* __restrict__ bl,
+// How the threads that run one work item synchronize between dependent phases.
+//
+// BlockSync: one workgroup, workgroup barriers, the restitution vote in LDS.
+struct BlockSync
+{
+	static constexpr bool kGrid = false;
+	__device__ inline int tid() const
+	{
+		return threadIdx.x;
+	}
+	__device__ inline int nt() const
+	{
+		return blockDim.x;
+	}
+	__device__ inline void sync() const
+	{
+		__syncthreads();
+	}
+};
+
+// GridSync: every workgroup of a cooperative launch (all resident at once), a grid-wide barrier
+// between phases and the vote in global memory. The barrier counts arrivals on one word that the
+// host zeroes before the launch: barrier n waits until n x blocks workgroups have arrived. The
+// arrival is a device-scope release and the wait a device-scope acquire, after a workgroup
+// barrier: each phase's writes by any thread (body state, impulses, in whichever XCD's L2 they
+// are) are visible to every workgroup of the next phase. Every
+// wait is bounded: past kGridSpinLimit polls it sets the error word, which ends every other wait
+// at once and which the host turns into an error after the step.
+constexpr unsigned kGridSpinLimit = 1u << 24;
+struct GridSync
+{
+	static constexpr bool kGrid = true;
+	unsigned* arrivals;
+	int* vote;
+	int* error;
+	int tid_, nt_;
+	unsigned blocks;
+	mutable unsigned target;
+	__device__ inline int tid() const
+	{
+		return tid_;
+	}
+	__device__ inline int nt() const
+	{
+		return nt_;
+	}
+	__device__ inline void sync() const
+	{
+		__syncthreads();
+		if ( threadIdx.x == 0 )
+		{
+			target += blocks;
+			__hip_atomic_fetch_add( arrivals, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
+			unsigned polls = 0;
+			while ( __hip_atomic_load( arrivals, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT ) < target )
+			{
+				if ( __hip_atomic_load( error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) != 0 )
+					break;
+				__builtin_amdgcn_s_sleep( 1 );
+				if ( ++polls > kGridSpinLimit )
+				{
+					__hip_atomic_store( error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+					break;
+				}
+			}
+		}
+		__syncthreads();
+	}
+};
+
+// All stages of one step for one work item, by the threads of X (one workgroup, or a cooperative
+// grid for a very large item), with X's barriers between dependent phases. `C` holds the item's
+// private state; `bl` its body list (sim indices) and `sc` the contact id of each of its slots.
+template <class S, class Sync>
+__device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const WorkItem& wi, const int* __restrict__ bl,
 							const int* __restrict__ sc, const int* __restrict__ levels, EnlargeRecord* enl, int* enlCount,
 							int enlCapacity )
 {
-	const int tid = threadIdx.x, nt = blockDim.x;
+	const int tid = X.tid(), nt = X.nt();
 	const float4* gstate = reinterpret_cast<const float4*>( W.state );
 	const int total = wi.groupStart[kGroups];
-	// the workgroup's vote on skipping the colour restitution sweeps (set here, visible after the
+	// the threads' vote on skipping the colour restitution sweeps (set here, visible after the
 	// staging barrier; cleared by any thread that finds a slot or body that needs the sweeps)
-	__shared__ int restitutionVote;
+	__shared__ int blockVote;
 	if ( tid == 0 )
-		restitutionVote = 1;
+	{
+		if constexpr ( Sync::kGrid )
+			__hip_atomic_store( X.vote, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+		else
+			blockVote = 1;
+	}
 
 	// stage private body state and masses (and the static body's identity state)
 	if constexpr ( S::kIdentityBody )
@@ -434,7 +509,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 		C.b1( i ) = gstate[2 * b + 1];
 		C.mass( i ) = make_float2( W.sims[b].invMass, W.sims[b].invI );
 	}
-	__syncthreads();
+	X.sync();
 
 	// prepare (all groups; the overflow group uses the same per-point formulas), with the contact
 	// of a thread's next slot loaded while it prepares the current one
@@ -452,7 +527,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			cur = next;
 		}
 	}
-	__syncthreads();
+	X.sync();
 
 	// read-only quads of this thread's slot in the next colour group (q0 holds its bodies),
 	// loaded one phase ahead, before the barrier: the sweeps visit the same slots in every pass,
@@ -491,7 +566,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				integrate_velocity_t( C, tid + r * nt, vt[r], W );
 		for ( int i = tid + 2 * nt; i < wi.bodyCount; i += nt )
 			integrate_velocity( C, i, W.sims[bl[i]], W );
-		__syncthreads();
+		X.sync();
 
 		// warm start: overflow (always, src/solver.c:639), then colours if enabled
 		for ( int L = 0; L < levelCount; ++L )
@@ -501,7 +576,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				int2 lb = C.bodies( k );
 				ovf_warm_start( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount );
 			}
-			__syncthreads();
+			X.sync();
 		}
 		if ( warm )
 			for ( int g = 1; g < kGroups; ++g )
@@ -524,7 +599,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 					}
 				}
 				ahead( nk );
-				__syncthreads();
+				X.sync();
 			}
 
 		// solve with bias
@@ -535,7 +610,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				int2 lb = C.bodies( k );
 				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, soft );
 			}
-			__syncthreads();
+			X.sync();
 		}
 		for ( int g = 1; g < kGroups; ++g )
 		{
@@ -557,12 +632,12 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				}
 			}
 			ahead( nk );
-			__syncthreads();
+			X.sync();
 		}
 
 		for ( int i = tid; i < wi.bodyCount; i += nt )
 			integrate_position( C, i, h );
-		__syncthreads();
+		X.sync();
 
 		// relax
 		for ( int L = 0; L < levelCount; ++L )
@@ -572,7 +647,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				int2 lb = C.bodies( k );
 				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, soft );
 			}
-			__syncthreads();
+			X.sync();
 		}
 		for ( int g = 1; g < kGroups; ++g )
 		{
@@ -594,7 +669,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 				}
 			}
 			ahead( nk );
-			__syncthreads();
+			X.sync();
 		}
 	}
 
@@ -607,7 +682,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			int2 lb = C.bodies( k );
 			ovf_restitution( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, threshold );
 		}
-		__syncthreads();
+		X.sync();
 	}
 	// The colour sweeps would return their input bit for bit when every colour slot's stage is
 	// the identity and every body is plain (restitution_is_identity): then they are skipped.
@@ -617,9 +692,18 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 	for ( int i = tid; i < wi.bodyCount && identity; i += nt )
 		identity = body_is_plain( C, i );
 	if ( !identity )
-		restitutionVote = 0; // every writer writes the same value
-	__syncthreads();
-	identity = restitutionVote != 0;
+	{
+		// every writer writes the same value
+		if constexpr ( Sync::kGrid )
+			__hip_atomic_store( X.vote, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+		else
+			blockVote = 0;
+	}
+	X.sync();
+	if constexpr ( Sync::kGrid )
+		identity = __hip_atomic_load( X.vote, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) != 0;
+	else
+		identity = blockVote != 0;
 	for ( int g = 1; g < kGroups && !identity; ++g )
 	{
 		int b = wi.groupStart[g], e = wi.groupStart[g + 1];
@@ -640,7 +724,7 @@ __device__ void solve_item( const S& C, const DevWorld& W, const WorkItem& wi, c
 			}
 		}
 		ahead( nk );
-		__syncthreads();
+		X.sync();
 	}
 
 	// store impulses (contact_solver.c:442-471 overflow: pointCount points; 1180-1293 lanes: both);
@@ -718,7 +802,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 			C.rec[k * cfQuads] = make_float4( __builtin_bit_cast( float, b.x ), __builtin_bit_cast( float, b.y ), 0.0f, 0.0f );
 		}
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
-		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
+		solve_item( C, BlockSync{}, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else if constexpr ( Store == kStoreBodiesLds )
 	{
@@ -734,7 +818,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount + 1 );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
-		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
+		solve_item( C, BlockSync{}, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else
 	{
@@ -744,10 +828,52 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = slotBodies + wi.slotStart;
 		C.P = priv + 2 * (size_t)wi.bodyStart;
 		C.M = mass + wi.bodyStart;
-		solve_item( C, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
+		solve_item( C, BlockSync{}, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
 	}
 }
 
+
+// One very large HBM-store work item over a cooperative grid of workgroups (all resident at once):
+// solve_item with grid-wide barriers where the workgroup path has workgroup barriers. The phases
+// apply the same per-slot and per-body functions to the same data (a colour's slots share no
+// body, so which thread takes which slot does not matter; overflow levels as in the workgroup
+// path), so the results are the same bits. sync: [0] barrier arrivals (zero at launch), [1] the
+// restitution vote; error: set when a barrier wait exceeded its bound; abortFlag: as k_solve's.
+__global__ void __launch_bounds__( kGridSolveBlock ) k_solve_grid( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+																 int itemIndex, const int* __restrict__ bodyList,
+																 const int* __restrict__ slotContact, const int2* __restrict__ slotBodies,
+																 const int* __restrict__ ovfLevels, float* cfBase, float4* priv, float2* mass,
+																 EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error,
+																 const int* abortFlag )
+{
+	// (the flag was written before the launch began, so every workgroup reads the same value)
+	if ( abortFlag != nullptr && *abortFlag != 0 )
+		return;
+	const WorkItem& wi = items[itemIndex];
+	const DevWorld& W = worlds[wi.world];
+	GlobalStore C;
+	C.rec = reinterpret_cast<float4*>( cfBase );
+	C.base = wi.slotStart;
+	C.sb = slotBodies + wi.slotStart;
+	C.P = priv + 2 * (size_t)wi.bodyStart;
+	C.M = mass + wi.bodyStart;
+	GridSync X;
+	X.arrivals = sync;
+	X.vote = reinterpret_cast<int*>( sync + 1 );
+	X.error = error;
+	X.tid_ = blockIdx.x * blockDim.x + threadIdx.x;
+	X.nt_ = gridDim.x * blockDim.x;
+	X.blocks = gridDim.x;
+	X.target = 0;
+	solve_item( C, X, W, wi, bodyList + wi.bodyStart, slotContact + wi.slotStart, ovfLevels + wi.ovfLevelBegin, enl, enlCount,
+				enlCapacity );
+}
+
+__global__ void k_grid_reset( unsigned* sync )
+{
+	if ( threadIdx.x < 2 )
+		sync[threadIdx.x] = 0u;
+}
 
 // One phase of one very large work item (HBM store), spread over a grid: thread i takes body or
 // slot `first + i` of the phase's range. The host launches the phases in solve_item's order, one
@@ -933,6 +1059,39 @@ hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* item
 	return hipGetLastError();
 }
 
+
+int solve_grid_capacity()
+{
+	static int capacity = -1;
+	if ( capacity < 0 )
+	{
+		int device = 0, cooperative = 0, cus = 0, perCu = 0;
+		capacity = 0;
+		if ( hipGetDevice( &device ) == hipSuccess &&
+			 hipDeviceGetAttribute( &cooperative, hipDeviceAttributeCooperativeLaunch, device ) == hipSuccess && cooperative &&
+			 hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, device ) == hipSuccess &&
+			 hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, reinterpret_cast<const void*>( &k_solve_grid ), kGridSolveBlock, 0 ) ==
+				 hipSuccess )
+			capacity = cus * perCu;
+	}
+	return capacity;
+}
+
+hipError_t launch_solve_grid( const DevWorld* worlds, const WorkItem* items, int itemIndex, int blocks, const int* bodyList,
+							  const int* slotContact, const int2* slotBodies, const int* ovfLevels, float* cf, float4* priv, float2* mass,
+							  EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error, const int* abortFlag,
+							  hipStream_t stream )
+{
+	if ( blocks < 1 || blocks > solve_grid_capacity() )
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL( k_grid_reset, dim3( 1 ), dim3( 64 ), 0, stream, sync );
+	hipError_t e = hipGetLastError();
+	if ( e != hipSuccess )
+		return e;
+	void* args[] = { &worlds, &items, &itemIndex, &bodyList, &slotContact, &slotBodies, &ovfLevels, &cf,
+					 &priv, &mass, &enl, &enlCount, &enlCapacity, &sync, &error, &abortFlag };
+	return hipLaunchCooperativeKernel( k_solve_grid, dim3( blocks ), dim3( kGridSolveBlock ), args, 0, stream );
+}
 
 hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, int first, int count,
 							   const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, float4* priv,

@@ -17,10 +17,38 @@ hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* item
 						 float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
 						 const int* abortFlag, hipStream_t stream );
 
+// one very large HBM-store work item over a cooperative grid of `blocks` workgroups of
+// kGridSolveBlock threads (k_solve_grid); sync holds two words it uses, error is set if a barrier
+// wait exceeded its bound. solve_grid_capacity: the most workgroups one such launch may have (all
+// resident at once), 0 when the device has no cooperative launches.
+int solve_grid_capacity();
+hipError_t launch_solve_grid( const DevWorld* worlds, const WorkItem* items, int itemIndex, int blocks, const int* bodyList,
+							  const int* slotContact, const int2* slotBodies, const int* ovfLevels, float* cf, float4* priv, float2* mass,
+							  EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error, const int* abortFlag,
+							  hipStream_t stream );
+
 // one phase of a very large work item over a grid (k_split_phase; phases in SplitPhase)
 hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, int first, int count,
 							   const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, float4* priv,
 							   float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
+
+// broadphase pair queries (broadphase.hip)
+hipError_t launch_pair_erase( unsigned long long* table, uint32_t mask, const unsigned long long* keys, int count, hipStream_t s );
+hipError_t launch_pair_insert( unsigned long long* table, uint32_t mask, const unsigned long long* keys, int count, int* error,
+							   hipStream_t s );
+hipError_t launch_mark_moved( uint32_t* bits, const int* moves, int count, bool set, hipStream_t s );
+hipError_t launch_find_pairs( const DevBroadphase& bp, const int* moves, int moveCount, PairRecord* out, int capacity, int* counters,
+							  hipStream_t s );
+// the pair set's hash on the host (the same probe sequence as the kernels')
+inline uint64_t pair_hash_host( uint64_t k )
+{
+	k ^= k >> 30;
+	k *= 0xbf58476d1ce4e5b9ull;
+	k ^= k >> 27;
+	k *= 0x94d049bb133111ebull;
+	k ^= k >> 31;
+	return k;
+}
 
 // replay and adjoint (adjoint.hip)
 int adjoint_scratch_floats( int bodyCount, int slotCount, int subSteps );

@@ -183,6 +183,7 @@ static b2ShapeId createShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 			fatal( "unsupported shape type", __FILE__, __LINE__ );
 	}
 	s.id = shapeId;
+	w->shapesVersion += 1;
 	s.bodyId = body->id;
 	s.type = type;
 	s.density = def->density;
@@ -293,6 +294,8 @@ static void createContact( World* w, Shape* shapeA, Shape* shapeB )
 		b->contactCount += 1;
 	}
 	w->pairSet.insert( pairKey( shapeA->id, shapeB->id ) );
+	if ( w->devicePairs.tableValid )
+		w->devicePairs.pendingKeys.push_back( pairKey( shapeA->id, shapeB->id ) );
 	w->nonTouching.push_back( contactId );
 
 	ContactInit init{};
@@ -304,6 +307,19 @@ static void createContact( World* w, Shape* shapeA, Shape* shapeB )
 	init.restitution = shapeA->restitution > shapeB->restitution ? shapeA->restitution : shapeB->restitution;
 	w->pendingInits.push_back( init );
 	w->contactsDirty = true;
+}
+
+// Marks the work items of a contact's bodies for rebuilding: its colour or the overflow order
+// around it changed (begin or end of touching, destruction of a touching contact, or a move
+// within the overflow list).
+static void markContactBodies( World* w, const Contact& c )
+{
+	for ( int e = 0; e < 2; ++e )
+	{
+		const Body& b = w->bodies[c.edges[e].bodyId];
+		if ( b.setIndex == kAwakeSet )
+			w->dirtySims.push_back( b.localIndex );
+	}
 }
 
 static void removeFromList( World* w, std::vector<int>& list, int localIndex )
@@ -365,7 +381,8 @@ static void addContactToGraph( World* w, Contact& c )
 	w->colors[colorIndex].push_back( c.contactId );
 }
 
-// src/constraint_graph.c:185-214
+// src/constraint_graph.c:185-214. The overflow constraints run in their list's order, so a
+// contact moved into the removed one's place there rebuilds its work item.
 static void removeContactFromGraph( World* w, int bodyIdA, int bodyIdB, int colorIndex, int localIndex )
 {
 	if ( colorIndex != kOverflow )
@@ -373,13 +390,19 @@ static void removeContactFromGraph( World* w, int bodyIdA, int bodyIdB, int colo
 		w->colorBodySets[colorIndex].clear( bodyIdA );
 		w->colorBodySets[colorIndex].clear( bodyIdB );
 	}
-	removeFromList( w, w->colors[colorIndex], localIndex );
+	std::vector<int>& list = w->colors[colorIndex];
+	const bool moves = localIndex < (int)list.size() - 1;
+	removeFromList( w, list, localIndex );
+	if ( colorIndex == kOverflow && moves )
+		markContactBodies( w, w->contacts[list[localIndex]] );
 }
 
 // src/contact.c:335-414
 void destroyContactImpl( World* w, Contact* c )
 {
 	w->pairSet.erase( pairKey( c->shapeIdA, c->shapeIdB ) );
+	if ( w->devicePairs.tableValid )
+		w->devicePairs.pendingKeys.push_back( pairKey( c->shapeIdA, c->shapeIdB ) );
 	int contactId = c->contactId;
 	for ( int e = 0; e < 2; ++e )
 	{
@@ -399,14 +422,14 @@ void destroyContactImpl( World* w, Contact* c )
 		removeContactFromGraph( w, c->edges[0].bodyId, c->edges[1].bodyId, c->colorIndex, c->localIndex );
 	else
 		removeFromList( w, w->nonTouching, c->localIndex );
+	if ( wasTouching )
+		markContactBodies( w, *c );
 	c->contactId = -1;
 	c->setIndex = -1;
 	c->colorIndex = -1;
 	c->localIndex = -1;
 	w->contactIdPool.free( contactId );
 	w->contactsDirty = true;
-	if ( wasTouching )
-		w->topologyDirty = true;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -482,13 +505,209 @@ static void findPairs( const World* w, int begin, int end, std::vector<MovePair>
 	}
 }
 
+DeviceBroadphase::~DeviceBroadphase()
+{
+	for ( DevTreeNode*& p : nodes )
+		if ( p )
+			(void)hipFree( p );
+	for ( void* p : { (void*)shapes, (void*)table, (void*)moved, (void*)moves, (void*)keys, (void*)pairs, (void*)counters } )
+		if ( p )
+			(void)hipFree( p );
+	if ( hostCounters )
+		(void)hipHostFree( hostCounters );
+}
+
+// The device pair set brought up to the host's: rebuilt from it when it is not valid or would be
+// over half full (counting tombstones), else the keys created or destroyed since the last update
+// are erased (those the host no longer has) and inserted (those it has; a key already there is
+// left as it is).
+static void syncPairTable( World* w, hipStream_t s )
+{
+	DeviceBroadphase& D = w->devicePairs;
+	const size_t live = w->pairSet.size();
+	if ( D.tableValid && 2 * ( live + (size_t)D.tableTombstones + D.pendingKeys.size() ) < D.tableSize )
+	{
+		std::vector<uint64_t>& keys = D.pendingKeys;
+		std::sort( keys.begin(), keys.end() );
+		keys.erase( std::unique( keys.begin(), keys.end() ), keys.end() );
+		std::stable_partition( keys.begin(), keys.end(), [&]( uint64_t k ) { return w->pairSet.count( k ) == 0; } );
+		const int erase = (int)( std::find_if( keys.begin(), keys.end(), [&]( uint64_t k ) { return w->pairSet.count( k ) != 0; } ) -
+								 keys.begin() );
+		const int n = (int)keys.size();
+		if ( n > 0 )
+		{
+			grow( D.keys, D.capKeys, n );
+			HIP_CHECK( hipMemcpyAsync( D.keys, keys.data(), sizeof( uint64_t ) * n, hipMemcpyHostToDevice, s ) );
+			HIP_CHECK( launch_pair_erase( D.table, D.tableSize - 1, D.keys, erase, s ) );
+			HIP_CHECK( launch_pair_insert( D.table, D.tableSize - 1, D.keys + erase, n - erase, D.counters + 1, s ) );
+			// the key list is read by the copy: wait before it changes
+			HIP_CHECK( hipStreamSynchronize( s ) );
+		}
+		D.tableTombstones += erase;
+		keys.clear();
+		return;
+	}
+	uint32_t size = 1024;
+	while ( size < 4 * live )
+		size *= 2;
+	std::vector<unsigned long long> table( size, kPairEmpty );
+	for ( uint64_t k : w->pairSet )
+	{
+		uint32_t i = (uint32_t)pair_hash_host( k ) & ( size - 1 );
+		while ( table[i] != kPairEmpty )
+			i = ( i + 1 ) & ( size - 1 );
+		table[i] = k;
+	}
+	if ( size != D.tableSize || D.table == nullptr )
+	{
+		if ( D.table )
+		{
+			HIP_CHECK( hipStreamSynchronize( s ) );
+			HIP_CHECK( hipFree( D.table ) );
+		}
+		HIP_CHECK( hipMalloc( (void**)&D.table, sizeof( unsigned long long ) * size ) );
+		D.tableSize = size;
+	}
+	HIP_CHECK( hipMemcpyAsync( D.table, table.data(), sizeof( unsigned long long ) * size, hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	D.tableTombstones = 0;
+	D.tableValid = true;
+	D.pendingKeys.clear();
+}
+
+// src/broad_phase.c:333-436 with the queries on the device (k_find_pairs): the trees and shape
+// filters are uploaded where they changed, the pair set brought up to date, and the new pairs come
+// back with the index of the move that found them and their rank among that move's pairs; the
+// contacts are then created here in the reference's order (moves in order, each move's pairs most
+// recent first, as the reference walks its prepended lists).
+static void updatePairsOnDevice( World* w )
+{
+	DeviceBroadphase& D = w->devicePairs;
+	hipStream_t s = stream();
+	const int moveCount = (int)w->moveArray.size();
+	auto t0 = std::chrono::steady_clock::now();
+	auto mark = [&]( int k ) { D.phaseMs[k] += std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t0 ).count(); };
+	if ( D.counters == nullptr )
+	{
+		HIP_CHECK( hipMalloc( (void**)&D.counters, 2 * sizeof( int ) ) );
+		HIP_CHECK( hipHostMalloc( (void**)&D.hostCounters, 2 * sizeof( int ), hipHostMallocDefault ) );
+	}
+	HIP_CHECK( hipMemsetAsync( D.counters, 0, 2 * sizeof( int ), s ) );
+	syncPairTable( w, s );
+	mark( 0 );
+	DevBroadphase bp{};
+	int maxNodes = 1;
+	for ( int t = 0; t < 3; ++t )
+	{
+		const DynamicTree& tree = w->trees[t];
+		const int n = (int)tree.nodes.size();
+		maxNodes = std::max( maxNodes, n );
+		if ( D.uploadedVersion[t] != tree.version || D.nodes[t] == nullptr || D.capNodes[t] < n )
+		{
+			grow( D.nodes[t], D.capNodes[t], std::max( n, 1 ) );
+			D.staging.resize( n );
+			for ( int i = 0; i < n; ++i )
+			{
+				const TreeNode& src = tree.nodes[i];
+				D.staging[i] = DevTreeNode{ src.aabb.lowerBound.x, src.aabb.lowerBound.y, src.aabb.upperBound.x, src.aabb.upperBound.y,
+											src.height == 0 ? -1 : src.child1, src.height == 0 ? -1 : src.child2, src.userData,
+											src.categoryBits };
+			}
+			if ( n > 0 )
+				HIP_CHECK( hipMemcpyAsync( D.nodes[t], D.staging.data(), sizeof( DevTreeNode ) * n, hipMemcpyHostToDevice, s ) );
+			HIP_CHECK( hipStreamSynchronize( s ) ); // the staging copy is reused for the next tree
+			D.uploadedVersion[t] = tree.version;
+		}
+		bp.nodes[t] = D.nodes[t];
+		bp.roots[t] = tree.root;
+	}
+	const int ns = (int)w->shapes.size();
+	if ( D.uploadedShapes != w->shapesVersion || D.shapes == nullptr || D.capShapes < ns )
+	{
+		grow( D.shapes, D.capShapes, std::max( ns, 1 ) );
+		std::vector<DevShapeFilter> filters( ns );
+		for ( int i = 0; i < ns; ++i )
+		{
+			const Shape& sh = w->shapes[i];
+			if ( sh.id < 0 )
+				continue;
+			filters[i] = DevShapeFilter{ sh.bodyId, (int)w->bodies[sh.bodyId].type, (uint32_t)sh.filter.categoryBits,
+										 (uint32_t)sh.filter.maskBits, (int)sh.filter.groupIndex, 0 };
+		}
+		if ( ns > 0 )
+			HIP_CHECK( hipMemcpyAsync( D.shapes, filters.data(), sizeof( DevShapeFilter ) * ns, hipMemcpyHostToDevice, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		D.uploadedShapes = w->shapesVersion;
+	}
+	mark( 1 );
+	// the moved-proxy bitmap covers every proxy key of the three trees (all zero between queries)
+	const int words = ( 4 * maxNodes + 31 ) / 32;
+	if ( D.moved == nullptr || D.capMovedWords < words )
+	{
+		grow( D.moved, D.capMovedWords, words );
+		HIP_CHECK( hipMemsetAsync( D.moved, 0, sizeof( uint32_t ) * D.capMovedWords, s ) );
+	}
+	grow( D.moves, D.capMoves, moveCount );
+	HIP_CHECK( hipMemcpyAsync( D.moves, w->moveArray.data(), sizeof( int ) * moveCount, hipMemcpyHostToDevice, s ) );
+	bp.shapes = D.shapes;
+	bp.moved = D.moved;
+	bp.pairs = D.table;
+	bp.pairMask = D.tableSize - 1;
+	grow( D.pairs, D.capPairs, std::max( 1024, moveCount ) );
+	HIP_CHECK( launch_mark_moved( D.moved, D.moves, moveCount, true, s ) );
+	int found = 0;
+	for ( int attempt = 0;; ++attempt )
+	{
+		HIP_CHECK( launch_find_pairs( bp, D.moves, moveCount, D.pairs, D.capPairs, D.counters, s ) );
+		HIP_CHECK( hipMemcpyAsync( D.hostCounters, D.counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		if ( D.hostCounters[1] != 0 )
+			fatal( "a device pair query overflowed its traversal stack or the pair table", __FILE__, __LINE__ );
+		found = D.hostCounters[0];
+		if ( found <= D.capPairs )
+			break;
+		B2MI_CHECK( attempt == 0 );
+		// more pairs than the buffer holds: a larger one, and the queries once more
+		grow( D.pairs, D.capPairs, found );
+		HIP_CHECK( hipMemsetAsync( D.counters, 0, sizeof( int ), s ) );
+	}
+	mark( 2 );
+	HIP_CHECK( launch_mark_moved( D.moved, D.moves, moveCount, false, s ) );
+	D.found.resize( found );
+	if ( found > 0 )
+		HIP_CHECK( hipMemcpyAsync( D.found.data(), D.pairs, sizeof( PairRecord ) * found, hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	mark( 3 );
+	std::sort( D.found.begin(), D.found.end(), []( const PairRecord& a, const PairRecord& b ) {
+		return a.query != b.query ? a.query < b.query : a.rank > b.rank;
+	} );
+	for ( const PairRecord& p : D.found )
+	{
+		B2MI_CHECK( 0 <= p.shapeIdA && p.shapeIdA < ns && 0 <= p.shapeIdB && p.shapeIdB < ns );
+		createContact( w, &w->shapes[p.shapeIdA], &w->shapes[p.shapeIdB] );
+	}
+	for ( int key : w->moveArray )
+		w->moveSet.clear( (uint32_t)key );
+	w->moveArray.clear();
+	mark( 4 );
+	D.phaseMs[5] += (float)found;
+	D.phaseMs[6] += (float)moveCount;
+	D.phaseMs[7] += 1.0f;
+}
+
 // src/broad_phase.c:333-436: the queries of the moved proxies (in parallel chunks when
-// `parallel`), then contact creation in move order on this thread
+// `parallel`), then contact creation in move order on this thread. A world stepped alone with many
+// moved proxies queries on the device instead (updatePairsOnDevice).
 static void updatePairs( World* w, bool parallel )
 {
 	int moveCount = (int)w->moveArray.size();
 	if ( moveCount == 0 )
 		return;
+	if ( parallel && w->devicePairsMin >= 0 && moveCount >= w->devicePairsMin )
+	{
+		updatePairsOnDevice( w );
+		return;
+	}
 	constexpr int kChunk = 128;
 	int chunks = ( moveCount + kChunk - 1 ) / kChunk;
 	std::vector<std::vector<MovePair>> pairs( chunks );
@@ -524,190 +743,173 @@ static int findRoot( std::vector<int>& parent, int x )
 	return x;
 }
 
-// The work items of one world: islands (connected components of dynamic bodies through
-// touching contacts; static and kinematic bodies do not join islands) packed in order up to
-// `target` constraint slots, each item's slots grouped as overflow first, then colours 0..10.
-// Writes w->solver with indices local to its lists; runs on any host thread (touches this
-// world only).
-// `parallel`: the stages run over index ranges on the host pool (the caller is not a pool task:
-// the pool does not nest); each range writes only its own outputs, so the lists are the same
-// either way.
-static void buildWorldItems( World* w, int target, int budget, int fullBudget, bool parallel )
+// a body as a touching contact's solver reference: >= 0 a dynamic body (its awake sim index),
+// <= -2 a kinematic body (-2 - its sim index), -1 a static body
+static inline int awakeRef( const Body& b )
 {
-	SolverLists& L = w->solver;
-	L.items.clear();
-	L.bodyList.clear();
-	L.slotContact.clear();
-	L.ovfLevels.clear();
-	L.slotBodies.clear();
-	L.maxGroupSize = 0;
-	L.maxItemLds = 0;
-	int na = (int)w->awakeSimBody.size();
-	if ( na == 0 )
-	{
-		w->topologyDirty = false;
-		return;
-	}
-	const int parts = parallel ? 64 : 1;
-	auto forRanges = [&]( int n, const std::function<void( int, int )>& fn ) {
-		auto part = [&]( int p ) { fn( (int)( (int64_t)n * p / parts ), (int)( (int64_t)n * ( p + 1 ) / parts ) ); };
-		if ( parallel )
-			hostPoolRun( parts, part );
-		else
-			part( 0 );
-	};
-	// What the rest needs of each touching contact, in colour-list order (overflow last): its id
-	// and its two bodies as awake-set references (>= 0: dynamic body, its sim index; <= -2:
-	// kinematic body, -2 - its sim index; -1: static). Gathered by passes over the body and
-	// contact arrays (a contact's colour and position in that colour's list are stored in it),
-	// not by following the colour lists into the contact array.
-	const int bodyCount = (int)w->bodies.size();
-	std::vector<int> refOf( bodyCount );
-	forRanges( bodyCount, [&]( int lo, int hi ) {
-		for ( int i = lo; i < hi; ++i )
-		{
-			const Body& b = w->bodies[i];
-			refOf[i] = b.setIndex != 2 ? -1 : b.type == b2_dynamicBody ? b.localIndex : -2 - b.localIndex;
-		}
-	} );
-	struct Src
-	{
-		int id, a, b;
-	};
-	int colorStart[kColorCount + 1];
-	colorStart[0] = 0;
-	for ( int c = 0; c < kColorCount; ++c )
-		colorStart[c + 1] = colorStart[c] + (int)w->colors[c].size();
-	std::vector<Src> src( colorStart[kColorCount] );
-	{
-		std::atomic<int> placed{ 0 };
-		forRanges( (int)w->contacts.size(), [&]( int lo, int hi ) {
-			int n = 0;
-			for ( int cid = lo; cid < hi; ++cid )
-			{
-				const Contact& ct = w->contacts[cid];
-				if ( ct.contactId < 0 || ct.colorIndex < 0 )
-					continue;
-				const int c = ct.colorIndex, li = ct.localIndex;
-				B2MI_CHECK( c < kColorCount && 0 <= li && li < (int)w->colors[c].size() && w->colors[c][li] == cid );
-				src[colorStart[c] + li] = Src{ cid, refOf[ct.edges[0].bodyId], refOf[ct.edges[1].bodyId] };
-				n += 1;
-			}
-			placed.fetch_add( n );
-		} );
-		// each list position names exactly one contact, so all of them were written
-		B2MI_CHECK( placed.load() == colorStart[kColorCount] );
-	}
+	return b.setIndex != kAwakeSet ? -1 : b.type == b2_dynamicBody ? b.localIndex : -2 - b.localIndex;
+}
 
-	std::vector<int> parent( na );
-	for ( int i = 0; i < na; ++i )
+// A touching contact for the item builder: its id and its two bodies' references.
+struct Src
+{
+	int id, a, b;
+};
+
+// Runs fn over `parts` index ranges of [0, n), on the host pool when `parallel` (the caller is
+// then not a pool task), each range writing only its own outputs.
+static void forRanges( bool parallel, int n, const std::function<void( int, int )>& fn )
+{
+	const int parts = parallel ? 64 : 1;
+	auto part = [&]( int p ) { fn( (int)( (int64_t)n * p / parts ), (int)( (int64_t)n * ( p + 1 ) / parts ) ); };
+	if ( parallel )
+		hostPoolRun( parts, part );
+	else
+		part( 0 );
+}
+
+// The components of a set of awake bodies and their touching contacts, as the item builder takes
+// them. Component k's bodies, in ascending sim index, are compBodies[compBodyStart[k] ..
+// compBodyStart[k+1]); the contacts are src in colour-list order (colour c at colorStart[c] ..
+// colorStart[c+1]); per colour they are ordered by component (a stable counting sort, so each
+// component keeps the colour list's order): component k's contacts of colour c are
+// src[bySlot[colorStart[c] + p]] for p in compStart[c][k] .. compStart[c][k+1], where compStart[c]
+// starts at compStart.data() + c * (nc + 1).
+struct ComponentSet
+{
+	int nc = 0;
+	std::vector<int> compBodyStart, compBodies, compSlots;
+	std::vector<Src> src;
+	int colorStart[kColorCount + 1] = {};
+	std::vector<int> compStart, bySlot;
+	int compSize( int k ) const
+	{
+		return compBodyStart[k + 1] - compBodyStart[k];
+	}
+};
+
+// Fills the components of cs for the bodies `sims` (ascending sim indices) and the contacts
+// already in cs.src / cs.colorStart: union-find over the dynamic-dynamic contacts, components in
+// order of their smallest sim index (a root is the smallest position of its set, so it is numbered
+// before the rest of it); a kinematic body is a component of its own. posOf maps each sim of
+// `sims` to its position there.
+static void makeComponents( ComponentSet& cs, const std::vector<int>& sims, const std::vector<int>& posOf, bool parallel )
+{
+	const int m = (int)sims.size();
+	std::vector<int> parent( m );
+	for ( int i = 0; i < m; ++i )
 		parent[i] = i;
-	for ( const Src& e : src )
+	for ( const Src& e : cs.src )
 		if ( e.a >= 0 && e.b >= 0 )
 		{
-			int ra = findRoot( parent, e.a ), rb = findRoot( parent, e.b );
+			int pa = posOf[e.a], pb = posOf[e.b];
+			B2MI_CHECK( pa >= 0 && pb >= 0 );
+			int ra = findRoot( parent, pa ), rb = findRoot( parent, pb );
 			if ( ra != rb )
 				parent[std::max( ra, rb )] = std::min( ra, rb );
 		}
-	// Components in order of their smallest sim index (a root is the smallest index of its
-	// set, so it is numbered before the rest of it); kinematic bodies alone. Component k's
-	// bodies, in ascending sim index, are compBodies[compBodyStart[k] .. compBodyStart[k+1]).
-	std::vector<int> compOf( na );
+	std::vector<int> compOf( m );
 	int nc = 0;
-	for ( int i = 0; i < na; ++i )
+	for ( int i = 0; i < m; ++i )
 	{
 		int r = findRoot( parent, i );
 		compOf[i] = r == i ? nc++ : compOf[r];
 	}
-	std::vector<int> compBodyStart( nc + 1, 0 ), compBodies( na ), compSlots( nc, 0 );
-	for ( int i = 0; i < na; ++i )
-		compBodyStart[compOf[i] + 1] += 1;
+	cs.nc = nc;
+	cs.compBodyStart.assign( nc + 1, 0 );
+	cs.compBodies.resize( m );
+	cs.compSlots.assign( nc, 0 );
+	for ( int i = 0; i < m; ++i )
+		cs.compBodyStart[compOf[i] + 1] += 1;
 	for ( int k = 0; k < nc; ++k )
-		compBodyStart[k + 1] += compBodyStart[k];
+		cs.compBodyStart[k + 1] += cs.compBodyStart[k];
 	{
-		std::vector<int> fill( compBodyStart.begin(), compBodyStart.end() - 1 );
-		for ( int i = 0; i < na; ++i )
-			compBodies[fill[compOf[i]]++] = i;
+		std::vector<int> fill( cs.compBodyStart.begin(), cs.compBodyStart.end() - 1 );
+		for ( int i = 0; i < m; ++i )
+			cs.compBodies[fill[compOf[i]]++] = sims[i];
 	}
-	auto compSize = [&]( int k ) { return compBodyStart[k + 1] - compBodyStart[k]; };
-	// A contact belongs to the component of its dynamic body. Per colour, the contacts are
-	// ordered by component (a stable counting sort, so each component keeps the colour list's
-	// order): component k's contacts of colour c are bySlot[compStart[c][k] .. compStart[c][k+1]).
-	std::vector<int> compStart( (size_t)kColorCount * ( nc + 1 ), 0 );
-	std::vector<int> bySlot( src.size() );
-	{
-		std::vector<int> compOfSrc( src.size() );
-		forRanges( (int)src.size(), [&]( int lo, int hi ) {
-			for ( int q = lo; q < hi; ++q )
-			{
-				int a = src[q].a >= 0 ? src[q].a : src[q].b;
-				B2MI_CHECK( a >= 0 );
-				compOfSrc[q] = compOf[a];
-			}
-		} );
-		auto sortColor = [&]( int c ) {
-			int* cs = compStart.data() + (size_t)c * ( nc + 1 );
-			for ( int q = colorStart[c]; q < colorStart[c + 1]; ++q )
-				cs[compOfSrc[q] + 1] += 1;
-			for ( int k = 0; k < nc; ++k )
-				cs[k + 1] += cs[k];
-			std::vector<int> fill( cs, cs + nc );
-			for ( int q = colorStart[c]; q < colorStart[c + 1]; ++q )
-				bySlot[colorStart[c] + fill[compOfSrc[q]]++] = q;
-		};
-		if ( parallel )
-			hostPoolRun( kColorCount, sortColor );
-		else
-			for ( int c = 0; c < kColorCount; ++c )
-				sortColor( c );
-		for ( int c = 0; c < kColorCount; ++c )
+	// a contact belongs to the component of its dynamic body
+	const int ns = (int)cs.src.size();
+	cs.compStart.assign( (size_t)kColorCount * ( nc + 1 ), 0 );
+	cs.bySlot.resize( ns );
+	std::vector<int> compOfSrc( ns );
+	forRanges( parallel, ns, [&]( int lo, int hi ) {
+		for ( int q = lo; q < hi; ++q )
 		{
-			const int* cs = compStart.data() + (size_t)c * ( nc + 1 );
-			for ( int k = 0; k < nc; ++k )
-				compSlots[k] += cs[k + 1] - cs[k];
+			int a = cs.src[q].a >= 0 ? cs.src[q].a : cs.src[q].b;
+			B2MI_CHECK( a >= 0 && posOf[a] >= 0 );
+			compOfSrc[q] = compOf[posOf[a]];
 		}
+	} );
+	auto sortColor = [&]( int c ) {
+		int* st = cs.compStart.data() + (size_t)c * ( nc + 1 );
+		for ( int q = cs.colorStart[c]; q < cs.colorStart[c + 1]; ++q )
+			st[compOfSrc[q] + 1] += 1;
+		for ( int k = 0; k < nc; ++k )
+			st[k + 1] += st[k];
+		std::vector<int> fill( st, st + nc );
+		for ( int q = cs.colorStart[c]; q < cs.colorStart[c + 1]; ++q )
+			cs.bySlot[cs.colorStart[c] + fill[compOfSrc[q]]++] = q;
+	};
+	if ( parallel )
+		hostPoolRun( kColorCount, sortColor );
+	else
+		for ( int c = 0; c < kColorCount; ++c )
+			sortColor( c );
+	for ( int c = 0; c < kColorCount; ++c )
+	{
+		const int* st = cs.compStart.data() + (size_t)c * ( nc + 1 );
+		for ( int k = 0; k < nc; ++k )
+			cs.compSlots[k] += st[k + 1] - st[k];
 	}
+}
 
-	// greedy packing in component order: item i holds components [span[i].x, span[i].y)
+// One work item built into lists of its own (body, slot and level indices relative to the item).
+struct BuiltItem
+{
+	WorkItem wi;
+	std::vector<int> bodies, slots, levels;
+	std::vector<int2> slotBodies;
+	int maxGroup = 0;
+};
+
+// Packs the components of cs in order into items of up to `target` constraint slots (and 4 x
+// target bodies) and builds each item: owned bodies, kinematic shadows, slots grouped as
+// overflow first then colours 0..10, overflow levels, workgroup size, the next-colour table and
+// the store for the LDS budget. posOfSim (one entry per awake sim, all -1) is the item-local
+// position of each owned sim while its item is built: items own disjoint bodies, and each clears
+// its entries when done.
+static std::vector<BuiltItem> packItems( const World* w, const ComponentSet& cs, int target, int budget, int fullBudget,
+										 std::vector<int>& posOfSim, bool parallel )
+{
 	std::vector<int2> spans;
-	for ( int ci = 0; ci < nc; )
+	for ( int ci = 0; ci < cs.nc; )
 	{
 		int cj = ci;
 		int slots = 0, bodies = 0;
 		do
 		{
-			slots += compSlots[cj];
-			bodies += compSize( cj );
+			slots += cs.compSlots[cj];
+			bodies += cs.compSize( cj );
 			cj += 1;
-		} while ( cj < nc && slots + compSlots[cj] <= target && bodies + compSize( cj ) <= 4 * target );
+		} while ( cj < cs.nc && slots + cs.compSlots[cj] <= target && bodies + cs.compSize( cj ) <= 4 * target );
 		spans.push_back( make_int2( ci, cj ) );
 		ci = cj;
 	}
-	// Each item is built into lists of its own (body, slot and level indices relative to the
-	// item), then the lists are laid end to end. posOfSim is the item-local position of each
-	// owned sim: items own disjoint bodies, and each clears its entries when done.
-	struct Built
-	{
-		WorkItem wi;
-		std::vector<int> bodies, slots, levels;
-		std::vector<int2> slotBodies;
-		int maxGroup = 0;
-	};
 	const int ni = (int)spans.size();
-	std::vector<Built> built( ni );
-	std::vector<int> posOfSim( na, -1 );
+	std::vector<BuiltItem> built( ni );
 	auto buildItem = [&]( int item ) {
-		Built& out = built[item];
+		BuiltItem& out = built[item];
 		const int ci = spans[item].x, cj = spans[item].y;
 		WorkItem& wi = out.wi;
 		wi = WorkItem{};
 		wi.world = w->groupIndex;
 		// owned bodies
 		for ( int k = ci; k < cj; ++k )
-			for ( int q = compBodyStart[k]; q < compBodyStart[k + 1]; ++q )
+			for ( int q = cs.compBodyStart[k]; q < cs.compBodyStart[k + 1]; ++q )
 			{
-				posOfSim[compBodies[q]] = (int)out.bodies.size();
-				out.bodies.push_back( compBodies[q] );
+				posOfSim[cs.compBodies[q]] = (int)out.bodies.size();
+				out.bodies.push_back( cs.compBodies[q] );
 			}
 		wi.ownedCount = (int)out.bodies.size();
 		// local index of a body reference: owned bodies by position, kinematic bodies as read-only
@@ -715,7 +917,11 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		std::vector<std::pair<int, int>> shadowMap;
 		auto localIndex = [&]( int r ) -> int {
 			if ( r >= 0 )
+			{
+				// a dynamic body of the item's contacts is one of its own
+				B2MI_CHECK( posOfSim[r] >= 0 );
 				return posOfSim[r];
+			}
 			if ( r == -1 )
 				return -1;
 			int sim = -2 - r;
@@ -733,11 +939,11 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		{
 			int color = gidx == 0 ? kOverflow : gidx - 1;
 			wi.groupStart[gidx] = (int)out.slots.size();
-			const int* cs = compStart.data() + (size_t)color * ( nc + 1 );
+			const int* st = cs.compStart.data() + (size_t)color * ( cs.nc + 1 );
 			for ( int k = ci; k < cj; ++k )
-				for ( int p = cs[k]; p < cs[k + 1]; ++p )
+				for ( int p = st[k]; p < st[k + 1]; ++p )
 				{
-					const Src& e = src[bySlot[colorStart[color] + p]];
+					const Src& e = cs.src[cs.bySlot[cs.colorStart[color] + p]];
 					int la = localIndex( e.a );
 					int lb = localIndex( e.b );
 					out.slots.push_back( e.id );
@@ -747,8 +953,8 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 			itemMaxGroup = std::max( itemMaxGroup, size );
 		}
 		for ( int k = ci; k < cj; ++k )
-			for ( int q = compBodyStart[k]; q < compBodyStart[k + 1]; ++q )
-				posOfSim[compBodies[q]] = -1;
+			for ( int q = cs.compBodyStart[k]; q < cs.compBodyStart[k + 1]; ++q )
+				posOfSim[cs.compBodies[q]] = -1;
 		// Overflow levels (group 0, src/contact_solver.c:212-345 runs it serially): a slot's level
 		// is one past the last level that changed one of its bodies, and the slots are ordered by
 		// level, keeping their order within a level. Static bodies are never changed; nor are
@@ -848,52 +1054,75 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		}
 		out.maxGroup = std::max( itemMaxGroup, wi.ownedCount );
 	};
-	forRanges( ni, [&]( int lo, int hi ) {
+	forRanges( parallel, ni, [&]( int lo, int hi ) {
 		for ( int i = lo; i < hi; ++i )
 			buildItem( i );
 	} );
-	// lay the items' lists end to end
+	return built;
+}
+
+// Appends built items to the world's lists (their offsets relative to the lists), records which
+// item owns each of their bodies, and refreshes the lists' maxima over the live items.
+static void appendItems( World* w, std::vector<BuiltItem>& built, bool parallel )
+{
+	SolverLists& L = w->solver;
+	int nb = (int)L.bodyList.size(), ns = (int)L.slotContact.size(), nl = (int)L.ovfLevels.size();
+	const int firstItem = (int)L.items.size();
+	for ( BuiltItem& b : built )
 	{
-		int nb = 0, ns = 0, nl = 0;
-		for ( Built& b : built )
-		{
-			b.wi.bodyStart = nb;
-			b.wi.slotStart = ns;
-			b.wi.ovfLevelBegin = nl;
-			nb += (int)b.bodies.size();
-			ns += (int)b.slots.size();
-			nl += (int)b.levels.size();
-			L.maxGroupSize = std::max( L.maxGroupSize, b.maxGroup );
-			L.maxItemLds = std::max( L.maxItemLds, b.wi.ldsBytes );
-		}
-		L.bodyList.resize( nb );
-		L.slotContact.resize( ns );
-		L.slotBodies.resize( ns );
-		L.ovfLevels.resize( nl );
-		forRanges( ni, [&]( int lo, int hi ) {
-			for ( int i = lo; i < hi; ++i )
-			{
-				const Built& b = built[i];
-				std::copy( b.bodies.begin(), b.bodies.end(), L.bodyList.begin() + b.wi.bodyStart );
-				std::copy( b.slots.begin(), b.slots.end(), L.slotContact.begin() + b.wi.slotStart );
-				std::copy( b.slotBodies.begin(), b.slotBodies.end(), L.slotBodies.begin() + b.wi.slotStart );
-				std::copy( b.levels.begin(), b.levels.end(), L.ovfLevels.begin() + b.wi.ovfLevelBegin );
-			}
-		} );
-		L.items.resize( ni );
-		for ( int i = 0; i < ni; ++i )
-			L.items[i] = built[i].wi;
+		b.wi.bodyStart = nb;
+		b.wi.slotStart = ns;
+		b.wi.ovfLevelBegin = nl;
+		nb += (int)b.bodies.size();
+		ns += (int)b.slots.size();
+		nl += (int)b.levels.size();
 	}
-	// every index the solver kernel dereferences is checked here, on the host
-	int na2 = (int)w->awakeSimBody.size();
-	forRanges( ni, [&]( int lo, int hi ) {
-		for ( int it = lo; it < hi; ++it )
+	L.bodyList.resize( nb );
+	L.slotContact.resize( ns );
+	L.slotBodies.resize( ns );
+	L.ovfLevels.resize( nl );
+	L.items.resize( firstItem + built.size() );
+	L.live.resize( firstItem + built.size(), 1 );
+	forRanges( parallel, (int)built.size(), [&]( int lo, int hi ) {
+		for ( int i = lo; i < hi; ++i )
+		{
+			const BuiltItem& b = built[i];
+			std::copy( b.bodies.begin(), b.bodies.end(), L.bodyList.begin() + b.wi.bodyStart );
+			std::copy( b.slots.begin(), b.slots.end(), L.slotContact.begin() + b.wi.slotStart );
+			std::copy( b.slotBodies.begin(), b.slotBodies.end(), L.slotBodies.begin() + b.wi.slotStart );
+			std::copy( b.levels.begin(), b.levels.end(), L.ovfLevels.begin() + b.wi.ovfLevelBegin );
+			L.items[firstItem + i] = b.wi;
+			for ( int k = 0; k < b.wi.ownedCount; ++k )
+				L.itemOfSim[b.bodies[k]] = firstItem + i;
+		}
+	} );
+	L.maxGroupSize = 0;
+	L.maxItemLds = 0;
+	for ( size_t i = 0; i < L.items.size(); ++i )
+		if ( L.live[i] )
+		{
+			int maxGroup = L.items[i].ownedCount;
+			for ( int g = 0; g < kGroups; ++g )
+				maxGroup = std::max( maxGroup, L.items[i].groupStart[g + 1] - L.items[i].groupStart[g] );
+			L.maxGroupSize = std::max( L.maxGroupSize, maxGroup );
+			L.maxItemLds = std::max( L.maxItemLds, L.items[i].ldsBytes );
+		}
+}
+
+// Every index the solver kernel dereferences for the items [first, end) of the world's lists,
+// checked on the host.
+static void checkItems( const World* w, int first, int end, bool parallel )
+{
+	const SolverLists& L = w->solver;
+	const int na = (int)w->awakeSimBody.size();
+	forRanges( parallel, end - first, [&]( int lo, int hi ) {
+		for ( int it = first + lo; it < first + hi; ++it )
 		{
 			const WorkItem& wi = L.items[it];
 			B2MI_CHECK( wi.bodyStart >= 0 && wi.bodyStart + wi.bodyCount <= (int)L.bodyList.size() );
 			B2MI_CHECK( 0 <= wi.ownedCount && wi.ownedCount <= wi.bodyCount );
 			for ( int i = 0; i < wi.bodyCount; ++i )
-				B2MI_CHECK( 0 <= L.bodyList[wi.bodyStart + i] && L.bodyList[wi.bodyStart + i] < na2 );
+				B2MI_CHECK( 0 <= L.bodyList[wi.bodyStart + i] && L.bodyList[wi.bodyStart + i] < na );
 			B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= (int)L.slotContact.size() );
 			B2MI_CHECK( wi.ldsBytes <= kMaxItemLdsBytes );
 			B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
@@ -911,7 +1140,184 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 			}
 		}
 	} );
+}
+
+// All work items of one world: islands (connected components of dynamic bodies through touching
+// contacts; static and kinematic bodies do not join islands) packed in order up to `target`
+// constraint slots, each item's slots grouped as overflow first, then colours 0..10. Writes
+// w->solver with indices local to its lists; runs on any host thread (touches this world only).
+// `parallel`: the stages run over index ranges on the host pool (the caller is not a pool task:
+// the pool does not nest); each range writes only its own outputs, so the lists are the same
+// either way.
+static void buildWorldItems( World* w, int target, int budget, int fullBudget, bool parallel )
+{
+	SolverLists& L = w->solver;
+	L = SolverLists{};
+	w->dirtySims.clear();
+	const int na = (int)w->awakeSimBody.size();
+	L.itemOfSim.assign( na, -1 );
+	w->posOfSim.assign( na, -1 );
 	w->topologyDirty = false;
+	w->solverPending = true;
+	w->pendingBodyFrom = w->pendingSlotFrom = 0;
+	if ( na == 0 )
+		return;
+	// What the rest needs of each touching contact, in colour-list order (overflow last), gathered
+	// by one pass over the contact array (a contact's colour and position in that colour's list
+	// are stored in it), not by following the colour lists into the contact array.
+	ComponentSet cs;
+	cs.colorStart[0] = 0;
+	for ( int c = 0; c < kColorCount; ++c )
+		cs.colorStart[c + 1] = cs.colorStart[c] + (int)w->colors[c].size();
+	cs.src.resize( cs.colorStart[kColorCount] );
+	{
+		std::atomic<int> placed{ 0 };
+		forRanges( parallel, (int)w->contacts.size(), [&]( int lo, int hi ) {
+			int n = 0;
+			for ( int cid = lo; cid < hi; ++cid )
+			{
+				const Contact& ct = w->contacts[cid];
+				if ( ct.contactId < 0 || ct.colorIndex < 0 )
+					continue;
+				const int c = ct.colorIndex, li = ct.localIndex;
+				B2MI_CHECK( c < kColorCount && 0 <= li && li < (int)w->colors[c].size() && w->colors[c][li] == cid );
+				cs.src[cs.colorStart[c] + li] =
+					Src{ cid, awakeRef( w->bodies[ct.edges[0].bodyId] ), awakeRef( w->bodies[ct.edges[1].bodyId] ) };
+				n += 1;
+			}
+			placed.fetch_add( n );
+		} );
+		// each list position names exactly one contact, so all of them were written
+		B2MI_CHECK( placed.load() == cs.colorStart[kColorCount] );
+	}
+	std::vector<int> sims( na ), posOf( na );
+	for ( int i = 0; i < na; ++i )
+		sims[i] = posOf[i] = i;
+	makeComponents( cs, sims, posOf, parallel );
+	std::vector<BuiltItem> built = packItems( w, cs, target, budget, fullBudget, w->posOfSim, parallel );
+	appendItems( w, built, parallel );
+	checkItems( w, 0, (int)L.items.size(), parallel );
+	L.liveBodies = (int)L.bodyList.size();
+	L.liveSlots = (int)L.slotContact.size();
+}
+
+// Rebuilds only the work items that own a body of w->dirtySims: their bodies are regathered into
+// components with the touching contacts they own (the contacts whose first dynamic body is one
+// of them), packed into new items appended to the world's lists, and the old items retired.
+// Items not touched keep their lists: a colour's slots share no non-static body, so the order
+// of slots within a colour does not change any result, and each item's overflow slots keep the
+// overflow list's order. Falls back to a full rebuild when the retired items' holes would make up
+// more than half the lists.
+static void rebuildDirtyItems( World* w, int target, int budget, int fullBudget, bool parallel )
+{
+	SolverLists& L = w->solver;
+	const int na = (int)w->awakeSimBody.size();
+	if ( (int)L.itemOfSim.size() != na || (int)w->posOfSim.size() != na )
+	{
+		buildWorldItems( w, target, budget, fullBudget, parallel );
+		return;
+	}
+	// the items to retire
+	std::vector<int> dead;
+	for ( int s : w->dirtySims )
+	{
+		B2MI_CHECK( 0 <= s && s < na );
+		int it = L.itemOfSim[s];
+		if ( it >= 0 && L.live[it] )
+		{
+			L.live[it] = 0;
+			dead.push_back( it );
+		}
+	}
+	w->dirtySims.clear();
+	if ( dead.empty() )
+		return;
+	// their owned bodies, ascending
+	std::vector<int> sims;
+	int deadBodies = 0, deadSlots = 0;
+	for ( int it : dead )
+	{
+		const WorkItem& wi = L.items[it];
+		for ( int i = 0; i < wi.ownedCount; ++i )
+			sims.push_back( L.bodyList[wi.bodyStart + i] );
+		deadBodies += wi.bodyCount;
+		deadSlots += wi.groupStart[kGroups];
+	}
+	if ( 2 * ( L.liveBodies - deadBodies ) < (int)L.bodyList.size() || 2 * ( L.liveSlots - deadSlots ) < (int)L.slotContact.size() )
+	{
+		buildWorldItems( w, target, budget, fullBudget, parallel );
+		return;
+	}
+	std::sort( sims.begin(), sims.end() );
+	std::vector<int>& posOf = L.posInSet; // all -1 between calls
+	if ( (int)posOf.size() != na )
+		posOf.assign( na, -1 );
+	for ( int i = 0; i < (int)sims.size(); ++i )
+		posOf[sims[i]] = i;
+	// the touching contacts these bodies own, in colour-list order
+	ComponentSet cs;
+	struct Key
+	{
+		int color, local;
+		Src s;
+	};
+	std::vector<Key> keys;
+	for ( int sim : sims )
+	{
+		const Body& body = w->bodies[w->awakeSimBody[sim]];
+		for ( int key = body.headContactKey; key != -1; )
+		{
+			const Contact& c = w->contacts[key >> 1];
+			const int next = c.edges[key & 1].nextKey;
+			if ( c.colorIndex >= 0 )
+			{
+				const Body& a = w->bodies[c.edges[0].bodyId];
+				const Body& b = w->bodies[c.edges[1].bodyId];
+				int ra = awakeRef( a ), rb = awakeRef( b );
+				int owner = ra >= 0 ? ra : rb;
+				if ( owner == sim )
+					keys.push_back( Key{ c.colorIndex, c.localIndex, Src{ c.contactId, ra, rb } } );
+			}
+			key = next;
+		}
+	}
+	std::sort( keys.begin(), keys.end(),
+			   []( const Key& x, const Key& y ) { return x.color != y.color ? x.color < y.color : x.local < y.local; } );
+	cs.src.resize( keys.size() );
+	for ( int c = 0; c <= kColorCount; ++c )
+		cs.colorStart[c] = 0;
+	for ( size_t q = 0; q < keys.size(); ++q )
+	{
+		cs.src[q] = keys[q].s;
+		cs.colorStart[keys[q].color + 1] += 1;
+	}
+	for ( int c = 0; c < kColorCount; ++c )
+		cs.colorStart[c + 1] += cs.colorStart[c];
+	makeComponents( cs, sims, posOf, false );
+	for ( int sim : sims )
+		posOf[sim] = -1;
+	std::vector<BuiltItem> built = packItems( w, cs, target, budget, fullBudget, w->posOfSim, false );
+	const int firstNew = (int)L.items.size();
+	const int bodyFrom = (int)L.bodyList.size(), slotFrom = (int)L.slotContact.size();
+	appendItems( w, built, false );
+	checkItems( w, firstNew, (int)L.items.size(), false );
+	for ( const BuiltItem& b : built )
+	{
+		deadBodies -= (int)b.bodies.size();
+		deadSlots -= (int)b.slots.size();
+	}
+	L.liveBodies -= deadBodies;
+	L.liveSlots -= deadSlots;
+	if ( !w->solverPending )
+	{
+		w->pendingBodyFrom = bodyFrom;
+		w->pendingSlotFrom = slotFrom;
+	}
+	else
+	{
+		w->pendingBodyFrom = std::min( w->pendingBodyFrom, bodyFrom );
+		w->pendingSlotFrom = std::min( w->pendingSlotFrom, slotFrom );
+	}
 	w->solverPending = true;
 }
 
@@ -953,6 +1359,7 @@ static void layoutGroup( StepGroup* g )
 		slots += w->slotRegion.cap;
 		collide += w->collideRegion.cap;
 		w->solverPending = true;
+		w->pendingBodyFrom = w->pendingSlotFrom = 0;
 		w->collidePending = true;
 	}
 	g->bodyListCount = bodies;
@@ -1056,16 +1463,22 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 	{
 		if ( w->solverPending )
 		{
+			// the entries not yet uploaded: all of them after a full rebuild or a new layout, the
+			// appended items' after a partial rebuild
 			const SolverLists& L = w->solver;
 			Region br = w->bodyRegion, sr = w->slotRegion;
-			std::copy( L.bodyList.begin(), L.bodyList.end(), g->hBodyList.begin() + br.base );
-			std::copy( L.slotContact.begin(), L.slotContact.end(), g->hSlotContact.begin() + sr.base );
-			std::copy( L.slotBodies.begin(), L.slotBodies.end(), g->hSlotBodies.begin() + sr.base );
-			bFirst = std::min( bFirst, br.base );
+			const int b0 = w->pendingBodyFrom, s0 = w->pendingSlotFrom;
+			B2MI_CHECK( (int)L.bodyList.size() <= br.cap && (int)L.slotContact.size() <= sr.cap );
+			std::copy( L.bodyList.begin() + b0, L.bodyList.end(), g->hBodyList.begin() + br.base + b0 );
+			std::copy( L.slotContact.begin() + s0, L.slotContact.end(), g->hSlotContact.begin() + sr.base + s0 );
+			std::copy( L.slotBodies.begin() + s0, L.slotBodies.end(), g->hSlotBodies.begin() + sr.base + s0 );
+			bFirst = std::min( bFirst, br.base + b0 );
 			bLast = std::max( bLast, br.base + (int)L.bodyList.size() );
-			sFirst = std::min( sFirst, sr.base );
+			sFirst = std::min( sFirst, sr.base + s0 );
 			sLast = std::max( sLast, sr.base + (int)L.slotContact.size() );
 			w->solverPending = false;
+			w->pendingBodyFrom = (int)L.bodyList.size();
+			w->pendingSlotFrom = (int)L.slotContact.size();
 		}
 		if ( w->collidePending )
 		{
@@ -1093,8 +1506,11 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		{
 			const int levelBase = (int)g->hOvfLevels.size();
 			g->hOvfLevels.insert( g->hOvfLevels.end(), w->solver.ovfLevels.begin(), w->solver.ovfLevels.end() );
-			for ( WorkItem wi : w->solver.items )
+			for ( size_t i = 0; i < w->solver.items.size(); ++i )
 			{
+				if ( !w->solver.live[i] )
+					continue;
+				WorkItem wi = w->solver.items[i];
 				wi.bodyStart += w->bodyRegion.base;
 				wi.slotStart += w->slotRegion.base;
 				wi.ovfLevelBegin += levelBase;
@@ -1124,10 +1540,15 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		} );
 		g->classes.clear();
 		g->splitItems.clear();
-		// very large HBM-store items run phase by phase over the whole grid (launchSplit) rather
+		// very large HBM-store items run over the whole grid (enqueueSolve) rather
 		// than in one workgroup; not in taped runs or chunked groups
-		const int splitSlots = g->splitSlots < 0 ? kDefaultSplitSlots : g->splitSlots;
+		// by default a one-world group runs a large island over the grid from fewer slots than a batch,
+		// whose other worlds' items share the CUs meanwhile
+		const int splitSlots = g->splitSlots >= 0 ? g->splitSlots : g->worlds.size() == 1 ? kDefaultSplitSlotsWorld : kDefaultSplitSlots;
 		const bool maySplit = splitSlots > 0 && g->tape == nullptr && chunks == 1;
+		g->splitPerPhase = g->gridBlocks < 0 || solve_grid_capacity() == 0;
+		if ( g->dGridSync == nullptr )
+			HIP_CHECK( hipMalloc( (void**)&g->dGridSync, 2 * sizeof( unsigned ) ) );
 		for ( int i = 0; i < (int)g->hItems.size(); ++i )
 		{
 			const WorkItem& wi = g->hItems[i];
@@ -1512,7 +1933,7 @@ static void applyChanges( World* w, const ChangeRecord* recs, int count )
 			c.flags |= 1u;
 			addContactToGraph( w, c );
 			removeFromList( w, w->nonTouching, localIndex );
-			w->topologyDirty = true;
+			markContactBodies( w, c );
 		}
 		else if ( r.flags & kSimStopped )
 		{
@@ -1522,7 +1943,7 @@ static void applyChanges( World* w, const ChangeRecord* recs, int count )
 			c.localIndex = (int)w->nonTouching.size();
 			w->nonTouching.push_back( c.contactId );
 			removeContactFromGraph( w, bodyIdA, bodyIdB, colorIndex, localIndex );
-			w->topologyDirty = true;
+			markContactBodies( w, c );
 		}
 	}
 }
@@ -1571,7 +1992,18 @@ struct WorldPick
 // item size class (the items' own sizes, blockSize 0), those on separate streams so that they
 // run side by side. With `abortFlag` set every workgroup returns at once when the flag is
 // nonzero (a speculative launch behind the narrowphase, void when contacts changed state).
-void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
+#define B2MI_TRY( expr )                 \
+	do                                   \
+	{                                    \
+		hipError_t e_ = ( expr );        \
+		if ( e_ != hipSuccess )          \
+			return e_;                   \
+	} while ( 0 )
+
+// Enqueues the solver launches of one step on stream s (and the extra streams `aux`, joined back
+// into s); returns the first error instead of failing, so that a stream capture of it can be
+// undone.
+hipError_t StepGroup::enqueueSolve( hipStream_t s, const hipStream_t* aux, const int* abortFlag )
 {
 	for ( const LaunchClass& c : classes )
 	{
@@ -1581,18 +2013,83 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 	auto launch = [&]( const LaunchClass& c, hipStream_t cs ) {
 		int block = blockSize > 0 ? blockSize : c.block;
 		int lds = blockSize > 0 ? maxItemLds : c.lds;
-		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
-								 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs ) );
+		return launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
+							 dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs );
 	};
-	B2MI_CHECK( splitItems.empty() || abortFlag == nullptr );
+	B2MI_CHECK( splitItems.empty() || abortFlag == nullptr || !splitPerPhase );
 	for ( int item : splitItems )
-		launchSplit( item, s );
+	{
+		if ( splitPerPhase )
+			B2MI_TRY( enqueueSplit( item, s ) );
+		else
+			B2MI_TRY( launch_solve_grid( dWorlds, dItems, item, gridBlocksFor( hItems[item] ), dBodyList, dSlotContact, dSlotBodies,
+										 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, dGridSync, dCounters + 3,
+										 abortFlag, s ) );
+	}
 	if ( classes.size() <= 1 )
 	{
 		if ( !classes.empty() )
-			launch( classes[0], s );
-		return;
+			B2MI_TRY( launch( classes[0], s ) );
+		return hipSuccess;
 	}
+	B2MI_TRY( hipEventRecord( forkEvent, s ) );
+	int n = (int)classes.size();
+	for ( int c = 0; c < n; ++c )
+	{
+		hipStream_t cs = c == 0 ? s : aux[( c - 1 ) % kAuxStreams];
+		if ( c > 0 && c <= kAuxStreams )
+			B2MI_TRY( hipStreamWaitEvent( cs, forkEvent, 0 ) );
+		B2MI_TRY( launch( classes[c], cs ) );
+	}
+	for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
+	{
+		B2MI_TRY( hipEventRecord( joinEvents[c - 1], aux[c - 1] ) );
+		B2MI_TRY( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
+	}
+	return hipSuccess;
+}
+
+// Everything a step's solver launches take as arguments or launch shape: two steps with the same
+// signature enqueue the same launches.
+SolveSignature StepGroup::solveSignature( const int* abortFlag ) const
+{
+	SolveSignature k;
+	memset( &k, 0, sizeof( k ) );
+	k.itemsVersion = itemsVersion;
+	k.abortFlag = abortFlag;
+	k.worlds = dWorlds;
+	k.items = dItems;
+	k.bodyList = dBodyList;
+	k.slotContact = dSlotContact;
+	k.slotBodies = dSlotBodies;
+	k.ovfLevels = dOvfLevels;
+	k.cons = dCons;
+	k.priv = dPriv;
+	k.mass = dMass;
+	k.enlarge = dEnlarge;
+	k.counters = dCounters;
+	k.gridSync = dGridSync;
+	k.enlargeCapacity = capEnlarge;
+	k.blockSize = blockSize;
+	k.maxItemLds = maxItemLds;
+	k.gridBlocks = gridBlocks;
+	k.splitPerPhase = splitPerPhase ? 1 : 0;
+	k.chunks = chunkCount;
+	return k;
+}
+
+// The solver launches of one step: one launch for every item at a fixed workgroup size, or one per
+// item size class (the items' own sizes, blockSize 0), those on separate streams so that they
+// run side by side; very large items over the whole grid first. With `abortFlag` set every
+// workgroup returns at once when the flag is nonzero (a speculative launch behind the
+// narrowphase, void when contacts changed state). When a step enqueues the same launches as the
+// step before it (the usual case once a scene's contacts have settled), the sequence is captured
+// once as a HIP graph and replayed from then on, until anything it bakes in changes; a capture
+// that the runtime refuses turns graphs off for the group. The capture is recorded on streams of
+// the group's own (recording executes nothing), so that other threads stepping other groups on the
+// shared streams meanwhile never enqueue into it; the replay runs on the shared stream s.
+void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
+{
 	if ( forkEvent == nullptr )
 	{
 		HIP_CHECK( hipEventCreateWithFlags( &forkEvent, hipEventDisableTiming ) );
@@ -1601,30 +2098,87 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		for ( hipEvent_t& e : collideDone )
 			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
 	}
-	HIP_CHECK( hipEventRecord( forkEvent, s ) );
-	int n = (int)classes.size();
-	for ( int c = 0; c < n; ++c )
+	if ( launchGraphs && !graphsRefused )
 	{
-		hipStream_t cs = c == 0 ? s : auxStream( ( c - 1 ) % kAuxStreams );
-		if ( c > 0 && c <= kAuxStreams )
-			HIP_CHECK( hipStreamWaitEvent( cs, forkEvent, 0 ) );
-		launch( classes[c], cs );
+		const SolveSignature key = solveSignature( abortFlag );
+		if ( graphExec != nullptr && memcmp( &key, &graphKey, sizeof( key ) ) == 0 )
+		{
+			HIP_CHECK( hipGraphLaunch( graphExec, s ) );
+			graphReplays += 1;
+			return;
+		}
+		const bool repeat = memcmp( &key, &lastSolveKey, sizeof( key ) ) == 0;
+		lastSolveKey = key;
+		if ( repeat && captureSolve( s, abortFlag ) )
+		{
+			graphKey = key;
+			HIP_CHECK( hipGraphLaunch( graphExec, s ) );
+			graphReplays += 1;
+			return;
+		}
 	}
-	for ( int c = 1; c < std::min( n, kAuxStreams + 1 ); ++c )
+	hipStream_t aux[kAuxStreams];
+	for ( int i = 0; i < kAuxStreams; ++i )
+		aux[i] = auxStream( i );
+	HIP_CHECK( enqueueSolve( s, aux, abortFlag ) );
+}
+
+bool StepGroup::captureSolve( hipStream_t, const int* abortFlag )
+{
+	if ( graphExec != nullptr )
 	{
-		HIP_CHECK( hipEventRecord( joinEvents[c - 1], auxStream( c - 1 ) ) );
-		HIP_CHECK( hipStreamWaitEvent( s, joinEvents[c - 1], 0 ) );
+		HIP_CHECK( hipGraphExecDestroy( graphExec ) );
+		graphExec = nullptr;
 	}
+	if ( captureStreams[0] == nullptr )
+		for ( hipStream_t& cs : captureStreams )
+			HIP_CHECK( hipStreamCreateWithFlags( &cs, hipStreamNonBlocking ) );
+	hipGraph_t graph = nullptr;
+	hipStream_t cs = captureStreams[0];
+	hipError_t e = hipStreamBeginCapture( cs, hipStreamCaptureModeThreadLocal );
+	if ( e == hipSuccess )
+	{
+		hipError_t le = enqueueSolve( cs, captureStreams + 1, abortFlag );
+		e = hipStreamEndCapture( cs, &graph );
+		if ( le != hipSuccess )
+			e = le;
+	}
+	if ( e == hipSuccess )
+		e = hipGraphInstantiate( &graphExec, graph, nullptr, nullptr, 0 );
+	if ( graph != nullptr )
+		(void)hipGraphDestroy( graph );
+	if ( e != hipSuccess )
+	{
+		(void)hipGetLastError();
+		graphExec = nullptr;
+		graphsRefused = true;
+		return false;
+	}
+	graphCaptures += 1;
+	return true;
+}
+
+// Workgroups of a very large item's cooperative launch: one thread per slot of its largest colour
+// group (or the set count), within what can be resident at once.
+int StepGroup::gridBlocksFor( const WorkItem& wi ) const
+{
+	int largest = 1;
+	for ( int g = 1; g < kGroups; ++g )
+		largest = std::max( largest, wi.groupStart[g + 1] - wi.groupStart[g] );
+	int blocks = gridBlocks > 0 ? gridBlocks : ( largest + kGridSolveBlock - 1 ) / kGridSolveBlock;
+	return std::max( 1, std::min( blocks, solve_grid_capacity() ) );
 }
 
 // The phases of one very large work item (solve_item's order), each one grid launch on stream s.
-void StepGroup::launchSplit( int item, hipStream_t s )
+hipError_t StepGroup::enqueueSplit( int item, hipStream_t s )
 {
+	hipError_t status = hipSuccess;
 	const WorkItem& wi = hItems[item];
 	const DevWorld& hw = hWorlds[wi.world];
 	auto run = [&]( int phase, int first, int count ) {
-		HIP_CHECK( launch_split_phase( dWorlds, dItems, item, phase, first, count, dBodyList, dSlotContact, dSlotBodies, dCons, dPriv,
-									   dMass, dEnlarge, dCounters + 1, capEnlarge, s ) );
+		if ( status == hipSuccess )
+			status = launch_split_phase( dWorlds, dItems, item, phase, first, count, dBodyList, dSlotContact, dSlotBodies, dCons, dPriv,
+										 dMass, dEnlarge, dCounters + 1, capEnlarge, s );
 	};
 	const int total = wi.groupStart[kGroups];
 	const int levelCount = wi.groupStart[1] > wi.groupStart[0] ? wi.ovfLevelCount : 0;
@@ -1656,6 +2210,7 @@ void StepGroup::launchSplit( int item, hipStream_t s )
 	colours( kSplitRestitution );
 	run( kSplitStoreImpulses, 0, total );
 	run( kSplitFinalize, 0, wi.ownedCount );
+	return status;
 }
 
 // The solver launches of one world chunk, its size classes side by side: class j on extra
@@ -1769,7 +2324,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 			regionFull.store( true );
 		if ( reshape )
 			w->topologyDirty = true;
-		if ( w->topologyDirty )
+		if ( w->topologyDirty || !w->dirtySims.empty() )
 			topology.add( w );
 		int ns = (int)w->shapes.size();
 		if ( ns != w->countedShapes )
@@ -1839,7 +2394,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 	// then needs no host round trip. Otherwise the launch returns at once and the solver runs
 	// after the state changes below. Groups whose contacts change at most steps (one large
 	// world) stop speculating after a miss and try again every 16th step.
-	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && splitItems.empty() &&
+	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && ( splitItems.empty() || !splitPerPhase ) &&
 						   ( specMisses == 0 || epoch % 16 == 0 );
 	// World chunks: consecutive worlds, so each chunk's narrowphase list is one range and its
 	// state changes go to that range of the change buffer with a counter of their own. A
@@ -1939,7 +2494,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 			if ( last[i] > first[i] )
 			{
 				markActive( worlds[i] );
-				if ( worlds[i]->topologyDirty )
+				if ( worlds[i]->topologyDirty || !worlds[i]->dirtySims.empty() )
 					changedTopology.push_back( worlds[i] );
 			}
 	}
@@ -1961,7 +2516,13 @@ void StepGroup::step( float timeStep, int subStepCount )
 		if ( !topology.list.empty() )
 		{
 			HostPool::get().run( (int)topology.list.size(),
-								 [&]( int i ) { buildWorldItems( topology.list[i], target, budget, fullBudget, topology.list.size() == 1 ); } );
+								 [&]( int i ) {
+									 World* w = topology.list[i];
+									 if ( w->topologyDirty )
+										 buildWorldItems( w, target, budget, fullBudget, topology.list.size() == 1 );
+									 else
+										 rebuildDirtyItems( w, target, budget, fullBudget, topology.list.size() == 1 );
+								 } );
 			bool full = false;
 			for ( World* w : topology.list )
 				full = full || (int)w->solver.bodyList.size() > w->bodyRegion.cap ||
@@ -1991,7 +2552,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 			joinAux( s );
 		}
 		HIP_CHECK( hipEventRecord( e2, s ) );
-		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, 3 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
 		hostPhases[7] = msSince( t0 );
 		HIP_CHECK( hipStreamSynchronize( s ) );
 	}
@@ -1999,6 +2560,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 		hostPhases[6] = hostPhases[7] = hostPhases[5];
 	msChanges = std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - tc ).count();
 	hostPhases[8] = msSince( t0 );
+	if ( hCounters[3] != 0 )
+		fatal( "a barrier wait of the grid solver exceeded its bound", __FILE__, __LINE__ );
 	int enlargeCount = hCounters[1];
 	B2MI_CHECK( enlargeCount <= capEnlarge );
 	auto te = std::chrono::steady_clock::now();

@@ -461,6 +461,7 @@ int DynamicTree::rebuild( bool fullBuild )
 		nodeIndex = stack[--stackCount];
 	}
 	root = buildTree( leafCount );
+	version += 1;
 	return leafCount;
 }
 

@@ -37,6 +37,7 @@ public:
 	int nodeCount = 0;
 	int freeList = 0;
 	int proxyCount = 0;
+	uint64_t version = 0; // incremented by every change of the nodes
 	std::vector<int> leafIndices;
 	std::vector<b2Vec2> leafCenters;
 
@@ -67,6 +68,7 @@ public:
 		n.height = 0;
 		insertLeaf( id, true );
 		proxyCount += 1;
+		version += 1;
 		return id;
 	}
 
@@ -75,6 +77,7 @@ public:
 		removeLeaf( id );
 		freeNode( id );
 		proxyCount -= 1;
+		version += 1;
 	}
 
 	void moveProxy( int id, b2AABB aabb )
@@ -82,10 +85,12 @@ public:
 		removeLeaf( id );
 		nodes[id].aabb = aabb;
 		insertLeaf( id, false );
+		version += 1;
 	}
 
 	void enlargeProxy( int id, b2AABB aabb )
 	{
+		version += 1;
 		nodes[id].aabb = aabb;
 		int p = nodes[id].parent;
 		while ( p != -1 )

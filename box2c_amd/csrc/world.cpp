@@ -154,6 +154,12 @@ StepGroup::~StepGroup()
 	devFree( dCons );
 	devFree( dPriv );
 	devFree( dMass );
+	devFree( dGridSync );
+	if ( graphExec )
+		(void)hipGraphExecDestroy( graphExec );
+	for ( hipStream_t cs : captureStreams )
+		if ( cs )
+			(void)hipStreamDestroy( cs );
 	for ( hipEvent_t e : events )
 		if ( e )
 			(void)hipEventDestroy( e );
@@ -1239,6 +1245,7 @@ B2_API void b2DestroyBody( b2BodyId bodyId )
 		}
 		w->trees[sh.proxyKey & 3].destroyProxy( sh.proxyKey >> 2 );
 		w->shapeIdPool.free( sh.id );
+		w->shapesVersion += 1;
 		sh.id = -1;
 	}
 	if ( body->setIndex == 0 )

@@ -165,14 +165,54 @@ struct Contact
 
 struct StepGroup;
 
-// work items of one world, with body and slot indices local to these lists
+// The device side of a world's pair queries: uploaded copies of the trees and shape filters, the
+// pair set as an open-addressing table kept in step with the host's by the keys created and
+// destroyed since it was last brought up to date, and the buffers of one query launch.
+struct DeviceBroadphase
+{
+	DevTreeNode* nodes[3] = {};
+	int capNodes[3] = {};
+	uint64_t uploadedVersion[3] = { ~0ull, ~0ull, ~0ull };
+	std::vector<DevTreeNode> staging;
+	DevShapeFilter* shapes = nullptr;
+	int capShapes = 0;
+	uint64_t uploadedShapes = ~0ull;
+	unsigned long long* table = nullptr;
+	uint32_t tableSize = 0; // a power of two
+	int tableTombstones = 0;
+	bool tableValid = false;
+	std::vector<uint64_t> pendingKeys; // pair keys created or destroyed since the table was updated
+	uint32_t* moved = nullptr;
+	int capMovedWords = 0;
+	int* moves = nullptr;
+	int capMoves = 0;
+	unsigned long long* keys = nullptr;
+	int capKeys = 0;
+	PairRecord* pairs = nullptr;
+	int capPairs = 0;
+	int* counters = nullptr;	 // [0] pairs found, [1] error
+	int* hostCounters = nullptr; // pinned
+	std::vector<PairRecord> found;
+	// accumulated host clock (ms since each update began) after: the pair-set update, the uploads,
+	// the queries, the readback and the contact creation; then pairs found, moves, updates
+	float phaseMs[8] = {};
+	~DeviceBroadphase();
+};
+
+// work items of one world, with body and slot indices local to these lists. Rebuilding only the
+// items whose bodies' contacts changed retires them (live = 0) and appends their replacements, so
+// the lists hold the retired items' entries as holes until the next full rebuild.
 struct SolverLists
 {
 	std::vector<WorkItem> items;
+	std::vector<char> live;		  // per item: 0 once retired
 	std::vector<int> bodyList;	  // awake sim indices
 	std::vector<int> slotContact; // contact ids
 	std::vector<int2> slotBodies; // local body indices (-1 static)
 	std::vector<int> ovfLevels;	  // overflow level boundaries of the items (relative to each item's slots)
+	std::vector<int> itemOfSim;	  // per awake sim: the live item that owns it
+	std::vector<int> posInSet;	  // per awake sim, -1 (scratch of a partial rebuild)
+	int liveBodies = 0, liveSlots = 0; // entries of the live items
 	int maxGroupSize = 0, maxItemLds = 0;
 };
 
@@ -219,6 +259,11 @@ struct World
 	BitSet moveSet;
 	std::vector<int> moveArray;
 	std::unordered_set<uint64_t> pairSet;
+	uint64_t shapesVersion = 0; // shapes created, destroyed or refiltered
+	// pair queries of a world stepped alone run on the device from this many moved proxies on (-1:
+	// never, the default); in a batch they run on the host pool, world by world
+	int devicePairsMin = -1;
+	DeviceBroadphase devicePairs;
 
 	b2Vec2 gravity{ 0.0f, -10.0f };
 	float hitEventThreshold = 1, restitutionThreshold = 1, contactPushoutVelocity = 3;
@@ -226,7 +271,9 @@ struct World
 	bool enableSleep = true, enableWarmStarting = true, enableContinuous = true;
 
 	b2Profile profile{};
-	bool topologyDirty = true; // colour lists / bodies changed since the work items were built
+	bool topologyDirty = true; // bodies or launch settings changed: rebuild every work item
+	std::vector<int> dirtySims; // awake sims whose contacts' colours changed: rebuild their items
+	std::vector<int> posOfSim;	// per awake sim, -1 (scratch of the item builder)
 	bool contactsDirty = true; // contacts created or destroyed since the narrowphase list was built
 
 	// this world's solver lists (indices local to the lists) and narrowphase list, and the
@@ -235,6 +282,7 @@ struct World
 	std::vector<int> collideIds;
 	Region bodyRegion, slotRegion, collideRegion;
 	bool solverPending = true, collidePending = true; // region contents not yet uploaded
+	int pendingBodyFrom = 0, pendingSlotFrom = 0;	  // solver-list entries from these on are not uploaded
 
 	// device arrays
 	float* dState = nullptr;
@@ -284,10 +332,20 @@ inline int chunkOfWorld( int worldIndex, int worldCount, int chunks )
 {
 	return (int)( (int64_t)worldIndex * chunks / worldCount );
 }
-// device counters of a step: [1] enlarged shapes, [2] bodies at the sleep time, [kChunkCounter + c]
-// state changes of chunk c
+// device counters of a step: [1] enlarged shapes, [2] bodies at the sleep time, [3] set when a
+// barrier wait of the grid solver exceeded its bound, [kChunkCounter + c] state changes of chunk c
 constexpr int kChunkCounter = 4;
 constexpr int kCounterCount = kChunkCounter + kMaxChunks;
+
+// What a step's solver launches bake in (see StepGroup::solveSignature).
+struct SolveSignature
+{
+	int itemsVersion;
+	const int* abortFlag;
+	const void *worlds, *items, *bodyList, *slotContact, *slotBodies, *ovfLevels, *cons, *priv, *mass, *enlarge, *counters,
+		*gridSync;
+	int enlargeCapacity, blockSize, maxItemLds, gridBlocks, splitPerPhase, chunks;
+};
 
 // a set of worlds advanced together by one narrowphase launch and one solver launch
 struct StepGroup
@@ -298,7 +356,9 @@ struct StepGroup
 	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kDefaultItemLdsBytes, 0 = HBM only)
 	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
 	int ldsFullBudget = -1;	  // max LDS bytes of an item keeping all its state in LDS (-1 = ldsBudget)
-	int splitSlots = -1;	  // HBM-store items of at least this many slots run phase by phase (-1 = kDefaultSplitSlots, 0 = never)
+	int splitSlots = -1;	  // HBM-store items of at least this many slots run over the grid (-1 = kDefaultSplitSlots, 0 = never)
+	int gridBlocks = 0;		  // workgroups of such an item's cooperative launch (0 = from its largest colour, -1 = one launch per phase)
+	unsigned* dGridSync = nullptr; // barrier arrivals and vote of the cooperative grid solver
 	int builtTarget = -1, builtBudget = -1, builtFullBudget = -1;				   // packing target and LDS budget of the current items
 	bool structureDirty = true;
 
@@ -355,8 +415,20 @@ struct StepGroup
 	float msCollide = 0, msSolve = 0, msHost = 0;
 	hipEvent_t events[4] = { nullptr, nullptr, nullptr, nullptr };
 	std::vector<LaunchClass> classes;
-	std::vector<int> splitItems; // items (indices into hItems) that run as one grid launch per phase
-	void launchSplit( int item, hipStream_t s );
+	std::vector<int> splitItems; // items (indices into hItems) that run over the whole grid
+	bool splitPerPhase = false;	 // those run as one launch per phase (no cooperative launches)
+	int gridBlocksFor( const WorkItem& wi ) const;
+	hipError_t enqueueSplit( int item, hipStream_t s );
+	hipError_t enqueueSolve( hipStream_t s, const hipStream_t* aux, const int* abortFlag );
+	// the solver launches of a step replayed from a HIP graph while they stay the same
+	bool launchGraphs = true;
+	bool graphsRefused = false; // a capture failed: direct launches from then on
+	hipGraphExec_t graphExec = nullptr;
+	hipStream_t captureStreams[1 + kAuxStreams] = {}; // the group's own, for recording captures
+	SolveSignature graphKey{}, lastSolveKey{};
+	int graphCaptures = 0, graphReplays = 0;
+	SolveSignature solveSignature( const int* abortFlag ) const;
+	bool captureSolve( hipStream_t s, const int* abortFlag );
 	// host clock (ms since the step began) at: pair update, collide-list build, collide launch,
 	// narrowphase wait, wait returned, change processing done, solver launch, solver wait, wait
 	// returned, enlarge processing done

@@ -91,12 +91,39 @@ B2_API void b2mi_Batch_SetLdsFullBudget( int batch, int fullMaxBytes );
 B2_API void b2mi_Batch_SetWorldChunks( int batch, int chunks );
 
 // A work item whose solver state is kept in HBM and that has at least `slots` constraint slots
-// (one very large island, such as a 40,000-shape pile) runs as one grid-wide launch per solver
-// phase (stage, prepare, each substep's integrations and colour and overflow-level passes,
-// restitution, store, finalize) instead of in a single workgroup; results are the same bits.
-// -1 = the default (16384 slots), 0 = never. Not in taped runs or chunked batches.
+// (one very large island, such as a 40,000-shape pile) runs over the whole GPU instead of in a
+// single workgroup: one cooperative launch whose workgroups meet at a grid-wide barrier between
+// dependent phases, or, on a device without cooperative launches, one launch per solver phase
+// (stage, prepare, each substep's integrations and colour and overflow-level passes,
+// restitution, store, finalize); results are the same bits either way. -1 = the default (4,096
+// slots for a world stepped alone, 16,384 in a batch), 0 = never. Not in taped runs or chunked
+// batches.
 B2_API void b2mi_Batch_SetSplitSlots( int batch, int slots );
 B2_API void b2mi_World_SetSplitSlots( b2WorldId worldId, int slots );
+
+// Workgroups (of 256 threads) of such an item's cooperative launch: 0 = one thread per slot of
+// its largest colour group, -1 = one launch per phase instead; at most what the device holds at
+// once. Results are the same bits for every count.
+B2_API void b2mi_Batch_SetGridBlocks( int batch, int blocks );
+B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks );
+
+// Broadphase pair queries of a world stepped alone run on the device (one thread per moved
+// proxy over uploaded copies of the trees, the contacts then created on the host in the
+// reference's order) when at least `minMoves` proxies moved; -1 = always on the host (the
+// default), 0 = always on the device. Results are the same bits either way.
+B2_API void b2mi_World_SetDevicePairs( b2WorldId worldId, int minMoves );
+// Accumulated over the device pair updates so far: host clock (ms since each update began) after
+// the pair-set update, the uploads, the queries, the readback and the contact creation; then the
+// pairs found, the moved proxies and the number of updates.
+B2_API void b2mi_World_GetDevicePairStats( b2WorldId worldId, float* stats );
+
+// HIP graphs for the solver launches (on by default): when a step enqueues the same solver
+// launches as the step before it, the sequence is captured as a graph and replayed while it stays
+// the same. Results are the same bits either way. The stats give the captures and replays so far.
+B2_API void b2mi_Batch_SetLaunchGraphs( int batch, int enabled );
+B2_API void b2mi_World_SetLaunchGraphs( b2WorldId worldId, int enabled );
+B2_API void b2mi_Batch_GetLaunchGraphStats( int batch, int* captures, int* replays );
+B2_API void b2mi_World_GetLaunchGraphStats( b2WorldId worldId, int* captures, int* replays );
 
 // Autotuner: steps the batch (or world) under each candidate launch shape (threads per
 // workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
@@ -119,8 +146,8 @@ B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity )
 
 // The work items of the last step's solver, in launch order, 4 ints each: constraint slots,
 // bodies (kinematic shadows included), store (0 HBM, 1 bodies in LDS, 2 LDS) and how the item
-// ran (0 one workgroup, 1 one grid launch per solver phase). Returns the item count and writes
-// at most `capacity` records.
+// ran (0 one workgroup, 1 one grid launch per solver phase, 2 one cooperative grid launch).
+// Returns the item count and writes at most `capacity` records.
 B2_API int b2mi_Batch_GetWorkItems( int batch, int* records, int capacity );
 B2_API int b2mi_World_GetWorkItems( b2WorldId worldId, int* records, int capacity );
 

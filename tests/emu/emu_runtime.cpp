@@ -35,12 +35,14 @@ struct Fiber
 	ucontext_t ctx;
 	void* stack = nullptr;
 	bool done = false;
-	bool waiting = false;
+	bool waiting = false; // at a workgroup barrier
+	uint3 block{ 0, 0, 0 };
 };
 
 struct Launch
 {
 	const std::function<void()>* body = nullptr;
+	bool cooperative = false;
 	std::vector<Fiber> fibers;
 	ucontext_t scheduler;
 	Fiber* running = nullptr;
@@ -117,6 +119,119 @@ void emu_syncthreads()
 #if EMU_ASAN
 	__sanitizer_finish_switch_fiber( fake, &L->schedulerStackBottom, &L->schedulerStackSize );
 #endif
+}
+
+void emu_yield()
+{
+	Launch* L = g_launch;
+	if ( L == nullptr || !L->cooperative )
+		return; // one workgroup at a time: nothing else could run meanwhile
+	Fiber* f = L->running;
+	f->waiting = false;
+#if EMU_ASAN
+	void* fake = nullptr;
+	__sanitizer_start_switch_fiber( &fake, L->schedulerStackBottom, L->schedulerStackSize );
+#endif
+	swapcontext( &f->ctx, &L->scheduler );
+#if EMU_ASAN
+	__sanitizer_finish_switch_fiber( fake, &L->schedulerStackBottom, &L->schedulerStackSize );
+#endif
+}
+
+// Every thread of every workgroup is a fiber of one scheduler. A workgroup's barrier opens when
+// all of its live threads wait at it; a polling thread (emu_yield) hands its turn on.
+void emu_launch_cooperative( dim3 grid, dim3 block, size_t sharedBytes, const std::function<void()>& body )
+{
+	if ( sharedBytes != 0 || grid.y != 1 || grid.z != 1 || block.y != 1 || block.z != 1 )
+	{
+		fprintf( stderr, "emu: cooperative launches take one-dimensional grids and no dynamic shared memory\n" );
+		abort();
+	}
+	const size_t threads = block.x, blocks = grid.x;
+	if ( threads == 0 || threads > 1024 || blocks == 0 || blocks > 64 )
+	{
+		fprintf( stderr, "emu: cooperative launch of %zu x %zu threads\n", blocks, threads );
+		abort();
+	}
+	Launch L;
+	L.body = &body;
+	L.cooperative = true;
+	L.fibers.resize( threads * blocks );
+	for ( size_t b = 0; b < blocks; ++b )
+		for ( size_t t = 0; t < threads; ++t )
+		{
+			Fiber& f = L.fibers[b * threads + t];
+			f.stack = takeStack();
+			f.state.tid = uint3{ (uint32_t)t, 0, 0 };
+			f.block = uint3{ (uint32_t)b, 0, 0 };
+			f.done = false;
+			f.waiting = false;
+			getcontext( &f.ctx );
+			f.ctx.uc_stack.ss_sp = f.stack;
+			f.ctx.uc_stack.ss_size = kStackBytes;
+			f.ctx.uc_link = nullptr;
+			makecontext( &f.ctx, fiberEntry, 0 );
+		}
+	Launch* outer = g_launch;
+	g_launch = &L;
+	emu_gridDim = grid;
+	emu_blockDim = block;
+#if EMU_ASAN
+	ASAN_POISON_MEMORY_REGION( reinterpret_cast<unsigned char*>( b2mi::lds ), b2mi::kEmuSharedBytes );
+#endif
+	for ( ;; )
+	{
+		// open the barriers whose workgroups are all there
+		for ( size_t b = 0; b < blocks; ++b )
+		{
+			size_t live = 0, waiting = 0;
+			for ( size_t t = 0; t < threads; ++t )
+			{
+				const Fiber& f = L.fibers[b * threads + t];
+				live += f.done ? 0 : 1;
+				waiting += ( !f.done && f.waiting ) ? 1 : 0;
+			}
+			if ( live > 0 && waiting == live )
+			{
+				if ( live != threads )
+				{
+					fprintf( stderr, "emu: %zu threads of workgroup %zu wait at a barrier that %zu others left\n", live, b,
+							 threads - live );
+					abort();
+				}
+				for ( size_t t = 0; t < threads; ++t )
+					L.fibers[b * threads + t].waiting = false;
+			}
+		}
+		size_t ran = 0, live = 0;
+		for ( Fiber& f : L.fibers )
+		{
+			if ( f.done )
+				continue;
+			live += 1;
+			if ( f.waiting )
+				continue;
+			emu_blockIdx = f.block;
+			runFiber( &L, &f );
+			ran += 1;
+		}
+		if ( live == 0 )
+			break;
+		if ( ran == 0 )
+		{
+			fprintf( stderr, "emu: cooperative launch deadlocked\n" );
+			abort();
+		}
+	}
+#if EMU_ASAN
+	ASAN_UNPOISON_MEMORY_REGION( reinterpret_cast<unsigned char*>( b2mi::lds ), b2mi::kEmuSharedBytes );
+#endif
+	{
+		std::lock_guard<std::mutex> lock( g_stackMutex );
+		for ( Fiber& f : L.fibers )
+			g_stackPool.push_back( f.stack );
+	}
+	g_launch = outer;
 }
 
 void emu_launch( dim3 grid, dim3 block, size_t sharedBytes, const std::function<void()>& body )
@@ -222,6 +337,46 @@ static ihipStream_t* const kStream = reinterpret_cast<ihipStream_t*>( 0x1 );
 hipError_t hipGetDeviceCount( int* count )
 {
 	*count = 1;
+	return hipSuccess;
+}
+hipError_t hipStreamDestroy( hipStream_t )
+{
+	return hipSuccess;
+}
+hipError_t hipStreamBeginCapture( hipStream_t, hipStreamCaptureMode )
+{
+	return hipErrorNotSupported;
+}
+hipError_t hipStreamEndCapture( hipStream_t, hipGraph_t* graph )
+{
+	*graph = nullptr;
+	return hipErrorNotSupported;
+}
+hipError_t hipGraphInstantiate( hipGraphExec_t* exec, hipGraph_t, void*, char*, size_t )
+{
+	*exec = nullptr;
+	return hipErrorNotSupported;
+}
+hipError_t hipGraphLaunch( hipGraphExec_t, hipStream_t )
+{
+	return hipErrorNotSupported;
+}
+hipError_t hipGraphExecDestroy( hipGraphExec_t )
+{
+	return hipSuccess;
+}
+hipError_t hipGraphDestroy( hipGraph_t )
+{
+	return hipSuccess;
+}
+hipError_t hipDeviceGetAttribute( int* value, hipDeviceAttribute_t attr, int )
+{
+	*value = attr == hipDeviceAttributeCooperativeLaunch ? 1 : attr == hipDeviceAttributeMultiprocessorCount ? 16 : 0;
+	return hipSuccess;
+}
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor( int* blocks, const void*, int, size_t )
+{
+	*blocks = 4; // 16 x 4: at most 64 workgroups in one cooperative launch
 	return hipSuccess;
 }
 hipError_t hipGetDevice( int* device )

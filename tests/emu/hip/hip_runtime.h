@@ -11,6 +11,8 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <type_traits>
+#include <utility>
 
 #define __global__
 #define __device__
@@ -91,6 +93,31 @@ extern thread_local dim3 emu_blockDim, emu_gridDim;
 
 void emu_syncthreads();
 #define __syncthreads() emu_syncthreads()
+// a thread that polls (s_sleep in a wait loop) gives the other threads of a cooperative launch
+// their turn
+void emu_yield();
+#define __builtin_amdgcn_s_sleep( n ) emu_yield()
+// one host thread runs every kernel thread in turn, so plain loads and stores are the device's
+// atomics and fences
+#define __threadfence() ( (void)0 )
+template <typename T>
+inline T __hip_atomic_load( const T* p, int, int )
+{
+	return *p;
+}
+template <typename T>
+inline void __hip_atomic_store( T* p, T v, int, int )
+{
+	*p = v;
+}
+template <typename T>
+inline T __hip_atomic_fetch_add( T* p, T v, int, int )
+{
+	T old = *p;
+	*p = old + v;
+	return old;
+}
+#define __HIP_MEMORY_SCOPE_AGENT 3
 
 inline int atomicAdd( int* p, int v )
 {
@@ -110,6 +137,31 @@ inline float atomicAdd( float* p, float v )
 	*p = old + v;
 	return old;
 }
+inline int atomicExch( int* p, int v )
+{
+	int old = *p;
+	*p = v;
+	return old;
+}
+inline unsigned long long atomicCAS( unsigned long long* p, unsigned long long compare, unsigned long long v )
+{
+	unsigned long long old = *p;
+	if ( old == compare )
+		*p = v;
+	return old;
+}
+inline uint32_t atomicOr( uint32_t* p, uint32_t v )
+{
+	uint32_t old = *p;
+	*p = old | v;
+	return old;
+}
+inline uint32_t atomicAnd( uint32_t* p, uint32_t v )
+{
+	uint32_t old = *p;
+	*p = old & v;
+	return old;
+}
 
 // ---- runtime API ----
 typedef enum hipError_t
@@ -118,6 +170,7 @@ typedef enum hipError_t
 	hipErrorInvalidValue = 1,
 	hipErrorOutOfMemory = 2,
 	hipErrorNoDevice = 100,
+	hipErrorNotSupported = 801,
 } hipError_t;
 
 typedef enum hipMemcpyKind
@@ -143,6 +196,23 @@ typedef ihipEvent_t* hipEvent_t;
 #define hipEventDisableTiming 2u
 #define hipHostMallocDefault 0u
 
+// stream capture is not emulated: a capture is refused, so callers launch directly
+struct ihipGraph_t;
+struct ihipGraphExec_t;
+typedef ihipGraph_t* hipGraph_t;
+typedef ihipGraphExec_t* hipGraphExec_t;
+typedef enum hipStreamCaptureMode
+{
+	hipStreamCaptureModeGlobal = 0,
+	hipStreamCaptureModeThreadLocal = 1,
+	hipStreamCaptureModeRelaxed = 2,
+} hipStreamCaptureMode;
+hipError_t hipStreamBeginCapture( hipStream_t s, hipStreamCaptureMode mode );
+hipError_t hipStreamEndCapture( hipStream_t s, hipGraph_t* graph );
+hipError_t hipGraphInstantiate( hipGraphExec_t* exec, hipGraph_t graph, void* errorNode, char* log, size_t logSize );
+hipError_t hipGraphLaunch( hipGraphExec_t exec, hipStream_t s );
+hipError_t hipGraphExecDestroy( hipGraphExec_t exec );
+hipError_t hipGraphDestroy( hipGraph_t graph );
 hipError_t hipGetDeviceCount( int* count );
 hipError_t hipGetDevice( int* device );
 // a __device__ variable is an ordinary host variable here, so its device address is its address
@@ -156,6 +226,7 @@ hipError_t hipMemcpyAsync( void* dst, const void* src, size_t bytes, hipMemcpyKi
 hipError_t hipMemsetAsync( void* dst, int value, size_t bytes, hipStream_t s );
 hipError_t hipStreamSynchronize( hipStream_t s );
 hipError_t hipStreamCreateWithFlags( hipStream_t* s, unsigned flags );
+hipError_t hipStreamDestroy( hipStream_t s );
 hipError_t hipStreamWaitEvent( hipStream_t s, hipEvent_t e, unsigned flags );
 hipError_t hipEventCreate( hipEvent_t* e );
 hipError_t hipEventCreateWithFlags( hipEvent_t* e, unsigned flags );
@@ -163,6 +234,13 @@ hipError_t hipEventRecord( hipEvent_t e, hipStream_t s );
 hipError_t hipEventElapsedTime( float* ms, hipEvent_t a, hipEvent_t b );
 hipError_t hipEventDestroy( hipEvent_t e );
 hipError_t hipFuncSetAttribute( const void* f, hipFuncAttribute attr, int value );
+typedef enum hipDeviceAttribute_t
+{
+	hipDeviceAttributeCooperativeLaunch = 1,
+	hipDeviceAttributeMultiprocessorCount = 2,
+} hipDeviceAttribute_t;
+hipError_t hipDeviceGetAttribute( int* value, hipDeviceAttribute_t attr, int device );
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor( int* blocks, const void* f, int blockSize, size_t sharedBytes );
 hipError_t hipGetLastError();
 const char* hipGetErrorString( hipError_t e );
 
@@ -171,3 +249,19 @@ void emu_launch( dim3 grid, dim3 block, size_t sharedBytes, const std::function<
 
 #define hipLaunchKernelGGL( kernel, grid, block, shared, stream, ... ) \
 	emu_launch( dim3( grid ), dim3( block ), ( shared ), [&]() { kernel( __VA_ARGS__ ); } )
+
+// A cooperative launch: every workgroup's threads are live at once (fibers of one scheduler, each
+// workgroup with its own barrier), so a grid-wide barrier built from atomics and polling works.
+void emu_launch_cooperative( dim3 grid, dim3 block, size_t sharedBytes, const std::function<void()>& body );
+template <typename... Args, size_t... I>
+inline void emu_call_kernel( void ( *k )( Args... ), void** args, std::index_sequence<I...> )
+{
+	k( *static_cast<typename std::remove_reference<Args>::type*>( args[I] )... );
+}
+template <typename... Args>
+inline hipError_t hipLaunchCooperativeKernel( void ( *k )( Args... ), dim3 grid, dim3 block, void** args, unsigned sharedBytes,
+											  hipStream_t )
+{
+	emu_launch_cooperative( grid, block, sharedBytes, [&]() { emu_call_kernel( k, args, std::index_sequence_for<Args...>{} ); } );
+	return hipSuccess;
+}

@@ -32,10 +32,14 @@ CASES = [
     "tests/test_gpu_parity.py::test_step_parity_solver_storage[hbm-tumbler]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[overflow_negzero]",
-    "tests/test_gpu_parity.py::test_step_parity_split_items[tumbler]",
-    "tests/test_gpu_parity.py::test_step_parity_split_items[overflow]",
+    "tests/test_gpu_parity.py::test_step_parity_split_items[per_phase-tumbler]",
+    "tests/test_gpu_parity.py::test_step_parity_split_items[grid3-tumbler]",
+    "tests/test_gpu_parity.py::test_step_parity_split_items[grid_auto-overflow]",
+    "tests/test_gpu_parity.py::test_step_parity_split_items[grid3-overflow_negzero]",
     "tests/test_gpu_parity.py::test_worlds_stepped_from_two_threads",
     "tests/test_gpu_parity.py::test_overflow_kinematic_negzero_set_after_first_step",
+    "tests/test_gpu_parity.py::test_step_parity_device_pairs[tumbler]",
+    "tests/test_gpu_parity.py::test_step_parity_device_pairs[shape_zoo]",
 ]
 
 

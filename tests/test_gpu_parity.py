@@ -123,9 +123,12 @@ def test_step_parity_solver_storage(name, kw, overrides, steps, substeps, budget
 
 @pytest.mark.parametrize("name", ["hello", "large_pyramid", "tumbler", "shape_zoo", "drop_boxes", "overflow",
                                   "overflow_negzero"])
-def test_step_parity_split_items(name):
-    """Every work item kept in HBM and run phase by phase over the grid (split threshold of one
-    slot): the per-phase launches give the oracle's bits, overflow levels and restitution included."""
+@pytest.mark.parametrize("grid", [-1, 0, 3], ids=["per_phase", "grid_auto", "grid3"])
+def test_step_parity_split_items(name, grid):
+    """Every work item kept in HBM and run over the whole grid (split threshold of one slot): as
+    one cooperative launch with grid-wide barriers (its automatic workgroup count, or three), or
+    as one launch per phase; each gives the oracle's bits, overflow levels and restitution
+    included."""
     scene = {s[0]: s for s in helpers.PARITY_SCENES}[name]
     _, kw, overrides, steps, substeps = scene
     gpu, cpu = helpers.product(), helpers.oracle()
@@ -133,6 +136,7 @@ def test_step_parity_split_items(name):
     wc, _ = helpers.make_world(cpu, name, kw, overrides)
     gpu.lib.b2mi_World_SetLdsBudget(wg, 0)
     gpu.lib.b2mi_World_SetSplitSlots(wg, 1)
+    gpu.lib.b2mi_World_SetGridBlocks(wg, grid)
     try:
         for s in range(min(steps, 40)):
             gpu.step(wg, substeps=substeps)
@@ -141,6 +145,9 @@ def test_step_parity_split_items(name):
         assert m is None, f"{name}: contacts differ: {m}"
         m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
         assert m is None, f"{name}: bodies differ: {m}"
+        items = gpu.work_items(world=wg)
+        modes = set(items[items[:, 0] > 0, 3].tolist())  # every item with a slot runs over the grid
+        assert not modes or modes == ({1} if grid < 0 else {2}), modes
     finally:
         gpu.destroy(wg)
         cpu.destroy(wc)
@@ -410,5 +417,84 @@ def test_island_sleep_fails_loudly():
         assert b"sleep" in gpu.lib.b2mi_GetLastError()
     finally:
         gpu.lib.b2SetAssertFcn(None)
+        gpu.destroy(wg)
+        cpu.destroy(wc)
+
+
+@pytest.mark.parametrize("case", ["pyramid", "per_phase", "grid", "batch"])
+def test_launch_graph_replays_keep_parity(case):
+    """Once a step enqueues the same solver launches as the step before, they are captured as a
+    HIP graph and replayed, with the step's own launches, copies and host work between replays:
+    every step still matches the oracle bit for bit, and the same run with graphs off gives the
+    same bits. The per-phase launches of a very large item are replayed from a graph too."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    if case == "batch":
+        batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 12, 4)
+        try:
+            for s in range(40):
+                gpu.batch_step(batch)
+                for wc in worlds_c:
+                    cpu.step(wc)
+                if s % 10 == 9:
+                    _compare_worlds(gpu, cpu, worlds_g, worlds_c, f"graph batch step {s}")
+            captures, replays = gpu.graph_stats(batch=batch)
+            assert replays > 0, (captures, replays)
+        finally:
+            gpu.lib.b2mi_DestroyBatch(batch)
+            for wc in worlds_c:
+                cpu.destroy(wc)
+        return
+    kw = {"base": 20}
+    runs = []
+    for graphs in (1, 0):
+        wg, _ = helpers.make_world(gpu, "large_pyramid", kw, {})
+        gpu.lib.b2mi_World_SetLaunchGraphs(wg, graphs)
+        if case != "pyramid":
+            gpu.lib.b2mi_World_SetLdsBudget(wg, 0)
+            gpu.lib.b2mi_World_SetSplitSlots(wg, 1)
+            gpu.lib.b2mi_World_SetGridBlocks(wg, -1 if case == "per_phase" else 2)
+        wc = helpers.make_world(cpu, "large_pyramid", kw, {})[0] if graphs else None
+        try:
+            for s in range(60):
+                gpu.step(wg)
+                if wc is not None:
+                    cpu.step(wc)
+                    m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+                    assert m is None, f"{case} step {s}: {m}"
+            runs.append((helpers.masked(gpu.dump_contacts(wg)), gpu.dump_bodies(wg), gpu.graph_stats(world=wg)))
+            if wc is not None:
+                m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+                assert m is None, f"{case}: {m}"
+        finally:
+            gpu.destroy(wg)
+            if wc is not None:
+                cpu.destroy(wc)
+    assert np.array_equal(runs[0][0].view(np.uint8), runs[1][0].view(np.uint8))
+    assert np.array_equal(runs[0][1].view(np.uint8), runs[1][1].view(np.uint8))
+    print(case, "graph captures/replays with graphs on:", runs[0][2], "off:", runs[1][2])
+    assert runs[1][2] == (0, 0)
+    if case != "grid":  # a cooperative launch may be refused by stream capture; then launches stay direct
+        assert runs[0][2][1] > 0, runs[0][2]
+
+
+@pytest.mark.parametrize("name,kw,overrides,steps,substeps", helpers.PARITY_SCENES,
+                         ids=[s[0] for s in helpers.PARITY_SCENES])
+def test_step_parity_device_pairs(name, kw, overrides, steps, substeps):
+    """Broadphase pair queries on the device for every step (threshold of zero moved proxies):
+    the new pairs, and so the contact ids, colours and everything after them, are the oracle's."""
+    gpu, cpu = helpers.product(), helpers.oracle()
+    wg, _ = helpers.make_world(gpu, name, kw, overrides)
+    wc, _ = helpers.make_world(cpu, name, kw, overrides)
+    gpu.lib.b2mi_World_SetDevicePairs(wg, 0)
+    try:
+        for s in range(steps):
+            gpu.step(wg, substeps=substeps)
+            cpu.step(wc, substeps=substeps)
+            if s % 10 == 9 or s < 3:
+                m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+                assert m is None, f"{name} step {s}: contacts differ: {m}"
+        m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+        assert m is None, f"{name}: bodies differ: {m}"
+    finally:
         gpu.destroy(wg)
         cpu.destroy(wc)
