@@ -407,6 +407,7 @@ class B2Api:
         L.b2mi_Batch_SetGridBlocks.argtypes = [c_int, c_int]
         L.b2mi_World_SetGridBlocks.argtypes = [WorldId, c_int]
         L.b2mi_World_SetDevicePairs.argtypes = [WorldId, c_int]
+        L.b2mi_World_GetDevicePairStats.argtypes = [WorldId, POINTER(c_float)]
         L.b2mi_Batch_SetLaunchGraphs.argtypes = [c_int, c_int]
         L.b2mi_World_SetLaunchGraphs.argtypes = [WorldId, c_int]
         L.b2mi_Batch_GetLaunchGraphStats.argtypes = [c_int, POINTER(c_int), POINTER(c_int)]
@@ -471,6 +472,13 @@ class B2Api:
         else:
             self.lib.b2mi_Batch_GetLaunchGraphStats(batch, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def device_pair_stats(self, world: WorldId) -> dict:
+        """The device pair queries' accumulated phase clocks (ms) and counts."""
+        v = (c_float * 9)()
+        self.lib.b2mi_World_GetDevicePairStats(world, v)
+        keys = ("table_ms", "uploads_ms", "queries_ms", "readback_ms", "contacts_ms", "pairs", "moves", "updates", "spilled")
+        return dict(zip(keys, list(v)))
 
     def world_timing(self, world: WorldId):
         a, b, c = c_float(), c_float(), c_float()

@@ -343,8 +343,9 @@ constexpr int pointField( int field, int p )
 // ---- broadphase pair queries on the device -------------------------------------------------
 
 // A node of a dynamic tree as the pair queries read it (uploaded from the host's trees, which keep
-// the reference's node layout and structure): its fat AABB, children (-1, -1 for a leaf), the
-// shape id of a leaf and its category bits.
+// the reference's node layout and structure): its fat AABB, children, the shape id of a leaf and
+// its category bits. A leaf has child1 = -1 and child2 = its position among the tree's leaves in the
+// order a depth-first query meets them (child2's subtree before child1's).
 struct DevTreeNode
 {
 	float lowerX, lowerY, upperX, upperY;
@@ -375,18 +376,35 @@ struct DevBroadphase
 	uint32_t pairMask;				 // table size - 1 (a power of two)
 };
 
-// a new pair found by the query of move `query`, the `rank`-th pair that query found
+// A new pair found by the query of move `query`. `rank` orders the pairs of one query as the
+// reference's depth-first query meets them: (tree's place in the query order) << 28 | leaf position.
 struct PairRecord
 {
 	int query, rank, shapeIdA, shapeIdB;
 };
 
+// A subtree that a query which ran out of its per-thread visit budget had still to visit:
+// `entry` = (tree << 30) | node. Each goes on in a workgroup of k_find_pairs_spilled.
+struct PairSpill
+{
+	int query;
+	uint32_t entry;
+};
+
 // pair-set slots: never a key (shape ids are below 2^31)
 constexpr unsigned long long kPairEmpty = ~0ull;
 constexpr unsigned long long kPairTombstone = ~0ull - 1;
-// threads per workgroup of k_find_pairs and depth of each thread's traversal stack (in LDS)
+// threads per workgroup of k_find_pairs, depth of each thread's traversal stack (in LDS) and the
+// nodes one thread visits before it spills the rest of its query
 constexpr int kPairBlock = 128;
 constexpr int kPairStack = 64;
+constexpr int kPairBudget = 512;
+// k_find_pairs_spilled: threads per workgroup, shared traversal stack entries, workgroups
+constexpr int kSpillBlock = 256;
+constexpr int kSpillStack = 12288;
+constexpr int kSpillGrid = 512;
+// the broadphase counters: pairs found, error, spilled subtrees
+constexpr int kPairCounters = 3;
 
 // Pose of a body at a step boundary (taped runs): center of mass, rotation, velocities
 struct StepPose

@@ -259,7 +259,7 @@ B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks )
 B2_API void b2mi_World_GetDevicePairStats( b2WorldId worldId, float* stats )
 {
 	World* w = getWorld( worldId );
-	for ( int i = 0; i < 8; ++i )
+	for ( int i = 0; i < 9; ++i )
 		stats[i] = w->devicePairs.phaseMs[i];
 }
 

@@ -38,7 +38,10 @@ hipError_t launch_pair_insert( unsigned long long* table, uint32_t mask, const u
 							   hipStream_t s );
 hipError_t launch_mark_moved( uint32_t* bits, const int* moves, int count, bool set, hipStream_t s );
 hipError_t launch_find_pairs( const DevBroadphase& bp, const int* moves, int moveCount, PairRecord* out, int capacity, int* counters,
-							  hipStream_t s );
+							  PairSpill* spills, int spillCapacity, hipStream_t s );
+// the spilled subtrees (`spilled` of them, as counted by launch_find_pairs)
+hipError_t launch_find_pairs_spilled( const DevBroadphase& bp, const int* moves, int spilled, PairRecord* out, int capacity, int* counters,
+									  const PairSpill* spills, int spillCapacity, hipStream_t s );
 // the pair set's hash on the host (the same probe sequence as the kernels')
 inline uint64_t pair_hash_host( uint64_t k )
 {

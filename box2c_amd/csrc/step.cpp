@@ -510,7 +510,10 @@ DeviceBroadphase::~DeviceBroadphase()
 	for ( DevTreeNode*& p : nodes )
 		if ( p )
 			(void)hipFree( p );
-	for ( void* p : { (void*)shapes, (void*)table, (void*)moved, (void*)moves, (void*)keys, (void*)pairs, (void*)counters } )
+	for ( DevTreeNode*& p : staging )
+		if ( p )
+			(void)hipHostFree( p );
+	for ( void* p : { (void*)shapes, (void*)table, (void*)moved, (void*)moves, (void*)keys, (void*)pairs, (void*)spills, (void*)counters } )
 		if ( p )
 			(void)hipFree( p );
 	if ( hostCounters )
@@ -527,11 +530,10 @@ static void syncPairTable( World* w, hipStream_t s )
 	const size_t live = w->pairSet.size();
 	if ( D.tableValid && 2 * ( live + (size_t)D.tableTombstones + D.pendingKeys.size() ) < D.tableSize )
 	{
+		// each key touched since the last update (a key may repeat): erased where the host's set
+		// no longer holds it, inserted where it does (both kernels take repeats)
 		std::vector<uint64_t>& keys = D.pendingKeys;
-		std::sort( keys.begin(), keys.end() );
-		keys.erase( std::unique( keys.begin(), keys.end() ), keys.end() );
-		std::stable_partition( keys.begin(), keys.end(), [&]( uint64_t k ) { return w->pairSet.count( k ) == 0; } );
-		const int erase = (int)( std::find_if( keys.begin(), keys.end(), [&]( uint64_t k ) { return w->pairSet.count( k ) != 0; } ) -
+		const int erase = (int)( std::partition( keys.begin(), keys.end(), [&]( uint64_t k ) { return w->pairSet.count( k ) == 0; } ) -
 								 keys.begin() );
 		const int n = (int)keys.size();
 		if ( n > 0 )
@@ -551,13 +553,12 @@ static void syncPairTable( World* w, hipStream_t s )
 	while ( size < 4 * live )
 		size *= 2;
 	std::vector<unsigned long long> table( size, kPairEmpty );
-	for ( uint64_t k : w->pairSet )
-	{
+	w->pairSet.forEach( [&]( uint64_t k ) {
 		uint32_t i = (uint32_t)pair_hash_host( k ) & ( size - 1 );
 		while ( table[i] != kPairEmpty )
 			i = ( i + 1 ) & ( size - 1 );
 		table[i] = k;
-	}
+	} );
 	if ( size != D.tableSize || D.table == nullptr )
 	{
 		if ( D.table )
@@ -589,10 +590,10 @@ static void updatePairsOnDevice( World* w )
 	auto mark = [&]( int k ) { D.phaseMs[k] += std::chrono::duration<float, std::milli>( std::chrono::steady_clock::now() - t0 ).count(); };
 	if ( D.counters == nullptr )
 	{
-		HIP_CHECK( hipMalloc( (void**)&D.counters, 2 * sizeof( int ) ) );
-		HIP_CHECK( hipHostMalloc( (void**)&D.hostCounters, 2 * sizeof( int ), hipHostMallocDefault ) );
+		HIP_CHECK( hipMalloc( (void**)&D.counters, kPairCounters * sizeof( int ) ) );
+		HIP_CHECK( hipHostMalloc( (void**)&D.hostCounters, kPairCounters * sizeof( int ), hipHostMallocDefault ) );
 	}
-	HIP_CHECK( hipMemsetAsync( D.counters, 0, 2 * sizeof( int ), s ) );
+	HIP_CHECK( hipMemsetAsync( D.counters, 0, kPairCounters * sizeof( int ), s ) );
 	syncPairTable( w, s );
 	mark( 0 );
 	DevBroadphase bp{};
@@ -605,17 +606,42 @@ static void updatePairsOnDevice( World* w )
 		if ( D.uploadedVersion[t] != tree.version || D.nodes[t] == nullptr || D.capNodes[t] < n )
 		{
 			grow( D.nodes[t], D.capNodes[t], std::max( n, 1 ) );
-			D.staging.resize( n );
+			if ( D.capStaging[t] < n )
+			{
+				// pinned, so the copy runs at full speed; the last update's copies have completed
+				if ( D.staging[t] )
+					HIP_CHECK( hipHostFree( D.staging[t] ) );
+				D.capStaging[t] = std::max( n, 2 * D.capStaging[t] );
+				HIP_CHECK( hipHostMalloc( (void**)&D.staging[t], sizeof( DevTreeNode ) * D.capStaging[t], hipHostMallocDefault ) );
+			}
+			DevTreeNode* staging = D.staging[t];
 			for ( int i = 0; i < n; ++i )
 			{
 				const TreeNode& src = tree.nodes[i];
-				D.staging[i] = DevTreeNode{ src.aabb.lowerBound.x, src.aabb.lowerBound.y, src.aabb.upperBound.x, src.aabb.upperBound.y,
+				staging[i] = DevTreeNode{ src.aabb.lowerBound.x, src.aabb.lowerBound.y, src.aabb.upperBound.x, src.aabb.upperBound.y,
 											src.height == 0 ? -1 : src.child1, src.height == 0 ? -1 : src.child2, src.userData,
 											src.categoryBits };
 			}
+			// each leaf's position in the depth-first order of a query (child2's subtree first)
+			int position = 0;
+			D.walk.clear();
+			if ( tree.root != -1 )
+				D.walk.push_back( tree.root );
+			while ( !D.walk.empty() )
+			{
+				const int id = D.walk.back();
+				D.walk.pop_back();
+				DevTreeNode& node = staging[id];
+				if ( node.child1 == -1 )
+				{
+					node.child2 = position++;
+					continue;
+				}
+				D.walk.push_back( node.child1 );
+				D.walk.push_back( node.child2 );
+			}
 			if ( n > 0 )
-				HIP_CHECK( hipMemcpyAsync( D.nodes[t], D.staging.data(), sizeof( DevTreeNode ) * n, hipMemcpyHostToDevice, s ) );
-			HIP_CHECK( hipStreamSynchronize( s ) ); // the staging copy is reused for the next tree
+				HIP_CHECK( hipMemcpyAsync( D.nodes[t], staging, sizeof( DevTreeNode ) * n, hipMemcpyHostToDevice, s ) );
 			D.uploadedVersion[t] = tree.version;
 		}
 		bp.nodes[t] = D.nodes[t];
@@ -654,22 +680,32 @@ static void updatePairsOnDevice( World* w )
 	bp.pairs = D.table;
 	bp.pairMask = D.tableSize - 1;
 	grow( D.pairs, D.capPairs, std::max( 1024, moveCount ) );
+	grow( D.spills, D.capSpills, 4096 );
 	HIP_CHECK( launch_mark_moved( D.moved, D.moves, moveCount, true, s ) );
 	int found = 0;
 	for ( int attempt = 0;; ++attempt )
 	{
-		HIP_CHECK( launch_find_pairs( bp, D.moves, moveCount, D.pairs, D.capPairs, D.counters, s ) );
-		HIP_CHECK( hipMemcpyAsync( D.hostCounters, D.counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( launch_find_pairs( bp, D.moves, moveCount, D.pairs, D.capPairs, D.counters, D.spills, D.capSpills, s ) );
+		HIP_CHECK( hipMemcpyAsync( D.hostCounters, D.counters, kPairCounters * sizeof( int ), hipMemcpyDeviceToHost, s ) );
 		HIP_CHECK( hipStreamSynchronize( s ) );
+		const int spilled = D.hostCounters[2];
+		if ( spilled > 0 && spilled <= D.capSpills && D.hostCounters[1] == 0 )
+		{
+			HIP_CHECK( launch_find_pairs_spilled( bp, D.moves, spilled, D.pairs, D.capPairs, D.counters, D.spills, D.capSpills, s ) );
+			HIP_CHECK( hipMemcpyAsync( D.hostCounters, D.counters, kPairCounters * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+			HIP_CHECK( hipStreamSynchronize( s ) );
+			D.phaseMs[8] += (float)spilled;
+		}
 		if ( D.hostCounters[1] != 0 )
 			fatal( "a device pair query overflowed its traversal stack or the pair table", __FILE__, __LINE__ );
 		found = D.hostCounters[0];
-		if ( found <= D.capPairs )
+		if ( found <= D.capPairs && spilled <= D.capSpills )
 			break;
 		B2MI_CHECK( attempt == 0 );
-		// more pairs than the buffer holds: a larger one, and the queries once more
+		// more pairs or spilled subtrees than the buffers hold: larger ones, and the queries once more
 		grow( D.pairs, D.capPairs, found );
-		HIP_CHECK( hipMemsetAsync( D.counters, 0, sizeof( int ), s ) );
+		grow( D.spills, D.capSpills, spilled );
+		HIP_CHECK( hipMemsetAsync( D.counters, 0, kPairCounters * sizeof( int ), s ) );
 	}
 	mark( 2 );
 	HIP_CHECK( launch_mark_moved( D.moved, D.moves, moveCount, false, s ) );

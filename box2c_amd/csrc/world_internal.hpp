@@ -168,12 +168,112 @@ struct StepGroup;
 // The device side of a world's pair queries: uploaded copies of the trees and shape filters, the
 // pair set as an open-addressing table kept in step with the host's by the keys created and
 // destroyed since it was last brought up to date, and the buffers of one query launch.
+// The set of shape pair keys that have a contact (src/broad_phase.h pairSet): open addressing with
+// linear probing over a power-of-two table kept at most half full, erase by backward shift (no
+// tombstones). Only membership matters; the order of a walk over the keys is the table's.
+class PairSet
+{
+public:
+	size_t size() const
+	{
+		return count_;
+	}
+	bool count( uint64_t key ) const
+	{
+		if ( count_ == 0 )
+			return false;
+		for ( size_t i = home( key );; i = ( i + 1 ) & mask_ )
+		{
+			if ( slots_[i] == key )
+				return true;
+			if ( slots_[i] == kEmpty )
+				return false;
+		}
+	}
+	void insert( uint64_t key )
+	{
+		if ( 2 * ( count_ + 1 ) > slots_.size() )
+			grow();
+		size_t i = home( key );
+		while ( slots_[i] != kEmpty )
+		{
+			if ( slots_[i] == key )
+				return;
+			i = ( i + 1 ) & mask_;
+		}
+		slots_[i] = key;
+		count_ += 1;
+	}
+	void erase( uint64_t key )
+	{
+		if ( count_ == 0 )
+			return;
+		size_t i = home( key );
+		while ( slots_[i] != key )
+		{
+			if ( slots_[i] == kEmpty )
+				return;
+			i = ( i + 1 ) & mask_;
+		}
+		// backward shift: move later keys of the run into the hole when their home allows it
+		size_t j = i;
+		for ( ;; )
+		{
+			j = ( j + 1 ) & mask_;
+			if ( slots_[j] == kEmpty )
+				break;
+			const size_t h = home( slots_[j] );
+			// the key at j may fill the hole at i unless its home lies cyclically in (i, j]
+			if ( ( ( j - h ) & mask_ ) >= ( ( j - i ) & mask_ ) )
+			{
+				slots_[i] = slots_[j];
+				i = j;
+			}
+		}
+		slots_[i] = kEmpty;
+		count_ -= 1;
+	}
+	template <typename F>
+	void forEach( F&& f ) const
+	{
+		for ( uint64_t k : slots_ )
+			if ( k != kEmpty )
+				f( k );
+	}
+
+private:
+	static constexpr uint64_t kEmpty = ~0ull; // never a key: shape ids are below 2^31
+	size_t home( uint64_t k ) const
+	{
+		k ^= k >> 33;
+		k *= 0xff51afd7ed558ccdull;
+		k ^= k >> 33;
+		return (size_t)k & mask_;
+	}
+	void grow()
+	{
+		std::vector<uint64_t> old;
+		old.swap( slots_ );
+		slots_.assign( old.empty() ? 1024 : 2 * old.size(), kEmpty );
+		mask_ = slots_.size() - 1;
+		count_ = 0;
+		for ( uint64_t k : old )
+			if ( k != kEmpty )
+				insert( k );
+	}
+	std::vector<uint64_t> slots_;
+	size_t mask_ = 0;
+	size_t count_ = 0;
+};
+
 struct DeviceBroadphase
 {
 	DevTreeNode* nodes[3] = {};
 	int capNodes[3] = {};
 	uint64_t uploadedVersion[3] = { ~0ull, ~0ull, ~0ull };
-	std::vector<DevTreeNode> staging;
+	DevTreeNode* staging[3] = {}; // pinned host copies of the trees as uploaded
+	int capStaging[3] = {};
+	std::vector<int> walk; // the depth-first walk that numbers the leaves of a staged tree
 	DevShapeFilter* shapes = nullptr;
 	int capShapes = 0;
 	uint64_t uploadedShapes = ~0ull;
@@ -190,12 +290,15 @@ struct DeviceBroadphase
 	int capKeys = 0;
 	PairRecord* pairs = nullptr;
 	int capPairs = 0;
-	int* counters = nullptr;	 // [0] pairs found, [1] error
+	PairSpill* spills = nullptr;
+	int capSpills = 0;
+	int* counters = nullptr;	 // [0] pairs found, [1] error, [2] spilled subtrees
 	int* hostCounters = nullptr; // pinned
 	std::vector<PairRecord> found;
 	// accumulated host clock (ms since each update began) after: the pair-set update, the uploads,
-	// the queries, the readback and the contact creation; then pairs found, moves, updates
-	float phaseMs[8] = {};
+	// the queries, the readback and the contact creation; then pairs found, moves, updates and
+	// spilled subtrees
+	float phaseMs[9] = {};
 	~DeviceBroadphase();
 };
 
@@ -258,11 +361,11 @@ struct World
 	DynamicTree trees[3];
 	BitSet moveSet;
 	std::vector<int> moveArray;
-	std::unordered_set<uint64_t> pairSet;
+	PairSet pairSet;
 	uint64_t shapesVersion = 0; // shapes created, destroyed or refiltered
 	// pair queries of a world stepped alone run on the device from this many moved proxies on (-1:
-	// never, the default); in a batch they run on the host pool, world by world
-	int devicePairsMin = -1;
+	// never); in a batch they run on the host pool, world by world
+	int devicePairsMin = 2048;
 	DeviceBroadphase devicePairs;
 
 	b2Vec2 gravity{ 0.0f, -10.0f };
@@ -357,7 +460,7 @@ struct StepGroup
 	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
 	int ldsFullBudget = -1;	  // max LDS bytes of an item keeping all its state in LDS (-1 = ldsBudget)
 	int splitSlots = -1;	  // HBM-store items of at least this many slots run over the grid (-1 = kDefaultSplitSlots, 0 = never)
-	int gridBlocks = 0;		  // workgroups of such an item's cooperative launch (0 = from its largest colour, -1 = one launch per phase)
+	int gridBlocks = -1;	  // workgroups of such an item's cooperative launch (0 = from its largest colour, -1 = one launch per phase)
 	unsigned* dGridSync = nullptr; // barrier arrivals and vote of the cooperative grid solver
 	int builtTarget = -1, builtBudget = -1, builtFullBudget = -1;				   // packing target and LDS budget of the current items
 	bool structureDirty = true;

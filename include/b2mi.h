@@ -102,19 +102,21 @@ B2_API void b2mi_Batch_SetSplitSlots( int batch, int slots );
 B2_API void b2mi_World_SetSplitSlots( b2WorldId worldId, int slots );
 
 // Workgroups (of 256 threads) of such an item's cooperative launch: 0 = one thread per slot of
-// its largest colour group, -1 = one launch per phase instead; at most what the device holds at
-// once. Results are the same bits for every count.
+// its largest colour group, -1 = one launch per phase instead (the default: replayed as a launch
+// graph it measured faster than the grid barrier on the base-100 pyramid); at most what the
+// device holds at once. Results are the same bits for every count.
 B2_API void b2mi_Batch_SetGridBlocks( int batch, int blocks );
 B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks );
 
 // Broadphase pair queries of a world stepped alone run on the device (one thread per moved
 // proxy over uploaded copies of the trees, the contacts then created on the host in the
-// reference's order) when at least `minMoves` proxies moved; -1 = always on the host (the
-// default), 0 = always on the device. Results are the same bits either way.
+// reference's order) when at least `minMoves` proxies moved; -1 = always on the host, 0 = always
+// on the device; default 2048. Results are the same bits either way.
 B2_API void b2mi_World_SetDevicePairs( b2WorldId worldId, int minMoves );
-// Accumulated over the device pair updates so far: host clock (ms since each update began) after
-// the pair-set update, the uploads, the queries, the readback and the contact creation; then the
-// pairs found, the moved proxies and the number of updates.
+// Accumulated over the device pair updates so far (9 values): host clock (ms since each update
+// began) after the pair-set update, the uploads, the queries, the readback and the contact
+// creation; then the pairs found, the moved proxies, the number of updates and the subtrees that
+// queries over their visit budget handed to the workgroup pass.
 B2_API void b2mi_World_GetDevicePairStats( b2WorldId worldId, float* stats );
 
 // HIP graphs for the solver launches (on by default): when a step enqueues the same solver

@@ -137,6 +137,12 @@ inline float atomicAdd( float* p, float v )
 	*p = old + v;
 	return old;
 }
+inline int atomicSub( int* p, int v )
+{
+	int old = *p;
+	*p = old - v;
+	return old;
+}
 inline int atomicExch( int* p, int v )
 {
 	int old = *p;

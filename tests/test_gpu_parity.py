@@ -495,6 +495,11 @@ def test_step_parity_device_pairs(name, kw, overrides, steps, substeps):
                 assert m is None, f"{name} step {s}: contacts differ: {m}"
         m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
         assert m is None, f"{name}: bodies differ: {m}"
+        stats = gpu.device_pair_stats(wg)
+        assert stats["updates"] > 0, stats
+        if name == "tumbler":
+            # the container walls' queries run past one thread's visit budget
+            assert stats["spilled"] > 0, stats
     finally:
         gpu.destroy(wg)
         cpu.destroy(wc)
