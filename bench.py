@@ -50,24 +50,39 @@ def algorithmic_bytes(n_bodies: int, n_contacts: int, substeps: int) -> dict:
     return {"solve": solve, "collide": collide, "step": solve + collide}
 
 
-def roofline(workload: str, n_bodies: int, n_contacts: int, ms_solve: float) -> dict:
-    """The solver kernel's algorithmic-byte rate against the HBM peak, with the measured HBM
-    bytes per launch of the same workload when a counter summary is committed for it."""
+def roofline(workload: str, n_bodies: int, n_contacts: int, ms_solve: float, ms_collide: float = 0.0) -> dict:
+    """The solver's algorithmic-byte rate against the HBM peak (achieved / peak = frac), with the
+    measured HBM bytes of the same workload's solver per step (`traffic`, from the committed
+    rocprofv3 counter summary profiles/hbm_<workload>.json when there is one) and the rate they
+    give; the narrowphase kernel's figures beside them."""
     ab = algorithmic_bytes(n_bodies, n_contacts, SUBSTEPS)
     achieved = ab["solve"] / (ms_solve * 1e-3) / 1e9 if ms_solve > 0 else 0.0
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
            "traffic": None, "kernel": "k_solve", "algorithmic_bytes_per_launch": ab["solve"],
-           "note": "achieved = algorithmic bytes (each field once per pass) / mean k_solve duration; the kernel keeps "
-                   "work items in LDS across passes, so this can exceed what crosses HBM: measured_* is the HBM traffic"}
+           "note": "achieved = algorithmic bytes (each field once per pass) / the step's solver time (HIP events around its "
+                   "launches); the kernel keeps work items in LDS across passes, so this can exceed what crosses HBM: "
+                   "traffic and measured_* are the HBM bytes rocprofv3 counts"}
+    collide = {"kernel": "k_collide", "algorithmic_bytes_per_launch": ab["collide"],
+               "achieved": ab["collide"] / (ms_collide * 1e-3) / 1e9 if ms_collide > 0 else 0.0}
     tfile = os.path.join(ROOT, "profiles", f"hbm_{workload}.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        if t.get("workload") == workload and t.get("k_solve_bytes_per_launch"):
-            traffic = float(t["k_solve_bytes_per_launch"])
+        if t.get("workload") == workload and t.get("solve_bytes_per_step"):
+            traffic = float(t["solve_bytes_per_step"])
+            launches = int(t.get("solve_launches_per_step", 1))
             out["traffic"] = traffic
+            if launches > 1:
+                out["kernel"] = f"solver ({launches} launches per step)"
+                out["traffic_note"] = "per step: the sum over the step's solver launches"
             rate = traffic / (ms_solve * 1e-3) / 1e9 if ms_solve > 0 else 0.0
             out["measured_hbm_GBs"] = rate
             out["measured_hbm_frac"] = rate / HBM_PEAK_GBS
+            if t.get("collide_bytes_per_launch") and ms_collide > 0:
+                collide["traffic"] = float(t["collide_bytes_per_launch"])
+                collide["measured_hbm_GBs"] = collide["traffic"] / (ms_collide * 1e-3) / 1e9
+                collide["measured_hbm_frac"] = collide["measured_hbm_GBs"] / HBM_PEAK_GBS
+                collide["traffic_over_algorithmic"] = collide["traffic"] / ab["collide"] if ab["collide"] else None
+    out["collide"] = collide
     return out
 
 
@@ -208,7 +223,7 @@ def world_leg(api, name: str, scene: str, kw: dict, steps: int, warmup: int, hos
         "touching_contacts": r["contacts"], "graph_colors": r["colors"], "steps": steps, "warmup": warmup,
         "ms_per_step": 1e3 * r["elapsed"] / steps, "ms_solve_kernel": r["ms_solve"], "ms_collide_kernel": r["ms_collide"],
         "ms_step_phases": r["host_ms"], "work_items": r["work_items"],
-        "roofline": roofline(name, r["bodies"], r["contacts"], r["ms_solve"]), "cpu_baseline": None,
+        "roofline": roofline(name, r["bodies"], r["contacts"], r["ms_solve"], r["ms_collide"]), "cpu_baseline": None,
     }
     if cpu:
         progress(f"{name}: CPU reference, {cpu_steps} steps at {host['threads']} threads and the thread curve")
@@ -339,7 +354,7 @@ def batched_leg(api, args, host: dict, cpu: bool):
         "ms_per_step": 1e3 * rb["elapsed"] / args.batch_steps, "ms_solve_kernel": rb["ms_solve"],
         "ms_collide_kernel": rb["ms_collide"], "ms_host": rb["ms_host"], "work_items": rb["work_items"],
         "launch_shape": rb["launch_shape"], "cpu_baseline": None,
-        "roofline": roofline(name, rb["bodies"], rb["slots"], rb["ms_solve"]),
+        "roofline": roofline(name, rb["bodies"], rb["slots"], rb["ms_solve"], rb["ms_collide"]),
     }
     k = min(args.batch_steps, args.batch_cpu_steps)
     leg["value_cpu_window"] = rb["bodies"] * SUBSTEPS * k / sum(rb["per_step"][:k])

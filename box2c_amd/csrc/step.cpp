@@ -2134,7 +2134,10 @@ void StepGroup::launchSolve( hipStream_t s, const int* abortFlag )
 		for ( hipEvent_t& e : collideDone )
 			HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
 	}
-	if ( launchGraphs && !graphsRefused )
+	// a graph pays for itself over a sequence of launches: per-phase launches of a split item or
+	// launches on several streams; one plain launch goes straight to the stream
+	const bool sequence = ( !splitItems.empty() && splitPerPhase ) || classes.size() > 1;
+	if ( launchGraphs && !graphsRefused && sequence )
 	{
 		const SolveSignature key = solveSignature( abortFlag );
 		if ( graphExec != nullptr && memcmp( &key, &graphKey, sizeof( key ) ) == 0 )

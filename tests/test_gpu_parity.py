@@ -426,10 +426,13 @@ def test_launch_graph_replays_keep_parity(case):
     """Once a step enqueues the same solver launches as the step before, they are captured as a
     HIP graph and replayed, with the step's own launches, copies and host work between replays:
     every step still matches the oracle bit for bit, and the same run with graphs off gives the
-    same bits. The per-phase launches of a very large item are replayed from a graph too."""
+    same bits. The per-phase launches of a very large item are replayed from a graph too; a step
+    whose solver is one plain launch (the pyramid here) is not captured."""
     gpu, cpu = helpers.product(), helpers.oracle()
     if case == "batch":
-        batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 12, 4)
+        batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 10, 5)
+        # two stores (as in the mixed-store test), so two launches side by side every step
+        gpu.lib.b2mi_Batch_SetLdsBudget(batch, 96 * 1024)
         try:
             for s in range(40):
                 gpu.batch_step(batch)
@@ -473,7 +476,9 @@ def test_launch_graph_replays_keep_parity(case):
     assert np.array_equal(runs[0][1].view(np.uint8), runs[1][1].view(np.uint8))
     print(case, "graph captures/replays with graphs on:", runs[0][2], "off:", runs[1][2])
     assert runs[1][2] == (0, 0)
-    if case != "grid":  # a cooperative launch may be refused by stream capture; then launches stay direct
+    if case == "pyramid":
+        assert runs[0][2] == (0, 0), runs[0][2]
+    elif case == "per_phase":
         assert runs[0][2][1] > 0, runs[0][2]
 
 
