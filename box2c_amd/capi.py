@@ -126,6 +126,38 @@ class Profile(Structure):
         "hitEvents broadphase continuous").split()]
 
 
+class BodyEvents(Structure):
+    _fields_ = [("moveEvents", c_void_p), ("moveCount", c_int32)]
+
+
+class SensorEvents(Structure):
+    _fields_ = [("beginEvents", c_void_p), ("endEvents", c_void_p), ("beginCount", c_int32), ("endCount", c_int32)]
+
+
+class ContactEvents(Structure):
+    _fields_ = [("beginEvents", c_void_p), ("endEvents", c_void_p), ("hitEvents", c_void_p),
+                ("beginCount", c_int32), ("endCount", c_int32), ("hitCount", c_int32)]
+
+
+_ID_DTYPE = np.dtype([("index1", "<i4"), ("world0", "<u2"), ("revision", "<u2")])
+TOUCH_EVENT_DTYPE = np.dtype([("shapeIdA", _ID_DTYPE), ("shapeIdB", _ID_DTYPE)])
+HIT_EVENT_DTYPE = np.dtype([("shapeIdA", _ID_DTYPE), ("shapeIdB", _ID_DTYPE), ("point", "<f4", 2), ("normal", "<f4", 2),
+                            ("approachSpeed", "<f4")])
+MOVE_EVENT_DTYPE = np.dtype({"names": ["transform", "bodyId", "userData", "fellAsleep"],
+                             "formats": [("<f4", 4), _ID_DTYPE, "<u8", "u1"], "offsets": [0, 16, 24, 32], "itemsize": 40})
+
+
+def _event_array(ptr, count, dtype):
+    """A copy of `count` events at `ptr`, field by field (struct padding reads as zero)."""
+    raw = np.zeros(count, dtype=dtype)
+    if count > 0:
+        ctypes.memmove(raw.ctypes.data, ptr, count * dtype.itemsize)
+    out = np.zeros(count, dtype=dtype)
+    for f in dtype.names:
+        out[f] = raw[f]
+    return out
+
+
 class ContactRecord(Structure):
     _fields_ = [("contactId", c_int32), ("shapeIdA", c_int32), ("shapeIdB", c_int32),
                 ("colorIndex", c_int32), ("localIndex", c_int32), ("simFlags", c_uint32),
@@ -254,6 +286,24 @@ class B2Api:
             f = getattr(L, name)
             f.argtypes = [POINTER(sa), Transform, POINTER(sb), Transform]
             f.restype = Manifold
+        L.b2World_GetBodyEvents.argtypes = [WorldId]
+        L.b2World_GetBodyEvents.restype = BodyEvents
+        L.b2World_GetSensorEvents.argtypes = [WorldId]
+        L.b2World_GetSensorEvents.restype = SensorEvents
+        L.b2World_GetContactEvents.argtypes = [WorldId]
+        L.b2World_GetContactEvents.restype = ContactEvents
+        L.b2Shape_EnableContactEvents.argtypes = [ShapeId, c_bool]
+        L.b2Shape_AreContactEventsEnabled.argtypes = [ShapeId]
+        L.b2Shape_AreContactEventsEnabled.restype = c_bool
+        L.b2Shape_EnableHitEvents.argtypes = [ShapeId, c_bool]
+        L.b2Shape_AreHitEventsEnabled.argtypes = [ShapeId]
+        L.b2Shape_AreHitEventsEnabled.restype = c_bool
+        L.b2Body_EnableHitEvents.argtypes = [BodyId, c_bool]
+        L.b2Body_GetShapeCount.argtypes = [BodyId]
+        L.b2Body_GetShapeCount.restype = c_int
+        L.b2Body_GetShapes.argtypes = [BodyId, POINTER(ShapeId), c_int]
+        L.b2Body_GetShapes.restype = c_int
+        L.b2World_SetHitEventThreshold.argtypes = [WorldId, c_float]
         L.b2mi_World_DumpContacts.argtypes = [WorldId, c_void_p, c_int]
         L.b2mi_World_DumpContacts.restype = c_int
         L.b2mi_World_DumpBodies.argtypes = [WorldId, c_void_p, c_int]
@@ -362,6 +412,25 @@ class B2Api:
         else:
             raise ValueError(f"unknown scene {name}")
         return w, list(ids)[:n]
+
+    def body_shapes(self, body: BodyId) -> list:
+        n = self.lib.b2Body_GetShapeCount(body)
+        arr = (ShapeId * max(n, 1))()
+        m = self.lib.b2Body_GetShapes(body, arr, n)
+        return [arr[i] for i in range(m)]
+
+    # ---- events of the last step ----
+    def events(self, world: WorldId) -> dict:
+        """The last step's events as arrays: contact begin/end/hit, sensor begin/end, body moves."""
+        c = self.lib.b2World_GetContactEvents(world)
+        se = self.lib.b2World_GetSensorEvents(world)
+        b = self.lib.b2World_GetBodyEvents(world)
+        return {"begin": _event_array(c.beginEvents, c.beginCount, TOUCH_EVENT_DTYPE),
+                "end": _event_array(c.endEvents, c.endCount, TOUCH_EVENT_DTYPE),
+                "hit": _event_array(c.hitEvents, c.hitCount, HIT_EVENT_DTYPE),
+                "sensor_begin": _event_array(se.beginEvents, se.beginCount, TOUCH_EVENT_DTYPE),
+                "sensor_end": _event_array(se.endEvents, se.endCount, TOUCH_EVENT_DTYPE),
+                "move": _event_array(b.moveEvents, b.moveCount, MOVE_EVENT_DTYPE)}
 
     # ---- inspection ----
     def dump_contacts(self, world: WorldId) -> np.ndarray:

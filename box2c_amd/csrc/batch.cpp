@@ -192,6 +192,8 @@ B2_API void b2mi_Batch_SetWorldChunks( int batch, int chunks )
 B2_API void b2mi_Batch_Step( int batch, float timeStep, int subStepCount )
 {
 	StepGroup* g = getBatch( batch );
+	for ( World* w : g->worlds )
+		clearStepEvents( w );
 	if ( timeStep == 0.0f || g->worlds.empty() )
 		return;
 	g->step( timeStep, subStepCount );

@@ -278,7 +278,7 @@ static void createContact( World* w, Shape* shapeA, Shape* shapeB )
 	c.localIndex = (int)w->nonTouching.size();
 	c.shapeIdA = shapeA->id;
 	c.shapeIdB = shapeB->id;
-	c.flags = 0;
+	c.flags = ( shapeA->enableContactEvents || shapeB->enableContactEvents ) ? kContactEvents : 0u;
 	Body* bodies[2] = { &w->bodies[shapeA->bodyId], &w->bodies[shapeB->bodyId] };
 	for ( int e = 0; e < 2; ++e )
 	{
@@ -1952,28 +1952,52 @@ static void forEachWorld( StepGroup* g, const std::function<void( World* )>& fn 
 }
 
 // serial state changes of one world in contact-id order (src/world.c:506-667)
+// the last step's events go when the next step begins (src/world.c:685-692)
+void clearStepEvents( World* w )
+{
+	w->moveEvents.clear();
+	w->sensorBegin.clear();
+	w->sensorEnd.clear();
+	w->contactBegin.clear();
+	w->contactEnd.clear();
+	w->contactHit.clear();
+	w->lazyEvents = false;
+}
+
 static void applyChanges( World* w, const ChangeRecord* recs, int count )
 {
+	const uint16_t world0 = (uint16_t)w->slot;
 	for ( int i = 0; i < count; ++i )
 	{
 		const ChangeRecord& r = recs[i];
 		Contact& c = w->contacts[r.contactId];
 		int colorIndex = c.colorIndex, localIndex = c.localIndex;
+		// the touch events of this change (src/world.c:544-551, 583-590, 632-639)
+		const bool events = ( c.flags & kContactEvents ) != 0;
+		const Shape& shapeA = w->shapes[c.shapeIdA];
+		const Shape& shapeB = w->shapes[c.shapeIdB];
+		const b2ShapeId shapeIdA{ shapeA.id + 1, world0, shapeA.revision }, shapeIdB{ shapeB.id + 1, world0, shapeB.revision };
 		if ( r.flags & kSimDisjoint )
 		{
-			c.flags &= ~1u;
+			if ( events && ( c.flags & kContactTouching ) )
+				w->contactEnd.push_back( b2ContactEndTouchEvent{ shapeIdA, shapeIdB } );
+			c.flags &= ~kContactTouching;
 			destroyContactImpl( w, &c );
 		}
 		else if ( r.flags & kSimStarted )
 		{
-			c.flags |= 1u;
+			if ( events )
+				w->contactBegin.push_back( b2ContactBeginTouchEvent{ shapeIdA, shapeIdB } );
+			c.flags |= kContactTouching;
 			addContactToGraph( w, c );
 			removeFromList( w, w->nonTouching, localIndex );
 			markContactBodies( w, c );
 		}
 		else if ( r.flags & kSimStopped )
 		{
-			c.flags &= ~1u;
+			if ( events )
+				w->contactEnd.push_back( b2ContactEndTouchEvent{ shapeIdA, shapeIdB } );
+			c.flags &= ~kContactTouching;
 			int bodyIdA = c.edges[0].bodyId, bodyIdB = c.edges[1].bodyId;
 			c.colorIndex = -1;
 			c.localIndex = (int)w->nonTouching.size();
@@ -2299,6 +2323,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	float msPairs = 0.0f, msBroad = 0.0f, msChanges = 0.0f, msEnlarge = 0.0f;
 	hipStream_t s = stream();
 	B2MI_CHECK( stepping == false );
+	for ( World* w : worlds )
+		clearStepEvents( w );
 	stepping = true;
 	profile = b2Profile{};
 	const int nw = (int)worlds.size();
@@ -2637,8 +2663,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 	profile.broadphase = msBroad;
 	profile.buildIslands = msChanges;
 	profile.finalizeBodies = msEnlarge;
-	// device state moved on: host mirrors and downloaded contacts of every world are stale
+	// device state moved on: host mirrors and downloaded contacts of every world are stale, and
+	// the move and hit events of the step are there to gather
 	epoch += 1;
+	for ( World* w : worlds )
+		w->lazyEvents = true;
 	stepping = false;
 }
 
@@ -2719,6 +2748,41 @@ B2_API float b2Shape_GetDensity( b2ShapeId shapeId )
 
 // Friction and restitution are mixed into a contact when it is created
 // (src/contact.c:323-324), so changing them affects contacts created afterwards.
+// src/shape.c:1010-1065: contact events apply to contacts created afterwards, hit events from the
+// next step's narrowphase on
+B2_API void b2Shape_EnableContactEvents( b2ShapeId shapeId, bool flag )
+{
+	B2MI_SHAPE( shapeId );
+	if ( worldLocked( w ) )
+		return;
+	shape->enableContactEvents = flag;
+}
+
+B2_API bool b2Shape_AreContactEventsEnabled( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->enableContactEvents;
+}
+
+B2_API void b2Shape_EnableHitEvents( b2ShapeId shapeId, bool flag )
+{
+	B2MI_SHAPE( shapeId );
+	if ( worldLocked( w ) )
+		return;
+	b2mi_internal_beginEdit( w );
+	shape->enableHitEvents = flag;
+	if ( flag )
+		w->devShapes[shape->id].flags |= kShapeHitEvents;
+	else
+		w->devShapes[shape->id].flags &= ~kShapeHitEvents;
+}
+
+B2_API bool b2Shape_AreHitEventsEnabled( b2ShapeId shapeId )
+{
+	B2MI_SHAPE( shapeId );
+	return shape->enableHitEvents;
+}
+
 B2_API void b2Shape_SetFriction( b2ShapeId shapeId, float friction )
 {
 	B2MI_SHAPE( shapeId );
@@ -2799,7 +2863,7 @@ B2_API int b2Shape_GetContactData( b2ShapeId shapeId, b2ContactData* contactData
 	for ( int key = body.headContactKey; key != -1 && index < capacity; )
 	{
 		const Contact& c = w->contacts[key >> 1];
-		if ( ( c.shapeIdA == shape->id || c.shapeIdB == shape->id ) && ( c.flags & 1u ) )
+		if ( ( c.shapeIdA == shape->id || c.shapeIdB == shape->id ) && ( c.flags & kContactTouching ) )
 		{
 			const Shape& a = w->shapes[c.shapeIdA];
 			const Shape& b = w->shapes[c.shapeIdB];
@@ -2824,6 +2888,7 @@ B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount )
 	w->profile = b2Profile{};
 	if ( w->group )
 		w->group->profile = b2Profile{};
+	clearStepEvents( w );
 	if ( timeStep == 0.0f )
 		return;
 	if ( w->group == nullptr )

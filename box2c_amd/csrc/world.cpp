@@ -261,6 +261,76 @@ static void downloadContacts( World* w )
 	w->contactsEpoch = stepEpoch( w );
 }
 
+// The last step's move and hit events, gathered from the device state on first demand: a move
+// event per awake body in awake order with its transform after finalize (src/solver.c:226-231),
+// and for each touching contact with hit events, colour by colour in local order, the manifold
+// point of the fastest approach above the threshold that took an impulse (src/solver.c:1670-1723).
+// Begin/end touch events were recorded by the step's state changes.
+void gatherLazyEvents( World* w )
+{
+	if ( !w->lazyEvents )
+		return;
+	w->lazyEvents = false;
+	const uint16_t world0 = (uint16_t)w->slot;
+	const int na = (int)w->awakeSimBody.size();
+	w->moveEvents.resize( na );
+	if ( na > 0 )
+	{
+		std::vector<DevPose> poses( na );
+		HIP_CHECK( hipMemcpyAsync( poses.data(), w->dPoses, sizeof( DevPose ) * na, hipMemcpyDeviceToHost, stream() ) );
+		HIP_CHECK( hipStreamSynchronize( stream() ) );
+		for ( int i = 0; i < na; ++i )
+		{
+			const Body& b = w->bodies[w->awakeSimBody[i]];
+			b2BodyMoveEvent& e = w->moveEvents[i];
+			e.transform = b2Transform{ { poses[i].p[0], poses[i].p[1] }, { poses[i].q[0], poses[i].q[1] } };
+			e.bodyId = b2BodyId{ b.id + 1, world0, b.revision };
+			e.userData = b.userData;
+			e.fellAsleep = false;
+		}
+	}
+	bool hits = false;
+	for ( const Shape& sh : w->shapes )
+		if ( sh.id >= 0 && sh.enableHitEvents )
+		{
+			hits = true;
+			break;
+		}
+	if ( !hits )
+		return;
+	downloadContacts( w );
+	for ( int color = 0; color < kColorCount; ++color )
+		for ( int contactId : w->colors[color] )
+		{
+			const DevContact& dc = w->devContacts[contactId];
+			if ( ( dc.simFlags & kSimHitEvent ) == 0 )
+				continue;
+			const DevContactOut& out = w->devContactOut[contactId];
+			b2ContactHitEvent event{};
+			event.approachSpeed = w->hitEventThreshold;
+			bool hit = false;
+			for ( int k = 0; k < dc.manifold.pointCount; ++k )
+			{
+				const float approachSpeed = -out.normalVelocity[k];
+				if ( approachSpeed > event.approachSpeed && dc.manifold.points[k].normalImpulse > 0.0f )
+				{
+					event.approachSpeed = approachSpeed;
+					event.point = b2Vec2{ out.pointX[k], out.pointY[k] };
+					hit = true;
+				}
+			}
+			if ( !hit )
+				continue;
+			const Contact& c = w->contacts[contactId];
+			const Shape& a = w->shapes[c.shapeIdA];
+			const Shape& b = w->shapes[c.shapeIdB];
+			event.normal = b2Vec2{ dc.manifold.normalX, dc.manifold.normalY };
+			event.shapeIdA = b2ShapeId{ a.id + 1, world0, a.revision };
+			event.shapeIdB = b2ShapeId{ b.id + 1, world0, b.revision };
+			w->contactHit.push_back( event );
+		}
+}
+
 // writes the host mirror (bodies, shapes, per-body shape lists) to the device
 static void uploadBodies( World* w )
 {
@@ -342,10 +412,12 @@ static void uploadBodies( World* w )
 		w->group->structureDirty = true;
 }
 
-// the host mirror is about to be modified by an API call
+// the host mirror is about to be modified by an API call (the last step's events are taken
+// first, as the step left them)
 static void beginEdit( World* w )
 {
 	B2MI_CHECK( worldLocked( w ) == false );
+	gatherLazyEvents( w );
 	downloadMirror( w );
 	w->bodiesDirty = true;
 }
@@ -507,7 +579,37 @@ B2_API void b2World_SetRestitutionThreshold( b2WorldId worldId, float value )
 
 B2_API void b2World_SetHitEventThreshold( b2WorldId worldId, float value )
 {
-	getWorld( worldId )->hitEventThreshold = b2ClampFloat( value, 0.0f, FLT_MAX );
+	World* w = getWorld( worldId );
+	gatherLazyEvents( w ); // the last step's hit events used the threshold of that step
+	w->hitEventThreshold = b2ClampFloat( value, 0.0f, FLT_MAX );
+}
+
+// src/world.c:1366-1413
+B2_API b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId )
+{
+	World* w = getWorld( worldId );
+	if ( worldLocked( w ) )
+		return b2BodyEvents{};
+	gatherLazyEvents( w );
+	return b2BodyEvents{ w->moveEvents.data(), (int32_t)w->moveEvents.size() };
+}
+
+B2_API b2SensorEvents b2World_GetSensorEvents( b2WorldId worldId )
+{
+	World* w = getWorld( worldId );
+	if ( worldLocked( w ) )
+		return b2SensorEvents{};
+	return b2SensorEvents{ w->sensorBegin.data(), w->sensorEnd.data(), (int32_t)w->sensorBegin.size(), (int32_t)w->sensorEnd.size() };
+}
+
+B2_API b2ContactEvents b2World_GetContactEvents( b2WorldId worldId )
+{
+	World* w = getWorld( worldId );
+	if ( worldLocked( w ) )
+		return b2ContactEvents{};
+	gatherLazyEvents( w );
+	return b2ContactEvents{ w->contactBegin.data(),		  w->contactEnd.data(),			  w->contactHit.data(),
+							(int32_t)w->contactBegin.size(), (int32_t)w->contactEnd.size(), (int32_t)w->contactHit.size() };
 }
 
 // A custom filter or pre-solve callback would have to run inside the device narrowphase and
@@ -853,7 +955,23 @@ B2_API b2BodyType b2Body_GetType( b2BodyId bodyId )
 B2_API void b2Body_SetUserData( b2BodyId bodyId, void* userData )
 {
 	B2MI_BODY( bodyId );
+	gatherLazyEvents( w ); // the last step's move events carry the user data they had
 	body->userData = userData;
+}
+
+// src/body.c:1654-1664
+B2_API void b2Body_EnableHitEvents( b2BodyId bodyId, bool enableHitEvents )
+{
+	B2MI_BODY( bodyId );
+	beginEdit( w );
+	for ( int id = body->headShapeId; id != -1; id = w->shapes[id].nextShapeId )
+	{
+		w->shapes[id].enableHitEvents = enableHitEvents;
+		if ( enableHitEvents )
+			w->devShapes[id].flags |= kShapeHitEvents;
+		else
+			w->devShapes[id].flags &= ~kShapeHitEvents;
+	}
 }
 
 B2_API void* b2Body_GetUserData( b2BodyId bodyId )
@@ -1140,7 +1258,7 @@ B2_API int b2Body_GetContactData( b2BodyId bodyId, b2ContactData* contactData, i
 	for ( int key = body->headContactKey; key != -1 && index < capacity; )
 	{
 		const Contact& c = w->contacts[key >> 1];
-		if ( c.flags & 1u )
+		if ( c.flags & kContactTouching )
 		{
 			const Shape& a = w->shapes[c.shapeIdA];
 			const Shape& b = w->shapes[c.shapeIdB];

@@ -160,8 +160,12 @@ struct Contact
 	ContactEdge edges[2];
 	int shapeIdA = -1, shapeIdB = -1;
 	int contactId = -1;
-	uint32_t flags = 0; // bit0 touching
+	uint32_t flags = 0; // kContactTouching, kContactEvents
 };
+
+// Contact::flags (src/contact.h b2ContactFlags)
+constexpr uint32_t kContactTouching = 0x1u;
+constexpr uint32_t kContactEvents = 0x20u; // either shape had contact events enabled at creation
 
 struct StepGroup;
 
@@ -367,6 +371,17 @@ struct World
 	// never); in a batch they run on the host pool, world by world
 	int devicePairsMin = 2048;
 	DeviceBroadphase devicePairs;
+
+	// events of the last step (src/world.c:1366-1413). Begin/end touch events are recorded by the
+	// state changes; move and hit events are gathered from the device state when first asked for,
+	// or before an API call changes what they report (lazyEvents marks them as still to gather).
+	std::vector<b2BodyMoveEvent> moveEvents;
+	std::vector<b2SensorBeginTouchEvent> sensorBegin;
+	std::vector<b2SensorEndTouchEvent> sensorEnd;
+	std::vector<b2ContactBeginTouchEvent> contactBegin;
+	std::vector<b2ContactEndTouchEvent> contactEnd;
+	std::vector<b2ContactHitEvent> contactHit;
+	bool lazyEvents = false;
 
 	b2Vec2 gravity{ 0.0f, -10.0f };
 	float hitEventThreshold = 1, restitutionThreshold = 1, contactPushoutVelocity = 3;
@@ -587,6 +602,10 @@ inline uint64_t stepEpoch( const World* w )
 {
 	return w->group != nullptr ? w->group->epoch : 0;
 }
+
+// step.cpp / world.cpp: the events of a world's last step
+void clearStepEvents( World* w );
+void gatherLazyEvents( World* w );
 
 // tape.cpp
 void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSize );

@@ -2,9 +2,10 @@
 // reference's include/box2d/box2d.h. Each declaration keeps the reference's name, signature
 // and argument meaning; the reference line that declares it is cited so a maintainer can
 // check the binding. In a step, the narrowphase, the solver stages and finalize run as HIP
-// kernels on the GPU; the order-defining bookkeeping (broadphase pair update and dynamic-tree
-// maintenance, contact creation, begin/end-touch processing and graph colouring, work-item
-// building) runs on the host. See b2mi.h for the device/batch/adjoint extensions.
+// kernels on the GPU, and so do the broadphase pair queries of a world stepped alone with many
+// moved proxies; the order-defining bookkeeping (dynamic-tree maintenance, contact creation,
+// begin/end-touch processing and its events, graph colouring, work-item building) runs on the
+// host. See b2mi.h for the device/batch/adjoint extensions.
 #pragma once
 
 #include "base.h"
@@ -17,6 +18,9 @@ B2_API b2WorldId b2CreateWorld( const b2WorldDef* def );											  // :27
 B2_API void b2DestroyWorld( b2WorldId worldId );														  // :30
 B2_API bool b2World_IsValid( b2WorldId id );															  // :33
 B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount );						  // :39
+B2_API b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId );											  // :45
+B2_API b2SensorEvents b2World_GetSensorEvents( b2WorldId worldId );										  // :48
+B2_API b2ContactEvents b2World_GetContactEvents( b2WorldId worldId );									  // :51
 B2_API void b2World_EnableSleeping( b2WorldId worldId, bool flag );										  // :100
 B2_API void b2World_EnableContinuous( b2WorldId worldId, bool flag );									  // :106
 B2_API void b2World_SetRestitutionThreshold( b2WorldId worldId, float value );							  // :111
@@ -66,6 +70,7 @@ B2_API void b2Body_SetGravityScale( b2BodyId bodyId, float gravityScale );						
 B2_API float b2Body_GetGravityScale( b2BodyId bodyId );													  // :339
 B2_API bool b2Body_IsAwake( b2BodyId bodyId );															  // :342
 B2_API bool b2Body_IsBullet( b2BodyId bodyId );															  // :381
+B2_API void b2Body_EnableHitEvents( b2BodyId bodyId, bool enableHitEvents );								  // :385
 B2_API int b2Body_GetShapeCount( b2BodyId bodyId );														  // :388
 B2_API int b2Body_GetShapes( b2BodyId bodyId, b2ShapeId* shapeArray, int capacity );					  // :392
 B2_API int b2Body_GetContactCapacity( b2BodyId bodyId );												  // :402
@@ -86,6 +91,10 @@ B2_API float b2Shape_GetFriction( b2ShapeId shapeId );													   // :475
 B2_API void b2Shape_SetRestitution( b2ShapeId shapeId, float restitution );								   // :479
 B2_API float b2Shape_GetRestitution( b2ShapeId shapeId );												   // :482
 B2_API b2Filter b2Shape_GetFilter( b2ShapeId shapeId );													   // :485
+B2_API void b2Shape_EnableContactEvents( b2ShapeId shapeId, bool flag );									   // :500
+B2_API bool b2Shape_AreContactEventsEnabled( b2ShapeId shapeId );											   // :503
+B2_API void b2Shape_EnableHitEvents( b2ShapeId shapeId, bool flag );										   // :515
+B2_API bool b2Shape_AreHitEventsEnabled( b2ShapeId shapeId );												   // :518
 B2_API b2Circle b2Shape_GetCircle( b2ShapeId shapeId );													   // :527
 B2_API b2Segment b2Shape_GetSegment( b2ShapeId shapeId );												   // :530
 B2_API b2Capsule b2Shape_GetCapsule( b2ShapeId shapeId );												   // :537

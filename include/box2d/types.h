@@ -163,6 +163,79 @@ typedef struct b2ContactData
 	b2Manifold manifold;
 } b2ContactData;
 
+// ---- events of the last step (reference include/box2d/types.h:876-1005) ----
+// Filled by b2World_Step and valid until the next step; the arrays are owned by the world.
+
+// a shape began (end: stopped) overlapping a sensor shape
+typedef struct b2SensorBeginTouchEvent
+{
+	b2ShapeId sensorShapeId;
+	b2ShapeId visitorShapeId;
+} b2SensorBeginTouchEvent;
+
+typedef struct b2SensorEndTouchEvent
+{
+	b2ShapeId sensorShapeId;
+	b2ShapeId visitorShapeId;
+} b2SensorEndTouchEvent;
+
+typedef struct b2SensorEvents
+{
+	b2SensorBeginTouchEvent* beginEvents;
+	b2SensorEndTouchEvent* endEvents;
+	int32_t beginCount;
+	int32_t endCount;
+} b2SensorEvents;
+
+// two shapes began (end: stopped) touching
+typedef struct b2ContactBeginTouchEvent
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+} b2ContactBeginTouchEvent;
+
+typedef struct b2ContactEndTouchEvent
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+} b2ContactEndTouchEvent;
+
+// two shapes met faster than the world's hit event threshold: the point of the fastest
+// approach, the manifold normal (A to B) and that approach speed (positive)
+typedef struct b2ContactHitEvent
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+	b2Vec2 point;
+	b2Vec2 normal;
+	float approachSpeed;
+} b2ContactHitEvent;
+
+typedef struct b2ContactEvents
+{
+	b2ContactBeginTouchEvent* beginEvents;
+	b2ContactEndTouchEvent* endEvents;
+	b2ContactHitEvent* hitEvents;
+	int32_t beginCount;
+	int32_t endCount;
+	int32_t hitCount;
+} b2ContactEvents;
+
+// one per body the step moved (every awake body, in awake order): its new transform
+typedef struct b2BodyMoveEvent
+{
+	b2Transform transform;
+	b2BodyId bodyId;
+	void* userData;
+	bool fellAsleep;
+} b2BodyMoveEvent;
+
+typedef struct b2BodyEvents
+{
+	b2BodyMoveEvent* moveEvents;
+	int32_t moveCount;
+} b2BodyEvents;
+
 // User filter and pre-solve hooks. They would run inside the data-parallel narrowphase;
 // this library refuses them (see b2World_SetCustomFilterCallback).
 typedef bool b2CustomFilterFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, void* context );

@@ -381,6 +381,14 @@ typedef struct orWorld
 	int stepIndex;
 
 	b2Profile profile;
+
+	// events of the last step (src/world.h: bodyMoveEventArray, sensor/contact event arrays)
+	OR_ARRAY( b2BodyMoveEvent ) moveEvents;
+	OR_ARRAY( b2SensorBeginTouchEvent ) sensorBegin;
+	OR_ARRAY( b2SensorEndTouchEvent ) sensorEnd;
+	OR_ARRAY( b2ContactBeginTouchEvent ) contactBegin;
+	OR_ARRAY( b2ContactEndTouchEvent ) contactEnd;
+	OR_ARRAY( b2ContactHitEvent ) contactHit;
 } orWorld;
 
 typedef struct orStepContext

@@ -557,6 +557,13 @@ static void or_finalize_bodies( orWorld* world, const orStepContext* ctx, orBitS
 #else
 		sim->transform.p = b2Sub( sim->center, b2RotateVector( sim->transform.q, sim->localCenter ) );
 #endif
+		// solver.c:226-231
+		b2BodyMoveEvent* moveEvent = world->moveEvents.data + simIndex;
+		moveEvent->transform = sim->transform;
+		moveEvent->bodyId = ( b2BodyId ){ sim->bodyId + 1, (uint16_t)world->worldId, body->revision };
+		moveEvent->userData = body->userData;
+		moveEvent->fellAsleep = false;
+
 		sim->force = b2Vec2_zero;
 		sim->torque = 0.0f;
 		body->isSpeedCapped = sim->isSpeedCapped;
@@ -749,11 +756,49 @@ void or_solve( orWorld* world, orStepContext* ctx )
 		or_lane_store( cons + i );
 	free( cons );
 
-	// finalize (solver.c:1628-1652)
+	// finalize (solver.c:1628-1652), one move event per awake body (solver.c:1170-1172)
+	OR_RESERVE( world->moveEvents, awakeBodyCount );
+	world->moveEvents.count = awakeBodyCount;
 	orBitSet enlargedSims = { 0 };
 	or_bitset_set_count_and_clear( &enlargedSims, (uint32_t)awakeBodyCount );
 	orIntArray fastBodies = { 0 };
 	or_finalize_bodies( world, ctx, &enlargedSims, &fastBodies );
+
+	// hit events (solver.c:1670-1723): every colour, overflow included, in local order; the
+	// manifold point of the fastest approach above the threshold that received an impulse
+	for ( int c = 0; c < OR_GRAPH_COLORS; ++c )
+	{
+		const orGraphColor* color = world->colors + c;
+		for ( int i = 0; i < color->contacts.count; ++i )
+		{
+			const orContactSim* cs = color->contacts.data + i;
+			if ( ( cs->simFlags & OR_SIM_HIT_EVENT ) == 0 )
+				continue;
+			b2ContactHitEvent event = { 0 };
+			event.approachSpeed = world->hitEventThreshold;
+			bool hit = false;
+			for ( int p = 0; p < cs->manifold.pointCount; ++p )
+			{
+				const b2ManifoldPoint* mp = cs->manifold.points + p;
+				float approachSpeed = -mp->normalVelocity;
+				if ( approachSpeed > event.approachSpeed && mp->normalImpulse > 0.0f )
+				{
+					event.approachSpeed = approachSpeed;
+					event.point = mp->point;
+					hit = true;
+				}
+			}
+			if ( hit )
+			{
+				event.normal = cs->manifold.normal;
+				const orShape* shapeA = world->shapes.data + cs->shapeIdA;
+				const orShape* shapeB = world->shapes.data + cs->shapeIdB;
+				event.shapeIdA = ( b2ShapeId ){ shapeA->id + 1, (uint16_t)world->worldId, shapeA->revision };
+				event.shapeIdB = ( b2ShapeId ){ shapeB->id + 1, (uint16_t)world->worldId, shapeB->revision };
+				OR_PUSH( world->contactHit, event );
+			}
+		}
+	}
 
 	// enlarge proxies in sim order, building the next move array (solver.c:1741-1797)
 	for ( uint32_t w = 0; w < enlargedSims.blockCount; ++w )
@@ -811,6 +856,13 @@ B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount )
 	orWorld* world = or_get_world( worldId );
 	if ( world->locked )
 		return;
+	// the last step's events go (world.c:685-692)
+	world->moveEvents.count = 0;
+	world->sensorBegin.count = 0;
+	world->sensorEnd.count = 0;
+	world->contactBegin.count = 0;
+	world->contactEnd.count = 0;
+	world->contactHit.count = 0;
 	memset( &world->profile, 0, sizeof( world->profile ) );
 	if ( timeStep == 0.0f )
 		return;

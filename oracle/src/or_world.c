@@ -173,6 +173,12 @@ B2_API void b2DestroyWorld( b2WorldId worldId )
 	or_bitset_free( &world->moveSet );
 	OR_FREE( world->moveArray );
 	or_keyset_free( &world->pairSet );
+	OR_FREE( world->moveEvents );
+	OR_FREE( world->sensorBegin );
+	OR_FREE( world->sensorEnd );
+	OR_FREE( world->contactBegin );
+	OR_FREE( world->contactEnd );
+	OR_FREE( world->contactHit );
 	uint16_t revision = world->revision;
 	memset( world, 0, sizeof( *world ) );
 	world->worldId = OR_NULL;
@@ -1535,7 +1541,8 @@ void or_collide( orWorld* world, orStepContext* ctx )
 	for ( int i = 0; i < world->awakeContacts.count; ++i )
 		or_collide_one( world, world->awakeContacts.data + i, &stateBits );
 
-	// serial state changes in contact-id order (world.c:506-667)
+	// serial state changes in contact-id order (world.c:506-667), with their begin/end touch
+	// events (world.c:544-551, 583-590, 632-639)
 	for ( uint32_t k = 0; k < stateBits.blockCount; ++k )
 	{
 		uint64_t bits = stateBits.bits[k];
@@ -1547,13 +1554,28 @@ void or_collide( orWorld* world, orStepContext* ctx )
 			int localIndex = contact->localIndex;
 			orContactSim* cs = or_get_contact_sim( world, contact );
 			uint32_t simFlags = cs->simFlags;
+			const orShape* shapeA = world->shapes.data + contact->shapeIdA;
+			const orShape* shapeB = world->shapes.data + contact->shapeIdB;
+			b2ShapeId shapeIdA = { shapeA->id + 1, (uint16_t)world->worldId, shapeA->revision };
+			b2ShapeId shapeIdB = { shapeB->id + 1, (uint16_t)world->worldId, shapeB->revision };
+			bool events = ( contact->flags & OR_CONTACT_ENABLE_CONTACT_EVENTS ) != 0;
 			if ( simFlags & OR_SIM_DISJOINT )
 			{
+				if ( ( contact->flags & OR_CONTACT_TOUCHING ) && events )
+				{
+					b2ContactEndTouchEvent e = { shapeIdA, shapeIdB };
+					OR_PUSH( world->contactEnd, e );
+				}
 				contact->flags &= ~OR_CONTACT_TOUCHING;
 				or_destroy_contact( world, contact );
 			}
 			else if ( simFlags & OR_SIM_STARTED )
 			{
+				if ( events )
+				{
+					b2ContactBeginTouchEvent e = { shapeIdA, shapeIdB };
+					OR_PUSH( world->contactBegin, e );
+				}
 				contact->flags |= OR_CONTACT_TOUCHING;
 				cs->simFlags &= ~OR_SIM_STARTED;
 				or_add_contact_to_graph( world, cs, contact );
@@ -1563,6 +1585,11 @@ void or_collide( orWorld* world, orStepContext* ctx )
 			{
 				cs->simFlags &= ~OR_SIM_STOPPED;
 				contact->flags &= ~OR_CONTACT_TOUCHING;
+				if ( contact->flags & OR_CONTACT_ENABLE_CONTACT_EVENTS )
+				{
+					b2ContactEndTouchEvent e = { shapeIdA, shapeIdB };
+					OR_PUSH( world->contactEnd, e );
+				}
 				int bodyIdA = contact->edges[0].bodyId, bodyIdB = contact->edges[1].bodyId;
 				// b2AddNonTouchingContact then b2RemoveContactFromGraph
 				contact->colorIndex = OR_NULL;
@@ -1575,6 +1602,66 @@ void or_collide( orWorld* world, orStepContext* ctx )
 		}
 	}
 	or_bitset_free( &stateBits );
+}
+
+// ---- events (src/world.c:1366-1413; src/shape.c:1010-1065; src/body.c:1654-1664) ----
+B2_API b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId )
+{
+	orWorld* world = or_get_world( worldId );
+	if ( world->locked )
+		return ( b2BodyEvents ){ 0 };
+	return ( b2BodyEvents ){ world->moveEvents.data, world->moveEvents.count };
+}
+
+B2_API b2SensorEvents b2World_GetSensorEvents( b2WorldId worldId )
+{
+	orWorld* world = or_get_world( worldId );
+	if ( world->locked )
+		return ( b2SensorEvents ){ 0 };
+	return ( b2SensorEvents ){ world->sensorBegin.data, world->sensorEnd.data, world->sensorBegin.count, world->sensorEnd.count };
+}
+
+B2_API b2ContactEvents b2World_GetContactEvents( b2WorldId worldId )
+{
+	orWorld* world = or_get_world( worldId );
+	if ( world->locked )
+		return ( b2ContactEvents ){ 0 };
+	return ( b2ContactEvents ){ world->contactBegin.data, world->contactEnd.data, world->contactHit.data,
+								world->contactBegin.count, world->contactEnd.count, world->contactHit.count };
+}
+
+B2_API void b2Shape_EnableContactEvents( b2ShapeId shapeId, bool flag )
+{
+	orWorld* world = or_get_world_index( shapeId.world0 );
+	if ( world->locked )
+		return;
+	or_get_shape_full( world, shapeId )->enableContactEvents = flag;
+}
+
+B2_API bool b2Shape_AreContactEventsEnabled( b2ShapeId shapeId )
+{
+	return or_get_shape_full( or_get_world_index( shapeId.world0 ), shapeId )->enableContactEvents;
+}
+
+B2_API void b2Shape_EnableHitEvents( b2ShapeId shapeId, bool flag )
+{
+	orWorld* world = or_get_world_index( shapeId.world0 );
+	if ( world->locked )
+		return;
+	or_get_shape_full( world, shapeId )->enableHitEvents = flag;
+}
+
+B2_API bool b2Shape_AreHitEventsEnabled( b2ShapeId shapeId )
+{
+	return or_get_shape_full( or_get_world_index( shapeId.world0 ), shapeId )->enableHitEvents;
+}
+
+B2_API void b2Body_EnableHitEvents( b2BodyId bodyId, bool enableHitEvents )
+{
+	orWorld* world = or_get_world_index( bodyId.world0 );
+	orBody* body = or_get_body_full( world, bodyId );
+	for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
+		world->shapes.data[id].enableHitEvents = enableHitEvents;
 }
 
 // ---- inspection (include/b2mi.h) ----
