@@ -10,7 +10,8 @@ solve, integrate positions, relax), restitution, store impulses) and finalize.
 
 Further legs, each an object in the same JSON line: the reference's own CPU benchmark scene
 (large pyramid of base 100), the batch of 4,096 uneven pyramid worlds, the differentiable
-1,024-world drop-box rollout with its adjoint, and the 40,000-shape tumbler.
+1,024-world drop-box rollout with its adjoint, the 40,000-shape tumbler, and the reference's own
+joint benchmarks (its motorized tumbler with 2,025 boxes, and the 100 x 100 joint grid).
 
 Each leg carries
   roofline      its solver kernel (k_solve, the dominant launch): algorithmic bytes per launch
@@ -432,7 +433,9 @@ def main():
     ap.add_argument("--tumbler-shapes", type=int, default=40000, help="shapes of the tumbler leg (0 = skip it)")
     ap.add_argument("--tumbler-steps", type=int, default=600)
     ap.add_argument("--tumbler-cpu-steps", type=int, default=600, help="steps of the tumbler leg's CPU sample")
-    ap.add_argument("--legs", default="plain,large_pyramid,batched,adjoint,tumbler",
+    ap.add_argument("--joint-tumbler-steps", type=int, default=750, help="timed steps of the joint tumbler leg")
+    ap.add_argument("--joint-grid-steps", type=int, default=500, help="timed steps of the joint grid leg")
+    ap.add_argument("--legs", default="plain,large_pyramid,batched,adjoint,tumbler,joint_tumbler,joint_grid",
                     help="comma-separated legs to run (a profiling run may keep one; the JSON line then lacks the others)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain copy kernel of known size (for HBM counter scaling)")
@@ -486,6 +489,15 @@ def main():
         out["tumbler"] = world_leg(api, f"tumbler_{args.tumbler_shapes}", "tumbler", {"count": args.tumbler_shapes, "seed": 5},
                                    args.tumbler_steps, args.warmup, host, cpu, min(args.tumbler_steps, args.tumbler_cpu_steps),
                                    min(args.tumbler_steps, args.tumbler_cpu_steps, 30))
+    # the reference's own joint benchmarks (benchmark/tumbler.c: a motorized revolute container with
+    # 45 x 45 boxes, 750 steps; benchmark/joint_grid.c: 100 x 100 circles on 19,800 revolute joints
+    # in zero gravity, 500 steps), each after one warm-up step as benchmark/main.c:194-205 runs them
+    if "joint_tumbler" in legs and args.joint_tumbler_steps > 0:
+        out["joint_tumbler"] = world_leg(api, "joint_tumbler_45x45", "joint_tumbler", {"grid": 45}, args.joint_tumbler_steps, 1, host,
+                                         cpu, args.joint_tumbler_steps, min(args.joint_tumbler_steps, 100))
+    if "joint_grid" in legs and args.joint_grid_steps > 0:
+        out["joint_grid"] = world_leg(api, "joint_grid_100x100", "joint_grid", {"N": 100, "gravity": 0.0}, args.joint_grid_steps, 1,
+                                      host, cpu, args.joint_grid_steps, min(args.joint_grid_steps, 100))
     out["host"] = host
     progress("done")
     print(json.dumps(out), flush=True)

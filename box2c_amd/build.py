@@ -21,8 +21,8 @@ SCENES = os.path.join(HERE, "libb2mi_scenes.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("B2MI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "adjoint.hip", "broadphase.hip", "world.cpp", "step.cpp", "batch.cpp", "tape.cpp", "tree.cpp", "geometry.cpp",
-           "collide_api.cpp"]
+SOURCES = ["kernels.hip", "adjoint.hip", "broadphase.hip", "world.cpp", "joint.cpp", "step.cpp", "batch.cpp", "tape.cpp", "tree.cpp",
+           "geometry.cpp", "collide_api.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -44,6 +44,10 @@ class ShapeId(Structure):
     _fields_ = [("index1", c_int32), ("world0", c_uint16), ("revision", c_uint16)]
 
 
+class JointId(Structure):
+    _fields_ = [("index1", c_int32), ("world0", c_uint16), ("revision", c_uint16)]
+
+
 class WorldDef(Structure):
     _fields_ = [("gravity", Vec2), ("restitutionThreshold", c_float),
                 ("contactPushoutVelocity", c_float), ("hitEventThreshold", c_float),
@@ -191,6 +195,7 @@ CONTACT_DTYPE = np.dtype([("contactId", "<i4"), ("shapeIdA", "<i4"), ("shapeIdB"
 BODY_DTYPE = np.dtype([("bodyId", "<i4"), ("type", "<i4"), ("setIndex", "<i4"),
                        ("localIndex", "<i4"), ("p", "<f4", 2), ("q", "<f4", 2),
                        ("center", "<f4", 2), ("v", "<f4", 2), ("w", "<f4"), ("pad", "<f4")])
+JOINT_DTYPE = np.dtype([("jointId", "<i4"), ("type", "<i4"), ("force", "<f4", 2), ("torque", "<f4")])
 assert CONTACT_DTYPE.itemsize == ctypes.sizeof(ContactRecord)
 assert BODY_DTYPE.itemsize == ctypes.sizeof(BodyRecord)
 
@@ -304,6 +309,37 @@ class B2Api:
         L.b2Body_GetShapes.argtypes = [BodyId, POINTER(ShapeId), c_int]
         L.b2Body_GetShapes.restype = c_int
         L.b2World_SetHitEventThreshold.argtypes = [WorldId, c_float]
+        # joints
+        L.b2Body_GetJointCount.argtypes = [BodyId]
+        L.b2Body_GetJointCount.restype = c_int
+        L.b2Body_GetJoints.argtypes = [BodyId, POINTER(JointId), c_int]
+        L.b2Body_GetJoints.restype = c_int
+        L.b2DestroyJoint.argtypes = [JointId]
+        L.b2Joint_IsValid.argtypes = [JointId]
+        L.b2Joint_IsValid.restype = c_bool
+        L.b2Joint_GetType.argtypes = [JointId]
+        L.b2Joint_GetType.restype = c_int
+        L.b2Joint_GetConstraintForce.argtypes = [JointId]
+        L.b2Joint_GetConstraintForce.restype = Vec2
+        L.b2Joint_GetConstraintTorque.argtypes = [JointId]
+        L.b2Joint_GetConstraintTorque.restype = c_float
+        L.b2Joint_GetBodyA.argtypes = [JointId]
+        L.b2Joint_GetBodyA.restype = BodyId
+        L.b2Joint_GetBodyB.argtypes = [JointId]
+        L.b2Joint_GetBodyB.restype = BodyId
+        L.b2Joint_GetLocalAnchorA.argtypes = [JointId]
+        L.b2Joint_GetLocalAnchorA.restype = Vec2
+        L.b2Joint_GetLocalAnchorB.argtypes = [JointId]
+        L.b2Joint_GetLocalAnchorB.restype = Vec2
+        L.b2Body_GetTransform.argtypes = [BodyId]
+        L.b2Body_GetTransform.restype = Transform
+        L.b2RevoluteJoint_SetMotorSpeed.argtypes = [JointId, c_float]
+        L.b2RevoluteJoint_GetMotorTorque.argtypes = [JointId]
+        L.b2RevoluteJoint_GetMotorTorque.restype = c_float
+        L.b2RevoluteJoint_GetAngle.argtypes = [JointId]
+        L.b2RevoluteJoint_GetAngle.restype = c_float
+        L.b2RevoluteJoint_EnableLimit.argtypes = [JointId, c_bool]
+        L.b2RevoluteJoint_SetLimits.argtypes = [JointId, c_float, c_float]
         L.b2mi_World_DumpContacts.argtypes = [WorldId, c_void_p, c_int]
         L.b2mi_World_DumpContacts.restype = c_int
         L.b2mi_World_DumpBodies.argtypes = [WorldId, c_void_p, c_int]
@@ -328,6 +364,9 @@ class B2Api:
                                        c_float, c_float, c_float, c_float, c_float, c_int,
                                        POINTER(WorldId), c_void_p, c_int]
             S.scene_step_n.argtypes = [WorldId, c_float, c_int, c_int]
+            S.scene_joint_tumbler.argtypes = [POINTER(WorldDef), c_int, POINTER(WorldId), c_void_p, c_int]
+            S.scene_joint_grid.argtypes = [POINTER(WorldDef), c_int, c_float, POINTER(WorldId), c_void_p, c_int]
+            S.scene_joint_zoo.argtypes = [POINTER(WorldDef), c_int, c_uint32, c_uint64, POINTER(WorldId), c_void_p, c_int]
 
     # ---- worlds ----
     def default_world_def(self, **overrides) -> WorldDef:
@@ -409,9 +448,45 @@ class B2Api:
                                kw.get("restitution", 0.0), kw.get("linearDamping", 0.0),
                                kw.get("angularDamping", 0.0), int(kw.get("ground", True)),
                                ctypes.byref(w), ids, len(ids))
+        elif name == "joint_tumbler":
+            grid = kw.get("grid", 45)
+            ids = self._bodies(1 + grid * grid)
+            n = S.scene_joint_tumbler(ctypes.byref(d), grid, ctypes.byref(w), ids, len(ids))
+        elif name == "joint_grid":
+            N = kw.get("N", 100)
+            ids = self._bodies(N * N)
+            n = S.scene_joint_grid(ctypes.byref(d), N, kw.get("gravity", 0.0), ctypes.byref(w), ids, len(ids))
+        elif name == "joint_zoo":
+            spokes = kw.get("spokes", 16)
+            ids = self._bodies(64 + spokes)
+            n = S.scene_joint_zoo(ctypes.byref(d), spokes, kw.get("types", 0x7F), kw.get("seed", 1),
+                                  ctypes.byref(w), ids, len(ids))
         else:
             raise ValueError(f"unknown scene {name}")
         return w, list(ids)[:n]
+
+    # ---- joints ----
+    def joints_of(self, bodies) -> list:
+        """The distinct joints attached to `bodies`, in ascending joint id."""
+        seen = {}
+        for b in bodies:
+            n = self.lib.b2Body_GetJointCount(b)
+            if n == 0:
+                continue
+            arr = (JointId * n)()
+            m = self.lib.b2Body_GetJoints(b, arr, n)
+            for i in range(m):
+                seen[arr[i].index1] = JointId(arr[i].index1, arr[i].world0, arr[i].revision)
+        return [seen[k] for k in sorted(seen)]
+
+    def joint_dump(self, joints) -> np.ndarray:
+        """Per joint: type, constraint force (x, y) and torque, as float32 bits comparable across
+        implementations (the force and torque are the accumulated impulses times 1/h)."""
+        out = np.zeros(len(joints), dtype=JOINT_DTYPE)
+        for i, j in enumerate(joints):
+            f = self.lib.b2Joint_GetConstraintForce(j)
+            out[i] = (j.index1, self.lib.b2Joint_GetType(j), (f.x, f.y), self.lib.b2Joint_GetConstraintTorque(j))
+        return out
 
     def body_shapes(self, body: BodyId) -> list:
         n = self.lib.b2Body_GetShapeCount(body)

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "box2d/math_functions.h"
+
 namespace b2mi
 {
 
@@ -172,6 +174,120 @@ static_assert( sizeof( DevPose ) == 32, "pose layout" );
 // Body state (the reference's 32-byte b2BodyState): two float4 per body
 //   s0 = (v.x, v.y, w, flags), s1 = (dp.x, dp.y, dq.c, dq.s)
 
+// ---- joints (src/joint.h: b2JointSim and the per-type structs) ------------------------------
+
+// DevJoint::flags
+enum : uint32_t
+{
+	kJointSpring = 1u << 0,
+	kJointLimit = 1u << 1,
+	kJointMotor = 1u << 2,
+};
+
+struct DevSoft
+{
+	float biasRate, massScale, impulseScale;
+};
+
+// The per-type parts keep the reference's fields (src/joint.h:13-178) except the body indices,
+// which a joint's solver slot carries, and the enable flags, which are bits of DevJoint::flags.
+struct DevDistanceJoint
+{
+	float length, hertz, dampingRatio, minLength, maxLength, maxMotorForce, motorSpeed;
+	float impulse, lowerImpulse, upperImpulse, motorImpulse;
+	b2Vec2 anchorA, anchorB, deltaCenter;
+	DevSoft distanceSoftness;
+	float axialMass;
+};
+
+struct DevMotorJoint
+{
+	b2Vec2 linearOffset;
+	float angularOffset;
+	b2Vec2 linearImpulse;
+	float angularImpulse;
+	float maxForce, maxTorque, correctionFactor;
+	b2Vec2 anchorA, anchorB, deltaCenter;
+	float deltaAngle;
+	b2Mat22 linearMass;
+	float angularMass;
+};
+
+struct DevMouseJoint
+{
+	b2Vec2 targetA;
+	float hertz, dampingRatio, maxForce;
+	b2Vec2 linearImpulse;
+	float angularImpulse;
+	DevSoft linearSoftness, angularSoftness;
+	b2Vec2 anchorB, deltaCenter;
+	b2Mat22 linearMass;
+};
+
+struct DevPrismaticJoint
+{
+	b2Vec2 localAxisA, impulse;
+	float springImpulse, motorImpulse, lowerImpulse, upperImpulse;
+	float hertz, dampingRatio, maxMotorForce, motorSpeed, referenceAngle, lowerTranslation, upperTranslation;
+	b2Vec2 anchorA, anchorB, axisA, deltaCenter;
+	float deltaAngle, axialMass;
+	DevSoft springSoftness;
+};
+
+struct DevRevoluteJoint
+{
+	b2Vec2 linearImpulse;
+	float springImpulse, motorImpulse, lowerImpulse, upperImpulse;
+	float hertz, dampingRatio, maxMotorTorque, motorSpeed, referenceAngle, lowerAngle, upperAngle;
+	b2Vec2 anchorA, anchorB, deltaCenter;
+	float deltaAngle, axialMass;
+	DevSoft springSoftness;
+};
+
+struct DevWeldJoint
+{
+	float referenceAngle, linearHertz, linearDampingRatio, angularHertz, angularDampingRatio;
+	DevSoft linearSoftness, angularSoftness;
+	b2Vec2 linearImpulse;
+	float angularImpulse;
+	b2Vec2 anchorA, anchorB, deltaCenter;
+	float deltaAngle, axialMass;
+};
+
+struct DevWheelJoint
+{
+	b2Vec2 localAxisA;
+	float perpImpulse, motorImpulse, springImpulse, lowerImpulse, upperImpulse;
+	float maxMotorTorque, motorSpeed, lowerTranslation, upperTranslation, hertz, dampingRatio;
+	b2Vec2 anchorA, anchorB, axisA, deltaCenter;
+	float perpMass, motorMass, axialMass;
+	DevSoft springSoftness;
+};
+
+// One joint as the solver keeps it, by joint id: parameters, accumulated impulses (warm start
+// across steps) and what prepare computes at the start of each step (anchors, masses,
+// softness). The solver reads and writes it in place; a joint belongs to one work item.
+struct DevJoint
+{
+	int type; // b2JointType
+	uint32_t flags;
+	int pad[2];
+	b2Vec2 localOriginAnchorA, localOriginAnchorB;
+	float invMassA, invMassB, invIA, invIB;
+	union
+	{
+		DevDistanceJoint distance;
+		DevMotorJoint motor;
+		DevMouseJoint mouse;
+		DevPrismaticJoint prismatic;
+		DevRevoluteJoint revolute;
+		DevWeldJoint weld;
+		DevWheelJoint wheel;
+		float words[36];
+	};
+};
+static_assert( sizeof( DevJoint ) == 192, "joint record layout" );
+
 struct DevWorld
 {
 	float* state;		 // [2 * 4 * awakeCapacity]
@@ -183,10 +299,12 @@ struct DevWorld
 	int* bodyShapes;	 // per awake sim: shape ids, head-first body list order
 	DevContact* contacts; // by contact id
 	DevContactOut* contactOut; // by contact id
+	DevJoint* joints;		   // by joint id
 	float gravityX, gravityY;
 	float dt, inv_dt, h, inv_h;
 	float contactSoft[3]; // biasRate, massScale, impulseScale
 	float staticSoft[3];
+	float jointSoft[3]; // src/world.c:739-743
 	float restitutionThreshold, pushout;
 	float maxLinearSpeed, maxAngularSpeed;
 	float warmStartScale;
@@ -218,7 +336,20 @@ struct WorkItem
 	// sweep order (cyclically, g itself last), 0 if none: 4 bits at bit 4 (g % 8) of
 	// nextGroup[g / 8]. A sweep loads a lane's slot of that group one phase ahead.
 	int nextGroup[2];
+	// The item's joints follow its contact slots in the slot lists, grouped the same way: slots
+	// jointGroupStart[g] .. jointGroupStart[g + 1] (relative to slotStart; jointGroupStart[0] =
+	// groupStart[kGroups]) hold the joint ids of group g (0 = overflow, serial in the reference's
+	// overflow order, in levels at ovfLevels[jointLevelBegin .. + jointLevelCount]).
+	int jointGroupStart[kGroups + 1];
+	int jointLevelBegin;
+	int jointLevelCount;
 };
+
+// every slot of a work item, contacts and joints
+__host__ __device__ inline int item_slots( const WorkItem& wi )
+{
+	return wi.jointGroupStart[kGroups];
+}
 
 // the next non-empty colour group after g (see WorkItem::nextGroup)
 __host__ __device__ inline int next_group( const WorkItem& wi, int g )
@@ -244,6 +375,11 @@ enum SplitPhase : int
 	kSplitRestitution,
 	kSplitStoreImpulses,
 	kSplitFinalize,
+	// the joints' stages (one launch per overflow level or colour, before that stage's contacts)
+	kSplitJointPrepare,
+	kSplitJointWarmStart,
+	kSplitJointSolve,
+	kSplitJointRelax,
 	kSplitPhaseCount
 };
 // default slot count from which an HBM-store work item runs over the grid (a cooperative launch,

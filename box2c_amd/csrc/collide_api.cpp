@@ -3,6 +3,7 @@
 // k_collide runs on the GPU (narrowphase.hpp). Returned manifolds carry the points, anchors,
 // separations and feature ids; impulses, normal velocity and the persisted flag are zero,
 // as the reference's functions leave them.
+#include "joints.hpp"
 #include "narrowphase.hpp"
 
 #include "box2d/collision.h"
@@ -148,6 +149,12 @@ B2_API b2Manifold b2CollideSegmentAndPolygon( const b2Segment* segmentA, b2Trans
 											  b2Transform xfB )
 {
 	return polygons( capsule_poly( segmentA->point1, segmentA->point2, 0.0f ), xfA, to_poly( polygonB ), xfB );
+}
+
+B2_API void b2mi_Atan2( const float* y, const float* x, float* out, int n )
+{
+	for ( int i = 0; i < n; ++i )
+		out[i] = b2mi::fdlibm_atan2f( y[i], x[i] );
 }
 
 } // extern "C"

@@ -21,7 +21,7 @@
 #include "box2d/math_functions.h"
 
 #include "narrowphase.hpp"
-#include "solver.hpp"
+#include "joints.hpp"
 
 namespace b2mi
 {
@@ -474,11 +474,12 @@ struct GridSync
 
 // All stages of one step for one work item, by the threads of X (one workgroup, or a cooperative
 // grid for a very large item), with X's barriers between dependent phases. `C` holds the item's
-// private state; `bl` its body list (sim indices) and `sc` the contact id of each of its slots.
+// private state; `bl` its body list (sim indices), `sc` the contact or joint id of each of its
+// slots and `sb` each slot's two local bodies (in HBM; the joints' are read from there).
 template <class S, class Sync>
 __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const WorkItem& wi, const int* __restrict__ bl,
-							const int* __restrict__ sc, const int* __restrict__ levels, EnlargeRecord* enl, int* enlCount,
-							int enlCapacity )
+							const int* __restrict__ sc, const int2* __restrict__ sb, const int* __restrict__ levels,
+							const int* __restrict__ jointLevels, EnlargeRecord* enl, int* enlCount, int enlCapacity )
 {
 	const int tid = X.tid(), nt = X.nt();
 	const float4* gstate = reinterpret_cast<const float4*>( W.state );
@@ -511,8 +512,15 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 	}
 	X.sync();
 
-	// prepare (all groups; the overflow group uses the same per-point formulas), with the contact
-	// of a thread's next slot loaded while it prepares the current one
+	// prepare the joints (src/solver.c:598-606: before the contacts; they write only their own
+	// records) and the contacts (all groups; the overflow group uses the same per-point formulas),
+	// with the contact of a thread's next slot loaded while it prepares the current one
+	const int jointBegin = wi.jointGroupStart[0], jointEnd = wi.jointGroupStart[kGroups];
+	for ( int k = jointBegin + tid; k < jointEnd; k += nt )
+	{
+		int2 lb = sb[k];
+		prepare_joint( W.joints[sc[k]], joint_body_sim( W, bl, lb.x ), joint_body_sim( W, bl, lb.y ), W );
+	}
 	{
 		PrepInput cur;
 		if ( tid < total )
@@ -551,6 +559,27 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 	const bool warm = ( W.flags & kWorldWarmStart ) != 0;
 	const Softness soft = world_softness( W );
 
+	// the overflow joints run level by level before the overflow contacts, the joints of a colour
+	// beside its contacts (a colour's joints and contacts share no body), src/solver.c:638-681
+	const int jointLevelCount = wi.jointGroupStart[1] > wi.jointGroupStart[0] ? wi.jointLevelCount : 0;
+	auto jointStage = [&]( int b, int e, int stage ) {
+		for ( int k = b + tid; k < e; k += nt )
+		{
+			int2 lb = sb[k];
+			if ( stage == 0 )
+				warm_start_joint( C, W.joints[sc[k]], lb.x, lb.y );
+			else
+				solve_joint( C, W.joints[sc[k]], lb.x, lb.y, W, stage == 1 );
+		}
+	};
+	auto overflowJoints = [&]( int stage ) {
+		for ( int L = 0; L < jointLevelCount; ++L )
+		{
+			jointStage( jointLevels[L], jointLevels[L + 1], stage );
+			X.sync();
+		}
+	};
+
 	// the velocity terms of this thread's first two bodies stay in registers over the substeps
 	VelocityTerms vt[2] = {};
 #pragma unroll
@@ -569,6 +598,7 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 		X.sync();
 
 		// warm start: overflow (always, src/solver.c:639), then colours if enabled
+		overflowJoints( 0 );
 		for ( int L = 0; L < levelCount; ++L )
 		{
 			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
@@ -582,27 +612,33 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 			for ( int g = 1; g < kGroups; ++g )
 			{
 				int b = wi.groupStart[g], e = wi.groupStart[g + 1];
-				if ( b == e )
+				int jb = wi.jointGroupStart[g], je = wi.jointGroupStart[g + 1];
+				if ( b == e && jb == je )
 					continue;
-				const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
-				for ( int k = b + tid; k < e; k += nt )
+				jointStage( jb, je, 0 );
+				if ( b < e )
 				{
-					if ( S::kPrefetch && k == preSlot )
+					const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
+					for ( int k = b + tid; k < e; k += nt )
 					{
-						int2 lb = quad_bodies( pre[0] );
-						lane_warm_start_q( C, k, lb.x, lb.y, pre );
+						if ( S::kPrefetch && k == preSlot )
+						{
+							int2 lb = quad_bodies( pre[0] );
+							lane_warm_start_q( C, k, lb.x, lb.y, pre );
+						}
+						else
+						{
+							int2 lb = C.bodies( k );
+							lane_warm_start( C, k, lb.x, lb.y );
+						}
 					}
-					else
-					{
-						int2 lb = C.bodies( k );
-						lane_warm_start( C, k, lb.x, lb.y );
-					}
+					ahead( nk );
 				}
-				ahead( nk );
 				X.sync();
 			}
 
 		// solve with bias
+		overflowJoints( 1 );
 		for ( int L = 0; L < levelCount; ++L )
 		{
 			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
@@ -615,23 +651,28 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 		for ( int g = 1; g < kGroups; ++g )
 		{
 			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
-			if ( b == e )
+			int jb = wi.jointGroupStart[g], je = wi.jointGroupStart[g + 1];
+			if ( b == e && jb == je )
 				continue;
-			const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
-			for ( int k = b + tid; k < e; k += nt )
+			jointStage( jb, je, 1 );
+			if ( b < e )
 			{
-				if ( S::kPrefetch && k == preSlot )
+				const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
+				for ( int k = b + tid; k < e; k += nt )
 				{
-					int2 lb = quad_bodies( pre[0] );
-					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, true, soft );
+					if ( S::kPrefetch && k == preSlot )
+					{
+						int2 lb = quad_bodies( pre[0] );
+						lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, true, soft );
+					}
+					else
+					{
+						int2 lb = C.bodies( k );
+						lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, soft );
+					}
 				}
-				else
-				{
-					int2 lb = C.bodies( k );
-					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, true, soft );
-				}
+				ahead( nk );
 			}
-			ahead( nk );
 			X.sync();
 		}
 
@@ -640,6 +681,7 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 		X.sync();
 
 		// relax
+		overflowJoints( 2 );
 		for ( int L = 0; L < levelCount; ++L )
 		{
 			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
@@ -652,23 +694,28 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 		for ( int g = 1; g < kGroups; ++g )
 		{
 			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
-			if ( b == e )
+			int jb = wi.jointGroupStart[g], je = wi.jointGroupStart[g + 1];
+			if ( b == e && jb == je )
 				continue;
-			const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
-			for ( int k = b + tid; k < e; k += nt )
+			jointStage( jb, je, 2 );
+			if ( b < e )
 			{
-				if ( S::kPrefetch && k == preSlot )
+				const int nk = S::kPrefetch ? next_colour_slot( wi, g, tid ) : -1;
+				for ( int k = b + tid; k < e; k += nt )
 				{
-					int2 lb = quad_bodies( pre[0] );
-					lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, false, soft );
+					if ( S::kPrefetch && k == preSlot )
+					{
+						int2 lb = quad_bodies( pre[0] );
+						lane_solve_q( C, k, lb.x, lb.y, pre, inv_h, minBiasVel, false, soft );
+					}
+					else
+					{
+						int2 lb = C.bodies( k );
+						lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, soft );
+					}
 				}
-				else
-				{
-					int2 lb = C.bodies( k );
-					lane_solve( C, k, lb.x, lb.y, inv_h, minBiasVel, false, soft );
-				}
+				ahead( nk );
 			}
-			ahead( nk );
 			X.sync();
 		}
 	}
@@ -802,7 +849,8 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 			C.rec[k * cfQuads] = make_float4( __builtin_bit_cast( float, b.x ), __builtin_bit_cast( float, b.y ), 0.0f, 0.0f );
 		}
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
-		solve_item( C, BlockSync{}, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
+		solve_item( C, BlockSync{}, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
+					ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else if constexpr ( Store == kStoreBodiesLds )
 	{
@@ -818,7 +866,8 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount + 1 );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
-		solve_item( C, BlockSync{}, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
+		solve_item( C, BlockSync{}, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
+					ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else
 	{
@@ -828,7 +877,8 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = slotBodies + wi.slotStart;
 		C.P = priv + 2 * (size_t)wi.bodyStart;
 		C.M = mass + wi.bodyStart;
-		solve_item( C, BlockSync{}, W, wi, bl, sc, ovfLevels + wi.ovfLevelBegin, enl, enlCount, enlCapacity );
+		solve_item( C, BlockSync{}, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
+					ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 	}
 }
 
@@ -865,8 +915,8 @@ __global__ void __launch_bounds__( kGridSolveBlock ) k_solve_grid( const DevWorl
 	X.nt_ = gridDim.x * blockDim.x;
 	X.blocks = gridDim.x;
 	X.target = 0;
-	solve_item( C, X, W, wi, bodyList + wi.bodyStart, slotContact + wi.slotStart, ovfLevels + wi.ovfLevelBegin, enl, enlCount,
-				enlCapacity );
+	solve_item( C, X, W, wi, bodyList + wi.bodyStart, slotContact + wi.slotStart, slotBodies + wi.slotStart,
+				ovfLevels + wi.ovfLevelBegin, ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 }
 
 __global__ void k_grid_reset( unsigned* sync )
@@ -918,6 +968,25 @@ __global__ void __launch_bounds__( 256 ) k_split_phase( const DevWorld* __restri
 		{
 			int2 lb = C.bodies( k );
 			prepare_slot( C, k, load_prep( W.contacts + sc[k] ), lb.x, lb.y, W );
+			break;
+		}
+		case kSplitJointPrepare:
+		{
+			int2 lb = C.bodies( k );
+			prepare_joint( W.joints[sc[k]], joint_body_sim( W, bl, lb.x ), joint_body_sim( W, bl, lb.y ), W );
+			break;
+		}
+		case kSplitJointWarmStart:
+		{
+			int2 lb = C.bodies( k );
+			warm_start_joint( C, W.joints[sc[k]], lb.x, lb.y );
+			break;
+		}
+		case kSplitJointSolve:
+		case kSplitJointRelax:
+		{
+			int2 lb = C.bodies( k );
+			solve_joint( C, W.joints[sc[k]], lb.x, lb.y, W, phase == kSplitJointSolve );
 			break;
 		}
 		case kSplitIntegrateVelocities:

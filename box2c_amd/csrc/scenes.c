@@ -21,6 +21,7 @@
 #include <string.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #if defined( __GNUC__ )
 	#define SCENE_API __attribute__( ( visibility( "default" ) ) )
@@ -702,4 +703,455 @@ SCENE_API void scene_step_n( b2WorldId worldId, float timeStep, int subStepCount
 {
 	for ( int i = 0; i < n; ++i )
 		b2World_Step( worldId, timeStep, subStepCount );
+}
+
+// ---- joint scenes ----
+
+// The reference's own tumbler benchmark (benchmark/tumbler.c:7-81): a dynamic box container of
+// density 50 on a revolute joint to a static ground body, its motor at 25 deg/s with a 1e8 N m
+// torque limit, filled with a gridCount x gridCount grid of 0.125 m boxes at 0.4 m pitch (the
+// reference's Release build uses 45).
+SCENE_API int scene_joint_tumbler( const b2WorldDef* def, int gridCount, b2WorldId* outWorld, b2BodyId* outBodies, int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	b2BodyDef groundDef = b2DefaultBodyDef();
+	b2BodyId groundId = b2CreateBody( worldId, &groundDef );
+	int n = 0;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Vec2 ){ 0.0f, 10.0f };
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.density = 50.0f;
+		b2Polygon polygon = b2MakeOffsetBox( 0.5f, 10.0f, ( b2Vec2 ){ 10.0f, 0.0f }, 0.0f );
+		b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+		polygon = b2MakeOffsetBox( 0.5f, 10.0f, ( b2Vec2 ){ -10.0f, 0.0f }, 0.0f );
+		b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+		polygon = b2MakeOffsetBox( 10.0f, 0.5f, ( b2Vec2 ){ 0.0f, 10.0f }, 0.0f );
+		b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+		polygon = b2MakeOffsetBox( 10.0f, 0.5f, ( b2Vec2 ){ 0.0f, -10.0f }, 0.0f );
+		b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+		record( outBodies, capacity, n++, bodyId );
+
+		b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+		jd.bodyIdA = groundId;
+		jd.bodyIdB = bodyId;
+		jd.localAnchorA = ( b2Vec2 ){ 0.0f, 10.0f };
+		jd.localAnchorB = ( b2Vec2 ){ 0.0f, 0.0f };
+		jd.referenceAngle = 0.0f;
+		jd.motorSpeed = ( b2_pi / 180.0f ) * 25.0f;
+		jd.maxMotorTorque = 1e8f;
+		jd.enableMotor = true;
+		b2CreateRevoluteJoint( worldId, &jd );
+	}
+	b2Polygon polygon = b2MakeBox( 0.125f, 0.125f );
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	float y = -0.2f * gridCount + 10.0f;
+	for ( int i = 0; i < gridCount; ++i )
+	{
+		float x = -0.2f * gridCount;
+		for ( int j = 0; j < gridCount; ++j )
+		{
+			bodyDef.position = ( b2Vec2 ){ x, y };
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+			record( outBodies, capacity, n++, bodyId );
+			x += 0.4f;
+		}
+		y += 0.4f;
+	}
+	return n;
+}
+
+// The reference's joint grid benchmark (benchmark/joint_grid.c:8-83): an N x N grid of 0.4 m
+// circles, each joined to its upper and left neighbours by revolute joints, the middle seven of
+// the top row static, circles not colliding with each other. `gravityY` replaces the benchmark's
+// zero gravity when nonzero. Records the dynamic bodies.
+SCENE_API int scene_joint_grid( const b2WorldDef* def, int N, float gravityY, b2WorldId* outWorld, b2BodyId* outBodies, int capacity )
+{
+	b2WorldDef wd = *def;
+	wd.gravity = ( b2Vec2 ){ 0.0f, gravityY };
+	b2WorldId worldId = scene_create_world( &wd );
+	*outWorld = worldId;
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+	shapeDef.filter.categoryBits = 2;
+	shapeDef.filter.maskBits = ~2u;
+	b2Circle circle = { { 0.0f, 0.0f }, 0.4f };
+	b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.enableSleep = false;
+	b2BodyId* bodies = malloc( (size_t)N * N * sizeof( b2BodyId ) );
+	int n = 0, index = 0;
+	for ( int k = 0; k < N; ++k )
+	{
+		for ( int i = 0; i < N; ++i )
+		{
+			float fk = (float)k, fi = (float)i;
+			if ( k >= N / 2 - 3 && k <= N / 2 + 3 && i == 0 )
+				bodyDef.type = b2_staticBody;
+			else
+				bodyDef.type = b2_dynamicBody;
+			bodyDef.position = ( b2Vec2 ){ fk, -fi };
+			b2BodyId body = b2CreateBody( worldId, &bodyDef );
+			b2CreateCircleShape( body, &shapeDef, &circle );
+			if ( i > 0 )
+			{
+				jd.bodyIdA = bodies[index - 1];
+				jd.bodyIdB = body;
+				jd.localAnchorA = ( b2Vec2 ){ 0.0f, -0.5f };
+				jd.localAnchorB = ( b2Vec2 ){ 0.0f, 0.5f };
+				b2CreateRevoluteJoint( worldId, &jd );
+			}
+			if ( k > 0 )
+			{
+				jd.bodyIdA = bodies[index - N];
+				jd.bodyIdB = body;
+				jd.localAnchorA = ( b2Vec2 ){ 0.5f, 0.0f };
+				jd.localAnchorB = ( b2Vec2 ){ -0.5f, 0.0f };
+				b2CreateRevoluteJoint( worldId, &jd );
+			}
+			bodies[index++] = body;
+			if ( bodyDef.type == b2_dynamicBody )
+				record( outBodies, capacity, n++, body );
+		}
+	}
+	free( bodies );
+	return n;
+}
+
+static b2BodyId zoo_box( b2WorldId worldId, b2BodyType type, float x, float y, float hx, float hy, float angle, b2BodyId* outBodies,
+						 int capacity, int* n )
+{
+	b2BodyDef bd = b2DefaultBodyDef();
+	bd.type = type;
+	bd.position = ( b2Vec2 ){ x, y };
+	bd.rotation = ( b2Rot ){ cosf( angle ), sinf( angle ) };
+	b2BodyId id = b2CreateBody( worldId, &bd );
+	b2ShapeDef sd = b2DefaultShapeDef();
+	b2Polygon box = b2MakeBox( hx, hy );
+	b2CreatePolygonShape( id, &sd, &box );
+	record( outBodies, capacity, ( *n )++, id );
+	return id;
+}
+
+// Every joint type the step solves, with springs, limits and motors, joints to static and
+// kinematic bodies, collideConnected on and off, and a hub body with `spokes` joints (beyond the
+// eleven graph colours, so the rest are overflow joints), all falling or swinging into contact
+// with boxes and the ground. `types` is a bitmask over b2JointType of the joint kinds to include.
+SCENE_API int scene_joint_zoo( const b2WorldDef* def, int spokes, uint32_t types, uint64_t seed, b2WorldId* outWorld,
+							   b2BodyId* outBodies, int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	Pcg32 rng = pcg32_seed( seed, 11 );
+	int n = 0;
+	b2BodyId ground;
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.position = ( b2Vec2 ){ 0.0f, -1.0f };
+		ground = b2CreateBody( worldId, &bd );
+		b2Polygon box = b2MakeBox( 60.0f, 1.0f );
+		b2ShapeDef sd = b2DefaultShapeDef();
+		b2CreatePolygonShape( ground, &sd, &box );
+	}
+	// a few boxes on the ground for the chains to hit
+	for ( int i = 0; i < 6; ++i )
+		zoo_box( worldId, b2_dynamicBody, -14.0f + 0.9f * i, 0.5f, 0.4f, 0.5f, 0.0f, outBodies, capacity, &n );
+
+	if ( types & ( 1u << b2_revoluteJoint ) )
+	{
+		// a pendulum chain from a static anchor: a limited link, a sprung link, a motorized link
+		b2BodyId prev = ground;
+		b2Vec2 anchor = { -12.0f, 9.0f };
+		for ( int i = 0; i < 8; ++i )
+		{
+			b2BodyId link = zoo_box( worldId, b2_dynamicBody, anchor.x + 0.5f + 1.0f * i, anchor.y, 0.5f, 0.125f,
+									 pcg32_uniform( &rng, -0.05f, 0.05f ), outBodies, capacity, &n );
+			b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+			jd.bodyIdA = prev;
+			jd.bodyIdB = link;
+			jd.localAnchorA = i == 0 ? ( b2Vec2 ){ anchor.x, anchor.y + 1.0f } : ( b2Vec2 ){ 0.5f, 0.0f };
+			jd.localAnchorB = ( b2Vec2 ){ -0.5f, 0.0f };
+			if ( i == 2 )
+			{
+				jd.enableLimit = true;
+				jd.lowerAngle = -0.25f * b2_pi;
+				jd.upperAngle = 0.125f * b2_pi;
+			}
+			if ( i == 4 )
+			{
+				jd.enableSpring = true;
+				jd.hertz = 2.0f;
+				jd.dampingRatio = 0.3f;
+			}
+			if ( i == 6 )
+			{
+				jd.enableMotor = true;
+				jd.motorSpeed = 1.5f;
+				jd.maxMotorTorque = 5.0f;
+			}
+			b2CreateRevoluteJoint( worldId, &jd );
+			prev = link;
+		}
+		// a wheel on a static axle with a motor and a limit, and two overlapping boxes that
+		// collide through their joint
+		b2BodyId wheel = zoo_box( worldId, b2_dynamicBody, 4.0f, 3.0f, 0.8f, 0.8f, 0.3f, outBodies, capacity, &n );
+		b2RevoluteJointDef wj = b2DefaultRevoluteJointDef();
+		wj.bodyIdA = ground;
+		wj.bodyIdB = wheel;
+		wj.localAnchorA = ( b2Vec2 ){ 4.0f, 4.0f };
+		wj.referenceAngle = 0.3f;
+		wj.enableMotor = true;
+		wj.motorSpeed = -2.0f;
+		wj.maxMotorTorque = 50.0f;
+		wj.enableLimit = true;
+		wj.lowerAngle = -1.0f;
+		wj.upperAngle = 1.0f;
+		b2CreateRevoluteJoint( worldId, &wj );
+		b2BodyId a = zoo_box( worldId, b2_dynamicBody, 7.0f, 1.0f, 0.5f, 0.5f, 0.0f, outBodies, capacity, &n );
+		b2BodyId b = zoo_box( worldId, b2_dynamicBody, 7.6f, 1.2f, 0.5f, 0.5f, 0.2f, outBodies, capacity, &n );
+		b2RevoluteJointDef cj = b2DefaultRevoluteJointDef();
+		cj.bodyIdA = a;
+		cj.bodyIdB = b;
+		cj.localAnchorA = ( b2Vec2 ){ 0.3f, 0.0f };
+		cj.localAnchorB = ( b2Vec2 ){ -0.3f, -0.2f };
+		cj.collideConnected = true;
+		b2CreateRevoluteJoint( worldId, &cj );
+		// a box hanging from a kinematic carriage moving sideways
+		b2BodyDef kd = b2DefaultBodyDef();
+		kd.type = b2_kinematicBody;
+		kd.position = ( b2Vec2 ){ 12.0f, 8.0f };
+		kd.linearVelocity = ( b2Vec2 ){ -0.5f, 0.0f };
+		kd.angularVelocity = 0.2f;
+		b2BodyId carriage = b2CreateBody( worldId, &kd );
+		b2ShapeDef sd = b2DefaultShapeDef();
+		b2Polygon cbox = b2MakeBox( 1.0f, 0.2f );
+		b2CreatePolygonShape( carriage, &sd, &cbox );
+		b2BodyId hanging = zoo_box( worldId, b2_dynamicBody, 12.0f, 6.5f, 0.3f, 0.6f, 0.1f, outBodies, capacity, &n );
+		b2RevoluteJointDef kj = b2DefaultRevoluteJointDef();
+		kj.bodyIdA = carriage;
+		kj.bodyIdB = hanging;
+		kj.localAnchorA = ( b2Vec2 ){ 0.0f, -0.3f };
+		kj.localAnchorB = ( b2Vec2 ){ 0.0f, 1.2f };
+		b2CreateRevoluteJoint( worldId, &kj );
+	}
+
+	if ( types & ( 1u << b2_weldJoint ) )
+	{
+		// a welded beam (rigid welds and one soft weld) falling onto the ground boxes
+		b2BodyId prev = b2_nullBodyId;
+		for ( int i = 0; i < 6; ++i )
+		{
+			b2BodyId seg = zoo_box( worldId, b2_dynamicBody, -15.0f + 0.8f * i, 4.0f + 0.1f * i, 0.4f, 0.15f, 0.05f, outBodies,
+									capacity, &n );
+			if ( i > 0 )
+			{
+				b2WeldJointDef wd = b2DefaultWeldJointDef();
+				wd.bodyIdA = prev;
+				wd.bodyIdB = seg;
+				wd.localAnchorA = ( b2Vec2 ){ 0.4f, 0.0f };
+				wd.localAnchorB = ( b2Vec2 ){ -0.4f, 0.0f };
+				wd.referenceAngle = 0.0f;
+				if ( i == 3 )
+				{
+					wd.linearHertz = 5.0f;
+					wd.linearDampingRatio = 0.7f;
+					wd.angularHertz = 3.0f;
+					wd.angularDampingRatio = 0.5f;
+				}
+				b2CreateWeldJoint( worldId, &wd );
+			}
+			prev = seg;
+		}
+		// a box welded to the ground at an angle
+		b2BodyId tilted = zoo_box( worldId, b2_dynamicBody, 16.0f, 2.0f, 0.3f, 1.0f, 0.4f, outBodies, capacity, &n );
+		b2WeldJointDef gd = b2DefaultWeldJointDef();
+		gd.bodyIdA = ground;
+		gd.bodyIdB = tilted;
+		gd.localAnchorA = ( b2Vec2 ){ 16.0f, 2.0f };
+		gd.localAnchorB = ( b2Vec2 ){ 0.0f, -1.0f };
+		gd.referenceAngle = 0.4f;
+		b2CreateWeldJoint( worldId, &gd );
+	}
+
+	if ( types & ( 1u << b2_distanceJoint ) )
+	{
+		// a rope of boxes from a static anchor: rigid links, a sprung and limited link, a motorized
+		// sprung link
+		b2BodyId prev = ground;
+		for ( int i = 0; i < 5; ++i )
+		{
+			b2BodyId link = zoo_box( worldId, b2_dynamicBody, -20.0f + 1.1f * ( i + 1 ), 12.0f, 0.2f, 0.2f, 0.0f, outBodies, capacity, &n );
+			b2DistanceJointDef dd = b2DefaultDistanceJointDef();
+			dd.bodyIdA = prev;
+			dd.bodyIdB = link;
+			dd.localAnchorA = i == 0 ? ( b2Vec2 ){ -20.0f, 13.0f } : ( b2Vec2 ){ 0.2f, 0.0f };
+			dd.localAnchorB = ( b2Vec2 ){ -0.2f, 0.0f };
+			dd.length = 0.7f;
+			if ( i == 2 )
+			{
+				dd.enableSpring = true;
+				dd.hertz = 3.0f;
+				dd.dampingRatio = 0.2f;
+				dd.enableLimit = true;
+				dd.minLength = 0.5f;
+				dd.maxLength = 1.0f;
+			}
+			if ( i == 3 )
+			{
+				dd.enableSpring = true;
+				dd.hertz = 0.0f;
+				dd.enableMotor = true;
+				dd.motorSpeed = 0.5f;
+				dd.maxMotorForce = 20.0f;
+			}
+			b2CreateDistanceJoint( worldId, &dd );
+			prev = link;
+		}
+	}
+
+	if ( types & ( 1u << b2_prismaticJoint ) )
+	{
+		// a slider on a static rail with limits and a motor, and a sprung slider between two boxes
+		b2BodyId slider = zoo_box( worldId, b2_dynamicBody, -6.0f, 2.0f, 0.5f, 0.3f, 0.0f, outBodies, capacity, &n );
+		b2PrismaticJointDef pd = b2DefaultPrismaticJointDef();
+		pd.bodyIdA = ground;
+		pd.bodyIdB = slider;
+		pd.localAnchorA = ( b2Vec2 ){ -6.0f, 3.0f };
+		pd.localAxisA = ( b2Vec2 ){ 1.0f, 0.2f };
+		pd.enableLimit = true;
+		pd.lowerTranslation = -2.0f;
+		pd.upperTranslation = 1.5f;
+		pd.enableMotor = true;
+		pd.motorSpeed = 1.0f;
+		pd.maxMotorForce = 50.0f;
+		b2CreatePrismaticJoint( worldId, &pd );
+		b2BodyId base = zoo_box( worldId, b2_dynamicBody, -3.0f, 1.0f, 0.6f, 0.3f, 0.0f, outBodies, capacity, &n );
+		b2BodyId top = zoo_box( worldId, b2_dynamicBody, -3.0f, 2.5f, 0.3f, 0.3f, 0.1f, outBodies, capacity, &n );
+		b2PrismaticJointDef sd = b2DefaultPrismaticJointDef();
+		sd.bodyIdA = base;
+		sd.bodyIdB = top;
+		sd.localAnchorA = ( b2Vec2 ){ 0.0f, 1.2f };
+		sd.localAxisA = ( b2Vec2 ){ 0.0f, 1.0f };
+		sd.referenceAngle = 0.1f;
+		sd.enableSpring = true;
+		sd.hertz = 2.0f;
+		sd.dampingRatio = 0.5f;
+		b2CreatePrismaticJoint( worldId, &sd );
+	}
+
+	if ( types & ( 1u << b2_wheelJoint ) )
+	{
+		// a car: a chassis on two wheels with suspension springs, a driven rear wheel and a limited
+		// front wheel
+		b2BodyId chassis = zoo_box( worldId, b2_dynamicBody, 22.0f, 2.0f, 1.2f, 0.25f, 0.0f, outBodies, capacity, &n );
+		for ( int k = 0; k < 2; ++k )
+		{
+			b2BodyDef bd = b2DefaultBodyDef();
+			bd.type = b2_dynamicBody;
+			bd.position = ( b2Vec2 ){ 22.0f + ( k == 0 ? -0.9f : 0.9f ), 1.2f };
+			b2BodyId wheel = b2CreateBody( worldId, &bd );
+			b2ShapeDef sd = b2DefaultShapeDef();
+			b2Circle c = { { 0.0f, 0.0f }, 0.4f };
+			b2CreateCircleShape( wheel, &sd, &c );
+			record( outBodies, capacity, n++, wheel );
+			b2WheelJointDef wd = b2DefaultWheelJointDef();
+			wd.bodyIdA = chassis;
+			wd.bodyIdB = wheel;
+			wd.localAnchorA = ( b2Vec2 ){ k == 0 ? -0.9f : 0.9f, -0.8f };
+			wd.localAxisA = ( b2Vec2 ){ 0.0f, 1.0f };
+			wd.hertz = 4.0f;
+			wd.dampingRatio = 0.7f;
+			if ( k == 0 )
+			{
+				wd.enableMotor = true;
+				wd.motorSpeed = -5.0f;
+				wd.maxMotorTorque = 20.0f;
+			}
+			else
+			{
+				wd.enableLimit = true;
+				wd.lowerTranslation = -0.25f;
+				wd.upperTranslation = 0.25f;
+			}
+			b2CreateWheelJoint( worldId, &wd );
+		}
+	}
+
+	if ( types & ( 1u << b2_motorJoint ) )
+	{
+		// a box driven toward an offset and angle from the ground body
+		b2BodyId driven = zoo_box( worldId, b2_dynamicBody, 26.0f, 1.0f, 0.4f, 0.4f, 0.0f, outBodies, capacity, &n );
+		b2MotorJointDef md = b2DefaultMotorJointDef();
+		md.bodyIdA = ground;
+		md.bodyIdB = driven;
+		md.linearOffset = ( b2Vec2 ){ 27.0f, 4.0f };
+		md.angularOffset = 0.5f;
+		md.maxForce = 500.0f;
+		md.maxTorque = 100.0f;
+		md.correctionFactor = 0.3f;
+		b2CreateMotorJoint( worldId, &md );
+	}
+
+	if ( types & ( 1u << b2_mouseJoint ) )
+	{
+		// a box dragged toward a target above it
+		b2BodyId dragged = zoo_box( worldId, b2_dynamicBody, 30.0f, 0.5f, 0.5f, 0.5f, 0.2f, outBodies, capacity, &n );
+		b2MouseJointDef md = b2DefaultMouseJointDef();
+		md.bodyIdA = ground;
+		md.bodyIdB = dragged;
+		md.target = ( b2Vec2 ){ 31.0f, 5.0f };
+		md.hertz = 5.0f;
+		md.dampingRatio = 0.7f;
+		md.maxForce = 200.0f;
+		b2CreateMouseJoint( worldId, &md );
+	}
+
+	if ( spokes > 0 )
+	{
+		// a hub joined to `spokes` bars around it: joints past the eleventh colour overflow
+		b2BodyId hub = zoo_box( worldId, b2_dynamicBody, 0.0f, 10.0f, 0.6f, 0.6f, 0.0f, outBodies, capacity, &n );
+		for ( int i = 0; i < spokes; ++i )
+		{
+			float a = 2.0f * b2_pi * i / spokes;
+			b2Vec2 dir = { cosf( a ), sinf( a ) };
+			b2BodyId bar = zoo_box( worldId, b2_dynamicBody, 1.6f * dir.x, 10.0f + 1.6f * dir.y, 0.5f, 0.08f, a, outBodies, capacity,
+									&n );
+			b2Vec2 pivot = { 0.9f * dir.x, 10.0f + 0.9f * dir.y };
+			if ( ( types & ( 1u << b2_weldJoint ) ) && ( i % 3 ) == 2 )
+			{
+				b2WeldJointDef wd = b2DefaultWeldJointDef();
+				wd.bodyIdA = hub;
+				wd.bodyIdB = bar;
+				wd.localAnchorA = ( b2Vec2 ){ pivot.x, pivot.y - 10.0f };
+				wd.localAnchorB = ( b2Vec2 ){ -0.7f, 0.0f };
+				wd.referenceAngle = a;
+				wd.angularHertz = i % 2 ? 4.0f : 0.0f;
+				wd.angularDampingRatio = 0.5f;
+				b2CreateWeldJoint( worldId, &wd );
+			}
+			else
+			{
+				b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+				jd.bodyIdA = hub;
+				jd.bodyIdB = bar;
+				jd.localAnchorA = ( b2Vec2 ){ pivot.x, pivot.y - 10.0f };
+				jd.localAnchorB = ( b2Vec2 ){ -0.7f, 0.0f };
+				jd.referenceAngle = a;
+				jd.enableLimit = ( i % 2 ) == 0;
+				jd.lowerAngle = -0.3f;
+				jd.upperAngle = 0.3f;
+				jd.enableSpring = ( i % 4 ) == 1;
+				jd.hertz = 1.5f;
+				jd.dampingRatio = 0.2f;
+				b2CreateRevoluteJoint( worldId, &jd );
+			}
+		}
+	}
+	return n;
 }

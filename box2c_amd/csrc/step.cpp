@@ -398,6 +398,9 @@ static void removeContactFromGraph( World* w, int bodyIdA, int bodyIdB, int colo
 }
 
 // src/contact.c:335-414
+bool jointsAllowCollision( const World* w, const Body& bodyA, const Body& bodyB );
+void uploadJoints( World* w );
+
 void destroyContactImpl( World* w, Contact* c )
 {
 	w->pairSet.erase( pairKey( c->shapeIdA, c->shapeIdB ) );
@@ -489,7 +492,12 @@ static void findPairs( const World* w, int begin, int end, std::vector<MovePair>
 					return true;
 				if ( !shouldShapesCollide( sa.filter, sb.filter ) )
 					return true;
-				if ( w->bodies[sa.bodyId].type != b2_dynamicBody && w->bodies[sb.bodyId].type != b2_dynamicBody )
+				const Body& ba = w->bodies[sa.bodyId];
+				const Body& bb = w->bodies[sb.bodyId];
+				if ( ba.type != b2_dynamicBody && bb.type != b2_dynamicBody )
+					return true;
+				// src/body.c:1731-1757: a joint between the bodies may turn their collision off
+				if ( ( ba.jointCount > 0 || bb.jointCount > 0 ) && !jointsAllowCollision( w, ba, bb ) )
 					return true;
 				pairs.push_back( MovePair{ shapeIdA, shapeIdB, *head } );
 				*head = (int)pairs.size() - 1;
@@ -720,6 +728,12 @@ static void updatePairsOnDevice( World* w )
 	for ( const PairRecord& p : D.found )
 	{
 		B2MI_CHECK( 0 <= p.shapeIdA && p.shapeIdA < ns && 0 <= p.shapeIdB && p.shapeIdB < ns );
+		// the joint part of the pair filter (src/body.c:1731-1757), which the device query leaves
+		// to the host
+		const Body& ba = w->bodies[w->shapes[p.shapeIdA].bodyId];
+		const Body& bb = w->bodies[w->shapes[p.shapeIdB].bodyId];
+		if ( ( ba.jointCount > 0 || bb.jointCount > 0 ) && !jointsAllowCollision( w, ba, bb ) )
+			continue;
 		createContact( w, &w->shapes[p.shapeIdA], &w->shapes[p.shapeIdB] );
 	}
 	for ( int key : w->moveArray )
@@ -818,6 +832,11 @@ struct ComponentSet
 	std::vector<Src> src;
 	int colorStart[kColorCount + 1] = {};
 	std::vector<int> compStart, bySlot;
+	// the awake joints, laid out as the contacts (colour-list order, then by component); a joint
+	// belongs to the component of its first dynamic body
+	std::vector<Src> jsrc;
+	int jColorStart[kColorCount + 1] = {};
+	std::vector<int> jCompStart, jBySlot;
 	int compSize( int k ) const
 	{
 		return compBodyStart[k + 1] - compBodyStart[k];
@@ -835,15 +854,16 @@ static void makeComponents( ComponentSet& cs, const std::vector<int>& sims, cons
 	std::vector<int> parent( m );
 	for ( int i = 0; i < m; ++i )
 		parent[i] = i;
-	for ( const Src& e : cs.src )
-		if ( e.a >= 0 && e.b >= 0 )
-		{
-			int pa = posOf[e.a], pb = posOf[e.b];
-			B2MI_CHECK( pa >= 0 && pb >= 0 );
-			int ra = findRoot( parent, pa ), rb = findRoot( parent, pb );
-			if ( ra != rb )
-				parent[std::max( ra, rb )] = std::min( ra, rb );
-		}
+	for ( const std::vector<Src>* list : { &cs.src, &cs.jsrc } )
+		for ( const Src& e : *list )
+			if ( e.a >= 0 && e.b >= 0 )
+			{
+				int pa = posOf[e.a], pb = posOf[e.b];
+				B2MI_CHECK( pa >= 0 && pb >= 0 );
+				int ra = findRoot( parent, pa ), rb = findRoot( parent, pb );
+				if ( ra != rb )
+					parent[std::max( ra, rb )] = std::min( ra, rb );
+			}
 	std::vector<int> compOf( m );
 	int nc = 0;
 	for ( int i = 0; i < m; ++i )
@@ -898,6 +918,30 @@ static void makeComponents( ComponentSet& cs, const std::vector<int>& sims, cons
 		for ( int k = 0; k < nc; ++k )
 			cs.compSlots[k] += st[k + 1] - st[k];
 	}
+	// the joints, sorted the same way (few; serial)
+	const int nj = (int)cs.jsrc.size();
+	cs.jCompStart.assign( (size_t)kColorCount * ( nc + 1 ), 0 );
+	cs.jBySlot.resize( nj );
+	std::vector<int> compOfJoint( nj );
+	for ( int q = 0; q < nj; ++q )
+	{
+		int a = cs.jsrc[q].a >= 0 ? cs.jsrc[q].a : cs.jsrc[q].b;
+		B2MI_CHECK( a >= 0 && posOf[a] >= 0 );
+		compOfJoint[q] = compOf[posOf[a]];
+	}
+	for ( int c = 0; c < kColorCount; ++c )
+	{
+		int* st = cs.jCompStart.data() + (size_t)c * ( nc + 1 );
+		for ( int q = cs.jColorStart[c]; q < cs.jColorStart[c + 1]; ++q )
+			st[compOfJoint[q] + 1] += 1;
+		for ( int k = 0; k < nc; ++k )
+			st[k + 1] += st[k];
+		std::vector<int> fill( st, st + nc );
+		for ( int q = cs.jColorStart[c]; q < cs.jColorStart[c + 1]; ++q )
+			cs.jBySlot[cs.jColorStart[c] + fill[compOfJoint[q]]++] = q;
+		for ( int k = 0; k < nc; ++k )
+			cs.compSlots[k] += st[k + 1] - st[k];
+	}
 }
 
 // One work item built into lists of its own (body, slot and level indices relative to the item).
@@ -907,6 +951,7 @@ struct BuiltItem
 	std::vector<int> bodies, slots, levels;
 	std::vector<int2> slotBodies;
 	int maxGroup = 0;
+	int jointLevelAt = 0; // where the joints' overflow levels start in `levels`
 };
 
 // Packs the components of cs in order into items of up to `target` constraint slots (and 4 x
@@ -970,6 +1015,7 @@ static std::vector<BuiltItem> packItems( const World* w, const ComponentSet& cs,
 			return li;
 		};
 		int itemMaxGroup = 0;
+		int groupSize[kGroups] = {};
 		// group 0 = overflow (colour 11) then colours 0..10
 		for ( int gidx = 0; gidx < kGroups; ++gidx )
 		{
@@ -985,78 +1031,106 @@ static std::vector<BuiltItem> packItems( const World* w, const ComponentSet& cs,
 					out.slots.push_back( e.id );
 					out.slotBodies.push_back( make_int2( la, lb ) );
 				}
-			int size = (int)out.slots.size() - wi.groupStart[gidx];
-			itemMaxGroup = std::max( itemMaxGroup, size );
+			groupSize[gidx] = (int)out.slots.size() - wi.groupStart[gidx];
 		}
+		wi.groupStart[kGroups] = (int)out.slots.size();
+		// Overflow levels (src/contact_solver.c:212-345 and src/joint.c:1106-1163 run the overflow
+		// group serially): a slot's level is one past the last level that changed one of its bodies,
+		// and the slots are ordered by level, keeping their order within a level. Static bodies are
+		// never changed; nor are kinematic ones, whose zero inverse mass and inertia leave their
+		// velocity bits as they are, unless a component is -0 (then they count as changed).
+		// The joints' levels come after the contacts' (all overflow joints run first).
+		auto changed = [&]( int li ) {
+			if ( li < 0 )
+				return false;
+			if ( li < wi.ownedCount )
+				return true;
+			const float* st = &w->awakeState[8 * (size_t)out.bodies[li]];
+			for ( int c = 0; c < 3; ++c )
+				if ( st[c] == 0.0f && std::signbit( st[c] ) )
+					return true;
+			return false;
+		};
+		// orders slots [first, end) by level and appends the level boundaries; returns the count
+		auto makeLevels = [&]( int first, int end ) -> int {
+			if ( end <= first )
+				return 0;
+			const int nb = (int)out.bodies.size();
+			std::vector<int> last( nb, -1 ), level( end - first );
+			int maxLevel = 0;
+			for ( int k = first; k < end; ++k )
+			{
+				int2 lb = out.slotBodies[k];
+				bool ca = changed( lb.x ), cb = changed( lb.y );
+				int lv = 0;
+				if ( ca )
+					lv = std::max( lv, last[lb.x] + 1 );
+				if ( cb )
+					lv = std::max( lv, last[lb.y] + 1 );
+				if ( ca )
+					last[lb.x] = lv;
+				if ( cb )
+					last[lb.y] = lv;
+				level[k - first] = lv;
+				maxLevel = std::max( maxLevel, lv );
+			}
+			std::vector<int> order( end - first );
+			for ( int k = 0; k < end - first; ++k )
+				order[k] = k;
+			std::stable_sort( order.begin(), order.end(), [&]( int a, int b ) { return level[a] < level[b]; } );
+			std::vector<int> ids( end - first );
+			std::vector<int2> bodies( end - first );
+			for ( int k = 0; k < end - first; ++k )
+			{
+				ids[k] = out.slots[first + order[k]];
+				bodies[k] = out.slotBodies[first + order[k]];
+			}
+			std::copy( ids.begin(), ids.end(), out.slots.begin() + first );
+			std::copy( bodies.begin(), bodies.end(), out.slotBodies.begin() + first );
+			const size_t before = out.levels.size();
+			for ( int k = 0; k < end - first; ++k )
+				if ( k == 0 || level[order[k]] != level[order[k - 1]] )
+					out.levels.push_back( first + k );
+			out.levels.push_back( end );
+			B2MI_CHECK( (int)( out.levels.size() - before ) == maxLevel + 2 );
+			return maxLevel + 1;
+		};
+		wi.ovfLevelCount = makeLevels( wi.groupStart[0], wi.groupStart[1] );
+		// the joints, grouped as the contacts after them: a static body as -1 - its static sim index
+		for ( int gidx = 0; gidx < kGroups; ++gidx )
+		{
+			int color = gidx == 0 ? kOverflow : gidx - 1;
+			wi.jointGroupStart[gidx] = (int)out.slots.size();
+			const int* st = cs.jCompStart.data() + (size_t)color * ( cs.nc + 1 );
+			for ( int k = ci; k < cj; ++k )
+				for ( int p = st[k]; p < st[k + 1]; ++p )
+				{
+					const Src& e = cs.jsrc[cs.jBySlot[cs.jColorStart[color] + p]];
+					const Joint& jt = w->joints[e.id];
+					int lb[2];
+					for ( int side = 0; side < 2; ++side )
+					{
+						const Body& body = w->bodies[jt.edges[side].bodyId];
+						lb[side] = body.setIndex == kStaticSet ? -1 - body.localIndex : localIndex( side == 0 ? e.a : e.b );
+					}
+					out.slots.push_back( e.id );
+					out.slotBodies.push_back( make_int2( lb[0], lb[1] ) );
+				}
+			groupSize[gidx] += (int)out.slots.size() - wi.jointGroupStart[gidx];
+		}
+		wi.jointGroupStart[kGroups] = (int)out.slots.size();
+		out.jointLevelAt = (int)out.levels.size();
+		wi.jointLevelCount = makeLevels( wi.jointGroupStart[0], wi.jointGroupStart[1] );
 		for ( int k = ci; k < cj; ++k )
 			for ( int q = cs.compBodyStart[k]; q < cs.compBodyStart[k + 1]; ++q )
 				posOfSim[cs.compBodies[q]] = -1;
-		// Overflow levels (group 0, src/contact_solver.c:212-345 runs it serially): a slot's level
-		// is one past the last level that changed one of its bodies, and the slots are ordered by
-		// level, keeping their order within a level. Static bodies are never changed; nor are
-		// kinematic ones, whose zero inverse mass and inertia leave their velocity bits as they
-		// are, unless a component is -0 (then they count as changed).
-		wi.ovfLevelCount = 0;
-		{
-			const int first = wi.groupStart[0], end = wi.groupStart[1];
-			if ( end > first )
-			{
-				const int nb = (int)out.bodies.size();
-				auto changed = [&]( int li ) {
-					if ( li < 0 )
-						return false;
-					if ( li < wi.ownedCount )
-						return true;
-					const float* st = &w->awakeState[8 * (size_t)out.bodies[li]];
-					for ( int c = 0; c < 3; ++c )
-						if ( st[c] == 0.0f && std::signbit( st[c] ) )
-							return true;
-					return false;
-				};
-				std::vector<int> last( nb, -1 ), level( end - first );
-				int maxLevel = 0;
-				for ( int k = first; k < end; ++k )
-				{
-					int2 lb = out.slotBodies[k];
-					bool ca = changed( lb.x ), cb = changed( lb.y );
-					int lv = 0;
-					if ( ca )
-						lv = std::max( lv, last[lb.x] + 1 );
-					if ( cb )
-						lv = std::max( lv, last[lb.y] + 1 );
-					if ( ca )
-						last[lb.x] = lv;
-					if ( cb )
-						last[lb.y] = lv;
-					level[k - first] = lv;
-					maxLevel = std::max( maxLevel, lv );
-				}
-				std::vector<int> order( end - first );
-				for ( int k = 0; k < end - first; ++k )
-					order[k] = k;
-				std::stable_sort( order.begin(), order.end(), [&]( int a, int b ) { return level[a] < level[b]; } );
-				std::vector<int> contacts( end - first );
-				std::vector<int2> bodies( end - first );
-				for ( int k = 0; k < end - first; ++k )
-				{
-					contacts[k] = out.slots[first + order[k]];
-					bodies[k] = out.slotBodies[first + order[k]];
-				}
-				std::copy( contacts.begin(), contacts.end(), out.slots.begin() + first );
-				std::copy( bodies.begin(), bodies.end(), out.slotBodies.begin() + first );
-				for ( int k = 0; k < end - first; ++k )
-					if ( k == 0 || level[order[k]] != level[order[k - 1]] )
-						out.levels.push_back( wi.groupStart[0] + k );
-				out.levels.push_back( wi.groupStart[1] );
-				wi.ovfLevelCount = maxLevel + 1;
-				B2MI_CHECK( (int)out.levels.size() == wi.ovfLevelCount + 1 );
-			}
-		}
-		// per-item workgroup: one thread per slot of its largest colour group
+		for ( int g = 0; g < kGroups; ++g )
+			itemMaxGroup = std::max( itemMaxGroup, groupSize[g] );
+		// per-item workgroup: one thread per slot of its largest group (a colour's contacts and
+		// joints run in the same phase)
 		wi.block = 64;
 		while ( wi.block < itemMaxGroup && wi.block < kSolveMaxBlock )
 			wi.block *= 2;
-		wi.groupStart[kGroups] = (int)out.slots.size();
 		wi.bodyCount = (int)out.bodies.size();
 		wi.nextGroup[0] = wi.nextGroup[1] = 0;
 		for ( int g = 1; g < kGroups; ++g )
@@ -1109,6 +1183,7 @@ static void appendItems( World* w, std::vector<BuiltItem>& built, bool parallel 
 		b.wi.bodyStart = nb;
 		b.wi.slotStart = ns;
 		b.wi.ovfLevelBegin = nl;
+		b.wi.jointLevelBegin = nl + b.jointLevelAt;
 		nb += (int)b.bodies.size();
 		ns += (int)b.slots.size();
 		nl += (int)b.levels.size();
@@ -1139,7 +1214,8 @@ static void appendItems( World* w, std::vector<BuiltItem>& built, bool parallel 
 		{
 			int maxGroup = L.items[i].ownedCount;
 			for ( int g = 0; g < kGroups; ++g )
-				maxGroup = std::max( maxGroup, L.items[i].groupStart[g + 1] - L.items[i].groupStart[g] );
+				maxGroup = std::max( maxGroup, L.items[i].groupStart[g + 1] - L.items[i].groupStart[g] + L.items[i].jointGroupStart[g + 1] -
+													L.items[i].jointGroupStart[g] );
 			L.maxGroupSize = std::max( L.maxGroupSize, maxGroup );
 			L.maxItemLds = std::max( L.maxItemLds, L.items[i].ldsBytes );
 		}
@@ -1151,6 +1227,7 @@ static void checkItems( const World* w, int first, int end, bool parallel )
 {
 	const SolverLists& L = w->solver;
 	const int na = (int)w->awakeSimBody.size();
+	const int nst = (int)w->staticSimBody.size();
 	forRanges( parallel, end - first, [&]( int lo, int hi ) {
 		for ( int it = first + lo; it < first + hi; ++it )
 		{
@@ -1159,7 +1236,17 @@ static void checkItems( const World* w, int first, int end, bool parallel )
 			B2MI_CHECK( 0 <= wi.ownedCount && wi.ownedCount <= wi.bodyCount );
 			for ( int i = 0; i < wi.bodyCount; ++i )
 				B2MI_CHECK( 0 <= L.bodyList[wi.bodyStart + i] && L.bodyList[wi.bodyStart + i] < na );
-			B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + wi.groupStart[kGroups] <= (int)L.slotContact.size() );
+			B2MI_CHECK( wi.slotStart >= 0 && wi.slotStart + item_slots( wi ) <= (int)L.slotContact.size() );
+			B2MI_CHECK( wi.jointGroupStart[0] == wi.groupStart[kGroups] );
+			for ( int k = 0; k < kGroups; ++k )
+				B2MI_CHECK( wi.jointGroupStart[k] <= wi.jointGroupStart[k + 1] );
+			for ( int k = wi.jointGroupStart[0]; k < item_slots( wi ); ++k )
+			{
+				// a joint's bodies: item-local, or -1 - a static sim index
+				int2 lb = L.slotBodies[wi.slotStart + k];
+				B2MI_CHECK( lb.x < wi.bodyCount && lb.y < wi.bodyCount && ( lb.x >= 0 || -1 - lb.x < nst ) && ( lb.y >= 0 || -1 - lb.y < nst ) );
+				B2MI_CHECK( 0 <= L.slotContact[wi.slotStart + k] && L.slotContact[wi.slotStart + k] < w->jointIdPool.nextIndex );
+			}
 			B2MI_CHECK( wi.ldsBytes <= kMaxItemLdsBytes );
 			B2MI_CHECK( ( wi.store == kStoreHbm && wi.ldsBytes == 0 ) ||
 						( wi.store == kStoreBodiesLds && wi.ldsBytes == itemBodyLdsBytes( wi.bodyCount, wi.groupStart[kGroups] ) ) ||
@@ -1226,6 +1313,18 @@ static void buildWorldItems( World* w, int target, int budget, int fullBudget, b
 		// each list position names exactly one contact, so all of them were written
 		B2MI_CHECK( placed.load() == cs.colorStart[kColorCount] );
 	}
+	// the awake joints in colour-list order
+	cs.jColorStart[0] = 0;
+	for ( int c = 0; c < kColorCount; ++c )
+		cs.jColorStart[c + 1] = cs.jColorStart[c] + (int)w->jointColors[c].size();
+	cs.jsrc.resize( cs.jColorStart[kColorCount] );
+	for ( int c = 0; c < kColorCount; ++c )
+		for ( size_t li = 0; li < w->jointColors[c].size(); ++li )
+		{
+			const Joint& jt = w->joints[w->jointColors[c][li]];
+			cs.jsrc[cs.jColorStart[c] + li] =
+				Src{ jt.jointId, awakeRef( w->bodies[jt.edges[0].bodyId] ), awakeRef( w->bodies[jt.edges[1].bodyId] ) };
+		}
 	std::vector<int> sims( na ), posOf( na );
 	for ( int i = 0; i < na; ++i )
 		sims[i] = posOf[i] = i;
@@ -1277,7 +1376,7 @@ static void rebuildDirtyItems( World* w, int target, int budget, int fullBudget,
 		for ( int i = 0; i < wi.ownedCount; ++i )
 			sims.push_back( L.bodyList[wi.bodyStart + i] );
 		deadBodies += wi.bodyCount;
-		deadSlots += wi.groupStart[kGroups];
+		deadSlots += item_slots( wi );
 	}
 	if ( 2 * ( L.liveBodies - deadBodies ) < (int)L.bodyList.size() || 2 * ( L.liveSlots - deadSlots ) < (int)L.slotContact.size() )
 	{
@@ -1329,6 +1428,37 @@ static void rebuildDirtyItems( World* w, int target, int budget, int fullBudget,
 	}
 	for ( int c = 0; c < kColorCount; ++c )
 		cs.colorStart[c + 1] += cs.colorStart[c];
+	// and the joints these bodies own (a joint's owner is its first dynamic body), in colour-list
+	// order
+	keys.clear();
+	for ( int sim : sims )
+	{
+		const Body& body = w->bodies[w->awakeSimBody[sim]];
+		for ( int key = body.headJointKey; key != -1; )
+		{
+			const Joint& jt = w->joints[key >> 1];
+			const int next = jt.edges[key & 1].nextKey;
+			if ( jt.setIndex == kAwakeSet )
+			{
+				int ra = awakeRef( w->bodies[jt.edges[0].bodyId] ), rb = awakeRef( w->bodies[jt.edges[1].bodyId] );
+				if ( ( ra >= 0 ? ra : rb ) == sim )
+					keys.push_back( Key{ jt.colorIndex, jt.localIndex, Src{ jt.jointId, ra, rb } } );
+			}
+			key = next;
+		}
+	}
+	std::sort( keys.begin(), keys.end(),
+			   []( const Key& x, const Key& y ) { return x.color != y.color ? x.color < y.color : x.local < y.local; } );
+	cs.jsrc.resize( keys.size() );
+	for ( int c = 0; c <= kColorCount; ++c )
+		cs.jColorStart[c] = 0;
+	for ( size_t q = 0; q < keys.size(); ++q )
+	{
+		cs.jsrc[q] = keys[q].s;
+		cs.jColorStart[keys[q].color + 1] += 1;
+	}
+	for ( int c = 0; c < kColorCount; ++c )
+		cs.jColorStart[c + 1] += cs.jColorStart[c];
 	makeComponents( cs, sims, posOf, false );
 	for ( int sim : sims )
 		posOf[sim] = -1;
@@ -1443,8 +1573,8 @@ static void validateItems( const StepGroup* g )
 	{
 		B2MI_CHECK( 0 <= wi.world && wi.world < nw );
 		const World* w = g->worlds[wi.world];
-		const int total = wi.groupStart[kGroups];
-		B2MI_CHECK( wi.groupStart[0] == 0 && total >= 0 && wi.bodyCount >= 0 );
+		const int total = item_slots( wi );
+		B2MI_CHECK( wi.groupStart[0] == 0 && total >= 0 && wi.bodyCount >= 0 && wi.jointGroupStart[0] == wi.groupStart[kGroups] );
 		B2MI_CHECK( w->bodyRegion.base <= wi.bodyStart && wi.bodyStart + wi.bodyCount <= w->bodyRegion.base + w->bodyRegion.cap );
 		B2MI_CHECK( wi.bodyStart + wi.bodyCount <= g->capBodyList && 2 * ( wi.bodyStart + wi.bodyCount ) <= g->capPriv &&
 					wi.bodyStart + wi.bodyCount <= g->capMass );
@@ -1462,6 +1592,19 @@ static void validateItems( const StepGroup* g )
 			for ( int L = 0; L < wi.ovfLevelCount; ++L )
 				B2MI_CHECK( lv[L] < lv[L + 1] );
 		}
+		if ( wi.jointGroupStart[1] > wi.jointGroupStart[0] )
+		{
+			B2MI_CHECK( wi.jointLevelCount >= 1 && wi.jointLevelBegin >= 0 &&
+						wi.jointLevelBegin + wi.jointLevelCount + 1 <= (int)g->hOvfLevels.size() &&
+						wi.jointLevelBegin + wi.jointLevelCount + 1 <= g->capOvfLevels );
+			const int* lv = g->hOvfLevels.data() + wi.jointLevelBegin;
+			B2MI_CHECK( lv[0] == wi.jointGroupStart[0] && lv[wi.jointLevelCount] == wi.jointGroupStart[1] );
+			for ( int L = 0; L < wi.jointLevelCount; ++L )
+				B2MI_CHECK( lv[L] < lv[L + 1] );
+		}
+		// the joint records the item's slots name lie inside its world's joint array
+		for ( int k = wi.jointGroupStart[0]; k < total; ++k )
+			B2MI_CHECK( 0 <= g->hSlotContact[wi.slotStart + k] && g->hSlotContact[wi.slotStart + k] < w->capJoints );
 		B2MI_CHECK( wi.ldsBytes >= 0 && wi.ldsBytes <= kMaxItemLdsBytes );
 		bodySpans.push_back( make_int2( wi.bodyStart, wi.bodyStart + wi.bodyCount ) );
 		slotSpans.push_back( make_int2( wi.slotStart, wi.slotStart + total ) );
@@ -1550,6 +1693,7 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 				wi.bodyStart += w->bodyRegion.base;
 				wi.slotStart += w->slotRegion.base;
 				wi.ovfLevelBegin += levelBase;
+				wi.jointLevelBegin += levelBase;
 				g->hItems.push_back( wi );
 			}
 			g->maxGroupSize = std::max( g->maxGroupSize, w->solver.maxGroupSize );
@@ -1646,6 +1790,7 @@ static void fillDevWorld( DevWorld& d, World* w, float timeStep, int subStepCoun
 	d.bodyShapes = w->dBodyShapes;
 	d.contacts = w->dContacts;
 	d.contactOut = w->dContactOut;
+	d.joints = w->dJoints;
 	d.gravityX = w->gravity.x;
 	d.gravityY = w->gravity.y;
 	// src/world.c:717-746
@@ -1658,6 +1803,7 @@ static void fillDevWorld( DevWorld& d, World* w, float timeStep, int subStepCoun
 	float contactHertz = b2MinFloat( w->contactHertz, 0.25f * d.inv_h );
 	makeSoft( contactHertz, w->contactDampingRatio, d.h, d.contactSoft );
 	makeSoft( 2.0f * contactHertz, w->contactDampingRatio, d.h, d.staticSoft );
+	makeSoft( b2MinFloat( w->jointHertz, 0.125f * d.inv_h ), w->jointDampingRatio, d.h, d.jointSoft );
 	d.restitutionThreshold = w->restitutionThreshold;
 	d.pushout = w->contactPushoutVelocity;
 	d.maxLinearSpeed = 4.0f * d.inv_dt;				  // b2_maxTranslation * inv_dt
@@ -2247,30 +2393,42 @@ hipError_t StepGroup::enqueueSplit( int item, hipStream_t s )
 	const int levelCount = wi.groupStart[1] > wi.groupStart[0] ? wi.ovfLevelCount : 0;
 	const int* levels = hOvfLevels.data() + wi.ovfLevelBegin;
 	B2MI_CHECK( levelCount == 0 || wi.ovfLevelBegin + levelCount < (int)hOvfLevels.size() );
-	auto overflow = [&]( int phase ) {
+	// the overflow joints in levels before the overflow contacts, the joints of a colour in a
+	// launch before the colour's contacts (they share no body with them)
+	const int jointLevelCount = wi.jointGroupStart[1] > wi.jointGroupStart[0] ? wi.jointLevelCount : 0;
+	const int* jointLevels = hOvfLevels.data() + wi.jointLevelBegin;
+	B2MI_CHECK( jointLevelCount == 0 || wi.jointLevelBegin + jointLevelCount < (int)hOvfLevels.size() );
+	auto overflow = [&]( int jointPhase, int phase ) {
+		for ( int L = 0; L < jointLevelCount && jointPhase >= 0; ++L )
+			run( jointPhase, jointLevels[L], jointLevels[L + 1] - jointLevels[L] );
 		for ( int L = 0; L < levelCount; ++L )
 			run( phase, levels[L], levels[L + 1] - levels[L] );
 	};
-	auto colours = [&]( int phase ) {
+	auto colours = [&]( int jointPhase, int phase ) {
 		for ( int g = 1; g < kGroups; ++g )
+		{
+			if ( jointPhase >= 0 )
+				run( jointPhase, wi.jointGroupStart[g], wi.jointGroupStart[g + 1] - wi.jointGroupStart[g] );
 			run( phase, wi.groupStart[g], wi.groupStart[g + 1] - wi.groupStart[g] );
+		}
 	};
 	run( kSplitStage, 0, wi.bodyCount );
+	run( kSplitJointPrepare, wi.jointGroupStart[0], wi.jointGroupStart[kGroups] - wi.jointGroupStart[0] );
 	run( kSplitPrepare, 0, total );
 	for ( int sub = 0; sub < hw.subSteps; ++sub )
 	{
 		run( kSplitIntegrateVelocities, 0, wi.bodyCount );
-		overflow( kSplitOverflowWarmStart );
+		overflow( kSplitJointWarmStart, kSplitOverflowWarmStart );
 		if ( hw.flags & kWorldWarmStart )
-			colours( kSplitWarmStart );
-		overflow( kSplitOverflowSolve );
-		colours( kSplitSolve );
+			colours( kSplitJointWarmStart, kSplitWarmStart );
+		overflow( kSplitJointSolve, kSplitOverflowSolve );
+		colours( kSplitJointSolve, kSplitSolve );
 		run( kSplitIntegratePositions, 0, wi.bodyCount );
-		overflow( kSplitOverflowRelax );
-		colours( kSplitRelax );
+		overflow( kSplitJointRelax, kSplitOverflowRelax );
+		colours( kSplitJointRelax, kSplitRelax );
 	}
-	overflow( kSplitOverflowRestitution );
-	colours( kSplitRestitution );
+	overflow( -1, kSplitOverflowRestitution );
+	colours( -1, kSplitRestitution );
 	run( kSplitStoreImpulses, 0, total );
 	run( kSplitFinalize, 0, wi.ownedCount );
 	return status;
@@ -2397,7 +2555,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 			shapeDelta.fetch_add( ns - w->countedShapes );
 			w->countedShapes = ns;
 		}
-		if ( w->bodiesDirty || !w->pendingInits.empty() || w->contactIdPool.nextIndex > w->capContacts || w->dContacts == nullptr )
+		if ( w->bodiesDirty || w->jointsDirty || !w->pendingInits.empty() || w->contactIdPool.nextIndex > w->capContacts ||
+			 w->dContacts == nullptr )
 			serial.add( w );
 		else
 			refreshParams( w );
@@ -2410,6 +2569,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	{
 		if ( w->bodiesDirty )
 			b2mi_internal_uploadBodies( w );
+		if ( w->jointsDirty )
+			uploadJoints( w );
 		ensureContactCapacity( w );
 		hInits.insert( hInits.end(), w->pendingInits.begin(), w->pendingInits.end() );
 		w->pendingInits.clear();

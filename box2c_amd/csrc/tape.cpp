@@ -171,6 +171,8 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 				fatal( "a taped run does not support overflow constraints (more than 11 graph colours)", __FILE__, __LINE__ );
 			if ( wi.ownedCount != wi.bodyCount )
 				fatal( "a taped run does not support kinematic bodies", __FILE__, __LINE__ );
+			if ( item_slots( wi ) != wi.groupStart[kGroups] )
+				fatal( "a taped run does not support joints", __FILE__, __LINE__ );
 			itemsOfWorld[wi.world] += 1;
 			v.usedSlots += wi.groupStart[kGroups] - wi.groupStart[1];
 			for ( int k = 0; k < wi.groupStart[kGroups]; ++k )

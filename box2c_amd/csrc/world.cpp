@@ -133,6 +133,7 @@ World::~World()
 	devFree( dBodyShapes );
 	devFree( dContacts );
 	devFree( dContactOut );
+	devFree( dJoints );
 }
 
 StepGroup::~StepGroup()
@@ -424,6 +425,8 @@ static void beginEdit( World* w )
 
 static void updateBodyMass( World* w, Body* body );
 void destroyContactImpl( World* w, Contact* c );
+void destroyBodyJoints( World* w, Body* body );
+int jointColorCount( const World* w, int color );
 
 void b2mi_internal_beginEdit( World* w )
 {
@@ -667,10 +670,11 @@ B2_API b2Counters b2World_GetCounters( b2WorldId worldId )
 	c.bodyCount = w->bodyIdPool.count();
 	c.shapeCount = w->shapeIdPool.count();
 	c.contactCount = w->contactIdPool.count();
+	c.jointCount = w->jointIdPool.count();
 	c.staticTreeHeight = w->trees[b2_staticBody].height();
 	c.treeHeight = std::max( w->trees[b2_dynamicBody].height(), w->trees[b2_kinematicBody].height() );
 	for ( int i = 0; i < kColorCount; ++i )
-		c.colorCounts[i] = (int)w->colors[i].size();
+		c.colorCounts[i] = (int)w->colors[i].size() + jointColorCount( w, i );
 	return c;
 }
 
@@ -1341,6 +1345,7 @@ B2_API void b2DestroyBody( b2BodyId bodyId )
 {
 	B2MI_BODY( bodyId );
 	beginEdit( w );
+	destroyBodyJoints( w, body );
 	for ( int key = body->headContactKey; key != -1; )
 	{
 		Contact* c = &w->contacts[key >> 1];

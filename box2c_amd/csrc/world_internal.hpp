@@ -115,6 +115,8 @@ struct Body
 	int contactCount = 0;
 	int headShapeId = -1;
 	int shapeCount = 0;
+	int headJointKey = -1;
+	int jointCount = 0;
 	int id = -1;
 	b2BodyType type = b2_staticBody;
 	uint16_t revision = 0;
@@ -161,6 +163,28 @@ struct Contact
 	int shapeIdA = -1, shapeIdB = -1;
 	int contactId = -1;
 	uint32_t flags = 0; // kContactTouching, kContactEvents
+};
+
+// A joint's host record (src/joint.h:182-205): its id, set, colour and place in that colour's
+// list, and the edges of the two bodies' joint lists. Its parameters, impulses and prepared terms
+// are a DevJoint, kept on the device by joint id (World::dJoints) with a host mirror.
+struct JointEdge
+{
+	int bodyId, prevKey, nextKey;
+};
+
+struct Joint
+{
+	void* userData = nullptr;
+	int setIndex = -1; // -1 free, 0 static (both bodies static), 2 awake
+	int colorIndex = -1;
+	int localIndex = -1;
+	JointEdge edges[2];
+	int jointId = -1;
+	b2JointType type = b2_revoluteJoint;
+	uint16_t revision = 0;
+	float drawSize = 1.0f;
+	bool collideConnected = false;
 };
 
 // Contact::flags (src/contact.h b2ContactFlags)
@@ -361,6 +385,20 @@ struct World
 	std::vector<int> nonTouching; // awake non-touching contact ids (array order of the reference)
 	std::vector<int> colors[kColorCount];
 	BitSet colorBodySets[kColorCount];
+
+	// joints: host records, each colour's joint list (awake joints; the order of the reference's
+	// colour arrays, which fixes the overflow joints' serial order) and the static set's list
+	std::vector<Joint> joints;
+	IdPool jointIdPool;
+	std::vector<int> jointColors[kColorCount];
+	std::vector<int> staticJoints;
+	// the joints' device records (by joint id) and their host mirror: the mirror is current when
+	// jointsDirty (edits not yet uploaded) or jointsEpoch is the group's step epoch
+	std::vector<DevJoint> devJoints;
+	DevJoint* dJoints = nullptr;
+	int capJoints = 0;
+	bool jointsDirty = false;
+	uint64_t jointsEpoch = 0;
 
 	DynamicTree trees[3];
 	BitSet moveSet;

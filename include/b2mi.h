@@ -224,5 +224,10 @@ B2_API int b2mi_Batch_ReplayTapeNoAlloc( int batch, const float* startPoses, con
 B2_API void b2mi_Batch_Backward( int batch, const float* gradEnd, float* gradStart, float* gradParams );
 
 // ---- device ----------------------------------------------------------------------------
+// The arc tangent the joint kernels use (the fdlibm float algorithm, so that joint angles round as
+// the host C library's atan2f does), evaluated on the host over n argument pairs: out[i] =
+// atan2(y[i], x[i]). No counterpart in the reference; for checking the kernels' angles.
+B2_API void b2mi_Atan2( const float* y, const float* x, float* out, int n );
+
 B2_API int b2mi_DeviceCount( void );
 B2_API const char* b2mi_GetLastError( void );

@@ -2,8 +2,9 @@
 //
 // Field order and types follow the reference's include/box2d/types.h (task callbacks 30-48,
 // b2WorldDef 56-109, b2BodyDef 128-192, b2Filter 197-213, b2ShapeType 227-235, b2ShapeDef
-// 241-286, b2Profile 397-421, b2Counters 424-438, b2ContactData 1010-1015) so that a
-// definition filled in by unchanged application code means the same thing here.
+// 241-286, b2Profile 397-421, b2Counters 424-438, the joint definitions 446-857, b2ContactData
+// 1010-1015) so that a definition filled in by unchanged application code means the same thing
+// here.
 #pragma once
 
 #include "base.h"
@@ -162,6 +163,162 @@ typedef struct b2ContactData
 	b2ShapeId shapeIdB;
 	b2Manifold manifold;
 } b2ContactData;
+
+// ---- joints (reference include/box2d/types.h:446-857) ----
+
+typedef enum b2JointType // :446
+{
+	b2_distanceJoint,
+	b2_motorJoint,
+	b2_mouseJoint,
+	b2_prismaticJoint,
+	b2_revoluteJoint,
+	b2_weldJoint,
+	b2_wheelJoint,
+} b2JointType;
+
+typedef struct b2DistanceJointDef // :464
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 localAnchorA;
+	b2Vec2 localAnchorB;
+	float length;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float minLength;
+	float maxLength;
+	bool enableMotor;
+	float maxMotorForce;
+	float motorSpeed;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2DistanceJointDef;
+
+B2_API b2DistanceJointDef b2DefaultDistanceJointDef( void ); // :521
+
+typedef struct b2MotorJointDef // :527
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 linearOffset;
+	float angularOffset;
+	float maxForce;
+	float maxTorque;
+	float correctionFactor;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2MotorJointDef;
+
+B2_API b2MotorJointDef b2DefaultMotorJointDef( void ); // :562
+
+typedef struct b2MouseJointDef // :569
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 target;
+	float hertz;
+	float dampingRatio;
+	float maxForce;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2MouseJointDef;
+
+B2_API b2MouseJointDef b2DefaultMouseJointDef( void ); // :601
+
+typedef struct b2PrismaticJointDef // :610
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 localAnchorA;
+	b2Vec2 localAnchorB;
+	b2Vec2 localAxisA;
+	float referenceAngle;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float lowerTranslation;
+	float upperTranslation;
+	bool enableMotor;
+	float maxMotorForce;
+	float motorSpeed;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2PrismaticJointDef;
+
+B2_API b2PrismaticJointDef b2DefaultPrismaticJointDef( void ); // :669
+
+typedef struct b2RevoluteJointDef // :683
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 localAnchorA;
+	b2Vec2 localAnchorB;
+	float referenceAngle;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float lowerAngle;
+	float upperAngle;
+	bool enableMotor;
+	float maxMotorTorque;
+	float motorSpeed;
+	float drawSize;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2RevoluteJointDef;
+
+B2_API b2RevoluteJointDef b2DefaultRevoluteJointDef( void ); // :743
+
+typedef struct b2WeldJointDef // :751
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 localAnchorA;
+	b2Vec2 localAnchorB;
+	float referenceAngle;
+	float linearHertz;
+	float angularHertz;
+	float linearDampingRatio;
+	float angularDampingRatio;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2WeldJointDef;
+
+B2_API b2WeldJointDef b2DefaultWeldJointDef( void ); // :792
+
+typedef struct b2WheelJointDef // :801
+{
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Vec2 localAnchorA;
+	b2Vec2 localAnchorB;
+	b2Vec2 localAxisA;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float lowerTranslation;
+	float upperTranslation;
+	bool enableMotor;
+	float maxMotorTorque;
+	float motorSpeed;
+	bool collideConnected;
+	void* userData;
+	int32_t internalValue;
+} b2WheelJointDef;
+
+B2_API b2WheelJointDef b2DefaultWheelJointDef( void ); // :857
 
 // ---- events of the last step (reference include/box2d/types.h:876-1005) ----
 // Filled by b2World_Step and valid until the next step; the arrays are owned by the world.

@@ -3,6 +3,7 @@
 #include "or_internal.h"
 
 #include <float.h>
+#include <math.h>
 #include <stdio.h>
 
 // ---- assert hook (src/core.c:23-37) ----
@@ -433,4 +434,12 @@ b2AABB or_compute_shape_aabb( const orShape* shape, b2Transform xf )
 		default:
 			return ( b2AABB ){ xf.p, xf.p };
 	}
+}
+
+// The C library's atan2f over n argument pairs (b2RelativeAngle's arc tangent,
+// include/box2d/math_functions.h:387-394), for checking the kernels' restatement of it.
+B2_API void oracle_Atan2( const float* y, const float* x, float* out, int n )
+{
+	for ( int i = 0; i < n; ++i )
+		out[i] = atan2f( y[i], x[i] );
 }

@@ -208,6 +208,8 @@ typedef struct orBody
 	int contactCount;
 	int headShapeId;
 	int shapeCount;
+	int headJointKey;
+	int jointCount;
 	float sleepThreshold;
 	int id;
 	b2BodyType type;
@@ -332,6 +334,73 @@ typedef struct orContactSim
 } orContactSim;
 
 typedef OR_ARRAY( orContactSim ) orContactSimArray;
+
+typedef struct orSoftness
+{
+	real biasRate, massScale, impulseScale;
+} orSoftness;
+
+// ---- joints (src/joint.h:13-178, 182-226) ----
+typedef struct orMat22r
+{
+	orVec2r cx, cy;
+} orMat22r;
+
+typedef struct orJointEdge
+{
+	int bodyId;
+	int prevKey;
+	int nextKey;
+} orJointEdge;
+
+typedef struct orJoint
+{
+	void* userData;
+	int setIndex; // OR_NULL when free; 0 static set, 2 awake set
+	int colorIndex;
+	int localIndex;
+	orJointEdge edges[2];
+	int jointId;
+	b2JointType type;
+	uint16_t revision;
+	float drawSize;
+	bool collideConnected;
+} orJoint;
+
+typedef struct orJointSim
+{
+	int jointId;
+	int bodyIdA, bodyIdB;
+	b2JointType type;
+	b2Vec2 localOriginAnchorA, localOriginAnchorB;
+	real invMassA, invMassB, invIA, invIB;
+	int indexA, indexB; // awake sim index, OR_NULL for a static body
+	bool enableSpring, enableLimit, enableMotor;
+	// parameters
+	float referenceAngle; // revolute, prismatic, weld
+	float hertz, dampingRatio; // spring of revolute, prismatic, wheel, distance; mouse
+	float linearHertz, linearDampingRatio, angularHertz, angularDampingRatio; // weld
+	float lower, upper; // revolute angles, prismatic and wheel translations, distance lengths
+	float maxMotor;		// max motor torque (revolute, wheel) or force (prismatic, distance)
+	float motorSpeed;
+	float length; // distance
+	b2Vec2 localAxisA; // prismatic, wheel
+	b2Vec2 linearOffset; // motor
+	float angularOffset, maxForce, maxTorque, correctionFactor; // motor, mouse (maxForce)
+	b2Vec2 target; // mouse
+	// accumulated impulses
+	orVec2r linearImpulse; // revolute, weld, motor, mouse; prismatic's 2D impulse
+	real angularImpulse; // weld, motor, mouse
+	real impulse; // distance axial, wheel perpendicular
+	real springImpulse, motorImpulse, lowerImpulse, upperImpulse;
+	// per-step terms of prepare
+	orVec2r anchorA, anchorB, deltaCenter, axisA;
+	real deltaAngle, axialMass, perpMass, motorMass, angularMass;
+	orMat22r linearMass;
+	orSoftness springSoftness, linearSoftness, angularSoftness;
+} orJointSim;
+
+typedef OR_ARRAY( orJointSim ) orJointSimArray;
 typedef OR_ARRAY( orBodySim ) orBodySimArray;
 typedef OR_ARRAY( orBodyState ) orBodyStateArray;
 
@@ -339,12 +408,8 @@ typedef struct orGraphColor
 {
 	orBitSet bodySet;
 	orContactSimArray contacts;
+	orJointSimArray joints;
 } orGraphColor;
-
-typedef struct orSoftness
-{
-	real biasRate, massScale, impulseScale;
-} orSoftness;
 
 typedef struct orWorld
 {
@@ -365,6 +430,11 @@ typedef struct orWorld
 
 	OR_ARRAY( orContact ) contacts;
 	orIdPool contactIdPool;
+
+	OR_ARRAY( orJoint ) joints;
+	orIdPool jointIdPool;
+	orJointSimArray staticJoints; // joints between two static bodies (the static set's)
+	float inv_h;				  // of the last step (src/world.c:731)
 
 	orGraphColor colors[OR_GRAPH_COLORS];
 
@@ -395,7 +465,7 @@ typedef struct orStepContext
 {
 	float dt, inv_dt, h, inv_h;
 	int subStepCount;
-	orSoftness contactSoftness, staticSoftness;
+	orSoftness contactSoftness, staticSoftness, jointSoftness;
 	float restitutionThreshold;
 	bool enableWarmStarting;
 } orStepContext;
@@ -417,6 +487,12 @@ void or_destroy_contact( orWorld* world, orContact* contact );
 void or_add_contact_to_graph( orWorld* world, orContactSim* contactSim, orContact* contact );
 void or_remove_contact_from_graph( orWorld* world, int bodyIdA, int bodyIdB, int colorIndex, int localIndex );
 void or_collide( orWorld* world, orStepContext* ctx );
+bool or_should_bodies_collide( orWorld* world, orBody* bodyA, orBody* bodyB );
+void or_destroy_joint_internal( orWorld* world, orJoint* joint );
+void or_prepare_joint( orWorld* world, orJointSim* joint, const orStepContext* ctx );
+void or_warm_start_joint( orJointSim* joint, orBodyState* states );
+void or_solve_joint( orJointSim* joint, orBodyState* states, const orStepContext* ctx, bool useBias );
+orSoftness or_make_softness( real hertz, real zeta, real h );
 void or_solve( orWorld* world, orStepContext* ctx );
 void or_assert_fail( const char* what, const char* file, int line );
 

@@ -57,6 +57,11 @@ static orSoftness or_make_soft( real hertz, real zeta, real h )
 	return ( orSoftness ){ omega / a1, a2 * a3, a3 };
 }
 
+orSoftness or_make_softness( real hertz, real zeta, real h )
+{
+	return or_make_soft( hertz, zeta, h );
+}
+
 static const orBodyState or_identity_state = { { 0, 0 }, 0, 0, { 0, 0 }, { 1, 0 } };
 
 // ---- prepare (contact_solver.c:597-808 colour lanes; 21-147 overflow) ----
@@ -696,16 +701,25 @@ void or_solve( orWorld* world, orStepContext* ctx )
 	orBodyState* states = world->awakeStates.data;
 	real warmStartScale = world->enableWarmStarting ? 1 : 0;
 
-	// active colours and constraints (solver.c:1145-1341)
+	// active colours: those with contacts or joints (solver.c:1145-1171)
 	int activeColors[OR_GRAPH_COLORS];
 	int activeColorCount = 0, total = 0;
 	for ( int i = 0; i < OR_OVERFLOW; ++i )
 	{
-		if ( world->colors[i].contacts.count > 0 )
+		if ( world->colors[i].contacts.count + world->colors[i].joints.count > 0 )
 		{
 			activeColors[activeColorCount++] = i;
 			total += world->colors[i].contacts.count;
 		}
+	}
+
+	// prepare joints: the colours' (b2_stagePrepareJoints) then the overflow's
+	// (solver.c:598-618); each writes only its own joint
+	for ( int c = 0; c < OR_GRAPH_COLORS; ++c )
+	{
+		orJointSimArray* joints = &world->colors[c].joints;
+		for ( int i = 0; i < joints->count; ++i )
+			or_prepare_joint( world, joints->data + i, ctx );
 	}
 	orGraphColor* overflow = world->colors + OR_OVERFLOW;
 	int overflowCount = overflow->contacts.count;
@@ -727,23 +741,50 @@ void or_solve( orWorld* world, orStepContext* ctx )
 	real inv_h = ctx->inv_h;
 	real pushout = world->contactPushoutVelocity;
 	real minBiasVel = -pushout;
+	orJointSimArray* overflowJoints = &overflow->joints;
 	for ( int sub = 0; sub < ctx->subStepCount; ++sub )
 	{
 		or_integrate_velocities( world, ctx );
+		// warm start: overflow joints and contacts always (solver.c:638-640), the colours' joint
+		// and contact blocks when warm starting is on (solver.c:398-410)
+		for ( int i = 0; i < overflowJoints->count; ++i )
+			or_warm_start_joint( overflowJoints->data + i, states );
 		for ( int i = 0; i < overflowCount; ++i )
 			or_overflow_warm_start( ovf + i, states );
 		if ( world->enableWarmStarting )
-			for ( int i = 0; i < colorStart[activeColorCount]; ++i )
-				or_lane_warm_start( cons + i, states );
+			for ( int a = 0; a < activeColorCount; ++a )
+			{
+				orJointSimArray* joints = &world->colors[activeColors[a]].joints;
+				for ( int i = 0; i < joints->count; ++i )
+					or_warm_start_joint( joints->data + i, states );
+				for ( int i = colorStart[a]; i < colorStart[a + 1]; ++i )
+					or_lane_warm_start( cons + i, states );
+			}
+		for ( int i = 0; i < overflowJoints->count; ++i )
+			or_solve_joint( overflowJoints->data + i, states, ctx, true );
 		for ( int i = 0; i < overflowCount; ++i )
 			or_overflow_solve( ovf + i, states, inv_h, pushout, true );
-		for ( int i = 0; i < colorStart[activeColorCount]; ++i )
-			or_lane_solve( cons + i, states, inv_h, minBiasVel, true );
+		for ( int a = 0; a < activeColorCount; ++a )
+		{
+			orJointSimArray* joints = &world->colors[activeColors[a]].joints;
+			for ( int i = 0; i < joints->count; ++i )
+				or_solve_joint( joints->data + i, states, ctx, true );
+			for ( int i = colorStart[a]; i < colorStart[a + 1]; ++i )
+				or_lane_solve( cons + i, states, inv_h, minBiasVel, true );
+		}
 		or_integrate_positions( world, ctx );
+		for ( int i = 0; i < overflowJoints->count; ++i )
+			or_solve_joint( overflowJoints->data + i, states, ctx, false );
 		for ( int i = 0; i < overflowCount; ++i )
 			or_overflow_solve( ovf + i, states, inv_h, pushout, false );
-		for ( int i = 0; i < colorStart[activeColorCount]; ++i )
-			or_lane_solve( cons + i, states, inv_h, minBiasVel, false );
+		for ( int a = 0; a < activeColorCount; ++a )
+		{
+			orJointSimArray* joints = &world->colors[activeColors[a]].joints;
+			for ( int i = 0; i < joints->count; ++i )
+				or_solve_joint( joints->data + i, states, ctx, false );
+			for ( int i = colorStart[a]; i < colorStart[a + 1]; ++i )
+				or_lane_solve( cons + i, states, inv_h, minBiasVel, false );
+		}
 	}
 	real threshold = ctx->restitutionThreshold;
 	for ( int i = 0; i < overflowCount; ++i )
@@ -884,10 +925,13 @@ B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount )
 	// the double build keeps the float step sizes and derives softness in double
 	ctx.contactSoftness = or_make_soft( contactHertz, world->contactDampingRatio, ctx.h );
 	ctx.staticSoftness = or_make_soft( 2.0 * (double)contactHertz, world->contactDampingRatio, ctx.h );
+	ctx.jointSoftness = or_make_soft( b2MinFloat( world->jointHertz, 0.125f * ctx.inv_h ), world->jointDampingRatio, ctx.h );
 #else
 	ctx.contactSoftness = or_make_soft( contactHertz, world->contactDampingRatio, ctx.h );
 	ctx.staticSoftness = or_make_soft( 2.0f * contactHertz, world->contactDampingRatio, ctx.h );
+	ctx.jointSoftness = or_make_soft( b2MinFloat( world->jointHertz, 0.125f * ctx.inv_h ), world->jointDampingRatio, ctx.h );
 #endif
+	world->inv_h = ctx.inv_h;
 	ctx.restitutionThreshold = world->restitutionThreshold;
 	ctx.enableWarmStarting = world->enableWarmStarting;
 

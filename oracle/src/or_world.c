@@ -163,10 +163,14 @@ B2_API void b2DestroyWorld( b2WorldId worldId )
 	OR_FREE( world->shapeIdPool.freeArray );
 	OR_FREE( world->contacts );
 	OR_FREE( world->contactIdPool.freeArray );
+	OR_FREE( world->joints );
+	OR_FREE( world->jointIdPool.freeArray );
+	OR_FREE( world->staticJoints );
 	for ( int i = 0; i < OR_GRAPH_COLORS; ++i )
 	{
 		or_bitset_free( &world->colors[i].bodySet );
 		OR_FREE( world->colors[i].contacts );
+		OR_FREE( world->colors[i].joints );
 	}
 	for ( int i = 0; i < 3; ++i )
 		or_tree_destroy( world->trees + i );
@@ -262,10 +266,11 @@ B2_API b2Counters b2World_GetCounters( b2WorldId worldId )
 	s.bodyCount = or_id_count( &world->bodyIdPool );
 	s.shapeCount = or_id_count( &world->shapeIdPool );
 	s.contactCount = or_id_count( &world->contactIdPool );
+	s.jointCount = or_id_count( &world->jointIdPool );
 	s.staticTreeHeight = or_tree_height( world->trees + b2_staticBody );
 	s.treeHeight = b2MaxInt( or_tree_height( world->trees + b2_dynamicBody ), or_tree_height( world->trees + b2_kinematicBody ) );
 	for ( int i = 0; i < OR_GRAPH_COLORS; ++i )
-		s.colorCounts[i] = world->colors[i].contacts.count;
+		s.colorCounts[i] = world->colors[i].contacts.count + world->colors[i].joints.count;
 	return s;
 }
 
@@ -330,6 +335,8 @@ B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def )
 	body->revision = revision + 1;
 	body->headShapeId = OR_NULL;
 	body->headContactKey = OR_NULL;
+	body->headJointKey = OR_NULL;
+	body->jointCount = 0;
 	body->id = bodyId;
 	body->sleepThreshold = def->sleepThreshold;
 	body->type = def->type;
@@ -1259,6 +1266,13 @@ B2_API void b2DestroyBody( b2BodyId bodyId )
 {
 	OR_BODY( bodyId );
 	OR_ASSERT( world->locked == false );
+	// the attached joints first (src/body.c:349-362)
+	for ( int jointKey = body->headJointKey; jointKey != OR_NULL; )
+	{
+		orJoint* joint = world->joints.data + ( jointKey >> 1 );
+		jointKey = joint->edges[jointKey & 1].nextKey;
+		or_destroy_joint_internal( world, joint );
+	}
 	int key = body->headContactKey;
 	while ( key != OR_NULL )
 	{
@@ -1377,10 +1391,11 @@ static bool or_pair_query_callback( int proxyId, int shapeId, void* context )
 		return true;
 	if ( or_should_shapes_collide( shapeA->filter, shapeB->filter ) == false )
 		return true;
-	// b2ShouldBodiesCollide (body.c:1723-1760) without joints: at least one dynamic body
+	// b2ShouldBodiesCollide (body.c:1723-1760): at least one dynamic body, no joint between them
+	// that turns their collision off
 	orBody* bodyA = world->bodies.data + shapeA->bodyId;
 	orBody* bodyB = world->bodies.data + shapeB->bodyId;
-	if ( bodyA->type != b2_dynamicBody && bodyB->type != b2_dynamicBody )
+	if ( or_should_bodies_collide( world, bodyA, bodyB ) == false )
 		return true;
 
 	orMovePair pair = { shapeIdA, shapeIdB, *q->head };
