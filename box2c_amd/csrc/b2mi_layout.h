@@ -390,6 +390,12 @@ constexpr int kDefaultSplitSlots = 16384;
 constexpr int kDefaultSplitSlotsWorld = 4096;
 // threads per workgroup of the cooperative grid solver
 constexpr int kGridSolveBlock = 256;
+// Overflow levels of an HBM-store item run in runs: consecutive levels of at most
+// kLevelCacheSlots slots in all run in one workgroup (of kSplitLevelBlock threads in a grid-wide
+// item) with the bodies they touch and their records in LDS, a workgroup barrier per level; a
+// larger level runs on its own over all the item's threads.
+constexpr int kLevelCacheSlots = 128;
+constexpr int kSplitLevelBlock = 256;
 
 // Where a work item keeps its solver state during the step
 enum ItemStore : int

@@ -258,6 +258,25 @@ B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks )
 	setGridBlocks( worldGroup( worldId ), blocks );
 }
 
+// HBM-store items run consecutive overflow levels of at most this many slots in all as one run
+// with its bodies and records in LDS (-1 = the default, kLevelCacheSlots, which is also the
+// largest; 0 = none)
+static void setLevelRunSlots( StepGroup* g, int slots )
+{
+	g->levelRunSlots = slots < 0 ? kLevelCacheSlots : std::min( slots, kLevelCacheSlots );
+	g->allActive = true;
+}
+
+B2_API void b2mi_Batch_SetLevelRunSlots( int batch, int slots )
+{
+	setLevelRunSlots( getBatch( batch ), slots );
+}
+
+B2_API void b2mi_World_SetLevelRunSlots( b2WorldId worldId, int slots )
+{
+	setLevelRunSlots( worldGroup( worldId ), slots );
+}
+
 B2_API void b2mi_World_GetDevicePairStats( b2WorldId worldId, float* stats )
 {
 	World* w = getWorld( worldId );
@@ -339,7 +358,8 @@ B2_API int b2mi_Batch_GetSolverLaunches( int batch, int* records, int capacity )
 	return n;
 }
 
-// The work items of the last solver launch, in launch order: per item (constraint slots, bodies
+// The work items of the last solver launch, in launch order: per item (constraint slots, contacts
+// and joints, bodies
 // including kinematic shadows, store, how it ran: 0 one workgroup, 1 one grid launch per phase,
 // 2 one cooperative grid launch).
 static int workItems( StepGroup* g, int* records, int capacity )
@@ -353,7 +373,7 @@ static int workItems( StepGroup* g, int* records, int capacity )
 		const WorkItem& wi = g->hItems[i];
 		if ( records != nullptr && n < capacity )
 		{
-			records[4 * n + 0] = wi.groupStart[kGroups];
+			records[4 * n + 0] = item_slots( wi );
 			records[4 * n + 1] = wi.bodyCount;
 			records[4 * n + 2] = wi.store;
 			records[4 * n + 3] = mode[i];

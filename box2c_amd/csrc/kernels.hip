@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <float.h>
+#include <type_traits>
 
 #include "b2mi_layout.h"
 #include "box2d/math_functions.h"
@@ -408,6 +409,7 @@ __device__ inline int next_colour_slot( const WorkItem& wi, int g, int tid )
 struct BlockSync
 {
 	static constexpr bool kGrid = false;
+	int levelRun; // overflow levels of an HBM-store item: the largest cached run (slots)
 	__device__ inline int tid() const
 	{
 		return threadIdx.x;
@@ -439,6 +441,7 @@ struct GridSync
 	int* error;
 	int tid_, nt_;
 	unsigned blocks;
+	int levelRun; // overflow levels of an HBM-store item: the largest cached run (slots), run by workgroup 0
 	mutable unsigned target;
 	__device__ inline int tid() const
 	{
@@ -471,6 +474,138 @@ struct GridSync
 		__syncthreads();
 	}
 };
+
+// One run of overflow levels [L, E) of lv (slot bounds within the item, at most kLevelCacheSlots
+// slots in all) of an HBM-store item, for one overflow contact or joint level phase (`phase`, a
+// SplitPhase), by the threads of one workgroup, all of which call it. It stages in LDS the
+// distinct bodies the run's slots touch (state and masses, found through an LDS hash table), each
+// slot's two bodies as cache indices, and for contacts each slot's record and point count; runs
+// the levels on that copy with a workgroup barrier after each level; then writes back what the
+// levels write (the bodies' velocities and the records' written quads) and ends with a workgroup
+// barrier. The same functions on the same values in the same level order as on the store itself,
+// so the same bits. Kept out of line: the solver kernels call it only between colour sweeps, and
+// inlined its registers would add to theirs.
+__device__ __attribute__( ( noinline ) ) void cached_level_run( const GlobalStore src, const DevWorld& W, const int* __restrict__ sc,
+															   const int* __restrict__ lv, int L, int E, int phase )
+{
+	__shared__ RunCache rc;
+	const int tid = threadIdx.x, nt = blockDim.x;
+	const int s0 = lv[L], s1 = lv[E];
+	if ( s1 - s0 > kLevelCacheSlots )
+		return; // never: the callers form runs of at most kLevelCacheSlots slots
+	const bool contacts = phase == kSplitOverflowWarmStart || phase == kSplitOverflowSolve || phase == kSplitOverflowRelax ||
+						  phase == kSplitOverflowRestitution;
+	constexpr unsigned mask = kRunCacheHash - 1;
+	for ( int h = tid; h < kRunCacheHash; h += nt )
+		rc.hashKey[h] = -1;
+	if ( tid == 0 )
+		rc.count = 0;
+	__syncthreads();
+	// the distinct dynamic bodies (at most two per slot, so the table is at most half full)
+	for ( int k = s0 + tid; k < s1; k += nt )
+	{
+		const int2 b = src.bodies( k );
+#pragma unroll
+		for ( int j = 0; j < 2; ++j )
+		{
+			const int x = j == 0 ? b.x : b.y;
+			if ( x < 0 )
+				continue;
+			unsigned h = ( (unsigned)x * 2654435761u ) & mask;
+			for ( ;; )
+			{
+				const int prev = atomicCAS( &rc.hashKey[h], -1, x );
+				if ( prev == -1 || prev == x )
+					break;
+				h = ( h + 1 ) & mask;
+			}
+		}
+	}
+	__syncthreads();
+	for ( int h = tid; h < kRunCacheHash; h += nt )
+	{
+		const int x = rc.hashKey[h];
+		if ( x >= 0 )
+		{
+			const int i = atomicAdd( &rc.count, 1 );
+			rc.hashVal[h] = i;
+			rc.key[i] = x;
+		}
+	}
+	__syncthreads();
+	const int n = rc.count;
+	for ( int i = tid; i < n; i += nt )
+	{
+		const int b = rc.key[i];
+		rc.P[2 * i] = src.b0( b );
+		rc.P[2 * i + 1] = src.b1( b );
+		rc.M[i] = src.mass( b );
+	}
+	auto find = [&]( int x ) {
+		if ( x < 0 )
+			return x;
+		unsigned h = ( (unsigned)x * 2654435761u ) & mask;
+		while ( rc.hashKey[h] != x )
+			h = ( h + 1 ) & mask;
+		return rc.hashVal[h];
+	};
+	for ( int k = s0 + tid; k < s1; k += nt )
+	{
+		const int2 b = src.bodies( k );
+		rc.sb[k - s0] = make_int2( find( b.x ), find( b.y ) );
+		rc.pc[k - s0] = contacts ? W.contacts[sc[k]].manifold.pointCount : 0;
+		if ( contacts )
+		{
+#pragma unroll
+			for ( int q = 0; q < cfQuads; ++q )
+				rc.rec[( k - s0 ) * cfQuads + q] = src.quad( k, q );
+		}
+	}
+	__syncthreads();
+	const RunStore C{ &rc, s0 };
+	const Softness soft = world_softness( W );
+	for ( int M = L; M < E; ++M )
+	{
+		for ( int k = lv[M] + tid; k < lv[M + 1]; k += nt )
+		{
+			const int2 b = rc.sb[k - s0];
+			const int pc = rc.pc[k - s0];
+			switch ( phase )
+			{
+				case kSplitOverflowWarmStart:
+					ovf_warm_start( C, k, b.x, b.y, pc );
+					break;
+				case kSplitOverflowSolve:
+				case kSplitOverflowRelax:
+					ovf_solve( C, k, b.x, b.y, pc, W.inv_h, W.pushout, phase == kSplitOverflowSolve, soft );
+					break;
+				case kSplitOverflowRestitution:
+					ovf_restitution( C, k, b.x, b.y, pc, W.restitutionThreshold );
+					break;
+				case kSplitJointWarmStart:
+					warm_start_joint( C, W.joints[sc[k]], b.x, b.y );
+					break;
+				case kSplitJointSolve:
+				case kSplitJointRelax:
+					solve_joint( C, W.joints[sc[k]], b.x, b.y, W, phase == kSplitJointSolve );
+					break;
+				default:
+					break;
+			}
+		}
+		__syncthreads();
+	}
+	for ( int i = tid; i < n; i += nt )
+		src.b0( rc.key[i] ) = rc.P[2 * i];
+	if ( contacts )
+		for ( int k = s0 + tid; k < s1; k += nt )
+		{
+#pragma unroll
+			for ( int q = cfMutableQuad; q < cfQuads; ++q )
+				src.setQuad( k, q, rc.rec[( k - s0 ) * cfQuads + q] );
+		}
+	__syncthreads();
+}
 
 // All stages of one step for one work item, by the threads of X (one workgroup, or a cooperative
 // grid for a very large item), with X's barriers between dependent phases. `C` holds the item's
@@ -572,12 +707,59 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 				solve_joint( C, W.joints[sc[k]], lb.x, lb.y, W, stage == 1 );
 		}
 	};
-	auto overflowJoints = [&]( int stage ) {
-		for ( int L = 0; L < jointLevelCount; ++L )
-		{
-			jointStage( jointLevels[L], jointLevels[L + 1], stage );
+	// Runs levels [0, count) of lv in order, the slots of a level in parallel (they share no body
+	// that any of them changes): `body( store, slot, bodyA, bodyB, pointCount )`. An HBM-store item
+	// runs consecutive levels of at most X.levelRun slots in all as one cached run
+	// (cached_level_run; in a grid by workgroup 0 alone, then one grid barrier); a larger level,
+	// and every level of the LDS stores, runs over all of X's threads with X's barrier after it.
+	auto runLevels = [&]( const int* lv, int count, int phase, auto&& body ) {
+		const bool contacts = phase < kSplitJointPrepare;
+		auto plain = [&]( int L ) {
+			for ( int k = lv[L] + tid; k < lv[L + 1]; k += nt )
+			{
+				const int2 lb = contacts ? C.bodies( k ) : sb[k];
+				body( k, lb.x, lb.y, contacts ? W.contacts[sc[k]].manifold.pointCount : 0 );
+			}
 			X.sync();
+		};
+		if constexpr ( std::is_same<S, GlobalStore>::value )
+		{
+			const int run = X.levelRun < kLevelCacheSlots ? X.levelRun : kLevelCacheSlots;
+			int L = 0;
+			while ( L < count )
+			{
+				if ( run <= 0 || lv[L + 1] - lv[L] > run )
+				{
+					plain( L );
+					++L;
+					continue;
+				}
+				int E = L + 1;
+				while ( E < count && lv[E + 1] - lv[L] <= run )
+					++E;
+				if constexpr ( Sync::kGrid )
+				{
+					if ( blockIdx.x == 0 )
+						cached_level_run( C, W, sc, lv, L, E, phase );
+					X.sync();
+				}
+				else
+					cached_level_run( C, W, sc, lv, L, E, phase );
+				L = E;
+			}
 		}
+		else
+			for ( int L = 0; L < count; ++L )
+				plain( L );
+	};
+	auto overflowJoints = [&]( int stage ) {
+		runLevels( jointLevels, jointLevelCount, stage == 0 ? kSplitJointWarmStart : stage == 1 ? kSplitJointSolve : kSplitJointRelax,
+				   [&]( int k, int la, int lb, int ) {
+					   if ( stage == 0 )
+						   warm_start_joint( C, W.joints[sc[k]], la, lb );
+					   else
+						   solve_joint( C, W.joints[sc[k]], la, lb, W, stage == 1 );
+				   } );
 	};
 
 	// the velocity terms of this thread's first two bodies stay in registers over the substeps
@@ -599,15 +781,9 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 
 		// warm start: overflow (always, src/solver.c:639), then colours if enabled
 		overflowJoints( 0 );
-		for ( int L = 0; L < levelCount; ++L )
-		{
-			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
-			{
-				int2 lb = C.bodies( k );
-				ovf_warm_start( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount );
-			}
-			X.sync();
-		}
+		runLevels( levels, levelCount, kSplitOverflowWarmStart, [&]( int k, int la, int lb, int pc ) {
+			ovf_warm_start( C, k, la, lb, pc );
+		} );
 		if ( warm )
 			for ( int g = 1; g < kGroups; ++g )
 			{
@@ -639,15 +815,9 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 
 		// solve with bias
 		overflowJoints( 1 );
-		for ( int L = 0; L < levelCount; ++L )
-		{
-			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
-			{
-				int2 lb = C.bodies( k );
-				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, true, soft );
-			}
-			X.sync();
-		}
+		runLevels( levels, levelCount, kSplitOverflowSolve, [&]( int k, int la, int lb, int pc ) {
+			ovf_solve( C, k, la, lb, pc, inv_h, pushout, true, soft );
+		} );
 		for ( int g = 1; g < kGroups; ++g )
 		{
 			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
@@ -682,15 +852,9 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 
 		// relax
 		overflowJoints( 2 );
-		for ( int L = 0; L < levelCount; ++L )
-		{
-			for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
-			{
-				int2 lb = C.bodies( k );
-				ovf_solve( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, inv_h, pushout, false, soft );
-			}
-			X.sync();
-		}
+		runLevels( levels, levelCount, kSplitOverflowRelax, [&]( int k, int la, int lb, int pc ) {
+			ovf_solve( C, k, la, lb, pc, inv_h, pushout, false, soft );
+		} );
 		for ( int g = 1; g < kGroups; ++g )
 		{
 			int b = wi.groupStart[g], e = wi.groupStart[g + 1];
@@ -722,15 +886,9 @@ __device__ void solve_item( const S& C, const Sync& X, const DevWorld& W, const 
 
 	// restitution
 	const float threshold = W.restitutionThreshold;
-	for ( int L = 0; L < levelCount; ++L )
-	{
-		for ( int k = levels[L] + tid; k < levels[L + 1]; k += nt )
-		{
-			int2 lb = C.bodies( k );
-			ovf_restitution( C, k, lb.x, lb.y, W.contacts[sc[k]].manifold.pointCount, threshold );
-		}
-		X.sync();
-	}
+	runLevels( levels, levelCount, kSplitOverflowRestitution, [&]( int k, int la, int lb, int pc ) {
+		ovf_restitution( C, k, la, lb, pc, threshold );
+	} );
 	// The colour sweeps would return their input bit for bit when every colour slot's stage is
 	// the identity and every body is plain (restitution_is_identity): then they are skipped.
 	bool identity = W.subSteps > 0;
@@ -825,7 +983,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 												   const int* __restrict__ bodyList, const int* __restrict__ slotContact,
 												   const int2* __restrict__ slotBodies, const int* __restrict__ ovfLevels, float* cfBase,
 												   float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
-												   int itemOffset, const int* abortFlag )
+												   int itemOffset, int levelRun, const int* abortFlag )
 {
 	if ( abortFlag != nullptr && *abortFlag != 0 )
 		return;
@@ -849,7 +1007,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 			C.rec[k * cfQuads] = make_float4( __builtin_bit_cast( float, b.x ), __builtin_bit_cast( float, b.y ), 0.0f, 0.0f );
 		}
 		// the staging loop inside solve_item ends with a barrier before any C.bodies() read
-		solve_item( C, BlockSync{}, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
+		solve_item( C, BlockSync{ levelRun }, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
 					ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else if constexpr ( Store == kStoreBodiesLds )
@@ -866,7 +1024,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = reinterpret_cast<int2*>( C.M + wi.bodyCount + 1 );
 		for ( int k = threadIdx.x; k < total; k += blockDim.x )
 			C.sb[k] = slotBodies[wi.slotStart + k];
-		solve_item( C, BlockSync{}, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
+		solve_item( C, BlockSync{ levelRun }, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
 					ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 	}
 	else
@@ -877,7 +1035,7 @@ __global__ void __launch_bounds__( kSolveMaxBlock ) k_solve( const DevWorld* __r
 		C.sb = slotBodies + wi.slotStart;
 		C.P = priv + 2 * (size_t)wi.bodyStart;
 		C.M = mass + wi.bodyStart;
-		solve_item( C, BlockSync{}, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
+		solve_item( C, BlockSync{ levelRun }, W, wi, bl, sc, slotBodies + wi.slotStart, ovfLevels + wi.ovfLevelBegin,
 					ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
 	}
 }
@@ -894,7 +1052,7 @@ __global__ void __launch_bounds__( kGridSolveBlock ) k_solve_grid( const DevWorl
 																 const int* __restrict__ slotContact, const int2* __restrict__ slotBodies,
 																 const int* __restrict__ ovfLevels, float* cfBase, float4* priv, float2* mass,
 																 EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error,
-																 const int* abortFlag )
+																 int levelRun, const int* abortFlag )
 {
 	// (the flag was written before the launch began, so every workgroup reads the same value)
 	if ( abortFlag != nullptr && *abortFlag != 0 )
@@ -914,6 +1072,7 @@ __global__ void __launch_bounds__( kGridSolveBlock ) k_solve_grid( const DevWorl
 	X.tid_ = blockIdx.x * blockDim.x + threadIdx.x;
 	X.nt_ = gridDim.x * blockDim.x;
 	X.blocks = gridDim.x;
+	X.levelRun = levelRun;
 	X.target = 0;
 	solve_item( C, X, W, wi, bodyList + wi.bodyStart, slotContact + wi.slotStart, slotBodies + wi.slotStart,
 				ovfLevels + wi.ovfLevelBegin, ovfLevels + wi.jointLevelBegin, enl, enlCount, enlCapacity );
@@ -925,32 +1084,14 @@ __global__ void k_grid_reset( unsigned* sync )
 		sync[threadIdx.x] = 0u;
 }
 
-// One phase of one very large work item (HBM store), spread over a grid: thread i takes body or
-// slot `first + i` of the phase's range. The host launches the phases in solve_item's order, one
-// kernel each, so every workgroup barrier of solve_item becomes a kernel boundary; each phase
-// applies the same per-body and per-slot functions to the same data, so the results are the
-// same bits (a colour's slots share no body, so their order does not matter; overflow levels
-// as in solve_item).
-__global__ void __launch_bounds__( 256 ) k_split_phase( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
-														int itemIndex, int phase, int first, int count,
-														const int* __restrict__ bodyList, const int* __restrict__ slotContact,
-														const int2* __restrict__ slotBodies, float* cfBase, float4* priv, float2* mass,
-														EnlargeRecord* enl, int* enlCount, int enlCapacity )
+// One slot or body of one phase of a very large work item (HBM store): `k` is the body or slot
+// index within the item. The host launches the phases in solve_item's order, so every workgroup
+// barrier of solve_item becomes a kernel boundary; each phase applies the same per-body and
+// per-slot functions to the same data, so the results are the same bits (a colour's slots share
+// no body, so their order does not matter; overflow levels as in solve_item).
+__device__ void split_work( int phase, int k, const WorkItem& wi, const DevWorld& W, const GlobalStore& C, const int* __restrict__ bl,
+							const int* __restrict__ sc, EnlargeRecord* enl, int* enlCount, int enlCapacity )
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( i >= count )
-		return;
-	const WorkItem& wi = items[itemIndex];
-	const DevWorld& W = worlds[wi.world];
-	GlobalStore C;
-	C.rec = reinterpret_cast<float4*>( cfBase );
-	C.base = wi.slotStart;
-	C.sb = slotBodies + wi.slotStart;
-	C.P = priv + 2 * (size_t)wi.bodyStart;
-	C.M = mass + wi.bodyStart;
-	const int* bl = bodyList + wi.bodyStart;
-	const int* sc = slotContact + wi.slotStart;
-	const int k = first + i;
 	const Softness soft = world_softness( W );
 	const float minBiasVel = -W.pushout;
 	switch ( phase )
@@ -1068,6 +1209,53 @@ __global__ void __launch_bounds__( 256 ) k_split_phase( const DevWorld* __restri
 	}
 }
 
+__device__ inline GlobalStore split_store( const WorkItem& wi, const int2* slotBodies, float* cfBase, float4* priv, float2* mass )
+{
+	GlobalStore C;
+	C.rec = reinterpret_cast<float4*>( cfBase );
+	C.base = wi.slotStart;
+	C.sb = slotBodies + wi.slotStart;
+	C.P = priv + 2 * (size_t)wi.bodyStart;
+	C.M = mass + wi.bodyStart;
+	return C;
+}
+
+// One phase of one very large work item spread over a grid: thread i < jointCount takes joint
+// slot `jointFirst + i` of jointPhase, the next `count` threads slot or body `first + i` of phase
+// (a colour's joints and contacts share no body, so they run side by side).
+__global__ void __launch_bounds__( 256 ) k_split_phase( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+														int itemIndex, int phase, int first, int count, int jointPhase, int jointFirst,
+														int jointCount, const int* __restrict__ bodyList, const int* __restrict__ slotContact,
+														const int2* __restrict__ slotBodies, float* cfBase, float4* priv, float2* mass,
+														EnlargeRecord* enl, int* enlCount, int enlCapacity )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= jointCount + count )
+		return;
+	const WorkItem& wi = items[itemIndex];
+	const DevWorld& W = worlds[wi.world];
+	const GlobalStore C = split_store( wi, slotBodies, cfBase, priv, mass );
+	const int* bl = bodyList + wi.bodyStart;
+	const int* sc = slotContact + wi.slotStart;
+	if ( i < jointCount )
+		split_work( jointPhase, jointFirst + i, wi, W, C, bl, sc, enl, enlCount, enlCapacity );
+	else
+		split_work( phase, first + i - jointCount, wi, W, C, bl, sc, enl, enlCount, enlCapacity );
+}
+
+// One cached run of overflow levels of one very large work item in one workgroup: levels
+// [levelBegin, levelBegin + levelCount) of `levels` (slot bounds within the item, at most
+// kLevelCacheSlots slots in all), an overflow contact or joint level phase (cached_level_run).
+__global__ void __launch_bounds__( kSplitLevelBlock ) k_split_levels( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
+																	 int itemIndex, int phase, const int* __restrict__ levels, int levelBegin,
+																	 int levelCount, const int* __restrict__ slotContact,
+																	 const int2* __restrict__ slotBodies, float* cfBase, float4* priv, float2* mass )
+{
+	const WorkItem& wi = items[itemIndex];
+	cached_level_run( split_store( wi, slotBodies, cfBase, priv, mass ), worlds[wi.world], slotContact + wi.slotStart, levels + levelBegin, 0,
+					  levelCount, phase );
+}
+
 } // namespace b2mi
 
 // ---- host launchers ----
@@ -1096,11 +1284,15 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize,
 						 int ldsBytes, const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels,
 						 float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
-						 const int* abortFlag, hipStream_t stream )
+						 int levelRun, const int* abortFlag, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
 	if ( ldsBytes < 0 || ldsBytes > kMaxItemLdsBytes || blockSize < 64 || blockSize > kSolveMaxBlock )
+		return hipErrorInvalidValue;
+	// HBM-store items use no dynamic LDS (the kernel's static LDS, the overflow run cache, would
+	// not leave room for the largest dynamic block)
+	if ( store == kStoreHbm && ldsBytes != 0 )
 		return hipErrorInvalidValue;
 	const void* kernels[3] = { reinterpret_cast<const void*>( &k_solve<kStoreHbm> ),
 							   reinterpret_cast<const void*>( &k_solve<kStoreBodiesLds> ),
@@ -1117,7 +1309,7 @@ hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* item
 	}
 #define B2MI_LAUNCH_SOLVE( S )                                                                                                  \
 	hipLaunchKernelGGL( k_solve<S>, dim3( itemCount ), dim3( blockSize ), ldsBytes, stream, worlds, items, bodyList, slotContact, \
-						slotBodies, ovfLevels, cf, priv, mass, enl, enlCount, enlCapacity, itemOffset, abortFlag )
+						slotBodies, ovfLevels, cf, priv, mass, enl, enlCount, enlCapacity, itemOffset, levelRun, abortFlag )
 	if ( store == kStoreLds )
 		B2MI_LAUNCH_SOLVE( kStoreLds );
 	else if ( store == kStoreBodiesLds )
@@ -1148,8 +1340,8 @@ int solve_grid_capacity()
 
 hipError_t launch_solve_grid( const DevWorld* worlds, const WorkItem* items, int itemIndex, int blocks, const int* bodyList,
 							  const int* slotContact, const int2* slotBodies, const int* ovfLevels, float* cf, float4* priv, float2* mass,
-							  EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error, const int* abortFlag,
-							  hipStream_t stream )
+							  EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error, int levelRun,
+							  const int* abortFlag, hipStream_t stream )
 {
 	if ( blocks < 1 || blocks > solve_grid_capacity() )
 		return hipErrorInvalidValue;
@@ -1158,21 +1350,42 @@ hipError_t launch_solve_grid( const DevWorld* worlds, const WorkItem* items, int
 	if ( e != hipSuccess )
 		return e;
 	void* args[] = { &worlds, &items, &itemIndex, &bodyList, &slotContact, &slotBodies, &ovfLevels, &cf,
-					 &priv, &mass, &enl, &enlCount, &enlCapacity, &sync, &error, &abortFlag };
+					 &priv, &mass, &enl, &enlCount, &enlCapacity, &sync, &error, &levelRun, &abortFlag };
 	return hipLaunchCooperativeKernel( k_solve_grid, dim3( blocks ), dim3( kGridSolveBlock ), args, 0, stream );
 }
 
 hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, int first, int count,
-							   const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, float4* priv,
-							   float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream )
+							   int jointPhase, int jointFirst, int jointCount, const int* bodyList, const int* slotContact,
+							   const int2* slotBodies, float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount,
+							   int enlCapacity, hipStream_t stream )
 {
-	if ( count <= 0 )
+	count = count > 0 ? count : 0;
+	jointCount = jointCount > 0 ? jointCount : 0;
+	if ( count + jointCount == 0 )
 		return hipSuccess;
-	if ( phase < 0 || phase >= kSplitPhaseCount || first < 0 )
+	if ( ( count > 0 && ( phase < 0 || phase >= kSplitPhaseCount || first < 0 ) ) ||
+		 ( jointCount > 0 && ( jointPhase < kSplitJointPrepare || jointPhase > kSplitJointRelax || jointFirst < 0 ) ) )
 		return hipErrorInvalidValue;
-	int block = 256, grid = ( count + block - 1 ) / block;
-	hipLaunchKernelGGL( k_split_phase, dim3( grid ), dim3( block ), 0, stream, worlds, items, itemIndex, phase, first, count, bodyList,
-						slotContact, slotBodies, cf, priv, mass, enl, enlCount, enlCapacity );
+	int block = 256, grid = ( count + jointCount + block - 1 ) / block;
+	hipLaunchKernelGGL( k_split_phase, dim3( grid ), dim3( block ), 0, stream, worlds, items, itemIndex, phase, first, count, jointPhase,
+						jointFirst, jointCount, bodyList, slotContact, slotBodies, cf, priv, mass, enl, enlCount, enlCapacity );
+	return hipGetLastError();
+}
+
+hipError_t launch_split_levels( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, const int* levels, int levelBegin,
+								int levelCount, const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf,
+								float4* priv, float2* mass, hipStream_t stream )
+{
+	if ( levelCount <= 0 )
+		return hipSuccess;
+	// the level phases only (they touch no enlarge record)
+	const bool level = phase == kSplitOverflowWarmStart || phase == kSplitOverflowSolve || phase == kSplitOverflowRelax ||
+					   phase == kSplitOverflowRestitution || phase == kSplitJointWarmStart || phase == kSplitJointSolve ||
+					   phase == kSplitJointRelax;
+	if ( !level || levelBegin < 0 )
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL( k_split_levels, dim3( 1 ), dim3( kSplitLevelBlock ), 0, stream, worlds, items, itemIndex, phase, levels, levelBegin,
+						levelCount, slotContact, slotBodies, cf, priv, mass );
 	return hipGetLastError();
 }
 

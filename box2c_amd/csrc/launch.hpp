@@ -15,22 +15,32 @@ hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, 
 hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize,
 						 int ldsBytes, const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels,
 						 float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity,
-						 const int* abortFlag, hipStream_t stream );
+						 int levelRun, const int* abortFlag, hipStream_t stream );
 
 // one very large HBM-store work item over a cooperative grid of `blocks` workgroups of
 // kGridSolveBlock threads (k_solve_grid); sync holds two words it uses, error is set if a barrier
 // wait exceeded its bound. solve_grid_capacity: the most workgroups one such launch may have (all
-// resident at once), 0 when the device has no cooperative launches.
+// resident at once), 0 when the device has no cooperative launches. levelRun: consecutive overflow
+// levels of at most this many slots in all run as one cached run in workgroup 0 (0 = none).
 int solve_grid_capacity();
 hipError_t launch_solve_grid( const DevWorld* worlds, const WorkItem* items, int itemIndex, int blocks, const int* bodyList,
 							  const int* slotContact, const int2* slotBodies, const int* ovfLevels, float* cf, float4* priv, float2* mass,
-							  EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error, const int* abortFlag,
-							  hipStream_t stream );
+							  EnlargeRecord* enl, int* enlCount, int enlCapacity, unsigned* sync, int* error, int levelRun,
+							  const int* abortFlag, hipStream_t stream );
 
-// one phase of a very large work item over a grid (k_split_phase; phases in SplitPhase)
+// one phase of a very large work item over a grid (k_split_phase; phases in SplitPhase): `count`
+// slots or bodies from `first` of phase, beside `jointCount` joint slots from jointFirst of
+// jointPhase (a joint phase, or ignored when jointCount is 0)
 hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, int first, int count,
-							   const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf, float4* priv,
-							   float2* mass, EnlargeRecord* enl, int* enlCount, int enlCapacity, hipStream_t stream );
+							   int jointPhase, int jointFirst, int jointCount, const int* bodyList, const int* slotContact,
+							   const int2* slotBodies, float* cf, float4* priv, float2* mass, EnlargeRecord* enl, int* enlCount,
+							   int enlCapacity, hipStream_t stream );
+// overflow levels [levelBegin, levelBegin + levelCount) of one very large work item, at most
+// kLevelCacheSlots slots in all, as one cached run in one workgroup (k_split_levels), for an
+// overflow or joint level phase; levels is the device level list
+hipError_t launch_split_levels( const DevWorld* worlds, const WorkItem* items, int itemIndex, int phase, const int* levels, int levelBegin,
+								int levelCount, const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf,
+								float4* priv, float2* mass, hipStream_t stream );
 
 // broadphase pair queries (broadphase.hip)
 hipError_t launch_pair_erase( unsigned long long* table, uint32_t mask, const unsigned long long* keys, int count, hipStream_t s );

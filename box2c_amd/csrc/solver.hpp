@@ -235,6 +235,68 @@ struct LdsStore
 	}
 };
 
+// A run of overflow levels of an HBM-store item staged in LDS (cached_level_run): the distinct
+// bodies its slots touch (state, masses, item-local index), each slot's two bodies as cache
+// indices (a static body stays negative), and for contacts each slot's record and point count;
+// an open-addressing table maps item-local body indices to cache indices while it is built.
+constexpr int kRunCacheBodies = 2 * kLevelCacheSlots;
+constexpr int kRunCacheHash = 4 * kLevelCacheSlots;
+struct RunCache
+{
+	float4 P[2 * kRunCacheBodies];
+	float4 rec[kLevelCacheSlots * cfQuads];
+	float2 M[kRunCacheBodies];
+	int key[kRunCacheBodies];
+	int2 sb[kLevelCacheSlots];
+	int pc[kLevelCacheSlots];
+	int hashKey[kRunCacheHash];
+	int hashVal[kRunCacheHash];
+	int count;
+};
+
+// The store a cached run's levels use: slot k (of the run, first slot `base`) and cache body i.
+struct RunStore
+{
+	RunCache* c;
+	int base;
+	static constexpr bool kIdentityBody = false;
+	static constexpr bool kPrefetch = false;
+	__device__ inline float& at( int field, int k ) const
+	{
+		return reinterpret_cast<float*>( c->rec + ( k - base ) * cfQuads )[field];
+	}
+	__device__ inline float4 quad( int k, int q ) const
+	{
+		return c->rec[( k - base ) * cfQuads + q];
+	}
+	__device__ inline void setQuad( int k, int q, float4 v ) const
+	{
+		c->rec[( k - base ) * cfQuads + q] = v;
+	}
+	__device__ inline void loadQuads( int k, float4 ( &r )[cfMutableQuad] ) const
+	{
+#pragma unroll
+		for ( int q = 0; q < cfMutableQuad; ++q )
+			r[q] = quad( k, q );
+	}
+	__device__ inline int2 bodies( int k ) const
+	{
+		return c->sb[k - base];
+	}
+	__device__ inline float4& b0( int i ) const
+	{
+		return c->P[2 * i];
+	}
+	__device__ inline float4& b1( int i ) const
+	{
+		return c->P[2 * i + 1];
+	}
+	__device__ inline float2& mass( int i ) const
+	{
+		return c->M[i];
+	}
+};
+
 // the slot bodies a record's q0 carries (int bits in x and y)
 __device__ inline int2 quad_bodies( float4 q0 )
 {

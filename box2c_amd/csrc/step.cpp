@@ -1715,7 +1715,7 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 				return a.block > b.block;
 			if ( a.store != b.store )
 				return a.store < b.store;
-			int ca = a.groupStart[kGroups] + a.bodyCount, cb = b.groupStart[kGroups] + b.bodyCount;
+			int ca = item_slots( a ) + a.bodyCount, cb = item_slots( b ) + b.bodyCount;
 			return ca > cb;
 		} );
 		g->classes.clear();
@@ -1732,7 +1732,7 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		for ( int i = 0; i < (int)g->hItems.size(); ++i )
 		{
 			const WorkItem& wi = g->hItems[i];
-			if ( maySplit && wi.store == kStoreHbm && wi.groupStart[kGroups] >= splitSlots )
+			if ( maySplit && wi.store == kStoreHbm && item_slots( wi ) >= splitSlots )
 			{
 				g->splitItems.push_back( i );
 				continue;
@@ -1957,6 +1957,19 @@ static void checkIslandSleep( World* w )
 			const Contact& ct = w->contacts[id];
 			const Body& a = w->bodies[ct.edges[0].bodyId];
 			const Body& b = w->bodies[ct.edges[1].bodyId];
+			if ( a.setIndex != kAwakeSet || b.setIndex != kAwakeSet )
+				continue;
+			int ra = findRoot( parent, a.localIndex ), rb = findRoot( parent, b.localIndex );
+			if ( ra != rb )
+				parent[std::max( ra, rb )] = std::min( ra, rb );
+		}
+	// joints link islands too (src/island.c:298-380)
+	for ( int c = 0; c < kColorCount; ++c )
+		for ( int id : w->jointColors[c] )
+		{
+			const Joint& jt = w->joints[id];
+			const Body& a = w->bodies[jt.edges[0].bodyId];
+			const Body& b = w->bodies[jt.edges[1].bodyId];
 			if ( a.setIndex != kAwakeSet || b.setIndex != kAwakeSet )
 				continue;
 			int ra = findRoot( parent, a.localIndex ), rb = findRoot( parent, b.localIndex );
@@ -2218,9 +2231,11 @@ hipError_t StepGroup::enqueueSolve( hipStream_t s, const hipStream_t* aux, const
 	}
 	auto launch = [&]( const LaunchClass& c, hipStream_t cs ) {
 		int block = blockSize > 0 ? blockSize : c.block;
-		int lds = blockSize > 0 ? maxItemLds : c.lds;
+		// (HBM-store items keep nothing in dynamic LDS; their kernel's static LDS holds the
+		// overflow run cache)
+		int lds = c.store == kStoreHbm ? 0 : blockSize > 0 ? maxItemLds : c.lds;
 		return launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies, dOvfLevels,
-							 dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, cs );
+							 dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, levelRunSlots, abortFlag, cs );
 	};
 	B2MI_CHECK( splitItems.empty() || abortFlag == nullptr || !splitPerPhase );
 	for ( int item : splitItems )
@@ -2230,7 +2245,7 @@ hipError_t StepGroup::enqueueSolve( hipStream_t s, const hipStream_t* aux, const
 		else
 			B2MI_TRY( launch_solve_grid( dWorlds, dItems, item, gridBlocksFor( hItems[item] ), dBodyList, dSlotContact, dSlotBodies,
 										 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, dGridSync, dCounters + 3,
-										 abortFlag, s ) );
+										 levelRunSlots, abortFlag, s ) );
 	}
 	if ( classes.size() <= 1 )
 	{
@@ -2279,6 +2294,7 @@ SolveSignature StepGroup::solveSignature( const int* abortFlag ) const
 	k.blockSize = blockSize;
 	k.maxItemLds = maxItemLds;
 	k.gridBlocks = gridBlocks;
+	k.levelRun = levelRunSlots;
 	k.splitPerPhase = splitPerPhase ? 1 : 0;
 	k.chunks = chunkCount;
 	return k;
@@ -2368,52 +2384,73 @@ bool StepGroup::captureSolve( hipStream_t, const int* abortFlag )
 }
 
 // Workgroups of a very large item's cooperative launch: one thread per slot of its largest colour
-// group (or the set count), within what can be resident at once.
+// group, contacts and joints (or the set count), within what can be resident at once.
 int StepGroup::gridBlocksFor( const WorkItem& wi ) const
 {
 	int largest = 1;
 	for ( int g = 1; g < kGroups; ++g )
-		largest = std::max( largest, wi.groupStart[g + 1] - wi.groupStart[g] );
+		largest = std::max( largest, wi.groupStart[g + 1] - wi.groupStart[g] + wi.jointGroupStart[g + 1] - wi.jointGroupStart[g] );
 	int blocks = gridBlocks > 0 ? gridBlocks : ( largest + kGridSolveBlock - 1 ) / kGridSolveBlock;
 	return std::max( 1, std::min( blocks, solve_grid_capacity() ) );
 }
 
-// The phases of one very large work item (solve_item's order), each one grid launch on stream s.
+// The phases of one very large work item (solve_item's order) on stream s: each phase one grid
+// launch, a colour's joints and contacts in one launch, and each run of consecutive overflow
+// levels of at most levelRunSlots slots in all one single-workgroup launch with the run's bodies
+// and records in LDS (k_split_levels) rather than a launch per level.
 hipError_t StepGroup::enqueueSplit( int item, hipStream_t s )
 {
 	hipError_t status = hipSuccess;
 	const WorkItem& wi = hItems[item];
 	const DevWorld& hw = hWorlds[wi.world];
-	auto run = [&]( int phase, int first, int count ) {
+	auto run = [&]( int phase, int first, int count, int jointPhase = -1, int jointFirst = 0, int jointCount = 0 ) {
 		if ( status == hipSuccess )
-			status = launch_split_phase( dWorlds, dItems, item, phase, first, count, dBodyList, dSlotContact, dSlotBodies, dCons, dPriv,
-										 dMass, dEnlarge, dCounters + 1, capEnlarge, s );
+			status = launch_split_phase( dWorlds, dItems, item, phase, first, count, jointPhase, jointFirst, jointCount, dBodyList,
+										 dSlotContact, dSlotBodies, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, s );
 	};
 	const int total = wi.groupStart[kGroups];
 	const int levelCount = wi.groupStart[1] > wi.groupStart[0] ? wi.ovfLevelCount : 0;
-	const int* levels = hOvfLevels.data() + wi.ovfLevelBegin;
 	B2MI_CHECK( levelCount == 0 || wi.ovfLevelBegin + levelCount < (int)hOvfLevels.size() );
-	// the overflow joints in levels before the overflow contacts, the joints of a colour in a
-	// launch before the colour's contacts (they share no body with them)
+	// the overflow joints in levels before the overflow contacts, the joints of a colour beside
+	// the colour's contacts (they share no body with them)
 	const int jointLevelCount = wi.jointGroupStart[1] > wi.jointGroupStart[0] ? wi.jointLevelCount : 0;
-	const int* jointLevels = hOvfLevels.data() + wi.jointLevelBegin;
 	B2MI_CHECK( jointLevelCount == 0 || wi.jointLevelBegin + jointLevelCount < (int)hOvfLevels.size() );
+	// levels [begin, begin + count) of the level list: consecutive levels of at most
+	// levelRunSlots slots in all as one cached run in one workgroup, a larger level over the grid
+	const int runSlots = std::min( levelRunSlots, kLevelCacheSlots );
+	auto levelsOf = [&]( int phase, int begin, int count ) {
+		const int* lv = hOvfLevels.data() + begin;
+		int L = 0;
+		while ( L < count && status == hipSuccess )
+		{
+			if ( runSlots <= 0 || lv[L + 1] - lv[L] > runSlots )
+			{
+				run( phase, lv[L], lv[L + 1] - lv[L] );
+				++L;
+				continue;
+			}
+			int E = L + 1;
+			while ( E < count && lv[E + 1] - lv[L] <= runSlots )
+				++E;
+			status = launch_split_levels( dWorlds, dItems, item, phase, dOvfLevels, begin + L, E - L, dBodyList, dSlotContact, dSlotBodies,
+										  dCons, dPriv, dMass, s );
+			L = E;
+		}
+	};
 	auto overflow = [&]( int jointPhase, int phase ) {
-		for ( int L = 0; L < jointLevelCount && jointPhase >= 0; ++L )
-			run( jointPhase, jointLevels[L], jointLevels[L + 1] - jointLevels[L] );
-		for ( int L = 0; L < levelCount; ++L )
-			run( phase, levels[L], levels[L + 1] - levels[L] );
+		if ( jointPhase >= 0 )
+			levelsOf( jointPhase, wi.jointLevelBegin, jointLevelCount );
+		levelsOf( phase, wi.ovfLevelBegin, levelCount );
 	};
 	auto colours = [&]( int jointPhase, int phase ) {
 		for ( int g = 1; g < kGroups; ++g )
 		{
-			if ( jointPhase >= 0 )
-				run( jointPhase, wi.jointGroupStart[g], wi.jointGroupStart[g + 1] - wi.jointGroupStart[g] );
-			run( phase, wi.groupStart[g], wi.groupStart[g + 1] - wi.groupStart[g] );
+			const int joints = jointPhase >= 0 ? wi.jointGroupStart[g + 1] - wi.jointGroupStart[g] : 0;
+			run( phase, wi.groupStart[g], wi.groupStart[g + 1] - wi.groupStart[g], jointPhase, wi.jointGroupStart[g], joints );
 		}
 	};
 	run( kSplitStage, 0, wi.bodyCount );
-	run( kSplitJointPrepare, wi.jointGroupStart[0], wi.jointGroupStart[kGroups] - wi.jointGroupStart[0] );
+	run( -1, 0, 0, kSplitJointPrepare, wi.jointGroupStart[0], wi.jointGroupStart[kGroups] - wi.jointGroupStart[0] );
 	run( kSplitPrepare, 0, total );
 	for ( int sub = 0; sub < hw.subSteps; ++sub )
 	{
@@ -2446,7 +2483,9 @@ void StepGroup::launchSolveChunk( int chunk, hipEvent_t ready, const int* abortF
 		if ( c.chunk != chunk )
 			continue;
 		int block = blockSize > 0 ? blockSize : c.block;
-		int lds = blockSize > 0 ? maxItemLds : c.lds;
+		// (HBM-store items keep nothing in dynamic LDS; their kernel's static LDS holds the
+		// overflow run cache)
+		int lds = c.store == kStoreHbm ? 0 : blockSize > 0 ? maxItemLds : c.lds;
 		B2MI_CHECK( block >= 64 && block <= kSolveMaxBlock && block % 64 == 0 );
 		int k = j++ % kAuxStreams;
 		hipStream_t as = auxStream( k );
@@ -2454,7 +2493,7 @@ void StepGroup::launchSolveChunk( int chunk, hipEvent_t ready, const int* abortF
 			HIP_CHECK( hipStreamWaitEvent( as, ready, 0 ) );
 		used[k] = true;
 		HIP_CHECK( launch_solve( c.store, dWorlds, dItems, c.first, c.count, block, lds, dBodyList, dSlotContact, dSlotBodies,
-								 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, abortFlag, as ) );
+								 dOvfLevels, dCons, dPriv, dMass, dEnlarge, dCounters + 1, capEnlarge, levelRunSlots, abortFlag, as ) );
 	}
 	for ( int k = 0; k < kAuxStreams; ++k )
 		if ( used[k] )

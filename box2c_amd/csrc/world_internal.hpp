@@ -500,7 +500,7 @@ struct SolveSignature
 	const int* abortFlag;
 	const void *worlds, *items, *bodyList, *slotContact, *slotBodies, *ovfLevels, *cons, *priv, *mass, *enlarge, *counters,
 		*gridSync;
-	int enlargeCapacity, blockSize, maxItemLds, gridBlocks, splitPerPhase, chunks;
+	int enlargeCapacity, blockSize, maxItemLds, gridBlocks, levelRun, splitPerPhase, chunks;
 };
 
 // a set of worlds advanced together by one narrowphase launch and one solver launch
@@ -514,6 +514,7 @@ struct StepGroup
 	int ldsFullBudget = -1;	  // max LDS bytes of an item keeping all its state in LDS (-1 = ldsBudget)
 	int splitSlots = -1;	  // HBM-store items of at least this many slots run over the grid (-1 = kDefaultSplitSlots, 0 = never)
 	int gridBlocks = -1;	  // workgroups of such an item's cooperative launch (0 = from its largest colour, -1 = one launch per phase)
+	int levelRunSlots = kLevelCacheSlots; // HBM-store items: consecutive overflow levels of at most this many slots in all run as one cached run
 	unsigned* dGridSync = nullptr; // barrier arrivals and vote of the cooperative grid solver
 	int builtTarget = -1, builtBudget = -1, builtFullBudget = -1;				   // packing target and LDS budget of the current items
 	bool structureDirty = true;

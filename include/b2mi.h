@@ -108,6 +108,15 @@ B2_API void b2mi_World_SetSplitSlots( b2WorldId worldId, int slots );
 B2_API void b2mi_Batch_SetGridBlocks( int batch, int blocks );
 B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks );
 
+// Overflow levels (the serial chain of the constraints no colour could take, split into levels
+// whose constraints share no body) of an item kept in HBM: consecutive levels of at most `slots`
+// slots in all run as one run in one workgroup (in a grid-wide item, between two grid barriers or
+// in one launch) with the bodies they touch and their records copied into LDS; a larger level
+// runs on its own over all the item's threads. -1 = the default and largest value (128), 0 = no
+// runs. Results are the same bits for every value.
+B2_API void b2mi_Batch_SetLevelRunSlots( int batch, int slots );
+B2_API void b2mi_World_SetLevelRunSlots( b2WorldId worldId, int slots );
+
 // Broadphase pair queries of a world stepped alone run on the device (one thread per moved
 // proxy over uploaded copies of the trees, the contacts then created on the host in the
 // reference's order) when at least `minMoves` proxies moved; -1 = always on the host, 0 = always

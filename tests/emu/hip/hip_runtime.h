@@ -149,6 +149,13 @@ inline int atomicExch( int* p, int v )
 	*p = v;
 	return old;
 }
+inline int atomicCAS( int* p, int compare, int v )
+{
+	int old = *p;
+	if ( old == compare )
+		*p = v;
+	return old;
+}
 inline unsigned long long atomicCAS( unsigned long long* p, unsigned long long compare, unsigned long long v )
 {
 	unsigned long long old = *p;

@@ -153,6 +153,35 @@ def test_step_parity_split_items(name, grid):
         cpu.destroy(wc)
 
 
+@pytest.mark.parametrize("name", ["overflow", "overflow_negzero", "tumbler"])
+@pytest.mark.parametrize("grid", [-1, 0], ids=["per_phase", "grid_auto"])
+@pytest.mark.parametrize("run", [0, 1, 4], ids=["levels_over_grid", "run1", "run4"])
+def test_step_parity_overflow_level_runs(name, grid, run):
+    """The overflow levels of an item run over the grid: each level over the whole grid (a run
+    limit of 0), or the levels of at most one or four slots in one workgroup between grid barriers
+    (or in one launch) and the larger ones over the grid; every split gives the oracle's bits."""
+    scene = {s[0]: s for s in helpers.PARITY_SCENES}[name]
+    _, kw, overrides, steps, substeps = scene
+    gpu, cpu = helpers.product(), helpers.oracle()
+    wg, _ = helpers.make_world(gpu, name, kw, overrides)
+    wc, _ = helpers.make_world(cpu, name, kw, overrides)
+    gpu.lib.b2mi_World_SetLdsBudget(wg, 0)
+    gpu.lib.b2mi_World_SetSplitSlots(wg, 1)
+    gpu.lib.b2mi_World_SetGridBlocks(wg, grid)
+    gpu.lib.b2mi_World_SetLevelRunSlots(wg, run)
+    try:
+        for s in range(min(steps, 30)):
+            gpu.step(wg, substeps=substeps)
+            cpu.step(wc, substeps=substeps)
+            m = helpers.first_body_mismatch(gpu.dump_bodies(wg), cpu.dump_bodies(wc))
+            assert m is None, f"{name} step {s}: bodies differ: {m}"
+        m = helpers.first_contact_mismatch(gpu.dump_contacts(wg), cpu.dump_contacts(wc))
+        assert m is None, f"{name}: contacts differ: {m}"
+    finally:
+        gpu.destroy(wg)
+        cpu.destroy(wc)
+
+
 def _batch_worlds(gpu, cpu, n, seed):
     batch = gpu.create_batch()
     gpu.use_batch(batch)

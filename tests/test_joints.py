@@ -216,6 +216,60 @@ def test_joint_tumbler_island_modes(grid):
     o.destroy(wo)
 
 
+# every zoo item over the grid (a split threshold of one slot): the hub's overflow joints in level
+# runs of one workgroup (the default), or each level over the whole grid
+ZOO_MODES = [pytest.param(-1, -1, id="per_phase"), pytest.param(-1, 0, id="per_phase-levels_over_grid"),
+             pytest.param(0, -1, id="grid_auto"), pytest.param(3, 0, id="grid3-levels_over_grid")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid,run", ZOO_MODES)
+def test_joint_zoo_island_modes(grid, run):
+    p = helpers.product()
+    o = helpers.oracle()
+    kw, ov = {"spokes": 16, "types": ZOO_TYPES, "seed": 3}, {"enableContinous": False}
+    wp, bp = helpers.make_world(p, "joint_zoo", kw, ov)
+    wo, bo = helpers.make_world(o, "joint_zoo", kw, ov)
+    p.lib.b2mi_World_SetLdsBudget(wp, 0)
+    p.lib.b2mi_World_SetSplitSlots(wp, 1)
+    p.lib.b2mi_World_SetGridBlocks(wp, grid)
+    p.lib.b2mi_World_SetLevelRunSlots(wp, run)
+    jp, jo = p.joints_of(bp), o.joints_of(bo)
+    for s in range(60):
+        p.step(wp)
+        o.step(wo)
+        assert helpers.first_body_mismatch(p.dump_bodies(wp), o.dump_bodies(wo)) is None, s
+        assert np.array_equal(_joint_bits(p, jp), _joint_bits(o, jo)), s
+    items = p.work_items(world=wp)
+    modes = set(items[items[:, 0] > 0, 3].tolist())
+    assert modes == ({1} if grid < 0 else {2}), modes
+    p.destroy(wp)
+    o.destroy(wo)
+
+
+@pytest.mark.gpu
+def test_joint_net_runs_over_the_grid():
+    """A one-island net of 4,900 joints and no contacts, its state in HBM, counts its joint slots
+    toward the large-island threshold of a world stepped alone (4,096 slots), so it runs over the
+    grid, one launch per phase, and gives the oracle's bits."""
+    p = helpers.product()
+    o = helpers.oracle()
+    wp, bp = helpers.make_world(p, "joint_grid", {"N": 50, "gravity": -10.0}, {})
+    wo, bo = helpers.make_world(o, "joint_grid", {"N": 50, "gravity": -10.0}, {})
+    p.lib.b2mi_World_SetLdsBudget(wp, 0)
+    jp, jo = p.joints_of(bp), o.joints_of(bo)
+    assert len(jp) >= 4096
+    for s in range(12):
+        p.step(wp)
+        o.step(wo)
+        assert helpers.first_body_mismatch(p.dump_bodies(wp), o.dump_bodies(wo)) is None, s
+        assert np.array_equal(_joint_bits(p, jp), _joint_bits(o, jo)), s
+    items = p.work_items(world=wp)
+    assert items.shape[0] == 1 and items[0, 0] == len(jp) and items[0, 3] == 1, items
+    p.destroy(wp)
+    o.destroy(wo)
+
+
 @pytest.mark.gpu
 def test_joint_edits_between_steps():
     """Motor speed, limits and joint destruction between steps, on both implementations."""
