@@ -264,15 +264,17 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, au
         tb, _ = build_batch(api, n_worlds, seed)
         api.batch_step(tb, n=warmup)
         best = api.lib.b2mi_Batch_Autotune(tb, DT, SUBSTEPS, 2)
-        b, t, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        api.lib.b2mi_Batch_GetLaunchShape(tb, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l))
-        shape = (b.value, t.value, l.value)
-        tuned = {"block": b.value, "target_slots": t.value, "lds_budget": l.value, "tuned_solve_ms": best}
+        b, t, l, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        api.lib.b2mi_Batch_GetLaunchShape(tb, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l), ctypes.byref(f))
+        shape = (b.value, t.value, l.value, f.value)
+        tuned = {"block": b.value, "target_slots": t.value, "lds_budget": l.value, "lds_full_budget": f.value,
+                 "tuned_solve_ms": best}
         api.lib.b2mi_DestroyBatch(tb)
     batch, n_bodies = build_batch(api, n_worlds, seed)
     if shape is not None:
         api.lib.b2mi_Batch_Configure(batch, shape[0], shape[1])
         api.lib.b2mi_Batch_SetLdsBudget(batch, shape[2])
+        api.lib.b2mi_Batch_SetLdsFullBudget(batch, shape[3] if len(shape) > 3 else -1)
     api.batch_step(batch, n=warmup)
     t_solve = t_collide = t_host = 0.0
     per_step = []
@@ -292,7 +294,8 @@ def run_gpu_batch(api, n_worlds: int, steps: int, warmup: int, seed: int = 7, au
     return {"bodies": n_bodies, "elapsed": elapsed, "per_step": per_step, "ms_solve": t_solve / steps,
             "ms_collide": t_collide / steps, "ms_host": t_host / steps, "work_items": int(len(items)),
             "slots": int(items[:, 0].sum()) if len(items) else contacts,
-            "launch_shape": tuned or ({"block": shape[0], "target_slots": shape[1], "lds_budget": shape[2]} if shape else None)}
+            "launch_shape": tuned or ({"block": shape[0], "target_slots": shape[1], "lds_budget": shape[2],
+                                       "lds_full_budget": shape[3] if len(shape) > 3 else -1} if shape else None)}
 
 
 def cpu_baseline_batch(host: dict, n_worlds: int, warmup: int, steps: int, curve_steps: int, seed: int = 7):
@@ -428,7 +431,7 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--batch-cpu-steps", type=int, default=30, help="steps of the batched leg's CPU sample")
     ap.add_argument("--batch-no-autotune", action="store_true", help="batched leg at the default launch shape")
-    ap.add_argument("--batch-shape", default="", help="block,target,ldsBudget: skip the autotuner, use this shape")
+    ap.add_argument("--batch-shape", default="", help="block,target,ldsBudget[,ldsFullBudget]: skip the autotuner, use this shape")
     ap.add_argument("--adjoint-worlds", type=int, default=1024, help="worlds of the adjoint leg (0 = skip it)")
     ap.add_argument("--tumbler-shapes", type=int, default=40000, help="shapes of the tumbler leg (0 = skip it)")
     ap.add_argument("--tumbler-steps", type=int, default=600)

@@ -538,7 +538,7 @@ class B2Api:
         L.b2mi_Batch_Autotune.restype = c_float
         L.b2mi_World_Autotune.argtypes = [WorldId, c_float, c_int, c_int]
         L.b2mi_World_Autotune.restype = c_float
-        L.b2mi_Batch_GetLaunchShape.argtypes = [c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]
+        L.b2mi_Batch_GetLaunchShape.argtypes = [c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int)]
         L.b2mi_Batch_SetLdsBudget.argtypes = [c_int, c_int]
         L.b2mi_World_SetLdsBudget.argtypes = [WorldId, c_int]
         L.b2mi_Batch_SetLdsFullBudget.argtypes = [c_int, c_int]

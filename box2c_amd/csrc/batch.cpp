@@ -31,52 +31,64 @@ static StepGroup* worldGroup( b2WorldId worldId )
 	return w->group;
 }
 
-static void setShape( StepGroup* g, int blockSize, int targetSlots, int ldsBudget )
+static void setShape( StepGroup* g, int blockSize, int targetSlots, int ldsBudget, int ldsFullBudget )
 {
 	g->blockSize = blockSize;
 	g->targetSlots = targetSlots;
 	g->ldsBudget = ldsBudget;
+	g->ldsFullBudget = ldsFullBudget;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
 	g->allActive = true;
 }
 
 // Autotuner of the solver launch shape. Every shape gives bit-identical results, so the trials
-// are ordinary steps of the simulation: each candidate (threads per workgroup x work-item
-// packing target x LDS budget) runs one step to rebuild its work items and then `trialSteps`
-// measured steps; the candidate with the lowest solver-kernel time (device events) is kept.
+// are ordinary steps of the simulation: each candidate runs one step to rebuild its work items
+// and then `trialSteps` measured steps; the candidate with the lowest solver-kernel time (device
+// events) is kept. First every (threads per workgroup x work-item packing target x LDS budget)
+// with all of an item's state in LDS whenever it fits the budget; then, at the best of those,
+// smaller limits on the footprint of an item that keeps all its state in LDS (larger items keep
+// only bodies and impulses there), which let more items share a CU.
 static float autotune( StepGroup* g, float timeStep, int subStepCount, int trialSteps )
 {
 	static const int blocks[] = { 0, 128, 256, 512, 1024 };
 	static const int targets[] = { 128, 256, 512 };
 	static const int budgets[] = { 0, 64 * 1024, 96 * 1024, 160 * 1024 };
+	static const int fullBudgets[] = { 96 * 1024, 64 * 1024, 48 * 1024, 32 * 1024 };
 	float best = 1e30f;
-	int bestBlock = 0, bestTarget = 0, bestBudget = -1;
+	int bestBlock = 0, bestTarget = 0, bestBudget = -1, bestFull = -1;
 	trialSteps = trialSteps < 1 ? 1 : trialSteps;
+	auto trial = [&]( int block, int target, int budget, int full ) {
+		setShape( g, block, target, budget, full );
+		g->step( timeStep, subStepCount );
+		float t = 1e30f;
+		for ( int k = 0; k < trialSteps; ++k )
+		{
+			g->step( timeStep, subStepCount );
+			t = g->msSolve < t ? g->msSolve : t;
+		}
+		if ( t < best )
+		{
+			best = t;
+			bestBlock = block;
+			bestTarget = target;
+			bestBudget = budget;
+			bestFull = full;
+		}
+	};
 	for ( int budget : budgets )
 		for ( int target : targets )
 			for ( int block : blocks )
-			{
-				setShape( g, block, target, budget );
-				g->step( timeStep, subStepCount );
-				float t = 1e30f;
-				for ( int k = 0; k < trialSteps; ++k )
-				{
-					g->step( timeStep, subStepCount );
-					t = g->msSolve < t ? g->msSolve : t;
-				}
-				if ( t < best )
-				{
-					best = t;
-					bestBlock = block;
-					bestTarget = target;
-					bestBudget = budget;
-				}
-			}
-	setShape( g, bestBlock, bestTarget, bestBudget );
+				trial( block, target, budget, -1 );
+	const int budget = bestBudget;
+	for ( int full : fullBudgets )
+		if ( full < budget )
+			trial( bestBlock, bestTarget, budget, full );
+	setShape( g, bestBlock, bestTarget, bestBudget, bestFull );
 	g->tunedBlock = bestBlock;
 	g->tunedTarget = bestTarget;
 	g->tunedBudget = bestBudget;
+	g->tunedFullBudget = bestFull;
 	return best;
 }
 
@@ -216,7 +228,7 @@ B2_API int b2mi_Batch_GetWorkItemCount( int batch )
 B2_API void b2mi_World_Configure( b2WorldId worldId, int blockSize, int targetSlots )
 {
 	StepGroup* g = worldGroup( worldId );
-	setShape( g, blockSize, targetSlots, g->ldsBudget );
+	setShape( g, blockSize, targetSlots, g->ldsBudget, g->ldsFullBudget );
 }
 
 // HBM-store work items of at least `slots` constraint slots run as one grid launch per solver
@@ -318,7 +330,7 @@ B2_API void b2mi_World_GetLaunchGraphStats( b2WorldId worldId, int* captures, in
 B2_API void b2mi_World_SetLdsBudget( b2WorldId worldId, int maxBytes )
 {
 	StepGroup* g = worldGroup( worldId );
-	setShape( g, g->blockSize, g->targetSlots, maxBytes );
+	setShape( g, g->blockSize, g->targetSlots, maxBytes, g->ldsFullBudget );
 }
 
 B2_API float b2mi_World_Autotune( b2WorldId worldId, float timeStep, int subStepCount, int trialSteps )
@@ -404,12 +416,14 @@ B2_API void b2mi_Batch_GetHostTimeline( int batch, float* marks )
 		marks[i] = g->hostPhases[i];
 }
 
-B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget )
+B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget, int* ldsFullBudget )
 {
 	StepGroup* g = getBatch( batch );
 	*blockSize = g->blockSize;
 	*targetSlots = g->targetSlots;
 	*ldsBudget = g->ldsBudget;
+	if ( ldsFullBudget != nullptr )
+		*ldsFullBudget = g->ldsFullBudget;
 }
 
 B2_API void b2mi_World_GetTiming( b2WorldId worldId, float* msCollide, float* msSolve, float* msHost )

@@ -510,7 +510,7 @@ struct StepGroup
 	int blockSize = 0;		  // 0 = automatic
 	int targetSlots = 0;	  // work-item packing target (constraint slots), 0 = automatic
 	int ldsBudget = -1;		  // max LDS bytes per work item (-1 = kDefaultItemLdsBytes, 0 = HBM only)
-	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1; // last autotuner choice
+	int tunedBlock = 0, tunedTarget = 0, tunedBudget = -1, tunedFullBudget = -1; // last autotuner choice
 	int ldsFullBudget = -1;	  // max LDS bytes of an item keeping all its state in LDS (-1 = ldsBudget)
 	int splitSlots = -1;	  // HBM-store items of at least this many slots run over the grid (-1 = kDefaultSplitSlots, 0 = never)
 	int gridBlocks = -1;	  // workgroups of such an item's cooperative launch (0 = from its largest colour, -1 = one launch per phase)

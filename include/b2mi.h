@@ -138,12 +138,15 @@ B2_API void b2mi_World_GetLaunchGraphStats( b2WorldId worldId, int* captures, in
 
 // Autotuner: steps the batch (or world) under each candidate launch shape (threads per
 // workgroup, work-item packing target, LDS budget), trialSteps measured steps each after one
-// rebuild step, and keeps the shape with the fastest solver kernel. Every shape gives
-// bit-identical results, so these are ordinary simulation steps (5 x 3 x 4 candidates, i.e.
-// 60 x (1 + trialSteps) steps). Returns the best solver-kernel time in milliseconds.
+// rebuild step, then at the best shape under smaller limits (96, 64, 48, 32 KiB) on the
+// footprint of an item that keeps all its state in LDS (b2mi_Batch_SetLdsFullBudget), and keeps
+// the shape with the fastest solver kernel. Every shape gives bit-identical results, so these
+// are ordinary simulation steps (5 x 3 x 4 + 4 candidates, i.e. 64 x (1 + trialSteps) steps).
+// Returns the best solver-kernel time in milliseconds. GetLaunchShape reads the shape back
+// (ldsFullBudget may be null).
 B2_API float b2mi_Batch_Autotune( int batch, float timeStep, int subStepCount, int trialSteps );
 B2_API float b2mi_World_Autotune( b2WorldId worldId, float timeStep, int subStepCount, int trialSteps );
-B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget );
+B2_API void b2mi_Batch_GetLaunchShape( int batch, int* blockSize, int* targetSlots, int* ldsBudget, int* ldsFullBudget );
 
 // Host clock of the last step, in ms since it began, at ten points: pair update started,
 // collide lists started, narrowphase launched, narrowphase wait started, wait returned, solver
