@@ -105,6 +105,35 @@ class SegmentDistanceResult(Structure):
                 ("distanceSquared", c_float)]
 
 
+class DistanceProxy(Structure):
+    _fields_ = [("points", Vec2 * 8), ("count", c_int), ("radius", c_float)]
+
+
+class DistanceCache(Structure):
+    _fields_ = [("count", ctypes.c_uint16), ("indexA", ctypes.c_uint8 * 3), ("indexB", ctypes.c_uint8 * 3)]
+
+
+class DistanceInput(Structure):
+    _fields_ = [("proxyA", DistanceProxy), ("proxyB", DistanceProxy), ("transformA", Transform), ("transformB", Transform),
+                ("useRadii", ctypes.c_bool)]
+
+
+class DistanceOutput(Structure):
+    _fields_ = [("pointA", Vec2), ("pointB", Vec2), ("distance", c_float), ("iterations", c_int), ("simplexCount", c_int)]
+
+
+class Sweep(Structure):
+    _fields_ = [("localCenter", Vec2), ("c1", Vec2), ("c2", Vec2), ("q1", Rot), ("q2", Rot)]
+
+
+class TOIInput(Structure):
+    _fields_ = [("proxyA", DistanceProxy), ("proxyB", DistanceProxy), ("sweepA", Sweep), ("sweepB", Sweep), ("tMax", c_float)]
+
+
+class TOIOutput(Structure):
+    _fields_ = [("state", c_int), ("t", c_float)]
+
+
 class ManifoldPoint(Structure):
     _fields_ = [("point", Vec2), ("anchorA", Vec2), ("anchorB", Vec2), ("separation", c_float),
                 ("normalImpulse", c_float), ("tangentImpulse", c_float),
@@ -287,6 +316,14 @@ class B2Api:
             f.restype = AABB
         L.b2SegmentDistance.argtypes = [Vec2, Vec2, Vec2, Vec2]
         L.b2SegmentDistance.restype = SegmentDistanceResult
+        L.b2MakeProxy.argtypes = [POINTER(Vec2), c_int, c_float]
+        L.b2MakeProxy.restype = DistanceProxy
+        L.b2GetSweepTransform.argtypes = [POINTER(Sweep), c_float]
+        L.b2GetSweepTransform.restype = Transform
+        L.b2ShapeDistance.argtypes = [POINTER(DistanceCache), POINTER(DistanceInput), ctypes.c_void_p, c_int]
+        L.b2ShapeDistance.restype = DistanceOutput
+        L.b2TimeOfImpact.argtypes = [POINTER(TOIInput)]
+        L.b2TimeOfImpact.restype = TOIOutput
         for name, sa, sb in COLLIDE_FUNCTIONS:
             f = getattr(L, name)
             f.argtypes = [POINTER(sa), Transform, POINTER(sb), Transform]
@@ -367,6 +404,7 @@ class B2Api:
             S.scene_joint_tumbler.argtypes = [POINTER(WorldDef), c_int, POINTER(WorldId), c_void_p, c_int]
             S.scene_joint_grid.argtypes = [POINTER(WorldDef), c_int, c_float, POINTER(WorldId), c_void_p, c_int]
             S.scene_joint_zoo.argtypes = [POINTER(WorldDef), c_int, c_uint32, c_uint64, POINTER(WorldId), c_void_p, c_int]
+            S.scene_continuous.argtypes = [POINTER(WorldDef), c_int, c_uint64, c_int, POINTER(WorldId), c_void_p, c_int]
 
     # ---- worlds ----
     def default_world_def(self, **overrides) -> WorldDef:
@@ -461,6 +499,10 @@ class B2Api:
             ids = self._bodies(64 + spokes)
             n = S.scene_joint_zoo(ctypes.byref(d), spokes, kw.get("types", 0x7F), kw.get("seed", 1),
                                   ctypes.byref(w), ids, len(ids))
+        elif name == "continuous":
+            count = kw.get("count", 40)
+            ids = self._bodies(count + 1)
+            n = S.scene_continuous(ctypes.byref(d), count, kw.get("seed", 1), kw.get("bullets", 0), ctypes.byref(w), ids, len(ids))
         else:
             raise ValueError(f"unknown scene {name}")
         return w, list(ids)[:n]

@@ -1,10 +1,11 @@
-// The reference's public narrowphase functions (include/box2d/collision.h, src/manifold.c,
-// src/distance.c:32-105) exported from the host compile of the same manifold code that
-// k_collide runs on the GPU (narrowphase.hpp). Returned manifolds carry the points, anchors,
+// The reference's public narrowphase, distance and time-of-impact functions
+// (include/box2d/collision.h, src/manifold.c, src/distance.c) exported from the host compile of
+// the same code that k_collide and k_continuous run on the GPU (narrowphase.hpp, toi.hpp). Returned manifolds carry the points, anchors,
 // separations and feature ids; impulses, normal velocity and the persisted flag are zero,
 // as the reference's functions leave them.
 #include "joints.hpp"
 #include "narrowphase.hpp"
+#include "toi.hpp"
 
 #include "box2d/collision.h"
 
@@ -149,6 +150,30 @@ B2_API b2Manifold b2CollideSegmentAndPolygon( const b2Segment* segmentA, b2Trans
 											  b2Transform xfB )
 {
 	return polygons( capsule_poly( segmentA->point1, segmentA->point2, 0.0f ), xfA, to_poly( polygonB ), xfB );
+}
+
+// src/distance.c:109-120
+B2_API b2DistanceProxy b2MakeProxy( const b2Vec2* vertices, int32_t count, float radius )
+{
+	return make_proxy( vertices, count, radius );
+}
+
+// src/distance.c:13-30
+B2_API b2Transform b2GetSweepTransform( const b2Sweep* sweep, float time )
+{
+	return sweep_transform( *sweep, time );
+}
+
+// src/distance.c:459-623
+B2_API b2DistanceOutput b2ShapeDistance( b2DistanceCache* cache, const b2DistanceInput* input, b2Simplex* simplexes, int simplexCapacity )
+{
+	return shape_distance( *cache, *input, simplexes, simplexCapacity );
+}
+
+// src/distance.c:999-1237
+B2_API b2TOIOutput b2TimeOfImpact( const b2TOIInput* input )
+{
+	return time_of_impact( *input );
 }
 
 B2_API void b2mi_Atan2( const float* y, const float* x, float* out, int n )

@@ -1155,3 +1155,103 @@ SCENE_API int scene_joint_zoo( const b2WorldDef* def, int spokes, uint32_t types
 	}
 	return n;
 }
+
+// Continuous collision: `count` bodies of every shape thrown down at 40-160 m/s (and up to 30 m/s
+// sideways, spinning) at thin static shapes of one ground body: a 0.1 m floor, a slanted plank, a
+// small circle and two segment walls, all thin enough that a body crosses them within a step. A
+// long bar spins at 40 rad/s on a revolute motor about a point of the ground body; its tips sweep
+// the floor, which its joint keeps out of its time of impact. The first `bullets` thrown bodies
+// are bullets.
+SCENE_API int scene_continuous( const b2WorldDef* def, int count, uint64_t seed, int bullets, b2WorldId* outWorld, b2BodyId* outBodies,
+								int capacity )
+{
+	b2WorldId worldId = scene_create_world( def );
+	*outWorld = worldId;
+	Pcg32 rng = pcg32_seed( seed, 13 );
+	int n = 0;
+	b2BodyDef gd = b2DefaultBodyDef();
+	b2BodyId ground = b2CreateBody( worldId, &gd );
+	b2ShapeDef sd = b2DefaultShapeDef();
+	{
+		b2Polygon floor = b2MakeOffsetBox( 30.0f, 0.05f, ( b2Vec2 ){ 0.0f, 0.0f }, 0.0f );
+		b2CreatePolygonShape( ground, &sd, &floor );
+		b2Polygon plank = b2MakeOffsetBox( 4.0f, 0.05f, ( b2Vec2 ){ 10.0f, 6.0f }, 0.5f );
+		b2CreatePolygonShape( ground, &sd, &plank );
+		b2Circle post = { { -10.0f, 4.0f }, 0.3f };
+		b2CreateCircleShape( ground, &sd, &post );
+		b2Segment left = { { -30.0f, 0.0f }, { -30.0f, 40.0f } };
+		b2Segment right = { { 30.0f, 40.0f }, { 30.0f, 0.0f } };
+		b2CreateSegmentShape( ground, &sd, &left );
+		b2CreateSegmentShape( ground, &sd, &right );
+	}
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.type = b2_dynamicBody;
+		bd.position = ( b2Vec2 ){ 0.0f, 3.05f };
+		b2BodyId bar = b2CreateBody( worldId, &bd );
+		b2Polygon box = b2MakeBox( 3.0f, 0.05f );
+		b2CreatePolygonShape( bar, &sd, &box );
+		b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+		jd.bodyIdA = ground;
+		jd.bodyIdB = bar;
+		jd.localAnchorA = ( b2Vec2 ){ 0.0f, 3.05f };
+		jd.localAnchorB = ( b2Vec2 ){ 0.0f, 0.0f };
+		jd.enableMotor = true;
+		jd.motorSpeed = 40.0f;
+		jd.maxMotorTorque = 1.0e6f;
+		b2CreateRevoluteJoint( worldId, &jd );
+		if ( n < capacity )
+			outBodies[n++] = bar;
+	}
+	for ( int i = 0; i < count; ++i )
+	{
+		b2BodyDef bd = b2DefaultBodyDef();
+		bd.type = b2_dynamicBody;
+		bd.isBullet = i < bullets;
+		bd.position = ( b2Vec2 ){ pcg32_uniform( &rng, -25.0f, 25.0f ), pcg32_uniform( &rng, 4.0f, 20.0f ) };
+		bd.rotation = b2MakeRot( pcg32_uniform( &rng, -3.0f, 3.0f ) );
+		bd.linearVelocity = ( b2Vec2 ){ pcg32_uniform( &rng, -30.0f, 30.0f ), pcg32_uniform( &rng, -160.0f, -40.0f ) };
+		bd.angularVelocity = pcg32_uniform( &rng, -20.0f, 20.0f );
+		b2BodyId body = b2CreateBody( worldId, &bd );
+		switch ( i % 5 )
+		{
+			case 0:
+			{
+				b2Polygon box = b2MakeBox( pcg32_uniform( &rng, 0.05f, 0.3f ), pcg32_uniform( &rng, 0.05f, 0.3f ) );
+				b2CreatePolygonShape( body, &sd, &box );
+				break;
+			}
+			case 1:
+			{
+				b2Circle circle = { { 0.0f, 0.0f }, pcg32_uniform( &rng, 0.05f, 0.25f ) };
+				b2CreateCircleShape( body, &sd, &circle );
+				break;
+			}
+			case 2:
+			{
+				float h = pcg32_uniform( &rng, 0.1f, 0.4f );
+				b2Capsule capsule = { { -h, 0.0f }, { h, 0.0f }, pcg32_uniform( &rng, 0.04f, 0.12f ) };
+				b2CreateCapsuleShape( body, &sd, &capsule );
+				break;
+			}
+			case 3:
+			{
+				b2Polygon rounded = b2MakeRoundedBox( pcg32_uniform( &rng, 0.05f, 0.2f ), pcg32_uniform( &rng, 0.05f, 0.2f ), 0.03f );
+				b2CreatePolygonShape( body, &sd, &rounded );
+				break;
+			}
+			default:
+			{
+				// two shapes: a box with a circle on one end
+				b2Polygon box = b2MakeOffsetBox( 0.25f, 0.06f, ( b2Vec2 ){ 0.1f, 0.0f }, 0.0f );
+				b2CreatePolygonShape( body, &sd, &box );
+				b2Circle knob = { { -0.2f, 0.0f }, 0.1f };
+				b2CreateCircleShape( body, &sd, &knob );
+				break;
+			}
+		}
+		if ( n < capacity )
+			outBodies[n++] = body;
+	}
+	return n;
+}

@@ -96,6 +96,97 @@ B2_API b2AABB b2ComputeSegmentAABB( const b2Segment* shape, b2Transform transfor
 // Segment closest points (reference src/distance.c:32-105).
 B2_API b2SegmentDistanceResult b2SegmentDistance( b2Vec2 p1, b2Vec2 q1, b2Vec2 p2, b2Vec2 q2 );
 
+// GJK distance and time of impact (reference include/box2d/collision.h:316-455,
+// src/distance.c:13-30, 105-623, 785-1237). The types keep the reference's layouts: the
+// continuous stage of the step builds them for every candidate pair.
+typedef struct b2DistanceProxy
+{
+	b2Vec2 points[b2_maxPolygonVertices];
+	int32_t count;
+	float radius;
+} b2DistanceProxy;
+
+typedef struct b2DistanceCache
+{
+	uint16_t count;
+	uint8_t indexA[3];
+	uint8_t indexB[3];
+} b2DistanceCache;
+
+typedef struct b2DistanceInput
+{
+	b2DistanceProxy proxyA;
+	b2DistanceProxy proxyB;
+	b2Transform transformA;
+	b2Transform transformB;
+	bool useRadii;
+} b2DistanceInput;
+
+typedef struct b2DistanceOutput
+{
+	b2Vec2 pointA;
+	b2Vec2 pointB;
+	float distance;
+	int32_t iterations;
+	int32_t simplexCount;
+} b2DistanceOutput;
+
+typedef struct b2SimplexVertex
+{
+	b2Vec2 wA;
+	b2Vec2 wB;
+	b2Vec2 w;
+	float a;
+	int32_t indexA;
+	int32_t indexB;
+} b2SimplexVertex;
+
+typedef struct b2Simplex
+{
+	b2SimplexVertex v1, v2, v3;
+	int32_t count;
+} b2Simplex;
+
+typedef struct b2Sweep
+{
+	b2Vec2 localCenter;
+	b2Vec2 c1;
+	b2Vec2 c2;
+	b2Rot q1;
+	b2Rot q2;
+} b2Sweep;
+
+typedef struct b2TOIInput
+{
+	b2DistanceProxy proxyA;
+	b2DistanceProxy proxyB;
+	b2Sweep sweepA;
+	b2Sweep sweepB;
+	float tMax;
+} b2TOIInput;
+
+typedef enum b2TOIState
+{
+	b2_toiStateUnknown,
+	b2_toiStateFailed,
+	b2_toiStateOverlapped,
+	b2_toiStateHit,
+	b2_toiStateSeparated
+} b2TOIState;
+
+typedef struct b2TOIOutput
+{
+	b2TOIState state;
+	float t;
+} b2TOIOutput;
+
+// reference include/box2d/collision.h:389, 413, 428, 455
+B2_API b2DistanceOutput b2ShapeDistance( b2DistanceCache* cache, const b2DistanceInput* input, b2Simplex* simplexes,
+										 int simplexCapacity );
+B2_API b2DistanceProxy b2MakeProxy( const b2Vec2* vertices, int32_t count, float radius );
+B2_API b2Transform b2GetSweepTransform( const b2Sweep* sweep, float time );
+B2_API b2TOIOutput b2TimeOfImpact( const b2TOIInput* input );
+
 // Narrowphase manifolds (reference src/manifold.c:14-712).
 B2_API b2Manifold b2CollideCircles( const b2Circle* circleA, b2Transform xfA, const b2Circle* circleB, b2Transform xfB );
 B2_API b2Manifold b2CollideCapsuleAndCircle( const b2Capsule* capsuleA, b2Transform xfA, const b2Circle* circleB,

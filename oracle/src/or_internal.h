@@ -449,6 +449,7 @@ typedef struct orWorld
 	float contactHertz, contactDampingRatio, jointHertz, jointDampingRatio;
 	bool enableSleep, enableWarmStarting, enableContinuous;
 	int stepIndex;
+	int toiHits; // time-of-impact events so far (a test probe)
 
 	b2Profile profile;
 

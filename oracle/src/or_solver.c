@@ -514,7 +514,8 @@ static b2AABB or_expand( b2AABB a, float d )
 }
 
 // solver.c:165-351
-static void or_finalize_bodies( orWorld* world, const orStepContext* ctx, orBitSet* enlargedSims, orIntArray* fastBodies )
+static void or_finalize_bodies( orWorld* world, const orStepContext* ctx, orBitSet* enlargedSims, orIntArray* fastBodies,
+								orIntArray* bulletBodies )
 {
 	bool enableSleep = world->enableSleep;
 	bool enableContinuous = world->enableContinuous;
@@ -580,8 +581,10 @@ static void or_finalize_bodies( orWorld* world, const orStepContext* ctx, orBitS
 			const real safetyFactor = 0.5f;
 			if ( body->type == b2_dynamicBody && enableContinuous && maxVelocity * timeStep > safetyFactor * sim->minExtent )
 			{
-				OR_ASSERT( sim->isBullet == false && "bullet bodies are not supported" );
-				OR_PUSH( *fastBodies, simIndex );
+				if ( sim->isBullet )
+					OR_PUSH( *bulletBodies, simIndex );
+				else
+					OR_PUSH( *fastBodies, simIndex );
 				sim->isFast = true;
 			}
 			else
@@ -629,15 +632,70 @@ static void or_buffer_move( orWorld* world, int proxyKey )
 	}
 }
 
+// src/solver.c:786-794
 typedef struct orContinuousQuery
 {
 	orWorld* world;
+	orBodySim* fastSim;
 	orShape* fastShape;
-	int hits;
+	b2Vec2 centroid1, centroid2;
+	b2Sweep sweep;
+	float fraction;
 } orContinuousQuery;
 
-// solver.c:795-901 up to the time-of-impact call; any surviving candidate would need
-// b2TimeOfImpact, which this restatement does not implement (asserts instead).
+// b2MakeSweep (src/body.h:159-168)
+static b2Sweep or_make_sweep( const orBodySim* sim )
+{
+	b2Sweep s;
+	s.c1 = sim->center0;
+	s.c2 = sim->center;
+	s.q1 = sim->rotation0;
+	s.q2 = sim->transform.q;
+	s.localCenter = sim->localCenter;
+	return s;
+}
+
+// b2MakeShapeDistanceProxy (src/shape.c:713-734)
+static b2DistanceProxy or_shape_proxy( const orShape* shape )
+{
+	switch ( shape->type )
+	{
+		case b2_capsuleShape:
+			return b2MakeProxy( &shape->capsule.center1, 2, shape->capsule.radius );
+		case b2_circleShape:
+			return b2MakeProxy( &shape->circle.center, 1, shape->circle.radius );
+		case b2_polygonShape:
+			return b2MakeProxy( shape->polygon.vertices, shape->polygon.count, shape->polygon.radius );
+		case b2_segmentShape:
+			return b2MakeProxy( &shape->segment.point1, 2, 0.0f );
+		default:
+		{
+			b2DistanceProxy empty = { 0 };
+			return empty;
+		}
+	}
+}
+
+// b2GetShapeCentroid (src/shape.c:480-497)
+static b2Vec2 or_fast_centroid( const orShape* s )
+{
+	switch ( s->type )
+	{
+		case b2_capsuleShape:
+			return b2Lerp( s->capsule.center1, s->capsule.center2, 0.5f );
+		case b2_circleShape:
+			return s->circle.center;
+		case b2_polygonShape:
+			return s->polygon.centroid;
+		case b2_segmentShape:
+			return b2Lerp( s->segment.point1, s->segment.point2, 0.5f );
+		default:
+			return b2Vec2_zero;
+	}
+}
+
+// src/solver.c:796-901: the time of impact against one candidate shape, keeping the smallest
+// fraction (sensors, custom filters and smooth segments are not part of this restatement)
 static bool or_continuous_callback( int proxyId, int shapeId, void* context )
 {
 	(void)proxyId;
@@ -645,7 +703,8 @@ static bool or_continuous_callback( int proxyId, int shapeId, void* context )
 	orShape* fastShape = q->fastShape;
 	if ( shapeId == fastShape->id )
 		return true;
-	orShape* shape = q->world->shapes.data + shapeId;
+	orWorld* world = q->world;
+	orShape* shape = world->shapes.data + shapeId;
 	if ( shape->bodyId == fastShape->bodyId )
 		return true;
 	b2Filter a = fastShape->filter, b = shape->filter;
@@ -653,40 +712,135 @@ static bool or_continuous_callback( int proxyId, int shapeId, void* context )
 																		  : ( ( a.maskBits & b.categoryBits ) != 0 && ( a.categoryBits & b.maskBits ) != 0 );
 	if ( collide == false )
 		return true;
-	q->hits += 1;
+	orBody* body = world->bodies.data + shape->bodyId;
+	orBodySim* bodySim = or_get_body_sim( world, body );
+	OR_ASSERT( body->type == b2_staticBody || q->fastSim->isBullet );
+	if ( bodySim->isBullet )
+		return true;
+	orBody* fastBody = world->bodies.data + q->fastSim->bodyId;
+	if ( or_should_bodies_collide( world, fastBody, body ) == false )
+		return true;
+
+	b2TOIInput input;
+	input.proxyA = or_shape_proxy( shape );
+	input.proxyB = or_shape_proxy( fastShape );
+	input.sweepA = or_make_sweep( bodySim );
+	input.sweepB = q->sweep;
+	input.tMax = q->fraction;
+	b2TOIOutput output = b2TimeOfImpact( &input );
+	if ( 0.0f < output.t && output.t < q->fraction )
+		q->fraction = output.t;
+	else if ( 0.0f == output.t )
+	{
+		// a small circle at the fast shape's centroid
+		b2Vec2 centroid = or_fast_centroid( fastShape );
+		input.proxyB = b2MakeProxy( &centroid, 1, OR_SPECULATIVE );
+		output = b2TimeOfImpact( &input );
+		if ( 0.0f < output.t && output.t < q->fraction )
+			q->fraction = output.t;
+	}
 	return true;
 }
 
-// solver.c:904-1066 for a fast body whose sweep meets no static shape
+// src/solver.c:904-1066: the fast body's sweep against the static tree (and, for a bullet, the
+// kinematic and dynamic trees); on a hit the body moves back to the time of impact
 static void or_solve_continuous( orWorld* world, int simIndex )
 {
 	orBodySim* sim = world->awakeSims.data + simIndex;
 	orBody* body = world->bodies.data + sim->bodyId;
-	b2Transform xf2 = { b2Sub( sim->center, b2RotateVector( sim->transform.q, sim->localCenter ) ), sim->transform.q };
-	orContinuousQuery q = { world, NULL, 0 };
-	for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
+	b2Sweep sweep = or_make_sweep( sim );
+	b2Transform xf1 = { b2Sub( sweep.c1, b2RotateVector( sweep.q1, sweep.localCenter ) ), sweep.q1 };
+	b2Transform xf2 = { b2Sub( sweep.c2, b2RotateVector( sweep.q2, sweep.localCenter ) ), sweep.q2 };
+	orContinuousQuery q;
+	q.world = world;
+	q.fastSim = sim;
+	q.sweep = sweep;
+	q.fraction = 1.0f;
+	for ( int id = body->headShapeId; id != OR_NULL; )
 	{
 		orShape* fastShape = world->shapes.data + id;
+		id = fastShape->nextShapeId;
 		fastShape->isFast = false;
+		q.fastShape = fastShape;
+		q.centroid1 = b2TransformPoint( xf1, fastShape->localCentroid );
+		q.centroid2 = b2TransformPoint( xf2, fastShape->localCentroid );
 		b2AABB box1 = fastShape->aabb;
 		b2AABB box2 = or_compute_shape_aabb( fastShape, xf2 );
 		b2AABB box = b2AABB_Union( box1, box2 );
 		fastShape->aabb = box2;
-		q.fastShape = fastShape;
+		if ( fastShape->isSensor )
+			continue;
 		or_tree_query( world->trees + b2_staticBody, box, b2_defaultMaskBits, or_continuous_callback, &q );
-	}
-	OR_ASSERT( q.hits == 0 && "continuous collision against static shapes (time of impact) is not supported" );
-
-	sim->rotation0 = sim->transform.q;
-	sim->center0 = sim->center;
-	for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
-	{
-		orShape* shape = world->shapes.data + id;
-		if ( b2AABB_Contains( shape->fatAABB, shape->aabb ) == false )
+		if ( sim->isBullet )
 		{
-			shape->fatAABB = or_expand( shape->aabb, OR_AABB_MARGIN );
-			shape->enlargedAABB = true;
-			sim->enlargeAABB = true;
+			or_tree_query( world->trees + b2_kinematicBody, box, b2_defaultMaskBits, or_continuous_callback, &q );
+			or_tree_query( world->trees + b2_dynamicBody, box, b2_defaultMaskBits, or_continuous_callback, &q );
+		}
+	}
+
+	if ( q.fraction < 1.0f )
+	{
+		b2Rot rq = b2NLerp( sweep.q1, sweep.q2, q.fraction );
+		b2Vec2 c = b2Lerp( sweep.c1, sweep.c2, q.fraction );
+		b2Vec2 origin = b2Sub( c, b2RotateVector( rq, sweep.localCenter ) );
+		b2Transform transform = { origin, rq };
+		sim->transform = transform;
+		sim->center = c;
+		sim->rotation0 = rq;
+		sim->center0 = c;
+#if ORACLE_SOLVER_F64
+		sim->centerR = ( orVec2r ){ c.x, c.y };
+		sim->qR = ( orVec2r ){ rq.c, rq.s };
+#endif
+		world->toiHits += 1;
+		for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
+		{
+			orShape* shape = world->shapes.data + id;
+			b2AABB aabb = or_expand( or_compute_shape_aabb( shape, transform ), OR_SPECULATIVE );
+			shape->aabb = aabb;
+			if ( b2AABB_Contains( shape->fatAABB, aabb ) == false )
+			{
+				shape->fatAABB = or_expand( aabb, OR_AABB_MARGIN );
+				shape->enlargedAABB = true;
+				sim->enlargeAABB = true;
+			}
+		}
+	}
+	else
+	{
+		sim->rotation0 = sim->transform.q;
+		sim->center0 = sim->center;
+		for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
+		{
+			orShape* shape = world->shapes.data + id;
+			if ( b2AABB_Contains( shape->fatAABB, shape->aabb ) == false )
+			{
+				shape->fatAABB = or_expand( shape->aabb, OR_AABB_MARGIN );
+				shape->enlargedAABB = true;
+				sim->enlargeAABB = true;
+			}
+		}
+	}
+}
+
+// src/solver.c:1826-1873 (and 1889-1937 for bullets): the fast shapes whose fat AABB grew,
+// enlarged in the dynamic tree (they are already in the move buffer)
+static void or_enlarge_fast( orWorld* world, const orIntArray* list )
+{
+	for ( int i = 0; i < list->count; ++i )
+	{
+		orBodySim* sim = world->awakeSims.data + list->data[i];
+		if ( sim->enlargeAABB == false )
+			continue;
+		sim->enlargeAABB = false;
+		orBody* body = world->bodies.data + sim->bodyId;
+		for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
+		{
+			orShape* shape = world->shapes.data + id;
+			if ( shape->enlargedAABB == false )
+				continue;
+			shape->enlargedAABB = false;
+			or_tree_enlarge_proxy( world->trees + b2_dynamicBody, shape->proxyKey >> 2, shape->fatAABB );
 		}
 	}
 }
@@ -802,8 +956,8 @@ void or_solve( orWorld* world, orStepContext* ctx )
 	world->moveEvents.count = awakeBodyCount;
 	orBitSet enlargedSims = { 0 };
 	or_bitset_set_count_and_clear( &enlargedSims, (uint32_t)awakeBodyCount );
-	orIntArray fastBodies = { 0 };
-	or_finalize_bodies( world, ctx, &enlargedSims, &fastBodies );
+	orIntArray fastBodies = { 0 }, bulletBodies = { 0 };
+	or_finalize_bodies( world, ctx, &enlargedSims, &fastBodies, &bulletBodies );
 
 	// hit events (solver.c:1670-1723): every colour, overflow included, in local order; the
 	// manifold point of the fastest approach above the threshold that received an impulse
@@ -868,26 +1022,16 @@ void or_solve( orWorld* world, orStepContext* ctx )
 		}
 	}
 
-	// continuous (solver.c:1807-1875): the no-hit branch, then enlarge fast shapes
+	// continuous (solver.c:1807-1937): the fast bodies, their shapes enlarged in the dynamic tree,
+	// then the bullets (which see the fast bodies' final poses), theirs enlarged
 	for ( int i = 0; i < fastBodies.count; ++i )
 		or_solve_continuous( world, fastBodies.data[i] );
-	for ( int i = 0; i < fastBodies.count; ++i )
-	{
-		orBodySim* sim = world->awakeSims.data + fastBodies.data[i];
-		if ( sim->enlargeAABB == false )
-			continue;
-		sim->enlargeAABB = false;
-		orBody* body = world->bodies.data + sim->bodyId;
-		for ( int id = body->headShapeId; id != OR_NULL; id = world->shapes.data[id].nextShapeId )
-		{
-			orShape* shape = world->shapes.data + id;
-			if ( shape->enlargedAABB == false )
-				continue;
-			shape->enlargedAABB = false;
-			or_tree_enlarge_proxy( world->trees + b2_dynamicBody, shape->proxyKey >> 2, shape->fatAABB );
-		}
-	}
+	or_enlarge_fast( world, &fastBodies );
+	for ( int i = 0; i < bulletBodies.count; ++i )
+		or_solve_continuous( world, bulletBodies.data[i] );
+	or_enlarge_fast( world, &bulletBodies );
 	OR_FREE( fastBodies );
+	OR_FREE( bulletBodies );
 	or_bitset_free( &enlargedSims );
 }
 

@@ -1738,3 +1738,10 @@ B2_API int b2mi_World_DumpBodies( b2WorldId worldId, b2mi_BodyRecord* records, i
 	}
 	return n;
 }
+
+// The world's time-of-impact events so far (a probe for the tests that continuous collision moved
+// some body back; not part of the reference's API).
+B2_API int oracle_World_GetToiHits( b2WorldId worldId )
+{
+	return or_get_world( worldId )->toiHits;
+}

@@ -1,0 +1,227 @@
+"""Continuous collision: time of impact against static shapes (src/solver.c:787-1105,
+src/distance.c:13-1237).
+
+CPU: the time of impact and GJK distance of the reference built from its own sources, of the
+oracle and of the product's host export of the kernel code, bit for bit on seeded inputs (every
+TOI outcome: separated, hit, overlapped, failed); a scene of fast bodies thrown at thin static
+shapes, the oracle against the reference step by step; properties a plausible mistake breaks
+(fast bodies stop at a thin ground instead of tunnelling through it when continuous collision is
+on; a joint to the ground keeps its body out of the ground's time of impact).
+
+GPU: the HIP path against the oracle, bit for bit after every step, on the same scenes."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from box2c_amd import capi
+from tests import helpers
+
+STATES = ("unknown", "failed", "overlapped", "hit", "separated")
+
+
+def _proxy(rng, kind):
+    """A distance proxy of a random shape: box, rounded box, convex polygon, circle, capsule or
+    segment, a little off the origin."""
+    p = capi.DistanceProxy()
+    off = rng.uniform(-0.3, 0.3, 2)
+    if kind in ("box", "rounded"):
+        hx, hy = rng.uniform(0.05, 1.5, 2)
+        pts = [(-hx, -hy), (hx, -hy), (hx, hy), (-hx, hy)]
+        radius = 0.0 if kind == "box" else float(rng.uniform(0.01, 0.2))
+    elif kind == "polygon":
+        n = int(rng.integers(3, 9))
+        ang = np.sort(rng.uniform(0, 2 * math.pi, n))
+        r = rng.uniform(0.2, 1.0)
+        pts = [(r * math.cos(a), r * math.sin(a)) for a in ang]
+        radius = 0.0
+    elif kind == "circle":
+        pts = [(0.0, 0.0)]
+        radius = float(rng.uniform(0.02, 0.8))
+    else:
+        h = rng.uniform(0.05, 1.0)
+        pts = [(-h, 0.0), (h, 0.0)]
+        radius = float(rng.uniform(0.02, 0.4)) if kind == "capsule" else 0.0
+    for i, (x, y) in enumerate(pts):
+        p.points[i] = capi.Vec2(np.float32(x + off[0]), np.float32(y + off[1]))
+    p.count = len(pts)
+    p.radius = np.float32(radius)
+    return p
+
+
+def _rot(a):
+    return capi.Rot(np.float32(math.cos(a)), np.float32(math.sin(a)))
+
+
+def _sweep(rng, moving):
+    s = capi.Sweep()
+    s.localCenter = capi.Vec2(*[np.float32(v) for v in rng.uniform(-0.2, 0.2, 2)])
+    c1 = rng.uniform(-1, 1, 2)
+    if moving:
+        # thrown across the other shape: from one side to well past it
+        d = rng.uniform(-1, 1, 2)
+        d /= max(np.linalg.norm(d), 1e-3)
+        c1 = -d * rng.uniform(1.0, 4.0) + rng.uniform(-0.5, 0.5, 2)
+        c2 = c1 + d * rng.uniform(0.5, 8.0)
+        a1 = rng.uniform(-math.pi, math.pi)
+        a2 = a1 + rng.uniform(-2.0, 2.0)
+    else:
+        c2 = c1
+        a1 = a2 = rng.uniform(-math.pi, math.pi)
+    s.c1 = capi.Vec2(*[np.float32(v) for v in c1])
+    s.c2 = capi.Vec2(*[np.float32(v) for v in c2])
+    s.q1, s.q2 = _rot(a1), _rot(a2)
+    return s
+
+
+def _toi_cases(n, seed):
+    rng = np.random.default_rng(seed)
+    kinds = ("box", "rounded", "polygon", "circle", "capsule", "segment")
+    out = []
+    for _ in range(n):
+        inp = capi.TOIInput()
+        inp.proxyA = _proxy(rng, kinds[int(rng.integers(len(kinds)))])
+        inp.proxyB = _proxy(rng, kinds[int(rng.integers(len(kinds)))])
+        inp.sweepA = _sweep(rng, moving=bool(rng.random() < 0.2))
+        inp.sweepB = _sweep(rng, moving=True)
+        inp.tMax = np.float32(rng.choice([1.0, rng.uniform(0.05, 1.0)]))
+        out.append(inp)
+    return out
+
+
+def _toi_bits(api, cases):
+    res = []
+    for c in cases:
+        o = api.lib.b2TimeOfImpact(ctypes.byref(c))
+        res.append((o.state, np.float32(o.t).view(np.uint32).item()))
+    return res
+
+
+def test_time_of_impact_three_ways():
+    """The product's time of impact (the code k_continuous runs) and the oracle's give the
+    reference's state and the bits of its time on 6,000 seeded sweeps, each outcome represented."""
+    ref = helpers.reference()
+    cases = _toi_cases(6000, 11)
+    o, p = helpers.oracle(), helpers.product()
+    a, b = _toi_bits(o, cases), _toi_bits(p, cases)
+    assert a == b
+    states = {STATES[s] for s, _ in a}
+    assert {"separated", "hit", "overlapped"} <= states, states
+    if ref is not None:
+        assert _toi_bits(ref, cases) == a
+
+
+def _distance_bits(api, cases, radii):
+    res = []
+    for c in cases:
+        cache = capi.DistanceCache()
+        inp = capi.DistanceInput()
+        inp.proxyA, inp.proxyB = c.proxyA, c.proxyB
+        inp.transformA = api.lib.b2GetSweepTransform(ctypes.byref(c.sweepA), ctypes.c_float(0.5))
+        inp.transformB = api.lib.b2GetSweepTransform(ctypes.byref(c.sweepB), ctypes.c_float(0.5))
+        inp.useRadii = radii
+        d = api.lib.b2ShapeDistance(ctypes.byref(cache), ctypes.byref(inp), None, 0)
+        vals = np.array([d.pointA.x, d.pointA.y, d.pointB.x, d.pointB.y, d.distance, inp.transformB.p.x, inp.transformB.q.s],
+                        np.float32)
+        res.append((vals.view(np.uint32).tobytes(), d.iterations, cache.count, bytes(cache.indexA[:]), bytes(cache.indexB[:])))
+    return res
+
+
+@pytest.mark.parametrize("radii", [False, True])
+def test_shape_distance_three_ways(radii):
+    """GJK distance (witness points, distance, iterations, simplex cache) and the sweep
+    transform at t = 0.5, product and oracle against the reference."""
+    ref = helpers.reference()
+    cases = _toi_cases(3000, 5)
+    o, p = helpers.oracle(), helpers.product()
+    a = _distance_bits(o, cases, radii)
+    assert _distance_bits(p, cases, radii) == a
+    if ref is not None:
+        assert _distance_bits(ref, cases, radii) == a
+
+
+def test_time_of_impact_closed_forms():
+    """Closed forms the reference's tolerances fix: a circle of radius r moving at a wall-like box
+    stops at a separation near the linear slop, at the time the straight-line motion gives."""
+    o = helpers.oracle()
+    for speed in (10.0, 50.0, 400.0):
+        inp = capi.TOIInput()
+        box = capi.DistanceProxy()
+        for i, (x, y) in enumerate([(-5, -0.05), (5, -0.05), (5, 0.05), (-5, 0.05)]):
+            box.points[i] = capi.Vec2(x, y)
+        box.count, box.radius = 4, 0.0
+        circle = capi.DistanceProxy()
+        circle.points[0] = capi.Vec2(0.0, 0.0)
+        circle.count, circle.radius = 1, 0.25
+        inp.proxyA, inp.proxyB = box, circle
+        inp.sweepA = capi.Sweep(capi.Vec2(0, 0), capi.Vec2(0, 0), capi.Vec2(0, 0), capi.Rot(1, 0), capi.Rot(1, 0))
+        # contact (center at 0.05 + 0.25) halfway through the step's motion
+        y0 = 0.3 + 0.5 * speed / 60.0
+        inp.sweepB = capi.Sweep(capi.Vec2(0, 0), capi.Vec2(0, y0), capi.Vec2(0, y0 - speed / 60.0), capi.Rot(1, 0), capi.Rot(1, 0))
+        inp.tMax = 1.0
+        out = o.lib.b2TimeOfImpact(ctypes.byref(inp))
+        assert STATES[out.state] == "hit"
+        # target separation: radius - slop, within the tolerance of a quarter slop
+        y = y0 - out.t * speed / 60.0
+        gap = (y - 0.05) - 0.25
+        assert abs(gap - (-0.005)) <= 0.25 * 0.005 + 1e-4, (speed, out.t, gap)
+        # the center stops at 0.05 + 0.25 - slop: t = (y0 - 0.295) / (speed / 60)
+        expected = (y0 - 0.295) / (speed / 60.0)
+        assert abs(out.t - expected) <= (0.25 * 0.005 + 1e-4) / (speed / 60.0), (speed, out.t, expected)
+
+
+# scene kwargs, world-def overrides, steps
+CONT_SCENES = [
+    pytest.param({"count": 40, "seed": 3}, {}, 150, id="thrown40"),
+    pytest.param({"count": 60, "seed": 7}, {}, 90, id="thrown60"),
+    pytest.param({"count": 40, "seed": 5, "bullets": 10}, {}, 120, id="bullets10"),
+]
+
+
+def _toi_hits(api, w):
+    api.lib.oracle_World_GetToiHits.argtypes = [capi.WorldId]
+    return api.lib.oracle_World_GetToiHits(w)
+
+
+def _compare_run(a, b, kw, ov, steps):
+    wa, ba = helpers.make_world(a, "continuous", kw, ov)
+    wb, bb = helpers.make_world(b, "continuous", kw, ov)
+    for s in range(steps):
+        a.step(wa)
+        b.step(wb)
+        m = helpers.first_body_mismatch(a.dump_bodies(wa), b.dump_bodies(wb))
+        assert m is None, f"step {s}: {m}"
+        m = helpers.first_contact_mismatch(a.dump_contacts(wa), b.dump_contacts(wb))
+        assert m is None, f"step {s}: {m}"
+    a.destroy(wa)
+    b.destroy(wb)
+
+
+@pytest.mark.parametrize("kw,ov,steps", CONT_SCENES)
+def test_oracle_continuous_matches_reference(kw, ov, steps):
+    ref = helpers.reference()
+    if ref is None:
+        pytest.skip("reference build not available")
+    _compare_run(helpers.oracle(), ref, kw, ov, steps)
+
+
+def test_continuous_stops_tunnelling():
+    """With continuous collision the thrown bodies stop on the 0.1 m floor (and stay inside the
+    walls); without it most of them pass through it in their first step. (The reference's
+    conservative advancement gives up on a sweep that starts overlapped, so one body in 60 may
+    still get through, the same one in the reference's run.) The oracle counts its time-of-impact
+    events, so the hit branch is known to run."""
+    o = helpers.oracle()
+    below = {}
+    for cont in (True, False):
+        w, bodies = helpers.make_world(o, "continuous", {"count": 60, "seed": 7}, {"enableContinous": cont})
+        o.step(w, n=40)
+        b = o.dump_bodies(w)
+        dyn = b["type"] == 2
+        below[cont] = int((b["center"][dyn, 1] < -0.06).sum())
+        if cont:
+            assert _toi_hits(o, w) > 40
+            assert (np.abs(b["center"][dyn, 0]) < 30.0).all()
+        o.destroy(w)
+    assert below[True] <= 2 and below[False] > 30, below
