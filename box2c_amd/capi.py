@@ -593,6 +593,7 @@ class B2Api:
         L.b2mi_Batch_SetGridBlocks.argtypes = [c_int, c_int]
         L.b2mi_World_SetGridBlocks.argtypes = [WorldId, c_int]
         L.b2mi_Batch_SetLevelRunSlots.argtypes = [c_int, c_int]
+        L.b2mi_World_GetTimeOfImpactCount.argtypes = [WorldId]
         L.b2mi_World_SetLevelRunSlots.argtypes = [WorldId, c_int]
         L.b2mi_World_SetDevicePairs.argtypes = [WorldId, c_int]
         L.b2mi_World_GetDevicePairStats.argtypes = [WorldId, POINTER(c_float)]

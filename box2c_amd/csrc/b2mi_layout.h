@@ -414,12 +414,16 @@ struct ChangeRecord
 	int pad;
 };
 
-// shape AABB update of finalize (sorted by world, sim index on the host)
+// Broadphase work of finalize and of the continuous stage (sorted by world, sim index on the
+// host): a shape whose fat AABB grew (kEnlarged, from finalize; kFastEnlarged, a fast body's
+// shape after its sweep), or a fast body (kFastBody, shapeId -1, with kBulletFast for a bullet),
+// whose shapes finalize leaves to the continuous stage.
 enum : uint32_t
 {
 	kEnlarged = 1u << 0,
-	kFast = 1u << 1,
+	kFastBody = 1u << 1,
 	kBulletFast = 1u << 2,
+	kFastEnlarged = 1u << 3,
 };
 
 struct EnlargeRecord
@@ -429,8 +433,8 @@ struct EnlargeRecord
 	int shapeId;
 	uint32_t flags;
 	float fatAABB[4];
-	float sweepBox[4]; // fast shapes: union of previous and final AABB (static-tree query box)
 };
+
 
 // new contact: device record to scatter into DevWorld::contacts[contactId]
 struct ContactInit
@@ -505,6 +509,20 @@ struct DevShapeFilter
 	uint32_t categoryBits, maskBits;
 	int groupIndex;
 	int pad;
+};
+
+// One fast body of the continuous stage (k_continuous) and what its query needs: the trees of
+// its world (static, kinematic, dynamic; the latter two for a bullet), the shape filters of its
+// world and the bodies its joints keep it from colliding with, noCollide[noCollideBegin ..
+// noCollideEnd) in DevShape::body form (sim index, or static sim index | kStaticRef).
+struct ContinuousBody
+{
+	int world;
+	int simIndex;
+	int noCollideBegin, noCollideEnd;
+	const DevTreeNode* nodes[3];
+	int roots[3];
+	const DevShapeFilter* filters;
 };
 
 // the device side of one world's broadphase, passed to k_find_pairs by value

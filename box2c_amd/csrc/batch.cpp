@@ -289,6 +289,14 @@ B2_API void b2mi_World_SetLevelRunSlots( b2WorldId worldId, int slots )
 	setLevelRunSlots( worldGroup( worldId ), slots );
 }
 
+// bodies that continuous collision moved back to a time of impact, over all steps of the world's
+// group (the world alone, or its batch)
+B2_API int b2mi_World_GetTimeOfImpactCount( b2WorldId worldId )
+{
+	World* w = getWorld( worldId );
+	return w->group != nullptr ? w->group->toiHits : 0;
+}
+
 B2_API void b2mi_World_GetDevicePairStats( b2WorldId worldId, float* stats )
 {
 	World* w = getWorld( worldId );

@@ -23,6 +23,7 @@
 
 #include "narrowphase.hpp"
 #include "joints.hpp"
+#include "toi.hpp"
 
 namespace b2mi
 {
@@ -255,8 +256,9 @@ __device__ inline bool aabb_contains( const float* a, const float* b )
 	return a[0] <= b[0] && a[1] <= b[1] && b[2] <= a[2] && b[3] <= a[3];
 }
 
-// solver.c:165-351 (+ the no-hit branch of continuous, solver.c:1035-1066) for one body.
-// enlCount[0] counts the enlarge records, enlCount[1] the bodies at the sleep time.
+// solver.c:165-351 for one body. enlCount[0] counts the enlarge records, enlCount[1] the bodies
+// at the sleep time. A fast body keeps its start pose (center0, rotation0) and its shapes' AABBs
+// for its sweep and leaves one kFastBody record: the continuous stage (k_continuous) finishes it.
 __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float4 s1, int worldIndex, EnlargeRecord* enl,
 							   int* enlCount, int enlCapacity )
 {
@@ -316,7 +318,22 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 	// (src/solver.c:281-298); the host checks the islands of such bodies
 	if ( sim.sleepTime >= K_TIME_TO_SLEEP )
 		atomicAdd( enlCount + 1, 1 );
-	// center0/rotation0: immediately for normal bodies, after the (no-hit) sweep check for fast ones
+	if ( isFast )
+	{
+		int slot = atomicAdd( enlCount, 1 );
+		if ( slot < enlCapacity )
+		{
+			EnlargeRecord r;
+			r.world = worldIndex;
+			r.simIndex = simIndex;
+			r.shapeId = -1;
+			r.flags = kFastBody | ( ( sim.flags & kBodyBullet ) ? kBulletFast : 0u );
+			for ( int c = 0; c < 4; ++c )
+				r.fatAABB[c] = 0.0f;
+			enl[slot] = r;
+		}
+		return;
+	}
 	sim.center0[0] = center.x;
 	sim.center0[1] = center.y;
 	sim.rotation0[0] = q.c;
@@ -329,49 +346,20 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 	{
 		int shapeId = W.bodyShapes[sim.shapeStart + k];
 		DevShape& shape = W.shapes[shapeId];
-		uint32_t flags = 0;
-		float box[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
-		if ( isFast )
+		float aabb[4];
+		shape_aabb( shape, xf, aabb );
+		aabb[0] -= K_SPECULATIVE;
+		aabb[1] -= K_SPECULATIVE;
+		aabb[2] += K_SPECULATIVE;
+		aabb[3] += K_SPECULATIVE;
+		for ( int c = 0; c < 4; ++c )
+			shape.aabb[c] = aabb[c];
+		if ( aabb_contains( shape.fatAABB, aabb ) == false )
 		{
-			float box2[4];
-			shape_aabb( shape, xf, box2 );
-			box[0] = minr( shape.aabb[0], box2[0] );
-			box[1] = minr( shape.aabb[1], box2[1] );
-			box[2] = maxr( shape.aabb[2], box2[2] );
-			box[3] = maxr( shape.aabb[3], box2[3] );
-			for ( int c = 0; c < 4; ++c )
-				shape.aabb[c] = box2[c];
-			flags = kFast | ( ( sim.flags & kBodyBullet ) ? kBulletFast : 0u );
-			if ( aabb_contains( shape.fatAABB, box2 ) == false )
-			{
-				shape.fatAABB[0] = box2[0] - K_AABB_MARGIN;
-				shape.fatAABB[1] = box2[1] - K_AABB_MARGIN;
-				shape.fatAABB[2] = box2[2] + K_AABB_MARGIN;
-				shape.fatAABB[3] = box2[3] + K_AABB_MARGIN;
-				flags |= kEnlarged;
-			}
-		}
-		else
-		{
-			float aabb[4];
-			shape_aabb( shape, xf, aabb );
-			aabb[0] -= K_SPECULATIVE;
-			aabb[1] -= K_SPECULATIVE;
-			aabb[2] += K_SPECULATIVE;
-			aabb[3] += K_SPECULATIVE;
-			for ( int c = 0; c < 4; ++c )
-				shape.aabb[c] = aabb[c];
-			if ( aabb_contains( shape.fatAABB, aabb ) == false )
-			{
-				shape.fatAABB[0] = aabb[0] - K_AABB_MARGIN;
-				shape.fatAABB[1] = aabb[1] - K_AABB_MARGIN;
-				shape.fatAABB[2] = aabb[2] + K_AABB_MARGIN;
-				shape.fatAABB[3] = aabb[3] + K_AABB_MARGIN;
-				flags |= kEnlarged;
-			}
-		}
-		if ( flags )
-		{
+			shape.fatAABB[0] = aabb[0] - K_AABB_MARGIN;
+			shape.fatAABB[1] = aabb[1] - K_AABB_MARGIN;
+			shape.fatAABB[2] = aabb[2] + K_AABB_MARGIN;
+			shape.fatAABB[3] = aabb[3] + K_AABB_MARGIN;
 			int slot = atomicAdd( enlCount, 1 );
 			if ( slot < enlCapacity )
 			{
@@ -379,12 +367,9 @@ __device__ void finalize_body( const DevWorld& W, int simIndex, float4 s0, float
 				r.world = worldIndex;
 				r.simIndex = simIndex;
 				r.shapeId = shapeId;
-				r.flags = flags;
+				r.flags = kEnlarged;
 				for ( int c = 0; c < 4; ++c )
-				{
 					r.fatAABB[c] = shape.fatAABB[c];
-					r.sweepBox[c] = box[c];
-				}
 				enl[slot] = r;
 			}
 		}
@@ -1256,6 +1241,235 @@ __global__ void __launch_bounds__( kSplitLevelBlock ) k_split_levels( const DevW
 					  levelCount, phase );
 }
 
+// ---- continuous collision (src/solver.c:787-1066) ----
+
+// b2MakeShapeDistanceProxy (src/shape.c:713-734): every shape keeps its points in vert(i)
+__device__ inline b2DistanceProxy shape_proxy( const DevShape& s )
+{
+	b2Vec2 pts[b2_maxPolygonVertices];
+	const int n = s.count < b2_maxPolygonVertices ? s.count : b2_maxPolygonVertices;
+	for ( int i = 0; i < n; ++i )
+		pts[i] = v2( s.vert( i )[0], s.vert( i )[1] );
+	return make_proxy( pts, n, s.type == kSegment ? 0.0f : s.radius );
+}
+
+// b2GetShapeCentroid (src/shape.c:480-497)
+__device__ inline b2Vec2 shape_centroid( const DevShape& s )
+{
+	if ( s.type == kPolygon )
+		return v2( s.centroid[0], s.centroid[1] );
+	if ( s.type == kCircle )
+		return v2( s.vert( 0 )[0], s.vert( 0 )[1] );
+	return b2Lerp( v2( s.vert( 0 )[0], s.vert( 0 )[1] ), v2( s.vert( 1 )[0], s.vert( 1 )[1] ), 0.5f );
+}
+
+// b2MakeSweep (src/body.h:159-168) of an awake body, and of a static body, whose center is its
+// origin (src/body.c:530-532) and which does not move
+__device__ inline b2Sweep sim_sweep( const DevBodySim& sim )
+{
+	b2Sweep s;
+	s.localCenter = v2( sim.localCenter[0], sim.localCenter[1] );
+	s.c1 = v2( sim.center0[0], sim.center0[1] );
+	s.c2 = v2( sim.center[0], sim.center[1] );
+	s.q1.c = sim.rotation0[0];
+	s.q1.s = sim.rotation0[1];
+	s.q2.c = sim.q[0];
+	s.q2.s = sim.q[1];
+	return s;
+}
+
+__device__ inline b2Sweep static_sweep( const DevPose& pose )
+{
+	b2Sweep s;
+	s.localCenter = v2( 0.0f, 0.0f );
+	s.c1 = s.c2 = v2( pose.p[0], pose.p[1] );
+	s.q1.c = s.q2.c = pose.q[0];
+	s.q1.s = s.q2.s = pose.q[1];
+	return s;
+}
+
+// b2ShouldShapesCollide (src/shape.h:75-83)
+__device__ inline bool filters_collide( const DevShapeFilter& a, const DevShapeFilter& b )
+{
+	if ( a.groupIndex == b.groupIndex && a.groupIndex != 0 )
+		return a.groupIndex > 0;
+	return ( a.maskBits & b.categoryBits ) != 0 && ( a.categoryBits & b.maskBits ) != 0;
+}
+
+constexpr int kContinuousStack = 256;
+
+// src/solver.c:904-1066 for fast body i of `list` (one thread each): each shape's sweep from
+// its last AABB to its final one is queried against the static tree (and, for a bullet, the
+// kinematic and dynamic trees) in the reference's depth-first order, and the time of impact
+// against every candidate that passes the reference's filters (src/solver.c:796-901) keeps the
+// smallest fraction. With a hit the body moves back to that fraction of its sweep; either way
+// its start pose becomes its final one and its shapes' fat AABBs grow where they must
+// (kFastEnlarged records). hits counts the bodies moved back; error is set when a query stack
+// overflows (the step then fails).
+__global__ void k_continuous( const DevWorld* __restrict__ worlds, const ContinuousBody* __restrict__ list, int count,
+							  const int* __restrict__ noCollide, EnlargeRecord* enl, int* enlCount, int enlCapacity, int* hits, int* error )
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= count )
+		return;
+	const ContinuousBody& cb = list[i];
+	const DevWorld& W = worlds[cb.world];
+	DevBodySim& sim = W.sims[cb.simIndex];
+	const b2Sweep sweep = sim_sweep( sim );
+	const b2Transform xf2 = { b2Sub( sweep.c2, b2RotateVector( sweep.q2, sweep.localCenter ) ), sweep.q2 };
+	const bool bullet = ( sim.flags & kBodyBullet ) != 0;
+	float fraction = 1.0f;
+	int stack[kContinuousStack];
+
+	for ( int k = 0; k < sim.shapeCount; ++k )
+	{
+		const int fastId = W.bodyShapes[sim.shapeStart + k];
+		DevShape& fs = W.shapes[fastId];
+		float box2[4];
+		shape_aabb( fs, xf2, box2 );
+		const float box[4] = { minr( fs.aabb[0], box2[0] ), minr( fs.aabb[1], box2[1] ), maxr( fs.aabb[2], box2[2] ),
+							   maxr( fs.aabb[3], box2[3] ) };
+		for ( int c = 0; c < 4; ++c )
+			fs.aabb[c] = box2[c];
+		const DevShapeFilter fastFilter = cb.filters[fastId];
+		const b2DistanceProxy fastProxy = shape_proxy( fs );
+
+		// the candidate shape `shapeId` of the query (b2ContinuousQueryCallback)
+		auto candidate = [&]( int shapeId ) {
+			if ( shapeId == fastId )
+				return;
+			const DevShape& other = W.shapes[shapeId];
+			if ( other.body == fs.body )
+				return;
+			if ( filters_collide( fastFilter, cb.filters[shapeId] ) == false )
+				return;
+			b2Sweep otherSweep;
+			if ( other.body & kStaticRef )
+				otherSweep = static_sweep( W.staticPoses[other.body & ~kStaticRef] );
+			else
+			{
+				const DevBodySim& os = W.sims[other.body];
+				if ( os.flags & kBodyBullet )
+					return;
+				otherSweep = sim_sweep( os );
+			}
+			for ( int j = cb.noCollideBegin; j < cb.noCollideEnd; ++j )
+				if ( noCollide[j] == other.body )
+					return;
+			b2TOIInput input;
+			input.proxyA = shape_proxy( other );
+			input.proxyB = fastProxy;
+			input.sweepA = otherSweep;
+			input.sweepB = sweep;
+			input.tMax = fraction;
+			b2TOIOutput out = time_of_impact( input );
+			if ( 0.0f < out.t && out.t < fraction )
+				fraction = out.t;
+			else if ( 0.0f == out.t )
+			{
+				// a small circle around the fast shape's centroid
+				const b2Vec2 centroid = shape_centroid( fs );
+				input.proxyB = make_proxy( &centroid, 1, K_SPECULATIVE );
+				out = time_of_impact( input );
+				if ( 0.0f < out.t && out.t < fraction )
+					fraction = out.t;
+			}
+		};
+
+		const int trees = bullet ? 3 : 1;
+		for ( int t = 0; t < trees; ++t )
+		{
+			const DevTreeNode* nodes = cb.nodes[t];
+			if ( nodes == nullptr || cb.roots[t] < 0 )
+				continue;
+			// b2DynamicTree_Query (src/dynamic_tree.c:1120-1159): child2's subtree first
+			int top = 0;
+			stack[top++] = cb.roots[t];
+			while ( top > 0 )
+			{
+				const int id = stack[--top];
+				const DevTreeNode node = nodes[id];
+				if ( node.categoryBits == 0u || box[0] > node.upperX || box[1] > node.upperY || node.lowerX > box[2] ||
+					 node.lowerY > box[3] )
+					continue;
+				if ( node.child1 == -1 )
+				{
+					candidate( node.userData );
+					continue;
+				}
+				if ( top + 2 > kContinuousStack )
+				{
+					atomicExch( error, 1 );
+					return;
+				}
+				stack[top++] = node.child1;
+				stack[top++] = node.child2;
+			}
+		}
+	}
+
+	b2Transform xf = xf2;
+	float pad = 0.0f;
+	if ( fraction < 1.0f )
+	{
+		// back to the time of impact
+		const b2Rot q = b2NLerp( sweep.q1, sweep.q2, fraction );
+		const b2Vec2 c = b2Lerp( sweep.c1, sweep.c2, fraction );
+		xf.q = q;
+		xf.p = b2Sub( c, b2RotateVector( q, sweep.localCenter ) );
+		sim.p[0] = xf.p.x;
+		sim.p[1] = xf.p.y;
+		sim.q[0] = q.c;
+		sim.q[1] = q.s;
+		sim.center[0] = c.x;
+		sim.center[1] = c.y;
+		DevPose& pose = W.poses[cb.simIndex];
+		pose.p[0] = xf.p.x;
+		pose.p[1] = xf.p.y;
+		pose.q[0] = q.c;
+		pose.q[1] = q.s;
+		pad = K_SPECULATIVE;
+		atomicAdd( hits, 1 );
+	}
+	sim.center0[0] = sim.center[0];
+	sim.center0[1] = sim.center[1];
+	sim.rotation0[0] = sim.q[0];
+	sim.rotation0[1] = sim.q[1];
+	for ( int k = 0; k < sim.shapeCount; ++k )
+	{
+		const int shapeId = W.bodyShapes[sim.shapeStart + k];
+		DevShape& shape = W.shapes[shapeId];
+		if ( fraction < 1.0f )
+		{
+			float aabb[4];
+			shape_aabb( shape, xf, aabb );
+			shape.aabb[0] = aabb[0] - pad;
+			shape.aabb[1] = aabb[1] - pad;
+			shape.aabb[2] = aabb[2] + pad;
+			shape.aabb[3] = aabb[3] + pad;
+		}
+		if ( aabb_contains( shape.fatAABB, shape.aabb ) == false )
+		{
+			shape.fatAABB[0] = shape.aabb[0] - K_AABB_MARGIN;
+			shape.fatAABB[1] = shape.aabb[1] - K_AABB_MARGIN;
+			shape.fatAABB[2] = shape.aabb[2] + K_AABB_MARGIN;
+			shape.fatAABB[3] = shape.aabb[3] + K_AABB_MARGIN;
+			int slot = atomicAdd( enlCount, 1 );
+			if ( slot < enlCapacity )
+			{
+				EnlargeRecord r;
+				r.world = cb.world;
+				r.simIndex = cb.simIndex;
+				r.shapeId = shapeId;
+				r.flags = kFastEnlarged | ( bullet ? kBulletFast : 0u );
+				for ( int c = 0; c < 4; ++c )
+					r.fatAABB[c] = shape.fatAABB[c];
+				enl[slot] = r;
+			}
+		}
+	}
+}
+
 } // namespace b2mi
 
 // ---- host launchers ----
@@ -1369,6 +1583,17 @@ hipError_t launch_split_phase( const DevWorld* worlds, const WorkItem* items, in
 	int block = 256, grid = ( count + jointCount + block - 1 ) / block;
 	hipLaunchKernelGGL( k_split_phase, dim3( grid ), dim3( block ), 0, stream, worlds, items, itemIndex, phase, first, count, jointPhase,
 						jointFirst, jointCount, bodyList, slotContact, slotBodies, cf, priv, mass, enl, enlCount, enlCapacity );
+	return hipGetLastError();
+}
+
+hipError_t launch_continuous( const DevWorld* worlds, const ContinuousBody* list, int count, const int* noCollide, EnlargeRecord* enl,
+							  int* enlCount, int enlCapacity, int* hits, int* error, hipStream_t stream )
+{
+	if ( count <= 0 )
+		return hipSuccess;
+	int block = 64, grid = ( count + block - 1 ) / block;
+	hipLaunchKernelGGL( k_continuous, dim3( grid ), dim3( block ), 0, stream, worlds, list, count, noCollide, enl, enlCount, enlCapacity,
+						hits, error );
 	return hipGetLastError();
 }
 

@@ -42,6 +42,10 @@ hipError_t launch_split_levels( const DevWorld* worlds, const WorkItem* items, i
 								int levelCount, const int* bodyList, const int* slotContact, const int2* slotBodies, float* cf,
 								float4* priv, float2* mass, hipStream_t stream );
 
+// the continuous stage over `count` fast bodies (k_continuous)
+hipError_t launch_continuous( const DevWorld* worlds, const ContinuousBody* list, int count, const int* noCollide, EnlargeRecord* enl,
+							  int* enlCount, int enlCapacity, int* hits, int* error, hipStream_t stream );
+
 // broadphase pair queries (broadphase.hip)
 hipError_t launch_pair_erase( unsigned long long* table, uint32_t mask, const unsigned long long* keys, int count, hipStream_t s );
 hipError_t launch_pair_insert( unsigned long long* table, uint32_t mask, const unsigned long long* keys, int count, int* error,

@@ -584,6 +584,78 @@ static void syncPairTable( World* w, hipStream_t s )
 	D.pendingKeys.clear();
 }
 
+// The device copy of tree t of world w (DevTreeNode form: leaves have child1 = -1 and child2 =
+// their position in a query's depth-first order, child2's subtree first), uploaded on stream s
+// when the tree changed since the last upload (the pinned staging buffer is rewritten only by the
+// next upload of the same tree, after the stream has been synchronized).
+static void syncDeviceTree( World* w, int t, hipStream_t s )
+{
+	DeviceBroadphase& D = w->devicePairs;
+	const DynamicTree& tree = w->trees[t];
+	const int n = (int)tree.nodes.size();
+	if ( D.uploadedVersion[t] == tree.version && D.nodes[t] != nullptr && D.capNodes[t] >= n )
+		return;
+	grow( D.nodes[t], D.capNodes[t], std::max( n, 1 ) );
+	if ( D.capStaging[t] < n )
+	{
+		// pinned, so the copy runs at full speed; the last update's copies have completed
+		if ( D.staging[t] )
+			HIP_CHECK( hipHostFree( D.staging[t] ) );
+		D.capStaging[t] = std::max( n, 2 * D.capStaging[t] );
+		HIP_CHECK( hipHostMalloc( (void**)&D.staging[t], sizeof( DevTreeNode ) * D.capStaging[t], hipHostMallocDefault ) );
+	}
+	DevTreeNode* staging = D.staging[t];
+	for ( int i = 0; i < n; ++i )
+	{
+		const TreeNode& src = tree.nodes[i];
+		staging[i] = DevTreeNode{ src.aabb.lowerBound.x, src.aabb.lowerBound.y, src.aabb.upperBound.x, src.aabb.upperBound.y,
+								  src.height == 0 ? -1 : src.child1, src.height == 0 ? -1 : src.child2, src.userData, src.categoryBits };
+	}
+	int position = 0;
+	D.walk.clear();
+	if ( tree.root != -1 )
+		D.walk.push_back( tree.root );
+	while ( !D.walk.empty() )
+	{
+		const int id = D.walk.back();
+		D.walk.pop_back();
+		DevTreeNode& node = staging[id];
+		if ( node.child1 == -1 )
+		{
+			node.child2 = position++;
+			continue;
+		}
+		D.walk.push_back( node.child1 );
+		D.walk.push_back( node.child2 );
+	}
+	if ( n > 0 )
+		HIP_CHECK( hipMemcpyAsync( D.nodes[t], staging, sizeof( DevTreeNode ) * n, hipMemcpyHostToDevice, s ) );
+	D.uploadedVersion[t] = tree.version;
+}
+
+// the device copy of world w's shape filters (by shape id), uploaded when shapes changed
+static void syncShapeFilters( World* w, hipStream_t s )
+{
+	DeviceBroadphase& D = w->devicePairs;
+	const int ns = (int)w->shapes.size();
+	if ( D.uploadedShapes == w->shapesVersion && D.shapes != nullptr && D.capShapes >= ns )
+		return;
+	grow( D.shapes, D.capShapes, std::max( ns, 1 ) );
+	std::vector<DevShapeFilter> filters( ns );
+	for ( int i = 0; i < ns; ++i )
+	{
+		const Shape& sh = w->shapes[i];
+		if ( sh.id < 0 )
+			continue;
+		filters[i] = DevShapeFilter{ sh.bodyId, (int)w->bodies[sh.bodyId].type, (uint32_t)sh.filter.categoryBits,
+									 (uint32_t)sh.filter.maskBits, (int)sh.filter.groupIndex, 0 };
+	}
+	if ( ns > 0 )
+		HIP_CHECK( hipMemcpyAsync( D.shapes, filters.data(), sizeof( DevShapeFilter ) * ns, hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	D.uploadedShapes = w->shapesVersion;
+}
+
 // src/broad_phase.c:333-436 with the queries on the device (k_find_pairs): the trees and shape
 // filters are uploaded where they changed, the pair set brought up to date, and the new pairs come
 // back with the index of the move that found them and their rank among that move's pairs; the
@@ -608,71 +680,12 @@ static void updatePairsOnDevice( World* w )
 	int maxNodes = 1;
 	for ( int t = 0; t < 3; ++t )
 	{
-		const DynamicTree& tree = w->trees[t];
-		const int n = (int)tree.nodes.size();
-		maxNodes = std::max( maxNodes, n );
-		if ( D.uploadedVersion[t] != tree.version || D.nodes[t] == nullptr || D.capNodes[t] < n )
-		{
-			grow( D.nodes[t], D.capNodes[t], std::max( n, 1 ) );
-			if ( D.capStaging[t] < n )
-			{
-				// pinned, so the copy runs at full speed; the last update's copies have completed
-				if ( D.staging[t] )
-					HIP_CHECK( hipHostFree( D.staging[t] ) );
-				D.capStaging[t] = std::max( n, 2 * D.capStaging[t] );
-				HIP_CHECK( hipHostMalloc( (void**)&D.staging[t], sizeof( DevTreeNode ) * D.capStaging[t], hipHostMallocDefault ) );
-			}
-			DevTreeNode* staging = D.staging[t];
-			for ( int i = 0; i < n; ++i )
-			{
-				const TreeNode& src = tree.nodes[i];
-				staging[i] = DevTreeNode{ src.aabb.lowerBound.x, src.aabb.lowerBound.y, src.aabb.upperBound.x, src.aabb.upperBound.y,
-											src.height == 0 ? -1 : src.child1, src.height == 0 ? -1 : src.child2, src.userData,
-											src.categoryBits };
-			}
-			// each leaf's position in the depth-first order of a query (child2's subtree first)
-			int position = 0;
-			D.walk.clear();
-			if ( tree.root != -1 )
-				D.walk.push_back( tree.root );
-			while ( !D.walk.empty() )
-			{
-				const int id = D.walk.back();
-				D.walk.pop_back();
-				DevTreeNode& node = staging[id];
-				if ( node.child1 == -1 )
-				{
-					node.child2 = position++;
-					continue;
-				}
-				D.walk.push_back( node.child1 );
-				D.walk.push_back( node.child2 );
-			}
-			if ( n > 0 )
-				HIP_CHECK( hipMemcpyAsync( D.nodes[t], staging, sizeof( DevTreeNode ) * n, hipMemcpyHostToDevice, s ) );
-			D.uploadedVersion[t] = tree.version;
-		}
+		syncDeviceTree( w, t, s );
+		maxNodes = std::max( maxNodes, (int)w->trees[t].nodes.size() );
 		bp.nodes[t] = D.nodes[t];
-		bp.roots[t] = tree.root;
+		bp.roots[t] = w->trees[t].root;
 	}
-	const int ns = (int)w->shapes.size();
-	if ( D.uploadedShapes != w->shapesVersion || D.shapes == nullptr || D.capShapes < ns )
-	{
-		grow( D.shapes, D.capShapes, std::max( ns, 1 ) );
-		std::vector<DevShapeFilter> filters( ns );
-		for ( int i = 0; i < ns; ++i )
-		{
-			const Shape& sh = w->shapes[i];
-			if ( sh.id < 0 )
-				continue;
-			filters[i] = DevShapeFilter{ sh.bodyId, (int)w->bodies[sh.bodyId].type, (uint32_t)sh.filter.categoryBits,
-										 (uint32_t)sh.filter.maskBits, (int)sh.filter.groupIndex, 0 };
-		}
-		if ( ns > 0 )
-			HIP_CHECK( hipMemcpyAsync( D.shapes, filters.data(), sizeof( DevShapeFilter ) * ns, hipMemcpyHostToDevice, s ) );
-		HIP_CHECK( hipStreamSynchronize( s ) );
-		D.uploadedShapes = w->shapesVersion;
-	}
+	syncShapeFilters( w, s );
 	mark( 1 );
 	// the moved-proxy bitmap covers every proxy key of the three trees (all zero between queries)
 	const int words = ( 4 * maxNodes + 31 ) / 32;
@@ -725,6 +738,7 @@ static void updatePairsOnDevice( World* w )
 	std::sort( D.found.begin(), D.found.end(), []( const PairRecord& a, const PairRecord& b ) {
 		return a.query != b.query ? a.query < b.query : a.rank > b.rank;
 	} );
+	const int ns = (int)w->shapes.size();
 	for ( const PairRecord& p : D.found )
 	{
 		B2MI_CHECK( 0 <= p.shapeIdA && p.shapeIdA < ns && 0 <= p.shapeIdB && p.shapeIdB < ns );
@@ -1853,11 +1867,25 @@ static void bufferMove( World* w, int proxyKey )
 }
 
 // Enlarge pass after finalize (src/solver.c:1741-1797 and 1877-1942 for fast shapes).
-static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
+static void processEnlarged( World* w, const EnlargeRecord* recs, int count, bool bullets )
 {
 	if ( count == 0 )
 		return;
-	// sim order (a counting sort on the sim index); within a body the head-first shape list order
+	if ( bullets )
+	{
+		// src/solver.c:1889-1937: the bullets' shapes whose fat AABB grew (already moved)
+		for ( int i = 0; i < count; ++i )
+			if ( ( recs[i].flags & kFastEnlarged ) && ( recs[i].flags & kBulletFast ) )
+			{
+				Shape& s = w->shapes[recs[i].shapeId];
+				s.fatAABB = { { recs[i].fatAABB[0], recs[i].fatAABB[1] }, { recs[i].fatAABB[2], recs[i].fatAABB[3] } };
+				w->trees[b2_dynamicBody].enlargeProxy( s.proxyKey >> 2, s.fatAABB );
+			}
+		return;
+	}
+	// src/solver.c:1741-1797 in sim order (a counting sort on the sim index; within a body the
+	// head-first shape list order): an enlarged shape's proxy grows and moves, a fast body's
+	// shapes move (their AABBs come from the continuous stage)
 	int maxSim = 0;
 	for ( int i = 0; i < count; ++i )
 	{
@@ -1866,22 +1894,25 @@ static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
 	}
 	std::vector<int> start( maxSim + 2, 0 );
 	for ( int i = 0; i < count; ++i )
-		start[recs[i].simIndex + 1] += 1;
+		if ( recs[i].flags & ( kEnlarged | kFastBody ) )
+			start[recs[i].simIndex + 1] += 1;
 	for ( int k = 0; k <= maxSim; ++k )
 		start[k + 1] += start[k];
-	std::vector<const EnlargeRecord*> order( count );
+	std::vector<const EnlargeRecord*> order( start[maxSim + 1] );
 	{
 		std::vector<int> fill( start.begin(), start.end() - 1 );
 		for ( int i = 0; i < count; ++i )
-			order[fill[recs[i].simIndex]++] = recs + i;
+			if ( recs[i].flags & ( kEnlarged | kFastBody ) )
+				order[fill[recs[i].simIndex]++] = recs + i;
 	}
 	for ( int k = 0; k <= maxSim; ++k )
 		if ( start[k + 1] - start[k] > 1 )
 		{
-			// several shapes of one body: order them by their position in its shape list
+			// several enlarged shapes of one body: order them by their position in its shape list
 			std::vector<std::pair<int, const EnlargeRecord*>> byRank;
 			for ( int q = start[k]; q < start[k + 1]; ++q )
 			{
+				B2MI_CHECK( order[q]->flags & kEnlarged );
 				const Body& b = w->bodies[w->shapes[order[q]->shapeId].bodyId];
 				int r = 0;
 				for ( int id = b.headShapeId; id != -1 && id != order[q]->shapeId; id = w->shapes[id].nextShapeId )
@@ -1897,37 +1928,25 @@ static void processEnlarged( World* w, const EnlargeRecord* recs, int count )
 		}
 	for ( const EnlargeRecord* r : order )
 	{
+		if ( r->flags & kFastBody )
+		{
+			const Body& b = w->bodies[w->awakeSimBody[r->simIndex]];
+			for ( int id = b.headShapeId; id != -1; id = w->shapes[id].nextShapeId )
+				bufferMove( w, w->shapes[id].proxyKey );
+			continue;
+		}
 		Shape& s = w->shapes[r->shapeId];
-		b2AABB fat = { { r->fatAABB[0], r->fatAABB[1] }, { r->fatAABB[2], r->fatAABB[3] } };
-		if ( r->flags & kEnlarged )
-			s.fatAABB = fat;
-		if ( r->flags & kFast )
-		{
-			if ( r->flags & kBulletFast )
-				fatal( "continuous collision of bullet bodies is not supported", __FILE__, __LINE__ );
-			// the sweep must not meet a static shape, or time of impact would be needed
-			b2AABB box = { { r->sweepBox[0], r->sweepBox[1] }, { r->sweepBox[2], r->sweepBox[3] } };
-			int hits = 0;
-			w->trees[b2_staticBody].query( box, b2_defaultMaskBits, [&]( int, int shapeId ) {
-				const Shape& other = w->shapes[shapeId];
-				if ( other.bodyId != s.bodyId && shouldShapesCollide( s.filter, other.filter ) )
-					hits += 1;
-				return true;
-			} );
-			if ( hits )
-				fatal( "continuous collision against static shapes (time of impact) is not supported", __FILE__, __LINE__ );
-			bufferMove( w, s.proxyKey );
-		}
-		else if ( r->flags & kEnlarged )
-		{
-			w->trees[s.proxyKey & 3].enlargeProxy( s.proxyKey >> 2, fat );
-			bufferMove( w, s.proxyKey );
-		}
+		s.fatAABB = { { r->fatAABB[0], r->fatAABB[1] }, { r->fatAABB[2], r->fatAABB[3] } };
+		w->trees[s.proxyKey & 3].enlargeProxy( s.proxyKey >> 2, s.fatAABB );
+		bufferMove( w, s.proxyKey );
 	}
-	for ( const EnlargeRecord* r : order )
-		if ( ( r->flags & kFast ) && ( r->flags & kEnlarged ) )
+	// src/solver.c:1826-1873: the fast (non-bullet) shapes whose fat AABB grew in the continuous
+	// stage, enlarged in the dynamic tree
+	for ( int i = 0; i < count; ++i )
+		if ( ( recs[i].flags & kFastEnlarged ) && !( recs[i].flags & kBulletFast ) )
 		{
-			const Shape& s = w->shapes[r->shapeId];
+			Shape& s = w->shapes[recs[i].shapeId];
+			s.fatAABB = { { recs[i].fatAABB[0], recs[i].fatAABB[1] }, { recs[i].fatAABB[2], recs[i].fatAABB[3] } };
 			w->trees[b2_dynamicBody].enlargeProxy( s.proxyKey >> 2, s.fatAABB );
 		}
 }
@@ -2471,6 +2490,91 @@ hipError_t StepGroup::enqueueSplit( int item, hipStream_t s )
 	return status;
 }
 
+// src/solver.c:1806-1937 on the device: every fast body (bullets = false) or bullet (true) of
+// hEnlarge's kFastBody records sweeps against the device copies of its world's trees (the static
+// tree; for a bullet also the kinematic and dynamic ones, as the enlarge pass left them). Each
+// body's joints that do not collide their bodies name the bodies its query skips
+// (b2ShouldBodiesCollide, src/body.c:1724-1760).
+void StepGroup::runContinuous( bool bullets )
+{
+	hipStream_t s = stream();
+	std::vector<ContinuousBody> list;
+	std::vector<int> noCollide;
+	std::vector<char> synced( worlds.size(), 0 );
+	for ( const EnlargeRecord& r : hEnlarge )
+	{
+		if ( !( r.flags & kFastBody ) || ( ( r.flags & kBulletFast ) != 0 ) != bullets )
+			continue;
+		World* w = worlds[r.world];
+		DeviceBroadphase& D = w->devicePairs;
+		if ( !synced[r.world] )
+		{
+			syncDeviceTree( w, b2_staticBody, s );
+			if ( bullets )
+			{
+				syncDeviceTree( w, b2_kinematicBody, s );
+				syncDeviceTree( w, b2_dynamicBody, s );
+			}
+			syncShapeFilters( w, s );
+			synced[r.world] = 1;
+		}
+		ContinuousBody cb{};
+		cb.world = r.world;
+		cb.simIndex = r.simIndex;
+		cb.noCollideBegin = (int)noCollide.size();
+		const Body& body = w->bodies[w->awakeSimBody[r.simIndex]];
+		for ( int key = body.headJointKey; key != -1; )
+		{
+			const Joint& j = w->joints[key >> 1];
+			const int other = j.edges[( key & 1 ) ^ 1].bodyId;
+			if ( j.collideConnected == false )
+			{
+				const Body& ob = w->bodies[other];
+				noCollide.push_back( ob.setIndex == 0 ? ( ob.localIndex | kStaticRef ) : ob.localIndex );
+			}
+			key = j.edges[key & 1].nextKey;
+		}
+		cb.noCollideEnd = (int)noCollide.size();
+		for ( int t = 0; t < 3; ++t )
+		{
+			const bool used = t == b2_staticBody || bullets;
+			cb.nodes[t] = used ? D.nodes[t] : nullptr;
+			cb.roots[t] = used ? w->trees[t].root : -1;
+		}
+		cb.filters = D.shapes;
+		list.push_back( cb );
+	}
+	if ( list.empty() )
+		return;
+	grow( dContinuous, capContinuous, (int)list.size() );
+	grow( dNoCollide, capNoCollide, std::max( (int)noCollide.size(), 1 ) );
+	HIP_CHECK( hipMemcpyAsync( dContinuous, list.data(), sizeof( ContinuousBody ) * list.size(), hipMemcpyHostToDevice, s ) );
+	if ( !noCollide.empty() )
+		HIP_CHECK( hipMemcpyAsync( dNoCollide, noCollide.data(), sizeof( int ) * noCollide.size(), hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipMemsetAsync( dCounters + kContinuousCounter, 0, 2 * sizeof( int ), s ) );
+	HIP_CHECK( launch_continuous( dWorlds, dContinuous, (int)list.size(), dNoCollide, dEnlarge, dCounters + 1, capEnlarge,
+								  dCounters + kContinuousCounter, dCounters + kContinuousCounter + 1, s ) );
+	HIP_CHECK( hipMemcpyAsync( hCounters + 1, dCounters + 1, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipMemcpyAsync( hCounters + kContinuousCounter, dCounters + kContinuousCounter, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	if ( hCounters[kContinuousCounter + 1] != 0 )
+		fatal( "a continuous-collision tree query exceeded its stack", __FILE__, __LINE__ );
+	const int hits = hCounters[kContinuousCounter];
+	toiHits += hits;
+	if ( hits > 0 && tape != nullptr )
+		fatal( "a taped run does not support a time of impact (a fast body that continuous collision moved back)", __FILE__, __LINE__ );
+	const int total = hCounters[1];
+	const int have = (int)hEnlarge.size();
+	B2MI_CHECK( have <= total && total <= capEnlarge );
+	if ( total > have )
+	{
+		hEnlarge.resize( total );
+		HIP_CHECK( hipMemcpyAsync( hEnlarge.data() + have, dEnlarge + have, sizeof( EnlargeRecord ) * ( total - have ), hipMemcpyDeviceToHost,
+								   s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+	}
+}
+
 // The solver launches of one world chunk, its size classes side by side: class j on extra
 // stream j (mod the stream count), each stream first waiting for `ready` (recorded on the
 // group's stream) and then joined back into it through joinEvents.
@@ -2647,7 +2751,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	publishLists( this, relayout ? nullptr : &collideDirty.list, relayout );
 	HIP_CHECK( hipMemsetAsync( dCounters, 0, sizeof( int ) * kCounterCount, s ) );
 	shapeTotal += shapeDelta.load();
-	grow( dEnlarge, capEnlarge, std::max( shapeTotal, 1 ) );
+	// a record per enlarged shape, per fast body and per fast shape that grew
+	grow( dEnlarge, capEnlarge, std::max( 2 * shapeTotal, 1 ) );
 	// events bracket each kernel launch alone (the stream is idle while the host works)
 	if ( events[0] == nullptr )
 		for ( hipEvent_t& e : events )
@@ -2835,17 +2940,38 @@ void StepGroup::step( float timeStep, int subStepCount )
 		hEnlarge.resize( enlargeCount );
 		HIP_CHECK( hipMemcpyAsync( hEnlarge.data(), dEnlarge, sizeof( EnlargeRecord ) * enlargeCount, hipMemcpyDeviceToHost, s ) );
 		HIP_CHECK( hipStreamSynchronize( s ) );
+		// continuous collision (src/solver.c:1806-1937): the fast bodies sweep against the static
+		// trees on the device (no host state is involved), then the enlarge pass on the host, then
+		// the bullets against the trees it updated
+		bool bullets = false;
+		for ( const EnlargeRecord& r : hEnlarge )
+			bullets = bullets || ( r.flags & ( kFastBody | kBulletFast ) ) == ( kFastBody | kBulletFast );
+		runContinuous( false );
 		std::stable_sort( hEnlarge.begin(), hEnlarge.end(),
 						  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
 		std::vector<int> first, last;
 		splitByWorld( hEnlarge, nw, first, last );
 		HostPool::get().run( nw, [&]( int i ) {
 			if ( last[i] > first[i] )
-				processEnlarged( worlds[i], hEnlarge.data() + first[i], last[i] - first[i] );
+				processEnlarged( worlds[i], hEnlarge.data() + first[i], last[i] - first[i], false );
 		} );
-		for ( int i = 0; i < nw; ++i )
-			if ( last[i] > first[i] )
-				markActive( worlds[i] );
+		if ( bullets )
+		{
+			const size_t done = hEnlarge.size();
+			runContinuous( true );
+			std::vector<EnlargeRecord> added( hEnlarge.begin() + done, hEnlarge.end() );
+			hEnlarge.resize( done );
+			std::stable_sort( added.begin(), added.end(),
+							  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
+			splitByWorld( added, nw, first, last );
+			HostPool::get().run( nw, [&]( int i ) {
+				if ( last[i] > first[i] )
+					processEnlarged( worlds[i], added.data() + first[i], last[i] - first[i], true );
+			} );
+			hEnlarge.insert( hEnlarge.end(), added.begin(), added.end() );
+		}
+		for ( const EnlargeRecord& r : hEnlarge )
+			markActive( worlds[r.world] );
 	}
 	msEnlarge = msSince( te );
 	if ( hCounters[2] > 0 )

@@ -489,9 +489,12 @@ inline int chunkOfWorld( int worldIndex, int worldCount, int chunks )
 	return (int)( (int64_t)worldIndex * chunks / worldCount );
 }
 // device counters of a step: [1] enlarged shapes, [2] bodies at the sleep time, [3] set when a
-// barrier wait of the grid solver exceeded its bound, [kChunkCounter + c] state changes of chunk c
+// barrier wait of the grid solver exceeded its bound, [kChunkCounter + c] state changes of chunk c,
+// [kContinuousCounter] bodies the continuous stage moved back to a time of impact and
+// [kContinuousCounter + 1] set when one of its tree queries ran out of stack
 constexpr int kChunkCounter = 4;
-constexpr int kCounterCount = kChunkCounter + kMaxChunks;
+constexpr int kContinuousCounter = kChunkCounter + kMaxChunks;
+constexpr int kCounterCount = kContinuousCounter + 2;
 
 // What a step's solver launches bake in (see StepGroup::solveSignature).
 struct SolveSignature
@@ -532,6 +535,11 @@ struct StepGroup
 	int* hCounters = nullptr; // pinned
 	EnlargeRecord* dEnlarge = nullptr;
 	int capEnlarge = 0;
+	ContinuousBody* dContinuous = nullptr; // fast bodies of a continuous pass, and their joints' bodies
+	int capContinuous = 0;
+	int* dNoCollide = nullptr;
+	int capNoCollide = 0;
+	int toiHits = 0; // bodies moved back to a time of impact, over all steps
 	ContactInit* dInits = nullptr;
 	int capInits = 0;
 	WorkItem* dItems = nullptr;
@@ -575,6 +583,9 @@ struct StepGroup
 	std::vector<int> splitItems; // items (indices into hItems) that run over the whole grid
 	bool splitPerPhase = false;	 // those run as one launch per phase (no cooperative launches)
 	int gridBlocksFor( const WorkItem& wi ) const;
+	// the continuous stage of the fast bodies (or bullets) among hEnlarge's records; the shape
+	// records it writes are appended to hEnlarge
+	void runContinuous( bool bullets );
 	hipError_t enqueueSplit( int item, hipStream_t s );
 	hipError_t enqueueSolve( hipStream_t s, const hipStream_t* aux, const int* abortFlag );
 	// the solver launches of a step replayed from a HIP graph while they stay the same

@@ -117,6 +117,11 @@ B2_API void b2mi_World_SetGridBlocks( b2WorldId worldId, int blocks );
 B2_API void b2mi_Batch_SetLevelRunSlots( int batch, int slots );
 B2_API void b2mi_World_SetLevelRunSlots( b2WorldId worldId, int slots );
 
+// Bodies that continuous collision moved back to a time of impact (src/solver.c:993-1031), over
+// all steps of the world's group: the world stepped alone, or its batch. No counterpart in the
+// reference, which keeps no such count.
+B2_API int b2mi_World_GetTimeOfImpactCount( b2WorldId worldId );
+
 // Broadphase pair queries of a world stepped alone run on the device (one thread per moved
 // proxy over uploaded copies of the trees, the contacts then created on the host in the
 // reference's order) when at least `minMoves` proxies moved; -1 = always on the host, 0 = always

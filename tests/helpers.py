@@ -123,6 +123,17 @@ def mask_manifold_bytes(b: np.ndarray) -> np.ndarray:
     return b
 
 
+def autotune_steps(api: B2Api, batch: int, trial_steps: int) -> int:
+    """The steps b2mi_Batch_Autotune took: one rebuild step and `trial_steps` measured steps for
+    each of its 60 shapes, and for each of the all-in-LDS footprint limits (96, 64, 48, 32 KiB)
+    below the LDS budget it chose."""
+    import ctypes
+    b, t, l, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    api.lib.b2mi_Batch_GetLaunchShape(batch, ctypes.byref(b), ctypes.byref(t), ctypes.byref(l), ctypes.byref(f))
+    fulls = sum(1 for x in (96, 64, 48, 32) if x * 1024 < l.value)
+    return (60 + fulls) * (1 + max(trial_steps, 1))
+
+
 def make_world(api: B2Api, name: str, kw: dict, overrides: dict):
     d = api.default_world_def(enableSleep=False)
     for k, v in overrides.items():

@@ -225,3 +225,52 @@ def test_continuous_stops_tunnelling():
             assert (np.abs(b["center"][dyn, 0]) < 30.0).all()
         o.destroy(w)
     assert below[True] <= 2 and below[False] > 30, below
+
+
+# ---------------------------------------------------------------------------------------------
+# GPU: the HIP path against the oracle
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kw,ov,steps", CONT_SCENES)
+def test_continuous_parity(kw, ov, steps):
+    """Fast bodies, bullets and a jointed spinning bar: bodies and contacts bit for bit after every
+    step, and as many bodies moved back to a time of impact as in the oracle."""
+    p, o = helpers.product(), helpers.oracle()
+    wp, _ = helpers.make_world(p, "continuous", kw, ov)
+    wo, _ = helpers.make_world(o, "continuous", kw, ov)
+    for s in range(steps):
+        p.step(wp)
+        o.step(wo)
+        m = helpers.first_body_mismatch(p.dump_bodies(wp), o.dump_bodies(wo))
+        assert m is None, f"step {s}: {m}"
+        m = helpers.first_contact_mismatch(p.dump_contacts(wp), o.dump_contacts(wo))
+        assert m is None, f"step {s}: {m}"
+    hits = _toi_hits(o, wo)
+    assert hits > 20 and p.lib.b2mi_World_GetTimeOfImpactCount(wp) == hits
+    p.destroy(wp)
+    o.destroy(wo)
+
+
+@pytest.mark.gpu
+def test_continuous_batch_parity():
+    """Four continuous worlds advanced together in one batch, each against its own oracle world."""
+    p, o = helpers.product(), helpers.oracle()
+    batch = p.create_batch()
+    p.use_batch(batch)
+    try:
+        worlds = [p.scene("continuous", count=30, seed=11 + i, bullets=4 * (i % 2))[0] for i in range(4)]
+    finally:
+        p.use_batch(None)
+    refs = [o.scene("continuous", count=30, seed=11 + i, bullets=4 * (i % 2))[0] for i in range(4)]
+    try:
+        for s in range(60):
+            p.batch_step(batch)
+            for wr in refs:
+                o.step(wr)
+            for i in range(4):
+                m = helpers.first_body_mismatch(p.dump_bodies(worlds[i]), o.dump_bodies(refs[i]))
+                assert m is None, f"world {i} step {s}: {m}"
+    finally:
+        p.lib.b2mi_DestroyBatch(batch)
+        for wr in refs:
+            o.destroy(wr)

@@ -150,10 +150,11 @@ def test_batch_4096_parity_sample_and_bounds():
             gpu.batch_step(batch)
             if s in (1, 5, 20):
                 check(s)
-        gpu.lib.b2mi_Batch_Autotune(batch, 1.0 / 60.0, 4, 2)  # 60 shapes x 3 steps, ordinary steps
-        check(200)
+        gpu.lib.b2mi_Batch_Autotune(batch, 1.0 / 60.0, 4, 2)  # ordinary steps, 3 per candidate shape
+        tuned = 20 + helpers.autotune_steps(gpu, batch, 2)
+        check(tuned)
         gpu.batch_step(batch, n=100)
-        check(300)
+        check(tuned + 100)
         items = gpu.work_items(batch=batch)
         assert len(items) >= n_worlds and set(items[:, 2].tolist()) >= {2}
         # every world: finite state, bounded penetration, nothing through its ground box

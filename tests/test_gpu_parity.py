@@ -245,9 +245,9 @@ def test_batch_mixed_stores_parity():
 
 
 def test_batch_autotune_keeps_parity():
-    """The autotuner's trial steps are ordinary steps: after tuning (60 candidate shapes, one
-    rebuild step and one measured step each) and more steps under the chosen shape, every world
-    still matches the oracle bit for bit."""
+    """The autotuner's trial steps are ordinary steps: after tuning (60 candidate shapes and up to
+    four footprint limits, one rebuild step and one measured step each) and more steps under the
+    chosen shape, every world still matches the oracle bit for bit."""
     gpu, cpu = helpers.product(), helpers.oracle()
     batch, worlds_g, worlds_c = _batch_worlds(gpu, cpu, 10, 5)
     try:
@@ -255,8 +255,9 @@ def test_batch_autotune_keeps_parity():
         best = gpu.lib.b2mi_Batch_Autotune(batch, 1.0 / 60.0, 4, 1)
         assert best > 0.0
         gpu.batch_step(batch, n=5)
+        tuned = helpers.autotune_steps(gpu, batch, 1)
         for wc in worlds_c:
-            cpu.step(wc, n=3 + 60 * 2 + 5)
+            cpu.step(wc, n=3 + tuned + 5)
         _compare_worlds(gpu, cpu, worlds_g, worlds_c, "autotune")
     finally:
         gpu.lib.b2mi_DestroyBatch(batch)
