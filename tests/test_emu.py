@@ -17,7 +17,7 @@ LIB = os.path.join(EMU, "_build", "libb2mi.so")
 
 # small cases that cover the taped run and its adjoint, the batch, the storage variants, the
 # per-item launch classes, the steps whose solver launch goes right behind the narrowphase, the
-# grid-wide item modes with their overflow level runs, and the joints
+# grid-wide item modes with their overflow level runs, the joints and continuous collision
 CASES = [
     "tests/test_gpu_adjoint.py::test_taped_run_matches_oracle",
     "tests/test_gpu_adjoint.py::test_replay_reproduces_run",
@@ -43,15 +43,14 @@ CASES = [
     "tests/test_gpu_parity.py::test_step_parity_device_pairs[shape_zoo]",
     "tests/test_events.py::test_events_match_oracle[drop_boxes]",
     "tests/test_events.py::test_events_taken_before_an_edit",
-    "tests/test_gpu_parity.py::test_step_parity_overflow_level_runs[run4-per_phase-overflow]",
-    "tests/test_gpu_parity.py::test_step_parity_overflow_level_runs[run1-grid_auto-overflow_negzero]",
     "tests/test_gpu_parity.py::test_step_parity_overflow_level_runs[levels_over_grid-grid_auto-overflow]",
     "tests/test_joints.py::test_joint_parity[zoo16]",
     "tests/test_joints.py::test_joint_parity[grid12g]",
     "tests/test_joints.py::test_joint_parity_stores[bodies_lds]",
     "tests/test_joints.py::test_joint_zoo_island_modes[per_phase]",
-    "tests/test_joints.py::test_joint_zoo_island_modes[grid3-levels_over_grid]",
     "tests/test_joints.py::test_joint_edits_between_steps",
+    "tests/test_continuous.py::test_continuous_parity[bullets10]",
+    "tests/test_continuous.py::test_continuous_batch_parity",
 ]
 
 
