@@ -1883,51 +1883,36 @@ static void processEnlarged( World* w, const EnlargeRecord* recs, int count, boo
 			}
 		return;
 	}
-	// src/solver.c:1741-1797 in sim order (a counting sort on the sim index; within a body the
-	// head-first shape list order): an enlarged shape's proxy grows and moves, a fast body's
+	// src/solver.c:1741-1797 in sim order, within a body the head-first shape list order (one sort
+	// on (sim index, shape rank) in a buffer the thread keeps: a batch calls this for thousands of
+	// worlds with a few records each): an enlarged shape's proxy grows and moves, a fast body's
 	// shapes move (their AABBs come from the continuous stage)
-	int maxSim = 0;
+	thread_local std::vector<std::pair<uint64_t, const EnlargeRecord*>> keyed;
+	keyed.clear();
+	const int awake = (int)w->awakeSimBody.size();
 	for ( int i = 0; i < count; ++i )
 	{
-		B2MI_CHECK( recs[i].simIndex >= 0 && recs[i].simIndex < (int)w->awakeSimBody.size() );
-		maxSim = std::max( maxSim, recs[i].simIndex );
-	}
-	std::vector<int> start( maxSim + 2, 0 );
-	for ( int i = 0; i < count; ++i )
-		if ( recs[i].flags & ( kEnlarged | kFastBody ) )
-			start[recs[i].simIndex + 1] += 1;
-	for ( int k = 0; k <= maxSim; ++k )
-		start[k + 1] += start[k];
-	std::vector<const EnlargeRecord*> order( start[maxSim + 1] );
-	{
-		std::vector<int> fill( start.begin(), start.end() - 1 );
-		for ( int i = 0; i < count; ++i )
-			if ( recs[i].flags & ( kEnlarged | kFastBody ) )
-				order[fill[recs[i].simIndex]++] = recs + i;
-	}
-	for ( int k = 0; k <= maxSim; ++k )
-		if ( start[k + 1] - start[k] > 1 )
+		const EnlargeRecord& r = recs[i];
+		if ( !( r.flags & ( kEnlarged | kFastBody ) ) )
+			continue;
+		B2MI_CHECK( r.simIndex >= 0 && r.simIndex < awake );
+		uint32_t rank = 0; // a fast body's one record first; an enlarged shape by its list position
+		if ( r.flags & kEnlarged )
 		{
-			// several enlarged shapes of one body: order them by their position in its shape list
-			std::vector<std::pair<int, const EnlargeRecord*>> byRank;
-			for ( int q = start[k]; q < start[k + 1]; ++q )
-			{
-				B2MI_CHECK( order[q]->flags & kEnlarged );
-				const Body& b = w->bodies[w->shapes[order[q]->shapeId].bodyId];
-				int r = 0;
-				for ( int id = b.headShapeId; id != -1 && id != order[q]->shapeId; id = w->shapes[id].nextShapeId )
-					r += 1;
-				byRank.push_back( { r, order[q] } );
-			}
-			std::sort( byRank.begin(), byRank.end(),
-					   []( const std::pair<int, const EnlargeRecord*>& a, const std::pair<int, const EnlargeRecord*>& b ) {
-						   return a.first < b.first;
-					   } );
-			for ( int q = start[k]; q < start[k + 1]; ++q )
-				order[q] = byRank[q - start[k]].second;
+			const Body& b = w->bodies[w->shapes[r.shapeId].bodyId];
+			rank = 1;
+			for ( int id = b.headShapeId; id != -1 && id != r.shapeId; id = w->shapes[id].nextShapeId )
+				rank += 1;
 		}
-	for ( const EnlargeRecord* r : order )
+		keyed.push_back( { ( (uint64_t)(uint32_t)r.simIndex << 32 ) | rank, &r } );
+	}
+	std::sort( keyed.begin(), keyed.end(),
+			   []( const std::pair<uint64_t, const EnlargeRecord*>& x, const std::pair<uint64_t, const EnlargeRecord*>& y ) {
+				   return x.first < y.first;
+			   } );
+	for ( const auto& kr : keyed )
 	{
+		const EnlargeRecord* r = kr.second;
 		if ( r->flags & kFastBody )
 		{
 			const Body& b = w->bodies[w->awakeSimBody[r->simIndex]];
@@ -2046,23 +2031,44 @@ public:
 				insidePool() = false;
 			}
 		} inside;
+		uint64_t g;
 		{
 			std::lock_guard<std::mutex> lk( m_ );
 			task_ = &fn;
 			count_ = n;
-			next_.store( 0 );
-			pending_ = (int)threads_.size();
-			gen_ += 1;
+			finished_.store( 0, std::memory_order_relaxed );
+			g = ++gen_;
+			next_.store( g << 32, std::memory_order_release );
+			published_.store( g, std::memory_order_release );
 		}
 		cv_.notify_all();
-		for ( int i; ( i = next_.fetch_add( 1 ) ) < n; )
-			fn( i );
+		work( g, fn, n );
+		// the tasks the workers claimed: spin briefly, then sleep until the last one signals
+		spinWhile( kCallerSpin, [&] { return finished_.load( std::memory_order_acquire ) < n; } );
 		std::unique_lock<std::mutex> lk( m_ );
-		done_.wait( lk, [&] { return pending_ == 0; } );
+		done_.wait( lk, [&] { return finished_.load( std::memory_order_acquire ) >= n; } );
 		task_ = nullptr;
 	}
 
 private:
+	// How long a worker polls for the next job before it sleeps, and the caller for the claimed
+	// tasks: a step's jobs come within a few milliseconds of each other, and waking a sleeping
+	// thread costs tens of microseconds (the whole of a small job).
+	static constexpr std::chrono::microseconds kWorkerSpin{ 2000 };
+	static constexpr std::chrono::microseconds kCallerSpin{ 500 };
+
+	template <class Pred>
+	static void spinWhile( std::chrono::microseconds limit, Pred busy )
+	{
+		const auto until = std::chrono::steady_clock::now() + limit;
+		while ( busy() )
+			for ( int i = 0; i < 64; ++i )
+				if ( !busy() )
+					return;
+				else if ( i == 63 && std::chrono::steady_clock::now() > until )
+					return;
+	}
+
 	HostPool()
 	{
 		unsigned hw = std::thread::hardware_concurrency();
@@ -2075,6 +2081,7 @@ private:
 		{
 			std::lock_guard<std::mutex> lk( m_ );
 			quit_ = true;
+			published_.store( ~0ull, std::memory_order_release );
 		}
 		cv_.notify_all();
 		for ( std::thread& t : threads_ )
@@ -2086,25 +2093,54 @@ private:
 		static thread_local bool inside = false;
 		return inside;
 	}
+	// Runs tasks of job g while any are left: a run of consecutive tasks (about a quarter of one
+	// thread's share, so the threads rarely meet on the counter) is claimed by advancing next_ =
+	// (job << 32 | next index) with a compare-and-swap that fails once another job has begun, so a
+	// late thread never takes a task, nor touches fn, of a job it was not given.
+	void work( uint64_t g, const std::function<void( int )>& fn, int n )
+	{
+		const int chunk = std::max( 1, n / ( 4 * ( (int)threads_.size() + 1 ) ) );
+		uint64_t cur = next_.load( std::memory_order_acquire );
+		for ( ;; )
+		{
+			const int begin = (int)( cur & 0xffffffffu );
+			if ( ( cur >> 32 ) != g || begin >= n )
+				return;
+			const int end = std::min( n, begin + chunk );
+			if ( !next_.compare_exchange_weak( cur, ( g << 32 ) | (uint64_t)end, std::memory_order_acq_rel, std::memory_order_acquire ) )
+				continue;
+			for ( int i = begin; i < end; ++i )
+				fn( i );
+			if ( finished_.fetch_add( end - begin, std::memory_order_acq_rel ) + ( end - begin ) == n )
+			{
+				std::lock_guard<std::mutex> lk( m_ );
+				done_.notify_one();
+			}
+			cur = next_.load( std::memory_order_acquire );
+		}
+	}
 	void worker()
 	{
 		insidePool() = true;
 		uint64_t seen = 0;
 		for ( ;; )
 		{
-			std::unique_lock<std::mutex> lk( m_ );
-			cv_.wait( lk, [&] { return quit_ || gen_ != seen; } );
-			if ( quit_ )
-				return;
-			seen = gen_;
-			const std::function<void( int )>* fn = task_;
-			int n = count_;
-			lk.unlock();
-			for ( int i; ( i = next_.fetch_add( 1 ) ) < n; )
-				( *fn )( i );
-			lk.lock();
-			if ( --pending_ == 0 )
-				done_.notify_one();
+			spinWhile( kWorkerSpin, [&] { return published_.load( std::memory_order_acquire ) == seen; } );
+			const std::function<void( int )>* fn;
+			int n;
+			uint64_t g;
+			{
+				std::unique_lock<std::mutex> lk( m_ );
+				cv_.wait( lk, [&] { return quit_ || gen_ != seen; } );
+				if ( quit_ )
+					return;
+				g = gen_;
+				fn = task_;
+				n = count_;
+			}
+			seen = g;
+			if ( fn != nullptr )
+				work( g, *fn, n );
 		}
 	}
 
@@ -2113,10 +2149,11 @@ private:
 	std::mutex m_;
 	std::condition_variable cv_, done_;
 	const std::function<void( int )>* task_ = nullptr;
-	int count_ = 0, pending_ = 0;
+	int count_ = 0;
 	uint64_t gen_ = 0;
 	bool quit_ = false;
-	std::atomic<int> next_{ 0 };
+	std::atomic<uint64_t> next_{ 0 }, published_{ 0 };
+	std::atomic<int> finished_{ 0 };
 };
 
 static void hostPoolRun( int n, const std::function<void( int )>& fn )
@@ -2188,6 +2225,21 @@ static void applyChanges( World* w, const ChangeRecord* recs, int count )
 
 // records sorted by world: [first, last) of each world
 template <typename R>
+static void sortByWorld( std::vector<R>& recs, int worldCount )
+{
+	// stable counting sort on the world index (records of one world keep their order)
+	std::vector<int> start( worldCount + 1, 0 );
+	for ( const R& r : recs )
+		start[r.world + 1] += 1;
+	for ( int i = 0; i < worldCount; ++i )
+		start[i + 1] += start[i];
+	std::vector<R> sorted( recs.size() );
+	for ( const R& r : recs )
+		sorted[start[r.world]++] = r;
+	recs.swap( sorted );
+}
+
+template <class R>
 static void splitByWorld( const std::vector<R>& recs, int worldCount, std::vector<int>& first, std::vector<int>& last )
 {
 	first.assign( worldCount, 0 );
@@ -2947,8 +2999,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 		for ( const EnlargeRecord& r : hEnlarge )
 			bullets = bullets || ( r.flags & ( kFastBody | kBulletFast ) ) == ( kFastBody | kBulletFast );
 		runContinuous( false );
-		std::stable_sort( hEnlarge.begin(), hEnlarge.end(),
-						  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
+		sortByWorld( hEnlarge, nw );
 		std::vector<int> first, last;
 		splitByWorld( hEnlarge, nw, first, last );
 		HostPool::get().run( nw, [&]( int i ) {
@@ -2961,8 +3012,7 @@ void StepGroup::step( float timeStep, int subStepCount )
 			runContinuous( true );
 			std::vector<EnlargeRecord> added( hEnlarge.begin() + done, hEnlarge.end() );
 			hEnlarge.resize( done );
-			std::stable_sort( added.begin(), added.end(),
-							  []( const EnlargeRecord& a, const EnlargeRecord& b ) { return a.world < b.world; } );
+			sortByWorld( added, nw );
 			splitByWorld( added, nw, first, last );
 			HostPool::get().run( nw, [&]( int i ) {
 				if ( last[i] > first[i] )
