@@ -83,19 +83,14 @@ __device__ inline int xcd_block( int b, int nblocks )
 	return x * q + ( x < r ? x : r ) + pos;
 }
 
-__global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __restrict__ list, int count,
-						   ChangeRecord* __restrict__ changes, int* __restrict__ changeCount )
+// One contact of world W (index `worldIndex`): its shapes' ids come with the list entry, so their
+// lines load beside the contact's own line rather than after it.
+__device__ __forceinline__ void collide_contact( const DevWorld& W, int worldIndex, int contactId, int shapeIdA, int shapeIdB,
+												 ChangeRecord* __restrict__ changes, int* __restrict__ changeCount )
 {
-	int i = xcd_block( blockIdx.x, gridDim.x ) * blockDim.x + threadIdx.x;
-	if ( i >= count )
-		return;
-	int2 item = list[i];
-	if ( item.x < 0 )
-		return; // unused entry of a world's region
-	const DevWorld& W = worlds[item.x];
-	DevContact* c = W.contacts + item.y;
-	const DevShape& A = W.shapes[c->shapeIdA];
-	const DevShape& B = W.shapes[c->shapeIdB];
+	DevContact* c = W.contacts + contactId;
+	const DevShape& A = W.shapes[shapeIdA];
+	const DevShape& B = W.shapes[shapeIdB];
 	uint32_t simFlags = c->simFlags;
 	uint32_t change = 0;
 
@@ -184,7 +179,7 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 		}
 		c->manifold = nm;
 		c->warmFrom = warmFrom;
-		W.contactOut[item.y] = out;
+		W.contactOut[contactId] = out;
 		if ( touching )
 			simFlags |= kSimTouching;
 		else
@@ -199,12 +194,32 @@ __global__ void k_collide( const DevWorld* __restrict__ worlds, const int2* __re
 	{
 		int slot = atomicAdd( changeCount, 1 );
 		ChangeRecord r;
-		r.world = item.x;
-		r.contactId = item.y;
+		r.world = worldIndex;
+		r.contactId = contactId;
 		r.flags = change;
 		r.pad = 0;
 		changes[slot] = r;
 	}
+}
+
+__global__ void k_collide( const DevWorld* __restrict__ worlds, const int4* __restrict__ list, int count,
+						   ChangeRecord* __restrict__ changes, int* __restrict__ changeCount )
+{
+	int i = xcd_block( blockIdx.x, gridDim.x ) * blockDim.x + threadIdx.x;
+	if ( i >= count )
+		return;
+	const int4 item = list[i];
+	if ( item.x < 0 )
+		return; // unused entry of a world's region
+	// a wave whose contacts all belong to one world (the usual case: each world's contacts are
+	// one range of the list) reads that world's pointers with scalar loads
+	// a wave whose contacts all belong to one world (the usual case: each world's contacts are
+	// one range of the list) reads that world's pointers with scalar loads
+	const int w0 = __builtin_amdgcn_readfirstlane( item.x );
+	if ( __all( item.x == w0 ) )
+		collide_contact( worlds[w0], w0, item.y, item.z, item.w, changes, changeCount );
+	else
+		collide_contact( worlds[item.x], item.x, item.y, item.z, item.w, changes, changeCount );
 }
 
 __device__ inline void shape_aabb( const DevShape& s, b2Transform xf, float* out )
@@ -1485,7 +1500,7 @@ hipError_t launch_init_contacts( const DevWorld* worlds, const ContactInit* init
 	return hipGetLastError();
 }
 
-hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, ChangeRecord* changes, int* changeCount,
+hipError_t launch_collide( const DevWorld* worlds, const int4* list, int count, ChangeRecord* changes, int* changeCount,
 						   hipStream_t stream )
 {
 	if ( count <= 0 )

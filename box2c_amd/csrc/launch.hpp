@@ -10,7 +10,7 @@ namespace b2mi
 {
 
 hipError_t launch_init_contacts( const DevWorld* worlds, const ContactInit* inits, int count, hipStream_t stream );
-hipError_t launch_collide( const DevWorld* worlds, const int2* list, int count, ChangeRecord* changes, int* changeCount,
+hipError_t launch_collide( const DevWorld* worlds, const int4* list, int count, ChangeRecord* changes, int* changeCount,
 						   hipStream_t stream );
 hipError_t launch_solve( int store, const DevWorld* worlds, const WorkItem* items, int itemOffset, int itemCount, int blockSize,
 						 int ldsBytes, const int* bodyList, const int* slotContact, const int2* slotBodies, const int* ovfLevels,

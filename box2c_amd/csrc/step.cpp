@@ -1548,7 +1548,7 @@ static void layoutGroup( StepGroup* g )
 	g->hBodyList.assign( bodies, 0 );
 	g->hSlotContact.assign( slots, 0 );
 	g->hSlotBodies.assign( slots, make_int2( -1, -1 ) );
-	g->hCollide.assign( collide, make_int2( -1, -1 ) );
+	g->hCollide.assign( collide, make_int4( -1, -1, -1, -1 ) );
 	grow( g->dBodyList, g->capBodyList, std::max( bodies, 1 ) );
 	int slotCap = g->capSlots;
 	grow( g->dSlotContact, slotCap, std::max( slots, 1 ) );
@@ -1677,8 +1677,15 @@ static void publishLists( StepGroup* g, const std::vector<World*>* pending, bool
 		{
 			Region cr = w->collideRegion;
 			for ( int k = 0; k < cr.cap; ++k )
-				g->hCollide[cr.base + k] =
-					k < (int)w->collideIds.size() ? make_int2( w->groupIndex, w->collideIds[k] ) : make_int2( -1, -1 );
+			{
+				if ( k < (int)w->collideIds.size() )
+				{
+					const Contact& c = w->contacts[w->collideIds[k]];
+					g->hCollide[cr.base + k] = make_int4( w->groupIndex, w->collideIds[k], c.shapeIdA, c.shapeIdB );
+				}
+				else
+					g->hCollide[cr.base + k] = make_int4( -1, -1, -1, -1 );
+			}
 			cFirst = std::min( cFirst, cr.base );
 			cLast = std::max( cLast, cr.base + cr.cap );
 			w->collidePending = false;

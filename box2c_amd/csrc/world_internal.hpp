@@ -525,7 +525,7 @@ struct StepGroup
 	// launch buffers (device)
 	DevWorld* dWorlds = nullptr;
 	int capWorlds = 0;
-	int2* dCollideList = nullptr;
+	int4* dCollideList = nullptr;
 	int capCollide = 0;
 	int collideCount = 0;
 	bool layoutValid = false; // regions of the worlds in the launch arrays are assigned
@@ -571,7 +571,7 @@ struct StepGroup
 	std::vector<int> hBodyList;
 	std::vector<int> hSlotContact;
 	std::vector<int2> hSlotBodies;
-	std::vector<int2> hCollide;
+	std::vector<int4> hCollide; // narrowphase list: (world, contact id, shape ids A and B), or -1
 	std::vector<ContactInit> hInits;
 	std::vector<ChangeRecord> hChanges;
 	std::vector<EnlargeRecord> hEnlarge;

@@ -93,6 +93,13 @@ extern thread_local dim3 emu_blockDim, emu_gridDim;
 
 void emu_syncthreads();
 #define __syncthreads() emu_syncthreads()
+// wave intrinsics, for one lane at a time: its own value and its own vote (code that takes a
+// wave-uniform branch then takes it with the lane's own value, which is what it computes)
+#define __builtin_amdgcn_readfirstlane( x ) ( x )
+inline bool __all( int predicate )
+{
+	return predicate != 0;
+}
 // a thread that polls (s_sleep in a wait loop) gives the other threads of a cooperative launch
 // their turn
 void emu_yield();
