@@ -387,6 +387,8 @@ def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed:
         g[:, :, 0:2] = 2.0 * (final[:, :, 0:2] - target)
         gf, ge, gv, gw, _ = r.backward(g)
         fwd, bwd = r.forward_seconds, r.backward_seconds
+        r.run(f, e, v0, w0, taped=False)  # the same rollout without a tape
+        plain = r.forward_seconds
         finite = bool(np.isfinite(gf).all() and np.isfinite(ge).all() and np.isfinite(gv).all() and np.isfinite(gw).all())
     finally:
         r.close()
@@ -395,6 +397,8 @@ def run_adjoint(api, worlds: int, boxes: int, steps: int, checkpoint: int, seed:
         "workload": f"drop_boxes_{worlds}x{boxes}_{steps}steps_adjoint", "bodies": n, "steps": steps,
         "checkpoint_interval": checkpoint, "forward_s": fwd, "backward_s": bwd, "adjoint_over_forward": bwd / fwd,
         "value_forward": n * SUBSTEPS * steps / fwd, "value_fwd_plus_adjoint": n * SUBSTEPS * steps / (fwd + bwd),
+        "untaped_forward_s": plain, "value_untaped": n * SUBSTEPS * steps / plain,
+        "fwd_plus_adjoint_over_untaped": (fwd + bwd) / plain,
         "unit": "bodies×substeps/s", "gradients_finite": finite,
         "note": "forward = the taped GPU step loop (narrowphase, solver, recording); backward = replay of each "
                 "checkpoint segment plus the adjoint kernels; manifolds held fixed per step; continuous collision off",

@@ -51,13 +51,23 @@ struct AdjScratch
 	float* sAdj;   // slot adjoints
 	int slots, bodies;
 
+	// Every record starts on a 16-byte boundary (the scratch base is 256-byte aligned and every
+	// region and record is a multiple of 4 floats), so neighbouring fields move as one vector.
 	__device__ inline float* applAt( int pass, int k ) const
 	{
-		return appl + ( (size_t)pass * slots + k ) * kApplFloats;
+		return static_cast<float*>( __builtin_assume_aligned( appl + ( (size_t)pass * slots + k ) * kApplFloats, 16 ) );
 	}
 	__device__ inline float* bodyAt( int record, int i ) const
 	{
-		return bodyT + ( (size_t)record * bodies + i ) * 8;
+		return static_cast<float*>( __builtin_assume_aligned( bodyT + ( (size_t)record * bodies + i ) * 8, 16 ) );
+	}
+	__device__ inline float* bodyAdj( int i ) const
+	{
+		return static_cast<float*>( __builtin_assume_aligned( bAdj + (size_t)12 * i, 16 ) );
+	}
+	__device__ inline float* slotAdj( int k ) const
+	{
+		return static_cast<float*>( __builtin_assume_aligned( sAdj + (size_t)8 * k, 16 ) );
 	}
 };
 
@@ -309,7 +319,7 @@ struct BodyAdj
 __device__ inline BodyAdj body_adj( const AdjScratch& a, int i )
 {
 	BodyAdj b;
-	b.p = i >= 0 ? a.bAdj + (size_t)12 * i : nullptr;
+	b.p = i >= 0 ? a.bodyAdj( i ) : nullptr;
 	return b;
 }
 
@@ -340,7 +350,7 @@ __device__ void lane_warm_start_adj( const S& C, const AdjScratch& a, int k, int
 	float mA = MA.x, mB = MB.x, iA = MA.y, iB = MB.y;
 	float nx = C.at( cfNormalX, k ), ny = C.at( cfNormalY, k );
 	float tx = ny, ty = 0.0f - nx;
-	float* sa = a.sAdj + (size_t)8 * k;
+	float* sa = a.slotAdj( k );
 	for ( int j = 0; j < 2; ++j )
 	{
 		float rAx = C.at( PF( AnchorAX, j ), k ), rAy = C.at( PF( AnchorAY, j ), k );
@@ -420,7 +430,7 @@ __device__ float lane_solve_adj( const S& C, const AdjScratch& a, int pass, int 
 
 	// reverse: friction point 1, 0, then normal point 1, 0
 	BodyAdj A = body_adj( a, la ), B = body_adj( a, lb );
-	float* sa = a.sAdj + (size_t)8 * k;
+	float* sa = a.slotAdj( k );
 	float gFriction = 0.0f;
 	float nBar[2] = { sa[0], sa[1] }; // adjoint of the normal impulses after this application
 	for ( int j = 1; j >= 0; --j )
@@ -509,7 +519,7 @@ __device__ float lane_restitution_adj( const S& C, const AdjScratch& a, int pass
 		lane_apply( bA, bB, mA, iA, mB, iB, rAx[j], rAy[j], rBx[j], rBy[j], impulse * nx, impulse * ny );
 	}
 	BodyAdj A = body_adj( a, la ), B = body_adj( a, lb );
-	float* sa = a.sAdj + (size_t)8 * k;
+	float* sa = a.slotAdj( k );
 	float gRest = 0.0f;
 	for ( int j = 1; j >= 0; --j )
 	{
@@ -531,7 +541,7 @@ __device__ float lane_restitution_adj( const S& C, const AdjScratch& a, int pass
 // reverse of integrate_velocity for body i from its taped state before the integration
 __device__ void integrate_velocity_adj( const AdjScratch& a, int i, const float* before, const DevBodySim& sim, const DevWorld& W )
 {
-	float* g = a.bAdj + (size_t)12 * i;
+	float* g = a.bodyAdj( i );
 	float h = W.h;
 	float maxLinearSpeed = W.maxLinearSpeed, maxAngularSpeed = W.maxAngularSpeed;
 	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
@@ -563,7 +573,7 @@ __device__ void integrate_velocity_adj( const AdjScratch& a, int i, const float*
 // reverse of integrate_position for body i from its taped state before the integration
 __device__ void integrate_position_adj( const AdjScratch& a, int i, const float* before, float h )
 {
-	float* g = a.bAdj + (size_t)12 * i;
+	float* g = a.bodyAdj( i );
 	float w = before[2], dqc = before[6], dqs = before[7];
 	float da = h * w;
 	float c2 = dqc - da * dqs;
@@ -648,7 +658,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 	// finalize^T: center' = center + dp, q' = normalize(dq q); velocities pass through
 	for ( int i = tid; i < wi.bodyCount; i += nt )
 	{
-		float* g = a.bAdj + (size_t)12 * i;
+		float* g = a.bodyAdj( i );
 		for ( int f = 0; f < 12; ++f )
 			g[f] = 0.0f;
 		if ( i >= wi.ownedCount )
@@ -680,7 +690,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 	for ( int k = tid; k < total; k += nt )
 	{
 		float4 l = lamOut[wi.slotStart + k];
-		float* sa = a.sAdj + (size_t)8 * k;
+		float* sa = a.slotAdj( k );
 		sa[0] = l.x;
 		sa[1] = l.y;
 		sa[2] = l.z;
@@ -704,7 +714,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 		for ( int k = b + tid; k < e; k += nt )
 		{
 			int2 lb = C.bodies( k );
-			a.sAdj[(size_t)8 * k + 7] += lane_restitution_adj( C, a, 3 * S, k, lb.x, lb.y, threshold );
+			a.slotAdj( k )[7] += lane_restitution_adj( C, a, 3 * S, k, lb.x, lb.y, threshold );
 		}
 		__syncthreads();
 	}
@@ -718,7 +728,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				a.sAdj[(size_t)8 * k + 6] += lane_solve_adj( C, a, 3 * sub + 2, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
+				a.slotAdj( k )[6] += lane_solve_adj( C, a, 3 * sub + 2, k, lb.x, lb.y, inv_h, minBiasVel, false, W );
 			}
 			__syncthreads();
 		}
@@ -733,7 +743,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 			for ( int k = b + tid; k < e; k += nt )
 			{
 				int2 lb = C.bodies( k );
-				a.sAdj[(size_t)8 * k + 6] += lane_solve_adj( C, a, 3 * sub + 1, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
+				a.slotAdj( k )[6] += lane_solve_adj( C, a, 3 * sub + 1, k, lb.x, lb.y, inv_h, minBiasVel, true, W );
 			}
 			__syncthreads();
 		}
@@ -762,7 +772,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 		int2 lb = C.bodies( k );
 		BodyAdj A = body_adj( a, lb.x ), B = body_adj( a, lb.y );
 		float nx = C.at( cfNormalX, k ), ny = C.at( cfNormalY, k );
-		float* sa = a.sAdj + (size_t)8 * k;
+		float* sa = a.slotAdj( k );
 		const DevContact& ct = sn[k];
 		int prev = prevSlot[wi.slotStart + k];
 		for ( int j = 0; j < 2; ++j )
@@ -787,7 +797,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 	// start-pose adjoints: center and rotation from finalize, velocities from the sweep
 	for ( int i = tid; i < wi.ownedCount; i += nt )
 	{
-		const float* g = a.bAdj + (size_t)12 * i;
+		const float* g = a.bodyAdj( i );
 		StepPose o;
 		o.cx = g[8];
 		o.cy = g[9];
@@ -805,8 +815,8 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 		float gf = 0.0f, gr = 0.0f;
 		for ( int k = 0; k < total; ++k )
 		{
-			gf += a.sAdj[(size_t)8 * k + 6];
-			gr += a.sAdj[(size_t)8 * k + 7];
+			gf += a.slotAdj( k )[6];
+			gr += a.slotAdj( k )[7];
 		}
 		paramGrad[2 * wi.world] += gf;
 		paramGrad[2 * wi.world + 1] += gr;

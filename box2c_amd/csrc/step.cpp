@@ -2823,7 +2823,10 @@ void StepGroup::step( float timeStep, int subStepCount )
 	// then needs no host round trip. Otherwise the launch returns at once and the solver runs
 	// after the state changes below. Groups whose contacts change at most steps (one large
 	// world) stop speculating after a miss and try again every 16th step.
-	const bool speculate = topology.list.empty() && tape == nullptr && itemCount > 0 && ( splitItems.empty() || !splitPerPhase ) &&
+	// A taped step records its snapshot ahead of the speculative launch (one world chunk only)
+	// and takes the record back when the launch is voided.
+	const bool speculate = topology.list.empty() && ( tape == nullptr || chunkCount == 1 ) && itemCount > 0 &&
+						   ( splitItems.empty() || !splitPerPhase ) &&
 						   ( specMisses == 0 || epoch % 16 == 0 );
 	// World chunks: consecutive worlds, so each chunk's narrowphase list is one range and its
 	// state changes go to that range of the change buffer with a counter of their own. A
@@ -2866,7 +2869,11 @@ void StepGroup::step( float timeStep, int subStepCount )
 		}
 	}
 	if ( speculate && chunks == 1 )
+	{
+		if ( tape )
+			tapeRecordStep( this, timeStep, subStepCount, blockSize > 0 ? std::min( blockSize, kAdjointMaxBlock ) : kAdjointMaxBlock );
 		launchSolve( s, dCounters + kChunkCounter );
+	}
 	if ( speculate && chunks > 1 )
 		joinAux( s );
 	if ( speculate )
@@ -2895,6 +2902,8 @@ void StepGroup::step( float timeStep, int subStepCount )
 	}
 	if ( speculate )
 		specMisses = changeCount > 0 ? specMisses + 1 : 0;
+	if ( speculate && tape && changeCount > 0 )
+		tapeUndoStep( this );
 	std::vector<World*> changedTopology;
 	if ( changeCount > 0 )
 	{

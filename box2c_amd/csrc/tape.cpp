@@ -28,6 +28,8 @@ struct TapeVersion
 	int* dSlotWorld = nullptr;
 	int* dScratchOffset = nullptr;
 	size_t scratchFloats = 0;
+	// host sources of the version's uploads, kept until the tape ends so the copies need no wait
+	std::vector<int> hOffsets, hSlotWorld;
 };
 
 struct TapeStep
@@ -37,8 +39,78 @@ struct TapeStep
 	int checkpoint; // index into checkpoints, or -1
 };
 
+// Device memory of a step group's tapes: chunks of at least 64 MiB handed out in order
+// (256-byte aligned). Ending a tape rewinds the arena without freeing it, so a recorded step
+// allocates nothing once the chunks cover a run's length; the chunks are freed with the group.
+struct TapeArena
+{
+	struct Chunk
+	{
+		char* base;
+		size_t size;
+	};
+	struct Mark
+	{
+		size_t current, used;
+	};
+	static constexpr size_t kChunkBytes = (size_t)64 << 20;
+	std::vector<Chunk> chunks;
+	size_t current = 0, used = 0; // chunk in use and the bytes of it handed out
+
+	template <typename T>
+	T* take( size_t n )
+	{
+		const size_t bytes = ( sizeof( T ) * std::max<size_t>( n, 1 ) + 255 ) & ~(size_t)255;
+		while ( current < chunks.size() && used + bytes > chunks[current].size )
+		{
+			current += 1;
+			used = 0;
+		}
+		if ( current == chunks.size() )
+		{
+			Chunk c;
+			// each new chunk at least doubles the arena: a long run takes few allocations
+			size_t held = 0;
+			for ( const Chunk& k : chunks )
+				held += k.size;
+			c.size = std::max( { bytes, kChunkBytes, held } );
+			HIP_CHECK( hipMalloc( (void**)&c.base, c.size ) );
+			chunks.push_back( c );
+			used = 0;
+		}
+		T* p = reinterpret_cast<T*>( chunks[current].base + used );
+		used += bytes;
+		return p;
+	}
+	Mark mark() const
+	{
+		return Mark{ current, used };
+	}
+	void rewind( Mark m )
+	{
+		current = m.current;
+		used = m.used;
+	}
+	void release()
+	{
+		for ( Chunk& c : chunks )
+			(void)hipFree( c.base );
+		chunks.clear();
+		current = used = 0;
+	}
+};
+
+// what one recorded step added, so a speculative record that the step voided can be taken back
+struct TapeUndo
+{
+	TapeArena::Mark mark;
+	size_t versions, checkpoints, steps;
+};
+
 struct Tape
 {
+	TapeUndo undo{};
+	TapeArena* arena = nullptr; // the group's, rewound when the tape ends
 	int interval = 10;
 	float timeStep = 0.0f;
 	int subSteps = 0;
@@ -53,43 +125,14 @@ struct Tape
 	std::vector<b2BodyId> bodies; // tape order: world by world, awake-set order
 };
 
-template <typename T>
-static T* devAlloc( size_t n )
-{
-	T* p = nullptr;
-	HIP_CHECK( hipMalloc( (void**)&p, sizeof( T ) * std::max<size_t>( n, 1 ) ) );
-	return p;
-}
-
-template <typename T>
-static void devRelease( T*& p )
-{
-	if ( p )
-		(void)hipFree( p );
-	p = nullptr;
-}
-
 void tapeFree( StepGroup* g )
 {
 	Tape* t = g->tape;
 	if ( t == nullptr )
 		return;
+	// every kernel that reads the tape's memory has finished before the arena hands it out again
 	(void)hipStreamSynchronize( stream() );
-	for ( TapeVersion& v : t->versions )
-	{
-		devRelease( v.dItems );
-		devRelease( v.dBodyList );
-		devRelease( v.dSlotContact );
-		devRelease( v.dSlotBodies );
-		devRelease( v.dSlotWorld );
-		devRelease( v.dScratchOffset );
-	}
-	for ( TapeStep& s : t->steps )
-		devRelease( s.dSnap );
-	for ( StepPose*& p : t->checkpoints )
-		devRelease( p );
-	devRelease( t->dSimBase );
-	devRelease( t->dSimCount );
+	t->arena->rewind( TapeArena::Mark{ 0, 0 } );
 	g->targetSlots = t->savedTarget;
 	for ( World* w : g->worlds )
 		w->topologyDirty = true;
@@ -98,10 +141,24 @@ void tapeFree( StepGroup* g )
 	g->tape = nullptr;
 }
 
+void tapeRelease( StepGroup* g )
+{
+	tapeFree( g );
+	if ( g->tapeArena )
+	{
+		g->tapeArena->release();
+		delete g->tapeArena;
+		g->tapeArena = nullptr;
+	}
+}
+
 static void tapeBegin( StepGroup* g, int interval )
 {
 	tapeFree( g );
 	Tape* t = new Tape();
+	if ( g->tapeArena == nullptr )
+		g->tapeArena = new TapeArena();
+	t->arena = g->tapeArena;
 	t->interval = interval > 0 ? interval : 10;
 	t->savedTarget = g->targetSlots;
 	for ( World* w : g->worlds )
@@ -120,8 +177,8 @@ static void tapeBegin( StepGroup* g, int interval )
 		}
 	}
 	hipStream_t s = stream();
-	t->dSimBase = devAlloc<int>( t->simBase.size() );
-	t->dSimCount = devAlloc<int>( t->simCount.size() );
+	t->dSimBase = t->arena->take<int>( t->simBase.size() );
+	t->dSimCount = t->arena->take<int>( t->simCount.size() );
 	if ( !t->simBase.empty() )
 	{
 		HIP_CHECK( hipMemcpyAsync( t->dSimBase, t->simBase.data(), sizeof( int ) * t->simBase.size(), hipMemcpyHostToDevice, s ) );
@@ -138,6 +195,7 @@ static void tapeBegin( StepGroup* g, int interval )
 void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSize )
 {
 	Tape* t = g->tape;
+	t->undo = TapeUndo{ t->arena->mark(), t->versions.size(), t->checkpoints.size(), t->steps.size() };
 	hipStream_t s = stream();
 	if ( t->steps.empty() )
 	{
@@ -184,16 +242,18 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 		for ( int n : itemsOfWorld )
 			B2MI_CHECK( n <= 1 );
 		v.scratchFloats = scratch;
-		v.dItems = devAlloc<WorkItem>( v.itemCount );
-		v.dBodyList = devAlloc<int>( v.bodyListCount );
-		v.dSlotContact = devAlloc<int>( v.slotCount );
-		v.dSlotBodies = devAlloc<int2>( v.slotCount );
-		v.dSlotWorld = devAlloc<int>( v.slotCount );
-		v.dScratchOffset = devAlloc<int>( v.itemCount );
+		v.dItems = t->arena->take<WorkItem>( v.itemCount );
+		v.dBodyList = t->arena->take<int>( v.bodyListCount );
+		v.dSlotContact = t->arena->take<int>( v.slotCount );
+		v.dSlotBodies = t->arena->take<int2>( v.slotCount );
+		v.dSlotWorld = t->arena->take<int>( v.slotCount );
+		v.dScratchOffset = t->arena->take<int>( v.itemCount );
+		v.hOffsets = std::move( offsets );
+		v.hSlotWorld = std::move( slotWorld );
 		if ( v.itemCount )
 		{
 			HIP_CHECK( hipMemcpyAsync( v.dItems, g->dItems, sizeof( WorkItem ) * v.itemCount, hipMemcpyDeviceToDevice, s ) );
-			HIP_CHECK( hipMemcpyAsync( v.dScratchOffset, offsets.data(), sizeof( int ) * v.itemCount, hipMemcpyHostToDevice, s ) );
+			HIP_CHECK( hipMemcpyAsync( v.dScratchOffset, v.hOffsets.data(), sizeof( int ) * v.itemCount, hipMemcpyHostToDevice, s ) );
 		}
 		if ( v.bodyListCount )
 			HIP_CHECK( hipMemcpyAsync( v.dBodyList, g->dBodyList, sizeof( int ) * v.bodyListCount, hipMemcpyDeviceToDevice, s ) );
@@ -201,11 +261,9 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 		{
 			HIP_CHECK( hipMemcpyAsync( v.dSlotContact, g->dSlotContact, sizeof( int ) * v.slotCount, hipMemcpyDeviceToDevice, s ) );
 			HIP_CHECK( hipMemcpyAsync( v.dSlotBodies, g->dSlotBodies, sizeof( int2 ) * v.slotCount, hipMemcpyDeviceToDevice, s ) );
-			HIP_CHECK( hipMemcpyAsync( v.dSlotWorld, slotWorld.data(), sizeof( int ) * v.slotCount, hipMemcpyHostToDevice, s ) );
+			HIP_CHECK( hipMemcpyAsync( v.dSlotWorld, v.hSlotWorld.data(), sizeof( int ) * v.slotCount, hipMemcpyHostToDevice, s ) );
 		}
-		// the host vectors above are read by the copies: wait before they go out of scope
-		HIP_CHECK( hipStreamSynchronize( s ) );
-		t->versions.push_back( v );
+		t->versions.push_back( std::move( v ) );
 	}
 
 	TapeStep st;
@@ -213,14 +271,26 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 	st.checkpoint = -1;
 	if ( t->steps.size() % (size_t)t->interval == 0 )
 	{
-		StepPose* cp = devAlloc<StepPose>( t->totalSims );
+		StepPose* cp = t->arena->take<StepPose>( t->totalSims );
 		HIP_CHECK( launch_gather_poses( g->dWorlds, t->dSimBase, t->dSimCount, (int)g->worlds.size(), t->maxSims, cp, s ) );
 		st.checkpoint = (int)t->checkpoints.size();
 		t->checkpoints.push_back( cp );
 	}
-	st.dSnap = devAlloc<DevContact>( g->slotCount );
+	st.dSnap = t->arena->take<DevContact>( g->slotCount );
 	HIP_CHECK( launch_snapshot( g->dWorlds, g->dItems, g->itemCount, g->dSlotContact, st.dSnap, s ) );
 	t->steps.push_back( st );
+}
+
+void tapeUndoStep( StepGroup* g )
+{
+	Tape* t = g->tape;
+	const TapeUndo& u = t->undo;
+	B2MI_CHECK( t->steps.size() == u.steps + 1 );
+	// the discarded records' kernels are ahead of any later use of their memory on the stream
+	t->steps.resize( u.steps );
+	t->checkpoints.resize( u.checkpoints );
+	t->versions.resize( u.versions );
+	t->arena->rewind( u.mark );
 }
 
 // d(loss)/d(start poses) and d(loss)/d(friction, restitution per world) from d(loss)/d(end poses)
@@ -258,18 +328,20 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 	}
 
 	const int K = t->interval;
-	StepPose* dAdj = devAlloc<StepPose>( n );
-	StepPose* dStates = devAlloc<StepPose>( (size_t)( K + 1 ) * n );
-	float* dParam = devAlloc<float>( 2 * W );
-	float* dCf = devAlloc<float>( (size_t)cfFloatCount * maxSlots );
-	float4* dPriv = devAlloc<float4>( 2 * (size_t)maxBodies );
-	float2* dMass = devAlloc<float2>( maxBodies );
-	float* dScratch = devAlloc<float>( maxScratch );
-	float* dLamA = devAlloc<float>( 4 * (size_t)maxSlots );
-	float* dLamB = devAlloc<float>( 4 * (size_t)maxSlots );
-	int* dPrevSlot = devAlloc<int>( maxSlots );
-	int* dContactBase = devAlloc<int>( W );
-	int2* dSlotOf = devAlloc<int2>( contactTotal );
+	// working memory of the pass from the tape's arena, taken back when the results are home
+	const TapeArena::Mark scratchMark = t->arena->mark();
+	StepPose* dAdj = t->arena->take<StepPose>( n );
+	StepPose* dStates = t->arena->take<StepPose>( (size_t)( K + 1 ) * n );
+	float* dParam = t->arena->take<float>( 2 * W );
+	float* dCf = t->arena->take<float>( (size_t)cfFloatCount * maxSlots );
+	float4* dPriv = t->arena->take<float4>( 2 * (size_t)maxBodies );
+	float2* dMass = t->arena->take<float2>( maxBodies );
+	float* dScratch = t->arena->take<float>( maxScratch );
+	float* dLamA = t->arena->take<float>( 4 * (size_t)maxSlots );
+	float* dLamB = t->arena->take<float>( 4 * (size_t)maxSlots );
+	int* dPrevSlot = t->arena->take<int>( maxSlots );
+	int* dContactBase = t->arena->take<int>( W );
+	int2* dSlotOf = t->arena->take<int2>( contactTotal );
 
 	std::vector<StepPose> h( n );
 	for ( int i = 0; i < n; ++i )
@@ -327,18 +399,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 		float* o = gradStart + 7 * i;
 		o[0] = h[i].cx, o[1] = h[i].cy, o[2] = h[i].qc, o[3] = h[i].qs, o[4] = h[i].vx, o[5] = h[i].vy, o[6] = h[i].w;
 	}
-	devRelease( dAdj );
-	devRelease( dStates );
-	devRelease( dParam );
-	devRelease( dCf );
-	devRelease( dPriv );
-	devRelease( dMass );
-	devRelease( dScratch );
-	devRelease( dLamA );
-	devRelease( dLamB );
-	devRelease( dPrevSlot );
-	devRelease( dContactBase );
-	devRelease( dSlotOf );
+	t->arena->rewind( scratchMark );
 }
 
 // The taped solver steps again from the given start poses (n x 7), optionally with every
@@ -357,13 +418,14 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 		maxSlots = std::max( maxSlots, v.slotCount );
 		maxBodies = std::max( maxBodies, v.bodyListCount );
 	}
-	StepPose* dA = devAlloc<StepPose>( n );
-	StepPose* dB = devAlloc<StepPose>( n );
-	float* dCf = devAlloc<float>( (size_t)cfFloatCount * maxSlots );
-	float4* dPriv = devAlloc<float4>( 2 * (size_t)maxBodies );
-	float2* dMass = devAlloc<float2>( maxBodies );
-	DevContact* dPatched = devAlloc<DevContact>( maxSlots );
-	float* dParams = devAlloc<float>( 2 * W );
+	const TapeArena::Mark scratchMark = t->arena->mark();
+	StepPose* dA = t->arena->take<StepPose>( n );
+	StepPose* dB = t->arena->take<StepPose>( n );
+	float* dCf = t->arena->take<float>( (size_t)cfFloatCount * maxSlots );
+	float4* dPriv = t->arena->take<float4>( 2 * (size_t)maxBodies );
+	float2* dMass = t->arena->take<float2>( maxBodies );
+	DevContact* dPatched = t->arena->take<DevContact>( maxSlots );
+	float* dParams = t->arena->take<float>( 2 * W );
 	std::vector<StepPose> h( n );
 	for ( int i = 0; i < n; ++i )
 	{
@@ -394,13 +456,7 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 		float* o = endPoses + 7 * i;
 		o[0] = h[i].cx, o[1] = h[i].cy, o[2] = h[i].qc, o[3] = h[i].qs, o[4] = h[i].vx, o[5] = h[i].vy, o[6] = h[i].w;
 	}
-	devRelease( dA );
-	devRelease( dB );
-	devRelease( dCf );
-	devRelease( dPriv );
-	devRelease( dMass );
-	devRelease( dPatched );
-	devRelease( dParams );
+	t->arena->rewind( scratchMark );
 }
 
 // ---- the replay without allocation --------------------------------------------------------
@@ -566,14 +622,15 @@ B2_API void b2mi_Batch_GetTapePoses( int batch, float* poses )
 	Tape* t = g->tape;
 	B2MI_CHECK( t != nullptr );
 	hipStream_t s = stream();
-	StepPose* d = devAlloc<StepPose>( t->totalSims );
+	const TapeArena::Mark scratchMark = t->arena->mark();
+	StepPose* d = t->arena->take<StepPose>( t->totalSims );
 	if ( g->dWorlds == nullptr )
 		fatal( "the batch has not been stepped", __FILE__, __LINE__ );
 	HIP_CHECK( launch_gather_poses( g->dWorlds, t->dSimBase, t->dSimCount, (int)g->worlds.size(), t->maxSims, d, s ) );
 	std::vector<StepPose> h( t->totalSims );
 	HIP_CHECK( hipMemcpyAsync( h.data(), d, sizeof( StepPose ) * h.size(), hipMemcpyDeviceToHost, s ) );
 	HIP_CHECK( hipStreamSynchronize( s ) );
-	devRelease( d );
+	t->arena->rewind( scratchMark );
 	for ( size_t i = 0; i < h.size(); ++i )
 	{
 		float* o = poses + 7 * i;

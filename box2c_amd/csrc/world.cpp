@@ -138,7 +138,7 @@ World::~World()
 
 StepGroup::~StepGroup()
 {
-	tapeFree( this );
+	tapeRelease( this );
 	devFree( dWorlds );
 	devFree( dCollideList );
 	devFree( dChanges );

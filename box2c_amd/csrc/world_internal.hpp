@@ -463,7 +463,8 @@ struct World
 	~World();
 };
 
-struct Tape; // taped run of a step group (tape.cpp)
+struct Tape;	   // taped run of a step group (tape.cpp)
+struct TapeArena; // device memory of the group's tapes (tape.cpp)
 
 // items of one world chunk and one solver workgroup size: a contiguous range of the
 // launch-ordered items
@@ -564,6 +565,7 @@ struct StepGroup
 	int maxItemLds = 0; // dynamic LDS bytes of the solver launch (largest LDS-resident item)
 	int itemsVersion = 0; // incremented by every work-item rebuild
 	Tape* tape = nullptr; // recording for the adjoint, when on
+	TapeArena* tapeArena = nullptr; // kept from one tape to the next, freed with the group
 
 	// host staging
 	std::vector<DevWorld> hWorlds;
@@ -659,7 +661,9 @@ void gatherLazyEvents( World* w );
 
 // tape.cpp
 void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSize );
-void tapeFree( StepGroup* g );
+void tapeFree( StepGroup* g );	 // ends the tape; its device memory stays with the group
+void tapeRelease( StepGroup* g ); // ends the tape and frees the group's tape memory
+void tapeUndoStep( StepGroup* g ); // takes back the last tapeRecordStep (a voided speculative launch)
 
 hipStream_t stream();
 hipStream_t auxStream( int i ); // extra streams (default priority) for concurrent launches

@@ -100,8 +100,10 @@ class DropBoxesRollout:
             self.api.lib.b2mi_DestroyBatch(self.batch)
             self.batch = None
 
-    def run(self, friction, restitution, v0, w0) -> np.ndarray:
-        """Final poses, shape (worlds, boxes, 7)."""
+    def run(self, friction, restitution, v0, w0, taped: bool = True) -> np.ndarray | None:
+        """Final poses, shape (worlds, boxes, 7). With taped=False the same steps run without a
+        tape (one work item per world all the same) and only forward_seconds is set: the
+        untaped cost of the rollout, for comparison."""
         self.close()
         api = self.api
         friction = np.ascontiguousarray(friction, np.float32)
@@ -123,6 +125,14 @@ class DropBoxesRollout:
             api.use_batch(None)
         for _ in range(self.pre_steps):
             api.lib.b2mi_Batch_Step(self.batch, ctypes.c_float(self.dt), self.substeps)
+        if not taped:
+            # the work items a tape asks for: one per world
+            api.lib.b2mi_Batch_Configure(self.batch, 0, 1 << 24)
+            t0 = time.perf_counter()
+            for _ in range(self.steps):
+                api.lib.b2mi_Batch_Step(self.batch, ctypes.c_float(self.dt), self.substeps)
+            self.forward_seconds = time.perf_counter() - t0  # a step returns with its device work done
+            return None
         api.lib.b2mi_Batch_BeginTape(self.batch, self.checkpoint)
         n = api.lib.b2mi_Batch_GetTapeBodies(self.batch, None, 0)
         assert n == self.worlds * self.boxes, "tape order must be world by world, box by box"
