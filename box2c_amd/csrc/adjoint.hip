@@ -626,6 +626,7 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_replay_step( const DevWo
 // stored impulses through the warm-start links (prevSlot: per slot of this step, the previous
 // step's slot of the same contact, or -1). paramGrad accumulates per world d/d(friction) and
 // d/d(restitution).
+template <bool kAdjLds>
 __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 																	const int* __restrict__ bodyList, const int2* __restrict__ slotBodies,
 																	const DevContact* __restrict__ snap, const StepPose* __restrict__ poseIn,
@@ -650,6 +651,13 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 	C.M = mass + wi.bodyStart;
 	Tape tape;
 	tape.a = carve( scratch + scratchOffset[blockIdx.x], wi.bodyCount, total, S );
+	if constexpr ( kAdjLds )
+	{
+		// every item of the launch has 12 bodyCount + 8 slots <= kAdjointLdsFloats (checked by the host)
+		__shared__ __align__( 16 ) float adjLds[kAdjointLdsFloats];
+		tape.a.bAdj = adjLds;
+		tape.a.sAdj = adjLds + (size_t)12 * wi.bodyCount;
+	}
 	const AdjScratch& a = tape.a;
 	const DevContact* sn = snap + wi.slotStart;
 
@@ -984,14 +992,18 @@ hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, i
 								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
 								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
 								float* cf, int cfCapacity, float4* priv, float2* mass, float* scratch, const int* scratchOffset,
-								hipStream_t stream )
+								bool adjointsInLds, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
 	if ( blockSize < 64 || blockSize > kAdjointMaxBlock )
 		return hipErrorInvalidValue;
-	hipLaunchKernelGGL( k_adjoint_step, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap, poseIn,
-						adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, mass, scratch, scratchOffset );
+	if ( adjointsInLds )
+		hipLaunchKernelGGL( k_adjoint_step<true>, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap,
+							poseIn, adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, mass, scratch, scratchOffset );
+	else
+		hipLaunchKernelGGL( k_adjoint_step<false>, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap,
+							poseIn, adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, mass, scratch, scratchOffset );
 	return hipGetLastError();
 }
 

@@ -584,6 +584,9 @@ constexpr int kDefaultItemLdsBytes = kMaxItemLdsBytes;
 // and adjoint kernels, which hold more state per thread, take at most kAdjointMaxBlock)
 constexpr int kSolveMaxBlock = 1024;
 constexpr int kAdjointMaxBlock = 256;
+// the adjoint kernel keeps a work item's body and slot adjoints (12 floats per body, 8 per slot)
+// in LDS when they fit in this many floats, and in its HBM scratch otherwise
+constexpr int kAdjointLdsFloats = 4096;
 // (the LDS stores hold one more body, the static body's identity state, at index bodyCount)
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {

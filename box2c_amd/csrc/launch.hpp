@@ -80,7 +80,7 @@ hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, i
 								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
 								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
 								float* cf, int cfCapacity, float4* priv, float2* mass, float* scratch, const int* scratchOffset,
-								hipStream_t stream );
+								bool adjointsInLds, hipStream_t stream );
 hipError_t launch_link_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase,
 							  const int2* slotOfContact, int prevStep, int* prevSlot, hipStream_t stream );
 hipError_t launch_mark_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase, int step,

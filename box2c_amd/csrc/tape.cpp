@@ -21,6 +21,7 @@ struct TapeVersion
 	int itemsVersion = -1;
 	int itemCount = 0, slotCount = 0, bodyListCount = 0, block = 64;
 	int usedSlots = 0; // slots that belong to work items
+	bool adjointsInLds = true; // every item's body and slot adjoints fit kAdjointLdsFloats
 	WorkItem* dItems = nullptr;
 	int* dBodyList = nullptr;
 	int* dSlotContact = nullptr;
@@ -212,6 +213,8 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 	{
 		TapeVersion v;
 		v.itemsVersion = g->itemsVersion;
+		// an LDS budget of 0 keeps the adjoint's state in HBM as well
+		v.adjointsInLds = g->ldsBudget != 0;
 		v.itemCount = g->itemCount;
 		v.slotCount = g->slotCount;
 		v.bodyListCount = g->bodyListCount;
@@ -233,6 +236,7 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 				fatal( "a taped run does not support joints", __FILE__, __LINE__ );
 			itemsOfWorld[wi.world] += 1;
 			v.usedSlots += wi.groupStart[kGroups] - wi.groupStart[1];
+			v.adjointsInLds = v.adjointsInLds && 12 * wi.bodyCount + 8 * wi.groupStart[kGroups] <= kAdjointLdsFloats;
 			for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
 				slotWorld[wi.slotStart + k] = wi.world;
 			B2MI_CHECK( scratch < ( (size_t)1 << 31 ) );
@@ -387,7 +391,8 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 			HIP_CHECK( launch_link_slots( v.dSlotContact, v.dSlotWorld, v.slotCount, dContactBase, dSlotOf, prevTag, dPrevSlot, s ) );
 			HIP_CHECK( launch_adjoint_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
 											dStates + (size_t)( step - c ) * n, dAdj, t->dSimBase, reinterpret_cast<const float4*>( lamOut ),
-											lamPrev, dPrevSlot, dParam, dCf, maxSlots, dPriv, dMass, dScratch, v.dScratchOffset, s ) );
+											lamPrev, dPrevSlot, dParam, dCf, maxSlots, dPriv, dMass, dScratch, v.dScratchOffset,
+											v.adjointsInLds, s ) );
 			std::swap( lamOut, lamPrev );
 		}
 	}

@@ -67,6 +67,7 @@ class DropBoxesRollout:
         self.substeps, self.checkpoint, self.seed, self.dt = substeps, checkpoint, seed, dt
         self.batch = None
         self.start = None
+        self.lds_budget = None  # b2mi_Batch_SetLdsBudget of the rollout's batch (None = default)
         self.forward_seconds = self.backward_seconds = 0.0
 
     def world_def(self):
@@ -112,6 +113,8 @@ class DropBoxesRollout:
         w0 = np.ascontiguousarray(w0, np.float32).reshape(self.worlds, self.boxes)
         d = self.world_def()
         self.batch = api.create_batch()
+        if self.lds_budget is not None:
+            api.lib.b2mi_Batch_SetLdsBudget(self.batch, int(self.lds_budget))
         api.use_batch(self.batch)
         try:
             for i in range(self.worlds):

@@ -155,6 +155,25 @@ def test_autograd_function():
         r.close()
 
 
+def test_adjoint_state_in_lds_or_hbm_bit_identical():
+    """The adjoint kernel keeps an item's body and slot adjoints in LDS when they fit and in HBM
+    under an LDS budget of 0: the same arithmetic, so the same gradients bit for bit."""
+    results = []
+    for budget in (None, 0):
+        r, (f, e, v0, w0) = rollout(3, 6, 30, 5, 13)
+        r.lds_budget = budget
+        try:
+            final = r.run(f, e, v0, w0)
+            rng = np.random.default_rng(2)
+            g = rng.normal(size=final.shape).astype(np.float32)
+            results.append((final,) + tuple(r.backward(g)))
+        finally:
+            r.close()
+    for a, b in zip(results[0], results[1]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert float(np.abs(results[0][3]).sum()) > 0.0
+
+
 @pytest.mark.parametrize("pre,steps", [(0, 40), (60, 10), (0, 90)])
 def test_adjoint_matches_fp64_autograd(pre, steps):
     """The float32 GPU adjoint against torch autograd through an FP64 restatement of the same
