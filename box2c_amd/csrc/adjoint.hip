@@ -28,6 +28,8 @@
 #include "b2mi_layout.h"
 #include "solver.hpp"
 
+#include <type_traits>
+
 namespace b2mi
 {
 
@@ -594,9 +596,56 @@ __device__ void integrate_position_adj( const AdjScratch& a, int i, const float*
 	g[2] += h * gda;
 }
 
+// ---- the solver state of a work item: in HBM, or in LDS as the step kernel's LDS store ----
+
+extern __shared__ float4 lds[];
+
+template <class S>
+__device__ inline S item_store( const WorkItem& wi, const int2* __restrict__ slotBodies, float* cfBase, float4* priv, float2* mass );
+
+template <>
+__device__ inline GlobalStore item_store<GlobalStore>( const WorkItem& wi, const int2* __restrict__ slotBodies, float* cfBase,
+													   float4* priv, float2* mass )
+{
+	GlobalStore C;
+	C.rec = reinterpret_cast<float4*>( cfBase );
+	C.base = wi.slotStart;
+	C.sb = slotBodies + wi.slotStart;
+	C.P = priv + 2 * (size_t)wi.bodyStart;
+	C.M = mass + wi.bodyStart;
+	return C;
+}
+
+// Dynamic LDS of 40 (bodyCount + 1) + 16 cfQuads slots bytes: body state and the static body's
+// identity state, constraint records, masses. Written here; read after the barrier that ends
+// the pose staging of replay_item.
+template <>
+__device__ inline LdsStore item_store<LdsStore>( const WorkItem& wi, const int2* __restrict__ slotBodies, float*, float4*, float2* )
+{
+	const int total = wi.groupStart[kGroups];
+	LdsStore C;
+	C.P = lds;
+	C.identity = wi.bodyCount;
+	C.rec = lds + 2 * ( wi.bodyCount + 1 );
+	C.M = reinterpret_cast<float2*>( C.rec + cfQuads * total );
+	for ( int k = threadIdx.x; k < total; k += blockDim.x )
+	{
+		int2 b = slotBodies[wi.slotStart + k];
+		C.rec[k * cfQuads] = make_float4( __builtin_bit_cast( float, b.x ), __builtin_bit_cast( float, b.y ), 0.0f, 0.0f );
+	}
+	if ( threadIdx.x == 0 )
+	{
+		C.b0( C.identity ) = make_float4( 0.0f, 0.0f, 0.0f, 0.0f );
+		C.b1( C.identity ) = make_float4( 0.0f, 0.0f, 1.0f, 0.0f );
+		C.mass( C.identity ) = make_float2( 0.0f, 0.0f );
+	}
+	return C;
+}
+
 // ---- kernels ----
 
 // One workgroup per work item: the step's solver from poseIn to poseOut (no tape).
+template <bool kLds>
 __global__ void __launch_bounds__( kAdjointMaxBlock ) k_replay_step( const DevWorld* __restrict__ worlds, const WorkItem* __restrict__ items,
 																   const int* __restrict__ bodyList, const int2* __restrict__ slotBodies,
 																   const DevContact* __restrict__ snap, const StepPose* __restrict__ poseIn,
@@ -606,12 +655,8 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_replay_step( const DevWo
 	const WorkItem& wi = items[blockIdx.x];
 	const DevWorld& W = worlds[wi.world];
 	const int* bl = bodyList + wi.bodyStart;
-	GlobalStore C;
-	C.rec = reinterpret_cast<float4*>( cfBase );
-	C.base = wi.slotStart;
-	C.sb = slotBodies + wi.slotStart;
-	C.P = priv + 2 * (size_t)wi.bodyStart;
-	C.M = mass + wi.bodyStart;
+	using Store = std::conditional_t<kLds, LdsStore, GlobalStore>;
+	const Store C = item_store<Store>( wi, slotBodies, cfBase, priv, mass );
 	NoTape tape;
 	replay_item( C, tape, W, wi, bl, snap + wi.slotStart, poseIn, simBase[wi.world] );
 	int base = simBase[wi.world];
@@ -643,12 +688,9 @@ __global__ void __launch_bounds__( kAdjointMaxBlock ) k_adjoint_step( const DevW
 	const int total = wi.groupStart[kGroups];
 	const int base = simBase[wi.world];
 	const int S = W.subSteps;
-	GlobalStore C;
-	C.rec = reinterpret_cast<float4*>( cfBase );
-	C.base = wi.slotStart;
-	C.sb = slotBodies + wi.slotStart;
-	C.P = priv + 2 * (size_t)wi.bodyStart;
-	C.M = mass + wi.bodyStart;
+	// with kAdjLds the solver state is in LDS too
+	using Store = std::conditional_t<kAdjLds, LdsStore, GlobalStore>;
+	const Store C = item_store<Store>( wi, slotBodies, cfBase, priv, mass );
 	Tape tape;
 	tape.a = carve( scratch + scratchOffset[blockIdx.x], wi.bodyCount, total, S );
 	if constexpr ( kAdjLds )
@@ -977,14 +1019,19 @@ hipError_t launch_gather_poses( const DevWorld* worlds, const int* simBase, cons
 
 hipError_t launch_replay_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 							   const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* poseOut,
-							   const int* simBase, float* cf, int cfCapacity, float4* priv, float2* mass, hipStream_t stream )
+							   const int* simBase, float* cf, int cfCapacity, float4* priv, float2* mass, int storeLdsBytes,
+							   hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
-	if ( blockSize < 64 || blockSize > kAdjointMaxBlock )
+	if ( blockSize < 64 || blockSize > kAdjointMaxBlock || storeLdsBytes < 0 || storeLdsBytes > kAdjointStoreLdsBytes )
 		return hipErrorInvalidValue;
-	hipLaunchKernelGGL( k_replay_step, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap, poseIn,
-						poseOut, simBase, cf, cfCapacity, priv, mass );
+	if ( storeLdsBytes > 0 )
+		hipLaunchKernelGGL( k_replay_step<true>, dim3( itemCount ), dim3( blockSize ), storeLdsBytes, stream, worlds, items, bodyList,
+							slotBodies, snap, poseIn, poseOut, simBase, cf, cfCapacity, priv, mass );
+	else
+		hipLaunchKernelGGL( k_replay_step<false>, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies,
+							snap, poseIn, poseOut, simBase, cf, cfCapacity, priv, mass );
 	return hipGetLastError();
 }
 
@@ -992,14 +1039,14 @@ hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, i
 								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
 								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
 								float* cf, int cfCapacity, float4* priv, float2* mass, float* scratch, const int* scratchOffset,
-								bool adjointsInLds, hipStream_t stream )
+								int storeLdsBytes, hipStream_t stream )
 {
 	if ( itemCount <= 0 )
 		return hipSuccess;
-	if ( blockSize < 64 || blockSize > kAdjointMaxBlock )
+	if ( blockSize < 64 || blockSize > kAdjointMaxBlock || storeLdsBytes < 0 || storeLdsBytes > kAdjointStoreLdsBytes )
 		return hipErrorInvalidValue;
-	if ( adjointsInLds )
-		hipLaunchKernelGGL( k_adjoint_step<true>, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap,
+	if ( storeLdsBytes > 0 )
+		hipLaunchKernelGGL( k_adjoint_step<true>, dim3( itemCount ), dim3( blockSize ), storeLdsBytes, stream, worlds, items, bodyList, slotBodies, snap,
 							poseIn, adj, simBase, lamOut, lamPrev, prevSlot, paramGrad, cf, cfCapacity, priv, mass, scratch, scratchOffset );
 	else
 		hipLaunchKernelGGL( k_adjoint_step<false>, dim3( itemCount ), dim3( blockSize ), 0, stream, worlds, items, bodyList, slotBodies, snap,

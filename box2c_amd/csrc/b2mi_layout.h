@@ -587,6 +587,8 @@ constexpr int kAdjointMaxBlock = 256;
 // the adjoint kernel keeps a work item's body and slot adjoints (12 floats per body, 8 per slot)
 // in LDS when they fit in this many floats, and in its HBM scratch otherwise
 constexpr int kAdjointLdsFloats = 4096;
+// and its solver state (40 (bodyCount + 1) + 16 cfQuads slots bytes) when it fits in this many bytes
+constexpr int kAdjointStoreLdsBytes = 48 * 1024;
 // (the LDS stores hold one more body, the static body's identity state, at index bodyCount)
 inline int itemLdsBytes( int bodyCount, int slotCount )
 {

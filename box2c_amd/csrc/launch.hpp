@@ -73,14 +73,17 @@ hipError_t launch_snapshot( const DevWorld* worlds, const WorkItem* items, int i
 							hipStream_t stream );
 hipError_t launch_gather_poses( const DevWorld* worlds, const int* simBase, const int* simCount, int worldCount, int maxSims,
 								StepPose* out, hipStream_t stream );
+// storeLdsBytes: 0 = the solver state (and the adjoint's body and slot adjoints) in HBM; else
+// the dynamic LDS bytes of the launch's largest item, whose solver state and adjoints are in LDS
 hipError_t launch_replay_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 							   const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* poseOut,
-							   const int* simBase, float* cf, int cfCapacity, float4* priv, float2* mass, hipStream_t stream );
+							   const int* simBase, float* cf, int cfCapacity, float4* priv, float2* mass, int storeLdsBytes,
+							   hipStream_t stream );
 hipError_t launch_adjoint_step( const DevWorld* worlds, const WorkItem* items, int itemCount, int blockSize, const int* bodyList,
 								const int2* slotBodies, const DevContact* snap, const StepPose* poseIn, StepPose* adj,
 								const int* simBase, const float4* lamOut, float* lamPrev, const int* prevSlot, float* paramGrad,
 								float* cf, int cfCapacity, float4* priv, float2* mass, float* scratch, const int* scratchOffset,
-								bool adjointsInLds, hipStream_t stream );
+								int storeLdsBytes, hipStream_t stream );
 hipError_t launch_link_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase,
 							  const int2* slotOfContact, int prevStep, int* prevSlot, hipStream_t stream );
 hipError_t launch_mark_slots( const int* slotContact, const int* slotWorld, int slotCount, const int* contactBase, int step,

@@ -21,7 +21,8 @@ struct TapeVersion
 	int itemsVersion = -1;
 	int itemCount = 0, slotCount = 0, bodyListCount = 0, block = 64;
 	int usedSlots = 0; // slots that belong to work items
-	bool adjointsInLds = true; // every item's body and slot adjoints fit kAdjointLdsFloats
+	bool adjointsInLds = true; // every item's solver state and adjoints fit the LDS limits
+	int storeLdsBytes = 0;	   // LDS of the largest item's solver state (when adjointsInLds)
 	WorkItem* dItems = nullptr;
 	int* dBodyList = nullptr;
 	int* dSlotContact = nullptr;
@@ -236,13 +237,17 @@ void tapeRecordStep( StepGroup* g, float timeStep, int subStepCount, int blockSi
 				fatal( "a taped run does not support joints", __FILE__, __LINE__ );
 			itemsOfWorld[wi.world] += 1;
 			v.usedSlots += wi.groupStart[kGroups] - wi.groupStart[1];
-			v.adjointsInLds = v.adjointsInLds && 12 * wi.bodyCount + 8 * wi.groupStart[kGroups] <= kAdjointLdsFloats;
+			const int storeBytes = 40 * ( wi.bodyCount + 1 ) + 16 * cfQuads * wi.groupStart[kGroups];
+			v.adjointsInLds = v.adjointsInLds && 12 * wi.bodyCount + 8 * wi.groupStart[kGroups] <= kAdjointLdsFloats &&
+							  storeBytes <= kAdjointStoreLdsBytes;
+			v.storeLdsBytes = std::max( v.storeLdsBytes, storeBytes );
 			for ( int k = 0; k < wi.groupStart[kGroups]; ++k )
 				slotWorld[wi.slotStart + k] = wi.world;
 			B2MI_CHECK( scratch < ( (size_t)1 << 31 ) );
 			offsets[i] = (int)scratch;
 			scratch += (size_t)adjoint_scratch_floats( wi.bodyCount, wi.groupStart[kGroups], subStepCount );
 		}
+		v.storeLdsBytes = v.adjointsInLds ? std::max( v.storeLdsBytes, 16 ) : 0;
 		for ( int n : itemsOfWorld )
 			B2MI_CHECK( n <= 1 );
 		v.scratchFloats = scratch;
@@ -373,7 +378,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 			const TapeVersion& v = t->versions[ts.version];
 			HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
 										   dStates + (size_t)( step - c ) * n, dStates + (size_t)( step - c + 1 ) * n, t->dSimBase, dCf,
-										   maxSlots, dPriv, dMass, s ) );
+										   maxSlots, dPriv, dMass, v.storeLdsBytes, s ) );
 		}
 		for ( int step = end - 1; step >= c; --step )
 		{
@@ -392,7 +397,7 @@ static void tapeBackward( StepGroup* g, const float* gradEnd, float* gradStart, 
 			HIP_CHECK( launch_adjoint_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, ts.dSnap,
 											dStates + (size_t)( step - c ) * n, dAdj, t->dSimBase, reinterpret_cast<const float4*>( lamOut ),
 											lamPrev, dPrevSlot, dParam, dCf, maxSlots, dPriv, dMass, dScratch, v.dScratchOffset,
-											v.adjointsInLds, s ) );
+											v.storeLdsBytes, s ) );
 			std::swap( lamOut, lamPrev );
 		}
 	}
@@ -451,7 +456,7 @@ static void tapeReplay( StepGroup* g, const float* startPoses, const float* worl
 			snap = dPatched;
 		}
 		HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, snap, dA, dB, t->dSimBase,
-									   dCf, maxSlots, dPriv, dMass, s ) );
+									   dCf, maxSlots, dPriv, dMass, v.storeLdsBytes, s ) );
 		std::swap( dA, dB );
 	}
 	HIP_CHECK( hipMemcpyAsync( h.data(), dA, sizeof( StepPose ) * n, hipMemcpyDeviceToHost, s ) );
@@ -577,7 +582,7 @@ static bool tapeReplayNoAlloc( StepGroup* g, const float* startPoses, const floa
 			snap = dPatched;
 		}
 		HIP_CHECK( launch_replay_step( g->dWorlds, v.dItems, v.itemCount, v.block, v.dBodyList, v.dSlotBodies, snap, dA, dB, t->dSimBase,
-									   dCf, maxSlots, dPriv, dMass, s ) );
+									   dCf, maxSlots, dPriv, dMass, v.storeLdsBytes, s ) );
 		std::swap( dA, dB );
 	}
 	HIP_CHECK( launch_rows_from_poses( dA, rows, n, s ) );

@@ -22,6 +22,7 @@ CASES = [
     "tests/test_gpu_adjoint.py::test_taped_run_matches_oracle",
     "tests/test_gpu_adjoint.py::test_replay_reproduces_run",
     "tests/test_gpu_adjoint.py::test_autograd_function",
+    "tests/test_gpu_adjoint.py::test_adjoint_state_in_lds_or_hbm_bit_identical",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[hello]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[shape_zoo]",
     "tests/test_gpu_parity.py::test_step_parity_with_oracle[polygon_zoo]",
