@@ -211,6 +211,7 @@ typedef struct b2mi_TapeSlot
 #define B2MI_TAPE_WORLD_PARAMS 16
 
 B2_API void b2mi_Batch_BeginTape( int batch, int checkpointInterval );
+// Ends the tape. Its device memory stays with the batch for the next tape (freed with the batch).
 B2_API void b2mi_Batch_EndTape( int batch );
 B2_API int b2mi_Batch_GetTapeStepCount( int batch );
 B2_API int b2mi_Batch_GetTapeBodies( int batch, b2BodyId* bodies, int capacity );
